@@ -1,0 +1,212 @@
+/*
+ * xmvb_amd.h — C ABI of the MI355X VBSCF energy+gradient engine.
+ *
+ * The engine evaluates one VBSCF energy and its analytic orbital gradient ("E+grad") for a
+ * structure set of nonorthogonal spin-coupled VB structures on one AMD Instinct MI355X
+ * (gfx950): RI AO->orbital integral transform, determinant-pair Lowdin/cofactor matrix
+ * elements, structure H/S assembly, the generalised eigenproblem H c = E S c and the
+ * cofactor-path backward, all on the device in FP64.
+ *
+ * Conventions (identical to the reference program Tao-Xia/xmvb-cpp):
+ *   - matrices are column-major;
+ *   - orbitals are orbital-major: value of orbital k on basis function mu at [k*n_bf + mu]
+ *     (VbScfInput::orbital_preparation_input.orbital_value_table with a dense index table,
+ *      src/vbscf/orbitals/preparation/input.hpp:13-167);
+ *   - RI factors B are n_aux x n_bf(n_bf+1)/2, column-major, AO pair (mu,nu) at
+ *     hi(hi+1)/2+lo (RiAoFactorization::metric_whitened_ao_pair_factors,
+ *     src/integrals/ao/ri.hpp:15-31; PairIndex, src/core/pair_index.hpp:15-55), so that
+ *     (mu nu|la si) = sum_P B[P,(mu nu)] B[P,(la si)];
+ *   - spin strings are occupation bitmasks over active orbitals (bit i = orbital i).
+ *
+ * Every function returns a vbe_status; no C++ exception crosses this boundary
+ * (the reference throws std::invalid_argument/runtime_error and maps them to exit code 2,
+ * src/cli/main.cpp:17-20). The text of the last error of a context is available through
+ * vbe_last_error(); errors raised before a context exists go to vbe_global_error().
+ *
+ * Threading: a context owns one HIP stream; calls on one context must not overlap.
+ * Different contexts are independent.
+ */
+#ifndef XMVB_AMD_H
+#define XMVB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum vbe_status {
+    VBE_OK = 0,
+    VBE_ERR_ARGUMENT = 1,     /* invalid argument / inconsistent sizes        (std::invalid_argument) */
+    VBE_ERR_NUMERIC = 2,      /* orbital normalisation or S_act factorisation failed (preparer.cpp:78-83,399-401) */
+    VBE_ERR_METRIC = 3,       /* structure metric not positive definite (dsygvd info > 0) */
+    VBE_ERR_DEVICE = 4,       /* HIP runtime error */
+    VBE_ERR_UNSUPPORTED = 5,  /* option combination this engine rejects (cuvbscf.cpp:38-50 precedent) */
+    VBE_ERR_MEMORY = 6,       /* HBM budget exceeded (std::length_error precedent) */
+    VBE_ERR_TOPOLOGY = 7      /* structure generation/expansion failed */
+} vbe_status;
+
+/* Electronic methods selected by bare $CTRL tokens. Mirrors ElectronicMethod
+ * (src/input/deck/metadata.hpp:19) plus HIPVBSCF, the token that selects this engine. */
+typedef enum vbe_method {
+    VBE_METHOD_VBSCF = 0,
+    VBE_METHOD_CUVBSCF = 1,
+    VBE_METHOD_CASSCF = 2,
+    VBE_METHOD_DFT = 3,
+    VBE_METHOD_HIPVBSCF = 4
+} vbe_method;
+
+typedef enum vbe_precision {
+    VBE_PRECISION_FP64 = 0 /* plain FP64 path; mixed-precision transforms are rejected for now */
+} vbe_precision;
+
+typedef enum vbe_eigensolver {
+    VBE_EIGENSOLVER_DENSE = 0 /* EIGENSOLVER=DENSE (linalg/eigensolver.cpp:365-387) */
+} vbe_eigensolver;
+
+typedef struct vbe_context vbe_context;
+typedef struct vbe_problem vbe_problem;
+typedef struct vbe_topology vbe_topology;
+
+typedef struct vbe_dims {
+    int32_t n_bf;        /* AO basis functions                                     */
+    int32_t n_act;       /* active VB orbitals (all electrons active, no core)     */
+    int32_t n_alpha;     /* active alpha electrons                                 */
+    int32_t n_beta;      /* active beta electrons                                  */
+    int32_t n_aux;       /* RI factor rows                                         */
+} vbe_dims;
+
+/* Signed structure -> determinant incidence (StructureExpansion, src/vbscf/structures/expansion/types.hpp:390-423). */
+typedef struct vbe_topology_view {
+    int32_t n_structures;
+    int32_t n_alpha_strings;
+    const uint64_t* alpha_masks;      /* ascending                                       */
+    int32_t n_beta_strings;
+    const uint64_t* beta_masks;       /* ascending                                       */
+    int32_t n_determinants;
+    const int32_t* det_alpha;         /* alpha string index per determinant              */
+    const int32_t* det_beta;          /* beta string index per determinant               */
+    const int64_t* term_offsets;      /* n_determinants + 1 CSR offsets                  */
+    const int32_t* term_structure;    /* structure index per term                        */
+    const double* term_coefficient;   /* signed coefficient per term                     */
+} vbe_topology_view;
+
+typedef struct vbe_flags {
+    int32_t precision;      /* vbe_precision   */
+    int32_t eigensolver;    /* vbe_eigensolver */
+    int32_t want_gradient;  /* 0: energy only (a trial evaluation), 1: energy + gradient */
+} vbe_flags;
+
+typedef struct vbe_phase_times {
+    /* device time per phase of the last evaluation, milliseconds (HIP events) */
+    float orbital_prep_ms, transform_ms, pair_forward_ms, structure_ms, eigensolve_ms, pair_backward_ms,
+        gradient_tail_ms, total_ms;
+} vbe_phase_times;
+
+/* ---- $CTRL method flag ------------------------------------------------------------------ */
+
+/* Parse the bare tokens of a $CTRL line into a method. Mirrors apply_ctrl_flag
+ * (src/input/deck/model.cpp:360-381): VBSCF/CUVBSCF/CASSCF/DFT plus HIPVBSCF; conflicting
+ * methods fail with VBE_ERR_ARGUMENT; TBVBSCF and VBPDFT are rejected as unsupported. Tokens
+ * that are not method tokens are ignored. *method is VBE_METHOD_VBSCF when no token is given. */
+int vbe_parse_ctrl_method(const char* ctrl_line, int32_t* method);
+
+/* Reject option combinations HIPVBSCF does not implement, in the manner of run_cuvbscf
+ * (src/cli/cuvbscf.cpp:38-50): eigensolver must be DENSE, two-electron mode must be RI (ri=1),
+ * exactly one state, no inactive (core) electrons. Returns VBE_OK or VBE_ERR_UNSUPPORTED and
+ * writes the reason to vbe_global_error(). */
+int vbe_check_hipvbscf_options(int32_t eigensolver, int32_t ri, int32_t n_states, int32_t n_inactive_electrons);
+
+/* ---- topology (host) ---------------------------------------------------------------------- */
+
+/* Closed-form structure count for a structure-class keyword ("COV", "FULL", "ION", "COV ION(1)"),
+ * count_generated_structures (src/input/loading/generated_structures.cpp:621-658). */
+int vbe_count_structures(const char* structure_class, int32_t n_act, int32_t n_electrons, int32_t multiplicity,
+                         uint64_t* count);
+
+/* Generate Rumer/Young structures (generated_structures.cpp:673-716) and expand them into signed
+ * determinants (StructureExpander::expand, src/vbscf/structures/expansion/expander.cpp:287-558).
+ * sizes[0..4] = n_structures, n_alpha_strings, n_beta_strings, n_determinants, n_terms. */
+int vbe_topology_build(const char* structure_class, int32_t n_act, int32_t n_electrons, int32_t multiplicity,
+                       vbe_topology** out, int64_t sizes[5]);
+/* Copy the arrays of a built topology into caller buffers sized from sizes[]; raw_structures
+ * (optional, n_structures x n_electrons labels, 1-based) may be NULL. */
+int vbe_topology_copy(const vbe_topology* topo, uint64_t* alpha_masks, uint64_t* beta_masks, int32_t* det_alpha,
+                      int32_t* det_beta, int64_t* term_offsets, int32_t* term_structure, double* term_coefficient,
+                      int32_t* raw_structures);
+int vbe_topology_view_of(const vbe_topology* topo, vbe_topology_view* view);
+void vbe_topology_free(vbe_topology* topo);
+
+/* ---- synthetic problems (host) -------------------------------------------------------------- */
+
+/* Seeded synthetic RI problem: mt19937_64(seed) with a Marsaglia polar normal sampler; draws, in
+ * this order and column-major: A ~ N(0,1) (n_bf^2) -> S = I + 0.1 A A^T / n_bf; X ~ N(0, var 0.5)
+ * -> h = (X + X^T)/2 - 2 I; for each aux row P: G_P ~ N(0, var 1/n_bf) -> B_P = (G_P + G_P^T)/2
+ * packed; C ~ N(0,1)/sqrt(n_bf) (orbital-major n_act x n_bf). scan_r >= 0 scales the blocks of h
+ * and B that couple the first and second half of the basis by exp(-0.5 scan_r) (bond-stretch scan
+ * geometry parameter); pass a negative scan_r for none. Output buffers: S, h (n_bf^2), B
+ * (n_aux * n_bf(n_bf+1)/2), C (n_act * n_bf). */
+int vbe_synthetic_problem(uint64_t seed, int32_t n_bf, int32_t n_act, int32_t n_aux, double scan_r, double* S,
+                          double* h, double* B, double* C);
+
+/* ---- device engine -------------------------------------------------------------------------- */
+
+/* Create a context on HIP device `device` with an HBM budget in bytes (0 = 90% of free memory,
+ * the cuda::Context precedent, src/cuda/memory/context.hpp:66-115). One HIP stream per context. */
+int vbe_context_create(int32_t device, uint64_t hbm_budget_bytes, vbe_context** out);
+void vbe_context_destroy(vbe_context* ctx);
+/* Copy the last error of a context into buf (NUL-terminated, truncated to len). */
+int vbe_last_error(const vbe_context* ctx, char* buf, size_t len);
+const char* vbe_global_error(void);
+
+/* Upload the molecule-static data of one problem: AO overlap S and core Hamiltonian h
+ * (n_bf x n_bf), RI factors B, the structure topology and E_nuc. Host buffers are copied; the
+ * problem stays resident in HBM until vbe_problem_release. Replaces the one-time upload of the
+ * CUDA precedent cuda::solve_scf (src/cuda/vb/scf.hpp:41-46). */
+int vbe_problem_upload(vbe_context* ctx, const vbe_dims* dims, const double* S_ao, const double* h_ao,
+                       const double* B_packed, const vbe_topology_view* topo, double e_nuc, vbe_problem** out);
+void vbe_problem_release(vbe_problem* prob);
+/* Device bytes held by the problem (static data + evaluation workspace). */
+int vbe_problem_device_bytes(const vbe_problem* prob, uint64_t* bytes);
+
+/* One evaluation from host orbitals: C_raw (n_act x n_bf, orbital-major), optional support mask
+ * (same layout, nonzero = coefficient slot present; NULL = dense). Writes E_total and, when
+ * flags.want_gradient, grad_raw (n_act x n_bf; zero on absent slots). E_states (optional) receives
+ * the lowest n_str eigenvalues + E_nuc. Host-side replacement of OrbitalGradientEvaluator::evaluate
+ * (src/vbscf/derivatives/gradient/orbital/evaluator.cpp:198-418) with EIGENSOLVER=DENSE. */
+int vbe_eval(vbe_context* ctx, vbe_problem* prob, const double* C_raw, const uint8_t* support, vbe_flags flags,
+             double* E_total, double* E_states, double* grad_raw);
+
+/* Same with device-resident orbitals and gradient (device pointers, n_act*n_bf doubles each;
+ * d_support may be NULL). Work is enqueued on the context stream and the call returns after the
+ * energy is known on the host. */
+int vbe_eval_device(vbe_context* ctx, vbe_problem* prob, const double* d_C_raw, const uint8_t* d_support,
+                    vbe_flags flags, double* E_total, double* d_grad_raw);
+
+/* Two-phase contract of VbScfObjective (src/vbscf/optimization/objective/function.cpp:89-275):
+ * vbe_eval_forward evaluates the energy of trial orbitals and keeps the forward state on the
+ * device; vbe_complete_gradient turns the retained forward state into the gradient. A new
+ * forward discards the previous one. */
+int vbe_eval_forward(vbe_context* ctx, vbe_problem* prob, const double* C_raw, const uint8_t* support,
+                     double* E_total);
+int vbe_complete_gradient(vbe_context* ctx, vbe_problem* prob, double* grad_raw);
+
+/* Structure matrices, eigenvector and active-space intermediates of the last evaluation
+ * (any pointer may be NULL): H, S (n_str^2), coef (n_str), S_act, h_act (n_act^2), packed
+ * active two-electron kernel (np(np+1)/2, np = n_act(n_act+1)/2). */
+int vbe_last_intermediates(vbe_context* ctx, vbe_problem* prob, double* H, double* S, double* coef, double* S_act,
+                           double* h_act, double* g_packed);
+
+/* Device time per phase of the last evaluation. */
+int vbe_last_phase_times(const vbe_problem* prob, vbe_phase_times* times);
+
+/* Count of determinant pairs whose overlap block was rank deficient in the last evaluation
+ * (handled by the singular-pair path). */
+int vbe_last_singular_pairs(const vbe_problem* prob, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XMVB_AMD_H */

@@ -1,0 +1,974 @@
+// HIP/CDNA4 (gfx950) kernels of the VBSCF energy+gradient engine, FP64 path.
+//
+// Phases of one evaluation and the reference functions they replace:
+//   k_orbital_prep        normalize_sparse_orbitals (orbitals/preparation/preparer.cpp:45-94)
+//   k_active_matrices     S_act = C^T S C (preparer.cpp:371-375), raw h_act
+//   k_ri_one_index        transform_ri_one_index (integrals/ao/ri_transform.cpp:10-37)
+//   k_ri_pair_factors     transform_ri_active_integrals (integrals/active/ri.cpp:34-96), raw frame
+//   k_ri_pair_gram        pack_ri_active_integrals (integrals/active/ri.cpp:14-31)
+//   k_pairs<false>        DeterminantPairEvaluator (determinants/pairs/evaluator.cpp:328-384) +
+//                         calc_same_spin_hamiltonian (determinants/algebra/hamiltonian.cpp:35-236)
+//   k_structure_*         build_tiled_matrix_form_structure_matrices (structures/assembly/hamiltonian_overlap.cpp:889-1021)
+//   k_gep_lowest          GeneralizedEigensolver::solve_dense (linalg/eigensolver.cpp:365-387)
+//   k_pairs<true>         build_{same,opposite}_spin backward (derivatives/hessian/responses/*/backward.cpp)
+//   k_ri_pair_adjoint,
+//   k_orbital_gradient    backpropagate_ri_packed_active_two_electron_gradient (integrals/response/ri.cpp:687-694),
+//                         pullback_active_matrices (integrals/active/matrix_adjoint.hpp:20-28)
+//   k_normalization_pullback  backpropagate_normalization_to_raw_slots (orbitals/pullback/operator.cpp:139-215)
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace xmvb_amd {
+
+__device__ __forceinline__ int pair_index(int a, int b) {
+    const int hi = a > b ? a : b, lo = a > b ? b : a;
+    return hi * (hi + 1) / 2 + lo;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    // 64-wide wavefront reduction, then across waves through LDS (red: >= 16 doubles)
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += red[w];
+    __syncthreads();
+    return t;
+}
+
+// ---------------------------------------------------------------------------------------------
+// orbital preparation: one block per orbital
+// ---------------------------------------------------------------------------------------------
+__global__ void k_orbital_prep(const double* __restrict__ C, const uint8_t* __restrict__ support,
+                               const double* __restrict__ S, const double* __restrict__ h, int n_bf, int n_act,
+                               double* __restrict__ Cn, double* __restrict__ SC, double* __restrict__ HC,
+                               double* __restrict__ nu2, int* __restrict__ flags) {
+    extern __shared__ double lds[];
+    double* c = lds;             // n_bf
+    double* red = lds + n_bf;    // 16
+    const int k = blockIdx.x;
+    for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
+        const size_t at = (size_t)k * n_bf + mu;
+        c[mu] = (support && !support[at]) ? 0.0 : C[at];
+    }
+    __syncthreads();
+    double q = 0.0;
+    for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
+        double sv = 0.0, hv = 0.0;
+        for (int nu = 0; nu < n_bf; ++nu) {
+            const double cv = c[nu];
+            sv += S[mu + (size_t)n_bf * nu] * cv;
+            hv += h[mu + (size_t)n_bf * nu] * cv;
+        }
+        SC[mu + (size_t)n_bf * k] = sv;
+        HC[mu + (size_t)n_bf * k] = hv;
+        q += c[mu] * sv;
+    }
+    q = block_sum(q, red);
+    const bool ok = isfinite(q) && q > 2.220446049250313e-16;
+    const double f = ok ? 1.0 / sqrt(q) : 0.0;
+    if (!ok && threadIdx.x == 0) atomicMax(&flags[FLAG_NORMALIZATION], k + 1);
+    for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
+        Cn[(size_t)k * n_bf + mu] = c[mu] * f;
+        SC[mu + (size_t)n_bf * k] *= f;
+        HC[mu + (size_t)n_bf * k] *= f;
+    }
+    if (threadIdx.x == 0) nu2[k] = q;
+}
+
+// S_act and raw h_act (n x n), symmetrised; thread 0 checks that S_act is positive definite.
+__global__ void k_active_matrices(const double* __restrict__ Cn, const double* __restrict__ SC,
+                                  const double* __restrict__ HC, int n_bf, int n, double* __restrict__ sact,
+                                  double* __restrict__ hact, int* __restrict__ flags) {
+    __shared__ double s_l[MAX_ACT * MAX_ACT], h_l[MAX_ACT * MAX_ACT];
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        const int i = t % n, j = t / n;
+        double sv = 0.0, hv = 0.0;
+        for (int mu = 0; mu < n_bf; ++mu) {
+            const double ci = Cn[(size_t)i * n_bf + mu];
+            sv += ci * SC[mu + (size_t)n_bf * j];
+            hv += ci * HC[mu + (size_t)n_bf * j];
+        }
+        s_l[t] = sv;
+        h_l[t] = hv;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        const int i = t % n, j = t / n;
+        sact[t] = 0.5 * (s_l[t] + s_l[j + n * i]);
+        hact[t] = 0.5 * (h_l[t] + h_l[j + n * i]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // Cholesky of the symmetrised S_act in place (the reference frame factorisation,
+        // vbscf/integrals/active/space.cpp:99-113, fails the evaluation when it breaks down)
+        double a[MAX_ACT * MAX_ACT];
+        for (int t = 0; t < n * n; ++t) a[t] = 0.5 * (s_l[t] + s_l[(t / n) + n * (t % n)]);
+        for (int j = 0; j < n; ++j) {
+            double d = a[j + n * j];
+            for (int k = 0; k < j; ++k) d -= a[j + n * k] * a[j + n * k];
+            if (!(d > 0.0)) {
+                flags[FLAG_ACTIVE_OVERLAP] = 1;
+                return;
+            }
+            d = sqrt(d);
+            a[j + n * j] = d;
+            for (int i = j + 1; i < n; ++i) {
+                double v = a[i + n * j];
+                for (int k = 0; k < j; ++k) v -= a[i + n * k] * a[j + n * k];
+                a[i + n * j] = v / d;
+            }
+        }
+    }
+}
+
+// T[A, q, mu] = sum_nu B[A, (mu nu)] Cn[nu, q]; T index A + n_aux*(q + n*mu).
+// grid (ceil(n_aux/blockDim), n_bf). Each packed B column is read by its two AO rows.
+__global__ void k_ri_one_index(const double* __restrict__ B, const double* __restrict__ Cn, int n_bf, int n,
+                               int n_aux, double* __restrict__ T) {
+    const int A = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mu = blockIdx.y;
+    if (A >= n_aux) return;
+    double acc[MAX_ACT];
+#pragma unroll
+    for (int q = 0; q < MAX_ACT; ++q) acc[q] = 0.0;
+    for (int nu = 0; nu < n_bf; ++nu) {
+        const double b = B[(size_t)A + (size_t)n_aux * (size_t)pair_index(mu, nu)];
+#pragma unroll
+        for (int q = 0; q < MAX_ACT; ++q)
+            if (q < n) acc[q] += b * Cn[(size_t)q * n_bf + nu];
+    }
+#pragma unroll
+    for (int q = 0; q < MAX_ACT; ++q)
+        if (q < n) T[(size_t)A + (size_t)n_aux * (q + (size_t)n * mu)] = acc[q];
+}
+
+// L[A, (p,q)] = sum_mu T[A,q,mu] Cn[mu,p], q <= p. grid (ceil(n_aux/blockDim), n).
+__global__ void k_ri_pair_factors(const double* __restrict__ T, const double* __restrict__ Cn, int n_bf, int n,
+                                  int n_aux, double* __restrict__ L) {
+    const int A = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (A >= n_aux) return;
+    double acc[MAX_ACT];
+#pragma unroll
+    for (int q = 0; q < MAX_ACT; ++q) acc[q] = 0.0;
+    for (int mu = 0; mu < n_bf; ++mu) {
+        const double cp = Cn[(size_t)p * n_bf + mu];
+        const double* t = T + (size_t)A + (size_t)n_aux * ((size_t)n * mu);
+#pragma unroll
+        for (int q = 0; q < MAX_ACT; ++q)
+            if (q <= p) acc[q] += t[(size_t)n_aux * q] * cp;
+    }
+#pragma unroll
+    for (int q = 0; q < MAX_ACT; ++q)
+        if (q <= p) L[(size_t)A + (size_t)n_aux * pair_index(p, q)] = acc[q];
+}
+
+// packed G[(P,Q)] = sum_A L[A,P] L[A,Q]; one wavefront per packed entry.
+__global__ void k_ri_pair_gram(const double* __restrict__ L, int n_aux, int np, double* __restrict__ G) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int ng = np * (np + 1) / 2;
+    if (wave >= ng) return;
+    int P = (int)((sqrt(8.0 * wave + 1.0) - 1.0) * 0.5);
+    while (P * (P + 1) / 2 > wave) --P;
+    while ((P + 1) * (P + 2) / 2 <= wave) ++P;
+    const int Q = wave - P * (P + 1) / 2;
+    double acc = 0.0;
+    for (int A = lane; A < n_aux; A += 64) acc += L[(size_t)A + (size_t)n_aux * P] * L[(size_t)A + (size_t)n_aux * Q];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) G[wave] = acc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// determinant pairs: one wavefront per unordered determinant pair (d <= e)
+// ---------------------------------------------------------------------------------------------
+
+// wave-uniform broadcast of lane `src`
+__device__ __forceinline__ double wave_bcast(double v, int src) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// k-th set bit of a mask (k < popcount)
+__device__ __forceinline__ int nth_bit(uint64_t m, int k) {
+    for (int i = 0; i < k; ++i) m &= m - 1;
+    return __builtin_ctzll(m);
+}
+
+struct BlockInverse {
+    double dens;    // this lane's D[r,l] = Y(l,r), Y = X^{-1}
+    double det;     // wave-uniform determinant
+    bool singular;  // wave-uniform: rank below the block size under the 1e-12 relative pivot rule
+};
+
+// Gauss-Jordan elimination with complete pivoting of the m x m block held one element per lane
+// (lane = r + 8*l), augmented with the identity. Pivot choice and the rank rule follow Eigen's
+// FullPivLU as used by DeterminantOverlapResolver (determinants/algebra/overlap.cpp:50-110):
+// largest remaining |entry|; rank counts pivots above 1e-12 * max|pivot|.
+__device__ BlockInverse invert_block(double x, int m, int lane) {
+    const int r = lane & 7, l = lane >> 3;
+    const bool valid = r < m && l < m;
+    double aug = (valid && r == l) ? 1.0 : 0.0;
+    uint32_t rows_used = 0, cols_used = 0;
+    int inversions = 0;
+    double det = 1.0, maxpiv = 0.0, minpiv = 1.0e308;
+    int my_row = 0;
+    double my_piv = 1.0;
+    bool singular = false;
+    for (int k = 0; k < m; ++k) {
+        double cand = (valid && !((rows_used >> r) & 1u) && !((cols_used >> l) & 1u)) ? fabs(x) : -1.0;
+        int idx = lane;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double oc = __shfl_xor(cand, off, 64);
+            const int oi = __shfl_xor(idx, off, 64);
+            if (oc > cand || (oc == cand && oi < idx)) {
+                cand = oc;
+                idx = oi;
+            }
+        }
+        if (!(cand > 0.0)) {  // exactly zero corner (or non-finite): rank deficient
+            singular = true;
+            break;
+        }
+        const int pr = idx & 7, pc = idx >> 3;
+        const double p = wave_bcast(x, idx);
+        maxpiv = fmax(maxpiv, cand);
+        minpiv = fmin(minpiv, cand);
+        inversions += __builtin_popcount(rows_used >> (pr + 1)) + __builtin_popcount(cols_used >> (pc + 1));
+        const double x_r_pc = __shfl(x, r + 8 * pc, 64);
+        const double x_pr_l = __shfl(x, pr + 8 * l, 64);
+        const double a_pr_l = __shfl(aug, pr + 8 * l, 64);
+        if (valid && r != pr) {
+            const double f = x_r_pc / p;
+            x -= f * x_pr_l;
+            aug -= f * a_pr_l;
+        }
+        if (l == pc) {
+            my_row = pr;
+            my_piv = p;
+        }
+        det *= p;
+        rows_used |= 1u << pr;
+        cols_used |= 1u << pc;
+    }
+    if (!singular && minpiv <= 1.0e-12 * maxpiv) singular = true;
+    BlockInverse out;
+    out.singular = singular;
+    out.det = (inversions & 1) ? -det : det;
+    // Y(c, j) = M(pivot row of c, j) / pivot(c); lane (r,l) wants Y(l, r)
+    const double a = __shfl(aug, (my_row & 7) + 8 * (r & 7), 64);
+    out.dens = (valid && !singular) ? a / my_piv : 0.0;
+    return out;
+}
+
+struct PairShared {
+    double* G;      // packed kernel, ng
+    double* sact;   // n x n
+    double* hact;   // n x n
+    double* dg;     // backward accumulators (ng), may be null
+    double* ds;     // n x n
+    double* dh;     // n x n
+};
+
+struct WaveSlab {
+    double D[2][64];
+    double F[2][64];
+    double Z[64];
+    int occL[2][8];
+    int occR[2][8];
+};
+
+template <bool Backward>
+__global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob, const double* __restrict__ sact_g,
+                                               const double* __restrict__ hact_g, int n, int ng,
+                                               const uint64_t* __restrict__ amask, const uint64_t* __restrict__ bmask,
+                                               const int* __restrict__ det_a, const int* __restrict__ det_b, int n_det,
+                                               long long n_pairs, double* __restrict__ Hdet, double* __restrict__ Sdet,
+                                               const double* __restrict__ dcoef, const double* __restrict__ energy,
+                                               double* __restrict__ partials, int* __restrict__ flags) {
+    extern __shared__ double lds[];
+    PairShared sh;
+    sh.G = lds;
+    sh.sact = sh.G + ng;
+    sh.hact = sh.sact + n * n;
+    double* tail = sh.hact + n * n;
+    if (Backward) {
+        sh.dg = tail;
+        sh.ds = sh.dg + ng;
+        sh.dh = sh.ds + n * n;
+        tail = sh.dh + n * n;
+    }
+    WaveSlab* slabs = reinterpret_cast<WaveSlab*>(tail);
+    for (int t = threadIdx.x; t < ng; t += blockDim.x) sh.G[t] = G_glob[t];
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        sh.sact[t] = sact_g[t];
+        sh.hact[t] = hact_g[t];
+    }
+    if (Backward) {
+        for (int t = threadIdx.x; t < ng; t += blockDim.x) sh.dg[t] = 0.0;
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            sh.ds[t] = 0.0;
+            sh.dh[t] = 0.0;
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave_in_block = threadIdx.x >> 6;
+    const int waves_per_block = blockDim.x >> 6;
+    WaveSlab& ws = slabs[wave_in_block];
+    const long long first = (long long)blockIdx.x * waves_per_block + wave_in_block;
+    const long long stride = (long long)gridDim.x * waves_per_block;
+    const double E = Backward ? energy[0] : 0.0;
+    const int r = lane & 7, l = lane >> 3;
+
+    for (long long t = first; t < n_pairs; t += stride) {
+        // unordered pair t = e(e+1)/2 + d, d <= e; d is the left (bra) determinant
+        long long e = (long long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while (e * (e + 1) / 2 > t) --e;
+        while ((e + 1) * (e + 2) / 2 <= t) ++e;
+        const int d = (int)(t - e * (e + 1) / 2);
+        const int ee = (int)e;
+        double weight = 0.0;
+        if (Backward) {
+            weight = dcoef[d] * dcoef[ee] * (d == ee ? 1.0 : 2.0);
+            if (weight == 0.0) continue;
+        }
+        const uint64_t mL[2] = {amask[det_a[d]], bmask[det_b[d]]};
+        const uint64_t mR[2] = {amask[det_a[ee]], bmask[det_b[ee]]};
+        int m[2];
+        for (int s = 0; s < 2; ++s) {
+            m[s] = __builtin_popcountll(mL[s]);
+            if (lane < m[s]) {
+                ws.occL[s][lane] = nth_bit(mL[s], lane);
+                ws.occR[s][lane] = nth_bit(mR[s], lane);
+            }
+        }
+        wave_sync();
+        // overlap blocks and their inverses
+        double det[2];
+        bool singular = false;
+        for (int s = 0; s < 2; ++s) {
+            double x = 0.0;
+            if (r < m[s] && l < m[s]) x = sh.sact[ws.occR[s][r] + n * ws.occL[s][l]];
+            if (m[s] == 0) {
+                det[s] = 1.0;
+                ws.D[s][lane] = 0.0;
+                continue;
+            }
+            const BlockInverse inv = invert_block(x, m[s], lane);
+            det[s] = inv.det;
+            singular = singular || inv.singular;
+            ws.D[s][lane] = inv.dens;
+        }
+        if (singular) {
+            if (lane == 0) atomicAdd(&flags[FLAG_SINGULAR_PAIRS], 1);
+            continue;
+        }
+        wave_sync();
+        // F_s[r,l] = dPhi/dD_s[r,l]: one-electron + same-spin (Coulomb - exchange) + opposite spin
+        double phi_part = 0.0;
+        for (int s = 0; s < 2; ++s) {
+            const int ms = m[s], mo = m[1 - s];
+            double f = 0.0;
+            if (r < ms && l < ms) {
+                const int Rr = ws.occR[s][r], Ll = ws.occL[s][l];
+                const double hv = sh.hact[Rr + n * Ll];
+                const int P1 = pair_index(Rr, Ll);
+                f = hv;
+                for (int l2 = 0; l2 < ms; ++l2) {
+                    const int L2 = ws.occL[s][l2];
+                    const int Px = pair_index(Rr, L2);
+                    for (int r2 = 0; r2 < ms; ++r2) {
+                        const int R2 = ws.occR[s][r2];
+                        const double d2 = ws.D[s][r2 + 8 * l2];
+                        const double J = sh.G[pair_index(P1, pair_index(R2, L2))];
+                        const double K = sh.G[pair_index(Px, pair_index(R2, Ll))];
+                        f += (J - K) * d2;
+                    }
+                }
+                for (int l2 = 0; l2 < mo; ++l2) {
+                    const int L2 = ws.occL[1 - s][l2];
+                    for (int r2 = 0; r2 < mo; ++r2) {
+                        const int R2 = ws.occR[1 - s][r2];
+                        f += sh.G[pair_index(P1, pair_index(R2, L2))] * ws.D[1 - s][r2 + 8 * l2];
+                    }
+                }
+                phi_part += 0.5 * ws.D[s][lane] * (hv + f);
+            }
+            ws.F[s][lane] = f;
+        }
+        const double phi = wave_sum(phi_part);
+        const double Dtot = det[0] * det[1];
+        if (!Backward) {
+            if (lane == 0) {
+                const double H = Dtot * phi;
+                Hdet[(size_t)d + (size_t)n_det * ee] = H;
+                Hdet[(size_t)ee + (size_t)n_det * d] = H;
+                Sdet[(size_t)d + (size_t)n_det * ee] = Dtot;
+                Sdet[(size_t)ee + (size_t)n_det * d] = Dtot;
+            }
+            continue;
+        }
+        // ---- backward: F_pair = w (H - E S) ----
+        const double wH = weight, wS = -E * weight;
+        const double c = Dtot;
+        wave_sync();
+        for (int s = 0; s < 2; ++s) {
+            const int ms = m[s];
+            if (ms == 0) continue;
+            // Z[a,l] = sum_b F[b,a] D[b,l]
+            double z = 0.0;
+            if (r < ms && l < ms)
+                for (int b = 0; b < ms; ++b) z += ws.F[s][b + 8 * r] * ws.D[s][b + 8 * l];
+            ws.Z[lane] = z;
+            wave_sync();
+            if (r < ms && l < ms) {
+                double dfd = 0.0;  // (D F^T D)[r,l] = sum_a D[r,a] Z[a,l]
+                for (int a = 0; a < ms; ++a) dfd += ws.D[s][r + 8 * a] * ws.Z[a + 8 * l];
+                const double Drl = ws.D[s][lane];
+                const int Rr = ws.occR[s][r], Ll = ws.occL[s][l];
+                atomicAdd(&sh.ds[Rr + n * Ll], c * ((wH * phi + wS) * Drl - wH * dfd));
+                atomicAdd(&sh.dh[Rr + n * Ll], c * wH * Drl);
+                // same-spin two-electron adjoint, r1 < r2 and l1 < l2 from this lane's (r,l)
+                const double cw = c * wH;
+                for (int l2 = l + 1; l2 < ms; ++l2) {
+                    const int L2 = ws.occL[s][l2];
+                    for (int r2 = r + 1; r2 < ms; ++r2) {
+                        const int R2 = ws.occR[s][r2];
+                        const double mnr = Drl * ws.D[s][r2 + 8 * l2] - ws.D[s][r + 8 * l2] * ws.D[s][r2 + 8 * l];
+                        atomicAdd(&sh.dg[pair_index(pair_index(Rr, Ll), pair_index(R2, L2))], cw * mnr);
+                        atomicAdd(&sh.dg[pair_index(pair_index(Rr, L2), pair_index(R2, Ll))], -cw * mnr);
+                    }
+                }
+                if (s == 0) {  // opposite spin, once per (alpha entry, beta entry)
+                    const int mo = m[1];
+                    const int Pa = pair_index(Rr, Ll);
+                    for (int l2 = 0; l2 < mo; ++l2) {
+                        const int L2 = ws.occL[1][l2];
+                        for (int r2 = 0; r2 < mo; ++r2) {
+                            const int R2 = ws.occR[1][r2];
+                            atomicAdd(&sh.dg[pair_index(pair_index(R2, L2), Pa)], cw * Drl * ws.D[1][r2 + 8 * l2]);
+                        }
+                    }
+                }
+            }
+            wave_sync();
+        }
+    }
+    if (Backward) {
+        __syncthreads();
+        const int stride_part = ng + 2 * n * n;
+        double* out = partials + (size_t)blockIdx.x * stride_part;
+        for (int t = threadIdx.x; t < ng; t += blockDim.x) out[t] = sh.dg[t];
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            out[ng + t] = sh.ds[t];
+            out[ng + n * n + t] = sh.dh[t];
+        }
+    }
+}
+
+// sum block partials in a fixed order: out[i] = sum_b partials[b*len + i]
+__global__ void k_reduce_partials(const double* __restrict__ partials, int n_blocks, int len, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    double acc = 0.0;
+    for (int b = 0; b < n_blocks; ++b) acc += partials[(size_t)b * len + i];
+    out[i] = acc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// structure matrices: Y = Hdet T (thread per (det, structure)), then T^T Y (thread per (I, J))
+// ---------------------------------------------------------------------------------------------
+__global__ void k_structure_right(const double* __restrict__ Hdet, const double* __restrict__ Sdet, int n_det,
+                                  int n_str, const long long* __restrict__ soff, const int* __restrict__ sdet,
+                                  const double* __restrict__ scoef, double* __restrict__ Yh, double* __restrict__ Ys) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)n_det * n_str) return;
+    const int d = (int)(id % n_det), J = (int)(id / n_det);
+    double h = 0.0, s = 0.0;
+    for (long long t = soff[J]; t < soff[J + 1]; ++t) {
+        const size_t at = (size_t)d + (size_t)n_det * sdet[t];
+        h += scoef[t] * Hdet[at];
+        s += scoef[t] * Sdet[at];
+    }
+    Yh[id] = h;
+    Ys[id] = s;
+}
+
+__global__ void k_structure_left(const double* __restrict__ Yh, const double* __restrict__ Ys, int n_det, int n_str,
+                                 const long long* __restrict__ soff, const int* __restrict__ sdet,
+                                 const double* __restrict__ scoef, double* __restrict__ Hs, double* __restrict__ Ss) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)n_str * n_str) return;
+    const int I = (int)(id % n_str), J = (int)(id / n_str);
+    double h = 0.0, s = 0.0;
+    for (long long t = soff[I]; t < soff[I + 1]; ++t) {
+        const size_t at = (size_t)sdet[t] + (size_t)n_det * J;
+        h += scoef[t] * Yh[at];
+        s += scoef[t] * Ys[at];
+    }
+    Hs[id] = h;
+    Ss[id] = s;
+}
+
+// a[d] = sum_terms coef * c[structure]
+__global__ void k_det_coefficients(const long long* __restrict__ off, const int* __restrict__ ts,
+                                   const double* __restrict__ tc, const double* __restrict__ coef, int n_det,
+                                   double* __restrict__ a) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n_det) return;
+    double acc = 0.0;
+    for (long long t = off[d]; t < off[d + 1]; ++t) acc += tc[t] * coef[ts[t]];
+    a[d] = acc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// generalised eigenproblem H c = E S c, lowest root, one workgroup.
+// Cholesky S = L L^T, M = L^{-1} H L^{-T}, Householder tridiagonalisation, bisection (Sturm
+// counts, multisection over the block) for the lowest eigenvalue, inverse iteration on the
+// tridiagonal, back-transformation, c = L^{-T} z so that c^T S c = 1.
+// work: gep_work_doubles(n) = 3*n*n + 9*n doubles.
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ Hs, const double* __restrict__ Ss,
+                                                     int n, double e_nuc, double* __restrict__ work,
+                                                     double* __restrict__ coef, double* __restrict__ energy,
+                                                     int* __restrict__ flags) {
+    double* Lm = work;              // n x n, lower Cholesky factor
+    double* M = work + n * n;       // n x n
+    double* V = work + 2 * n * n;   // Householder vectors (column k holds v_k)
+    double* dd = work + 3 * n * n;  // diagonal
+    double* ee = dd + n;            // off-diagonal
+    double* y = ee + n;             // tridiagonal eigenvector / scratch
+    double* beta = y + n;           // Householder scalars
+    double* tdl = beta + n;         // tridiagonal LU factors (inverse iteration)
+    double* tdu = tdl + n;
+    double* tdu2 = tdu + n;
+    double* tdg = tdu2 + n;
+    int* ipiv = reinterpret_cast<int*>(tdg + n);  // n ints in the last n doubles
+    __shared__ double red[16];
+    __shared__ double sh_scalar[4];
+    __shared__ int sh_flag;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int t = tid; t < n * n; t += nt) {
+        const int i = t % n, j = t / n;
+        Lm[t] = 0.5 * (Ss[t] + Ss[j + n * i]);
+        M[t] = 0.5 * (Hs[t] + Hs[j + n * i]);
+    }
+    if (tid == 0) sh_flag = 0;
+    __syncthreads();
+    // Cholesky (right-looking, lower)
+    for (int k = 0; k < n; ++k) {
+        if (tid == 0) {
+            const double a = Lm[k + n * k];
+            if (!(a > 0.0)) sh_flag = 1;
+            sh_scalar[0] = a > 0.0 ? sqrt(a) : 1.0;
+        }
+        __syncthreads();
+        if (sh_flag) break;
+        const double dk = sh_scalar[0];
+        for (int i = k + tid; i < n; i += nt) Lm[i + n * k] = (i == k) ? dk : Lm[i + n * k] / dk;
+        __syncthreads();
+        const int rem = n - k - 1;
+        for (int t = tid; t < rem * rem; t += nt) {
+            const int i = k + 1 + t % rem, j = k + 1 + t / rem;
+            if (j <= i) Lm[i + n * j] -= Lm[i + n * k] * Lm[j + n * k];
+        }
+        __syncthreads();
+    }
+    if (sh_flag) {
+        if (tid == 0) flags[FLAG_METRIC] = 1;
+        return;
+    }
+    // X = L^{-1} H (in M, forward substitution, right-looking)
+    for (int k = 0; k < n; ++k) {
+        const double lkk = Lm[k + n * k];
+        for (int j = tid; j < n; j += nt) M[k + n * j] /= lkk;
+        __syncthreads();
+        const int rem = n - k - 1;
+        for (int t = tid; t < rem * n; t += nt) {
+            const int i = k + 1 + t % rem, j = t / rem;
+            M[i + n * j] -= Lm[i + n * k] * M[k + n * j];
+        }
+        __syncthreads();
+    }
+    // transpose X into V, then M = L^{-1} X^T
+    for (int t = tid; t < n * n; t += nt) {
+        const int i = t % n, j = t / n;
+        V[t] = M[j + n * i];
+    }
+    __syncthreads();
+    for (int t = tid; t < n * n; t += nt) M[t] = V[t];
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const double lkk = Lm[k + n * k];
+        for (int j = tid; j < n; j += nt) M[k + n * j] /= lkk;
+        __syncthreads();
+        const int rem = n - k - 1;
+        for (int t = tid; t < rem * n; t += nt) {
+            const int i = k + 1 + t % rem, j = t / rem;
+            M[i + n * j] -= Lm[i + n * k] * M[k + n * j];
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < n * n; t += nt) {
+        const int i = t % n, j = t / n;
+        if (i < j) {
+            const double s = 0.5 * (M[i + n * j] + M[j + n * i]);
+            M[i + n * j] = s;
+            M[j + n * i] = s;
+        }
+    }
+    for (int t = tid; t < n * n; t += nt) V[t] = 0.0;
+    __syncthreads();
+    // Householder tridiagonalisation: for k, reflect rows/cols k+1..n-1 of column k
+    for (int k = 0; k < n - 2; ++k) {
+        const int m0 = k + 1, len = n - m0;
+        double part = 0.0;
+        for (int i = m0 + tid; i < n; i += nt) part += M[i + n * k] * M[i + n * k];
+        const double sigma = block_sum(part, red);
+        const double x0 = M[m0 + n * k];
+        const double alpha = sigma > 0.0 ? (x0 >= 0.0 ? -sqrt(sigma) : sqrt(sigma)) : 0.0;
+        // v = x - alpha e1, beta = 2 / v^T v
+        for (int i = m0 + tid; i < n; i += nt) V[i + n * k] = M[i + n * k] - (i == m0 ? alpha : 0.0);
+        __syncthreads();
+        const double vtv = sigma - 2.0 * alpha * x0 + alpha * alpha;
+        const double bk = (sigma > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
+        if (tid == 0) beta[k] = bk;
+        // p = bk * A v (trailing block), stored in y
+        for (int i = m0 + tid; i < n; i += nt) {
+            double acc = 0.0;
+            for (int j = m0; j < n; ++j) acc += M[i + n * j] * V[j + n * k];
+            y[i] = bk * acc;
+        }
+        __syncthreads();
+        double kp = 0.0;
+        for (int i = m0 + tid; i < n; i += nt) kp += V[i + n * k] * y[i];
+        const double K = 0.5 * bk * block_sum(kp, red);
+        // w = p - K v ; A -= v w^T + w v^T
+        for (int i = m0 + tid; i < n; i += nt) y[i] -= K * V[i + n * k];
+        __syncthreads();
+        for (int t = tid; t < len * len; t += nt) {
+            const int i = m0 + t % len, j = m0 + t / len;
+            M[i + n * j] -= V[i + n * k] * y[j] + y[i] * V[j + n * k];
+        }
+        for (int i = m0 + tid; i < n; i += nt) {
+            const double v = (i == m0 && bk != 0.0) ? alpha : (bk != 0.0 ? 0.0 : M[i + n * k]);
+            M[i + n * k] = v;
+            M[k + n * i] = v;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += nt) {
+        dd[i] = M[i + n * i];
+        ee[i] = (i + 1 < n) ? M[(i + 1) + n * i] : 0.0;
+    }
+    __syncthreads();
+    // lowest eigenvalue by multisection on Sturm counts
+    __shared__ double lo_hi[2];
+    if (tid == 0) {
+        double lo = 1e300, hi = -1e300;
+        for (int i = 0; i < n; ++i) {
+            const double rad = (i > 0 ? fabs(ee[i - 1]) : 0.0) + (i + 1 < n ? fabs(ee[i]) : 0.0);
+            lo = fmin(lo, dd[i] - rad);
+            hi = fmax(hi, dd[i] + rad);
+        }
+        const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
+        lo_hi[0] = lo - pad;
+        lo_hi[1] = hi + pad;
+    }
+    __syncthreads();
+    __shared__ int first_true;
+    for (int it = 0; it < 64; ++it) {
+        const double lo = lo_hi[0], hi = lo_hi[1];
+        if (!(hi - lo > 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)))) break;
+        if (tid == 0) first_true = nt;
+        __syncthreads();
+        // point t in (lo, hi]; count of eigenvalues < x (Sturm sequence of the LDL^T pivots)
+        const double x = lo + (hi - lo) * (double)(tid + 1) / (double)nt;
+        int count = 0;
+        double q = 1.0;
+        for (int i = 0; i < n; ++i) {
+            const double e2 = i > 0 ? ee[i - 1] * ee[i - 1] : 0.0;
+            q = dd[i] - x - (i > 0 ? e2 / q : 0.0);
+            if (q == 0.0) q = -1e-300;
+            if (q < 0.0) ++count;
+        }
+        if (count >= 1) atomicMin(&first_true, tid);
+        __syncthreads();
+        if (tid == 0) {
+            const int f = first_true;
+            const double nlo = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)nt;
+            const double nhi = f >= nt ? hi : lo + (hi - lo) * (double)(f + 1) / (double)nt;
+            lo_hi[0] = nlo;
+            lo_hi[1] = nhi;
+        }
+        __syncthreads();
+    }
+    const double lambda = 0.5 * (lo_hi[0] + lo_hi[1]);
+    // inverse iteration on T - lambda I (serial, partial-pivoting tridiagonal LU)
+    if (tid == 0) {
+        double* dl = tdl;
+        double* du = tdu;
+        double* du2 = tdu2;
+        double* dg = tdg;
+        for (int i = 0; i < n; ++i) {
+            dg[i] = dd[i] - lambda;
+            if (i + 1 < n) {
+                dl[i] = ee[i];
+                du[i] = ee[i];
+            }
+        }
+        // LU with partial pivoting (LAPACK dgttrf scheme)
+        for (int i = 0; i < n; ++i) du2[i] = 0.0;
+        for (int i = 0; i + 1 < n; ++i) {
+            if (fabs(dg[i]) >= fabs(dl[i])) {
+                ipiv[i] = i;
+                if (dg[i] == 0.0) dg[i] = 1e-300;
+                const double f = dl[i] / dg[i];
+                dl[i] = f;
+                dg[i + 1] -= f * du[i];
+            } else {
+                ipiv[i] = i + 1;
+                const double f = dg[i] / dl[i];
+                dg[i] = dl[i];
+                dl[i] = f;
+                const double t = du[i];
+                du[i] = dg[i + 1];
+                dg[i + 1] = t - f * dg[i + 1];
+                if (i + 2 < n) {
+                    du2[i] = du[i + 1];
+                    du[i + 1] = -f * du[i + 1];
+                }
+            }
+        }
+        if (n > 0) {
+            ipiv[n - 1] = n - 1;
+            if (dg[n - 1] == 0.0) dg[n - 1] = 1e-300;
+        }
+        for (int i = 0; i < n; ++i) y[i] = 1.0;
+        for (int iter = 0; iter < 3; ++iter) {
+            // solve L U z = y
+            for (int i = 0; i + 1 < n; ++i) {
+                if (ipiv[i] == i) {
+                    y[i + 1] -= dl[i] * y[i];
+                } else {
+                    const double t = y[i];
+                    y[i] = y[i + 1];
+                    y[i + 1] = t - dl[i] * y[i];
+                }
+            }
+            for (int i = n - 1; i >= 0; --i) {
+                double v = y[i];
+                if (i + 1 < n) v -= du[i] * y[i + 1];
+                if (i + 2 < n) v -= du2[i] * y[i + 2];
+                y[i] = v / dg[i];
+            }
+            double nrm = 0.0;
+            for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
+            nrm = sqrt(nrm);
+            for (int i = 0; i < n; ++i) y[i] /= nrm;
+        }
+        sh_scalar[1] = lambda;
+    }
+    __syncthreads();
+    // back-transform z = Q y with Q = H_0 H_1 ... H_{n-3}: apply H_{n-3} first
+    for (int k = n - 3; k >= 0; --k) {
+        const int m0 = k + 1;
+        double part = 0.0;
+        for (int i = m0 + tid; i < n; i += nt) part += V[i + n * k] * y[i];
+        const double s = beta[k] * block_sum(part, red);
+        for (int i = m0 + tid; i < n; i += nt) y[i] -= s * V[i + n * k];
+        __syncthreads();
+    }
+    // c = L^{-T} z (back substitution, right-looking over columns of L)
+    for (int k = n - 1; k >= 0; --k) {
+        if (tid == 0) y[k] /= Lm[k + n * k];
+        __syncthreads();
+        const double ck = y[k];
+        for (int i = tid; i < k; i += nt) y[i] -= Lm[k + n * i] * ck;
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += nt) coef[i] = y[i];
+    if (tid == 0) {
+        energy[0] = sh_scalar[1];
+        energy[1] = sh_scalar[1] + e_nuc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// gradient tail
+// ---------------------------------------------------------------------------------------------
+
+// dL[A,P] = sum_Q M[P,Q] L[A,Q], M[P,Q] = dg(P,Q) off the diagonal, 2 dg(P,P) on it.
+__global__ void k_ri_pair_adjoint(const double* __restrict__ L, const double* __restrict__ dg, int n_aux, int np,
+                                  double* __restrict__ dL) {
+    const int A = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = blockIdx.y;
+    if (A >= n_aux) return;
+    double acc = 0.0;
+    for (int Q = 0; Q < np; ++Q) {
+        const double m = (P == Q) ? 2.0 * dg[pair_index(P, P)] : dg[pair_index(P, Q)];
+        acc += m * L[(size_t)A + (size_t)n_aux * Q];
+    }
+    dL[(size_t)A + (size_t)n_aux * P] = acc;
+}
+
+// gcn[mu, p] = sum_i SC[mu,i] (ds+ds^T)[i,p] + HC[mu,i] (dh+dh^T)[i,p]
+//            + sum_q f_pq sum_A dL[A,(p,q)] T[A,q,mu],  f_pq = 2 if p == q else 1.
+// one block per AO index mu.
+__global__ void k_orbital_gradient(const double* __restrict__ T, const double* __restrict__ dL,
+                                   const double* __restrict__ SC, const double* __restrict__ HC,
+                                   const double* __restrict__ dsdh, int n_bf, int n, int n_aux, int ng,
+                                   double* __restrict__ gcn) {
+    __shared__ double red[16];
+    __shared__ double part[MAX_ACT];
+    const int mu = blockIdx.x;
+    double acc[MAX_ACT];
+#pragma unroll
+    for (int p = 0; p < MAX_ACT; ++p) acc[p] = 0.0;
+    for (int A = threadIdx.x; A < n_aux; A += blockDim.x) {
+        for (int q = 0; q < n; ++q) {
+            const double t = T[(size_t)A + (size_t)n_aux * (q + (size_t)n * mu)];
+#pragma unroll
+            for (int p = 0; p < MAX_ACT; ++p)
+                if (p < n) acc[p] += (p == q ? 2.0 : 1.0) * dL[(size_t)A + (size_t)n_aux * pair_index(p, q)] * t;
+        }
+    }
+    for (int p = 0; p < n; ++p) {
+        const double s = block_sum(acc[p], red);
+        if (threadIdx.x == 0) part[p] = s;
+    }
+    __syncthreads();
+    const double* ds = dsdh + ng;
+    const double* dh = dsdh + ng + n * n;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        double g = part[p];
+        for (int i = 0; i < n; ++i) {
+            g += SC[mu + (size_t)n_bf * i] * (ds[i + n * p] + ds[p + n * i]);
+            g += HC[mu + (size_t)n_bf * i] * (dh[i + n * p] + dh[p + n * i]);
+        }
+        gcn[mu + (size_t)n_bf * p] = g;
+    }
+}
+
+// raw-slot gradient: grad[k, mu] = (g - (g.cn_k) S cn_k)[mu] / sqrt(nu2_k) on present slots.
+__global__ void k_normalization_pullback(const double* __restrict__ gcn, const double* __restrict__ Cn,
+                                         const double* __restrict__ SC, const double* __restrict__ nu2,
+                                         const uint8_t* __restrict__ support, int n_bf, double* __restrict__ grad) {
+    __shared__ double red[16];
+    const int k = blockIdx.x;
+    double part = 0.0;
+    for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) part += gcn[mu + (size_t)n_bf * k] * Cn[(size_t)k * n_bf + mu];
+    const double dot = block_sum(part, red);
+    const double f = 1.0 / sqrt(nu2[k]);
+    for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
+        const size_t at = (size_t)k * n_bf + mu;
+        grad[at] = (support && !support[at]) ? 0.0 : f * (gcn[mu + (size_t)n_bf * k] - dot * SC[mu + (size_t)n_bf * k]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------------------------
+size_t pairs_lds_bytes(int n, int ng, bool backward) {
+    size_t doubles = (size_t)ng + 2 * (size_t)n * n;
+    if (backward) doubles += (size_t)ng + 2 * (size_t)n * n;
+    return doubles * sizeof(double) + 4 * sizeof(WaveSlab);
+}
+
+hipError_t configure_kernels(int n, int ng) {
+    const size_t fwd = pairs_lds_bytes(n, ng, false), bwd = pairs_lds_bytes(n, ng, true);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<false>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)fwd);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bwd);
+}
+
+void launch_orbital_prep(hipStream_t st, const double* C, const uint8_t* support, const double* S, const double* h,
+                         int n_bf, int n, double* Cn, double* SC, double* HC, double* nu2, int* flags) {
+    const size_t lds = ((size_t)n_bf + 16) * sizeof(double);
+    hipLaunchKernelGGL(k_orbital_prep, dim3(n), dim3(256), lds, st, C, support, S, h, n_bf, n, Cn, SC, HC, nu2, flags);
+}
+
+void launch_active_matrices(hipStream_t st, const double* Cn, const double* SC, const double* HC, int n_bf, int n,
+                            double* sact, double* hact, int* flags) {
+    hipLaunchKernelGGL(k_active_matrices, dim3(1), dim3(256), 0, st, Cn, SC, HC, n_bf, n, sact, hact, flags);
+}
+
+void launch_transform(hipStream_t st, const double* B, const double* Cn, int n_bf, int n, int n_aux, double* T,
+                      double* L, double* G) {
+    const int tb = 64;
+    hipLaunchKernelGGL(k_ri_one_index, dim3((n_aux + tb - 1) / tb, n_bf), dim3(tb), 0, st, B, Cn, n_bf, n, n_aux, T);
+    hipLaunchKernelGGL(k_ri_pair_factors, dim3((n_aux + tb - 1) / tb, n), dim3(tb), 0, st, T, Cn, n_bf, n, n_aux, L);
+    const int np = n * (n + 1) / 2, ng = np * (np + 1) / 2;
+    hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4), dim3(256), 0, st, L, n_aux, np, G);
+}
+
+void launch_pairs(hipStream_t st, bool backward, int n_blocks, const double* G, const double* sact, const double* hact,
+                  int n, int ng, const uint64_t* amask, const uint64_t* bmask, const int* det_a, const int* det_b,
+                  int n_det, double* Hdet, double* Sdet, const double* dcoef, const double* energy, double* partials,
+                  int* flags) {
+    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
+    const size_t lds = pairs_lds_bytes(n, ng, backward);
+    if (backward)
+        hipLaunchKernelGGL(k_pairs<true>, dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask, bmask, det_a,
+                           det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
+    else
+        hipLaunchKernelGGL(k_pairs<false>, dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask, bmask,
+                           det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
+}
+
+void launch_reduce_partials(hipStream_t st, const double* partials, int n_blocks, int len, double* out) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 255) / 256), dim3(256), 0, st, partials, n_blocks, len, out);
+}
+
+void launch_structure(hipStream_t st, const double* Hdet, const double* Sdet, int n_det, int n_str,
+                      const long long* soff, const int* sdet, const double* scoef, double* Yh, double* Ys, double* Hs,
+                      double* Ss) {
+    const long long a = (long long)n_det * n_str, b = (long long)n_str * n_str;
+    hipLaunchKernelGGL(k_structure_right, dim3((unsigned)((a + 255) / 256)), dim3(256), 0, st, Hdet, Sdet, n_det, n_str,
+                       soff, sdet, scoef, Yh, Ys);
+    hipLaunchKernelGGL(k_structure_left, dim3((unsigned)((b + 255) / 256)), dim3(256), 0, st, Yh, Ys, n_det, n_str,
+                       soff, sdet, scoef, Hs, Ss);
+}
+
+void launch_gep(hipStream_t st, const double* Hs, const double* Ss, int n, double e_nuc, double* work, double* coef,
+                double* energy, int* flags) {
+    hipLaunchKernelGGL(k_gep_lowest, dim3(1), dim3(1024), 0, st, Hs, Ss, n, e_nuc, work, coef, energy, flags);
+}
+
+void launch_det_coefficients(hipStream_t st, const long long* off, const int* ts, const double* tc, const double* coef,
+                             int n_det, double* a) {
+    hipLaunchKernelGGL(k_det_coefficients, dim3((n_det + 255) / 256), dim3(256), 0, st, off, ts, tc, coef, n_det, a);
+}
+
+void launch_gradient_tail(hipStream_t st, const double* T, const double* L, const double* dsum, const double* SC,
+                          const double* HC, const double* Cn, const double* nu2, const uint8_t* support, int n_bf,
+                          int n, int n_aux, double* dL, double* gcn, double* grad) {
+    const int np = n * (n + 1) / 2, ng = np * (np + 1) / 2;
+    const int tb = 64;
+    hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((n_aux + tb - 1) / tb, np), dim3(tb), 0, st, L, dsum, n_aux, np, dL);
+    hipLaunchKernelGGL(k_orbital_gradient, dim3(n_bf), dim3(256), 0, st, T, dL, SC, HC, dsum, n_bf, n, n_aux, ng, gcn);
+    hipLaunchKernelGGL(k_normalization_pullback, dim3(n), dim3(256), 0, st, gcn, Cn, SC, nu2, support, n_bf, grad);
+}
+
+}  // namespace xmvb_amd

@@ -1,0 +1,356 @@
+"""Python binding of the MI355X VBSCF energy+gradient engine (ctypes over include/xmvb_amd.h).
+
+The host-side mirror of the reference's evaluation interface:
+  * ``Topology`` - Rumer/Young structure generation + determinant expansion
+    (src/input/loading/generated_structures.cpp, src/vbscf/structures/expansion/expander.cpp);
+  * ``Context`` / ``Problem`` - device-resident problem and the two-phase evaluation contract of
+    VbScfObjective (src/vbscf/optimization/objective/function.cpp:89-275):
+    ``evaluate_forward`` (trial energy) then ``complete_gradient``; ``evaluate`` does both, like
+    OrbitalGradientEvaluator::evaluate (src/vbscf/derivatives/gradient/orbital/evaluator.cpp:198-418);
+  * ``parse_ctrl_method`` - the $CTRL method flag with the HIPVBSCF token (src/input/deck/model.cpp:360-381).
+
+Errors from the C ABI are raised as ``VbeError`` carrying the status code; there is no CPU
+fallback: without the built library every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libxmvb_amd.so")
+
+STATUS = {
+    0: "VBE_OK",
+    1: "VBE_ERR_ARGUMENT",
+    2: "VBE_ERR_NUMERIC",
+    3: "VBE_ERR_METRIC",
+    4: "VBE_ERR_DEVICE",
+    5: "VBE_ERR_UNSUPPORTED",
+    6: "VBE_ERR_MEMORY",
+    7: "VBE_ERR_TOPOLOGY",
+}
+METHODS = {0: "VBSCF", 1: "CUVBSCF", 2: "CASSCF", 3: "DFT", 4: "HIPVBSCF"}
+
+# every symbol include/xmvb_amd.h declares
+EXPORTED_SYMBOLS = (
+    "vbe_parse_ctrl_method", "vbe_check_hipvbscf_options", "vbe_count_structures", "vbe_topology_build",
+    "vbe_topology_copy", "vbe_topology_view_of", "vbe_topology_free", "vbe_synthetic_problem",
+    "vbe_context_create", "vbe_context_destroy", "vbe_last_error", "vbe_global_error", "vbe_problem_upload",
+    "vbe_problem_release", "vbe_problem_device_bytes", "vbe_eval", "vbe_eval_device", "vbe_eval_forward",
+    "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
+)
+
+
+class VbeError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Dims(C.Structure):
+    _fields_ = [("n_bf", C.c_int32), ("n_act", C.c_int32), ("n_alpha", C.c_int32), ("n_beta", C.c_int32),
+                ("n_aux", C.c_int32)]
+
+
+class TopologyView(C.Structure):
+    _fields_ = [
+        ("n_structures", C.c_int32),
+        ("n_alpha_strings", C.c_int32), ("alpha_masks", C.POINTER(C.c_uint64)),
+        ("n_beta_strings", C.c_int32), ("beta_masks", C.POINTER(C.c_uint64)),
+        ("n_determinants", C.c_int32), ("det_alpha", C.POINTER(C.c_int32)), ("det_beta", C.POINTER(C.c_int32)),
+        ("term_offsets", C.POINTER(C.c_int64)), ("term_structure", C.POINTER(C.c_int32)),
+        ("term_coefficient", C.POINTER(C.c_double)),
+    ]
+
+
+class Flags(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("eigensolver", C.c_int32), ("want_gradient", C.c_int32)]
+
+
+class PhaseTimes(C.Structure):
+    _fields_ = [(name, C.c_float) for name in (
+        "orbital_prep_ms", "transform_ms", "pair_forward_ms", "structure_ms", "eigensolve_ms", "pair_backward_ms",
+        "gradient_tail_ms", "total_ms")]
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+        lib = C.CDLL(LIB_PATH)
+        P, I32, I64, U64, D = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double
+        lib.vbe_global_error.restype = C.c_char_p
+        lib.vbe_parse_ctrl_method.argtypes = [C.c_char_p, C.POINTER(I32)]
+        lib.vbe_check_hipvbscf_options.argtypes = [I32, I32, I32, I32]
+        lib.vbe_count_structures.argtypes = [C.c_char_p, I32, I32, I32, C.POINTER(U64)]
+        lib.vbe_topology_build.argtypes = [C.c_char_p, I32, I32, I32, C.POINTER(P), P]
+        lib.vbe_topology_copy.argtypes = [P] * 9
+        lib.vbe_topology_view_of.argtypes = [P, C.POINTER(TopologyView)]
+        lib.vbe_topology_free.argtypes = [P]
+        lib.vbe_topology_free.restype = None
+        lib.vbe_synthetic_problem.argtypes = [U64, I32, I32, I32, D, P, P, P, P]
+        lib.vbe_context_create.argtypes = [I32, U64, C.POINTER(P)]
+        lib.vbe_context_destroy.argtypes = [P]
+        lib.vbe_context_destroy.restype = None
+        lib.vbe_last_error.argtypes = [P, C.c_char_p, C.c_size_t]
+        lib.vbe_problem_upload.argtypes = [P, C.POINTER(Dims), P, P, P, C.POINTER(TopologyView), D, C.POINTER(P)]
+        lib.vbe_problem_release.argtypes = [P]
+        lib.vbe_problem_release.restype = None
+        lib.vbe_problem_device_bytes.argtypes = [P, C.POINTER(U64)]
+        lib.vbe_eval.argtypes = [P, P, P, P, Flags, C.POINTER(D), P, P]
+        lib.vbe_eval_device.argtypes = [P, P, P, P, Flags, C.POINTER(D), P]
+        lib.vbe_eval_forward.argtypes = [P, P, P, P, C.POINTER(D)]
+        lib.vbe_complete_gradient.argtypes = [P, P, P]
+        lib.vbe_last_intermediates.argtypes = [P, P, P, P, P, P, P, P]
+        lib.vbe_last_phase_times.argtypes = [P, C.POINTER(PhaseTimes)]
+        lib.vbe_last_singular_pairs.argtypes = [P, C.POINTER(I64)]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check_global(rc: int):
+    if rc != 0:
+        raise VbeError(rc, library().vbe_global_error().decode())
+
+
+def parse_ctrl_method(ctrl_line: str) -> str:
+    out = C.c_int32()
+    _check_global(library().vbe_parse_ctrl_method(ctrl_line.encode(), C.byref(out)))
+    return METHODS[out.value]
+
+
+def check_hipvbscf_options(eigensolver: int = 0, ri: bool = True, n_states: int = 1, n_inactive_electrons: int = 0):
+    _check_global(library().vbe_check_hipvbscf_options(eigensolver, int(ri), n_states, n_inactive_electrons))
+
+
+def count_structures(structure_class: str, n_act: int, n_electrons: int, multiplicity: int = 1) -> int:
+    out = C.c_uint64()
+    _check_global(library().vbe_count_structures(structure_class.encode(), n_act, n_electrons, multiplicity,
+                                                 C.byref(out)))
+    return out.value
+
+
+@dataclass
+class Topology:
+    """Structure set expanded into signed determinants (StructureExpansion, expansion/types.hpp:390-423)."""
+    n_act: int
+    n_electrons: int
+    multiplicity: int
+    n_alpha: int
+    n_beta: int
+    raw_structures: np.ndarray  # (n_str, n_electrons) 1-based labels
+    alpha_masks: np.ndarray
+    beta_masks: np.ndarray
+    det_alpha: np.ndarray
+    det_beta: np.ndarray
+    term_offsets: np.ndarray
+    term_structure: np.ndarray
+    term_coefficient: np.ndarray
+
+    @property
+    def n_structures(self) -> int:
+        return self.raw_structures.shape[0]
+
+    @property
+    def n_determinants(self) -> int:
+        return self.det_alpha.shape[0]
+
+    @classmethod
+    def build(cls, structure_class: str, n_act: int, n_electrons: int, multiplicity: int = 1) -> "Topology":
+        lib = library()
+        handle = C.c_void_p()
+        sizes = np.zeros(5, np.int64)
+        _check_global(lib.vbe_topology_build(structure_class.encode(), n_act, n_electrons, multiplicity,
+                                             C.byref(handle), _ptr(sizes)))
+        try:
+            ns, na, nb, nd, nt = (int(x) for x in sizes)
+            raw = np.zeros((ns, n_electrons), np.int32)
+            am, bm = np.zeros(na, np.uint64), np.zeros(nb, np.uint64)
+            da, db = np.zeros(nd, np.int32), np.zeros(nd, np.int32)
+            off, ts, tc = np.zeros(nd + 1, np.int64), np.zeros(nt, np.int32), np.zeros(nt, np.float64)
+            _check_global(lib.vbe_topology_copy(handle, _ptr(am), _ptr(bm), _ptr(da), _ptr(db), _ptr(off), _ptr(ts),
+                                                _ptr(tc), _ptr(raw)))
+        finally:
+            lib.vbe_topology_free(handle)
+        n_beta = (n_electrons - multiplicity + 1) // 2
+        return cls(n_act, n_electrons, multiplicity, n_electrons - n_beta, n_beta, raw, am, bm, da, db, off, ts, tc)
+
+    def view(self) -> TopologyView:
+        v = TopologyView()
+        v.n_structures = self.n_structures
+        v.n_alpha_strings = len(self.alpha_masks)
+        v.alpha_masks = self.alpha_masks.ctypes.data_as(C.POINTER(C.c_uint64))
+        v.n_beta_strings = len(self.beta_masks)
+        v.beta_masks = self.beta_masks.ctypes.data_as(C.POINTER(C.c_uint64))
+        v.n_determinants = self.n_determinants
+        v.det_alpha = self.det_alpha.ctypes.data_as(C.POINTER(C.c_int32))
+        v.det_beta = self.det_beta.ctypes.data_as(C.POINTER(C.c_int32))
+        v.term_offsets = self.term_offsets.ctypes.data_as(C.POINTER(C.c_int64))
+        v.term_structure = self.term_structure.ctypes.data_as(C.POINTER(C.c_int32))
+        v.term_coefficient = self.term_coefficient.ctypes.data_as(C.POINTER(C.c_double))
+        return v
+
+    def incidence(self) -> np.ndarray:
+        """Dense (n_det, n_str) signed incidence matrix T."""
+        t = np.zeros((self.n_determinants, self.n_structures))
+        for d in range(self.n_determinants):
+            for k in range(self.term_offsets[d], self.term_offsets[d + 1]):
+                t[d, self.term_structure[k]] += self.term_coefficient[k]
+        return t
+
+
+@dataclass
+class SyntheticProblem:
+    n_bf: int
+    n_act: int
+    n_aux: int
+    S: np.ndarray  # (n_bf, n_bf) Fortran order
+    h: np.ndarray
+    B: np.ndarray  # (n_aux * n_bf(n_bf+1)/2,) column-major packed
+    C: np.ndarray  # (n_act, n_bf) orbital-major
+
+
+def synthetic_problem(seed: int, n_bf: int, n_act: int, n_aux: int, scan_r: float = -1.0) -> SyntheticProblem:
+    S = np.zeros((n_bf, n_bf), order="F")
+    h = np.zeros((n_bf, n_bf), order="F")
+    B = np.zeros(n_aux * (n_bf * (n_bf + 1) // 2))
+    Cm = np.zeros((n_act, n_bf))
+    _check_global(library().vbe_synthetic_problem(seed, n_bf, n_act, n_aux, scan_r, _ptr(S), _ptr(h), _ptr(B),
+                                                  _ptr(Cm)))
+    return SyntheticProblem(n_bf, n_act, n_aux, S, h, B, Cm)
+
+
+class Context:
+    """One HIP device + stream (cuda::Context precedent, src/cuda/memory/context.hpp:66-115)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self._lib = library()
+        self._h = C.c_void_p()
+        _check_global(self._lib.vbe_context_create(device, hbm_budget_bytes, C.byref(self._h)))
+
+    def error(self) -> str:
+        buf = C.create_string_buffer(4096)
+        self._lib.vbe_last_error(self._h, buf, 4096)
+        return buf.value.decode()
+
+    def check(self, rc: int):
+        if rc != 0:
+            raise VbeError(rc, self.error())
+
+    def close(self):
+        if self._h:
+            self._lib.vbe_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Problem:
+    """Device-resident molecule-static data of one VBSCF problem."""
+
+    def __init__(self, ctx: Context, n_bf: int, n_aux: int, topo: Topology, S: np.ndarray, h: np.ndarray,
+                 B: np.ndarray, e_nuc: float = 0.0):
+        self.ctx, self.topo = ctx, topo
+        self.n_bf, self.n_act, self.n_aux = n_bf, topo.n_act, n_aux
+        dims = Dims(n_bf, topo.n_act, topo.n_alpha, topo.n_beta, n_aux)
+        self._S = np.asfortranarray(S, dtype=np.float64)
+        self._hm = np.asfortranarray(h, dtype=np.float64)
+        self._B = np.ascontiguousarray(B, dtype=np.float64)
+        self._view = topo.view()
+        self._h = C.c_void_p()
+        ctx.check(ctx._lib.vbe_problem_upload(ctx._h, C.byref(dims), _ptr(self._S), _ptr(self._hm), _ptr(self._B),
+                                              C.byref(self._view), e_nuc, C.byref(self._h)))
+
+    @classmethod
+    def from_synthetic(cls, ctx: Context, sp: SyntheticProblem, topo: Topology, e_nuc: float = 0.0) -> "Problem":
+        return cls(ctx, sp.n_bf, sp.n_aux, topo, sp.S, sp.h, sp.B, e_nuc)
+
+    def _orbitals(self, Cm):
+        Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+        if Cm.shape != (self.n_act, self.n_bf):
+            raise ValueError(f"orbitals must have shape {(self.n_act, self.n_bf)}")
+        return Cm
+
+    def evaluate(self, Cm: np.ndarray, support: np.ndarray | None = None, gradient: bool = True):
+        """Energy (and gradient wrt raw orbital slots) of orbitals Cm (n_act, n_bf)."""
+        Cm = self._orbitals(Cm)
+        sup = None if support is None else np.ascontiguousarray(support, dtype=np.uint8)
+        e = C.c_double()
+        g = np.zeros_like(Cm) if gradient else None
+        self.ctx.check(self.ctx._lib.vbe_eval(self.ctx._h, self._h, _ptr(Cm), _ptr(sup), Flags(0, 0, int(gradient)),
+                                              C.byref(e), None, _ptr(g)))
+        return (e.value, g) if gradient else e.value
+
+    def evaluate_forward(self, Cm: np.ndarray, support: np.ndarray | None = None) -> float:
+        Cm = self._orbitals(Cm)
+        sup = None if support is None else np.ascontiguousarray(support, dtype=np.uint8)
+        e = C.c_double()
+        self.ctx.check(self.ctx._lib.vbe_eval_forward(self.ctx._h, self._h, _ptr(Cm), _ptr(sup), C.byref(e)))
+        return e.value
+
+    def complete_gradient(self) -> np.ndarray:
+        g = np.zeros((self.n_act, self.n_bf))
+        self.ctx.check(self.ctx._lib.vbe_complete_gradient(self.ctx._h, self._h, _ptr(g)))
+        return g
+
+    def evaluate_device(self, C_ptr: int, grad_ptr: int, gradient: bool = True, support_ptr: int | None = None):
+        """Evaluation with device-resident orbitals/gradient (raw device pointers)."""
+        e = C.c_double()
+        self.ctx.check(self.ctx._lib.vbe_eval_device(self.ctx._h, self._h, C.c_void_p(C_ptr),
+                                                     None if support_ptr is None else C.c_void_p(support_ptr),
+                                                     Flags(0, 0, int(gradient)), C.byref(e), C.c_void_p(grad_ptr)))
+        return e.value
+
+    def intermediates(self) -> dict:
+        ns, n = self.topo.n_structures, self.n_act
+        np_ = n * (n + 1) // 2
+        out = dict(H=np.zeros((ns, ns), order="F"), S=np.zeros((ns, ns), order="F"), coef=np.zeros(ns),
+                   S_act=np.zeros((n, n), order="F"), h_act=np.zeros((n, n), order="F"),
+                   g_packed=np.zeros(np_ * (np_ + 1) // 2))
+        self.ctx.check(self.ctx._lib.vbe_last_intermediates(self.ctx._h, self._h, *(_ptr(out[k]) for k in
+                                                                                     ("H", "S", "coef", "S_act",
+                                                                                      "h_act", "g_packed"))))
+        return out
+
+    def phase_times(self) -> dict:
+        t = PhaseTimes()
+        self.ctx._lib.vbe_last_phase_times(self._h, C.byref(t))
+        return {name: getattr(t, name) for name, _ in PhaseTimes._fields_}
+
+    def singular_pairs(self) -> int:
+        c = C.c_int64()
+        self.ctx._lib.vbe_last_singular_pairs(self._h, C.byref(c))
+        return c.value
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        self.ctx._lib.vbe_problem_device_bytes(self._h, C.byref(b))
+        return b.value
+
+    def close(self):
+        if self._h:
+            self.ctx._lib.vbe_problem_release(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
