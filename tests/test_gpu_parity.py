@@ -1,0 +1,112 @@
+"""HIP engine vs the FP64 CPU oracle, called through the C ABI (needs GPU 0).
+
+Tolerances: energies |dE| <= 1e-8 Eh (the FP64 path is expected near 1e-12),
+gradient max-norm 1e-7, structure H/S relative 1e-10.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import xmvb_amd as vb
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-8
+G_TOL = 1e-7
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = vb.Context(0)
+    yield c
+    c.close()
+
+
+def _case(ctx, kw, n_act, n_el, n_bf, n_aux, seed, mult=1):
+    sp = vb.synthetic_problem(seed, n_bf, n_act, n_aux)
+    topo = vb.Topology.build(kw, n_act, n_el, mult)
+    prob = vb.Problem.from_synthetic(ctx, sp, topo)
+    return sp, topo, prob
+
+
+@pytest.mark.parametrize("kw,n_act,n_el,n_bf,n_aux,seed", [
+    ("FULL", 2, 2, 10, 30, 1),     # H2-like: 1 covalent + 2 ionic structures, 1x1 blocks
+    ("COV", 4, 4, 16, 48, 2),
+    ("FULL", 4, 4, 20, 40, 3),
+    ("COV", 6, 6, 30, 90, 4),
+    ("FULL", 6, 6, 24, 60, 5),
+    ("COV", 8, 8, 40, 120, 6),
+])
+def test_energy_gradient_parity(ctx, kw, n_act, n_el, n_bf, n_aux, seed):
+    print(f"case {kw} n_act={n_act} n_bf={n_bf} n_aux={n_aux} seed={seed}", flush=True)
+    sp, topo, prob = _case(ctx, kw, n_act, n_el, n_bf, n_aux, seed)
+    e, g = prob.evaluate(sp.C)
+    ref = oracle.evaluate(n_bf, n_aux, topo, sp.S, sp.h, sp.B, sp.C)
+    assert abs(e - ref["energy"]) <= E_TOL
+    assert np.abs(g - ref["grad"]).max() <= G_TOL
+    mid = prob.intermediates()
+    for key in ("S_act", "h_act", "g_packed"):
+        assert np.abs(mid[key] - ref[key]).max() <= 1e-11 * max(1.0, np.abs(ref[key]).max()), key
+    for key in ("H", "S"):
+        assert np.abs(mid[key] - ref[key]).max() <= 1e-10 * np.abs(ref[key]).max(), key
+    prob.close()
+
+
+def test_medium_config_parity(ctx):
+    """n_bf=120, n_aux=360, n_act=10 (5a/5b), 42 covalent Rumer structures."""
+    print("case medium n_act=10 n_bf=120 n_aux=360 seed=0", flush=True)
+    sp, topo, prob = _case(ctx, "COV", 10, 10, 120, 360, 0)
+    assert topo.n_structures == 42 and topo.n_determinants == 252
+    e, g = prob.evaluate(sp.C)
+    ref = oracle.evaluate(120, 360, topo, sp.S, sp.h, sp.B, sp.C)
+    print(f"E gpu={e:.14f} oracle={ref['energy']:.14f} dE={abs(e - ref['energy']):.2e} "
+          f"dg={np.abs(g - ref['grad']).max():.2e}", flush=True)
+    assert abs(e - ref["energy"]) <= E_TOL
+    assert np.abs(g - ref["grad"]).max() <= G_TOL
+    prob.close()
+
+
+def test_two_phase_contract(ctx):
+    """evaluate_forward (trial energy) then complete_gradient equals one full evaluation."""
+    sp, topo, prob = _case(ctx, "COV", 6, 6, 24, 48, 9)
+    e_full, g_full = prob.evaluate(sp.C)
+    e_trial = prob.evaluate_forward(sp.C * 1.01)  # a discarded trial
+    e_fwd = prob.evaluate_forward(sp.C)
+    g = prob.complete_gradient()
+    assert e_fwd == pytest.approx(e_full, abs=1e-13)
+    assert np.abs(g - g_full).max() <= 1e-12
+    # orbital rescaling leaves the energy unchanged (normalisation inside the evaluation)
+    assert e_trial == pytest.approx(e_full, abs=1e-10)
+    prob.close()
+
+
+def test_energy_only_matches(ctx):
+    sp, topo, prob = _case(ctx, "COV", 6, 6, 24, 48, 10)
+    e = prob.evaluate(sp.C, gradient=False)
+    ref = oracle.evaluate(24, 48, topo, sp.S, sp.h, sp.B, sp.C, gradient=False)
+    assert abs(e - ref["energy"]) <= E_TOL
+    prob.close()
+
+
+def test_support_mask(ctx):
+    """Sparse orbital supports: gradient only on present slots, equal to the oracle's."""
+    sp, topo, prob = _case(ctx, "COV", 4, 4, 16, 32, 11)
+    rng = np.random.default_rng(5)
+    sup = (rng.random(sp.C.shape) < 0.7).astype(np.uint8)
+    sup[:, :4] = 1
+    e, g = prob.evaluate(sp.C, support=sup)
+    ref = oracle.evaluate(16, 32, topo, sp.S, sp.h, sp.B, sp.C, support=sup)
+    assert abs(e - ref["energy"]) <= E_TOL
+    assert np.abs(g - ref["grad"]).max() <= G_TOL
+    assert np.all(g[sup == 0] == 0.0)
+    prob.close()
+
+
+def test_normalisation_failure_is_reported(ctx):
+    sp, topo, prob = _case(ctx, "COV", 4, 4, 16, 32, 12)
+    bad = sp.C.copy()
+    bad[2] = 0.0
+    with pytest.raises(vb.VbeError) as err:
+        prob.evaluate(bad)
+    assert err.value.status == 2 and "orbital 2" in str(err.value)
+    prob.close()
