@@ -99,9 +99,11 @@ typedef struct vbe_flags {
 } vbe_flags;
 
 typedef struct vbe_phase_times {
-    /* device time per phase of the last evaluation, milliseconds (HIP events) */
-    float orbital_prep_ms, transform_ms, pair_forward_ms, structure_ms, eigensolve_ms, pair_backward_ms,
-        gradient_tail_ms, total_ms;
+    /* device time per phase of the last evaluation, milliseconds (HIP events on the context stream);
+     * ri_one_index_ms and pair_backward_kernel_ms isolate single kernels inside transform_ms and
+     * pair_backward_ms */
+    float orbital_prep_ms, transform_ms, ri_one_index_ms, pair_forward_ms, structure_ms, eigensolve_ms,
+        pair_backward_ms, pair_backward_kernel_ms, gradient_tail_ms, total_ms;
 } vbe_phase_times;
 
 /* ---- $CTRL method flag ------------------------------------------------------------------ */

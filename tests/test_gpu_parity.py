@@ -36,6 +36,9 @@ def _case(ctx, kw, n_act, n_el, n_bf, n_aux, seed, mult=1):
     ("COV", 6, 6, 30, 90, 4),
     ("FULL", 6, 6, 24, 60, 5),
     ("COV", 8, 8, 40, 120, 6),
+    ("COV", 14, 2, 28, 56, 7),     # n_act=14: packed adjoint accumulators, 1x1 blocks
+    ("COV", 12, 4, 30, 60, 8),     # 2x2 blocks over 12 orbitals
+    ("COV", 12, 12, 48, 96, 13),   # 6x6 blocks, 132 structures, 924 determinants
 ])
 def test_energy_gradient_parity(ctx, kw, n_act, n_el, n_bf, n_aux, seed):
     print(f"case {kw} n_act={n_act} n_bf={n_bf} n_aux={n_aux} seed={seed}", flush=True)

@@ -26,7 +26,10 @@ void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-enum Phase { PH_START, PH_PREP, PH_TRANSFORM, PH_PAIR_FWD, PH_STRUCT, PH_EIGEN, PH_PAIR_BWD, PH_TAIL, PH_COUNT };
+enum Phase {
+    PH_START, PH_PREP, PH_ONE_INDEX, PH_TRANSFORM, PH_PAIR_FWD, PH_STRUCT, PH_EIGEN, PH_BWD_BEGIN, PH_BWD_KERNEL,
+    PH_PAIR_BWD, PH_TAIL, PH_COUNT
+};
 
 }  // namespace
 
@@ -44,7 +47,7 @@ struct vbe_problem {
     int n_str = 0, n_det = 0, n_astr = 0, n_bstr = 0, np = 0, ng = 0;
     long long n_terms = 0;
     double e_nuc = 0.0;
-    int pair_blocks = 0;
+    int pair_blocks_fwd = 0, pair_blocks_bwd = 0;
     std::vector<void*> allocations;
     uint64_t bytes = 0;
     // static device data
@@ -156,11 +159,14 @@ void upload(T* dst, const T* src, size_t count, hipStream_t st) {
     check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
 }
 
-void check_flags(vbe_problem* p, bool after_gradient) {
+// Reads the device flags and energy with one synchronisation, then raises on any flagged failure.
+void finish(vbe_problem* p) {
     int flags[xmvb_amd::FLAG_COUNT];
+    double e[2];
     check(hipMemcpyAsync(flags, p->flags, sizeof(flags), hipMemcpyDeviceToHost, p->ctx->stream), "flags D2H");
+    check(hipMemcpyAsync(e, p->energy, sizeof(e), hipMemcpyDeviceToHost, p->ctx->stream), "energy D2H");
     check(hipStreamSynchronize(p->ctx->stream), "hipStreamSynchronize");
-    (void)after_gradient;
+    p->last_energy = e[1];
     p->last_singular = flags[xmvb_amd::FLAG_SINGULAR_PAIRS];
     if (flags[xmvb_amd::FLAG_NORMALIZATION])
         throw StatusError(VBE_ERR_NUMERIC, "orbital normalization failed for orbital " +
@@ -186,15 +192,18 @@ void enqueue_forward(vbe_problem* p) {
     launch_orbital_prep(st, p->C, p->last_support, p->S, p->h, d.n_bf, d.n_act, p->Cn, p->SC, p->HC, p->nu2, p->flags);
     launch_active_matrices(st, p->Cn, p->SC, p->HC, d.n_bf, d.n_act, p->sact, p->hact, p->flags);
     check(hipEventRecord(p->ev[PH_PREP], st), "hipEventRecord");
-    launch_transform(st, p->B, p->Cn, d.n_bf, d.n_act, d.n_aux, p->T, p->L, p->G);
+    launch_ri_one_index(st, p->B, p->Cn, d.n_bf, d.n_act, d.n_aux, p->T);
+    check(hipEventRecord(p->ev[PH_ONE_INDEX], st), "hipEventRecord");
+    launch_ri_pair_stage(st, p->T, p->Cn, d.n_bf, d.n_act, d.n_aux, p->L, p->G);
     check(hipEventRecord(p->ev[PH_TRANSFORM], st), "hipEventRecord");
-    launch_pairs(st, false, p->pair_blocks, p->G, p->sact, p->hact, d.n_act, p->ng, p->amask, p->bmask, p->det_a,
+    launch_pairs(st, false, p->pair_blocks_fwd, p->G, p->sact, p->hact, d.n_act, p->ng, p->amask, p->bmask, p->det_a,
                  p->det_b, p->n_det, p->Hdet, p->Sdet, nullptr, nullptr, nullptr, p->flags);
     check(hipGetLastError(), "pair forward launch");
     check(hipEventRecord(p->ev[PH_PAIR_FWD], st), "hipEventRecord");
     launch_structure(st, p->Hdet, p->Sdet, p->n_det, p->n_str, p->soff, p->sdet, p->scoef, p->Yh, p->Ys, p->Hs, p->Ss);
     check(hipEventRecord(p->ev[PH_STRUCT], st), "hipEventRecord");
-    launch_gep(st, p->Hs, p->Ss, p->n_str, p->e_nuc, p->gep_work, p->coef, p->energy, p->flags);
+    launch_gep(st, p->Hs, p->Ss, p->n_str, p->e_nuc, p->gep_work, p->coef, p->energy, p->flags, p->off, p->ts, p->tc,
+               p->n_det, p->dcoef);
     check(hipGetLastError(), "forward launch");
     check(hipEventRecord(p->ev[PH_EIGEN], st), "hipEventRecord");
 }
@@ -203,12 +212,13 @@ void enqueue_gradient(vbe_problem* p, double* grad_out) {
     hipStream_t st = p->ctx->stream;
     const auto& d = p->dims;
     using namespace xmvb_amd;
-    launch_det_coefficients(st, p->off, p->ts, p->tc, p->coef, p->n_det, p->dcoef);
-    launch_pairs(st, true, p->pair_blocks, p->G, p->sact, p->hact, d.n_act, p->ng, p->amask, p->bmask, p->det_a,
+    check(hipEventRecord(p->ev[PH_BWD_BEGIN], st), "hipEventRecord");
+    launch_pairs(st, true, p->pair_blocks_bwd, p->G, p->sact, p->hact, d.n_act, p->ng, p->amask, p->bmask, p->det_a,
                  p->det_b, p->n_det, nullptr, nullptr, p->dcoef, p->energy, p->partials, p->flags);
     check(hipGetLastError(), "pair backward launch");
+    check(hipEventRecord(p->ev[PH_BWD_KERNEL], st), "hipEventRecord");
     const int len = p->ng + 2 * d.n_act * d.n_act;
-    launch_reduce_partials(st, p->partials, p->pair_blocks, len, p->dsum);
+    launch_reduce_partials(st, p->partials, p->pair_blocks_bwd, len, p->dsum);
     check(hipEventRecord(p->ev[PH_PAIR_BWD], st), "hipEventRecord");
     launch_gradient_tail(st, p->T, p->L, p->dsum, p->SC, p->HC, p->Cn, p->nu2, p->last_support, d.n_bf, d.n_act,
                          d.n_aux, p->dL, p->gcn, grad_out);
@@ -226,19 +236,14 @@ void record_times(vbe_problem* p, bool with_gradient) {
     auto& t = p->times;
     t.orbital_prep_ms = elapsed(p->ev[PH_START], p->ev[PH_PREP]);
     t.transform_ms = elapsed(p->ev[PH_PREP], p->ev[PH_TRANSFORM]);
+    t.ri_one_index_ms = elapsed(p->ev[PH_PREP], p->ev[PH_ONE_INDEX]);
     t.pair_forward_ms = elapsed(p->ev[PH_TRANSFORM], p->ev[PH_PAIR_FWD]);
     t.structure_ms = elapsed(p->ev[PH_PAIR_FWD], p->ev[PH_STRUCT]);
     t.eigensolve_ms = elapsed(p->ev[PH_STRUCT], p->ev[PH_EIGEN]);
     t.pair_backward_ms = with_gradient ? elapsed(p->ev[PH_EIGEN], p->ev[PH_PAIR_BWD]) : 0.f;
+    t.pair_backward_kernel_ms = with_gradient ? elapsed(p->ev[PH_BWD_BEGIN], p->ev[PH_BWD_KERNEL]) : 0.f;
     t.gradient_tail_ms = with_gradient ? elapsed(p->ev[PH_PAIR_BWD], p->ev[PH_TAIL]) : 0.f;
     t.total_ms = elapsed(p->ev[PH_START], with_gradient ? p->ev[PH_TAIL] : p->ev[PH_EIGEN]);
-}
-
-double read_energy(vbe_problem* p) {
-    double e[2];
-    check(hipMemcpyAsync(e, p->energy, sizeof(e), hipMemcpyDeviceToHost, p->ctx->stream), "energy D2H");
-    check(hipStreamSynchronize(p->ctx->stream), "hipStreamSynchronize");
-    return e[1];
 }
 
 int eval_core(vbe_context* ctx, vbe_problem* p, const double* d_C, const uint8_t* d_support, bool want_grad,
@@ -251,14 +256,13 @@ int eval_core(vbe_context* ctx, vbe_problem* p, const double* d_C, const uint8_t
               "orbitals D2D");
     p->last_support = d_support;
     p->forward_valid = false;
+    // everything is enqueued back to back; a failure flagged by the forward (singular pairs,
+    // indefinite metric) makes the backward's output meaningless but never out of bounds, and
+    // is reported before any result is returned
     enqueue_forward(p);
-    check_flags(p, false);
-    if (want_grad) {
-        enqueue_gradient(p, d_grad ? d_grad : p->grad);
-        check_flags(p, true);
-    }
+    if (want_grad) enqueue_gradient(p, d_grad ? d_grad : p->grad);
+    finish(p);
     record_times(p, want_grad);
-    p->last_energy = read_energy(p);
     p->forward_valid = true;
     if (E_total) *E_total = p->last_energy;
     return VBE_OK;
@@ -374,7 +378,10 @@ int vbe_problem_upload(vbe_context* ctx, const vbe_dims* dims, const double* S_a
         // workspace
         const size_t nd = p->n_det, ns = p->n_str;
         const long long n_pairs = (long long)nd * (nd + 1) / 2;
-        p->pair_blocks = (int)std::min<long long>(std::max<long long>(1, (n_pairs + 3) / 4), 1024);
+        // forward: about two pairs per wavefront; backward: fewer blocks so that the per-block
+        // adjoint partials stay small
+        p->pair_blocks_fwd = (int)std::min<long long>(std::max<long long>(1, (n_pairs + 7) / 8), 8192);
+        p->pair_blocks_bwd = (int)std::min<long long>(std::max<long long>(1, (n_pairs + 3) / 4), 512);
         p->C = p->alloc<double>((size_t)n * nbf);
         p->grad = p->alloc<double>((size_t)n * nbf);
         p->Cn = p->alloc<double>((size_t)n * nbf);
@@ -397,13 +404,13 @@ int vbe_problem_upload(vbe_context* ctx, const vbe_dims* dims, const double* S_a
         p->energy = p->alloc<double>(2);
         p->dcoef = p->alloc<double>(nd);
         const size_t len = (size_t)p->ng + 2 * (size_t)n * n;
-        p->partials = p->alloc<double>((size_t)p->pair_blocks * len);
+        p->partials = p->alloc<double>((size_t)p->pair_blocks_bwd * len);
         p->dsum = p->alloc<double>(len);
         p->dL = p->alloc<double>((size_t)naux * p->np);
         p->gcn = p->alloc<double>((size_t)n * nbf);
         p->flags = p->alloc<int>(xmvb_amd::FLAG_COUNT);
         for (auto& e : p->ev) check(hipEventCreate(&e), "hipEventCreate");
-        check(xmvb_amd::configure_kernels(n, p->ng), "hipFuncSetAttribute");
+        check(xmvb_amd::configure_kernels(n, p->ng, p->n_str), "hipFuncSetAttribute");
         check(hipStreamSynchronize(st), "hipStreamSynchronize");
         *out = p.release();
         return VBE_OK;
@@ -477,7 +484,7 @@ int vbe_complete_gradient(vbe_context* ctx, vbe_problem* prob, double* grad_raw)
         if (!prob->forward_valid) throw StatusError(VBE_ERR_ARGUMENT, "no retained forward evaluation");
         check(hipSetDevice(ctx->device), "hipSetDevice");
         enqueue_gradient(prob, prob->grad);
-        check_flags(prob, true);
+        finish(prob);
         const size_t nc = (size_t)prob->dims.n_act * prob->dims.n_bf;
         check(hipMemcpyAsync(grad_raw, prob->grad, nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
               "gradient D2H");

@@ -104,26 +104,31 @@ __global__ void k_active_matrices(const double* __restrict__ Cn, const double* _
         hact[t] = 0.5 * (h_l[t] + h_l[j + n * i]);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // Cholesky of the symmetrised S_act in place (the reference frame factorisation,
-        // vbscf/integrals/active/space.cpp:99-113, fails the evaluation when it breaks down)
-        double a[MAX_ACT * MAX_ACT];
-        for (int t = 0; t < n * n; ++t) a[t] = 0.5 * (s_l[t] + s_l[(t / n) + n * (t % n)]);
+    // Cholesky of the symmetrised S_act (the reference's frame factorisation,
+    // vbscf/integrals/active/space.cpp:99-113, fails the evaluation when it breaks down);
+    // left-looking, wave 0, lane i owns row i.
+    if (threadIdx.x < 64) {
+        const int i = threadIdx.x;
+        double* a = h_l;  // h_l is no longer needed: reuse it as the factor
+        for (int t = i; t < n * n; t += 64) a[t] = 0.5 * (s_l[t] + s_l[(t / n) + n * (t % n)]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        bool ok = true;
         for (int j = 0; j < n; ++j) {
-            double d = a[j + n * j];
-            for (int k = 0; k < j; ++k) d -= a[j + n * k] * a[j + n * k];
-            if (!(d > 0.0)) {
-                flags[FLAG_ACTIVE_OVERLAP] = 1;
-                return;
+            double v = (i < n) ? a[i + n * j] : 0.0;
+            for (int k = 0; k < j; ++k) v -= (i < n ? a[i + n * k] : 0.0) * a[j + n * k];
+            const double djj = __shfl(v, j, 64);
+            if (!(djj > 0.0)) {
+                ok = false;
+                break;
             }
-            d = sqrt(d);
-            a[j + n * j] = d;
-            for (int i = j + 1; i < n; ++i) {
-                double v = a[i + n * j];
-                for (int k = 0; k < j; ++k) v -= a[i + n * k] * a[j + n * k];
-                a[i + n * j] = v / d;
-            }
+            const double d = sqrt(djj);
+            if (i == j) a[i + n * j] = d;
+            else if (i > j && i < n) a[i + n * j] = v / d;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
+        if (!ok && i == 0) flags[FLAG_ACTIVE_OVERLAP] = 1;
     }
 }
 
@@ -148,25 +153,27 @@ __global__ void k_ri_one_index(const double* __restrict__ B, const double* __res
         if (q < n) T[(size_t)A + (size_t)n_aux * (q + (size_t)n * mu)] = acc[q];
 }
 
-// L[A, (p,q)] = sum_mu T[A,q,mu] Cn[mu,p], q <= p. grid (ceil(n_aux/blockDim), n).
+// L[A, (p,q)] = sum_mu T[A,q,mu] Cn[mu,p], q <= p. One thread per (A, packed (p,q)):
+// grid (ceil(n_aux/blockDim), n(n+1)/2).
 __global__ void k_ri_pair_factors(const double* __restrict__ T, const double* __restrict__ Cn, int n_bf, int n,
                                   int n_aux, double* __restrict__ L) {
     const int A = blockIdx.x * blockDim.x + threadIdx.x;
-    const int p = blockIdx.y;
+    const int P = blockIdx.y;
     if (A >= n_aux) return;
-    double acc[MAX_ACT];
-#pragma unroll
-    for (int q = 0; q < MAX_ACT; ++q) acc[q] = 0.0;
-    for (int mu = 0; mu < n_bf; ++mu) {
-        const double cp = Cn[(size_t)p * n_bf + mu];
-        const double* t = T + (size_t)A + (size_t)n_aux * ((size_t)n * mu);
-#pragma unroll
-        for (int q = 0; q < MAX_ACT; ++q)
-            if (q <= p) acc[q] += t[(size_t)n_aux * q] * cp;
+    int p = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
+    while (p * (p + 1) / 2 > P) --p;
+    while ((p + 1) * (p + 2) / 2 <= P) ++p;
+    const int q = P - p * (p + 1) / 2;
+    const double* t = T + (size_t)A + (size_t)n_aux * q;
+    const double* c = Cn + (size_t)p * n_bf;
+    double acc0 = 0.0, acc1 = 0.0;
+    int mu = 0;
+    for (; mu + 1 < n_bf; mu += 2) {
+        acc0 += t[(size_t)n_aux * n * mu] * c[mu];
+        acc1 += t[(size_t)n_aux * n * (mu + 1)] * c[mu + 1];
     }
-#pragma unroll
-    for (int q = 0; q < MAX_ACT; ++q)
-        if (q <= p) L[(size_t)A + (size_t)n_aux * pair_index(p, q)] = acc[q];
+    if (mu < n_bf) acc0 += t[(size_t)n_aux * n * mu] * c[mu];
+    L[(size_t)A + (size_t)n_aux * P] = acc0 + acc1;
 }
 
 // packed G[(P,Q)] = sum_A L[A,P] L[A,Q]; one wavefront per packed entry.
@@ -279,24 +286,23 @@ __device__ BlockInverse invert_block(double x, int m, int lane) {
     return out;
 }
 
-struct PairShared {
-    double* G;      // packed kernel, ng
-    double* sact;   // n x n
-    double* hact;   // n x n
-    double* dg;     // backward accumulators (ng), may be null
-    double* ds;     // n x n
-    double* dh;     // n x n
-};
-
 struct WaveSlab {
-    double D[2][64];
-    double F[2][64];
+    double D[2][64];  // transition densities D_s[r + 8l] = Y_s(l, r)
+    double F[2][64];  // dPhi/dD_s
     double Z[64];
+    int PP[2][64];    // packed pair index (R_r, L_l) of each block entry
     int occL[2][8];
     int occR[2][8];
 };
 
-template <bool Backward>
+// One wavefront per unordered determinant pair (d <= e). The active two-electron kernel is held in
+// LDS as the full symmetric np x np matrix G2[P + np*Q] so every (pq|rs) lookup is one index add.
+// Forward: H_de = S_de * Phi_de, S_de = det X_alpha det X_beta (Lowdin rules in the regular
+// inverse form, DeterminantPairEvaluator::combine_spin_pair_evaluations, pairs/evaluator.cpp:354-384).
+// Backward: adjoints of S_act, h_act and the packed kernel for F = w (H_de - E S_de),
+// accumulated per block in LDS (full np x np when FullDg, packed otherwise) and written as
+// packed partials.
+template <bool Backward, bool FullDg>
 __global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob, const double* __restrict__ sact_g,
                                                const double* __restrict__ hact_g, int n, int ng,
                                                const uint64_t* __restrict__ amask, const uint64_t* __restrict__ bmask,
@@ -305,36 +311,40 @@ __global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob
                                                const double* __restrict__ dcoef, const double* __restrict__ energy,
                                                double* __restrict__ partials, int* __restrict__ flags) {
     extern __shared__ double lds[];
-    PairShared sh;
-    sh.G = lds;
-    sh.sact = sh.G + ng;
-    sh.hact = sh.sact + n * n;
-    double* tail = sh.hact + n * n;
+    const int np = n * (n + 1) / 2;
+    double* G2 = lds;
+    double* sact = G2 + np * np;
+    double* hact = sact + n * n;
+    double* tail = hact + n * n;
+    double *dg = nullptr, *ds = nullptr, *dh = nullptr;
     if (Backward) {
-        sh.dg = tail;
-        sh.ds = sh.dg + ng;
-        sh.dh = sh.ds + n * n;
-        tail = sh.dh + n * n;
+        dg = tail;
+        ds = dg + (FullDg ? np * np : ng);
+        dh = ds + n * n;
+        tail = dh + n * n;
     }
     WaveSlab* slabs = reinterpret_cast<WaveSlab*>(tail);
-    for (int t = threadIdx.x; t < ng; t += blockDim.x) sh.G[t] = G_glob[t];
+    for (int t = threadIdx.x; t < np * np; t += blockDim.x) {
+        const int P = t % np, Q = t / np;
+        G2[t] = G_glob[pair_index(P, Q)];
+    }
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-        sh.sact[t] = sact_g[t];
-        sh.hact[t] = hact_g[t];
+        sact[t] = sact_g[t];
+        hact[t] = hact_g[t];
     }
     if (Backward) {
-        for (int t = threadIdx.x; t < ng; t += blockDim.x) sh.dg[t] = 0.0;
+        const int ndg = FullDg ? np * np : ng;
+        for (int t = threadIdx.x; t < ndg; t += blockDim.x) dg[t] = 0.0;
         for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-            sh.ds[t] = 0.0;
-            sh.dh[t] = 0.0;
+            ds[t] = 0.0;
+            dh[t] = 0.0;
         }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int wave_in_block = threadIdx.x >> 6;
     const int waves_per_block = blockDim.x >> 6;
-    WaveSlab& ws = slabs[wave_in_block];
-    const long long first = (long long)blockIdx.x * waves_per_block + wave_in_block;
+    WaveSlab& ws = slabs[threadIdx.x >> 6];
+    const long long first = (long long)blockIdx.x * waves_per_block + (threadIdx.x >> 6);
     const long long stride = (long long)gridDim.x * waves_per_block;
     const double E = Backward ? energy[0] : 0.0;
     const int r = lane & 7, l = lane >> 3;
@@ -354,6 +364,7 @@ __global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob
         const uint64_t mL[2] = {amask[det_a[d]], bmask[det_b[d]]};
         const uint64_t mR[2] = {amask[det_a[ee]], bmask[det_b[ee]]};
         int m[2];
+#pragma unroll
         for (int s = 0; s < 2; ++s) {
             m[s] = __builtin_popcountll(mL[s]);
             if (lane < m[s]) {
@@ -365,9 +376,17 @@ __global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob
         // overlap blocks and their inverses
         double det[2];
         bool singular = false;
+#pragma unroll
         for (int s = 0; s < 2; ++s) {
+            const bool valid = r < m[s] && l < m[s];
             double x = 0.0;
-            if (r < m[s] && l < m[s]) x = sh.sact[ws.occR[s][r] + n * ws.occL[s][l]];
+            int ppv = 0;
+            if (valid) {
+                const int Rr = ws.occR[s][r], Ll = ws.occL[s][l];
+                x = sact[Rr + n * Ll];
+                ppv = pair_index(Rr, Ll);
+            }
+            ws.PP[s][lane] = ppv;
             if (m[s] == 0) {
                 det[s] = 1.0;
                 ws.D[s][lane] = 0.0;
@@ -383,32 +402,50 @@ __global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob
             continue;
         }
         wave_sync();
-        // F_s[r,l] = dPhi/dD_s[r,l]: one-electron + same-spin (Coulomb - exchange) + opposite spin
+        // F_s[r,l] = dPhi/dD_s[r,l] = h + sum (J - K) D_s + sum G D_other; Phi = 1/2 sum_s D_s.(h + F_s)
         double phi_part = 0.0;
+        int cross_l[2][8], cross_r[2][8];  // pp(R_r, L_l2) and pp(R_r2, L_l) for this lane, per spin
+#pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int ms = m[s], mo = m[1 - s];
             double f = 0.0;
-            if (r < ms && l < ms) {
-                const int Rr = ws.occR[s][r], Ll = ws.occL[s][l];
-                const double hv = sh.hact[Rr + n * Ll];
-                const int P1 = pair_index(Rr, Ll);
+            const bool valid = r < ms && l < ms;
+            const int Rr = valid ? ws.occR[s][r] : 0, Ll = valid ? ws.occL[s][l] : 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                cross_l[s][k] = (valid && k < ms) ? pair_index(Rr, ws.occL[s][k]) : 0;
+                cross_r[s][k] = (valid && k < ms) ? pair_index(ws.occR[s][k], Ll) : 0;
+            }
+            if (valid) {
+                const double hv = hact[Rr + n * Ll];
+                const int P1 = ws.PP[s][lane];
+                const double* g1 = G2 + (size_t)np * P1;
                 f = hv;
-                for (int l2 = 0; l2 < ms; ++l2) {
-                    const int L2 = ws.occL[s][l2];
-                    const int Px = pair_index(Rr, L2);
-                    for (int r2 = 0; r2 < ms; ++r2) {
-                        const int R2 = ws.occR[s][r2];
-                        const double d2 = ws.D[s][r2 + 8 * l2];
-                        const double J = sh.G[pair_index(P1, pair_index(R2, L2))];
-                        const double K = sh.G[pair_index(Px, pair_index(R2, Ll))];
-                        f += (J - K) * d2;
+#pragma unroll
+                for (int l2 = 0; l2 < 8; ++l2) {
+                    if (l2 < ms) {
+                        const double* gx = G2 + (size_t)np * cross_l[s][l2];
+#pragma unroll
+                        for (int r2 = 0; r2 < 8; ++r2) {
+                            if (r2 < ms) {
+                                const int idx = r2 + 8 * l2;
+                                const double J = g1[ws.PP[s][idx]];
+                                const double K = gx[cross_r[s][r2]];
+                                f += (J - K) * ws.D[s][idx];
+                            }
+                        }
                     }
                 }
-                for (int l2 = 0; l2 < mo; ++l2) {
-                    const int L2 = ws.occL[1 - s][l2];
-                    for (int r2 = 0; r2 < mo; ++r2) {
-                        const int R2 = ws.occR[1 - s][r2];
-                        f += sh.G[pair_index(P1, pair_index(R2, L2))] * ws.D[1 - s][r2 + 8 * l2];
+#pragma unroll
+                for (int l2 = 0; l2 < 8; ++l2) {
+                    if (l2 < mo) {
+#pragma unroll
+                        for (int r2 = 0; r2 < 8; ++r2) {
+                            if (r2 < mo) {
+                                const int idx = r2 + 8 * l2;
+                                f += g1[ws.PP[1 - s][idx]] * ws.D[1 - s][idx];
+                            }
+                        }
                     }
                 }
                 phi_part += 0.5 * ws.D[s][lane] * (hv + f);
@@ -430,42 +467,66 @@ __global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob
         // ---- backward: F_pair = w (H - E S) ----
         const double wH = weight, wS = -E * weight;
         const double c = Dtot;
+        const double cw = c * wH;
         wave_sync();
+#pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int ms = m[s];
             if (ms == 0) continue;
+            const bool valid = r < ms && l < ms;
             // Z[a,l] = sum_b F[b,a] D[b,l]
             double z = 0.0;
-            if (r < ms && l < ms)
-                for (int b = 0; b < ms; ++b) z += ws.F[s][b + 8 * r] * ws.D[s][b + 8 * l];
+            if (valid) {
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    if (b < ms) z += ws.F[s][b + 8 * r] * ws.D[s][b + 8 * l];
+            }
             ws.Z[lane] = z;
             wave_sync();
-            if (r < ms && l < ms) {
-                double dfd = 0.0;  // (D F^T D)[r,l] = sum_a D[r,a] Z[a,l]
-                for (int a = 0; a < ms; ++a) dfd += ws.D[s][r + 8 * a] * ws.Z[a + 8 * l];
+            if (valid) {
+                double dfd = 0.0;  // (D F^T D)[r,l]
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+                    if (a < ms) dfd += ws.D[s][r + 8 * a] * ws.Z[a + 8 * l];
                 const double Drl = ws.D[s][lane];
                 const int Rr = ws.occR[s][r], Ll = ws.occL[s][l];
-                atomicAdd(&sh.ds[Rr + n * Ll], c * ((wH * phi + wS) * Drl - wH * dfd));
-                atomicAdd(&sh.dh[Rr + n * Ll], c * wH * Drl);
-                // same-spin two-electron adjoint, r1 < r2 and l1 < l2 from this lane's (r,l)
-                const double cw = c * wH;
-                for (int l2 = l + 1; l2 < ms; ++l2) {
-                    const int L2 = ws.occL[s][l2];
-                    for (int r2 = r + 1; r2 < ms; ++r2) {
-                        const int R2 = ws.occR[s][r2];
-                        const double mnr = Drl * ws.D[s][r2 + 8 * l2] - ws.D[s][r + 8 * l2] * ws.D[s][r2 + 8 * l];
-                        atomicAdd(&sh.dg[pair_index(pair_index(Rr, Ll), pair_index(R2, L2))], cw * mnr);
-                        atomicAdd(&sh.dg[pair_index(pair_index(Rr, L2), pair_index(R2, Ll))], -cw * mnr);
+                atomicAdd(&ds[Rr + n * Ll], c * ((wH * phi + wS) * Drl - wH * dfd));
+                atomicAdd(&dh[Rr + n * Ll], cw * Drl);
+                const int P1 = ws.PP[s][lane];
+                // same-spin two-electron adjoint over r2 > r, l2 > l
+#pragma unroll
+                for (int l2 = 1; l2 < 8; ++l2) {
+                    if (l2 > l && l2 < ms) {
+#pragma unroll
+                        for (int r2 = 1; r2 < 8; ++r2) {
+                            if (r2 > r && r2 < ms) {
+                                const int idx = r2 + 8 * l2;
+                                const double mnr = Drl * ws.D[s][idx] - ws.D[s][r + 8 * l2] * ws.D[s][r2 + 8 * l];
+                                if (FullDg) {
+                                    atomicAdd(&dg[P1 + np * ws.PP[s][idx]], cw * mnr);
+                                    atomicAdd(&dg[cross_l[s][l2] + np * cross_r[s][r2]], -cw * mnr);
+                                } else {
+                                    atomicAdd(&dg[pair_index(P1, ws.PP[s][idx])], cw * mnr);
+                                    atomicAdd(&dg[pair_index(cross_l[s][l2], cross_r[s][r2])], -cw * mnr);
+                                }
+                            }
+                        }
                     }
                 }
                 if (s == 0) {  // opposite spin, once per (alpha entry, beta entry)
                     const int mo = m[1];
-                    const int Pa = pair_index(Rr, Ll);
-                    for (int l2 = 0; l2 < mo; ++l2) {
-                        const int L2 = ws.occL[1][l2];
-                        for (int r2 = 0; r2 < mo; ++r2) {
-                            const int R2 = ws.occR[1][r2];
-                            atomicAdd(&sh.dg[pair_index(pair_index(R2, L2), Pa)], cw * Drl * ws.D[1][r2 + 8 * l2]);
+#pragma unroll
+                    for (int l2 = 0; l2 < 8; ++l2) {
+                        if (l2 < mo) {
+#pragma unroll
+                            for (int r2 = 0; r2 < 8; ++r2) {
+                                if (r2 < mo) {
+                                    const int idx = r2 + 8 * l2;
+                                    const double v = cw * Drl * ws.D[1][idx];
+                                    if (FullDg) atomicAdd(&dg[P1 + np * ws.PP[1][idx]], v);
+                                    else atomicAdd(&dg[pair_index(P1, ws.PP[1][idx])], v);
+                                }
+                            }
                         }
                     }
                 }
@@ -477,21 +538,33 @@ __global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob
         __syncthreads();
         const int stride_part = ng + 2 * n * n;
         double* out = partials + (size_t)blockIdx.x * stride_part;
-        for (int t = threadIdx.x; t < ng; t += blockDim.x) out[t] = sh.dg[t];
+        if (FullDg) {
+            for (int t = threadIdx.x; t < ng; t += blockDim.x) {
+                int P = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+                while (P * (P + 1) / 2 > t) --P;
+                while ((P + 1) * (P + 2) / 2 <= t) ++P;
+                const int Q = t - P * (P + 1) / 2;
+                out[t] = (P == Q) ? dg[P + np * P] : dg[P + np * Q] + dg[Q + np * P];
+            }
+        } else {
+            for (int t = threadIdx.x; t < ng; t += blockDim.x) out[t] = dg[t];
+        }
         for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-            out[ng + t] = sh.ds[t];
-            out[ng + n * n + t] = sh.dh[t];
+            out[ng + t] = ds[t];
+            out[ng + n * n + t] = dh[t];
         }
     }
 }
 
-// sum block partials in a fixed order: out[i] = sum_b partials[b*len + i]
+// sum block partials in a fixed order: out[i] = sum_b partials[b*len + i], one wavefront per i
 __global__ void k_reduce_partials(const double* __restrict__ partials, int n_blocks, int len, double* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     if (i >= len) return;
     double acc = 0.0;
-    for (int b = 0; b < n_blocks; ++b) acc += partials[(size_t)b * len + i];
-    out[i] = acc;
+    for (int b = lane; b < n_blocks; b += 64) acc += partials[(size_t)b * len + i];
+    acc = wave_sum(acc);
+    if (lane == 0) out[i] = acc;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -542,15 +615,21 @@ __global__ void k_det_coefficients(const long long* __restrict__ off, const int*
 
 // ---------------------------------------------------------------------------------------------
 // generalised eigenproblem H c = E S c, lowest root, one workgroup.
-// Cholesky S = L L^T, M = L^{-1} H L^{-T}, Householder tridiagonalisation, bisection (Sturm
-// counts, multisection over the block) for the lowest eigenvalue, inverse iteration on the
-// tridiagonal, back-transformation, c = L^{-T} z so that c^T S c = 1.
-// work: gep_work_doubles(n) = 3*n*n + 9*n doubles.
+// Cholesky S = L L^T, M = L^{-1} H L^{-T}, Householder tridiagonalisation, multisection on
+// Sturm counts for the lowest eigenvalue, inverse iteration on the tridiagonal,
+// back-transformation, c = L^{-T} z so that c^T S c = 1.
+// Storage: gep_work_doubles(n) = 3*n*n + 9*n doubles, in LDS when it fits (InLds) else in the
+// global workspace. A single wavefront is used for small n so that the block barriers are free.
 // ---------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ Hs, const double* __restrict__ Ss,
-                                                     int n, double e_nuc, double* __restrict__ work,
-                                                     double* __restrict__ coef, double* __restrict__ energy,
-                                                     int* __restrict__ flags) {
+template <bool InLds>
+__global__ void __launch_bounds__(256) k_gep_lowest(const double* __restrict__ Hs, const double* __restrict__ Ss,
+                                                    int n, double e_nuc, double* __restrict__ gwork,
+                                                    double* __restrict__ coef, double* __restrict__ energy,
+                                                    int* __restrict__ flags, const long long* __restrict__ off,
+                                                    const int* __restrict__ ts, const double* __restrict__ tc,
+                                                    int n_det, double* __restrict__ dcoef) {
+    extern __shared__ double lds[];
+    double* work = InLds ? lds : gwork;
     double* Lm = work;              // n x n, lower Cholesky factor
     double* M = work + n * n;       // n x n
     double* V = work + 2 * n * n;   // Householder vectors (column k holds v_k)
@@ -565,25 +644,27 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
     int* ipiv = reinterpret_cast<int*>(tdg + n);  // n ints in the last n doubles
     __shared__ double red[16];
     __shared__ double sh_scalar[4];
+    __shared__ double lo_hi[2];
     __shared__ int sh_flag;
+    __shared__ int first_true;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int t = tid; t < n * n; t += nt) {
         const int i = t % n, j = t / n;
         Lm[t] = 0.5 * (Ss[t] + Ss[j + n * i]);
         M[t] = 0.5 * (Hs[t] + Hs[j + n * i]);
+        V[t] = 0.0;
     }
     if (tid == 0) sh_flag = 0;
     __syncthreads();
     // Cholesky (right-looking, lower)
     for (int k = 0; k < n; ++k) {
-        if (tid == 0) {
-            const double a = Lm[k + n * k];
-            if (!(a > 0.0)) sh_flag = 1;
-            sh_scalar[0] = a > 0.0 ? sqrt(a) : 1.0;
+        const double a = Lm[k + n * k];
+        if (!(a > 0.0)) {
+            if (tid == 0) flags[FLAG_METRIC] = 1;
+            return;  // uniform: every thread read the same diagonal after the last barrier
         }
+        const double dk = sqrt(a);
         __syncthreads();
-        if (sh_flag) break;
-        const double dk = sh_scalar[0];
         for (int i = k + tid; i < n; i += nt) Lm[i + n * k] = (i == k) ? dk : Lm[i + n * k] / dk;
         __syncthreads();
         const int rem = n - k - 1;
@@ -593,11 +674,7 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
         }
         __syncthreads();
     }
-    if (sh_flag) {
-        if (tid == 0) flags[FLAG_METRIC] = 1;
-        return;
-    }
-    // X = L^{-1} H (in M, forward substitution, right-looking)
+    // X = L^{-1} H  (in M; column-parallel forward substitution)
     for (int k = 0; k < n; ++k) {
         const double lkk = Lm[k + n * k];
         for (int j = tid; j < n; j += nt) M[k + n * j] /= lkk;
@@ -609,11 +686,8 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
         }
         __syncthreads();
     }
-    // transpose X into V, then M = L^{-1} X^T
-    for (int t = tid; t < n * n; t += nt) {
-        const int i = t % n, j = t / n;
-        V[t] = M[j + n * i];
-    }
+    // M <- L^{-1} X^T (X^T staged through V)
+    for (int t = tid; t < n * n; t += nt) V[t] = M[(t / n) + n * (t % n)];
     __syncthreads();
     for (int t = tid; t < n * n; t += nt) M[t] = V[t];
     __syncthreads();
@@ -630,15 +704,15 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
     }
     for (int t = tid; t < n * n; t += nt) {
         const int i = t % n, j = t / n;
+        V[t] = 0.0;
         if (i < j) {
             const double s = 0.5 * (M[i + n * j] + M[j + n * i]);
             M[i + n * j] = s;
             M[j + n * i] = s;
         }
     }
-    for (int t = tid; t < n * n; t += nt) V[t] = 0.0;
     __syncthreads();
-    // Householder tridiagonalisation: for k, reflect rows/cols k+1..n-1 of column k
+    // Householder tridiagonalisation: reflect rows/cols k+1..n-1 of column k
     for (int k = 0; k < n - 2; ++k) {
         const int m0 = k + 1, len = n - m0;
         double part = 0.0;
@@ -646,14 +720,12 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
         const double sigma = block_sum(part, red);
         const double x0 = M[m0 + n * k];
         const double alpha = sigma > 0.0 ? (x0 >= 0.0 ? -sqrt(sigma) : sqrt(sigma)) : 0.0;
-        // v = x - alpha e1, beta = 2 / v^T v
         for (int i = m0 + tid; i < n; i += nt) V[i + n * k] = M[i + n * k] - (i == m0 ? alpha : 0.0);
-        __syncthreads();
         const double vtv = sigma - 2.0 * alpha * x0 + alpha * alpha;
         const double bk = (sigma > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
         if (tid == 0) beta[k] = bk;
-        // p = bk * A v (trailing block), stored in y
-        for (int i = m0 + tid; i < n; i += nt) {
+        __syncthreads();
+        for (int i = m0 + tid; i < n; i += nt) {  // p = bk A v
             double acc = 0.0;
             for (int j = m0; j < n; ++j) acc += M[i + n * j] * V[j + n * k];
             y[i] = bk * acc;
@@ -662,18 +734,18 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
         double kp = 0.0;
         for (int i = m0 + tid; i < n; i += nt) kp += V[i + n * k] * y[i];
         const double K = 0.5 * bk * block_sum(kp, red);
-        // w = p - K v ; A -= v w^T + w v^T
-        for (int i = m0 + tid; i < n; i += nt) y[i] -= K * V[i + n * k];
+        for (int i = m0 + tid; i < n; i += nt) y[i] -= K * V[i + n * k];  // w = p - K v
         __syncthreads();
-        for (int t = tid; t < len * len; t += nt) {
+        for (int t = tid; t < len * len; t += nt) {  // A -= v w^T + w v^T
             const int i = m0 + t % len, j = m0 + t / len;
             M[i + n * j] -= V[i + n * k] * y[j] + y[i] * V[j + n * k];
         }
-        for (int i = m0 + tid; i < n; i += nt) {
-            const double v = (i == m0 && bk != 0.0) ? alpha : (bk != 0.0 ? 0.0 : M[i + n * k]);
-            M[i + n * k] = v;
-            M[k + n * i] = v;
-        }
+        if (bk != 0.0)
+            for (int i = m0 + tid; i < n; i += nt) {
+                const double v = (i == m0) ? alpha : 0.0;
+                M[i + n * k] = v;
+                M[k + n * i] = v;
+            }
         __syncthreads();
     }
     for (int i = tid; i < n; i += nt) {
@@ -682,7 +754,6 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
     }
     __syncthreads();
     // lowest eigenvalue by multisection on Sturm counts
-    __shared__ double lo_hi[2];
     if (tid == 0) {
         double lo = 1e300, hi = -1e300;
         for (int i = 0; i < n; ++i) {
@@ -695,17 +766,15 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
         lo_hi[1] = hi + pad;
     }
     __syncthreads();
-    __shared__ int first_true;
-    for (int it = 0; it < 64; ++it) {
+    for (int it = 0; it < 200; ++it) {
         const double lo = lo_hi[0], hi = lo_hi[1];
         if (!(hi - lo > 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)))) break;
         if (tid == 0) first_true = nt;
         __syncthreads();
-        // point t in (lo, hi]; count of eigenvalues < x (Sturm sequence of the LDL^T pivots)
         const double x = lo + (hi - lo) * (double)(tid + 1) / (double)nt;
         int count = 0;
         double q = 1.0;
-        for (int i = 0; i < n; ++i) {
+        for (int i = 0; i < n; ++i) {  // number of eigenvalues below x (LDL^T pivots)
             const double e2 = i > 0 ? ee[i - 1] * ee[i - 1] : 0.0;
             q = dd[i] - x - (i > 0 ? e2 / q : 0.0);
             if (q == 0.0) q = -1e-300;
@@ -715,81 +784,73 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
         __syncthreads();
         if (tid == 0) {
             const int f = first_true;
-            const double nlo = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)nt;
-            const double nhi = f >= nt ? hi : lo + (hi - lo) * (double)(f + 1) / (double)nt;
-            lo_hi[0] = nlo;
-            lo_hi[1] = nhi;
+            lo_hi[0] = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)nt;
+            lo_hi[1] = f >= nt ? hi : lo + (hi - lo) * (double)(f + 1) / (double)nt;
         }
         __syncthreads();
     }
     const double lambda = 0.5 * (lo_hi[0] + lo_hi[1]);
-    // inverse iteration on T - lambda I (serial, partial-pivoting tridiagonal LU)
+    // inverse iteration on T - lambda I (LU with partial pivoting, LAPACK dgttrf/dgttrs scheme)
     if (tid == 0) {
-        double* dl = tdl;
-        double* du = tdu;
-        double* du2 = tdu2;
-        double* dg = tdg;
         for (int i = 0; i < n; ++i) {
-            dg[i] = dd[i] - lambda;
+            tdg[i] = dd[i] - lambda;
+            tdu2[i] = 0.0;
             if (i + 1 < n) {
-                dl[i] = ee[i];
-                du[i] = ee[i];
+                tdl[i] = ee[i];
+                tdu[i] = ee[i];
             }
         }
-        // LU with partial pivoting (LAPACK dgttrf scheme)
-        for (int i = 0; i < n; ++i) du2[i] = 0.0;
         for (int i = 0; i + 1 < n; ++i) {
-            if (fabs(dg[i]) >= fabs(dl[i])) {
+            if (fabs(tdg[i]) >= fabs(tdl[i])) {
                 ipiv[i] = i;
-                if (dg[i] == 0.0) dg[i] = 1e-300;
-                const double f = dl[i] / dg[i];
-                dl[i] = f;
-                dg[i + 1] -= f * du[i];
+                if (tdg[i] == 0.0) tdg[i] = 1e-300;
+                const double f = tdl[i] / tdg[i];
+                tdl[i] = f;
+                tdg[i + 1] -= f * tdu[i];
             } else {
                 ipiv[i] = i + 1;
-                const double f = dg[i] / dl[i];
-                dg[i] = dl[i];
-                dl[i] = f;
-                const double t = du[i];
-                du[i] = dg[i + 1];
-                dg[i + 1] = t - f * dg[i + 1];
+                const double f = tdg[i] / tdl[i];
+                tdg[i] = tdl[i];
+                tdl[i] = f;
+                const double t = tdu[i];
+                tdu[i] = tdg[i + 1];
+                tdg[i + 1] = t - f * tdg[i + 1];
                 if (i + 2 < n) {
-                    du2[i] = du[i + 1];
-                    du[i + 1] = -f * du[i + 1];
+                    tdu2[i] = tdu[i + 1];
+                    tdu[i + 1] = -f * tdu[i + 1];
                 }
             }
         }
         if (n > 0) {
             ipiv[n - 1] = n - 1;
-            if (dg[n - 1] == 0.0) dg[n - 1] = 1e-300;
+            if (tdg[n - 1] == 0.0) tdg[n - 1] = 1e-300;
         }
         for (int i = 0; i < n; ++i) y[i] = 1.0;
         for (int iter = 0; iter < 3; ++iter) {
-            // solve L U z = y
             for (int i = 0; i + 1 < n; ++i) {
                 if (ipiv[i] == i) {
-                    y[i + 1] -= dl[i] * y[i];
+                    y[i + 1] -= tdl[i] * y[i];
                 } else {
                     const double t = y[i];
                     y[i] = y[i + 1];
-                    y[i + 1] = t - dl[i] * y[i];
+                    y[i + 1] = t - tdl[i] * y[i];
                 }
             }
             for (int i = n - 1; i >= 0; --i) {
                 double v = y[i];
-                if (i + 1 < n) v -= du[i] * y[i + 1];
-                if (i + 2 < n) v -= du2[i] * y[i + 2];
-                y[i] = v / dg[i];
+                if (i + 1 < n) v -= tdu[i] * y[i + 1];
+                if (i + 2 < n) v -= tdu2[i] * y[i + 2];
+                y[i] = v / tdg[i];
             }
             double nrm = 0.0;
             for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
-            nrm = sqrt(nrm);
-            for (int i = 0; i < n; ++i) y[i] /= nrm;
+            nrm = 1.0 / sqrt(nrm);
+            for (int i = 0; i < n; ++i) y[i] *= nrm;
         }
         sh_scalar[1] = lambda;
     }
     __syncthreads();
-    // back-transform z = Q y with Q = H_0 H_1 ... H_{n-3}: apply H_{n-3} first
+    // back-transform z = H_0 H_1 ... H_{n-3} y
     for (int k = n - 3; k >= 0; --k) {
         const int m0 = k + 1;
         double part = 0.0;
@@ -798,7 +859,7 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
         for (int i = m0 + tid; i < n; i += nt) y[i] -= s * V[i + n * k];
         __syncthreads();
     }
-    // c = L^{-T} z (back substitution, right-looking over columns of L)
+    // c = L^{-T} z (back substitution over the columns of L)
     for (int k = n - 1; k >= 0; --k) {
         if (tid == 0) y[k] /= Lm[k + n * k];
         __syncthreads();
@@ -810,6 +871,12 @@ __global__ void __launch_bounds__(1024) k_gep_lowest(const double* __restrict__ 
     if (tid == 0) {
         energy[0] = sh_scalar[1];
         energy[1] = sh_scalar[1] + e_nuc;
+    }
+    // determinant coefficients a_d = sum_terms t_dI c_I for the backward weights
+    for (int d = tid; d < n_det; d += nt) {
+        double acc = 0.0;
+        for (long long t = off[d]; t < off[d + 1]; ++t) acc += tc[t] * y[ts[t]];
+        dcoef[d] = acc;
     }
 }
 
@@ -888,19 +955,40 @@ __global__ void k_normalization_pullback(const double* __restrict__ gcn, const d
 // ---------------------------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------------------------
+// LDS of the pair kernels: full np x np kernel, S_act, h_act, per-wave slabs and, for the
+// backward, the adjoint accumulators (full np x np when that still fits, else packed).
+bool pairs_full_dg(int n, int ng) {
+    const size_t np = (size_t)n * (n + 1) / 2;
+    const size_t bytes = (2 * np * np + 4 * (size_t)n * n) * sizeof(double) + 4 * sizeof(WaveSlab);
+    (void)ng;
+    return bytes <= 150 * 1024;
+}
+
 size_t pairs_lds_bytes(int n, int ng, bool backward) {
-    size_t doubles = (size_t)ng + 2 * (size_t)n * n;
-    if (backward) doubles += (size_t)ng + 2 * (size_t)n * n;
+    const size_t np = (size_t)n * (n + 1) / 2;
+    size_t doubles = np * np + 2 * (size_t)n * n;
+    if (backward) doubles += (pairs_full_dg(n, ng) ? np * np : (size_t)ng) + 2 * (size_t)n * n;
     return doubles * sizeof(double) + 4 * sizeof(WaveSlab);
 }
 
-hipError_t configure_kernels(int n, int ng) {
+size_t gep_lds_bytes(int n);
+bool gep_in_lds(int n);
+
+hipError_t configure_kernels(int n, int ng, int n_str) {
     const size_t fwd = pairs_lds_bytes(n, ng, false), bwd = pairs_lds_bytes(n, ng, true);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<false>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<false, false>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)fwd);
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<true>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bwd);
+    e = pairs_full_dg(n, ng)
+            ? hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<true, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bwd)
+            : hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<true, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bwd);
+    if (e != hipSuccess) return e;
+    if (gep_in_lds(n_str))
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gep_lowest<true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)gep_lds_bytes(n_str));
+    return e;
 }
 
 void launch_orbital_prep(hipStream_t st, const double* C, const uint8_t* support, const double* S, const double* h,
@@ -914,12 +1002,16 @@ void launch_active_matrices(hipStream_t st, const double* Cn, const double* SC, 
     hipLaunchKernelGGL(k_active_matrices, dim3(1), dim3(256), 0, st, Cn, SC, HC, n_bf, n, sact, hact, flags);
 }
 
-void launch_transform(hipStream_t st, const double* B, const double* Cn, int n_bf, int n, int n_aux, double* T,
-                      double* L, double* G) {
+void launch_ri_one_index(hipStream_t st, const double* B, const double* Cn, int n_bf, int n, int n_aux, double* T) {
     const int tb = 64;
     hipLaunchKernelGGL(k_ri_one_index, dim3((n_aux + tb - 1) / tb, n_bf), dim3(tb), 0, st, B, Cn, n_bf, n, n_aux, T);
-    hipLaunchKernelGGL(k_ri_pair_factors, dim3((n_aux + tb - 1) / tb, n), dim3(tb), 0, st, T, Cn, n_bf, n, n_aux, L);
+}
+
+void launch_ri_pair_stage(hipStream_t st, const double* T, const double* Cn, int n_bf, int n, int n_aux, double* L,
+                          double* G) {
+    const int tb = 64;
     const int np = n * (n + 1) / 2, ng = np * (np + 1) / 2;
+    hipLaunchKernelGGL(k_ri_pair_factors, dim3((n_aux + tb - 1) / tb, np), dim3(tb), 0, st, T, Cn, n_bf, n, n_aux, L);
     hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4), dim3(256), 0, st, L, n_aux, np, G);
 }
 
@@ -929,16 +1021,19 @@ void launch_pairs(hipStream_t st, bool backward, int n_blocks, const double* G, 
                   int* flags) {
     const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
     const size_t lds = pairs_lds_bytes(n, ng, backward);
-    if (backward)
-        hipLaunchKernelGGL(k_pairs<true>, dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask, bmask, det_a,
-                           det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
+    if (!backward)
+        hipLaunchKernelGGL((k_pairs<false, false>), dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask,
+                           bmask, det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
+    else if (pairs_full_dg(n, ng))
+        hipLaunchKernelGGL((k_pairs<true, true>), dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask,
+                           bmask, det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
     else
-        hipLaunchKernelGGL(k_pairs<false>, dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask, bmask,
-                           det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
+        hipLaunchKernelGGL((k_pairs<true, false>), dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask,
+                           bmask, det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
 }
 
 void launch_reduce_partials(hipStream_t st, const double* partials, int n_blocks, int len, double* out) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 255) / 256), dim3(256), 0, st, partials, n_blocks, len, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4), dim3(256), 0, st, partials, n_blocks, len, out);
 }
 
 void launch_structure(hipStream_t st, const double* Hdet, const double* Sdet, int n_det, int n_str,
@@ -951,9 +1046,20 @@ void launch_structure(hipStream_t st, const double* Hdet, const double* Sdet, in
                        soff, sdet, scoef, Hs, Ss);
 }
 
+size_t gep_lds_bytes(int n) { return gep_work_doubles(n) * sizeof(double); }
+
+bool gep_in_lds(int n) { return gep_lds_bytes(n) <= 144 * 1024; }
+
 void launch_gep(hipStream_t st, const double* Hs, const double* Ss, int n, double e_nuc, double* work, double* coef,
-                double* energy, int* flags) {
-    hipLaunchKernelGGL(k_gep_lowest, dim3(1), dim3(1024), 0, st, Hs, Ss, n, e_nuc, work, coef, energy, flags);
+                double* energy, int* flags, const long long* off, const int* ts, const double* tc, int n_det,
+                double* dcoef) {
+    const int threads = n <= 64 ? 64 : 256;
+    if (gep_in_lds(n))
+        hipLaunchKernelGGL(k_gep_lowest<true>, dim3(1), dim3(threads), gep_lds_bytes(n), st, Hs, Ss, n, e_nuc, work,
+                           coef, energy, flags, off, ts, tc, n_det, dcoef);
+    else
+        hipLaunchKernelGGL(k_gep_lowest<false>, dim3(1), dim3(threads), 0, st, Hs, Ss, n, e_nuc, work, coef, energy,
+                           flags, off, ts, tc, n_det, dcoef);
 }
 
 void launch_det_coefficients(hipStream_t st, const long long* off, const int* ts, const double* tc, const double* coef,

@@ -25,14 +25,15 @@ enum DeviceFlag : int {
 inline std::size_t gep_work_doubles(int n) { return 3 * (std::size_t)n * n + 9 * (std::size_t)n; }
 
 std::size_t pairs_lds_bytes(int n, int ng, bool backward);
-hipError_t configure_kernels(int n, int ng);
+hipError_t configure_kernels(int n, int ng, int n_str);
 
 void launch_orbital_prep(hipStream_t st, const double* C, const uint8_t* support, const double* S, const double* h,
                          int n_bf, int n, double* Cn, double* SC, double* HC, double* nu2, int* flags);
 void launch_active_matrices(hipStream_t st, const double* Cn, const double* SC, const double* HC, int n_bf, int n,
                             double* sact, double* hact, int* flags);
-void launch_transform(hipStream_t st, const double* B, const double* Cn, int n_bf, int n, int n_aux, double* T,
-                      double* L, double* G);
+void launch_ri_one_index(hipStream_t st, const double* B, const double* Cn, int n_bf, int n, int n_aux, double* T);
+void launch_ri_pair_stage(hipStream_t st, const double* T, const double* Cn, int n_bf, int n, int n_aux, double* L,
+                          double* G);
 void launch_pairs(hipStream_t st, bool backward, int n_blocks, const double* G, const double* sact, const double* hact,
                   int n, int ng, const uint64_t* amask, const uint64_t* bmask, const int* det_a, const int* det_b,
                   int n_det, double* Hdet, double* Sdet, const double* dcoef, const double* energy, double* partials,
@@ -42,7 +43,8 @@ void launch_structure(hipStream_t st, const double* Hdet, const double* Sdet, in
                       const long long* soff, const int* sdet, const double* scoef, double* Yh, double* Ys, double* Hs,
                       double* Ss);
 void launch_gep(hipStream_t st, const double* Hs, const double* Ss, int n, double e_nuc, double* work, double* coef,
-                double* energy, int* flags);
+                double* energy, int* flags, const long long* off, const int* ts, const double* tc, int n_det,
+                double* dcoef);
 void launch_det_coefficients(hipStream_t st, const long long* off, const int* ts, const double* tc, const double* coef,
                              int n_det, double* a);
 void launch_gradient_tail(hipStream_t st, const double* T, const double* L, const double* dsum, const double* SC,

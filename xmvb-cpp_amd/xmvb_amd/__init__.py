@@ -73,8 +73,8 @@ class Flags(C.Structure):
 
 class PhaseTimes(C.Structure):
     _fields_ = [(name, C.c_float) for name in (
-        "orbital_prep_ms", "transform_ms", "pair_forward_ms", "structure_ms", "eigensolve_ms", "pair_backward_ms",
-        "gradient_tail_ms", "total_ms")]
+        "orbital_prep_ms", "transform_ms", "ri_one_index_ms", "pair_forward_ms", "structure_ms", "eigensolve_ms",
+        "pair_backward_ms", "pair_backward_kernel_ms", "gradient_tail_ms", "total_ms")]
 
 
 _lib = None
