@@ -62,6 +62,7 @@ struct vbe_problem {
     double* scoef = nullptr;
     // per-evaluation workspace
     double *C = nullptr, *grad = nullptr;
+    const double* cur_C = nullptr;  // orbitals of the current evaluation (problem buffer or caller's device array)
     uint8_t* support = nullptr;
     double *Cn = nullptr, *nu2 = nullptr, *SC = nullptr, *HC = nullptr, *sact = nullptr, *hact = nullptr;
     double *T = nullptr, *L = nullptr, *G = nullptr;
@@ -189,7 +190,7 @@ void enqueue_forward(vbe_problem* p) {
     using namespace xmvb_amd;
     check(hipMemsetAsync(p->flags, 0, sizeof(int) * FLAG_COUNT, st), "hipMemsetAsync flags");
     check(hipEventRecord(p->ev[PH_START], st), "hipEventRecord");
-    launch_orbital_prep(st, p->C, p->last_support, p->S, p->h, d.n_bf, d.n_act, p->Cn, p->SC, p->HC, p->nu2, p->flags);
+    launch_orbital_prep(st, p->cur_C, p->last_support, p->S, p->h, d.n_bf, d.n_act, p->Cn, p->SC, p->HC, p->nu2, p->flags);
     launch_active_matrices(st, p->Cn, p->SC, p->HC, d.n_bf, d.n_act, p->sact, p->hact, p->flags);
     check(hipEventRecord(p->ev[PH_PREP], st), "hipEventRecord");
     launch_ri_one_index(st, p->B, p->Cn, d.n_bf, d.n_act, d.n_aux, p->T);
@@ -250,10 +251,7 @@ int eval_core(vbe_context* ctx, vbe_problem* p, const double* d_C, const uint8_t
               double* E_total, double* d_grad) {
     if (!ctx || !p || p->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
     check(hipSetDevice(ctx->device), "hipSetDevice");
-    if (d_C != p->C)
-        check(hipMemcpyAsync(p->C, d_C, sizeof(double) * p->dims.n_act * p->dims.n_bf, hipMemcpyDeviceToDevice,
-                             ctx->stream),
-              "orbitals D2D");
+    p->cur_C = d_C;
     p->last_support = d_support;
     p->forward_valid = false;
     // everything is enqueued back to back; a failure flagged by the forward (singular pairs,
@@ -380,8 +378,8 @@ int vbe_problem_upload(vbe_context* ctx, const vbe_dims* dims, const double* S_a
         const long long n_pairs = (long long)nd * (nd + 1) / 2;
         // forward: about two pairs per wavefront; backward: fewer blocks so that the per-block
         // adjoint partials stay small
-        p->pair_blocks_fwd = (int)std::min<long long>(std::max<long long>(1, (n_pairs + 7) / 8), 8192);
-        p->pair_blocks_bwd = (int)std::min<long long>(std::max<long long>(1, (n_pairs + 3) / 4), 512);
+        p->pair_blocks_fwd = (int)std::min<long long>(std::max<long long>(1, (n_pairs + 15) / 16), 4096);
+        p->pair_blocks_bwd = (int)std::min<long long>(std::max<long long>(1, (n_pairs + 7) / 8), 512);
         p->C = p->alloc<double>((size_t)n * nbf);
         p->grad = p->alloc<double>((size_t)n * nbf);
         p->Cn = p->alloc<double>((size_t)n * nbf);

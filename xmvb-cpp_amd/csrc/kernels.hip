@@ -28,9 +28,47 @@ __device__ __forceinline__ int pair_index(int a, int b) {
     return hi * (hi + 1) / 2 + lo;
 }
 
+// wave-uniform broadcast of lane `src`
+__device__ __forceinline__ double wave_bcast(double v, int src) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 64-bit DPP move (both halves with the same control)
+template <int Ctrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), Ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), Ctrl, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// wave-uniform maximum: DPP within 16-lane rows (xor 1, xor 2, half-row mirror, row mirror),
+// then the four row maxima through readlane
+__device__ __forceinline__ double wave_max(double v) {
+    v = fmax(v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = fmax(v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = fmax(v, dpp_f64<0x141>(v));  // row_half_mirror
+    v = fmax(v, dpp_f64<0x140>(v));  // row_mirror
+    return fmax(fmax(wave_bcast(v, 0), wave_bcast(v, 16)), fmax(wave_bcast(v, 32), wave_bcast(v, 48)));
+}
+
+// wave-uniform sum: DPP within 16-lane rows, then the four row sums through readlane
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return (wave_bcast(v, 0) + wave_bcast(v, 16)) + (wave_bcast(v, 32) + wave_bcast(v, 48));
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
-    // 64-wide wavefront reduction, then across waves through LDS (red: >= 16 doubles)
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    // wavefront sum, then across waves through LDS (red: >= 16 doubles); a single-wave block
+    // needs no barrier
+    v = wave_sum(v);
+    if (blockDim.x <= 64) return v;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     __syncthreads();
     if (lane == 0) red[wave] = v;
@@ -58,16 +96,25 @@ __global__ void k_orbital_prep(const double* __restrict__ C, const uint8_t* __re
     }
     __syncthreads();
     double q = 0.0;
-    for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
-        double sv = 0.0, hv = 0.0;
-        for (int nu = 0; nu < n_bf; ++nu) {
-            const double cv = c[nu];
-            sv += S[mu + (size_t)n_bf * nu] * cv;
-            hv += h[mu + (size_t)n_bf * nu] * cv;
+    {
+        // (S c)[mu] = sum_nu S[nu, mu] c[nu] (S and h are symmetric): one wavefront per row mu,
+        // lanes over nu read contiguous columns
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int mu = wave; mu < n_bf; mu += nw) {
+            double sv = 0.0, hv = 0.0;
+            for (int nu = lane; nu < n_bf; nu += 64) {
+                const double cv = c[nu];
+                sv += S[nu + (size_t)n_bf * mu] * cv;
+                hv += h[nu + (size_t)n_bf * mu] * cv;
+            }
+            sv = wave_sum(sv);
+            hv = wave_sum(hv);
+            if (lane == 0) {
+                SC[mu + (size_t)n_bf * k] = sv;
+                HC[mu + (size_t)n_bf * k] = hv;
+                q += c[mu] * sv;
+            }
         }
-        SC[mu + (size_t)n_bf * k] = sv;
-        HC[mu + (size_t)n_bf * k] = hv;
-        q += c[mu] * sv;
     }
     q = block_sum(q, red);
     const bool ok = isfinite(q) && q > 2.220446049250313e-16;
@@ -86,16 +133,23 @@ __global__ void k_active_matrices(const double* __restrict__ Cn, const double* _
                                   const double* __restrict__ HC, int n_bf, int n, double* __restrict__ sact,
                                   double* __restrict__ hact, int* __restrict__ flags) {
     __shared__ double s_l[MAX_ACT * MAX_ACT], h_l[MAX_ACT * MAX_ACT];
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-        const int i = t % n, j = t / n;
-        double sv = 0.0, hv = 0.0;
-        for (int mu = 0; mu < n_bf; ++mu) {
-            const double ci = Cn[(size_t)i * n_bf + mu];
-            sv += ci * SC[mu + (size_t)n_bf * j];
-            hv += ci * HC[mu + (size_t)n_bf * j];
-        }
-        s_l[t] = sv;
-        h_l[t] = hv;
+    {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int i = 0; i < n; ++i)
+            for (int j = wave; j < n; j += nw) {  // one wavefront per entry, lanes over mu
+                double sv = 0.0, hv = 0.0;
+                for (int mu = lane; mu < n_bf; mu += 64) {
+                    const double ci = Cn[(size_t)i * n_bf + mu];
+                    sv += ci * SC[mu + (size_t)n_bf * j];
+                    hv += ci * HC[mu + (size_t)n_bf * j];
+                }
+                sv = wave_sum(sv);
+                hv = wave_sum(hv);
+                if (lane == 0) {
+                    s_l[i + n * j] = sv;
+                    h_l[i + n * j] = hv;
+                }
+            }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
@@ -133,24 +187,31 @@ __global__ void k_active_matrices(const double* __restrict__ Cn, const double* _
 }
 
 // T[A, q, mu] = sum_nu B[A, (mu nu)] Cn[nu, q]; T index A + n_aux*(q + n*mu).
-// grid (ceil(n_aux/blockDim), n_bf). Each packed B column is read by its two AO rows.
-__global__ void k_ri_one_index(const double* __restrict__ B, const double* __restrict__ Cn, int n_bf, int n,
-                               int n_aux, double* __restrict__ T) {
-    const int A = blockIdx.x * blockDim.x + threadIdx.x;
+// grid (ceil(n_aux/64), n_bf), four wavefronts per block splitting nu; lanes run over A so every
+// packed B column is read as contiguous 512-byte segments (each column is read by its two AO rows).
+__global__ void __launch_bounds__(256) k_ri_one_index(const double* __restrict__ B, const double* __restrict__ Cn,
+                                                      int n_bf, int n, int n_aux, double* __restrict__ T) {
+    __shared__ double part[4][MAX_ACT][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int A = blockIdx.x * 64 + lane;
     const int mu = blockIdx.y;
-    if (A >= n_aux) return;
+    const bool active = A < n_aux;
     double acc[MAX_ACT];
 #pragma unroll
     for (int q = 0; q < MAX_ACT; ++q) acc[q] = 0.0;
-    for (int nu = 0; nu < n_bf; ++nu) {
-        const double b = B[(size_t)A + (size_t)n_aux * (size_t)pair_index(mu, nu)];
+    for (int nu = wave; nu < n_bf; nu += 4) {
+        const double b = active ? B[(size_t)A + (size_t)n_aux * (size_t)pair_index(mu, nu)] : 0.0;
 #pragma unroll
         for (int q = 0; q < MAX_ACT; ++q)
             if (q < n) acc[q] += b * Cn[(size_t)q * n_bf + nu];
     }
 #pragma unroll
-    for (int q = 0; q < MAX_ACT; ++q)
-        if (q < n) T[(size_t)A + (size_t)n_aux * (q + (size_t)n * mu)] = acc[q];
+    for (int q = 0; q < MAX_ACT; ++q) part[wave][q][lane] = acc[q];
+    __syncthreads();
+    for (int q = wave; q < n; q += 4) {
+        const double v = (part[0][q][lane] + part[1][q][lane]) + (part[2][q][lane] + part[3][q][lane]);
+        if (active) T[(size_t)A + (size_t)n_aux * (q + (size_t)n * mu)] = v;
+    }
 }
 
 // L[A, (p,q)] = sum_mu T[A,q,mu] Cn[mu,p], q <= p. One thread per (A, packed (p,q)):
@@ -166,14 +227,17 @@ __global__ void k_ri_pair_factors(const double* __restrict__ T, const double* __
     const int q = P - p * (p + 1) / 2;
     const double* t = T + (size_t)A + (size_t)n_aux * q;
     const double* c = Cn + (size_t)p * n_bf;
-    double acc0 = 0.0, acc1 = 0.0;
+    const size_t step = (size_t)n_aux * n;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int mu = 0;
-    for (; mu + 1 < n_bf; mu += 2) {
-        acc0 += t[(size_t)n_aux * n * mu] * c[mu];
-        acc1 += t[(size_t)n_aux * n * (mu + 1)] * c[mu + 1];
+    for (; mu + 3 < n_bf; mu += 4) {
+        a0 += t[step * mu] * c[mu];
+        a1 += t[step * (mu + 1)] * c[mu + 1];
+        a2 += t[step * (mu + 2)] * c[mu + 2];
+        a3 += t[step * (mu + 3)] * c[mu + 3];
     }
-    if (mu < n_bf) acc0 += t[(size_t)n_aux * n * mu] * c[mu];
-    L[(size_t)A + (size_t)n_aux * P] = acc0 + acc1;
+    for (; mu < n_bf; ++mu) a0 += t[step * mu] * c[mu];
+    L[(size_t)A + (size_t)n_aux * P] = (a0 + a1) + (a2 + a3);
 }
 
 // packed G[(P,Q)] = sum_A L[A,P] L[A,Q]; one wavefront per packed entry.
@@ -195,19 +259,6 @@ __global__ void k_ri_pair_gram(const double* __restrict__ L, int n_aux, int np, 
 // ---------------------------------------------------------------------------------------------
 // determinant pairs: one wavefront per unordered determinant pair (d <= e)
 // ---------------------------------------------------------------------------------------------
-
-// wave-uniform broadcast of lane `src`
-__device__ __forceinline__ double wave_bcast(double v, int src) {
-    const long long bits = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), src);
-    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -241,30 +292,25 @@ __device__ BlockInverse invert_block(double x, int m, int lane) {
     double my_piv = 1.0;
     bool singular = false;
     for (int k = 0; k < m; ++k) {
-        double cand = (valid && !((rows_used >> r) & 1u) && !((cols_used >> l) & 1u)) ? fabs(x) : -1.0;
-        int idx = lane;
-        for (int off = 32; off > 0; off >>= 1) {
-            const double oc = __shfl_xor(cand, off, 64);
-            const int oi = __shfl_xor(idx, off, 64);
-            if (oc > cand || (oc == cand && oi < idx)) {
-                cand = oc;
-                idx = oi;
-            }
-        }
-        if (!(cand > 0.0)) {  // exactly zero corner (or non-finite): rank deficient
+        const double cand = (valid && !((rows_used >> r) & 1u) && !((cols_used >> l) & 1u)) ? fabs(x) : -1.0;
+        const double best = wave_max(cand);
+        if (!(best > 0.0)) {  // exactly zero corner (or non-finite): rank deficient
             singular = true;
             break;
         }
+        // lowest lane holding the maximum (deterministic tie-break)
+        const int idx = __builtin_ctzll(__ballot(cand == best));
         const int pr = idx & 7, pc = idx >> 3;
         const double p = wave_bcast(x, idx);
-        maxpiv = fmax(maxpiv, cand);
-        minpiv = fmin(minpiv, cand);
+        const double rp = 1.0 / p;
+        maxpiv = fmax(maxpiv, best);
+        minpiv = fmin(minpiv, best);
         inversions += __builtin_popcount(rows_used >> (pr + 1)) + __builtin_popcount(cols_used >> (pc + 1));
         const double x_r_pc = __shfl(x, r + 8 * pc, 64);
         const double x_pr_l = __shfl(x, pr + 8 * l, 64);
         const double a_pr_l = __shfl(aug, pr + 8 * l, 64);
         if (valid && r != pr) {
-            const double f = x_r_pc / p;
+            const double f = x_r_pc * rp;
             x -= f * x_pr_l;
             aug -= f * a_pr_l;
         }
@@ -303,7 +349,7 @@ struct WaveSlab {
 // accumulated per block in LDS (full np x np when FullDg, packed otherwise) and written as
 // packed partials.
 template <bool Backward, bool FullDg>
-__global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob, const double* __restrict__ sact_g,
+__global__ void __launch_bounds__(512) k_pairs(const double* __restrict__ G_glob, const double* __restrict__ sact_g,
                                                const double* __restrict__ hact_g, int n, int ng,
                                                const uint64_t* __restrict__ amask, const uint64_t* __restrict__ bmask,
                                                const int* __restrict__ det_a, const int* __restrict__ det_b, int n_det,
@@ -324,9 +370,10 @@ __global__ void __launch_bounds__(256) k_pairs(const double* __restrict__ G_glob
         tail = dh + n * n;
     }
     WaveSlab* slabs = reinterpret_cast<WaveSlab*>(tail);
-    for (int t = threadIdx.x; t < np * np; t += blockDim.x) {
-        const int P = t % np, Q = t / np;
-        G2[t] = G_glob[pair_index(P, Q)];
+    {
+        const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, ln = threadIdx.x & 63;
+        for (int Q = wave; Q < np; Q += nwaves)
+            for (int P = ln; P < np; P += 64) G2[P + np * Q] = G_glob[pair_index(P, Q)];
     }
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
         sact[t] = sact_g[t];
@@ -648,69 +695,74 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const double* __restrict__ H
     __shared__ int sh_flag;
     __shared__ int first_true;
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int t = tid; t < n * n; t += nt) {
-        const int i = t % n, j = t / n;
-        Lm[t] = 0.5 * (Ss[t] + Ss[j + n * i]);
-        M[t] = 0.5 * (Hs[t] + Hs[j + n * i]);
-        V[t] = 0.0;
-    }
+    for (int i = tid; i < n; i += nt)
+        for (int j = 0; j < n; ++j) {
+            Lm[i + n * j] = 0.5 * (Ss[i + n * j] + Ss[j + n * i]);
+            M[i + n * j] = 0.5 * (Hs[i + n * j] + Hs[j + n * i]);
+        }
     if (tid == 0) sh_flag = 0;
     __syncthreads();
-    // Cholesky (right-looking, lower)
+    // Cholesky (right-looking, lower); threads own rows, so no index division is needed
     for (int k = 0; k < n; ++k) {
         const double a = Lm[k + n * k];
         if (!(a > 0.0)) {
             if (tid == 0) flags[FLAG_METRIC] = 1;
             return;  // uniform: every thread read the same diagonal after the last barrier
         }
-        const double dk = sqrt(a);
+        const double dk = sqrt(a), rdk = 1.0 / dk;
         __syncthreads();
-        for (int i = k + tid; i < n; i += nt) Lm[i + n * k] = (i == k) ? dk : Lm[i + n * k] / dk;
+        for (int i = k + tid; i < n; i += nt) Lm[i + n * k] = (i == k) ? dk : Lm[i + n * k] * rdk;
         __syncthreads();
-        const int rem = n - k - 1;
-        for (int t = tid; t < rem * rem; t += nt) {
-            const int i = k + 1 + t % rem, j = k + 1 + t / rem;
-            if (j <= i) Lm[i + n * j] -= Lm[i + n * k] * Lm[j + n * k];
+        for (int i = k + 1 + tid; i < n; i += nt) {
+            const double lik = Lm[i + n * k];
+            for (int j = k + 1; j <= i; ++j) Lm[i + n * j] -= lik * Lm[j + n * k];
         }
         __syncthreads();
     }
-    // X = L^{-1} H  (in M; column-parallel forward substitution)
-    for (int k = 0; k < n; ++k) {
-        const double lkk = Lm[k + n * k];
-        for (int j = tid; j < n; j += nt) M[k + n * j] /= lkk;
-        __syncthreads();
-        const int rem = n - k - 1;
-        for (int t = tid; t < rem * n; t += nt) {
-            const int i = k + 1 + t % rem, j = t / rem;
-            M[i + n * j] -= Lm[i + n * k] * M[k + n * j];
+    // M <- L^{-1} H by forward substitution, one column per thread (no barriers inside)
+    for (int j = tid; j < n; j += nt) {
+        for (int i = 0; i < n; ++i) {
+            double v0 = M[i + n * j], v1 = 0.0, v2 = 0.0, v3 = 0.0;
+            int k = 0;
+            for (; k + 3 < i; k += 4) {
+                v0 -= Lm[i + n * k] * M[k + n * j];
+                v1 -= Lm[i + n * (k + 1)] * M[k + 1 + n * j];
+                v2 -= Lm[i + n * (k + 2)] * M[k + 2 + n * j];
+                v3 -= Lm[i + n * (k + 3)] * M[k + 3 + n * j];
+            }
+            for (; k < i; ++k) v0 -= Lm[i + n * k] * M[k + n * j];
+            M[i + n * j] = ((v0 + v1) + (v2 + v3)) / Lm[i + n * i];
         }
-        __syncthreads();
     }
-    // M <- L^{-1} X^T (X^T staged through V)
-    for (int t = tid; t < n * n; t += nt) V[t] = M[(t / n) + n * (t % n)];
     __syncthreads();
-    for (int t = tid; t < n * n; t += nt) M[t] = V[t];
+    // M <- L^{-1} M^T (transpose staged through V), then symmetrise
+    for (int i = tid; i < n; i += nt)
+        for (int j = 0; j < n; ++j) V[i + n * j] = M[j + n * i];
     __syncthreads();
-    for (int k = 0; k < n; ++k) {
-        const double lkk = Lm[k + n * k];
-        for (int j = tid; j < n; j += nt) M[k + n * j] /= lkk;
-        __syncthreads();
-        const int rem = n - k - 1;
-        for (int t = tid; t < rem * n; t += nt) {
-            const int i = k + 1 + t % rem, j = t / rem;
-            M[i + n * j] -= Lm[i + n * k] * M[k + n * j];
-        }
-        __syncthreads();
-    }
-    for (int t = tid; t < n * n; t += nt) {
-        const int i = t % n, j = t / n;
-        V[t] = 0.0;
-        if (i < j) {
-            const double s = 0.5 * (M[i + n * j] + M[j + n * i]);
-            M[i + n * j] = s;
-            M[j + n * i] = s;
+    for (int j = tid; j < n; j += nt) {
+        for (int i = 0; i < n; ++i) {
+            double v0 = V[i + n * j], v1 = 0.0, v2 = 0.0, v3 = 0.0;
+            int k = 0;
+            for (; k + 3 < i; k += 4) {
+                v0 -= Lm[i + n * k] * M[k + n * j];
+                v1 -= Lm[i + n * (k + 1)] * M[k + 1 + n * j];
+                v2 -= Lm[i + n * (k + 2)] * M[k + 2 + n * j];
+                v3 -= Lm[i + n * (k + 3)] * M[k + 3 + n * j];
+            }
+            for (; k < i; ++k) v0 -= Lm[i + n * k] * M[k + n * j];
+            M[i + n * j] = ((v0 + v1) + (v2 + v3)) / Lm[i + n * i];
         }
     }
+    __syncthreads();
+    for (int i = tid; i < n; i += nt)
+        for (int j = 0; j < n; ++j) {
+            V[i + n * j] = 0.0;
+            if (i < j) {
+                const double s = 0.5 * (M[i + n * j] + M[j + n * i]);
+                M[i + n * j] = s;
+                M[j + n * i] = s;
+            }
+        }
     __syncthreads();
     // Householder tridiagonalisation: reflect rows/cols k+1..n-1 of column k
     for (int k = 0; k < n - 2; ++k) {
@@ -726,9 +778,16 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const double* __restrict__ H
         if (tid == 0) beta[k] = bk;
         __syncthreads();
         for (int i = m0 + tid; i < n; i += nt) {  // p = bk A v
-            double acc = 0.0;
-            for (int j = m0; j < n; ++j) acc += M[i + n * j] * V[j + n * k];
-            y[i] = bk * acc;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int j = m0;
+            for (; j + 3 < n; j += 4) {
+                a0 += M[i + n * j] * V[j + n * k];
+                a1 += M[i + n * (j + 1)] * V[j + 1 + n * k];
+                a2 += M[i + n * (j + 2)] * V[j + 2 + n * k];
+                a3 += M[i + n * (j + 3)] * V[j + 3 + n * k];
+            }
+            for (; j < n; ++j) a0 += M[i + n * j] * V[j + n * k];
+            y[i] = bk * ((a0 + a1) + (a2 + a3));
         }
         __syncthreads();
         double kp = 0.0;
@@ -736,9 +795,9 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const double* __restrict__ H
         const double K = 0.5 * bk * block_sum(kp, red);
         for (int i = m0 + tid; i < n; i += nt) y[i] -= K * V[i + n * k];  // w = p - K v
         __syncthreads();
-        for (int t = tid; t < len * len; t += nt) {  // A -= v w^T + w v^T
-            const int i = m0 + t % len, j = m0 + t / len;
-            M[i + n * j] -= V[i + n * k] * y[j] + y[i] * V[j + n * k];
+        for (int i = m0 + tid; i < n; i += nt) {  // A -= v w^T + w v^T
+            const double vi = V[i + n * k], wi = y[i];
+            for (int j = m0; j < n; ++j) M[i + n * j] -= vi * y[j] + wi * V[j + n * k];
         }
         if (bk != 0.0)
             for (int i = m0 + tid; i < n; i += nt) {
@@ -890,44 +949,46 @@ __global__ void k_ri_pair_adjoint(const double* __restrict__ L, const double* __
     const int A = blockIdx.x * blockDim.x + threadIdx.x;
     const int P = blockIdx.y;
     if (A >= n_aux) return;
-    double acc = 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int Q = 0; Q < np; ++Q) {
         const double m = (P == Q) ? 2.0 * dg[pair_index(P, P)] : dg[pair_index(P, Q)];
-        acc += m * L[(size_t)A + (size_t)n_aux * Q];
+        acc[Q & 3] += m * L[(size_t)A + (size_t)n_aux * Q];
     }
-    dL[(size_t)A + (size_t)n_aux * P] = acc;
+    dL[(size_t)A + (size_t)n_aux * P] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
 // gcn[mu, p] = sum_i SC[mu,i] (ds+ds^T)[i,p] + HC[mu,i] (dh+dh^T)[i,p]
 //            + sum_q f_pq sum_A dL[A,(p,q)] T[A,q,mu],  f_pq = 2 if p == q else 1.
 // one block per AO index mu.
-__global__ void k_orbital_gradient(const double* __restrict__ T, const double* __restrict__ dL,
-                                   const double* __restrict__ SC, const double* __restrict__ HC,
-                                   const double* __restrict__ dsdh, int n_bf, int n, int n_aux, int ng,
-                                   double* __restrict__ gcn) {
-    __shared__ double red[16];
-    __shared__ double part[MAX_ACT];
+__global__ void __launch_bounds__(256) k_orbital_gradient(const double* __restrict__ T, const double* __restrict__ dL,
+                                                          const double* __restrict__ SC, const double* __restrict__ HC,
+                                                          const double* __restrict__ dsdh, int n_bf, int n, int n_aux,
+                                                          int ng, double* __restrict__ gcn) {
+    __shared__ double part[4][MAX_ACT];
     const int mu = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double acc[MAX_ACT];
 #pragma unroll
     for (int p = 0; p < MAX_ACT; ++p) acc[p] = 0.0;
-    for (int A = threadIdx.x; A < n_aux; A += blockDim.x) {
-        for (int q = 0; q < n; ++q) {
-            const double t = T[(size_t)A + (size_t)n_aux * (q + (size_t)n * mu)];
+    // contraction over the contiguous (A, q) plane of T for this mu
+    const double* tmu = T + (size_t)n_aux * n * mu;
+    for (int k = threadIdx.x; k < n_aux * n; k += blockDim.x) {
+        const int q = k / n_aux, A = k - q * n_aux;
+        const double t = tmu[k];
 #pragma unroll
-            for (int p = 0; p < MAX_ACT; ++p)
-                if (p < n) acc[p] += (p == q ? 2.0 : 1.0) * dL[(size_t)A + (size_t)n_aux * pair_index(p, q)] * t;
-        }
+        for (int p = 0; p < MAX_ACT; ++p)
+            if (p < n) acc[p] += (p == q ? 2.0 : 1.0) * dL[(size_t)A + (size_t)n_aux * pair_index(p, q)] * t;
     }
-    for (int p = 0; p < n; ++p) {
-        const double s = block_sum(acc[p], red);
-        if (threadIdx.x == 0) part[p] = s;
+#pragma unroll
+    for (int p = 0; p < MAX_ACT; ++p) {
+        const double s = wave_sum(acc[p]);
+        if (lane == 0 && p < n) part[wave][p] = s;
     }
     __syncthreads();
     const double* ds = dsdh + ng;
     const double* dh = dsdh + ng + n * n;
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
-        double g = part[p];
+        double g = (part[0][p] + part[1][p]) + (part[2][p] + part[3][p]);
         for (int i = 0; i < n; ++i) {
             g += SC[mu + (size_t)n_bf * i] * (ds[i + n * p] + ds[p + n * i]);
             g += HC[mu + (size_t)n_bf * i] * (dh[i + n * p] + dh[p + n * i]);
@@ -957,6 +1018,8 @@ __global__ void k_normalization_pullback(const double* __restrict__ gcn, const d
 // ---------------------------------------------------------------------------------------------
 // LDS of the pair kernels: full np x np kernel, S_act, h_act, per-wave slabs and, for the
 // backward, the adjoint accumulators (full np x np when that still fits, else packed).
+constexpr size_t kPairLdsLimit = 160 * 1024;
+
 bool pairs_full_dg(int n, int ng) {
     const size_t np = (size_t)n * (n + 1) / 2;
     const size_t bytes = (2 * np * np + 4 * (size_t)n * n) * sizeof(double) + 4 * sizeof(WaveSlab);
@@ -964,11 +1027,20 @@ bool pairs_full_dg(int n, int ng) {
     return bytes <= 150 * 1024;
 }
 
-size_t pairs_lds_bytes(int n, int ng, bool backward) {
+size_t pairs_lds_bytes(int n, int ng, bool backward, int waves) {
     const size_t np = (size_t)n * (n + 1) / 2;
     size_t doubles = np * np + 2 * (size_t)n * n;
     if (backward) doubles += (pairs_full_dg(n, ng) ? np * np : (size_t)ng) + 2 * (size_t)n * n;
-    return doubles * sizeof(double) + 4 * sizeof(WaveSlab);
+    return doubles * sizeof(double) + (size_t)waves * sizeof(WaveSlab);
+}
+
+// eight wavefronts per block when the LDS allows it, else four
+int pairs_waves(int n, int ng, bool backward) {
+    return pairs_lds_bytes(n, ng, backward, 8) <= kPairLdsLimit ? 8 : 4;
+}
+
+size_t pairs_lds_bytes(int n, int ng, bool backward) {
+    return pairs_lds_bytes(n, ng, backward, pairs_waves(n, ng, backward));
 }
 
 size_t gep_lds_bytes(int n);
@@ -1003,8 +1075,7 @@ void launch_active_matrices(hipStream_t st, const double* Cn, const double* SC, 
 }
 
 void launch_ri_one_index(hipStream_t st, const double* B, const double* Cn, int n_bf, int n, int n_aux, double* T) {
-    const int tb = 64;
-    hipLaunchKernelGGL(k_ri_one_index, dim3((n_aux + tb - 1) / tb, n_bf), dim3(tb), 0, st, B, Cn, n_bf, n, n_aux, T);
+    hipLaunchKernelGGL(k_ri_one_index, dim3((n_aux + 63) / 64, n_bf), dim3(256), 0, st, B, Cn, n_bf, n, n_aux, T);
 }
 
 void launch_ri_pair_stage(hipStream_t st, const double* T, const double* Cn, int n_bf, int n, int n_aux, double* L,
@@ -1021,14 +1092,15 @@ void launch_pairs(hipStream_t st, bool backward, int n_blocks, const double* G, 
                   int* flags) {
     const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
     const size_t lds = pairs_lds_bytes(n, ng, backward);
+    const int threads = 64 * pairs_waves(n, ng, backward);
     if (!backward)
-        hipLaunchKernelGGL((k_pairs<false, false>), dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask,
+        hipLaunchKernelGGL((k_pairs<false, false>), dim3(n_blocks), dim3(threads), lds, st, G, sact, hact, n, ng, amask,
                            bmask, det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
     else if (pairs_full_dg(n, ng))
-        hipLaunchKernelGGL((k_pairs<true, true>), dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask,
+        hipLaunchKernelGGL((k_pairs<true, true>), dim3(n_blocks), dim3(threads), lds, st, G, sact, hact, n, ng, amask,
                            bmask, det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
     else
-        hipLaunchKernelGGL((k_pairs<true, false>), dim3(n_blocks), dim3(256), lds, st, G, sact, hact, n, ng, amask,
+        hipLaunchKernelGGL((k_pairs<true, false>), dim3(n_blocks), dim3(threads), lds, st, G, sact, hact, n, ng, amask,
                            bmask, det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
 }
 
