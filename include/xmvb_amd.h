@@ -23,8 +23,8 @@
  * src/cli/main.cpp:17-20). The text of the last error of a context is available through
  * vbe_last_error(); errors raised before a context exists go to vbe_global_error().
  *
- * Threading: a context owns one HIP stream; calls on one context must not overlap.
- * Different contexts are independent.
+ * Threading: a problem or pool owns its HIP streams; calls on one problem or pool must not
+ * overlap. Different problems, pools and contexts are independent.
  */
 #ifndef XMVB_AMD_H
 #define XMVB_AMD_H
@@ -67,6 +67,7 @@ typedef enum vbe_eigensolver {
 
 typedef struct vbe_context vbe_context;
 typedef struct vbe_problem vbe_problem;
+typedef struct vbe_pool vbe_pool;
 typedef struct vbe_topology vbe_topology;
 
 typedef struct vbe_dims {
@@ -155,7 +156,8 @@ int vbe_synthetic_problem(uint64_t seed, int32_t n_bf, int32_t n_act, int32_t n_
 /* ---- device engine -------------------------------------------------------------------------- */
 
 /* Create a context on HIP device `device` with an HBM budget in bytes (0 = 90% of free memory,
- * the cuda::Context precedent, src/cuda/memory/context.hpp:66-115). One HIP stream per context. */
+ * the cuda::Context precedent, src/cuda/memory/context.hpp:66-115); every device allocation of
+ * the context's problems and pools is charged to the budget (VBE_ERR_MEMORY when exceeded). */
 int vbe_context_create(int32_t device, uint64_t hbm_budget_bytes, vbe_context** out);
 void vbe_context_destroy(vbe_context* ctx);
 /* Copy the last error of a context into buf (NUL-terminated, truncated to len). */
@@ -203,9 +205,35 @@ int vbe_last_intermediates(vbe_context* ctx, vbe_problem* prob, double* H, doubl
 /* Device time per phase of the last evaluation. */
 int vbe_last_phase_times(const vbe_problem* prob, vbe_phase_times* times);
 
-/* Count of determinant pairs whose overlap block was rank deficient in the last evaluation
- * (handled by the singular-pair path). */
+/* Count of determinant pairs whose overlap block was rank deficient in the last evaluation. */
 int vbe_last_singular_pairs(const vbe_problem* prob, int64_t* count);
+
+/* ---- batched and served evaluation: pools of resident problems ------------------------------
+ * No reference counterpart: the reference evaluates one problem per process. A pool holds
+ * `capacity` resident problem slots sharing one structure topology (same active space and
+ * structure set, e.g. the points of a bond-stretch scan or the problems of one size bucket);
+ * each slot has its own n_bf, n_aux, S, h, B and E_nuc. One evaluation launch processes any set
+ * of slots with one kernel per phase, and two streams let a second submission overlap the first
+ * (continuous batching). Orbital/gradient arrays are the per-slot n_act x n_bf blocks of the
+ * listed slots, concatenated in list order. */
+int vbe_pool_create(vbe_context* ctx, const vbe_topology_view* topo, int32_t n_act, int32_t n_alpha, int32_t n_beta,
+                    int32_t capacity, vbe_pool** out);
+void vbe_pool_destroy(vbe_pool* pool);
+/* Upload (or replace) the static data of one slot; allocates the slot's workspace. */
+int vbe_pool_load(vbe_pool* pool, int32_t slot, int32_t n_bf, int32_t n_aux, const double* S_ao, const double* h_ao,
+                  const double* B_packed, double e_nuc);
+int vbe_pool_device_bytes(const vbe_pool* pool, uint64_t* bytes);
+/* Synchronous batched evaluation with device-resident orbitals / gradients; energies[count] on the host. */
+int vbe_pool_eval_device(vbe_pool* pool, int32_t count, const int32_t* slots, const double* d_C,
+                         int32_t want_gradient, double* energies, double* d_grad);
+/* Asynchronous batched evaluation from host orbitals: returns a ticket at once; at most two
+ * submissions are in flight and a slot may belong to only one of them. */
+int vbe_pool_submit(vbe_pool* pool, int32_t count, const int32_t* slots, const double* C, int32_t want_gradient,
+                    int32_t* ticket);
+/* Wait for a ticket; energies[count] and (if requested at submit) the concatenated gradients. */
+int vbe_pool_wait(vbe_pool* pool, int32_t ticket, double* energies, double* grads);
+/* Device time per phase of the pool's most recently completed launch. */
+int vbe_pool_phase_times(const vbe_pool* pool, vbe_phase_times* times);
 
 #ifdef __cplusplus
 }

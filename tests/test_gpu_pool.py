@@ -1,0 +1,101 @@
+"""Batched pools (many resident problems per launch) and submit/wait serving through the C ABI.
+
+Each slot's energy and gradient must equal the oracle's (|dE| <= 1e-8, grad 1e-7) and must be
+the same evaluated alone and in a mixed batch (tolerance 1e-12: the per-slot arithmetic is the
+same, only the block partition of the adjoint sums can change)."""
+import numpy as np
+import pytest
+
+import oracle
+import xmvb_amd as vb
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = vb.Context(0)
+    yield c
+    c.close()
+
+
+def test_heterogeneous_slots_match_oracle(ctx):
+    topo = vb.Topology.build("COV", 6, 6, 1)
+    dims = [(20, 40), (31, 62), (45, 100), (24, 24)]
+    pool = vb.Pool(ctx, topo, capacity=len(dims))
+    sps = []
+    for s, (n_bf, n_aux) in enumerate(dims):
+        sp = vb.synthetic_problem(100 + s, n_bf, 6, n_aux)
+        pool.load_synthetic(s, sp, e_nuc=0.25 * s)
+        sps.append(sp)
+    print("case heterogeneous pool of 4 slots", flush=True)
+    e, g = pool.evaluate([0, 1, 2, 3], [sp.C for sp in sps])
+    for s, sp in enumerate(sps):
+        ref = oracle.evaluate(sp.n_bf, sp.n_aux, topo, sp.S, sp.h, sp.B, sp.C, e_nuc=0.25 * s)
+        assert abs(e[s] - ref["energy"]) <= 1e-8
+        assert np.abs(g[s] - ref["grad"]).max() <= 1e-7
+    pool.close()
+
+
+def test_alone_and_in_the_mix_agree(ctx):
+    topo = vb.Topology.build("COV", 8, 8, 1)
+    K = 6
+    pool = vb.Pool(ctx, topo, capacity=K)
+    sps = [vb.synthetic_problem(200 + s, 40, 8, 120) for s in range(K)]
+    for s, sp in enumerate(sps):
+        pool.load_synthetic(s, sp)
+    print("case alone vs mix, 6 slots", flush=True)
+    alone = [pool.evaluate([s], [sps[s].C]) for s in range(K)]
+    order = [3, 0, 5, 1, 4, 2]
+    e_mix, g_mix = pool.evaluate(order, [sps[s].C for s in order])
+    for i, s in enumerate(order):
+        assert abs(e_mix[i] - alone[s][0][0]) <= 1e-12
+        assert np.abs(g_mix[i] - alone[s][1][0]).max() <= 1e-12
+    # two submissions in flight on disjoint slots
+    t1 = pool.submit([0, 1, 2], [sps[s].C for s in (0, 1, 2)])
+    t2 = pool.submit([3, 4, 5], [sps[s].C for s in (3, 4, 5)])
+    e2, g2 = pool.wait(t2)
+    e1, g1 = pool.wait(t1)
+    for i, s in enumerate((0, 1, 2)):
+        assert abs(e1[i] - alone[s][0][0]) <= 1e-12
+    for i, s in enumerate((3, 4, 5)):
+        assert abs(e2[i] - alone[s][0][0]) <= 1e-12
+        assert np.abs(g2[i] - alone[s][1][0]).max() <= 1e-12
+    ref = oracle.evaluate(40, 120, topo, sps[4].S, sps[4].h, sps[4].B, sps[4].C)
+    assert abs(e2[1] - ref["energy"]) <= 1e-8
+    pool.close()
+
+
+def test_slot_conflict_is_rejected(ctx):
+    topo = vb.Topology.build("COV", 4, 4, 1)
+    pool = vb.Pool(ctx, topo, capacity=2)
+    sp = vb.synthetic_problem(7, 16, 4, 32)
+    pool.load_synthetic(0, sp)
+    pool.load_synthetic(1, sp)
+    t = pool.submit([0], [sp.C])
+    with pytest.raises(vb.VbeError):
+        pool.submit([0, 1], [sp.C, sp.C])  # slot 0 still pending
+    pool.wait(t)
+    with pytest.raises(vb.VbeError):
+        pool.evaluate([0, 0], [sp.C, sp.C])  # duplicate slot in one launch
+    e, _ = pool.evaluate([1], [sp.C])
+    assert np.isfinite(e[0])
+    pool.close()
+
+
+def test_medium_batch_matches_single_problem(ctx):
+    topo = vb.Topology.build("COV", 10, 10, 1)
+    K = 8
+    pool = vb.Pool(ctx, topo, capacity=K)
+    sps = [vb.synthetic_problem(s, 120, 10, 360) for s in range(K)]
+    for s, sp in enumerate(sps):
+        pool.load_synthetic(s, sp)
+    print("case medium batch of 8", flush=True)
+    e, g = pool.evaluate(list(range(K)), [sp.C for sp in sps])
+    prob = vb.Problem.from_synthetic(ctx, sps[0], topo)
+    e0, g0 = prob.evaluate(sps[0].C)
+    assert abs(e[0] - e0) <= 1e-12 and np.abs(g[0] - g0).max() <= 1e-12
+    ref = oracle.evaluate(120, 360, topo, sps[5].S, sps[5].h, sps[5].B, sps[5].C)
+    assert abs(e[5] - ref["energy"]) <= 1e-8 and np.abs(g[5] - ref["grad"]).max() <= 1e-7
+    prob.close()
+    pool.close()

@@ -1,4 +1,10 @@
-// Device engine: context, resident problem and the evaluation sequence behind the C ABI.
+// Device engine behind the C ABI: contexts, resident problem pools and the evaluation sequence.
+//
+// A pool holds `capacity` resident problem slots that share one structure topology (same active
+// space and structure set); each slot has its own AO dimensions, static data (S, h, B) and
+// workspace, all allocated when the slot is loaded, so evaluations allocate nothing. One
+// evaluation launch processes any subset of slots with one kernel per phase (blockIdx.z = slot).
+// A single problem (vbe_problem) is a pool of one slot.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,6 +20,11 @@
 
 namespace {
 
+using xmvb_amd::FLAG_COUNT;
+using xmvb_amd::LaunchShape;
+using xmvb_amd::SlotDesc;
+using xmvb_amd::TopoDesc;
+
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
@@ -26,58 +37,27 @@ void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-enum Phase {
-    PH_START, PH_PREP, PH_ONE_INDEX, PH_TRANSFORM, PH_PAIR_FWD, PH_STRUCT, PH_EIGEN, PH_BWD_BEGIN, PH_BWD_KERNEL,
-    PH_PAIR_BWD, PH_TAIL, PH_COUNT
-};
+constexpr int kLanes = 2;              // streams of a pool (in-flight submissions)
+constexpr int kPairBlocksFwd = 4096;   // target forward pair blocks per launch (all slots)
+constexpr int kPairBlocksBwd = 2048;   // target backward pair blocks per launch (all slots)
+constexpr int kMaxBwdBlocksPerSlot = 512;
 
 }  // namespace
 
 struct vbe_context {
     int device = 0;
-    hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t in_use = 0;
     std::string error;
 };
 
-struct vbe_problem {
-    vbe_context* ctx = nullptr;
-    vbe_dims dims{};
-    int n_str = 0, n_det = 0, n_astr = 0, n_bstr = 0, np = 0, ng = 0;
-    long long n_terms = 0;
-    double e_nuc = 0.0;
-    int pair_blocks_fwd = 0, pair_blocks_bwd = 0;
-    std::vector<void*> allocations;
-    uint64_t bytes = 0;
-    // static device data
-    double *S = nullptr, *h = nullptr, *B = nullptr;
-    uint64_t *amask = nullptr, *bmask = nullptr;
-    int *det_a = nullptr, *det_b = nullptr;
-    long long* off = nullptr;
-    int* ts = nullptr;
-    double* tc = nullptr;
-    long long* soff = nullptr;
-    int* sdet = nullptr;
-    double* scoef = nullptr;
-    // per-evaluation workspace
-    double *C = nullptr, *grad = nullptr;
-    const double* cur_C = nullptr;  // orbitals of the current evaluation (problem buffer or caller's device array)
-    uint8_t* support = nullptr;
-    double *Cn = nullptr, *nu2 = nullptr, *SC = nullptr, *HC = nullptr, *sact = nullptr, *hact = nullptr;
-    double *T = nullptr, *L = nullptr, *G = nullptr;
-    double *Hdet = nullptr, *Sdet = nullptr, *Yh = nullptr, *Ys = nullptr, *Hs = nullptr, *Ss = nullptr;
-    double *gep_work = nullptr, *coef = nullptr, *energy = nullptr, *dcoef = nullptr;
-    double *partials = nullptr, *dsum = nullptr, *dL = nullptr, *gcn = nullptr;
-    int* flags = nullptr;
-    hipEvent_t ev[PH_COUNT] = {};
-    // state of the last evaluation
-    bool forward_valid = false;
-    const uint8_t* last_support = nullptr;
-    double last_energy = 0.0;
-    int64_t last_singular = 0;
-    vbe_phase_times times{};
+namespace {
 
+// Device allocations charged to a context budget.
+struct DeviceArena {
+    vbe_context* ctx = nullptr;
+    std::vector<void*> ptrs;
+    uint64_t bytes = 0;
     template <typename T>
     T* alloc(size_t count) {
         const size_t nbytes = std::max<size_t>(count, 1) * sizeof(T);
@@ -85,17 +65,88 @@ struct vbe_problem {
             throw StatusError(VBE_ERR_MEMORY, "HBM budget exceeded while allocating problem storage");
         void* p = nullptr;
         check(hipMalloc(&p, nbytes), "hipMalloc");
-        allocations.push_back(p);
+        ptrs.push_back(p);
         bytes += nbytes;
         ctx->in_use += nbytes;
         return static_cast<T*>(p);
     }
-    ~vbe_problem() {
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        for (void* p : allocations) (void)hipFree(p);
+    void release() {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
         if (ctx) ctx->in_use -= std::min(ctx->in_use, bytes);
+        bytes = 0;
     }
+    ~DeviceArena() { release(); }
+};
+
+struct Slot {
+    bool loaded = false;
+    int n_bf = 0, n_aux = 0;
+    double e_nuc = 0.0;
+    std::unique_ptr<DeviceArena> arena;
+    SlotDesc desc{};           // workspace + static pointers; per-launch fields filled at launch
+    double* C_buf = nullptr;   // slot-owned orbitals / gradient / support for host-buffer evaluations
+    double* grad_buf = nullptr;
+    uint8_t* support_buf = nullptr;
+};
+
+// One stream with its descriptor table and compact per-launch outputs.
+struct Lane {
+    hipStream_t stream = nullptr;
+    SlotDesc* d_desc = nullptr;
+    SlotDesc* h_desc = nullptr;  // pinned
+    double* d_energy = nullptr;
+    int* d_flags = nullptr;
+    double* h_energy = nullptr;  // pinned
+    int* h_flags = nullptr;      // pinned
+    double* d_C = nullptr;       // compact orbitals / gradients of submissions
+    double* d_grad = nullptr;
+    double* h_C = nullptr;       // pinned staging
+    double* h_grad = nullptr;
+    size_t stage_doubles = 0;
+    hipEvent_t ev[xmvb_amd::EV_COUNT] = {};
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    bool with_grad = false;
+    bool events_valid = false;
+    int ticket = -1;
+    std::vector<int> slots;
+    std::vector<size_t> grad_offsets;
+};
+
+}  // namespace
+
+struct vbe_pool {
+    vbe_context* ctx = nullptr;
+    int n_act = 0, n_alpha = 0, n_beta = 0, capacity = 0;
+    TopoDesc topo{};
+    DeviceArena topo_arena;
+    std::vector<Slot> slots;
+    Lane lanes[kLanes];
+    int next_ticket = 0;
+    int last_lane = 0;
+    int64_t last_singular = 0;
+    vbe_phase_times times{};
+
+    ~vbe_pool() {
+        for (auto& l : lanes) {
+            if (l.stream) (void)hipStreamSynchronize(l.stream);
+            for (auto& e : l.ev)
+                if (e) (void)hipEventDestroy(e);
+            if (l.done) (void)hipEventDestroy(l.done);
+            if (l.h_desc) (void)hipHostFree(l.h_desc);
+            if (l.h_energy) (void)hipHostFree(l.h_energy);
+            if (l.h_flags) (void)hipHostFree(l.h_flags);
+            if (l.h_C) (void)hipHostFree(l.h_C);
+            if (l.h_grad) (void)hipHostFree(l.h_grad);
+            if (l.stream) (void)hipStreamDestroy(l.stream);
+        }
+    }
+};
+
+struct vbe_problem {
+    std::unique_ptr<vbe_pool> pool;
+    bool forward_valid = false;
 };
 
 namespace {
@@ -121,26 +172,25 @@ int guarded(vbe_context* ctx, F&& body) {
     }
 }
 
-void validate(const vbe_dims& d, const vbe_topology_view& t) {
-    if (d.n_bf <= 0 || d.n_act <= 0 || d.n_aux <= 0 || d.n_alpha < 0 || d.n_beta < 0)
-        throw StatusError(VBE_ERR_ARGUMENT, "dimensions must be positive");
-    if (d.n_act > xmvb_amd::MAX_ACT)
+void validate_topology(int n_act, int n_alpha, int n_beta, const vbe_topology_view& t) {
+    if (n_act <= 0 || n_alpha < 0 || n_beta < 0) throw StatusError(VBE_ERR_ARGUMENT, "dimensions must be positive");
+    if (n_act > xmvb_amd::MAX_ACT)
         throw StatusError(VBE_ERR_UNSUPPORTED, "n_act above " + std::to_string(xmvb_amd::MAX_ACT) +
                                                    " is not supported by the LDS-resident determinant-pair kernels");
-    if (d.n_alpha > xmvb_amd::MAX_SPIN_BLOCK || d.n_beta > xmvb_amd::MAX_SPIN_BLOCK)
+    if (n_alpha > xmvb_amd::MAX_SPIN_BLOCK || n_beta > xmvb_amd::MAX_SPIN_BLOCK)
         throw StatusError(VBE_ERR_UNSUPPORTED, "more than 8 electrons of one spin are not supported");
-    if (d.n_alpha > d.n_act || d.n_beta > d.n_act) throw StatusError(VBE_ERR_ARGUMENT, "more electrons than orbitals");
+    if (n_alpha > n_act || n_beta > n_act) throw StatusError(VBE_ERR_ARGUMENT, "more electrons than orbitals");
     if (t.n_structures <= 0 || t.n_determinants <= 0 || t.n_alpha_strings <= 0 || t.n_beta_strings <= 0)
         throw StatusError(VBE_ERR_ARGUMENT, "empty topology");
     if (!t.alpha_masks || !t.beta_masks || !t.det_alpha || !t.det_beta || !t.term_offsets || !t.term_structure ||
         !t.term_coefficient)
         throw StatusError(VBE_ERR_ARGUMENT, "null topology array");
-    const uint64_t valid = d.n_act == 64 ? ~0ull : ((1ull << d.n_act) - 1);
+    const uint64_t valid = n_act == 64 ? ~0ull : ((1ull << n_act) - 1);
     for (int i = 0; i < t.n_alpha_strings; ++i)
-        if ((t.alpha_masks[i] & ~valid) || __builtin_popcountll(t.alpha_masks[i]) != d.n_alpha)
+        if ((t.alpha_masks[i] & ~valid) || __builtin_popcountll(t.alpha_masks[i]) != n_alpha)
             throw StatusError(VBE_ERR_ARGUMENT, "alpha string inconsistent with n_act/n_alpha");
     for (int i = 0; i < t.n_beta_strings; ++i)
-        if ((t.beta_masks[i] & ~valid) || __builtin_popcountll(t.beta_masks[i]) != d.n_beta)
+        if ((t.beta_masks[i] & ~valid) || __builtin_popcountll(t.beta_masks[i]) != n_beta)
             throw StatusError(VBE_ERR_ARGUMENT, "beta string inconsistent with n_act/n_beta");
     for (int i = 0; i < t.n_determinants; ++i)
         if (t.det_alpha[i] < 0 || t.det_alpha[i] >= t.n_alpha_strings || t.det_beta[i] < 0 ||
@@ -160,71 +210,229 @@ void upload(T* dst, const T* src, size_t count, hipStream_t st) {
     check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
 }
 
-// Reads the device flags and energy with one synchronisation, then raises on any flagged failure.
-void finish(vbe_problem* p) {
-    int flags[xmvb_amd::FLAG_COUNT];
-    double e[2];
-    check(hipMemcpyAsync(flags, p->flags, sizeof(flags), hipMemcpyDeviceToHost, p->ctx->stream), "flags D2H");
-    check(hipMemcpyAsync(e, p->energy, sizeof(e), hipMemcpyDeviceToHost, p->ctx->stream), "energy D2H");
-    check(hipStreamSynchronize(p->ctx->stream), "hipStreamSynchronize");
-    p->last_energy = e[1];
-    p->last_singular = flags[xmvb_amd::FLAG_SINGULAR_PAIRS];
-    if (flags[xmvb_amd::FLAG_NORMALIZATION])
-        throw StatusError(VBE_ERR_NUMERIC, "orbital normalization failed for orbital " +
-                                               std::to_string(flags[xmvb_amd::FLAG_NORMALIZATION] - 1));
-    if (flags[xmvb_amd::FLAG_ACTIVE_OVERLAP])
-        throw StatusError(VBE_ERR_NUMERIC, "active orbital overlap matrix is not positive definite");
-    if (flags[xmvb_amd::FLAG_SINGULAR_PAIRS])
-        throw StatusError(VBE_ERR_UNSUPPORTED,
-                          std::to_string(flags[xmvb_amd::FLAG_SINGULAR_PAIRS]) +
-                              " determinant pairs have rank-deficient overlap blocks; the singular-pair device path "
-                              "is not implemented");
-    if (flags[xmvb_amd::FLAG_METRIC])
-        throw StatusError(VBE_ERR_METRIC, "structure metric is not positive definite");
+long long pair_count(const vbe_pool* p) { return (long long)p->topo.n_det * (p->topo.n_det + 1) / 2; }
+
+int bwd_blocks_cap(const vbe_pool* p) {
+    return (int)std::min<long long>(kMaxBwdBlocksPerSlot, std::max<long long>(1, (pair_count(p) + 7) / 8));
 }
 
-// Forward phases on the context stream: orbitals -> integrals -> pairs -> structures -> eigensolve.
-void enqueue_forward(vbe_problem* p) {
-    hipStream_t st = p->ctx->stream;
-    const auto& d = p->dims;
-    using namespace xmvb_amd;
-    check(hipMemsetAsync(p->flags, 0, sizeof(int) * FLAG_COUNT, st), "hipMemsetAsync flags");
-    check(hipEventRecord(p->ev[PH_START], st), "hipEventRecord");
-    launch_orbital_prep(st, p->cur_C, p->last_support, p->S, p->h, d.n_bf, d.n_act, p->Cn, p->SC, p->HC, p->nu2, p->flags);
-    launch_active_matrices(st, p->Cn, p->SC, p->HC, d.n_bf, d.n_act, p->sact, p->hact, p->flags);
-    check(hipEventRecord(p->ev[PH_PREP], st), "hipEventRecord");
-    launch_ri_one_index(st, p->B, p->Cn, d.n_bf, d.n_act, d.n_aux, p->T);
-    check(hipEventRecord(p->ev[PH_ONE_INDEX], st), "hipEventRecord");
-    launch_ri_pair_stage(st, p->T, p->Cn, d.n_bf, d.n_act, d.n_aux, p->L, p->G);
-    check(hipEventRecord(p->ev[PH_TRANSFORM], st), "hipEventRecord");
-    launch_pairs(st, false, p->pair_blocks_fwd, p->G, p->sact, p->hact, d.n_act, p->ng, p->amask, p->bmask, p->det_a,
-                 p->det_b, p->n_det, p->Hdet, p->Sdet, nullptr, nullptr, nullptr, p->flags);
-    check(hipGetLastError(), "pair forward launch");
-    check(hipEventRecord(p->ev[PH_PAIR_FWD], st), "hipEventRecord");
-    launch_structure(st, p->Hdet, p->Sdet, p->n_det, p->n_str, p->soff, p->sdet, p->scoef, p->Yh, p->Ys, p->Hs, p->Ss);
-    check(hipEventRecord(p->ev[PH_STRUCT], st), "hipEventRecord");
-    launch_gep(st, p->Hs, p->Ss, p->n_str, p->e_nuc, p->gep_work, p->coef, p->energy, p->flags, p->off, p->ts, p->tc,
-               p->n_det, p->dcoef);
-    check(hipGetLastError(), "forward launch");
-    check(hipEventRecord(p->ev[PH_EIGEN], st), "hipEventRecord");
+std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t, int n_act, int n_alpha, int n_beta,
+                                    int capacity) {
+    validate_topology(n_act, n_alpha, n_beta, t);
+    if (capacity <= 0) throw StatusError(VBE_ERR_ARGUMENT, "pool capacity must be positive");
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto p = std::make_unique<vbe_pool>();
+    p->ctx = ctx;
+    p->n_act = n_act;
+    p->n_alpha = n_alpha;
+    p->n_beta = n_beta;
+    p->capacity = capacity;
+    p->topo_arena.ctx = ctx;
+    for (auto& l : p->lanes) {
+        check(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& e : l.ev) check(hipEventCreate(&e), "hipEventCreate");
+        check(hipEventCreateWithFlags(&l.done, hipEventDisableTiming), "hipEventCreate");
+        check(hipHostMalloc(reinterpret_cast<void**>(&l.h_desc), sizeof(SlotDesc) * capacity, hipHostMallocDefault),
+              "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void**>(&l.h_energy), sizeof(double) * 2 * capacity, hipHostMallocDefault),
+              "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void**>(&l.h_flags), sizeof(int) * FLAG_COUNT * capacity,
+                            hipHostMallocDefault),
+              "hipHostMalloc");
+        l.d_desc = p->topo_arena.alloc<SlotDesc>(capacity);
+        l.d_energy = p->topo_arena.alloc<double>(2 * (size_t)capacity);
+        l.d_flags = p->topo_arena.alloc<int>((size_t)FLAG_COUNT * capacity);
+    }
+    hipStream_t st = p->lanes[0].stream;
+    const int nd = t.n_determinants, ns = t.n_structures;
+    const long long n_terms = t.term_offsets[nd];
+    // reverse incidence: structure -> (determinant, coefficient)
+    std::vector<long long> soff(ns + 1, 0);
+    for (long long k = 0; k < n_terms; ++k) ++soff[t.term_structure[k] + 1];
+    for (int s = 0; s < ns; ++s) soff[s + 1] += soff[s];
+    std::vector<int> sdet(n_terms);
+    std::vector<double> scoef(n_terms);
+    {
+        auto cursor = soff;
+        for (int d = 0; d < nd; ++d)
+            for (long long k = t.term_offsets[d]; k < t.term_offsets[d + 1]; ++k) {
+                const long long at = cursor[t.term_structure[k]]++;
+                sdet[at] = d;
+                scoef[at] = t.term_coefficient[k];
+            }
+    }
+    std::vector<long long> off(t.term_offsets, t.term_offsets + nd + 1);
+    auto& A = p->topo_arena;
+    TopoDesc& T = p->topo;
+    T.n = n_act;
+    T.np = n_act * (n_act + 1) / 2;
+    T.ng = T.np * (T.np + 1) / 2;
+    T.n_det = nd;
+    T.n_str = ns;
+    auto* amask = A.alloc<uint64_t>(t.n_alpha_strings);
+    auto* bmask = A.alloc<uint64_t>(t.n_beta_strings);
+    auto* det_a = A.alloc<int>(nd);
+    auto* det_b = A.alloc<int>(nd);
+    auto* d_off = A.alloc<long long>(nd + 1);
+    auto* ts = A.alloc<int>(n_terms);
+    auto* tc = A.alloc<double>(n_terms);
+    auto* d_soff = A.alloc<long long>(ns + 1);
+    auto* d_sdet = A.alloc<int>(n_terms);
+    auto* d_scoef = A.alloc<double>(n_terms);
+    upload(amask, t.alpha_masks, t.n_alpha_strings, st);
+    upload(bmask, t.beta_masks, t.n_beta_strings, st);
+    upload(det_a, t.det_alpha, nd, st);
+    upload(det_b, t.det_beta, nd, st);
+    upload(d_off, off.data(), off.size(), st);
+    upload(ts, t.term_structure, n_terms, st);
+    upload(tc, t.term_coefficient, n_terms, st);
+    upload(d_soff, soff.data(), soff.size(), st);
+    upload(d_sdet, sdet.data(), sdet.size(), st);
+    upload(d_scoef, scoef.data(), scoef.size(), st);
+    T.amask = amask;
+    T.bmask = bmask;
+    T.det_a = det_a;
+    T.det_b = det_b;
+    T.off = d_off;
+    T.ts = ts;
+    T.tc = tc;
+    T.soff = d_soff;
+    T.sdet = d_sdet;
+    T.scoef = d_scoef;
+    check(xmvb_amd::configure_kernels(n_act, T.ng, ns, std::max(n_alpha, n_beta)), "hipFuncSetAttribute");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    p->slots.resize(capacity);
+    return p;
 }
 
-void enqueue_gradient(vbe_problem* p, double* grad_out) {
-    hipStream_t st = p->ctx->stream;
-    const auto& d = p->dims;
-    using namespace xmvb_amd;
-    check(hipEventRecord(p->ev[PH_BWD_BEGIN], st), "hipEventRecord");
-    launch_pairs(st, true, p->pair_blocks_bwd, p->G, p->sact, p->hact, d.n_act, p->ng, p->amask, p->bmask, p->det_a,
-                 p->det_b, p->n_det, nullptr, nullptr, p->dcoef, p->energy, p->partials, p->flags);
-    check(hipGetLastError(), "pair backward launch");
-    check(hipEventRecord(p->ev[PH_BWD_KERNEL], st), "hipEventRecord");
-    const int len = p->ng + 2 * d.n_act * d.n_act;
-    launch_reduce_partials(st, p->partials, p->pair_blocks_bwd, len, p->dsum);
-    check(hipEventRecord(p->ev[PH_PAIR_BWD], st), "hipEventRecord");
-    launch_gradient_tail(st, p->T, p->L, p->dsum, p->SC, p->HC, p->Cn, p->nu2, p->last_support, d.n_bf, d.n_act,
-                         d.n_aux, p->dL, p->gcn, grad_out);
-    check(hipGetLastError(), "gradient launch");
-    check(hipEventRecord(p->ev[PH_TAIL], st), "hipEventRecord");
+void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, const double* h, const double* B,
+               double e_nuc) {
+    if (slot < 0 || slot >= p->capacity) throw StatusError(VBE_ERR_ARGUMENT, "slot index out of range");
+    if (n_bf <= 0 || n_aux <= 0 || !S || !h || !B) throw StatusError(VBE_ERR_ARGUMENT, "invalid slot data");
+    for (auto& l : p->lanes)
+        if (l.busy && std::find(l.slots.begin(), l.slots.end(), slot) != l.slots.end())
+            throw StatusError(VBE_ERR_ARGUMENT, "slot is in use by a pending submission");
+    check(hipSetDevice(p->ctx->device), "hipSetDevice");
+    Slot& s = p->slots[slot];
+    const TopoDesc& T = p->topo;
+    const size_t n = T.n, np = T.np, nd = T.n_det, ns = T.n_str;
+    const size_t nbp = (size_t)n_bf * (n_bf + 1) / 2;
+    const bool realloc = !s.arena || s.n_bf != n_bf || s.n_aux != n_aux;
+    if (realloc) {
+        s.arena.reset();
+        s.arena = std::make_unique<DeviceArena>();
+        s.arena->ctx = p->ctx;
+        auto& A = *s.arena;
+        SlotDesc d{};
+        d.S = A.alloc<double>((size_t)n_bf * n_bf);
+        d.h = A.alloc<double>((size_t)n_bf * n_bf);
+        d.B = A.alloc<double>((size_t)n_aux * nbp);
+        d.Cn = A.alloc<double>(n * n_bf);
+        d.SC = A.alloc<double>(n * n_bf);
+        d.HC = A.alloc<double>(n * n_bf);
+        d.nu2 = A.alloc<double>(n);
+        d.sact = A.alloc<double>(n * n);
+        d.hact = A.alloc<double>(n * n);
+        d.T = A.alloc<double>((size_t)n_aux * n * n_bf);
+        d.L = A.alloc<double>((size_t)n_aux * np);
+        d.G = A.alloc<double>(T.ng);
+        d.Hdet = A.alloc<double>(nd * nd);
+        d.Sdet = A.alloc<double>(nd * nd);
+        d.Yh = A.alloc<double>(nd * ns);
+        d.Ys = A.alloc<double>(nd * ns);
+        d.Hs = A.alloc<double>(ns * ns);
+        d.Ss = A.alloc<double>(ns * ns);
+        d.gep = A.alloc<double>(xmvb_amd::gep_work_doubles((int)ns));
+        d.coef = A.alloc<double>(ns);
+        d.dcoef = A.alloc<double>(nd);
+        const size_t len = (size_t)T.ng + 2 * n * n;
+        d.partials = A.alloc<double>((size_t)bwd_blocks_cap(p) * len);
+        d.dsum = A.alloc<double>(len);
+        d.dL = A.alloc<double>((size_t)n_aux * np);
+        d.gcn = A.alloc<double>(n * n_bf);
+        s.C_buf = A.alloc<double>(n * n_bf);
+        s.grad_buf = A.alloc<double>(n * n_bf);
+        s.support_buf = A.alloc<uint8_t>(n * n_bf);
+        s.desc = d;
+    }
+    s.n_bf = n_bf;
+    s.n_aux = n_aux;
+    s.e_nuc = e_nuc;
+    s.desc.n_bf = n_bf;
+    s.desc.n_aux = n_aux;
+    s.desc.e_nuc = e_nuc;
+    hipStream_t st = p->lanes[0].stream;
+    upload(const_cast<double*>(s.desc.S), S, (size_t)n_bf * n_bf, st);
+    upload(const_cast<double*>(s.desc.h), h, (size_t)n_bf * n_bf, st);
+    upload(const_cast<double*>(s.desc.B), B, (size_t)n_aux * nbp, st);
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    s.loaded = true;
+}
+
+// Per-slot inputs of one launch.
+struct SlotInput {
+    int slot;
+    const double* C;         // device orbitals
+    const uint8_t* support;  // device support mask or null
+    double* grad;            // device gradient output
+};
+
+LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
+    LaunchShape sh{};
+    for (const auto& x : in) {
+        sh.max_n_bf = std::max(sh.max_n_bf, p->slots[x.slot].n_bf);
+        sh.max_n_aux = std::max(sh.max_n_aux, p->slots[x.slot].n_aux);
+    }
+    sh.mmax = std::max(p->n_alpha, p->n_beta);
+    const long long pairs = pair_count(p);
+    const long long count = (long long)in.size();
+    // at least ~4 sweeps of 4 wavefronts' pair groups per forward block
+    const long long per_sweep = 4LL * (64 / xmvb_amd::pair_group(sh.mmax));
+    sh.pair_blocks_fwd = (int)std::clamp<long long>((kPairBlocksFwd + count - 1) / count, 1,
+                                                    std::max<long long>(1, (pairs + 4 * per_sweep - 1) / (4 * per_sweep)));
+    sh.pair_blocks_bwd = (int)std::clamp<long long>((kPairBlocksBwd + count - 1) / count, 1, bwd_blocks_cap(p));
+    return sh;
+}
+
+// Enqueue forward and/or backward for a set of slots on a lane, then the D2H of energies and flags.
+void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool with_grad, bool forward) {
+    const int count = (int)in.size();
+    if (count <= 0 || count > p->capacity) throw StatusError(VBE_ERR_ARGUMENT, "invalid number of slots");
+    hipStream_t st = lane.stream;
+    const LaunchShape sh = shape_for(p, in);
+    if (forward) {
+        std::vector<char> seen(p->capacity, 0);
+        for (int i = 0; i < count; ++i) {
+            const int s = in[i].slot;
+            if (s < 0 || s >= p->capacity || !p->slots[s].loaded)
+                throw StatusError(VBE_ERR_ARGUMENT, "slot not loaded");
+            if (seen[s]) throw StatusError(VBE_ERR_ARGUMENT, "a slot appears twice in one launch");
+            seen[s] = 1;
+            SlotDesc d = p->slots[s].desc;
+            d.C = in[i].C;
+            d.support = in[i].support;
+            d.grad = in[i].grad;
+            d.energy = lane.d_energy + 2 * i;
+            d.flags = lane.d_flags + FLAG_COUNT * i;
+            lane.h_desc[i] = d;
+        }
+        upload(lane.d_desc, lane.h_desc, count, st);
+        check(hipMemsetAsync(lane.d_flags, 0, sizeof(int) * FLAG_COUNT * count, st), "hipMemsetAsync");
+        xmvb_amd::launch_forward(st, lane.d_desc, count, p->topo, sh, lane.ev);
+        check(hipGetLastError(), "forward launch");
+    }
+    if (with_grad) {
+        xmvb_amd::launch_backward(st, lane.d_desc, count, p->topo, sh, lane.ev);
+        check(hipGetLastError(), "backward launch");
+    }
+    check(hipMemcpyAsync(lane.h_energy, lane.d_energy, sizeof(double) * 2 * count, hipMemcpyDeviceToHost, st),
+          "energy D2H");
+    check(hipMemcpyAsync(lane.h_flags, lane.d_flags, sizeof(int) * FLAG_COUNT * count, hipMemcpyDeviceToHost, st),
+          "flags D2H");
+    lane.slots.clear();
+    for (const auto& x : in) lane.slots.push_back(x.slot);
+    lane.with_grad = with_grad;
+    lane.events_valid = forward;
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -233,37 +441,101 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-void record_times(vbe_problem* p, bool with_gradient) {
-    auto& t = p->times;
-    t.orbital_prep_ms = elapsed(p->ev[PH_START], p->ev[PH_PREP]);
-    t.transform_ms = elapsed(p->ev[PH_PREP], p->ev[PH_TRANSFORM]);
-    t.ri_one_index_ms = elapsed(p->ev[PH_PREP], p->ev[PH_ONE_INDEX]);
-    t.pair_forward_ms = elapsed(p->ev[PH_TRANSFORM], p->ev[PH_PAIR_FWD]);
-    t.structure_ms = elapsed(p->ev[PH_PAIR_FWD], p->ev[PH_STRUCT]);
-    t.eigensolve_ms = elapsed(p->ev[PH_STRUCT], p->ev[PH_EIGEN]);
-    t.pair_backward_ms = with_gradient ? elapsed(p->ev[PH_EIGEN], p->ev[PH_PAIR_BWD]) : 0.f;
-    t.pair_backward_kernel_ms = with_gradient ? elapsed(p->ev[PH_BWD_BEGIN], p->ev[PH_BWD_KERNEL]) : 0.f;
-    t.gradient_tail_ms = with_gradient ? elapsed(p->ev[PH_PAIR_BWD], p->ev[PH_TAIL]) : 0.f;
-    t.total_ms = elapsed(p->ev[PH_START], with_gradient ? p->ev[PH_TAIL] : p->ev[PH_EIGEN]);
+// Wait for a lane, record phase times, raise on device flags.
+void finish(vbe_pool* p, Lane& lane) {
+    check(hipStreamSynchronize(lane.stream), "hipStreamSynchronize");
+    lane.busy = false;
+    using namespace xmvb_amd;
+    if (lane.events_valid) {
+        auto& t = p->times;
+        t.orbital_prep_ms = elapsed(lane.ev[EV_START], lane.ev[EV_PREP]);
+        t.transform_ms = elapsed(lane.ev[EV_PREP], lane.ev[EV_TRANSFORM]);
+        t.ri_one_index_ms = elapsed(lane.ev[EV_PREP], lane.ev[EV_ONE_INDEX]);
+        t.pair_forward_ms = elapsed(lane.ev[EV_TRANSFORM], lane.ev[EV_PAIR_FWD]);
+        t.structure_ms = elapsed(lane.ev[EV_PAIR_FWD], lane.ev[EV_STRUCT]);
+        t.eigensolve_ms = elapsed(lane.ev[EV_STRUCT], lane.ev[EV_EIGEN]);
+        t.pair_backward_ms = lane.with_grad ? elapsed(lane.ev[EV_EIGEN], lane.ev[EV_PAIR_BWD]) : 0.f;
+        t.pair_backward_kernel_ms = lane.with_grad ? elapsed(lane.ev[EV_BWD_BEGIN], lane.ev[EV_BWD_KERNEL]) : 0.f;
+        t.gradient_tail_ms = lane.with_grad ? elapsed(lane.ev[EV_PAIR_BWD], lane.ev[EV_TAIL]) : 0.f;
+        t.total_ms = elapsed(lane.ev[EV_START], lane.with_grad ? lane.ev[EV_TAIL] : lane.ev[EV_EIGEN]);
+    }
+    int64_t singular = 0;
+    std::string err;
+    int status = VBE_OK;
+    for (size_t i = 0; i < lane.slots.size(); ++i) {
+        const int* f = lane.h_flags + FLAG_COUNT * i;
+        singular += f[FLAG_SINGULAR_PAIRS];
+        if (status != VBE_OK) continue;
+        const std::string where = lane.slots.size() > 1 ? " (slot " + std::to_string(lane.slots[i]) + ")" : "";
+        if (f[FLAG_NORMALIZATION]) {
+            status = VBE_ERR_NUMERIC;
+            err = "orbital normalization failed for orbital " + std::to_string(f[FLAG_NORMALIZATION] - 1) + where;
+        } else if (f[FLAG_ACTIVE_OVERLAP]) {
+            status = VBE_ERR_NUMERIC;
+            err = "active orbital overlap matrix is not positive definite" + where;
+        } else if (f[FLAG_SINGULAR_PAIRS]) {
+            status = VBE_ERR_UNSUPPORTED;
+            err = std::to_string(f[FLAG_SINGULAR_PAIRS]) +
+                  " determinant pairs have rank-deficient overlap blocks; the singular-pair device path is not "
+                  "implemented" +
+                  where;
+        } else if (f[FLAG_METRIC]) {
+            status = VBE_ERR_METRIC;
+            err = "structure metric is not positive definite" + where;
+        }
+    }
+    p->last_singular = singular;
+    if (status != VBE_OK) throw StatusError(status, err);
 }
 
-int eval_core(vbe_context* ctx, vbe_problem* p, const double* d_C, const uint8_t* d_support, bool want_grad,
-              double* E_total, double* d_grad) {
-    if (!ctx || !p || p->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
-    check(hipSetDevice(ctx->device), "hipSetDevice");
-    p->cur_C = d_C;
-    p->last_support = d_support;
-    p->forward_valid = false;
-    // everything is enqueued back to back; a failure flagged by the forward (singular pairs,
-    // indefinite metric) makes the backward's output meaningless but never out of bounds, and
-    // is reported before any result is returned
-    enqueue_forward(p);
-    if (want_grad) enqueue_gradient(p, d_grad ? d_grad : p->grad);
-    finish(p);
-    record_times(p, want_grad);
-    p->forward_valid = true;
-    if (E_total) *E_total = p->last_energy;
-    return VBE_OK;
+Lane& acquire_lane(vbe_pool* p) {
+    for (int k = 0; k < kLanes; ++k) {
+        Lane& l = p->lanes[(p->last_lane + 1 + k) % kLanes];
+        if (!l.busy) return l;
+    }
+    throw StatusError(VBE_ERR_ARGUMENT, "both submission lanes are busy; wait for a ticket first");
+}
+
+void ensure_stage(vbe_pool* p, Lane& l, size_t doubles) {
+    if (l.stage_doubles >= doubles) return;
+    if (l.h_C) (void)hipHostFree(l.h_C);
+    if (l.h_grad) (void)hipHostFree(l.h_grad);
+    l.h_C = l.h_grad = nullptr;
+    check(hipHostMalloc(reinterpret_cast<void**>(&l.h_C), sizeof(double) * doubles, hipHostMallocDefault),
+          "hipHostMalloc");
+    check(hipHostMalloc(reinterpret_cast<void**>(&l.h_grad), sizeof(double) * doubles, hipHostMallocDefault),
+          "hipHostMalloc");
+    // compact device buffers live as long as the pool
+    l.d_C = p->topo_arena.alloc<double>(doubles);
+    l.d_grad = p->topo_arena.alloc<double>(doubles);
+    l.stage_doubles = doubles;
+}
+
+void check_flags_args(const vbe_flags& flags) {
+    if (flags.precision != VBE_PRECISION_FP64)
+        throw StatusError(VBE_ERR_UNSUPPORTED, "only the FP64 precision path is implemented");
+    if (flags.eigensolver != VBE_EIGENSOLVER_DENSE)
+        throw StatusError(VBE_ERR_UNSUPPORTED, "only EIGENSOLVER=DENSE is implemented");
+}
+
+// Evaluate slot 0 of a single problem synchronously; C/support/grad are device pointers.
+double eval_problem(vbe_problem* prob, const double* d_C, const uint8_t* d_support, bool want_grad, double* d_grad,
+                    bool forward) {
+    vbe_pool* p = prob->pool.get();
+    check(hipSetDevice(p->ctx->device), "hipSetDevice");
+    Lane& lane = p->lanes[0];
+    std::vector<SlotInput> in{{0, d_C, d_support, d_grad}};
+    if (forward) prob->forward_valid = false;
+    lane.busy = true;
+    try {
+        enqueue(p, lane, in, want_grad, forward);
+    } catch (...) {
+        lane.busy = false;
+        throw;
+    }
+    finish(p, lane);
+    prob->forward_valid = true;
+    return lane.h_energy[1];
 }
 
 }  // namespace
@@ -279,7 +551,6 @@ int vbe_context_create(int32_t device, uint64_t hbm_budget_bytes, vbe_context** 
         auto ctx = std::make_unique<vbe_context>();
         ctx->device = device;
         check(hipSetDevice(device), "hipSetDevice");
-        check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         if (hbm_budget_bytes == 0) {
             size_t free_b = 0, total_b = 0;
             check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
@@ -292,166 +563,193 @@ int vbe_context_create(int32_t device, uint64_t hbm_budget_bytes, vbe_context** 
     });
 }
 
-void vbe_context_destroy(vbe_context* ctx) {
-    if (!ctx) return;
-    if (ctx->stream) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipStreamDestroy(ctx->stream);
-    }
-    delete ctx;
-}
+void vbe_context_destroy(vbe_context* ctx) { delete ctx; }
 
 int vbe_last_error(const vbe_context* ctx, char* buf, size_t len) {
     if (!buf || len == 0) return VBE_ERR_ARGUMENT;
-    const std::string& e = ctx ? ctx->error : std::string(vbe_global_error());
+    const std::string e = ctx ? ctx->error : std::string(vbe_global_error());
     const size_t n = std::min(len - 1, e.size());
     std::memcpy(buf, e.data(), n);
     buf[n] = '\0';
     return VBE_OK;
 }
 
+// ---- pools ----------------------------------------------------------------------------------
+
+int vbe_pool_create(vbe_context* ctx, const vbe_topology_view* topo, int32_t n_act, int32_t n_alpha, int32_t n_beta,
+                    int32_t capacity, vbe_pool** out) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !topo || !out) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        *out = make_pool(ctx, *topo, n_act, n_alpha, n_beta, capacity).release();
+        return VBE_OK;
+    });
+}
+
+void vbe_pool_destroy(vbe_pool* pool) { delete pool; }
+
+int vbe_pool_load(vbe_pool* pool, int32_t slot, int32_t n_bf, int32_t n_aux, const double* S_ao, const double* h_ao,
+                  const double* B_packed, double e_nuc) {
+    return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
+        if (!pool) throw StatusError(VBE_ERR_ARGUMENT, "null pool");
+        load_slot(pool, slot, n_bf, n_aux, S_ao, h_ao, B_packed, e_nuc);
+        return VBE_OK;
+    });
+}
+
+int vbe_pool_device_bytes(const vbe_pool* pool, uint64_t* bytes) {
+    if (!pool || !bytes) return VBE_ERR_ARGUMENT;
+    uint64_t b = pool->topo_arena.bytes;
+    for (const auto& s : pool->slots)
+        if (s.arena) b += s.arena->bytes;
+    *bytes = b;
+    return VBE_OK;
+}
+
+int vbe_pool_eval_device(vbe_pool* pool, int32_t count, const int32_t* slots, const double* d_C,
+                         int32_t want_gradient, double* energies, double* d_grad) {
+    return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
+        if (!pool || !slots || !d_C || !energies || count <= 0) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        if (want_gradient && !d_grad) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
+        check(hipSetDevice(pool->ctx->device), "hipSetDevice");
+        Lane& lane = acquire_lane(pool);
+        std::vector<SlotInput> in;
+        size_t off = 0;
+        for (int i = 0; i < count; ++i) {
+            const int s = slots[i];
+            if (s < 0 || s >= pool->capacity || !pool->slots[s].loaded)
+                throw StatusError(VBE_ERR_ARGUMENT, "slot not loaded");
+            in.push_back({s, d_C + off, nullptr, want_gradient ? d_grad + off : nullptr});
+            off += (size_t)pool->n_act * pool->slots[s].n_bf;
+        }
+        lane.busy = true;
+        try {
+            enqueue(pool, lane, in, want_gradient != 0, true);
+        } catch (...) {
+            lane.busy = false;
+            throw;
+        }
+        pool->last_lane = int(&lane - pool->lanes);
+        finish(pool, lane);
+        for (int i = 0; i < count; ++i) energies[i] = lane.h_energy[2 * i + 1];
+        return VBE_OK;
+    });
+}
+
+int vbe_pool_submit(vbe_pool* pool, int32_t count, const int32_t* slots, const double* C, int32_t want_gradient,
+                    int32_t* ticket) {
+    return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
+        if (!pool || !slots || !C || !ticket || count <= 0) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        if (count > pool->capacity) throw StatusError(VBE_ERR_ARGUMENT, "more slots than the pool capacity");
+        check(hipSetDevice(pool->ctx->device), "hipSetDevice");
+        Lane& lane = acquire_lane(pool);
+        for (const auto& other : pool->lanes)
+            if (other.busy)
+                for (int i = 0; i < count; ++i)
+                    if (std::find(other.slots.begin(), other.slots.end(), slots[i]) != other.slots.end())
+                        throw StatusError(VBE_ERR_ARGUMENT, "slot is in use by a pending submission");
+        size_t total = 0;
+        for (int i = 0; i < count; ++i) {
+            if (slots[i] < 0 || slots[i] >= pool->capacity || !pool->slots[slots[i]].loaded)
+                throw StatusError(VBE_ERR_ARGUMENT, "slot not loaded");
+            total += (size_t)pool->n_act * pool->slots[slots[i]].n_bf;
+        }
+        ensure_stage(pool, lane, total);
+        std::memcpy(lane.h_C, C, total * sizeof(double));
+        upload(lane.d_C, lane.h_C, total, lane.stream);
+        std::vector<SlotInput> in;
+        lane.grad_offsets.clear();
+        size_t off = 0;
+        for (int i = 0; i < count; ++i) {
+            in.push_back({slots[i], lane.d_C + off, nullptr, want_gradient ? lane.d_grad + off : nullptr});
+            lane.grad_offsets.push_back(off);
+            off += (size_t)pool->n_act * pool->slots[slots[i]].n_bf;
+        }
+        lane.grad_offsets.push_back(total);
+        lane.busy = true;
+        try {
+            enqueue(pool, lane, in, want_gradient != 0, true);
+            if (want_gradient)
+                check(hipMemcpyAsync(lane.h_grad, lane.d_grad, total * sizeof(double), hipMemcpyDeviceToHost,
+                                     lane.stream),
+                      "gradient D2H");
+            check(hipEventRecord(lane.done, lane.stream), "hipEventRecord");
+        } catch (...) {
+            (void)hipStreamSynchronize(lane.stream);
+            lane.busy = false;
+            throw;
+        }
+        lane.ticket = pool->next_ticket++;
+        pool->last_lane = int(&lane - pool->lanes);
+        *ticket = lane.ticket;
+        return VBE_OK;
+    });
+}
+
+int vbe_pool_wait(vbe_pool* pool, int32_t ticket, double* energies, double* grads) {
+    return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
+        if (!pool || !energies) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        for (auto& lane : pool->lanes) {
+            if (!lane.busy || lane.ticket != ticket) continue;
+            const size_t count = lane.slots.size();
+            const bool g = lane.with_grad;
+            finish(pool, lane);
+            for (size_t i = 0; i < count; ++i) energies[i] = lane.h_energy[2 * i + 1];
+            if (g && grads) std::memcpy(grads, lane.h_grad, lane.grad_offsets.back() * sizeof(double));
+            return VBE_OK;
+        }
+        throw StatusError(VBE_ERR_ARGUMENT, "unknown or already completed ticket");
+    });
+}
+
+int vbe_pool_phase_times(const vbe_pool* pool, vbe_phase_times* times) {
+    if (!pool || !times) return VBE_ERR_ARGUMENT;
+    *times = pool->times;
+    return VBE_OK;
+}
+
+// ---- single problems (a pool of one slot) -------------------------------------------------
+
 int vbe_problem_upload(vbe_context* ctx, const vbe_dims* dims, const double* S_ao, const double* h_ao,
                        const double* B_packed, const vbe_topology_view* topo, double e_nuc, vbe_problem** out) {
     return guarded(ctx, [&]() -> int {
         if (!ctx || !dims || !S_ao || !h_ao || !B_packed || !topo || !out)
             throw StatusError(VBE_ERR_ARGUMENT, "null argument");
-        validate(*dims, *topo);
-        check(hipSetDevice(ctx->device), "hipSetDevice");
-        auto p = std::make_unique<vbe_problem>();
-        p->ctx = ctx;
-        p->dims = *dims;
-        p->e_nuc = e_nuc;
-        p->n_str = topo->n_structures;
-        p->n_det = topo->n_determinants;
-        p->n_astr = topo->n_alpha_strings;
-        p->n_bstr = topo->n_beta_strings;
-        p->n_terms = topo->term_offsets[topo->n_determinants];
-        const int n = dims->n_act, nbf = dims->n_bf, naux = dims->n_aux;
-        p->np = n * (n + 1) / 2;
-        p->ng = p->np * (p->np + 1) / 2;
-        const size_t nbp = (size_t)nbf * (nbf + 1) / 2;
-        hipStream_t st = ctx->stream;
-        // reverse incidence: structure -> (determinant, coefficient)
-        std::vector<long long> soff(p->n_str + 1, 0);
-        for (long long k = 0; k < p->n_terms; ++k) ++soff[topo->term_structure[k] + 1];
-        for (int s = 0; s < p->n_str; ++s) soff[s + 1] += soff[s];
-        std::vector<int> sdet(p->n_terms);
-        std::vector<double> scoef(p->n_terms);
-        {
-            auto cursor = soff;
-            for (int dd = 0; dd < p->n_det; ++dd)
-                for (long long k = topo->term_offsets[dd]; k < topo->term_offsets[dd + 1]; ++k) {
-                    const long long at = cursor[topo->term_structure[k]]++;
-                    sdet[at] = dd;
-                    scoef[at] = topo->term_coefficient[k];
-                }
-        }
-        std::vector<long long> off(topo->term_offsets, topo->term_offsets + p->n_det + 1);
-        p->S = p->alloc<double>((size_t)nbf * nbf);
-        p->h = p->alloc<double>((size_t)nbf * nbf);
-        p->B = p->alloc<double>((size_t)naux * nbp);
-        p->amask = p->alloc<uint64_t>(p->n_astr);
-        p->bmask = p->alloc<uint64_t>(p->n_bstr);
-        p->det_a = p->alloc<int>(p->n_det);
-        p->det_b = p->alloc<int>(p->n_det);
-        p->off = p->alloc<long long>(p->n_det + 1);
-        p->ts = p->alloc<int>(p->n_terms);
-        p->tc = p->alloc<double>(p->n_terms);
-        p->soff = p->alloc<long long>(p->n_str + 1);
-        p->sdet = p->alloc<int>(p->n_terms);
-        p->scoef = p->alloc<double>(p->n_terms);
-        upload(p->S, S_ao, (size_t)nbf * nbf, st);
-        upload(p->h, h_ao, (size_t)nbf * nbf, st);
-        upload(p->B, B_packed, (size_t)naux * nbp, st);
-        upload(p->amask, topo->alpha_masks, p->n_astr, st);
-        upload(p->bmask, topo->beta_masks, p->n_bstr, st);
-        upload(p->det_a, topo->det_alpha, p->n_det, st);
-        upload(p->det_b, topo->det_beta, p->n_det, st);
-        upload(p->off, off.data(), off.size(), st);
-        upload(p->ts, topo->term_structure, p->n_terms, st);
-        upload(p->tc, topo->term_coefficient, p->n_terms, st);
-        upload(p->soff, soff.data(), soff.size(), st);
-        upload(p->sdet, sdet.data(), sdet.size(), st);
-        upload(p->scoef, scoef.data(), scoef.size(), st);
-        // workspace
-        const size_t nd = p->n_det, ns = p->n_str;
-        const long long n_pairs = (long long)nd * (nd + 1) / 2;
-        // forward: about two pairs per wavefront; backward: fewer blocks so that the per-block
-        // adjoint partials stay small
-        p->pair_blocks_fwd = (int)std::min<long long>(std::max<long long>(1, (n_pairs + 15) / 16), 4096);
-        p->pair_blocks_bwd = (int)std::min<long long>(std::max<long long>(1, (n_pairs + 7) / 8), 512);
-        p->C = p->alloc<double>((size_t)n * nbf);
-        p->grad = p->alloc<double>((size_t)n * nbf);
-        p->Cn = p->alloc<double>((size_t)n * nbf);
-        p->nu2 = p->alloc<double>(n);
-        p->SC = p->alloc<double>((size_t)n * nbf);
-        p->HC = p->alloc<double>((size_t)n * nbf);
-        p->sact = p->alloc<double>((size_t)n * n);
-        p->hact = p->alloc<double>((size_t)n * n);
-        p->T = p->alloc<double>((size_t)naux * n * nbf);
-        p->L = p->alloc<double>((size_t)naux * p->np);
-        p->G = p->alloc<double>(p->ng);
-        p->Hdet = p->alloc<double>(nd * nd);
-        p->Sdet = p->alloc<double>(nd * nd);
-        p->Yh = p->alloc<double>(nd * ns);
-        p->Ys = p->alloc<double>(nd * ns);
-        p->Hs = p->alloc<double>(ns * ns);
-        p->Ss = p->alloc<double>(ns * ns);
-        p->gep_work = p->alloc<double>(xmvb_amd::gep_work_doubles((int)ns));
-        p->coef = p->alloc<double>(ns);
-        p->energy = p->alloc<double>(2);
-        p->dcoef = p->alloc<double>(nd);
-        const size_t len = (size_t)p->ng + 2 * (size_t)n * n;
-        p->partials = p->alloc<double>((size_t)p->pair_blocks_bwd * len);
-        p->dsum = p->alloc<double>(len);
-        p->dL = p->alloc<double>((size_t)naux * p->np);
-        p->gcn = p->alloc<double>((size_t)n * nbf);
-        p->flags = p->alloc<int>(xmvb_amd::FLAG_COUNT);
-        for (auto& e : p->ev) check(hipEventCreate(&e), "hipEventCreate");
-        check(xmvb_amd::configure_kernels(n, p->ng, p->n_str), "hipFuncSetAttribute");
-        check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        *out = p.release();
+        if (dims->n_bf <= 0 || dims->n_aux <= 0) throw StatusError(VBE_ERR_ARGUMENT, "dimensions must be positive");
+        auto prob = std::make_unique<vbe_problem>();
+        prob->pool = make_pool(ctx, *topo, dims->n_act, dims->n_alpha, dims->n_beta, 1);
+        load_slot(prob->pool.get(), 0, dims->n_bf, dims->n_aux, S_ao, h_ao, B_packed, e_nuc);
+        *out = prob.release();
         return VBE_OK;
     });
 }
 
-void vbe_problem_release(vbe_problem* prob) {
-    if (!prob) return;
-    if (prob->ctx && prob->ctx->stream) (void)hipStreamSynchronize(prob->ctx->stream);
-    delete prob;
-}
+void vbe_problem_release(vbe_problem* prob) { delete prob; }
 
 int vbe_problem_device_bytes(const vbe_problem* prob, uint64_t* bytes) {
     if (!prob || !bytes) return VBE_ERR_ARGUMENT;
-    *bytes = prob->bytes;
-    return VBE_OK;
+    return vbe_pool_device_bytes(prob->pool.get(), bytes);
 }
 
 int vbe_eval(vbe_context* ctx, vbe_problem* prob, const double* C_raw, const uint8_t* support, vbe_flags flags,
              double* E_total, double* E_states, double* grad_raw) {
     return guarded(ctx, [&]() -> int {
         if (!ctx || !prob || !C_raw || !E_total) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
-        if (flags.precision != VBE_PRECISION_FP64)
-            throw StatusError(VBE_ERR_UNSUPPORTED, "only the FP64 precision path is implemented");
-        if (flags.eigensolver != VBE_EIGENSOLVER_DENSE)
-            throw StatusError(VBE_ERR_UNSUPPORTED, "only EIGENSOLVER=DENSE is implemented");
+        if (prob->pool->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
+        check_flags_args(flags);
         if (flags.want_gradient && !grad_raw) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
-        const size_t nc = (size_t)prob->dims.n_act * prob->dims.n_bf;
+        vbe_pool* p = prob->pool.get();
+        Slot& s = p->slots[0];
+        const size_t nc = (size_t)p->n_act * s.n_bf;
+        hipStream_t st = p->lanes[0].stream;
         check(hipSetDevice(ctx->device), "hipSetDevice");
-        upload(prob->C, C_raw, nc, ctx->stream);
-        const uint8_t* dsup = nullptr;
-        if (support) {
-            if (!prob->support) prob->support = prob->alloc<uint8_t>(nc);
-            upload(prob->support, support, nc, ctx->stream);
-            dsup = prob->support;
-        }
-        const int rc = eval_core(ctx, prob, prob->C, dsup, flags.want_gradient != 0, E_total, prob->grad);
-        if (rc != VBE_OK) return rc;
+        upload(s.C_buf, C_raw, nc, st);
+        if (support) upload(s.support_buf, support, nc, st);
+        *E_total = eval_problem(prob, s.C_buf, support ? s.support_buf : nullptr, flags.want_gradient != 0,
+                                s.grad_buf, true);
         if (E_states) E_states[0] = *E_total;
         if (flags.want_gradient) {
-            check(hipMemcpyAsync(grad_raw, prob->grad, nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
-                  "gradient D2H");
-            check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+            check(hipMemcpyAsync(grad_raw, s.grad_buf, nc * sizeof(double), hipMemcpyDeviceToHost, st), "gradient D2H");
+            check(hipStreamSynchronize(st), "hipStreamSynchronize");
         }
         return VBE_OK;
     });
@@ -461,12 +759,11 @@ int vbe_eval_device(vbe_context* ctx, vbe_problem* prob, const double* d_C_raw, 
                     vbe_flags flags, double* E_total, double* d_grad_raw) {
     return guarded(ctx, [&]() -> int {
         if (!ctx || !prob || !d_C_raw || !E_total) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
-        if (flags.precision != VBE_PRECISION_FP64)
-            throw StatusError(VBE_ERR_UNSUPPORTED, "only the FP64 precision path is implemented");
-        if (flags.eigensolver != VBE_EIGENSOLVER_DENSE)
-            throw StatusError(VBE_ERR_UNSUPPORTED, "only EIGENSOLVER=DENSE is implemented");
+        if (prob->pool->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
+        check_flags_args(flags);
         if (flags.want_gradient && !d_grad_raw) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
-        return eval_core(ctx, prob, d_C_raw, d_support, flags.want_gradient != 0, E_total, d_grad_raw);
+        *E_total = eval_problem(prob, d_C_raw, d_support, flags.want_gradient != 0, d_grad_raw, true);
+        return VBE_OK;
     });
 }
 
@@ -480,13 +777,18 @@ int vbe_complete_gradient(vbe_context* ctx, vbe_problem* prob, double* grad_raw)
     return guarded(ctx, [&]() -> int {
         if (!ctx || !prob || !grad_raw) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
         if (!prob->forward_valid) throw StatusError(VBE_ERR_ARGUMENT, "no retained forward evaluation");
-        check(hipSetDevice(ctx->device), "hipSetDevice");
-        enqueue_gradient(prob, prob->grad);
-        finish(prob);
-        const size_t nc = (size_t)prob->dims.n_act * prob->dims.n_bf;
-        check(hipMemcpyAsync(grad_raw, prob->grad, nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
-              "gradient D2H");
-        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        vbe_pool* p = prob->pool.get();
+        Slot& s = p->slots[0];
+        // the retained descriptor table on the device still describes this slot's forward; the
+        // backward writes the gradient through its grad pointer, which is the slot buffer for
+        // host-buffer evaluations
+        p->lanes[0].h_desc[0].grad = s.grad_buf;
+        upload(p->lanes[0].d_desc, p->lanes[0].h_desc, 1, p->lanes[0].stream);
+        eval_problem(prob, s.C_buf, p->lanes[0].h_desc[0].support, true, s.grad_buf, false);
+        const size_t nc = (size_t)p->n_act * s.n_bf;
+        hipStream_t st = p->lanes[0].stream;
+        check(hipMemcpyAsync(grad_raw, s.grad_buf, nc * sizeof(double), hipMemcpyDeviceToHost, st), "gradient D2H");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
         return VBE_OK;
     });
 }
@@ -496,17 +798,19 @@ int vbe_last_intermediates(vbe_context* ctx, vbe_problem* prob, double* H, doubl
     return guarded(ctx, [&]() -> int {
         if (!ctx || !prob) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
         if (!prob->forward_valid) throw StatusError(VBE_ERR_ARGUMENT, "no retained forward evaluation");
-        hipStream_t st = ctx->stream;
-        const size_t ns = prob->n_str, n = prob->dims.n_act;
+        vbe_pool* p = prob->pool.get();
+        const SlotDesc& d = p->slots[0].desc;
+        hipStream_t st = p->lanes[0].stream;
+        const size_t ns = p->topo.n_str, n = p->n_act;
         auto get = [&](double* dst, const double* src, size_t count) {
             if (dst) check(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
         };
-        get(H, prob->Hs, ns * ns);
-        get(S, prob->Ss, ns * ns);
-        get(coef, prob->coef, ns);
-        get(S_act, prob->sact, n * n);
-        get(h_act, prob->hact, n * n);
-        get(g_packed, prob->G, prob->ng);
+        get(H, d.Hs, ns * ns);
+        get(S, d.Ss, ns * ns);
+        get(coef, d.coef, ns);
+        get(S_act, d.sact, n * n);
+        get(h_act, d.hact, n * n);
+        get(g_packed, d.G, p->topo.ng);
         check(hipStreamSynchronize(st), "hipStreamSynchronize");
         return VBE_OK;
     });
@@ -514,13 +818,13 @@ int vbe_last_intermediates(vbe_context* ctx, vbe_problem* prob, double* H, doubl
 
 int vbe_last_phase_times(const vbe_problem* prob, vbe_phase_times* times) {
     if (!prob || !times) return VBE_ERR_ARGUMENT;
-    *times = prob->times;
+    *times = prob->pool->times;
     return VBE_OK;
 }
 
 int vbe_last_singular_pairs(const vbe_problem* prob, int64_t* count) {
     if (!prob || !count) return VBE_ERR_ARGUMENT;
-    *count = prob->last_singular;
+    *count = prob->pool->last_singular;
     return VBE_OK;
 }
 

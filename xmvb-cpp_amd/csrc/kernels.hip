@@ -82,10 +82,19 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // ---------------------------------------------------------------------------------------------
 // orbital preparation: one block per orbital
 // ---------------------------------------------------------------------------------------------
-__global__ void k_orbital_prep(const double* __restrict__ C, const uint8_t* __restrict__ support,
-                               const double* __restrict__ S, const double* __restrict__ h, int n_bf, int n_act,
-                               double* __restrict__ Cn, double* __restrict__ SC, double* __restrict__ HC,
-                               double* __restrict__ nu2, int* __restrict__ flags) {
+__global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ C = sd.C;
+    const uint8_t* __restrict__ support = sd.support;
+    const double* __restrict__ S = sd.S;
+    const double* __restrict__ h = sd.h;
+    const int n_bf = sd.n_bf;
+    double* __restrict__ Cn = sd.Cn;
+    double* __restrict__ SC = sd.SC;
+    double* __restrict__ HC = sd.HC;
+    double* __restrict__ nu2 = sd.nu2;
+    int* __restrict__ flags = sd.flags;
+    (void)n_act;
     extern __shared__ double lds[];
     double* c = lds;             // n_bf
     double* red = lds + n_bf;    // 16
@@ -129,9 +138,15 @@ __global__ void k_orbital_prep(const double* __restrict__ C, const uint8_t* __re
 }
 
 // S_act and raw h_act (n x n), symmetrised; thread 0 checks that S_act is positive definite.
-__global__ void k_active_matrices(const double* __restrict__ Cn, const double* __restrict__ SC,
-                                  const double* __restrict__ HC, int n_bf, int n, double* __restrict__ sact,
-                                  double* __restrict__ hact, int* __restrict__ flags) {
+__global__ void k_active_matrices(const SlotDesc* __restrict__ slots, int n) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ Cn = sd.Cn;
+    const double* __restrict__ SC = sd.SC;
+    const double* __restrict__ HC = sd.HC;
+    const int n_bf = sd.n_bf;
+    double* __restrict__ sact = sd.sact;
+    double* __restrict__ hact = sd.hact;
+    int* __restrict__ flags = sd.flags;
     __shared__ double s_l[MAX_ACT * MAX_ACT], h_l[MAX_ACT * MAX_ACT];
     {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -189,8 +204,13 @@ __global__ void k_active_matrices(const double* __restrict__ Cn, const double* _
 // T[A, q, mu] = sum_nu B[A, (mu nu)] Cn[nu, q]; T index A + n_aux*(q + n*mu).
 // grid (ceil(n_aux/64), n_bf), four wavefronts per block splitting nu; lanes run over A so every
 // packed B column is read as contiguous 512-byte segments (each column is read by its two AO rows).
-__global__ void __launch_bounds__(256) k_ri_one_index(const double* __restrict__ B, const double* __restrict__ Cn,
-                                                      int n_bf, int n, int n_aux, double* __restrict__ T) {
+__global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict__ slots, int n) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n_bf = sd.n_bf, n_aux = sd.n_aux;
+    if ((int)blockIdx.y >= n_bf || (int)blockIdx.x * 64 >= n_aux) return;  // block-uniform
+    const double* __restrict__ B = sd.B;
+    const double* __restrict__ Cn = sd.Cn;
+    double* __restrict__ T = sd.T;
     __shared__ double part[4][MAX_ACT][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int A = blockIdx.x * 64 + lane;
@@ -216,8 +236,12 @@ __global__ void __launch_bounds__(256) k_ri_one_index(const double* __restrict__
 
 // L[A, (p,q)] = sum_mu T[A,q,mu] Cn[mu,p], q <= p. One thread per (A, packed (p,q)):
 // grid (ceil(n_aux/blockDim), n(n+1)/2).
-__global__ void k_ri_pair_factors(const double* __restrict__ T, const double* __restrict__ Cn, int n_bf, int n,
-                                  int n_aux, double* __restrict__ L) {
+__global__ void k_ri_pair_factors(const SlotDesc* __restrict__ slots, int n) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n_bf = sd.n_bf, n_aux = sd.n_aux;
+    const double* __restrict__ T = sd.T;
+    const double* __restrict__ Cn = sd.Cn;
+    double* __restrict__ L = sd.L;
     const int A = blockIdx.x * blockDim.x + threadIdx.x;
     const int P = blockIdx.y;
     if (A >= n_aux) return;
@@ -241,7 +265,11 @@ __global__ void k_ri_pair_factors(const double* __restrict__ T, const double* __
 }
 
 // packed G[(P,Q)] = sum_A L[A,P] L[A,Q]; one wavefront per packed entry.
-__global__ void k_ri_pair_gram(const double* __restrict__ L, int n_aux, int np, double* __restrict__ G) {
+__global__ void k_ri_pair_gram(const SlotDesc* __restrict__ slots, int np) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n_aux = sd.n_aux;
+    const double* __restrict__ L = sd.L;
+    double* __restrict__ G = sd.G;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int ng = np * (np + 1) / 2;
@@ -271,18 +299,66 @@ __device__ __forceinline__ int nth_bit(uint64_t m, int k) {
     return __builtin_ctzll(m);
 }
 
-struct BlockInverse {
-    double dens;    // this lane's D[r,l] = Y(l,r), Y = X^{-1}
-    double det;     // wave-uniform determinant
-    bool singular;  // wave-uniform: rank below the block size under the 1e-12 relative pivot rule
+// Lanes of a wavefront are split into groups of GROUP lanes and each group evaluates one
+// unordered determinant pair. An m x m overlap block (m <= MB) sits one entry per lane at
+// lane-in-group r + MB*l; blocks up to 4x4 use 16-lane groups (four pairs per wavefront), 5x5
+// 32-lane groups (two pairs), 6x6..8x8 the whole wavefront.
+template <int GROUP>
+struct GroupShape;
+template <>
+struct GroupShape<16> {
+    static constexpr int MB = 4;
+};
+template <>
+struct GroupShape<32> {
+    static constexpr int MB = 5;
+};
+template <>
+struct GroupShape<64> {
+    static constexpr int MB = 8;
 };
 
-// Gauss-Jordan elimination with complete pivoting of the m x m block held one element per lane
-// (lane = r + 8*l), augmented with the identity. Pivot choice and the rank rule follow Eigen's
-// FullPivLU as used by DeterminantOverlapResolver (determinants/algebra/overlap.cpp:50-110):
-// largest remaining |entry|; rank counts pivots above 1e-12 * max|pivot|.
-__device__ BlockInverse invert_block(double x, int m, int lane) {
-    const int r = lane & 7, l = lane >> 3;
+// per-group maximum of an int: DPP within 16-lane rows, then shuffles across rows
+template <int GROUP>
+__device__ __forceinline__ int group_max(int v) {
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false));  // row_mirror
+    if (GROUP >= 32) v = max(v, __shfl_xor(v, 16, 64));
+    if (GROUP >= 64) v = max(v, __shfl_xor(v, 32, 64));
+    return v;
+}
+
+// per-group sum; every lane of a group receives the bitwise-identical value
+template <int GROUP>
+__device__ __forceinline__ double group_sum(double v) {
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    if (GROUP >= 32) v += __shfl_xor(v, 16, 64);
+    if (GROUP >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+struct BlockInverse {
+    double dens;    // this lane's D[r,l] = Y(l,r), Y = X^{-1}
+    double det;     // group-uniform determinant
+    bool singular;  // group-uniform: rank below the block size under the 1e-12 relative pivot rule
+};
+
+// Gauss-Jordan elimination with complete pivoting of each group's m x m block, augmented with
+// the identity. Pivot choice and the rank rule follow Eigen's FullPivLU as used by
+// DeterminantOverlapResolver (determinants/algebra/overlap.cpp:50-110): the pivot is a largest
+// remaining |entry| (compared on the high 32 bits of |x|, lowest lane on ties) and the rank counts
+// pivots above 1e-12 * max|pivot|; an exactly zero remaining corner is rank deficient. The loop
+// runs m steps for every group; a group found singular finishes with meaningless values that the
+// caller discards. The inverse is read back per lane as D[r,l] = M(pivot row of l, r)/pivot(l).
+template <int GROUP>
+__device__ BlockInverse invert_group(double x, int m, int gbase, int r, int l) {
+    constexpr int MB = GroupShape<GROUP>::MB;
+    const uint64_t gmask = GROUP == 64 ? ~0ull : ((1ull << GROUP) - 1);
     const bool valid = r < m && l < m;
     double aug = (valid && r == l) ? 1.0 : 0.0;
     uint32_t rows_used = 0, cols_used = 0;
@@ -292,23 +368,22 @@ __device__ BlockInverse invert_block(double x, int m, int lane) {
     double my_piv = 1.0;
     bool singular = false;
     for (int k = 0; k < m; ++k) {
-        const double cand = (valid && !((rows_used >> r) & 1u) && !((cols_used >> l) & 1u)) ? fabs(x) : -1.0;
-        const double best = wave_max(cand);
-        if (!(best > 0.0)) {  // exactly zero corner (or non-finite): rank deficient
-            singular = true;
-            break;
-        }
-        // lowest lane holding the maximum (deterministic tie-break)
-        const int idx = __builtin_ctzll(__ballot(cand == best));
-        const int pr = idx & 7, pc = idx >> 3;
-        const double p = wave_bcast(x, idx);
-        const double rp = 1.0 / p;
-        maxpiv = fmax(maxpiv, best);
-        minpiv = fmin(minpiv, best);
+        const bool cand = valid && !((rows_used >> r) & 1u) && !((cols_used >> l) & 1u);
+        const int key = cand ? (int)(__double_as_longlong(fabs(x)) >> 32) : -1;
+        const int best = group_max<GROUP>(key);
+        const uint64_t bal = __ballot(key == best);
+        const int idx = __builtin_ctzll(((bal >> gbase) & gmask) | (1ull << (GROUP - 1)));
+        const int pr = idx % MB, pc = idx / MB;
+        const double p = __shfl(x, gbase + idx, 64);
+        if (!(best > 0)) singular = true;  // exactly zero (or denormal) corner
+        const double ap = fabs(p);
+        maxpiv = fmax(maxpiv, ap);
+        minpiv = fmin(minpiv, ap);
         inversions += __builtin_popcount(rows_used >> (pr + 1)) + __builtin_popcount(cols_used >> (pc + 1));
-        const double x_r_pc = __shfl(x, r + 8 * pc, 64);
-        const double x_pr_l = __shfl(x, pr + 8 * l, 64);
-        const double a_pr_l = __shfl(aug, pr + 8 * l, 64);
+        const double rp = singular ? 0.0 : 1.0 / p;
+        const double x_r_pc = __shfl(x, gbase + r + MB * pc, 64);
+        const double x_pr_l = __shfl(x, gbase + pr + MB * l, 64);
+        const double a_pr_l = __shfl(aug, gbase + pr + MB * l, 64);
         if (valid && r != pr) {
             const double f = x_r_pc * rp;
             x -= f * x_pr_l;
@@ -326,38 +401,49 @@ __device__ BlockInverse invert_block(double x, int m, int lane) {
     BlockInverse out;
     out.singular = singular;
     out.det = (inversions & 1) ? -det : det;
-    // Y(c, j) = M(pivot row of c, j) / pivot(c); lane (r,l) wants Y(l, r)
-    const double a = __shfl(aug, (my_row & 7) + 8 * (r & 7), 64);
+    const double a = __shfl(aug, gbase + my_row + MB * r, 64);
     out.dens = (valid && !singular) ? a / my_piv : 0.0;
     return out;
 }
 
-struct WaveSlab {
-    double D[2][64];  // transition densities D_s[r + 8l] = Y_s(l, r)
-    double F[2][64];  // dPhi/dD_s
-    double Z[64];
-    int PP[2][64];    // packed pair index (R_r, L_l) of each block entry
+template <int GROUP>
+struct GroupSlab {
+    double D[2][GROUP];  // transition densities D_s[r + MB l] = Y_s(l, r)
+    double F[2][GROUP];  // dPhi/dD_s
+    double Z[GROUP];
+    int PP[2][GROUP];    // packed pair index (R_r, L_l) of each block entry
     int occL[2][8];
     int occR[2][8];
 };
 
-// One wavefront per unordered determinant pair (d <= e). The active two-electron kernel is held in
-// LDS as the full symmetric np x np matrix G2[P + np*Q] so every (pq|rs) lookup is one index add.
+template <int GROUP>
+constexpr size_t slab_bytes_per_wave() {
+    return (64 / GROUP) * sizeof(GroupSlab<GROUP>);
+}
+
+// Determinant pairs. The active two-electron kernel is held in LDS as the full symmetric np x np
+// matrix G2[P + np*Q] so every (pq|rs) lookup is one index add.
 // Forward: H_de = S_de * Phi_de, S_de = det X_alpha det X_beta (Lowdin rules in the regular
 // inverse form, DeterminantPairEvaluator::combine_spin_pair_evaluations, pairs/evaluator.cpp:354-384).
 // Backward: adjoints of S_act, h_act and the packed kernel for F = w (H_de - E S_de),
 // accumulated per block in LDS (full np x np when FullDg, packed otherwise) and written as
 // packed partials.
-template <bool Backward, bool FullDg>
-__global__ void __launch_bounds__(512) k_pairs(const double* __restrict__ G_glob, const double* __restrict__ sact_g,
-                                               const double* __restrict__ hact_g, int n, int ng,
-                                               const uint64_t* __restrict__ amask, const uint64_t* __restrict__ bmask,
-                                               const int* __restrict__ det_a, const int* __restrict__ det_b, int n_det,
-                                               long long n_pairs, double* __restrict__ Hdet, double* __restrict__ Sdet,
-                                               const double* __restrict__ dcoef, const double* __restrict__ energy,
-                                               double* __restrict__ partials, int* __restrict__ flags) {
-    extern __shared__ double lds[];
+template <bool Backward, bool FullDg, int GROUP>
+__global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slots, TopoDesc topo) {
+    constexpr int MB = GroupShape<GROUP>::MB;
+    constexpr int GPW = 64 / GROUP;  // groups per wavefront
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ G_glob = sd.G;
+    const int n = topo.n, ng = topo.ng, n_det = topo.n_det;
     const int np = n * (n + 1) / 2;
+    const uint64_t* __restrict__ amask = topo.amask;
+    const uint64_t* __restrict__ bmask = topo.bmask;
+    const int* __restrict__ det_a = topo.det_a;
+    const int* __restrict__ det_b = topo.det_b;
+    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
+    int* __restrict__ flags = sd.flags;
+
+    extern __shared__ double lds[];
     double* G2 = lds;
     double* sact = G2 + np * np;
     double* hact = sact + n * n;
@@ -369,15 +455,15 @@ __global__ void __launch_bounds__(512) k_pairs(const double* __restrict__ G_glob
         dh = ds + n * n;
         tail = dh + n * n;
     }
-    WaveSlab* slabs = reinterpret_cast<WaveSlab*>(tail);
+    GroupSlab<GROUP>* slabs = reinterpret_cast<GroupSlab<GROUP>*>(tail);
     {
         const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, ln = threadIdx.x & 63;
         for (int Q = wave; Q < np; Q += nwaves)
             for (int P = ln; P < np; P += 64) G2[P + np * Q] = G_glob[pair_index(P, Q)];
     }
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-        sact[t] = sact_g[t];
-        hact[t] = hact_g[t];
+        sact[t] = sd.sact[t];
+        hact[t] = sd.hact[t];
     }
     if (Backward) {
         const int ndg = FullDg ? np * np : ng;
@@ -389,39 +475,45 @@ __global__ void __launch_bounds__(512) k_pairs(const double* __restrict__ G_glob
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int waves_per_block = blockDim.x >> 6;
-    WaveSlab& ws = slabs[threadIdx.x >> 6];
-    const long long first = (long long)blockIdx.x * waves_per_block + (threadIdx.x >> 6);
-    const long long stride = (long long)gridDim.x * waves_per_block;
-    const double E = Backward ? energy[0] : 0.0;
-    const int r = lane & 7, l = lane >> 3;
+    const int grp = lane / GROUP, lig = lane % GROUP, gbase = grp * GROUP;
+    const int r = lig % MB, l = lig / MB;
+    GroupSlab<GROUP>& gs = slabs[wave * GPW + grp];
+    const long long first = ((long long)blockIdx.x * waves_per_block + wave) * GPW;
+    const long long stride = (long long)gridDim.x * waves_per_block * GPW;
+    const double E = Backward ? sd.energy[0] : 0.0;
+    // every string of a topology has the same electron count per spin
+    const int m[2] = {__builtin_popcountll(amask[0]), __builtin_popcountll(bmask[0])};
 
-    for (long long t = first; t < n_pairs; t += stride) {
-        // unordered pair t = e(e+1)/2 + d, d <= e; d is the left (bra) determinant
-        long long e = (long long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while (e * (e + 1) / 2 > t) --e;
-        while ((e + 1) * (e + 2) / 2 <= t) ++e;
-        const int d = (int)(t - e * (e + 1) / 2);
-        const int ee = (int)e;
+    for (long long tb = first; tb < n_pairs; tb += stride) {
+        const long long t = tb + grp;
+        bool live = t < n_pairs;
+        int d = 0, ee = 0;
+        if (live) {
+            // unordered pair t = e(e+1)/2 + d, d <= e; d is the left (bra) determinant
+            long long e = (long long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+            while (e * (e + 1) / 2 > t) --e;
+            while ((e + 1) * (e + 2) / 2 <= t) ++e;
+            d = (int)(t - e * (e + 1) / 2);
+            ee = (int)e;
+        }
         double weight = 0.0;
         if (Backward) {
-            weight = dcoef[d] * dcoef[ee] * (d == ee ? 1.0 : 2.0);
-            if (weight == 0.0) continue;
+            weight = live ? sd.dcoef[d] * sd.dcoef[ee] * (d == ee ? 1.0 : 2.0) : 0.0;
+            live = live && weight != 0.0;
         }
         const uint64_t mL[2] = {amask[det_a[d]], bmask[det_b[d]]};
         const uint64_t mR[2] = {amask[det_a[ee]], bmask[det_b[ee]]};
-        int m[2];
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            m[s] = __builtin_popcountll(mL[s]);
-            if (lane < m[s]) {
-                ws.occL[s][lane] = nth_bit(mL[s], lane);
-                ws.occR[s][lane] = nth_bit(mR[s], lane);
+        for (int s = 0; s < 2; ++s)
+            if (lig < m[s]) {
+                gs.occL[s][lig] = nth_bit(mL[s], lig);
+                gs.occR[s][lig] = nth_bit(mR[s], lig);
             }
-        }
         wave_sync();
         // overlap blocks and their inverses
-        double det[2];
+        double det[2], dens[2];
         bool singular = false;
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -429,86 +521,87 @@ __global__ void __launch_bounds__(512) k_pairs(const double* __restrict__ G_glob
             double x = 0.0;
             int ppv = 0;
             if (valid) {
-                const int Rr = ws.occR[s][r], Ll = ws.occL[s][l];
+                const int Rr = gs.occR[s][r], Ll = gs.occL[s][l];
                 x = sact[Rr + n * Ll];
                 ppv = pair_index(Rr, Ll);
             }
-            ws.PP[s][lane] = ppv;
+            if (lig < GROUP) gs.PP[s][lig] = ppv;
             if (m[s] == 0) {
                 det[s] = 1.0;
-                ws.D[s][lane] = 0.0;
-                continue;
+                dens[s] = 0.0;
+            } else {
+                const BlockInverse inv = invert_group<GROUP>(x, m[s], gbase, r, l);
+                det[s] = inv.det;
+                dens[s] = inv.dens;
+                singular = singular || inv.singular;
             }
-            const BlockInverse inv = invert_block(x, m[s], lane);
-            det[s] = inv.det;
-            singular = singular || inv.singular;
-            ws.D[s][lane] = inv.dens;
+            gs.D[s][lig] = dens[s];
         }
-        if (singular) {
-            if (lane == 0) atomicAdd(&flags[FLAG_SINGULAR_PAIRS], 1);
-            continue;
+        if (live && singular) {
+            if (lig == 0) atomicAdd(&flags[FLAG_SINGULAR_PAIRS], 1);
         }
+        live = live && !singular;
         wave_sync();
         // F_s[r,l] = dPhi/dD_s[r,l] = h + sum (J - K) D_s + sum G D_other; Phi = 1/2 sum_s D_s.(h + F_s)
         double phi_part = 0.0;
-        int cross_l[2][8], cross_r[2][8];  // pp(R_r, L_l2) and pp(R_r2, L_l) for this lane, per spin
+        int cross_l[2][MB], cross_r[2][MB];  // pp(R_r, L_l2) and pp(R_r2, L_l) for this lane, per spin
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int ms = m[s], mo = m[1 - s];
             double f = 0.0;
-            const bool valid = r < ms && l < ms;
-            const int Rr = valid ? ws.occR[s][r] : 0, Ll = valid ? ws.occL[s][l] : 0;
+            const bool valid = live && r < ms && l < ms;
+            const int Rr = valid ? gs.occR[s][r] : 0, Ll = valid ? gs.occL[s][l] : 0;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                cross_l[s][k] = (valid && k < ms) ? pair_index(Rr, ws.occL[s][k]) : 0;
-                cross_r[s][k] = (valid && k < ms) ? pair_index(ws.occR[s][k], Ll) : 0;
+            for (int k = 0; k < MB; ++k) {
+                cross_l[s][k] = (valid && k < ms) ? pair_index(Rr, gs.occL[s][k]) : 0;
+                cross_r[s][k] = (valid && k < ms) ? pair_index(gs.occR[s][k], Ll) : 0;
             }
             if (valid) {
                 const double hv = hact[Rr + n * Ll];
-                const int P1 = ws.PP[s][lane];
-                const double* g1 = G2 + (size_t)np * P1;
+                const double* g1 = G2 + (size_t)np * gs.PP[s][lig];
                 f = hv;
 #pragma unroll
-                for (int l2 = 0; l2 < 8; ++l2) {
+                for (int l2 = 0; l2 < MB; ++l2) {
                     if (l2 < ms) {
                         const double* gx = G2 + (size_t)np * cross_l[s][l2];
 #pragma unroll
-                        for (int r2 = 0; r2 < 8; ++r2) {
+                        for (int r2 = 0; r2 < MB; ++r2) {
                             if (r2 < ms) {
-                                const int idx = r2 + 8 * l2;
-                                const double J = g1[ws.PP[s][idx]];
+                                const int idx = r2 + MB * l2;
+                                const double J = g1[gs.PP[s][idx]];
                                 const double K = gx[cross_r[s][r2]];
-                                f += (J - K) * ws.D[s][idx];
+                                f += (J - K) * gs.D[s][idx];
                             }
                         }
                     }
                 }
 #pragma unroll
-                for (int l2 = 0; l2 < 8; ++l2) {
+                for (int l2 = 0; l2 < MB; ++l2) {
                     if (l2 < mo) {
 #pragma unroll
-                        for (int r2 = 0; r2 < 8; ++r2) {
+                        for (int r2 = 0; r2 < MB; ++r2) {
                             if (r2 < mo) {
-                                const int idx = r2 + 8 * l2;
-                                f += g1[ws.PP[1 - s][idx]] * ws.D[1 - s][idx];
+                                const int idx = r2 + MB * l2;
+                                f += g1[gs.PP[1 - s][idx]] * gs.D[1 - s][idx];
                             }
                         }
                     }
                 }
-                phi_part += 0.5 * ws.D[s][lane] * (hv + f);
+                phi_part += 0.5 * dens[s] * (hv + f);
             }
-            ws.F[s][lane] = f;
+            gs.F[s][lig] = f;
         }
-        const double phi = wave_sum(phi_part);
+        const double phi = group_sum<GROUP>(phi_part);
         const double Dtot = det[0] * det[1];
         if (!Backward) {
-            if (lane == 0) {
+            if (live && lig == 0) {
                 const double H = Dtot * phi;
-                Hdet[(size_t)d + (size_t)n_det * ee] = H;
-                Hdet[(size_t)ee + (size_t)n_det * d] = H;
-                Sdet[(size_t)d + (size_t)n_det * ee] = Dtot;
-                Sdet[(size_t)ee + (size_t)n_det * d] = Dtot;
+                sd.Hdet[(size_t)d + (size_t)n_det * ee] = H;
+                sd.Hdet[(size_t)ee + (size_t)n_det * d] = H;
+                sd.Sdet[(size_t)d + (size_t)n_det * ee] = Dtot;
+                sd.Sdet[(size_t)ee + (size_t)n_det * d] = Dtot;
             }
+            wave_sync();
             continue;
         }
         // ---- backward: F_pair = w (H - E S) ----
@@ -520,40 +613,42 @@ __global__ void __launch_bounds__(512) k_pairs(const double* __restrict__ G_glob
         for (int s = 0; s < 2; ++s) {
             const int ms = m[s];
             if (ms == 0) continue;
-            const bool valid = r < ms && l < ms;
+            const bool valid = live && r < ms && l < ms;
             // Z[a,l] = sum_b F[b,a] D[b,l]
             double z = 0.0;
             if (valid) {
 #pragma unroll
-                for (int b = 0; b < 8; ++b)
-                    if (b < ms) z += ws.F[s][b + 8 * r] * ws.D[s][b + 8 * l];
+                for (int b = 0; b < MB; ++b)
+                    if (b < ms) z += gs.F[s][b + MB * r] * gs.D[s][b + MB * l];
             }
-            ws.Z[lane] = z;
+            gs.Z[lig] = z;
             wave_sync();
             if (valid) {
                 double dfd = 0.0;  // (D F^T D)[r,l]
 #pragma unroll
-                for (int a = 0; a < 8; ++a)
-                    if (a < ms) dfd += ws.D[s][r + 8 * a] * ws.Z[a + 8 * l];
-                const double Drl = ws.D[s][lane];
-                const int Rr = ws.occR[s][r], Ll = ws.occL[s][l];
+                for (int a2 = 0; a2 < MB; ++a2)
+                    if (a2 < ms) dfd += gs.D[s][r + MB * a2] * gs.Z[a2 + MB * l];
+                const double Drl = dens[s];
+                const int Rr = gs.occR[s][r], Ll = gs.occL[s][l];
                 atomicAdd(&ds[Rr + n * Ll], c * ((wH * phi + wS) * Drl - wH * dfd));
                 atomicAdd(&dh[Rr + n * Ll], cw * Drl);
-                const int P1 = ws.PP[s][lane];
+                const int P1 = gs.PP[s][lig];
                 // same-spin two-electron adjoint over r2 > r, l2 > l
 #pragma unroll
-                for (int l2 = 1; l2 < 8; ++l2) {
+                for (int l2 = 1; l2 < MB; ++l2) {
                     if (l2 > l && l2 < ms) {
 #pragma unroll
-                        for (int r2 = 1; r2 < 8; ++r2) {
+                        for (int r2 = 1; r2 < MB; ++r2) {
                             if (r2 > r && r2 < ms) {
-                                const int idx = r2 + 8 * l2;
-                                const double mnr = Drl * ws.D[s][idx] - ws.D[s][r + 8 * l2] * ws.D[s][r2 + 8 * l];
+                                const int idx = r2 + MB * l2;
+                                const double mnr =
+                                    Drl * gs.D[s][idx] - gs.D[s][r + MB * l2] * gs.D[s][r2 + MB * l];
+                                const int pu = gs.PP[s][idx];
                                 if (FullDg) {
-                                    atomicAdd(&dg[P1 + np * ws.PP[s][idx]], cw * mnr);
+                                    atomicAdd(&dg[P1 + np * pu], cw * mnr);
                                     atomicAdd(&dg[cross_l[s][l2] + np * cross_r[s][r2]], -cw * mnr);
                                 } else {
-                                    atomicAdd(&dg[pair_index(P1, ws.PP[s][idx])], cw * mnr);
+                                    atomicAdd(&dg[pair_index(P1, pu)], cw * mnr);
                                     atomicAdd(&dg[pair_index(cross_l[s][l2], cross_r[s][r2])], -cw * mnr);
                                 }
                             }
@@ -563,15 +658,16 @@ __global__ void __launch_bounds__(512) k_pairs(const double* __restrict__ G_glob
                 if (s == 0) {  // opposite spin, once per (alpha entry, beta entry)
                     const int mo = m[1];
 #pragma unroll
-                    for (int l2 = 0; l2 < 8; ++l2) {
+                    for (int l2 = 0; l2 < MB; ++l2) {
                         if (l2 < mo) {
 #pragma unroll
-                            for (int r2 = 0; r2 < 8; ++r2) {
+                            for (int r2 = 0; r2 < MB; ++r2) {
                                 if (r2 < mo) {
-                                    const int idx = r2 + 8 * l2;
-                                    const double v = cw * Drl * ws.D[1][idx];
-                                    if (FullDg) atomicAdd(&dg[P1 + np * ws.PP[1][idx]], v);
-                                    else atomicAdd(&dg[pair_index(P1, ws.PP[1][idx])], v);
+                                    const int idx = r2 + MB * l2;
+                                    const double v = cw * Drl * gs.D[1][idx];
+                                    const int pu = gs.PP[1][idx];
+                                    if (FullDg) atomicAdd(&dg[P1 + np * pu], v);
+                                    else atomicAdd(&dg[pair_index(P1, pu)], v);
                                 }
                             }
                         }
@@ -584,7 +680,7 @@ __global__ void __launch_bounds__(512) k_pairs(const double* __restrict__ G_glob
     if (Backward) {
         __syncthreads();
         const int stride_part = ng + 2 * n * n;
-        double* out = partials + (size_t)blockIdx.x * stride_part;
+        double* out = sd.partials + (size_t)blockIdx.x * stride_part;
         if (FullDg) {
             for (int t = threadIdx.x; t < ng; t += blockDim.x) {
                 int P = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -604,7 +700,10 @@ __global__ void __launch_bounds__(512) k_pairs(const double* __restrict__ G_glob
 }
 
 // sum block partials in a fixed order: out[i] = sum_b partials[b*len + i], one wavefront per i
-__global__ void k_reduce_partials(const double* __restrict__ partials, int n_blocks, int len, double* __restrict__ out) {
+__global__ void k_reduce_partials(const SlotDesc* __restrict__ slots, int n_blocks, int len) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ partials = sd.partials;
+    double* __restrict__ out = sd.dsum;
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (i >= len) return;
@@ -617,9 +716,16 @@ __global__ void k_reduce_partials(const double* __restrict__ partials, int n_blo
 // ---------------------------------------------------------------------------------------------
 // structure matrices: Y = Hdet T (thread per (det, structure)), then T^T Y (thread per (I, J))
 // ---------------------------------------------------------------------------------------------
-__global__ void k_structure_right(const double* __restrict__ Hdet, const double* __restrict__ Sdet, int n_det,
-                                  int n_str, const long long* __restrict__ soff, const int* __restrict__ sdet,
-                                  const double* __restrict__ scoef, double* __restrict__ Yh, double* __restrict__ Ys) {
+__global__ void k_structure_right(const SlotDesc* __restrict__ slots, TopoDesc topo) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ Hdet = sd.Hdet;
+    const double* __restrict__ Sdet = sd.Sdet;
+    const int n_det = topo.n_det, n_str = topo.n_str;
+    const long long* __restrict__ soff = topo.soff;
+    const int* __restrict__ sdet = topo.sdet;
+    const double* __restrict__ scoef = topo.scoef;
+    double* __restrict__ Yh = sd.Yh;
+    double* __restrict__ Ys = sd.Ys;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)n_det * n_str) return;
     const int d = (int)(id % n_det), J = (int)(id / n_det);
@@ -633,9 +739,16 @@ __global__ void k_structure_right(const double* __restrict__ Hdet, const double*
     Ys[id] = s;
 }
 
-__global__ void k_structure_left(const double* __restrict__ Yh, const double* __restrict__ Ys, int n_det, int n_str,
-                                 const long long* __restrict__ soff, const int* __restrict__ sdet,
-                                 const double* __restrict__ scoef, double* __restrict__ Hs, double* __restrict__ Ss) {
+__global__ void k_structure_left(const SlotDesc* __restrict__ slots, TopoDesc topo) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ Yh = sd.Yh;
+    const double* __restrict__ Ys = sd.Ys;
+    const int n_det = topo.n_det, n_str = topo.n_str;
+    const long long* __restrict__ soff = topo.soff;
+    const int* __restrict__ sdet = topo.sdet;
+    const double* __restrict__ scoef = topo.scoef;
+    double* __restrict__ Hs = sd.Hs;
+    double* __restrict__ Ss = sd.Ss;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)n_str * n_str) return;
     const int I = (int)(id % n_str), J = (int)(id / n_str);
@@ -649,16 +762,6 @@ __global__ void k_structure_left(const double* __restrict__ Yh, const double* __
     Ss[id] = s;
 }
 
-// a[d] = sum_terms coef * c[structure]
-__global__ void k_det_coefficients(const long long* __restrict__ off, const int* __restrict__ ts,
-                                   const double* __restrict__ tc, const double* __restrict__ coef, int n_det,
-                                   double* __restrict__ a) {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= n_det) return;
-    double acc = 0.0;
-    for (long long t = off[d]; t < off[d + 1]; ++t) acc += tc[t] * coef[ts[t]];
-    a[d] = acc;
-}
 
 // ---------------------------------------------------------------------------------------------
 // generalised eigenproblem H c = E S c, lowest root, one workgroup.
@@ -669,12 +772,21 @@ __global__ void k_det_coefficients(const long long* __restrict__ off, const int*
 // global workspace. A single wavefront is used for small n so that the block barriers are free.
 // ---------------------------------------------------------------------------------------------
 template <bool InLds>
-__global__ void __launch_bounds__(256) k_gep_lowest(const double* __restrict__ Hs, const double* __restrict__ Ss,
-                                                    int n, double e_nuc, double* __restrict__ gwork,
-                                                    double* __restrict__ coef, double* __restrict__ energy,
-                                                    int* __restrict__ flags, const long long* __restrict__ off,
-                                                    const int* __restrict__ ts, const double* __restrict__ tc,
-                                                    int n_det, double* __restrict__ dcoef) {
+__global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__ slots, TopoDesc topo) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ Hs = sd.Hs;
+    const double* __restrict__ Ss = sd.Ss;
+    const int n = topo.n_str;
+    const double e_nuc = sd.e_nuc;
+    double* __restrict__ gwork = sd.gep;
+    double* __restrict__ coef = sd.coef;
+    double* __restrict__ energy = sd.energy;
+    int* __restrict__ flags = sd.flags;
+    const long long* __restrict__ off = topo.off;
+    const int* __restrict__ ts = topo.ts;
+    const double* __restrict__ tc = topo.tc;
+    const int n_det = topo.n_det;
+    double* __restrict__ dcoef = sd.dcoef;
     extern __shared__ double lds[];
     double* work = InLds ? lds : gwork;
     double* Lm = work;              // n x n, lower Cholesky factor
@@ -692,7 +804,6 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const double* __restrict__ H
     __shared__ double red[16];
     __shared__ double sh_scalar[4];
     __shared__ double lo_hi[2];
-    __shared__ int sh_flag;
     __shared__ int first_true;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int i = tid; i < n; i += nt)
@@ -700,7 +811,6 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const double* __restrict__ H
             Lm[i + n * j] = 0.5 * (Ss[i + n * j] + Ss[j + n * i]);
             M[i + n * j] = 0.5 * (Hs[i + n * j] + Hs[j + n * i]);
         }
-    if (tid == 0) sh_flag = 0;
     __syncthreads();
     // Cholesky (right-looking, lower); threads own rows, so no index division is needed
     for (int k = 0; k < n; ++k) {
@@ -766,7 +876,7 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const double* __restrict__ H
     __syncthreads();
     // Householder tridiagonalisation: reflect rows/cols k+1..n-1 of column k
     for (int k = 0; k < n - 2; ++k) {
-        const int m0 = k + 1, len = n - m0;
+        const int m0 = k + 1;
         double part = 0.0;
         for (int i = m0 + tid; i < n; i += nt) part += M[i + n * k] * M[i + n * k];
         const double sigma = block_sum(part, red);
@@ -944,8 +1054,12 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const double* __restrict__ H
 // ---------------------------------------------------------------------------------------------
 
 // dL[A,P] = sum_Q M[P,Q] L[A,Q], M[P,Q] = dg(P,Q) off the diagonal, 2 dg(P,P) on it.
-__global__ void k_ri_pair_adjoint(const double* __restrict__ L, const double* __restrict__ dg, int n_aux, int np,
-                                  double* __restrict__ dL) {
+__global__ void k_ri_pair_adjoint(const SlotDesc* __restrict__ slots, int np) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n_aux = sd.n_aux;
+    const double* __restrict__ L = sd.L;
+    const double* __restrict__ dg = sd.dsum;
+    double* __restrict__ dL = sd.dL;
     const int A = blockIdx.x * blockDim.x + threadIdx.x;
     const int P = blockIdx.y;
     if (A >= n_aux) return;
@@ -960,10 +1074,16 @@ __global__ void k_ri_pair_adjoint(const double* __restrict__ L, const double* __
 // gcn[mu, p] = sum_i SC[mu,i] (ds+ds^T)[i,p] + HC[mu,i] (dh+dh^T)[i,p]
 //            + sum_q f_pq sum_A dL[A,(p,q)] T[A,q,mu],  f_pq = 2 if p == q else 1.
 // one block per AO index mu.
-__global__ void __launch_bounds__(256) k_orbital_gradient(const double* __restrict__ T, const double* __restrict__ dL,
-                                                          const double* __restrict__ SC, const double* __restrict__ HC,
-                                                          const double* __restrict__ dsdh, int n_bf, int n, int n_aux,
-                                                          int ng, double* __restrict__ gcn) {
+__global__ void __launch_bounds__(256) k_orbital_gradient(const SlotDesc* __restrict__ slots, int n, int ng) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n_bf = sd.n_bf, n_aux = sd.n_aux;
+    if ((int)blockIdx.x >= n_bf) return;  // block-uniform
+    const double* __restrict__ T = sd.T;
+    const double* __restrict__ dL = sd.dL;
+    const double* __restrict__ SC = sd.SC;
+    const double* __restrict__ HC = sd.HC;
+    const double* __restrict__ dsdh = sd.dsum;
+    double* __restrict__ gcn = sd.gcn;
     __shared__ double part[4][MAX_ACT];
     const int mu = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -998,9 +1118,15 @@ __global__ void __launch_bounds__(256) k_orbital_gradient(const double* __restri
 }
 
 // raw-slot gradient: grad[k, mu] = (g - (g.cn_k) S cn_k)[mu] / sqrt(nu2_k) on present slots.
-__global__ void k_normalization_pullback(const double* __restrict__ gcn, const double* __restrict__ Cn,
-                                         const double* __restrict__ SC, const double* __restrict__ nu2,
-                                         const uint8_t* __restrict__ support, int n_bf, double* __restrict__ grad) {
+__global__ void k_normalization_pullback(const SlotDesc* __restrict__ slots) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ gcn = sd.gcn;
+    const double* __restrict__ Cn = sd.Cn;
+    const double* __restrict__ SC = sd.SC;
+    const double* __restrict__ nu2 = sd.nu2;
+    const uint8_t* __restrict__ support = sd.support;
+    const int n_bf = sd.n_bf;
+    double* __restrict__ grad = sd.grad;
     __shared__ double red[16];
     const int k = blockIdx.x;
     double part = 0.0;
@@ -1016,137 +1142,137 @@ __global__ void k_normalization_pullback(const double* __restrict__ gcn, const d
 // ---------------------------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------------------------
-// LDS of the pair kernels: full np x np kernel, S_act, h_act, per-wave slabs and, for the
+// LDS of the pair kernels: full np x np kernel, S_act, h_act, per-wave group slabs and, for the
 // backward, the adjoint accumulators (full np x np when that still fits, else packed).
 constexpr size_t kPairLdsLimit = 160 * 1024;
+constexpr size_t kGepLdsLimit = 144 * 1024;
+
+// lanes per determinant pair for blocks of at most `mmax` electrons of one spin
+int pair_group(int mmax) { return mmax <= 4 ? 16 : (mmax == 5 ? 32 : 64); }
+
+size_t slab_bytes(int group) {
+    return group == 16 ? slab_bytes_per_wave<16>() : group == 32 ? slab_bytes_per_wave<32>() : slab_bytes_per_wave<64>();
+}
 
 bool pairs_full_dg(int n, int ng) {
     const size_t np = (size_t)n * (n + 1) / 2;
-    const size_t bytes = (2 * np * np + 4 * (size_t)n * n) * sizeof(double) + 4 * sizeof(WaveSlab);
+    const size_t bytes = (2 * np * np + 4 * (size_t)n * n) * sizeof(double) + 4 * slab_bytes(64);
     (void)ng;
     return bytes <= 150 * 1024;
 }
 
-size_t pairs_lds_bytes(int n, int ng, bool backward, int waves) {
+size_t pairs_lds_bytes(int n, int ng, bool backward, int waves, int group) {
     const size_t np = (size_t)n * (n + 1) / 2;
     size_t doubles = np * np + 2 * (size_t)n * n;
     if (backward) doubles += (pairs_full_dg(n, ng) ? np * np : (size_t)ng) + 2 * (size_t)n * n;
-    return doubles * sizeof(double) + (size_t)waves * sizeof(WaveSlab);
+    return doubles * sizeof(double) + (size_t)waves * slab_bytes(group);
 }
 
 // eight wavefronts per block when the LDS allows it, else four
-int pairs_waves(int n, int ng, bool backward) {
-    return pairs_lds_bytes(n, ng, backward, 8) <= kPairLdsLimit ? 8 : 4;
+int pairs_waves(int n, int ng, bool backward, int group) {
+    return pairs_lds_bytes(n, ng, backward, 8, group) <= kPairLdsLimit ? 8 : 4;
 }
 
-size_t pairs_lds_bytes(int n, int ng, bool backward) {
-    return pairs_lds_bytes(n, ng, backward, pairs_waves(n, ng, backward));
+template <bool B, bool F>
+hipError_t set_pairs_lds(int group, size_t bytes) {
+    const void* fn = group == 16   ? reinterpret_cast<const void*>(&k_pairs<B, F, 16>)
+                     : group == 32 ? reinterpret_cast<const void*>(&k_pairs<B, F, 32>)
+                                   : reinterpret_cast<const void*>(&k_pairs<B, F, 64>);
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-size_t gep_lds_bytes(int n);
-bool gep_in_lds(int n);
-
-hipError_t configure_kernels(int n, int ng, int n_str) {
-    const size_t fwd = pairs_lds_bytes(n, ng, false), bwd = pairs_lds_bytes(n, ng, true);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<false, false>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)fwd);
-    if (e != hipSuccess) return e;
-    e = pairs_full_dg(n, ng)
-            ? hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<true, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bwd)
-            : hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs<true, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bwd);
-    if (e != hipSuccess) return e;
-    if (gep_in_lds(n_str))
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gep_lowest<true>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)gep_lds_bytes(n_str));
-    return e;
-}
-
-void launch_orbital_prep(hipStream_t st, const double* C, const uint8_t* support, const double* S, const double* h,
-                         int n_bf, int n, double* Cn, double* SC, double* HC, double* nu2, int* flags) {
-    const size_t lds = ((size_t)n_bf + 16) * sizeof(double);
-    hipLaunchKernelGGL(k_orbital_prep, dim3(n), dim3(256), lds, st, C, support, S, h, n_bf, n, Cn, SC, HC, nu2, flags);
-}
-
-void launch_active_matrices(hipStream_t st, const double* Cn, const double* SC, const double* HC, int n_bf, int n,
-                            double* sact, double* hact, int* flags) {
-    hipLaunchKernelGGL(k_active_matrices, dim3(1), dim3(256), 0, st, Cn, SC, HC, n_bf, n, sact, hact, flags);
-}
-
-void launch_ri_one_index(hipStream_t st, const double* B, const double* Cn, int n_bf, int n, int n_aux, double* T) {
-    hipLaunchKernelGGL(k_ri_one_index, dim3((n_aux + 63) / 64, n_bf), dim3(256), 0, st, B, Cn, n_bf, n, n_aux, T);
-}
-
-void launch_ri_pair_stage(hipStream_t st, const double* T, const double* Cn, int n_bf, int n, int n_aux, double* L,
-                          double* G) {
-    const int tb = 64;
-    const int np = n * (n + 1) / 2, ng = np * (np + 1) / 2;
-    hipLaunchKernelGGL(k_ri_pair_factors, dim3((n_aux + tb - 1) / tb, np), dim3(tb), 0, st, T, Cn, n_bf, n, n_aux, L);
-    hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4), dim3(256), 0, st, L, n_aux, np, G);
-}
-
-void launch_pairs(hipStream_t st, bool backward, int n_blocks, const double* G, const double* sact, const double* hact,
-                  int n, int ng, const uint64_t* amask, const uint64_t* bmask, const int* det_a, const int* det_b,
-                  int n_det, double* Hdet, double* Sdet, const double* dcoef, const double* energy, double* partials,
-                  int* flags) {
-    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
-    const size_t lds = pairs_lds_bytes(n, ng, backward);
-    const int threads = 64 * pairs_waves(n, ng, backward);
-    if (!backward)
-        hipLaunchKernelGGL((k_pairs<false, false>), dim3(n_blocks), dim3(threads), lds, st, G, sact, hact, n, ng, amask,
-                           bmask, det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
-    else if (pairs_full_dg(n, ng))
-        hipLaunchKernelGGL((k_pairs<true, true>), dim3(n_blocks), dim3(threads), lds, st, G, sact, hact, n, ng, amask,
-                           bmask, det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
+template <bool B, bool F>
+void launch_pairs_variant(hipStream_t st, int group, dim3 grid, int threads, size_t lds, const SlotDesc* slots,
+                          const TopoDesc& topo) {
+    if (group == 16)
+        hipLaunchKernelGGL((k_pairs<B, F, 16>), grid, dim3(threads), lds, st, slots, topo);
+    else if (group == 32)
+        hipLaunchKernelGGL((k_pairs<B, F, 32>), grid, dim3(threads), lds, st, slots, topo);
     else
-        hipLaunchKernelGGL((k_pairs<true, false>), dim3(n_blocks), dim3(threads), lds, st, G, sact, hact, n, ng, amask,
-                           bmask, det_a, det_b, n_det, n_pairs, Hdet, Sdet, dcoef, energy, partials, flags);
+        hipLaunchKernelGGL((k_pairs<B, F, 64>), grid, dim3(threads), lds, st, slots, topo);
 }
 
-void launch_reduce_partials(hipStream_t st, const double* partials, int n_blocks, int len, double* out) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4), dim3(256), 0, st, partials, n_blocks, len, out);
-}
-
-void launch_structure(hipStream_t st, const double* Hdet, const double* Sdet, int n_det, int n_str,
-                      const long long* soff, const int* sdet, const double* scoef, double* Yh, double* Ys, double* Hs,
-                      double* Ss) {
-    const long long a = (long long)n_det * n_str, b = (long long)n_str * n_str;
-    hipLaunchKernelGGL(k_structure_right, dim3((unsigned)((a + 255) / 256)), dim3(256), 0, st, Hdet, Sdet, n_det, n_str,
-                       soff, sdet, scoef, Yh, Ys);
-    hipLaunchKernelGGL(k_structure_left, dim3((unsigned)((b + 255) / 256)), dim3(256), 0, st, Yh, Ys, n_det, n_str,
-                       soff, sdet, scoef, Hs, Ss);
+// Raise the dynamic-LDS ceiling of every LDS-heavy kernel variant to its limit. The attribute is
+// per function and process-wide, so pools of different active spaces never lower it for each other.
+hipError_t configure_kernels(int n, int ng, int n_str, int mmax) {
+    (void)n;
+    (void)ng;
+    (void)n_str;
+    (void)mmax;
+    hipError_t e = hipSuccess;
+    for (int group : {16, 32, 64}) {
+        if ((e = set_pairs_lds<false, false>(group, kPairLdsLimit)) != hipSuccess) return e;
+        if ((e = set_pairs_lds<true, true>(group, kPairLdsLimit)) != hipSuccess) return e;
+        if ((e = set_pairs_lds<true, false>(group, kPairLdsLimit)) != hipSuccess) return e;
+    }
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gep_lowest<true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGepLdsLimit);
 }
 
 size_t gep_lds_bytes(int n) { return gep_work_doubles(n) * sizeof(double); }
 
-bool gep_in_lds(int n) { return gep_lds_bytes(n) <= 144 * 1024; }
+bool gep_in_lds(int n) { return gep_lds_bytes(n) <= kGepLdsLimit; }
 
-void launch_gep(hipStream_t st, const double* Hs, const double* Ss, int n, double e_nuc, double* work, double* coef,
-                double* energy, int* flags, const long long* off, const int* ts, const double* tc, int n_det,
-                double* dcoef) {
-    const int threads = n <= 64 ? 64 : 256;
-    if (gep_in_lds(n))
-        hipLaunchKernelGGL(k_gep_lowest<true>, dim3(1), dim3(threads), gep_lds_bytes(n), st, Hs, Ss, n, e_nuc, work,
-                           coef, energy, flags, off, ts, tc, n_det, dcoef);
+void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape,
+                    hipEvent_t* ev) {
+    const int n = topo.n, np = topo.np, ng = topo.ng;
+    const unsigned z = (unsigned)count;
+    auto mark = [&](int k) {
+        if (ev) (void)hipEventRecord(ev[k], st);
+    };
+    mark(EV_START);
+    hipLaunchKernelGGL(k_orbital_prep, dim3(n, 1, z), dim3(256), ((size_t)shape.max_n_bf + 16) * sizeof(double), st,
+                       slots, n);
+    hipLaunchKernelGGL(k_active_matrices, dim3(1, 1, z), dim3(256), 0, st, slots, n);
+    mark(EV_PREP);
+    hipLaunchKernelGGL(k_ri_one_index, dim3((shape.max_n_aux + 63) / 64, shape.max_n_bf, z), dim3(256), 0, st, slots, n);
+    mark(EV_ONE_INDEX);
+    hipLaunchKernelGGL(k_ri_pair_factors, dim3((shape.max_n_aux + 63) / 64, np, z), dim3(64), 0, st, slots, n);
+    hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4, 1, z), dim3(256), 0, st, slots, np);
+    mark(EV_TRANSFORM);
+    const int group = pair_group(shape.mmax);
+    const int waves_f = pairs_waves(n, ng, false, group);
+    const size_t lds_f = pairs_lds_bytes(n, ng, false, waves_f, group);
+    launch_pairs_variant<false, false>(st, group, dim3(shape.pair_blocks_fwd, 1, z), 64 * waves_f, lds_f, slots, topo);
+    mark(EV_PAIR_FWD);
+    const long long a = (long long)topo.n_det * topo.n_str, b = (long long)topo.n_str * topo.n_str;
+    hipLaunchKernelGGL(k_structure_right, dim3((unsigned)((a + 255) / 256), 1, z), dim3(256), 0, st, slots, topo);
+    hipLaunchKernelGGL(k_structure_left, dim3((unsigned)((b + 255) / 256), 1, z), dim3(256), 0, st, slots, topo);
+    mark(EV_STRUCT);
+    const int threads = topo.n_str <= 64 ? 64 : 256;
+    if (gep_in_lds(topo.n_str))
+        hipLaunchKernelGGL(k_gep_lowest<true>, dim3(1, 1, z), dim3(threads), gep_lds_bytes(topo.n_str), st, slots,
+                           topo);
     else
-        hipLaunchKernelGGL(k_gep_lowest<false>, dim3(1), dim3(threads), 0, st, Hs, Ss, n, e_nuc, work, coef, energy,
-                           flags, off, ts, tc, n_det, dcoef);
+        hipLaunchKernelGGL(k_gep_lowest<false>, dim3(1, 1, z), dim3(threads), 0, st, slots, topo);
+    mark(EV_EIGEN);
 }
 
-void launch_det_coefficients(hipStream_t st, const long long* off, const int* ts, const double* tc, const double* coef,
-                             int n_det, double* a) {
-    hipLaunchKernelGGL(k_det_coefficients, dim3((n_det + 255) / 256), dim3(256), 0, st, off, ts, tc, coef, n_det, a);
-}
-
-void launch_gradient_tail(hipStream_t st, const double* T, const double* L, const double* dsum, const double* SC,
-                          const double* HC, const double* Cn, const double* nu2, const uint8_t* support, int n_bf,
-                          int n, int n_aux, double* dL, double* gcn, double* grad) {
-    const int np = n * (n + 1) / 2, ng = np * (np + 1) / 2;
-    const int tb = 64;
-    hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((n_aux + tb - 1) / tb, np), dim3(tb), 0, st, L, dsum, n_aux, np, dL);
-    hipLaunchKernelGGL(k_orbital_gradient, dim3(n_bf), dim3(256), 0, st, T, dL, SC, HC, dsum, n_bf, n, n_aux, ng, gcn);
-    hipLaunchKernelGGL(k_normalization_pullback, dim3(n), dim3(256), 0, st, gcn, Cn, SC, nu2, support, n_bf, grad);
+void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape,
+                     hipEvent_t* ev) {
+    const int n = topo.n, np = topo.np, ng = topo.ng;
+    const unsigned z = (unsigned)count;
+    auto mark = [&](int k) {
+        if (ev) (void)hipEventRecord(ev[k], st);
+    };
+    mark(EV_BWD_BEGIN);
+    const int group = pair_group(shape.mmax);
+    const int waves_b = pairs_waves(n, ng, true, group);
+    const size_t lds_b = pairs_lds_bytes(n, ng, true, waves_b, group);
+    if (pairs_full_dg(n, ng))
+        launch_pairs_variant<true, true>(st, group, dim3(shape.pair_blocks_bwd, 1, z), 64 * waves_b, lds_b, slots,
+                                         topo);
+    else
+        launch_pairs_variant<true, false>(st, group, dim3(shape.pair_blocks_bwd, 1, z), 64 * waves_b, lds_b, slots,
+                                          topo);
+    mark(EV_BWD_KERNEL);
+    const int len = ng + 2 * n * n;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, shape.pair_blocks_bwd, len);
+    mark(EV_PAIR_BWD);
+    hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((shape.max_n_aux + 63) / 64, np, z), dim3(64), 0, st, slots, np);
+    hipLaunchKernelGGL(k_orbital_gradient, dim3(shape.max_n_bf, 1, z), dim3(256), 0, st, slots, n, ng);
+    hipLaunchKernelGGL(k_normalization_pullback, dim3(n, 1, z), dim3(256), 0, st, slots);
+    mark(EV_TAIL);
 }
 
 }  // namespace xmvb_amd

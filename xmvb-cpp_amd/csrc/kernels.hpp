@@ -1,4 +1,9 @@
-// Launch wrappers of the engine's HIP kernels (kernels.hip).
+// Device-side descriptors and launch wrappers of the engine's HIP kernels (kernels.hip).
+//
+// Every kernel evaluates a set of resident problem slots at once: blockIdx.z selects an entry
+// of a per-launch SlotDesc table (device memory), which carries that slot's static data,
+// current orbitals, workspace and outputs. All slots of one launch share a TopoDesc (same
+// structure set), while n_bf and n_aux may differ per slot.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,7 +18,7 @@ constexpr int MAX_ACT = 14;
 // Largest occupied block (alpha or beta electrons): one element per lane of a 64-wide wavefront.
 constexpr int MAX_SPIN_BLOCK = 8;
 
-// device error flags
+// device error flags (per slot)
 enum DeviceFlag : int {
     FLAG_NORMALIZATION = 0,   // 1 + index of an orbital whose S-norm vanished
     FLAG_ACTIVE_OVERLAP = 1,  // S_act not positive definite
@@ -22,33 +27,64 @@ enum DeviceFlag : int {
     FLAG_COUNT = 8
 };
 
+// HIP events recorded between phases (optional)
+enum PhaseEvent : int {
+    EV_START, EV_PREP, EV_ONE_INDEX, EV_TRANSFORM, EV_PAIR_FWD, EV_STRUCT, EV_EIGEN, EV_BWD_BEGIN, EV_BWD_KERNEL,
+    EV_PAIR_BWD, EV_TAIL, EV_COUNT
+};
+
+struct SlotDesc {
+    // molecule-static data
+    const double* S;
+    const double* h;
+    const double* B;
+    int n_bf, n_aux;
+    double e_nuc;
+    // current evaluation: inputs and outputs
+    const double* C;
+    const uint8_t* support;
+    double* grad;
+    double* energy;  // [0] eigenvalue, [1] total energy
+    int* flags;      // FLAG_COUNT ints
+    // workspace
+    double *Cn, *SC, *HC, *nu2, *sact, *hact, *T, *L, *G;
+    double *Hdet, *Sdet, *Yh, *Ys, *Hs, *Ss, *gep, *coef, *dcoef;
+    double *partials, *dsum, *dL, *gcn;
+};
+
+struct TopoDesc {
+    int n, np, ng, n_det, n_str;
+    const uint64_t* amask;
+    const uint64_t* bmask;
+    const int* det_a;
+    const int* det_b;
+    const long long* off;   // determinant -> terms CSR
+    const int* ts;
+    const double* tc;
+    const long long* soff;  // structure -> determinants CSR
+    const int* sdet;
+    const double* scoef;
+};
+
+struct LaunchShape {
+    int max_n_bf, max_n_aux;
+    int pair_blocks_fwd, pair_blocks_bwd;  // blocks per slot
+    int mmax;                              // largest electron count of one spin
+};
+
 inline std::size_t gep_work_doubles(int n) { return 3 * (std::size_t)n * n + 9 * (std::size_t)n; }
 
-std::size_t pairs_lds_bytes(int n, int ng, bool backward);
-hipError_t configure_kernels(int n, int ng, int n_str);
+int pair_group(int mmax);
+bool pairs_full_dg(int n, int ng);
+std::size_t gep_lds_bytes(int n);
+bool gep_in_lds(int n);
+hipError_t configure_kernels(int n, int ng, int n_str, int mmax);
 
-void launch_orbital_prep(hipStream_t st, const double* C, const uint8_t* support, const double* S, const double* h,
-                         int n_bf, int n, double* Cn, double* SC, double* HC, double* nu2, int* flags);
-void launch_active_matrices(hipStream_t st, const double* Cn, const double* SC, const double* HC, int n_bf, int n,
-                            double* sact, double* hact, int* flags);
-void launch_ri_one_index(hipStream_t st, const double* B, const double* Cn, int n_bf, int n, int n_aux, double* T);
-void launch_ri_pair_stage(hipStream_t st, const double* T, const double* Cn, int n_bf, int n, int n_aux, double* L,
-                          double* G);
-void launch_pairs(hipStream_t st, bool backward, int n_blocks, const double* G, const double* sact, const double* hact,
-                  int n, int ng, const uint64_t* amask, const uint64_t* bmask, const int* det_a, const int* det_b,
-                  int n_det, double* Hdet, double* Sdet, const double* dcoef, const double* energy, double* partials,
-                  int* flags);
-void launch_reduce_partials(hipStream_t st, const double* partials, int n_blocks, int len, double* out);
-void launch_structure(hipStream_t st, const double* Hdet, const double* Sdet, int n_det, int n_str,
-                      const long long* soff, const int* sdet, const double* scoef, double* Yh, double* Ys, double* Hs,
-                      double* Ss);
-void launch_gep(hipStream_t st, const double* Hs, const double* Ss, int n, double e_nuc, double* work, double* coef,
-                double* energy, int* flags, const long long* off, const int* ts, const double* tc, int n_det,
-                double* dcoef);
-void launch_det_coefficients(hipStream_t st, const long long* off, const int* ts, const double* tc, const double* coef,
-                             int n_det, double* a);
-void launch_gradient_tail(hipStream_t st, const double* T, const double* L, const double* dsum, const double* SC,
-                          const double* HC, const double* Cn, const double* nu2, const uint8_t* support, int n_bf,
-                          int n, int n_aux, double* dL, double* gcn, double* grad);
+// forward: orbital prep, RI transform, determinant pairs, structure H/S, eigensolve (+ a_d)
+void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape,
+                    hipEvent_t* ev);
+// backward: pair adjoints, partial reduction, RI/orbital pullback to the raw-slot gradient
+void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape,
+                     hipEvent_t* ev);
 
 }  // namespace xmvb_amd

@@ -42,6 +42,8 @@ EXPORTED_SYMBOLS = (
     "vbe_context_create", "vbe_context_destroy", "vbe_last_error", "vbe_global_error", "vbe_problem_upload",
     "vbe_problem_release", "vbe_problem_device_bytes", "vbe_eval", "vbe_eval_device", "vbe_eval_forward",
     "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
+    "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_eval_device",
+    "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_phase_times",
 )
 
 
@@ -113,6 +115,15 @@ def library() -> C.CDLL:
         lib.vbe_last_intermediates.argtypes = [P, P, P, P, P, P, P, P]
         lib.vbe_last_phase_times.argtypes = [P, C.POINTER(PhaseTimes)]
         lib.vbe_last_singular_pairs.argtypes = [P, C.POINTER(I64)]
+        lib.vbe_pool_create.argtypes = [P, C.POINTER(TopologyView), I32, I32, I32, I32, C.POINTER(P)]
+        lib.vbe_pool_destroy.argtypes = [P]
+        lib.vbe_pool_destroy.restype = None
+        lib.vbe_pool_load.argtypes = [P, I32, I32, I32, P, P, P, D]
+        lib.vbe_pool_device_bytes.argtypes = [P, C.POINTER(U64)]
+        lib.vbe_pool_eval_device.argtypes = [P, I32, P, P, I32, P, P]
+        lib.vbe_pool_submit.argtypes = [P, I32, P, P, I32, C.POINTER(I32)]
+        lib.vbe_pool_wait.argtypes = [P, I32, P, P]
+        lib.vbe_pool_phase_times.argtypes = [P, C.POINTER(PhaseTimes)]
         _lib = lib
     return _lib
 
@@ -347,6 +358,91 @@ class Problem:
     def close(self):
         if self._h:
             self.ctx._lib.vbe_problem_release(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pool:
+    """Resident problem slots sharing one structure topology, evaluated in batched launches
+    (one kernel per phase for all listed slots) and served through submit/wait tickets."""
+
+    def __init__(self, ctx: Context, topo: Topology, capacity: int):
+        self.ctx, self.topo, self.capacity = ctx, topo, capacity
+        self.n_act = topo.n_act
+        self._view = topo.view()
+        self._h = C.c_void_p()
+        self.n_bf = [0] * capacity
+        ctx.check(ctx._lib.vbe_pool_create(ctx._h, C.byref(self._view), topo.n_act, topo.n_alpha, topo.n_beta,
+                                           capacity, C.byref(self._h)))
+
+    def load(self, slot: int, n_bf: int, n_aux: int, S, h, B, e_nuc: float = 0.0):
+        S = np.asfortranarray(S, dtype=np.float64)
+        h = np.asfortranarray(h, dtype=np.float64)
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        self.ctx.check(self.ctx._lib.vbe_pool_load(self._h, slot, n_bf, n_aux, _ptr(S), _ptr(h), _ptr(B), e_nuc))
+        self.n_bf[slot] = n_bf
+
+    def load_synthetic(self, slot: int, sp: "SyntheticProblem", e_nuc: float = 0.0):
+        self.load(slot, sp.n_bf, sp.n_aux, sp.S, sp.h, sp.B, e_nuc)
+
+    def _sizes(self, slots):
+        return [self.n_act * self.n_bf[s] for s in slots]
+
+    def evaluate_device(self, slots, C_ptr: int, grad_ptr: int, gradient: bool = True) -> np.ndarray:
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        e = np.zeros(len(sl))
+        self.ctx.check(self.ctx._lib.vbe_pool_eval_device(self._h, len(sl), _ptr(sl), C.c_void_p(C_ptr),
+                                                          int(gradient), _ptr(e),
+                                                          C.c_void_p(grad_ptr) if gradient else None))
+        return e
+
+    def submit(self, slots, orbitals, gradient: bool = True) -> int:
+        """orbitals: list of (n_act, n_bf_slot) arrays in slot order."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        flat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(o, dtype=np.float64).ravel()
+                                                    for o in orbitals]))
+        t = C.c_int32()
+        self.ctx.check(self.ctx._lib.vbe_pool_submit(self._h, len(sl), _ptr(sl), _ptr(flat), int(gradient),
+                                                     C.byref(t)))
+        self._pending = getattr(self, "_pending", {})
+        self._pending[t.value] = (list(sl), gradient)
+        return t.value
+
+    def wait(self, ticket: int):
+        slots, gradient = self._pending.pop(ticket)
+        e = np.zeros(len(slots))
+        sizes = self._sizes(slots)
+        g = np.zeros(sum(sizes)) if gradient else None
+        self.ctx.check(self.ctx._lib.vbe_pool_wait(self._h, ticket, _ptr(e), _ptr(g)))
+        grads = None
+        if gradient:
+            grads, at = [], 0
+            for s, sz in zip(slots, sizes):
+                grads.append(g[at:at + sz].reshape(self.n_act, self.n_bf[s]))
+                at += sz
+        return e, grads
+
+    def evaluate(self, slots, orbitals, gradient: bool = True):
+        return self.wait(self.submit(slots, orbitals, gradient))
+
+    def phase_times(self) -> dict:
+        t = PhaseTimes()
+        self.ctx._lib.vbe_pool_phase_times(self._h, C.byref(t))
+        return {name: getattr(t, name) for name, _ in PhaseTimes._fields_}
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        self.ctx._lib.vbe_pool_device_bytes(self._h, C.byref(b))
+        return b.value
+
+    def close(self):
+        if self._h:
+            self.ctx._lib.vbe_pool_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):
