@@ -162,6 +162,15 @@ int vbe_context_create(int32_t device, uint64_t hbm_budget_bytes, vbe_context** 
 void vbe_context_destroy(vbe_context* ctx);
 /* Copy the last error of a context into buf (NUL-terminated, truncated to len). */
 int vbe_last_error(const vbe_context* ctx, char* buf, size_t len);
+/* Determinant pairs are evaluated one per lane with row-pivoted inversion when both spin blocks
+ * satisfy max|X| max|X^-1| < limit; the other pairs go through the complete-pivoting path that
+ * applies DeterminantOverlapResolver's FullPivLU rank rule (src/vbscf/determinants/algebra/
+ * overlap.cpp:50-110) directly. Any limit in [0, 1e9] keeps results within rounding of the
+ * reference (below 1e9 a row-pivoted block is provably full rank under that rule); 0 sends every
+ * pair through the complete-pivoting path. Default 1e5: worse-conditioned blocks then follow the
+ * reference's pivot order, which keeps their rounding close to it. Applies to later evaluations.
+ * No counterpart in the reference, which has a single (CPU) pair path. */
+int vbe_set_pair_condition_limit(vbe_context* ctx, double limit);
 const char* vbe_global_error(void);
 
 /* Upload the molecule-static data of one problem: AO overlap S and core Hamiltonian h

@@ -113,3 +113,88 @@ def test_normalisation_failure_is_reported(ctx):
         prob.evaluate(bad)
     assert err.value.status == 2 and "orbital 2" in str(err.value)
     prob.close()
+
+
+@pytest.mark.parametrize("kw,n_act,n_el,mult,n_bf,n_aux,seed", [
+    ("COV", 6, 6, 1, 24, 48, 21),   # 3x3 / 3x3 blocks
+    ("COV", 6, 4, 3, 24, 48, 22),   # triplet: 3x3 alpha, 1x1 beta blocks
+    ("COV", 7, 5, 2, 28, 56, 23),   # doublet: 3x3 alpha, 2x2 beta blocks
+    ("COV", 10, 10, 1, 40, 80, 24),  # 5x5 blocks
+])
+@pytest.mark.parametrize("limit", [0.0, 1.0e3, None])
+def test_pair_condition_limit(kw, n_act, n_el, mult, n_bf, n_aux, seed, limit):
+    """Pairs routed through the complete-pivoting path (limit 0: all of them; 1e3: the
+    worse-conditioned ones; None: the default 1e5) give the oracle's energy and gradient. The
+    doublet case holds one alpha block with max|X| max|X^-1| = 1.5e7."""
+    print(f"case {kw} n_act={n_act} n_el={n_el} mult={mult} seed={seed} limit={limit}", flush=True)
+    c = vb.Context(0)
+    if limit is not None:
+        c.set_pair_condition_limit(limit)
+    sp, topo, prob = _case(c, kw, n_act, n_el, n_bf, n_aux, seed, mult)
+    e, g = prob.evaluate(sp.C)
+    ref = oracle.evaluate(n_bf, n_aux, topo, sp.S, sp.h, sp.B, sp.C)
+    assert abs(e - ref["energy"]) <= E_TOL
+    assert np.abs(g - ref["grad"]).max() <= G_TOL
+    mid = prob.intermediates()
+    for key in ("H", "S"):
+        assert np.abs(mid[key] - ref[key]).max() <= 1e-10 * np.abs(ref[key]).max(), key
+    prob.close()
+    c.close()
+
+
+def test_pair_condition_limit_max():
+    """At the largest limit (1e9) the 1.5e7-conditioned doublet block is inverted with row
+    pivoting: the energy keeps the 1e-8 bar; the gradient differs from the complete-pivoting
+    oracle by rounding amplified by that conditioning (observed 1.3e-7, bound 1e-6)."""
+    c = vb.Context(0)
+    c.set_pair_condition_limit(1.0e9)
+    sp, topo, prob = _case(c, "COV", 7, 5, 28, 56, 23, 2)
+    e, g = prob.evaluate(sp.C)
+    ref = oracle.evaluate(28, 56, topo, sp.S, sp.h, sp.B, sp.C)
+    assert abs(e - ref["energy"]) <= E_TOL
+    assert np.abs(g - ref["grad"]).max() <= 1e-6
+    prob.close()
+    c.close()
+
+
+def test_pair_condition_limit_range():
+    c = vb.Context(0)
+    for bad in (-1.0, 2.0e9, float("nan")):
+        with pytest.raises(vb.VbeError):
+            c.set_pair_condition_limit(bad)
+    c.close()
+
+
+@pytest.mark.parametrize("kw,n_frag_bf,a,b,n_aux", [
+    ("COV", 5, 2, 2, 12),    # 2x2 blocks, nullity 1..2
+    ("FULL", 5, 2, 2, 12),
+    ("COV", 6, 3, 3, 14),    # 3x3 blocks, nullity up to 3
+    ("COV", 12, 5, 5, 40),   # 5x5 blocks: lane kernel, deferral, singular list
+    ("COV", 8, 4, 2, 30),    # unequal fragments
+])
+@pytest.mark.parametrize("limit", [0.0, None])
+def test_singular_pairs(kw, n_frag_bf, a, b, n_aux, limit):
+    """Two fragments with zero cross overlap: many determinant overlap blocks are exactly singular
+    and go through the SVD-chart path; energy and gradient equal the oracle's (cofactor forward,
+    chart-node backward, itself pinned by finite differences)."""
+    from problems import fragment_problem
+    print(f"case singular {kw} frag_bf={n_frag_bf} act={a}+{b} limit={limit}", flush=True)
+    n_bf, S, h, B, Cm, sup = fragment_problem(3, n_frag_bf, a, b, n_aux)
+    topo = vb.Topology.build(kw, a + b, a + b, 1)
+    c = vb.Context(0)
+    if limit is not None:
+        c.set_pair_condition_limit(limit)
+    prob = vb.Problem(c, n_bf, n_aux, topo, S, h, B)
+    e, g = prob.evaluate(Cm, support=sup)
+    ref = oracle.evaluate(n_bf, n_aux, topo, S, h, B, Cm, support=sup)
+    assert prob.singular_pairs() > 0
+    print(f"singular pairs {prob.singular_pairs()} dE={abs(e - ref['energy']):.2e} "
+          f"dg={np.abs(g - ref['grad']).max():.2e}", flush=True)
+    assert abs(e - ref["energy"]) <= E_TOL
+    assert np.abs(g - ref["grad"]).max() <= G_TOL
+    assert np.all(g[sup == 0] == 0.0)
+    mid = prob.intermediates()
+    for key in ("H", "S"):
+        assert np.abs(mid[key] - ref[key]).max() <= 1e-10 * np.abs(ref[key]).max(), key
+    prob.close()
+    c.close()

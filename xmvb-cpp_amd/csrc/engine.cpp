@@ -41,6 +41,7 @@ constexpr int kLanes = 2;              // streams of a pool (in-flight submissio
 constexpr int kPairBlocksFwd = 4096;   // target forward pair blocks per launch (all slots)
 constexpr int kPairBlocksBwd = 2048;   // target backward pair blocks per launch (all slots)
 constexpr int kMaxBwdBlocksPerSlot = 512;
+constexpr int kLaneBlocks = 2048;      // target one-pair-per-lane blocks per launch (all slots)
 
 }  // namespace
 
@@ -48,6 +49,7 @@ struct vbe_context {
     int device = 0;
     uint64_t budget = 0;
     uint64_t in_use = 0;
+    double pair_condition_limit = xmvb_amd::kLaneConditionDefault;
     std::string error;
 };
 
@@ -182,6 +184,8 @@ void validate_topology(int n_act, int n_alpha, int n_beta, const vbe_topology_vi
     if (n_alpha > n_act || n_beta > n_act) throw StatusError(VBE_ERR_ARGUMENT, "more electrons than orbitals");
     if (t.n_structures <= 0 || t.n_determinants <= 0 || t.n_alpha_strings <= 0 || t.n_beta_strings <= 0)
         throw StatusError(VBE_ERR_ARGUMENT, "empty topology");
+    if ((long long)t.n_determinants * (t.n_determinants + 1) / 2 > 0x7fffffffLL)
+        throw StatusError(VBE_ERR_UNSUPPORTED, "more than 2^31 determinant pairs are not supported");
     if (!t.alpha_masks || !t.beta_masks || !t.det_alpha || !t.det_beta || !t.term_offsets || !t.term_structure ||
         !t.term_coefficient)
         throw StatusError(VBE_ERR_ARGUMENT, "null topology array");
@@ -346,10 +350,13 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
         d.coef = A.alloc<double>(ns);
         d.dcoef = A.alloc<double>(nd);
         const size_t len = (size_t)T.ng + 2 * n * n;
-        d.partials = A.alloc<double>((size_t)bwd_blocks_cap(p) * len);
+        d.partials =
+            A.alloc<double>((size_t)(bwd_blocks_cap(p) + xmvb_amd::kListBlocks + xmvb_amd::kSingularBlocks) * len);
         d.dsum = A.alloc<double>(len);
         d.dL = A.alloc<double>((size_t)n_aux * np);
         d.gcn = A.alloc<double>(n * n_bf);
+        d.defer = A.alloc<int>((size_t)pair_count(p));
+        d.singular = A.alloc<int>((size_t)pair_count(p));
         s.C_buf = A.alloc<double>(n * n_bf);
         s.grad_buf = A.alloc<double>(n * n_bf);
         s.support_buf = A.alloc<uint8_t>(n * n_bf);
@@ -391,6 +398,17 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
     sh.pair_blocks_fwd = (int)std::clamp<long long>((kPairBlocksFwd + count - 1) / count, 1,
                                                     std::max<long long>(1, (pairs + 4 * per_sweep - 1) / (4 * per_sweep)));
     sh.pair_blocks_bwd = (int)std::clamp<long long>((kPairBlocksBwd + count - 1) / count, 1, bwd_blocks_cap(p));
+    // one pair per lane: 256 pairs per block sweep
+    sh.n_alpha = p->n_alpha;
+    sh.n_beta = p->n_beta;
+    sh.lane_fwd = xmvb_amd::pairs_lane_supported(p->n_alpha, p->n_beta, false);
+    sh.lane_bwd = xmvb_amd::pairs_lane_supported(p->n_alpha, p->n_beta, true);
+    const long long lane_sweep = std::max<long long>(1, (pairs + 255) / 256);
+    sh.lane_blocks_fwd = (int)std::clamp<long long>((kLaneBlocks + count - 1) / count, 1, lane_sweep);
+    sh.lane_blocks_bwd = (int)std::clamp<long long>((kLaneBlocks + count - 1) / count, 1,
+                                                    std::min<long long>(lane_sweep, bwd_blocks_cap(p)));
+    sh.list_blocks = xmvb_amd::kListBlocks;
+    sh.lane_cond_limit = p->ctx->pair_condition_limit;
     return sh;
 }
 
@@ -473,12 +491,6 @@ void finish(vbe_pool* p, Lane& lane) {
         } else if (f[FLAG_ACTIVE_OVERLAP]) {
             status = VBE_ERR_NUMERIC;
             err = "active orbital overlap matrix is not positive definite" + where;
-        } else if (f[FLAG_SINGULAR_PAIRS]) {
-            status = VBE_ERR_UNSUPPORTED;
-            err = std::to_string(f[FLAG_SINGULAR_PAIRS]) +
-                  " determinant pairs have rank-deficient overlap blocks; the singular-pair device path is not "
-                  "implemented" +
-                  where;
         } else if (f[FLAG_METRIC]) {
             status = VBE_ERR_METRIC;
             err = "structure metric is not positive definite" + where;
@@ -564,6 +576,14 @@ int vbe_context_create(int32_t device, uint64_t hbm_budget_bytes, vbe_context** 
 }
 
 void vbe_context_destroy(vbe_context* ctx) { delete ctx; }
+
+int vbe_set_pair_condition_limit(vbe_context* ctx, double limit) {
+    if (!ctx) return fail(nullptr, VBE_ERR_ARGUMENT, "null context");
+    if (!(limit >= 0.0 && limit <= xmvb_amd::kLaneConditionMax))
+        return fail(ctx, VBE_ERR_ARGUMENT, "pair condition limit must lie in [0, 1e9]");
+    ctx->pair_condition_limit = limit;
+    return VBE_OK;
+}
 
 int vbe_last_error(const vbe_context* ctx, char* buf, size_t len) {
     if (!buf || len == 0) return VBE_ERR_ARGUMENT;

@@ -19,14 +19,11 @@
 
 #include <cstdint>
 
+#include "device.hpp"
 #include "kernels.hpp"
 
 namespace xmvb_amd {
 
-__device__ __forceinline__ int pair_index(int a, int b) {
-    const int hi = a > b ? a : b, lo = a > b ? b : a;
-    return hi * (hi + 1) / 2 + lo;
-}
 
 // wave-uniform broadcast of lane `src`
 __device__ __forceinline__ double wave_bcast(double v, int src) {
@@ -202,35 +199,71 @@ __global__ void k_active_matrices(const SlotDesc* __restrict__ slots, int n) {
 }
 
 // T[A, q, mu] = sum_nu B[A, (mu nu)] Cn[nu, q]; T index A + n_aux*(q + n*mu).
-// grid (ceil(n_aux/64), n_bf), four wavefronts per block splitting nu; lanes run over A so every
-// packed B column is read as contiguous 512-byte segments (each column is read by its two AO rows).
+// grid (ceil(n_aux/64), ceil(n_bf/16), slots), four wavefronts per block. A wavefront owns a 64-wide
+// aux tile (lanes run over A, so every packed B column is read as one contiguous 512-byte segment)
+// and four AO rows mu, so each Cn[nu, q] read from LDS feeds four accumulators. Every column is read
+// by both of its AO rows; the second read usually comes from the last-level cache because the
+// blocks of one slot are dispatched together. Cn is staged per chunk of nu as [nu][16] with zeros
+// beyond n, so NQ (n rounded up to even) accumulators per row need no guards.
+constexpr int kOneIndexChunk = 64;
+constexpr int kOneIndexRows = 4;  // AO rows per wavefront
+
+template <int NQ>
 __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
-    if ((int)blockIdx.y >= n_bf || (int)blockIdx.x * 64 >= n_aux) return;  // block-uniform
-    const double* __restrict__ B = sd.B;
+    if ((int)blockIdx.y * 4 * kOneIndexRows >= n_bf || (int)blockIdx.x * 64 >= n_aux) return;  // block-uniform
     const double* __restrict__ Cn = sd.Cn;
-    double* __restrict__ T = sd.T;
-    __shared__ double part[4][MAX_ACT][64];
+    __shared__ double cT[kOneIndexChunk][16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int A = blockIdx.x * 64 + lane;
-    const int mu = blockIdx.y;
-    const bool active = A < n_aux;
-    double acc[MAX_ACT];
+    const int Ac = A < n_aux ? A : n_aux - 1;  // clamped: loads stay in bounds, results are dropped
+    const int mu0 = (blockIdx.y * 4 + wave) * kOneIndexRows;
+    const bool rows = mu0 < n_bf;            // wave-uniform
+    int mu[kOneIndexRows];
 #pragma unroll
-    for (int q = 0; q < MAX_ACT; ++q) acc[q] = 0.0;
-    for (int nu = wave; nu < n_bf; nu += 4) {
-        const double b = active ? B[(size_t)A + (size_t)n_aux * (size_t)pair_index(mu, nu)] : 0.0;
+    for (int u = 0; u < kOneIndexRows; ++u) mu[u] = min(mu0 + u, n_bf - 1);
+    const double* __restrict__ Bp = sd.B + Ac;
+    double acc[kOneIndexRows][NQ];
 #pragma unroll
-        for (int q = 0; q < MAX_ACT; ++q)
-            if (q < n) acc[q] += b * Cn[(size_t)q * n_bf + nu];
+    for (int u = 0; u < kOneIndexRows; ++u)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[u][q] = 0.0;
+    for (int c0 = 0; c0 < n_bf; c0 += kOneIndexChunk) {
+        const int len = min(kOneIndexChunk, n_bf - c0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < kOneIndexChunk * 16; t += blockDim.x) {
+            const int v = t >> 4, q = t & 15;
+            cT[v][q] = (q < n && v < len) ? Cn[(size_t)q * n_bf + c0 + v] : 0.0;
+        }
+        __syncthreads();
+        if (!rows) continue;
+        for (int v = 0; v < len; v += 2) {
+            // two nu per step (the second is a zero-weight repeat of the first past the chunk end)
+            const int nu1 = c0 + v, nu2 = c0 + min(v + 1, len - 1);
+            double b1[kOneIndexRows], b2[kOneIndexRows];
+#pragma unroll
+            for (int u = 0; u < kOneIndexRows; ++u) {
+                b1[u] = Bp[(size_t)n_aux * pair_index(mu[u], nu1)];
+                b2[u] = Bp[(size_t)n_aux * pair_index(mu[u], nu2)];
+            }
+            const double* c1 = cT[v];
+            const double* c2 = cT[v + 1];  // row len (if reached) is all zeros
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const double x1 = c1[q], x2 = c2[q];
+#pragma unroll
+                for (int u = 0; u < kOneIndexRows; ++u) acc[u][q] += b1[u] * x1 + b2[u] * x2;
+            }
+        }
     }
+    if (!rows || A >= n_aux) return;
 #pragma unroll
-    for (int q = 0; q < MAX_ACT; ++q) part[wave][q][lane] = acc[q];
-    __syncthreads();
-    for (int q = wave; q < n; q += 4) {
-        const double v = (part[0][q][lane] + part[1][q][lane]) + (part[2][q][lane] + part[3][q][lane]);
-        if (active) T[(size_t)A + (size_t)n_aux * (q + (size_t)n * mu)] = v;
+    for (int u = 0; u < kOneIndexRows; ++u) {
+        if (mu0 + u >= n_bf) break;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            if (q < n) sd.T[(size_t)A + (size_t)n_aux * (q + (size_t)n * (mu0 + u))] = acc[u][q];
     }
 }
 
@@ -428,8 +461,10 @@ constexpr size_t slab_bytes_per_wave() {
 // Backward: adjoints of S_act, h_act and the packed kernel for F = w (H_de - E S_de),
 // accumulated per block in LDS (full np x np when FullDg, packed otherwise) and written as
 // packed partials.
-template <bool Backward, bool FullDg, int GROUP>
-__global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slots, TopoDesc topo) {
+// Listed: evaluate only the pairs of the slot's deferred list (left by the one-pair-per-lane kernel);
+// backward partial rows then start at part_offset.
+template <bool Backward, bool FullDg, int GROUP, bool Listed>
+__global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slots, TopoDesc topo, int part_offset) {
     constexpr int MB = GroupShape<GROUP>::MB;
     constexpr int GPW = 64 / GROUP;  // groups per wavefront
     const SlotDesc& sd = slots[blockIdx.z];
@@ -440,8 +475,8 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
     const uint64_t* __restrict__ bmask = topo.bmask;
     const int* __restrict__ det_a = topo.det_a;
     const int* __restrict__ det_b = topo.det_b;
-    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
     int* __restrict__ flags = sd.flags;
+    const long long n_pairs = Listed ? (long long)flags[FLAG_DEFERRED] : (long long)n_det * (n_det + 1) / 2;
 
     extern __shared__ double lds[];
     double* G2 = lds;
@@ -456,14 +491,14 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
         tail = dh + n * n;
     }
     GroupSlab<GROUP>* slabs = reinterpret_cast<GroupSlab<GROUP>*>(tail);
-    {
+    if (n_pairs > 0) {
         const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, ln = threadIdx.x & 63;
         for (int Q = wave; Q < np; Q += nwaves)
             for (int P = ln; P < np; P += 64) G2[P + np * Q] = G_glob[pair_index(P, Q)];
-    }
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-        sact[t] = sd.sact[t];
-        hact[t] = sd.hact[t];
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            sact[t] = sd.sact[t];
+            hact[t] = sd.hact[t];
+        }
     }
     if (Backward) {
         const int ndg = FullDg ? np * np : ng;
@@ -487,17 +522,12 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
     const int m[2] = {__builtin_popcountll(amask[0]), __builtin_popcountll(bmask[0])};
 
     for (long long tb = first; tb < n_pairs; tb += stride) {
-        const long long t = tb + grp;
-        bool live = t < n_pairs;
+        const long long ti = tb + grp;
+        bool live = ti < n_pairs;
         int d = 0, ee = 0;
-        if (live) {
-            // unordered pair t = e(e+1)/2 + d, d <= e; d is the left (bra) determinant
-            long long e = (long long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-            while (e * (e + 1) / 2 > t) --e;
-            while ((e + 1) * (e + 2) / 2 <= t) ++e;
-            d = (int)(t - e * (e + 1) / 2);
-            ee = (int)e;
-        }
+        // unordered pair t = e(e+1)/2 + d, d <= e; d is the left (bra) determinant
+        const long long tp = live ? (Listed ? (long long)sd.defer[ti] : ti) : 0;
+        if (live) unpack_pair(tp, d, ee);
         double weight = 0.0;
         if (Backward) {
             weight = live ? sd.dcoef[d] * sd.dcoef[ee] * (d == ee ? 1.0 : 2.0) : 0.0;
@@ -537,8 +567,10 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
             }
             gs.D[s][lig] = dens[s];
         }
-        if (live && singular) {
-            if (lig == 0) atomicAdd(&flags[FLAG_SINGULAR_PAIRS], 1);
+        if (!Backward && live && singular && lig == 0) {
+            // rank-deficient under the complete-pivoting rule: evaluated by k_pairs_singular
+            const int at = atomicAdd(&flags[FLAG_SINGULAR_PAIRS], 1);
+            sd.singular[at] = (int)tp;
         }
         live = live && !singular;
         wave_sync();
@@ -680,13 +712,11 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
     if (Backward) {
         __syncthreads();
         const int stride_part = ng + 2 * n * n;
-        double* out = sd.partials + (size_t)blockIdx.x * stride_part;
+        double* out = sd.partials + (size_t)(blockIdx.x + part_offset) * stride_part;
         if (FullDg) {
             for (int t = threadIdx.x; t < ng; t += blockDim.x) {
-                int P = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-                while (P * (P + 1) / 2 > t) --P;
-                while ((P + 1) * (P + 2) / 2 <= t) ++P;
-                const int Q = t - P * (P + 1) / 2;
+                int Q, P;
+                unpack_pair(t, Q, P);
                 out[t] = (P == Q) ? dg[P + np * P] : dg[P + np * Q] + dg[Q + np * P];
             }
         } else {
@@ -1173,23 +1203,44 @@ int pairs_waves(int n, int ng, bool backward, int group) {
     return pairs_lds_bytes(n, ng, backward, 8, group) <= kPairLdsLimit ? 8 : 4;
 }
 
-template <bool B, bool F>
+template <bool B, bool F, bool L>
 hipError_t set_pairs_lds(int group, size_t bytes) {
-    const void* fn = group == 16   ? reinterpret_cast<const void*>(&k_pairs<B, F, 16>)
-                     : group == 32 ? reinterpret_cast<const void*>(&k_pairs<B, F, 32>)
-                                   : reinterpret_cast<const void*>(&k_pairs<B, F, 64>);
+    const void* fn = group == 16   ? reinterpret_cast<const void*>(&k_pairs<B, F, 16, L>)
+                     : group == 32 ? reinterpret_cast<const void*>(&k_pairs<B, F, 32, L>)
+                                   : reinterpret_cast<const void*>(&k_pairs<B, F, 64, L>);
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <bool B, bool F>
+template <bool B, bool F, bool L>
 void launch_pairs_variant(hipStream_t st, int group, dim3 grid, int threads, size_t lds, const SlotDesc* slots,
-                          const TopoDesc& topo) {
+                          const TopoDesc& topo, int part_offset) {
     if (group == 16)
-        hipLaunchKernelGGL((k_pairs<B, F, 16>), grid, dim3(threads), lds, st, slots, topo);
+        hipLaunchKernelGGL((k_pairs<B, F, 16, L>), grid, dim3(threads), lds, st, slots, topo, part_offset);
     else if (group == 32)
-        hipLaunchKernelGGL((k_pairs<B, F, 32>), grid, dim3(threads), lds, st, slots, topo);
+        hipLaunchKernelGGL((k_pairs<B, F, 32, L>), grid, dim3(threads), lds, st, slots, topo, part_offset);
     else
-        hipLaunchKernelGGL((k_pairs<B, F, 64>), grid, dim3(threads), lds, st, slots, topo);
+        hipLaunchKernelGGL((k_pairs<B, F, 64, L>), grid, dim3(threads), lds, st, slots, topo, part_offset);
+}
+
+// group kernel over all pairs, or over the deferred list after the lane kernel
+void launch_pairs_group(hipStream_t st, bool backward, bool listed, int blocks, int part_offset, const SlotDesc* slots,
+                        int count, const TopoDesc& topo, int mmax) {
+    const int n = topo.n, ng = topo.ng;
+    const int group = pair_group(mmax);
+    const int waves = pairs_waves(n, ng, backward, group);
+    const size_t lds = pairs_lds_bytes(n, ng, backward, waves, group);
+    const dim3 grid(blocks, 1, count);
+    const int thr = 64 * waves;
+    if (!backward) {
+        if (listed) launch_pairs_variant<false, false, true>(st, group, grid, thr, lds, slots, topo, part_offset);
+        else launch_pairs_variant<false, false, false>(st, group, grid, thr, lds, slots, topo, part_offset);
+    } else if (pairs_full_dg(n, ng)) {
+        if (listed) launch_pairs_variant<true, true, true>(st, group, grid, thr, lds, slots, topo, part_offset);
+        else launch_pairs_variant<true, true, false>(st, group, grid, thr, lds, slots, topo, part_offset);
+    } else {
+        if (listed) launch_pairs_variant<true, false, true>(st, group, grid, thr, lds, slots, topo, part_offset);
+        else launch_pairs_variant<true, false, false>(st, group, grid, thr, lds, slots, topo, part_offset);
+    }
 }
 
 // Raise the dynamic-LDS ceiling of every LDS-heavy kernel variant to its limit. The attribute is
@@ -1201,10 +1252,15 @@ hipError_t configure_kernels(int n, int ng, int n_str, int mmax) {
     (void)mmax;
     hipError_t e = hipSuccess;
     for (int group : {16, 32, 64}) {
-        if ((e = set_pairs_lds<false, false>(group, kPairLdsLimit)) != hipSuccess) return e;
-        if ((e = set_pairs_lds<true, true>(group, kPairLdsLimit)) != hipSuccess) return e;
-        if ((e = set_pairs_lds<true, false>(group, kPairLdsLimit)) != hipSuccess) return e;
+        if ((e = set_pairs_lds<false, false, false>(group, kPairLdsLimit)) != hipSuccess) return e;
+        if ((e = set_pairs_lds<true, true, false>(group, kPairLdsLimit)) != hipSuccess) return e;
+        if ((e = set_pairs_lds<true, false, false>(group, kPairLdsLimit)) != hipSuccess) return e;
+        if ((e = set_pairs_lds<false, false, true>(group, kPairLdsLimit)) != hipSuccess) return e;
+        if ((e = set_pairs_lds<true, true, true>(group, kPairLdsLimit)) != hipSuccess) return e;
+        if ((e = set_pairs_lds<true, false, true>(group, kPairLdsLimit)) != hipSuccess) return e;
     }
+    if ((e = configure_pairs_lane()) != hipSuccess) return e;
+    if ((e = configure_pairs_singular()) != hipSuccess) return e;
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gep_lowest<true>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGepLdsLimit);
 }
@@ -1225,15 +1281,30 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
                        slots, n);
     hipLaunchKernelGGL(k_active_matrices, dim3(1, 1, z), dim3(256), 0, st, slots, n);
     mark(EV_PREP);
-    hipLaunchKernelGGL(k_ri_one_index, dim3((shape.max_n_aux + 63) / 64, shape.max_n_bf, z), dim3(256), 0, st, slots, n);
+    {
+        const dim3 grid((shape.max_n_aux + 63) / 64, (shape.max_n_bf + 4 * kOneIndexRows - 1) / (4 * kOneIndexRows), z);
+        switch ((n + 1) / 2) {
+            case 1: hipLaunchKernelGGL(k_ri_one_index<2>, grid, dim3(256), 0, st, slots, n); break;
+            case 2: hipLaunchKernelGGL(k_ri_one_index<4>, grid, dim3(256), 0, st, slots, n); break;
+            case 3: hipLaunchKernelGGL(k_ri_one_index<6>, grid, dim3(256), 0, st, slots, n); break;
+            case 4: hipLaunchKernelGGL(k_ri_one_index<8>, grid, dim3(256), 0, st, slots, n); break;
+            case 5: hipLaunchKernelGGL(k_ri_one_index<10>, grid, dim3(256), 0, st, slots, n); break;
+            case 6: hipLaunchKernelGGL(k_ri_one_index<12>, grid, dim3(256), 0, st, slots, n); break;
+            default: hipLaunchKernelGGL(k_ri_one_index<14>, grid, dim3(256), 0, st, slots, n); break;
+        }
+    }
     mark(EV_ONE_INDEX);
     hipLaunchKernelGGL(k_ri_pair_factors, dim3((shape.max_n_aux + 63) / 64, np, z), dim3(64), 0, st, slots, n);
     hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4, 1, z), dim3(256), 0, st, slots, np);
     mark(EV_TRANSFORM);
-    const int group = pair_group(shape.mmax);
-    const int waves_f = pairs_waves(n, ng, false, group);
-    const size_t lds_f = pairs_lds_bytes(n, ng, false, waves_f, group);
-    launch_pairs_variant<false, false>(st, group, dim3(shape.pair_blocks_fwd, 1, z), 64 * waves_f, lds_f, slots, topo);
+    if (shape.lane_fwd) {
+        launch_pairs_lane(st, false, shape.lane_blocks_fwd, slots, count, topo, shape.n_alpha, shape.n_beta,
+                          shape.lane_cond_limit);
+        launch_pairs_group(st, false, true, shape.list_blocks, 0, slots, count, topo, shape.mmax);
+    } else {
+        launch_pairs_group(st, false, false, shape.pair_blocks_fwd, 0, slots, count, topo, shape.mmax);
+    }
+    launch_pairs_singular(st, false, 0, slots, count, topo);
     mark(EV_PAIR_FWD);
     const long long a = (long long)topo.n_det * topo.n_str, b = (long long)topo.n_str * topo.n_str;
     hipLaunchKernelGGL(k_structure_right, dim3((unsigned)((a + 255) / 256), 1, z), dim3(256), 0, st, slots, topo);
@@ -1256,18 +1327,20 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
         if (ev) (void)hipEventRecord(ev[k], st);
     };
     mark(EV_BWD_BEGIN);
-    const int group = pair_group(shape.mmax);
-    const int waves_b = pairs_waves(n, ng, true, group);
-    const size_t lds_b = pairs_lds_bytes(n, ng, true, waves_b, group);
-    if (pairs_full_dg(n, ng))
-        launch_pairs_variant<true, true>(st, group, dim3(shape.pair_blocks_bwd, 1, z), 64 * waves_b, lds_b, slots,
-                                         topo);
-    else
-        launch_pairs_variant<true, false>(st, group, dim3(shape.pair_blocks_bwd, 1, z), 64 * waves_b, lds_b, slots,
-                                          topo);
+    int part_rows = shape.pair_blocks_bwd;
+    if (shape.lane_bwd) {
+        launch_pairs_lane(st, true, shape.lane_blocks_bwd, slots, count, topo, shape.n_alpha, shape.n_beta,
+                          shape.lane_cond_limit);
+        launch_pairs_group(st, true, true, shape.list_blocks, shape.lane_blocks_bwd, slots, count, topo, shape.mmax);
+        part_rows = shape.lane_blocks_bwd + shape.list_blocks;
+    } else {
+        launch_pairs_group(st, true, false, shape.pair_blocks_bwd, 0, slots, count, topo, shape.mmax);
+    }
+    launch_pairs_singular(st, true, part_rows, slots, count, topo);
+    part_rows += kSingularBlocks;
     mark(EV_BWD_KERNEL);
     const int len = ng + 2 * n * n;
-    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, shape.pair_blocks_bwd, len);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, part_rows, len);
     mark(EV_PAIR_BWD);
     hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((shape.max_n_aux + 63) / 64, np, z), dim3(64), 0, st, slots, np);
     hipLaunchKernelGGL(k_orbital_gradient, dim3(shape.max_n_bf, 1, z), dim3(256), 0, st, slots, n, ng);

@@ -22,8 +22,9 @@ constexpr int MAX_SPIN_BLOCK = 8;
 enum DeviceFlag : int {
     FLAG_NORMALIZATION = 0,   // 1 + index of an orbital whose S-norm vanished
     FLAG_ACTIVE_OVERLAP = 1,  // S_act not positive definite
-    FLAG_SINGULAR_PAIRS = 2,  // rank-deficient determinant overlap blocks met
+    FLAG_SINGULAR_PAIRS = 2,  // pairs with a rank-deficient overlap block (SlotDesc::singular)
     FLAG_METRIC = 3,          // structure metric not positive definite
+    FLAG_DEFERRED = 4,        // pairs left by the lane kernel for the lane-group kernel (SlotDesc::defer)
     FLAG_COUNT = 8
 };
 
@@ -50,6 +51,8 @@ struct SlotDesc {
     double *Cn, *SC, *HC, *nu2, *sact, *hact, *T, *L, *G;
     double *Hdet, *Sdet, *Yh, *Ys, *Hs, *Ss, *gep, *coef, *dcoef;
     double *partials, *dsum, *dL, *gcn;
+    int* defer;     // deferred unordered pair indices (capacity: all pairs)
+    int* singular;  // pairs with a rank-deficient block (capacity: all pairs)
 };
 
 struct TopoDesc {
@@ -70,11 +73,37 @@ struct LaunchShape {
     int max_n_bf, max_n_aux;
     int pair_blocks_fwd, pair_blocks_bwd;  // blocks per slot
     int mmax;                              // largest electron count of one spin
+    int n_alpha, n_beta;
+    bool lane_fwd, lane_bwd;               // one-pair-per-lane kernels for this block shape
+    int lane_blocks_fwd, lane_blocks_bwd;  // their blocks per slot
+    int list_blocks;                       // lane-group blocks per slot for the deferred pairs
+    double lane_cond_limit;                // lane kernel keeps blocks with max|X| max|X^-1| below this
 };
+
+// largest condition bound under which a row-pivoted lane block is provably regular under the
+// complete-pivoting rank rule (see pairs_lane.hip)
+constexpr double kLaneConditionMax = 1.0e9;
+// default bound: blocks conditioned worse than this follow the reference's pivot order exactly, so
+// their rounding matches the complete-pivoting LU closely (typically well under 1% of all pairs)
+constexpr double kLaneConditionDefault = 1.0e5;
+
+// blocks per slot that evaluate the deferred-pair list (and own partial rows after the lane kernel's)
+constexpr int kListBlocks = 4;
+// blocks per slot of the singular-pair kernel (partial rows after the pair kernels' rows)
+constexpr int kSingularBlocks = 4;
 
 inline std::size_t gep_work_doubles(int n) { return 3 * (std::size_t)n * n + 9 * (std::size_t)n; }
 
 int pair_group(int mmax);
+bool pairs_lane_supported(int n_alpha, int n_beta, bool backward);
+std::size_t pairs_lane_lds_bytes(int n, int ng, bool backward);
+hipError_t configure_pairs_lane();
+std::size_t pairs_singular_lds_bytes(int n, int ng, bool backward);
+hipError_t configure_pairs_singular();
+void launch_pairs_singular(hipStream_t st, bool backward, int part_offset, const SlotDesc* slots, int count,
+                           const TopoDesc& topo);
+void launch_pairs_lane(hipStream_t st, bool backward, int blocks, const SlotDesc* slots, int count,
+                       const TopoDesc& topo, int n_alpha, int n_beta, double cond_limit);
 bool pairs_full_dg(int n, int ng);
 std::size_t gep_lds_bytes(int n);
 bool gep_in_lds(int n);

@@ -1,0 +1,528 @@
+// Determinant pairs, one pair per lane.
+//
+// Each lane evaluates one unordered determinant pair entirely in registers: the two spin blocks
+// X_s(r, l) = S_act(R_r, L_l) are gathered from LDS, inverted by Gauss-Jordan elimination with row
+// pivoting, and the matrix element is assembled from the LDS-resident active kernel
+//   H_de = det X_a det X_b * Phi,  Phi = sum_s D_s.h + sum_s sum_{r<r2,l<l2} A_s minor_s + D_a.J[D_b],
+// the regular-inverse form of DeterminantPairEvaluator::combine_spin_pair_evaluations
+// (determinants/pairs/evaluator.cpp:354-384) with the same-spin Lowdin terms of
+// calc_same_spin_hamiltonian (determinants/algebra/hamiltonian.cpp:35-236) written over 2x2 minors.
+//
+// The reference decides regularity with a complete-pivoting LU (DeterminantOverlapResolver,
+// determinants/algebra/overlap.cpp:50-110: rank counts pivots above 1e-12 max|pivot|). A lane only
+// keeps a pair whose blocks satisfy max|X| max|X^-1| < cond_limit (at most 1e9): every
+// complete-pivoting pivot is at least 1/(m max|X^-1|) and none exceeds 16 max|X| for m <= 8
+// (Wilkinson's growth bound), so such a block has full rank under that rule. Any other pair is
+// appended to the slot's deferred list and evaluated by the lane-group kernel, which applies the
+// complete-pivoting rule itself.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace xmvb_amd {
+
+#ifndef LANE_WAVES
+#define LANE_WAVES
+#endif
+
+template <int M>
+struct LaneBlock {
+    static constexpr int K = M > 0 ? M : 1;
+    int R[K], L[K];  // occupied orbitals of the right (ket) and left (bra) strings
+    double a[K][K];  // X on entry; X^{-1} after inversion, so D[r][l] = a[l][r]
+    double det;
+};
+
+template <int M>
+__device__ __forceinline__ void occupied(uint32_t mask, int* out) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        out[k] = __builtin_ctz(mask);
+        mask &= mask - 1;
+    }
+}
+
+// In-place Gauss-Jordan inversion with row pivoting; the row interchanges are undone as column
+// interchanges in reverse order at the end. Returns whether the block is kept on the lane path.
+template <int M>
+__device__ __forceinline__ bool lane_invert(const double* __restrict__ sact, int n, double cond_limit, LaneBlock<M>& b) {
+    if constexpr (M == 0) {
+        b.det = 1.0;
+        return true;
+    } else {
+        auto& a = b.a;
+        double amax = 0.0;
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                a[i][j] = sact[b.R[i] + n * b.L[j]];
+                amax = fmax(amax, fabs(a[i][j]));
+            }
+        int piv[M];
+        double det = 1.0;
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            int p = k;
+            double best = fabs(a[k][k]);
+#pragma unroll
+            for (int i = k + 1; i < M; ++i) {
+                const double v = fabs(a[i][k]);
+                const bool g = v > best;
+                best = g ? v : best;
+                p = g ? i : p;
+            }
+            piv[k] = p;
+#pragma unroll
+            for (int i = k + 1; i < M; ++i) {
+                const bool s = p == i;
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    const double t = a[k][j];
+                    a[k][j] = s ? a[i][j] : t;
+                    a[i][j] = s ? t : a[i][j];
+                }
+            }
+            const double pv = a[k][k];
+            ok = ok && best > 0.0;
+            det = (p != k) ? -det * pv : det * pv;
+            const double inv = best > 0.0 ? 1.0 / pv : 0.0;
+            a[k][k] = 1.0;
+#pragma unroll
+            for (int j = 0; j < M; ++j) a[k][j] *= inv;
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                if (i == k) continue;
+                const double f = a[i][k];
+                a[i][k] = 0.0;
+#pragma unroll
+                for (int j = 0; j < M; ++j) a[i][j] -= f * a[k][j];
+            }
+        }
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k)
+#pragma unroll
+            for (int c = k + 1; c < M; ++c) {
+                const bool s = piv[k] == c;
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    const double t = a[i][k];
+                    a[i][k] = s ? a[i][c] : t;
+                    a[i][c] = s ? t : a[i][c];
+                }
+            }
+        double ymax = 0.0;
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) ymax = fmax(ymax, fabs(a[i][j]));
+        b.det = det;
+        return ok && amax * ymax < cond_limit;
+    }
+}
+
+// Same-spin two-electron energy over 2x2 minors: sum_{r<r2,l<l2} [(rl|r2l2) - (rl2|r2l)] minor.
+// pidx(r, l) is the packed pair index of block entry (R_r, L_l).
+template <int M, typename Idx>
+__device__ __forceinline__ double same_spin_energy(const double* __restrict__ G2, int np, const LaneBlock<M>& b,
+                                                   Idx pidx) {
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+#pragma unroll
+        for (int r2 = r + 1; r2 < M; ++r2)
+#pragma unroll
+            for (int l = 0; l < M; ++l)
+#pragma unroll
+                for (int l2 = l + 1; l2 < M; ++l2) {
+                    const double J = G2[pidx(r, l) * np + pidx(r2, l2)];
+                    const double K = G2[pidx(r, l2) * np + pidx(r2, l)];
+                    // D[r][l] = a[l][r]
+                    const double minor = b.a[l][r] * b.a[l2][r2] - b.a[l2][r] * b.a[l][r2];
+                    acc += (J - K) * minor;
+                }
+    return acc;
+}
+
+// Same-spin part of F = dPhi/dD: F[rl] += A(rl, r2l2) D[r2l2], A the antisymmetrised kernel.
+template <int M, typename Idx>
+__device__ __forceinline__ void same_spin_field(const double* __restrict__ G2, int np, const LaneBlock<M>& b, Idx pidx,
+                                                double (&F)[LaneBlock<M>::K][LaneBlock<M>::K]) {
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+#pragma unroll
+        for (int r2 = r + 1; r2 < M; ++r2)
+#pragma unroll
+            for (int l = 0; l < M; ++l)
+#pragma unroll
+                for (int l2 = l + 1; l2 < M; ++l2) {
+                    const double A = G2[pidx(r, l) * np + pidx(r2, l2)] - G2[pidx(r, l2) * np + pidx(r2, l)];
+                    F[r][l] += A * b.a[l2][r2];
+                    F[r2][l2] += A * b.a[l][r];
+                    F[r][l2] -= A * b.a[l][r2];
+                    F[r2][l] -= A * b.a[l2][r];
+                }
+}
+
+// Same-spin adjoint of the packed kernel: cw * minor at (rl, r2l2), minus it at (rl2, r2l).
+template <bool FullDg, int M, typename Idx>
+__device__ __forceinline__ void same_spin_dg(double* dg, int np, const LaneBlock<M>& b, Idx pidx, double cw) {
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+#pragma unroll
+        for (int r2 = r + 1; r2 < M; ++r2)
+#pragma unroll
+            for (int l = 0; l < M; ++l)
+#pragma unroll
+                for (int l2 = l + 1; l2 < M; ++l2) {
+                    const double v = cw * (b.a[l][r] * b.a[l2][r2] - b.a[l2][r] * b.a[l][r2]);
+                    if (FullDg) {
+                        atomicAdd(&dg[pidx(r, l) * np + pidx(r2, l2)], v);
+                        atomicAdd(&dg[pidx(r, l2) * np + pidx(r2, l)], -v);
+                    } else {
+                        atomicAdd(&dg[pair_index(pidx(r, l), pidx(r2, l2))], v);
+                        atomicAdd(&dg[pair_index(pidx(r, l2), pidx(r2, l))], -v);
+                    }
+                }
+}
+
+// ds += c [ (wH Phi + wS) D - wH D F^T D ], dh += c wH D over one spin block
+template <int M>
+__device__ __forceinline__ void overlap_adjoint(double* ds, double* dh, int n, const LaneBlock<M>& b,
+                                                const double (&F)[LaneBlock<M>::K][LaneBlock<M>::K], double c, double sc,
+                                                double wH) {
+#pragma unroll
+    for (int l = 0; l < M; ++l) {
+        double z[LaneBlock<M>::K];
+#pragma unroll
+        for (int a2 = 0; a2 < M; ++a2) {
+            double s = 0.0;
+#pragma unroll
+            for (int b2 = 0; b2 < M; ++b2) s += F[b2][a2] * b.a[l][b2];
+            z[a2] = s;
+        }
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+            double dfd = 0.0;
+#pragma unroll
+            for (int a2 = 0; a2 < M; ++a2) dfd += b.a[a2][r] * z[a2];
+            const double Drl = b.a[l][r];
+            const int at = b.R[r] + n * b.L[l];
+            atomicAdd(&ds[at], c * (sc * Drl - wH * dfd));
+            atomicAdd(&dh[at], c * wH * Drl);
+        }
+    }
+}
+
+// Launch-uniform view of the two spins: spin 0 holds M0 >= M1 electrons.
+struct SpinTables {
+    const uint64_t* mask0;
+    const uint64_t* mask1;
+    const int* det0;
+    const int* det1;
+};
+
+__device__ __forceinline__ SpinTables spin_tables(const TopoDesc& topo, int swap_spins) {
+    SpinTables t;
+    t.mask0 = swap_spins ? topo.bmask : topo.amask;
+    t.mask1 = swap_spins ? topo.amask : topo.bmask;
+    t.det0 = swap_spins ? topo.det_b : topo.det_a;
+    t.det1 = swap_spins ? topo.det_a : topo.det_b;
+    return t;
+}
+
+// LDS: full np x np kernel G2, S_act, h_act, then (backward) dg (full or packed), ds, dh.
+template <bool Backward, bool FullDg, int M0, int M1>
+__global__ void __launch_bounds__(256) LANE_WAVES k_pairs_lane(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins,
+                                                              double cond_limit) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n = topo.n, ng = topo.ng, n_det = topo.n_det;
+    const int np = n * (n + 1) / 2;
+    extern __shared__ double lds[];
+    double* G2 = lds;
+    double* sact = G2 + np * np;
+    double* hact = sact + n * n;
+    double* dg = hact + n * n;
+    double* ds = dg + (FullDg ? np * np : ng);
+    double* dh = ds + n * n;
+    {
+        const double* __restrict__ G_glob = sd.G;
+        const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, ln = threadIdx.x & 63;
+        for (int Q = wave; Q < np; Q += nwaves)
+            for (int P = ln; P < np; P += 64) G2[P + np * Q] = G_glob[pair_index(P, Q)];
+    }
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        sact[t] = sd.sact[t];
+        hact[t] = sd.hact[t];
+    }
+    if (Backward) {
+        const int ndg = FullDg ? np * np : ng;
+        for (int t = threadIdx.x; t < ndg; t += blockDim.x) dg[t] = 0.0;
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            ds[t] = 0.0;
+            dh[t] = 0.0;
+        }
+    }
+    __syncthreads();
+    const SpinTables tab = spin_tables(topo, swap_spins);
+    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
+    const double E = Backward ? sd.energy[0] : 0.0;
+
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_pairs;
+         t += (long long)gridDim.x * blockDim.x) {
+        int d, e;
+        unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
+        double w = 0.0;
+        if (Backward) {
+            w = sd.dcoef[d] * sd.dcoef[e] * (d == e ? 1.0 : 2.0);
+            if (w == 0.0) continue;
+        }
+        LaneBlock<M0> b0;
+        LaneBlock<M1> b1;
+        occupied<M0>((uint32_t)tab.mask0[tab.det0[e]], b0.R);
+        occupied<M0>((uint32_t)tab.mask0[tab.det0[d]], b0.L);
+        occupied<M1>((uint32_t)tab.mask1[tab.det1[e]], b1.R);
+        occupied<M1>((uint32_t)tab.mask1[tab.det1[d]], b1.L);
+        const bool ok0 = lane_invert<M0>(sact, n, cond_limit, b0);
+        const bool ok1 = lane_invert<M1>(sact, n, cond_limit, b1);
+        if (!(ok0 && ok1)) {
+            if (!Backward) {
+                const int slot = atomicAdd(&sd.flags[FLAG_DEFERRED], 1);
+                sd.defer[slot] = (int)t;
+            }
+            continue;
+        }
+        // spin-1 packed indices are kept (the opposite-spin loops reuse each one M0^2 times);
+        // spin-0 indices are recomputed where needed, which costs VALU only
+        int P1[LaneBlock<M1>::K][LaneBlock<M1>::K];
+#pragma unroll
+        for (int r = 0; r < M1; ++r)
+#pragma unroll
+            for (int l = 0; l < M1; ++l) P1[r][l] = pair_index(b1.R[r], b1.L[l]);
+        auto pidx0 = [&](int r, int l) { return pair_index(b0.R[r], b0.L[l]); };
+        auto pidx1 = [&](int r, int l) { return P1[r][l]; };
+        const double c = b0.det * b1.det;
+        if (!Backward) {
+            double phi = 0.0;
+#pragma unroll
+            for (int r = 0; r < M0; ++r)
+#pragma unroll
+                for (int l = 0; l < M0; ++l) phi += b0.a[l][r] * hact[b0.R[r] + n * b0.L[l]];
+#pragma unroll
+            for (int r = 0; r < M1; ++r)
+#pragma unroll
+                for (int l = 0; l < M1; ++l) phi += b1.a[l][r] * hact[b1.R[r] + n * b1.L[l]];
+            phi += same_spin_energy<M0>(G2, np, b0, pidx0);
+            phi += same_spin_energy<M1>(G2, np, b1, pidx1);
+#pragma unroll
+            for (int r = 0; r < M0; ++r)
+#pragma unroll
+                for (int l = 0; l < M0; ++l) {
+                    const double* g = G2 + pidx0(r, l) * np;
+                    double inner = 0.0;
+#pragma unroll
+                    for (int r2 = 0; r2 < M1; ++r2)
+#pragma unroll
+                        for (int l2 = 0; l2 < M1; ++l2) inner += g[P1[r2][l2]] * b1.a[l2][r2];
+                    phi += b0.a[l][r] * inner;
+                }
+            const double H = c * phi;
+            sd.Hdet[(size_t)d + (size_t)n_det * e] = H;
+            sd.Hdet[(size_t)e + (size_t)n_det * d] = H;
+            sd.Sdet[(size_t)d + (size_t)n_det * e] = c;
+            sd.Sdet[(size_t)e + (size_t)n_det * d] = c;
+            continue;
+        }
+        // ---- backward: adjoints of F_pair = w (H - E S) ----
+        // Reads of the kernel (F blocks) come first and the LDS atomics after, so the reads are
+        // not held behind outstanding atomics.
+        const double wH = w, wS = -E * w, cw = c * wH;
+        {
+            double F0[LaneBlock<M0>::K][LaneBlock<M0>::K];
+            double F1[LaneBlock<M1>::K][LaneBlock<M1>::K];
+            double phi_h = 0.0;
+#pragma unroll
+            for (int r = 0; r < M0; ++r)
+#pragma unroll
+                for (int l = 0; l < M0; ++l) {
+                    F0[r][l] = hact[b0.R[r] + n * b0.L[l]];
+                    phi_h += b0.a[l][r] * F0[r][l];
+                }
+#pragma unroll
+            for (int r = 0; r < M1; ++r)
+#pragma unroll
+                for (int l = 0; l < M1; ++l) {
+                    F1[r][l] = hact[b1.R[r] + n * b1.L[l]];
+                    phi_h += b1.a[l][r] * F1[r][l];
+                }
+            same_spin_field<M0>(G2, np, b0, pidx0, F0);
+            same_spin_field<M1>(G2, np, b1, pidx1, F1);
+            // opposite spin: each kernel entry feeds both F blocks
+#pragma unroll
+            for (int r = 0; r < M0; ++r)
+#pragma unroll
+                for (int l = 0; l < M0; ++l) {
+                    const double* g = G2 + pidx0(r, l) * np;
+                    const double D0 = b0.a[l][r];
+                    double acc = 0.0;
+#pragma unroll
+                    for (int r2 = 0; r2 < M1; ++r2)
+#pragma unroll
+                        for (int l2 = 0; l2 < M1; ++l2) {
+                            const double gv = g[P1[r2][l2]];
+                            acc += gv * b1.a[l2][r2];
+                            F1[r2][l2] += gv * D0;
+                        }
+                    F0[r][l] += acc;
+                }
+            double phi_f = 0.0;
+#pragma unroll
+            for (int r = 0; r < M0; ++r)
+#pragma unroll
+                for (int l = 0; l < M0; ++l) phi_f += b0.a[l][r] * F0[r][l];
+#pragma unroll
+            for (int r = 0; r < M1; ++r)
+#pragma unroll
+                for (int l = 0; l < M1; ++l) phi_f += b1.a[l][r] * F1[r][l];
+            const double sc = wH * 0.5 * (phi_h + phi_f) + wS;
+            overlap_adjoint<M0>(ds, dh, n, b0, F0, c, sc, wH);
+            overlap_adjoint<M1>(ds, dh, n, b1, F1, c, sc, wH);
+        }
+        same_spin_dg<FullDg, M0>(dg, np, b0, pidx0, cw);
+        same_spin_dg<FullDg, M1>(dg, np, b1, pidx1, cw);
+#pragma unroll
+        for (int r = 0; r < M0; ++r)
+#pragma unroll
+            for (int l = 0; l < M0; ++l) {
+                const int P = pidx0(r, l);
+                const double cwD0 = cw * b0.a[l][r];
+#pragma unroll
+                for (int r2 = 0; r2 < M1; ++r2)
+#pragma unroll
+                    for (int l2 = 0; l2 < M1; ++l2) {
+                        if (FullDg) atomicAdd(&dg[P * np + P1[r2][l2]], cwD0 * b1.a[l2][r2]);
+                        else atomicAdd(&dg[pair_index(P, P1[r2][l2])], cwD0 * b1.a[l2][r2]);
+                    }
+            }
+    }
+    if (Backward) {
+        __syncthreads();
+        const int stride_part = ng + 2 * n * n;
+        double* out = sd.partials + (size_t)blockIdx.x * stride_part;
+        if (FullDg) {
+            for (int t = threadIdx.x; t < ng; t += blockDim.x) {
+                int Q, P;
+                unpack_pair(t, Q, P);
+                out[t] = (P == Q) ? dg[P + np * P] : dg[P + np * Q] + dg[Q + np * P];
+            }
+        } else {
+            for (int t = threadIdx.x; t < ng; t += blockDim.x) out[t] = dg[t];
+        }
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            out[ng + t] = ds[t];
+            out[ng + n * n + t] = dh[t];
+        }
+    }
+}
+
+// spin-block shapes with a lane kernel: M0 >= M1, M0 <= 6, M0 - M1 <= 2
+template <int M0, int M1>
+struct LaneShape {
+    static constexpr bool forward = M0 >= M1 && M0 <= 6 && M0 - M1 <= 2;
+    static constexpr bool backward = forward && M0 <= 5;
+};
+
+size_t pairs_lane_lds_bytes(int n, int ng, bool backward) {
+    const size_t np = (size_t)n * (n + 1) / 2;
+    size_t doubles = np * np + 2 * (size_t)n * n;
+    if (backward) doubles += (pairs_full_dg(n, ng) ? np * np : (size_t)ng) + 2 * (size_t)n * n;
+    return doubles * sizeof(double);
+}
+
+namespace {
+
+template <bool B, bool F, int M0, int M1>
+hipError_t lane_launch(bool run, hipStream_t st, dim3 grid, size_t lds, const SlotDesc* slots, const TopoDesc& topo,
+                       int swap, double cond) {
+    if (!run)
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_lane<B, F, M0, M1>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL((k_pairs_lane<B, F, M0, M1>), grid, dim3(256), lds, st, slots, topo, swap, cond);
+    return hipSuccess;
+}
+
+template <int M0, int M1>
+hipError_t lane_dispatch_shape(bool run, bool backward, bool full_dg, hipStream_t st, dim3 grid, size_t lds,
+                               const SlotDesc* slots, const TopoDesc& topo, int swap, double cond) {
+    if (!backward) return lane_launch<false, false, M0, M1>(run, st, grid, lds, slots, topo, swap, cond);
+    if constexpr (LaneShape<M0, M1>::backward) {
+        return full_dg ? lane_launch<true, true, M0, M1>(run, st, grid, lds, slots, topo, swap, cond)
+                       : lane_launch<true, false, M0, M1>(run, st, grid, lds, slots, topo, swap, cond);
+    }
+    return hipErrorInvalidValue;
+}
+
+// calls f.template operator()<M0, M1>() for the lane shape (m0, m1), or returns false
+template <typename Fn>
+bool for_lane_shape(int m0, int m1, Fn&& f) {
+    bool hit = false;
+    auto one = [&](auto a, auto b) {
+        constexpr int A = decltype(a)::value, Bm = decltype(b)::value;
+        if (!hit && A == m0 && Bm == m1) {
+            hit = true;
+            f.template operator()<A, Bm>();
+        }
+    };
+    using std::integral_constant;
+#define XMVB_LANE(A, B) one(integral_constant<int, A>{}, integral_constant<int, B>{})
+    XMVB_LANE(1, 0); XMVB_LANE(1, 1); XMVB_LANE(2, 0); XMVB_LANE(2, 1); XMVB_LANE(2, 2);
+    XMVB_LANE(3, 1); XMVB_LANE(3, 2); XMVB_LANE(3, 3); XMVB_LANE(4, 2); XMVB_LANE(4, 3);
+    XMVB_LANE(4, 4); XMVB_LANE(5, 3); XMVB_LANE(5, 4); XMVB_LANE(5, 5); XMVB_LANE(6, 4);
+    XMVB_LANE(6, 5); XMVB_LANE(6, 6);
+#undef XMVB_LANE
+    return hit;
+}
+
+}  // namespace
+
+bool pairs_lane_supported(int n_alpha, int n_beta, bool backward) {
+    const int m0 = n_alpha >= n_beta ? n_alpha : n_beta, m1 = n_alpha >= n_beta ? n_beta : n_alpha;
+    bool ok = false;
+    for_lane_shape(m0, m1, [&]<int A, int B>() { ok = backward ? LaneShape<A, B>::backward : LaneShape<A, B>::forward; });
+    return ok;
+}
+
+hipError_t configure_pairs_lane() {
+    hipError_t err = hipSuccess;
+    for (int m0 = 1; m0 <= 6; ++m0)
+        for (int m1 = 0; m1 <= m0; ++m1)
+            for_lane_shape(m0, m1, [&]<int A, int B>() {
+                for (int b = 0; b < 2; ++b)
+                    for (int f = 0; f < 2; ++f) {
+                        if (b == 0 && f == 1) continue;
+                        if (b && !LaneShape<A, B>::backward) continue;
+                        const hipError_t e =
+                            lane_dispatch_shape<A, B>(false, b, f, nullptr, dim3(), 0, nullptr, TopoDesc{}, 0, 0.0);
+                        if (e != hipSuccess) err = e;
+                    }
+            });
+    return err;
+}
+
+void launch_pairs_lane(hipStream_t st, bool backward, int blocks, const SlotDesc* slots, int count,
+                       const TopoDesc& topo, int n_alpha, int n_beta, double cond_limit) {
+    const int swap = n_beta > n_alpha ? 1 : 0;
+    const int m0 = swap ? n_beta : n_alpha, m1 = swap ? n_alpha : n_beta;
+    const bool full = pairs_full_dg(topo.n, topo.ng);
+    const size_t lds = pairs_lane_lds_bytes(topo.n, topo.ng, backward);
+    const dim3 grid(blocks, 1, count);
+    for_lane_shape(m0, m1, [&]<int A, int B>() {
+        (void)lane_dispatch_shape<A, B>(true, backward, full, st, grid, lds, slots, topo, swap, cond_limit);
+    });
+}
+
+}  // namespace xmvb_amd

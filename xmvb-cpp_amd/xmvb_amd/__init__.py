@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "vbe_problem_release", "vbe_problem_device_bytes", "vbe_eval", "vbe_eval_device", "vbe_eval_forward",
     "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
     "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_eval_device",
-    "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_phase_times",
+    "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_phase_times", "vbe_set_pair_condition_limit",
 )
 
 
@@ -104,6 +104,7 @@ def library() -> C.CDLL:
         lib.vbe_context_destroy.argtypes = [P]
         lib.vbe_context_destroy.restype = None
         lib.vbe_last_error.argtypes = [P, C.c_char_p, C.c_size_t]
+        lib.vbe_set_pair_condition_limit.argtypes = [P, D]
         lib.vbe_problem_upload.argtypes = [P, C.POINTER(Dims), P, P, P, C.POINTER(TopologyView), D, C.POINTER(P)]
         lib.vbe_problem_release.argtypes = [P]
         lib.vbe_problem_release.restype = None
@@ -260,6 +261,11 @@ class Context:
     def check(self, rc: int):
         if rc != 0:
             raise VbeError(rc, self.error())
+
+    def set_pair_condition_limit(self, limit: float):
+        """Pairs whose spin blocks have max|X| max|X^-1| >= limit (0..1e9) take the
+        complete-pivoting path; 0 sends every pair there."""
+        self.check(self._lib.vbe_set_pair_condition_limit(self._h, C.c_double(limit)))
 
     def close(self):
         if self._h:
