@@ -17,6 +17,7 @@
 // complete-pivoting rule itself.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "device.hpp"
@@ -235,52 +236,162 @@ __device__ __forceinline__ SpinTables spin_tables(const TopoDesc& topo, int swap
     return t;
 }
 
-// LDS: full np x np kernel G2, S_act, h_act, then (backward) dg (full or packed), ds, dh.
-template <bool Backward, bool FullDg, int M0, int M1>
-__global__ void __launch_bounds__(256) LANE_WAVES k_pairs_lane(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins,
-                                                              double cond_limit) {
+// Same-spin adjoint of the kernel in compound space: every same-spin kernel term of a pair is
+// [(R_r L_l|R_r2 L_l2) - (R_r L_l2|R_r2 L_l)] times a 2x2 minor of D with r < r2, l < l2, so the
+// adjoint is carried by W[(p,s),(q,t)] (p = R_r < s = R_r2, q = L_l < t = L_l2), one atomic per
+// minor, and expanded into dg once per block (see w_to_dg).
+__device__ __forceinline__ int compound_index(int p, int s) { return s * (s - 1) / 2 + p; }  // p < s
+
+template <int M>
+__device__ __forceinline__ void same_spin_w(double* W, int nc, const LaneBlock<M>& b, double cw) {
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+#pragma unroll
+        for (int r2 = r + 1; r2 < M; ++r2) {
+            const int row = compound_index(b.R[r], b.R[r2]) * nc;
+#pragma unroll
+            for (int l = 0; l < M; ++l)
+#pragma unroll
+                for (int l2 = l + 1; l2 < M; ++l2) {
+                    const double v = cw * (b.a[l][r] * b.a[l2][r2] - b.a[l2][r] * b.a[l][r2]);
+                    atomicAdd(&W[row + compound_index(b.L[l], b.L[l2])], v);
+                }
+        }
+}
+
+template <bool FullDg>
+__device__ __forceinline__ void dg_add(double* dg, int np, int P, int Q, double v) {
+    if (FullDg) atomicAdd(&dg[P * np + Q], v);
+    else atomicAdd(&dg[pair_index(P, Q)], v);
+}
+
+__device__ __forceinline__ void compound_pair(int i, int& p, int& s) {
+    s = (int)((1.0 + sqrt(1.0 + 8.0 * i)) * 0.5);
+    while (s * (s - 1) / 2 > i) --s;
+    while ((s + 1) * s / 2 <= i) ++s;
+    p = i - s * (s - 1) / 2;
+}
+
+// W[(p,s),(q,t)] -> +W at ((p,q),(s,t)) and -W at ((p,t),(s,q)); block-cooperative
+template <bool FullDg>
+__device__ void w_to_dg(const double* W, int nc, double* dg, int np) {
+    for (int idx = threadIdx.x; idx < nc * nc; idx += blockDim.x) {
+        const double v = W[idx];
+        if (v == 0.0) continue;
+        int p, s, q, t;
+        compound_pair(idx / nc, p, s);
+        compound_pair(idx % nc, q, t);
+        dg_add<FullDg>(dg, np, pair_index(p, q), pair_index(s, t), v);
+        dg_add<FullDg>(dg, np, pair_index(p, t), pair_index(s, q), -v);
+    }
+}
+
+typedef double lane_double4 __attribute__((ext_vector_type(4)));
+
+// Opposite-spin adjoint of the kernel as outer products on the matrix cores:
+//   dg[P, Q] += sum_pairs cw * ga[P] * gb[Q],
+// ga / gb the alpha / beta transition densities scattered to packed pair indices. The wavefront's
+// 64 pairs go through in four chunks of 16: the chunk's lanes scatter their rows into zeroed LDS
+// rows (accumulating where two block entries share a packed index), every 16x16 tile of dg is
+// formed by four v_mfma_f64_16x16x4 steps (A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D[i][j] in lane j + 16(i%4), element i/4) and added to the LDS accumulator, and the rows are
+// zeroed again. Called by all lanes of the wavefront.
+template <bool FullDg, int M0, int M1>
+__device__ void opposite_dg_mfma(double* bufA, double* bufB, int np16, int np, double* dg, bool live,
+                                 const LaneBlock<M0>& b0, const LaneBlock<M1>& b1,
+                                 const int (&P1)[LaneBlock<M1>::K][LaneBlock<M1>::K], double cw) {
+    const int lane = threadIdx.x & 63;
+    const int k4 = lane >> 4, i16 = lane & 15;
+    const int tiles = np16 >> 4;
+#pragma unroll 1
+    for (int chunk = 0; chunk < 4; ++chunk) {
+        const bool mine = live && k4 == chunk;
+        double* rowA = bufA + i16 * np16;
+        double* rowB = bufB + i16 * np16;
+        if (mine) {
+#pragma unroll
+            for (int r = 0; r < M0; ++r)
+#pragma unroll
+                for (int l = 0; l < M0; ++l) rowA[pair_index(b0.R[r], b0.L[l])] += b0.a[l][r];
+#pragma unroll
+            for (int r = 0; r < M1; ++r)
+#pragma unroll
+                for (int l = 0; l < M1; ++l) rowB[P1[r][l]] += cw * b1.a[l][r];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+        for (int tp = 0; tp < tiles; ++tp)
+#pragma unroll 1
+            for (int tq = 0; tq < tiles; ++tq) {
+                lane_double4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double av = bufA[(4 * kk + k4) * np16 + 16 * tp + i16];
+                    const double bv = bufB[(4 * kk + k4) * np16 + 16 * tq + i16];
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                }
+                const int Q = 16 * tq + i16;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int P = 16 * tp + 4 * v + k4;
+                    if (P < np && Q < np && acc[v] != 0.0) dg_add<FullDg>(dg, np, P, Q, acc[v]);
+                }
+            }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (mine) {
+#pragma unroll
+            for (int r = 0; r < M0; ++r)
+#pragma unroll
+                for (int l = 0; l < M0; ++l) rowA[pair_index(b0.R[r], b0.L[l])] = 0.0;
+#pragma unroll
+            for (int r = 0; r < M1; ++r)
+#pragma unroll
+                for (int l = 0; l < M1; ++l) rowB[P1[r][l]] = 0.0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Optional LDS regions of the backward lane kernel (0 = not used).
+struct LaneBwdLayout {
+    int nc;    // n(n-1)/2: compound dimension of the same-spin W accumulator
+    int np16;  // np rounded up to 16: row length of the MFMA staging rows (16 rows x 2 per wavefront)
+};
+
+__device__ __forceinline__ void load_pair_tables(const SlotDesc& sd, int n, int np, double* G2, double* sact,
+                                                 double* hact) {
+    const double* __restrict__ G_glob = sd.G;
+    const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, ln = threadIdx.x & 63;
+    for (int Q = wave; Q < np; Q += nwaves)
+        for (int P = ln; P < np; P += 64) G2[P + np * Q] = G_glob[pair_index(P, Q)];
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        sact[t] = sd.sact[t];
+        hact[t] = sd.hact[t];
+    }
+}
+
+// Forward. LDS: full np x np kernel G2, S_act, h_act.
+template <int M0, int M1>
+__global__ void __launch_bounds__(256) LANE_WAVES
+    k_pairs_lane_fwd(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const int n = topo.n, ng = topo.ng, n_det = topo.n_det;
+    const int n = topo.n, n_det = topo.n_det;
     const int np = n * (n + 1) / 2;
     extern __shared__ double lds[];
     double* G2 = lds;
     double* sact = G2 + np * np;
     double* hact = sact + n * n;
-    double* dg = hact + n * n;
-    double* ds = dg + (FullDg ? np * np : ng);
-    double* dh = ds + n * n;
-    {
-        const double* __restrict__ G_glob = sd.G;
-        const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, ln = threadIdx.x & 63;
-        for (int Q = wave; Q < np; Q += nwaves)
-            for (int P = ln; P < np; P += 64) G2[P + np * Q] = G_glob[pair_index(P, Q)];
-    }
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-        sact[t] = sd.sact[t];
-        hact[t] = sd.hact[t];
-    }
-    if (Backward) {
-        const int ndg = FullDg ? np * np : ng;
-        for (int t = threadIdx.x; t < ndg; t += blockDim.x) dg[t] = 0.0;
-        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-            ds[t] = 0.0;
-            dh[t] = 0.0;
-        }
-    }
+    load_pair_tables(sd, n, np, G2, sact, hact);
     __syncthreads();
     const SpinTables tab = spin_tables(topo, swap_spins);
     const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
-    const double E = Backward ? sd.energy[0] : 0.0;
-
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_pairs;
          t += (long long)gridDim.x * blockDim.x) {
         int d, e;
         unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
-        double w = 0.0;
-        if (Backward) {
-            w = sd.dcoef[d] * sd.dcoef[e] * (d == e ? 1.0 : 2.0);
-            if (w == 0.0) continue;
-        }
         LaneBlock<M0> b0;
         LaneBlock<M1> b1;
         occupied<M0>((uint32_t)tab.mask0[tab.det0[e]], b0.R);
@@ -290,13 +401,11 @@ __global__ void __launch_bounds__(256) LANE_WAVES k_pairs_lane(const SlotDesc* _
         const bool ok0 = lane_invert<M0>(sact, n, cond_limit, b0);
         const bool ok1 = lane_invert<M1>(sact, n, cond_limit, b1);
         if (!(ok0 && ok1)) {
-            if (!Backward) {
-                const int slot = atomicAdd(&sd.flags[FLAG_DEFERRED], 1);
-                sd.defer[slot] = (int)t;
-            }
+            const int slot = atomicAdd(&sd.flags[FLAG_DEFERRED], 1);
+            sd.defer[slot] = (int)t;
             continue;
         }
-        // spin-1 packed indices are kept (the opposite-spin loops reuse each one M0^2 times);
+        // spin-1 packed indices are kept (the opposite-spin loop reuses each one M0^2 times);
         // spin-0 indices are recomputed where needed, which costs VALU only
         int P1[LaneBlock<M1>::K][LaneBlock<M1>::K];
 #pragma unroll
@@ -306,126 +415,236 @@ __global__ void __launch_bounds__(256) LANE_WAVES k_pairs_lane(const SlotDesc* _
         auto pidx0 = [&](int r, int l) { return pair_index(b0.R[r], b0.L[l]); };
         auto pidx1 = [&](int r, int l) { return P1[r][l]; };
         const double c = b0.det * b1.det;
-        if (!Backward) {
-            double phi = 0.0;
+        double phi = 0.0;
 #pragma unroll
-            for (int r = 0; r < M0; ++r)
+        for (int r = 0; r < M0; ++r)
 #pragma unroll
-                for (int l = 0; l < M0; ++l) phi += b0.a[l][r] * hact[b0.R[r] + n * b0.L[l]];
+            for (int l = 0; l < M0; ++l) phi += b0.a[l][r] * hact[b0.R[r] + n * b0.L[l]];
 #pragma unroll
-            for (int r = 0; r < M1; ++r)
+        for (int r = 0; r < M1; ++r)
 #pragma unroll
-                for (int l = 0; l < M1; ++l) phi += b1.a[l][r] * hact[b1.R[r] + n * b1.L[l]];
-            phi += same_spin_energy<M0>(G2, np, b0, pidx0);
-            phi += same_spin_energy<M1>(G2, np, b1, pidx1);
-#pragma unroll
-            for (int r = 0; r < M0; ++r)
-#pragma unroll
-                for (int l = 0; l < M0; ++l) {
-                    const double* g = G2 + pidx0(r, l) * np;
-                    double inner = 0.0;
-#pragma unroll
-                    for (int r2 = 0; r2 < M1; ++r2)
-#pragma unroll
-                        for (int l2 = 0; l2 < M1; ++l2) inner += g[P1[r2][l2]] * b1.a[l2][r2];
-                    phi += b0.a[l][r] * inner;
-                }
-            const double H = c * phi;
-            sd.Hdet[(size_t)d + (size_t)n_det * e] = H;
-            sd.Hdet[(size_t)e + (size_t)n_det * d] = H;
-            sd.Sdet[(size_t)d + (size_t)n_det * e] = c;
-            sd.Sdet[(size_t)e + (size_t)n_det * d] = c;
-            continue;
-        }
-        // ---- backward: adjoints of F_pair = w (H - E S) ----
-        // Reads of the kernel (F blocks) come first and the LDS atomics after, so the reads are
-        // not held behind outstanding atomics.
-        const double wH = w, wS = -E * w, cw = c * wH;
-        {
-            double F0[LaneBlock<M0>::K][LaneBlock<M0>::K];
-            double F1[LaneBlock<M1>::K][LaneBlock<M1>::K];
-            double phi_h = 0.0;
-#pragma unroll
-            for (int r = 0; r < M0; ++r)
-#pragma unroll
-                for (int l = 0; l < M0; ++l) {
-                    F0[r][l] = hact[b0.R[r] + n * b0.L[l]];
-                    phi_h += b0.a[l][r] * F0[r][l];
-                }
-#pragma unroll
-            for (int r = 0; r < M1; ++r)
-#pragma unroll
-                for (int l = 0; l < M1; ++l) {
-                    F1[r][l] = hact[b1.R[r] + n * b1.L[l]];
-                    phi_h += b1.a[l][r] * F1[r][l];
-                }
-            same_spin_field<M0>(G2, np, b0, pidx0, F0);
-            same_spin_field<M1>(G2, np, b1, pidx1, F1);
-            // opposite spin: each kernel entry feeds both F blocks
-#pragma unroll
-            for (int r = 0; r < M0; ++r)
-#pragma unroll
-                for (int l = 0; l < M0; ++l) {
-                    const double* g = G2 + pidx0(r, l) * np;
-                    const double D0 = b0.a[l][r];
-                    double acc = 0.0;
-#pragma unroll
-                    for (int r2 = 0; r2 < M1; ++r2)
-#pragma unroll
-                        for (int l2 = 0; l2 < M1; ++l2) {
-                            const double gv = g[P1[r2][l2]];
-                            acc += gv * b1.a[l2][r2];
-                            F1[r2][l2] += gv * D0;
-                        }
-                    F0[r][l] += acc;
-                }
-            double phi_f = 0.0;
-#pragma unroll
-            for (int r = 0; r < M0; ++r)
-#pragma unroll
-                for (int l = 0; l < M0; ++l) phi_f += b0.a[l][r] * F0[r][l];
-#pragma unroll
-            for (int r = 0; r < M1; ++r)
-#pragma unroll
-                for (int l = 0; l < M1; ++l) phi_f += b1.a[l][r] * F1[r][l];
-            const double sc = wH * 0.5 * (phi_h + phi_f) + wS;
-            overlap_adjoint<M0>(ds, dh, n, b0, F0, c, sc, wH);
-            overlap_adjoint<M1>(ds, dh, n, b1, F1, c, sc, wH);
-        }
-        same_spin_dg<FullDg, M0>(dg, np, b0, pidx0, cw);
-        same_spin_dg<FullDg, M1>(dg, np, b1, pidx1, cw);
+            for (int l = 0; l < M1; ++l) phi += b1.a[l][r] * hact[b1.R[r] + n * b1.L[l]];
+        phi += same_spin_energy<M0>(G2, np, b0, pidx0);
+        phi += same_spin_energy<M1>(G2, np, b1, pidx1);
 #pragma unroll
         for (int r = 0; r < M0; ++r)
 #pragma unroll
             for (int l = 0; l < M0; ++l) {
-                const int P = pidx0(r, l);
-                const double cwD0 = cw * b0.a[l][r];
+                const double* g = G2 + pidx0(r, l) * np;
+                double inner = 0.0;
+#pragma unroll
+                for (int r2 = 0; r2 < M1; ++r2)
+#pragma unroll
+                    for (int l2 = 0; l2 < M1; ++l2) inner += g[P1[r2][l2]] * b1.a[l2][r2];
+                phi += b0.a[l][r] * inner;
+            }
+        const double H = c * phi;
+        sd.Hdet[(size_t)d + (size_t)n_det * e] = H;
+        sd.Hdet[(size_t)e + (size_t)n_det * d] = H;
+        sd.Sdet[(size_t)d + (size_t)n_det * e] = c;
+        sd.Sdet[(size_t)e + (size_t)n_det * d] = c;
+    }
+}
+
+// The backward of a pair, F_pair = w (H - E S), runs as two kernels over the same blocks and pairs,
+// each recomputing the (cheap) inversions, so that neither holds everything in registers:
+//   k_pairs_lane_bwd_f: the F blocks (reads of the kernel), then the S_act / h_act adjoints
+//                       (ds, dh: the ds/dh columns of each block's partial row);
+//   k_pairs_lane_bwd_g: the adjoint of the packed kernel (the dg columns of the partial row),
+//                       same-spin through the compound W accumulator, opposite-spin on the matrix
+//                       cores; it needs no kernel values at all.
+// Both use the same wave-uniform pair loop, so a pair is skipped (zero weight, deferred) by both.
+
+// LDS: G2, S_act, h_act, ds, dh.
+template <int M0, int M1>
+__global__ void __launch_bounds__(256) LANE_WAVES
+    k_pairs_lane_bwd_f(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n = topo.n, ng = topo.ng, n_det = topo.n_det;
+    const int np = n * (n + 1) / 2;
+    extern __shared__ double lds[];
+    double* G2 = lds;
+    double* sact = G2 + np * np;
+    double* hact = sact + n * n;
+    double* ds = hact + n * n;
+    double* dh = ds + n * n;
+    load_pair_tables(sd, n, np, G2, sact, hact);
+    for (int t = threadIdx.x; t < 2 * n * n; t += blockDim.x) ds[t] = 0.0;
+    __syncthreads();
+    const SpinTables tab = spin_tables(topo, swap_spins);
+    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
+    const double E = sd.energy[0];
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_pairs;
+         t += (long long)gridDim.x * blockDim.x) {
+        int d, e;
+        unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
+        const double w = sd.dcoef[d] * sd.dcoef[e] * (d == e ? 1.0 : 2.0);
+        if (w == 0.0) continue;
+        LaneBlock<M0> b0;
+        LaneBlock<M1> b1;
+        occupied<M0>((uint32_t)tab.mask0[tab.det0[e]], b0.R);
+        occupied<M0>((uint32_t)tab.mask0[tab.det0[d]], b0.L);
+        occupied<M1>((uint32_t)tab.mask1[tab.det1[e]], b1.R);
+        occupied<M1>((uint32_t)tab.mask1[tab.det1[d]], b1.L);
+        const bool ok0 = lane_invert<M0>(sact, n, cond_limit, b0);
+        const bool ok1 = lane_invert<M1>(sact, n, cond_limit, b1);
+        if (!(ok0 && ok1)) continue;  // deferred: done by the lane-group kernel
+        int P1[LaneBlock<M1>::K][LaneBlock<M1>::K];
+#pragma unroll
+        for (int r = 0; r < M1; ++r)
+#pragma unroll
+            for (int l = 0; l < M1; ++l) P1[r][l] = pair_index(b1.R[r], b1.L[l]);
+        auto pidx0 = [&](int r, int l) { return pair_index(b0.R[r], b0.L[l]); };
+        auto pidx1 = [&](int r, int l) { return P1[r][l]; };
+        const double c = b0.det * b1.det;
+        const double wH = w, wS = -E * w;
+        double F0[LaneBlock<M0>::K][LaneBlock<M0>::K];
+        double F1[LaneBlock<M1>::K][LaneBlock<M1>::K];
+        double phi_h = 0.0;
+#pragma unroll
+        for (int r = 0; r < M0; ++r)
+#pragma unroll
+            for (int l = 0; l < M0; ++l) {
+                F0[r][l] = hact[b0.R[r] + n * b0.L[l]];
+                phi_h += b0.a[l][r] * F0[r][l];
+            }
+#pragma unroll
+        for (int r = 0; r < M1; ++r)
+#pragma unroll
+            for (int l = 0; l < M1; ++l) {
+                F1[r][l] = hact[b1.R[r] + n * b1.L[l]];
+                phi_h += b1.a[l][r] * F1[r][l];
+            }
+        same_spin_field<M0>(G2, np, b0, pidx0, F0);
+        same_spin_field<M1>(G2, np, b1, pidx1, F1);
+        // opposite spin: each kernel entry feeds both F blocks
+#pragma unroll
+        for (int r = 0; r < M0; ++r)
+#pragma unroll
+            for (int l = 0; l < M0; ++l) {
+                const double* g = G2 + pidx0(r, l) * np;
+                const double D0 = b0.a[l][r];
+                double acc = 0.0;
 #pragma unroll
                 for (int r2 = 0; r2 < M1; ++r2)
 #pragma unroll
                     for (int l2 = 0; l2 < M1; ++l2) {
-                        if (FullDg) atomicAdd(&dg[P * np + P1[r2][l2]], cwD0 * b1.a[l2][r2]);
-                        else atomicAdd(&dg[pair_index(P, P1[r2][l2])], cwD0 * b1.a[l2][r2]);
+                        const double gv = g[P1[r2][l2]];
+                        acc += gv * b1.a[l2][r2];
+                        F1[r2][l2] += gv * D0;
+                    }
+                F0[r][l] += acc;
+            }
+        double phi_f = 0.0;
+#pragma unroll
+        for (int r = 0; r < M0; ++r)
+#pragma unroll
+            for (int l = 0; l < M0; ++l) phi_f += b0.a[l][r] * F0[r][l];
+#pragma unroll
+        for (int r = 0; r < M1; ++r)
+#pragma unroll
+            for (int l = 0; l < M1; ++l) phi_f += b1.a[l][r] * F1[r][l];
+        const double sc = wH * 0.5 * (phi_h + phi_f) + wS;
+        overlap_adjoint<M0>(ds, dh, n, b0, F0, c, sc, wH);
+        overlap_adjoint<M1>(ds, dh, n, b1, F1, c, sc, wH);
+    }
+    __syncthreads();
+    double* out = sd.partials + (size_t)blockIdx.x * (ng + 2 * n * n) + ng;
+    for (int t = threadIdx.x; t < 2 * n * n; t += blockDim.x) out[t] = ds[t];
+}
+
+// LDS: S_act, dg (full np x np or packed), then the optional W (nc x nc) and MFMA staging rows
+// (per wavefront 2 x 16 x np16).
+// Fast: W and the MFMA update (when both fit in LDS); otherwise direct LDS atomics into dg.
+template <bool FullDg, bool Fast, int M0, int M1>
+__global__ void __launch_bounds__(256) LANE_WAVES
+    k_pairs_lane_bwd_g(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit,
+                       LaneBwdLayout lay) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n = topo.n, ng = topo.ng, n_det = topo.n_det;
+    const int np = n * (n + 1) / 2;
+    extern __shared__ double lds[];
+    double* sact = lds;
+    double* dg = sact + n * n;
+    double* W = dg + (FullDg ? np * np : ng);
+    double* bufs = W + lay.nc * lay.nc;
+    const int wave = threadIdx.x >> 6;
+    double* bufA = bufs + (size_t)wave * 32 * lay.np16;
+    double* bufB = bufA + 16 * lay.np16;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) sact[t] = sd.sact[t];
+    {
+        const int nzero = (FullDg ? np * np : ng) + lay.nc * lay.nc + (blockDim.x >> 6) * 32 * lay.np16;
+        for (int t = threadIdx.x; t < nzero; t += blockDim.x) dg[t] = 0.0;
+    }
+    __syncthreads();
+    const SpinTables tab = spin_tables(topo, swap_spins);
+    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
+    const int lane = threadIdx.x & 63;
+    // wave-uniform trip count: the MFMA update needs every lane of the wavefront
+    const long long first = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)));
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long tb = first; tb < n_pairs; tb += stride) {
+        const long long t = tb + lane;
+        int d = 0, e = 0;
+        bool live = t < n_pairs;
+        if (live) unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
+        const double w = live ? sd.dcoef[d] * sd.dcoef[e] * (d == e ? 1.0 : 2.0) : 0.0;
+        live = live && w != 0.0;
+        LaneBlock<M0> b0;
+        LaneBlock<M1> b1;
+        occupied<M0>((uint32_t)tab.mask0[tab.det0[e]], b0.R);
+        occupied<M0>((uint32_t)tab.mask0[tab.det0[d]], b0.L);
+        occupied<M1>((uint32_t)tab.mask1[tab.det1[e]], b1.R);
+        occupied<M1>((uint32_t)tab.mask1[tab.det1[d]], b1.L);
+        const bool ok0 = lane_invert<M0>(sact, n, cond_limit, b0);
+        const bool ok1 = lane_invert<M1>(sact, n, cond_limit, b1);
+        live = live && ok0 && ok1;  // deferred pairs are done by the lane-group kernel
+        int P1[LaneBlock<M1>::K][LaneBlock<M1>::K];
+#pragma unroll
+        for (int r = 0; r < M1; ++r)
+#pragma unroll
+            for (int l = 0; l < M1; ++l) P1[r][l] = pair_index(b1.R[r], b1.L[l]);
+        auto pidx0 = [&](int r, int l) { return pair_index(b0.R[r], b0.L[l]); };
+        auto pidx1 = [&](int r, int l) { return P1[r][l]; };
+        const double cw = live ? b0.det * b1.det * w : 0.0;
+        if constexpr (Fast) {
+            opposite_dg_mfma<FullDg, M0, M1>(bufA, bufB, lay.np16, np, dg, live, b0, b1, P1, cw);
+            if (live) {
+                same_spin_w<M0>(W, lay.nc, b0, cw);
+                same_spin_w<M1>(W, lay.nc, b1, cw);
+            }
+        } else if (live) {
+            same_spin_dg<FullDg, M0>(dg, np, b0, pidx0, cw);
+            same_spin_dg<FullDg, M1>(dg, np, b1, pidx1, cw);
+            {
+#pragma unroll
+                for (int r = 0; r < M0; ++r)
+#pragma unroll
+                    for (int l = 0; l < M0; ++l) {
+                        const int P = pidx0(r, l);
+                        const double cwD0 = cw * b0.a[l][r];
+#pragma unroll
+                        for (int r2 = 0; r2 < M1; ++r2)
+#pragma unroll
+                            for (int l2 = 0; l2 < M1; ++l2) dg_add<FullDg>(dg, np, P, P1[r2][l2], cwD0 * b1.a[l2][r2]);
                     }
             }
+        }
     }
-    if (Backward) {
+    __syncthreads();
+    if (Fast) {
+        w_to_dg<FullDg>(W, lay.nc, dg, np);
         __syncthreads();
-        const int stride_part = ng + 2 * n * n;
-        double* out = sd.partials + (size_t)blockIdx.x * stride_part;
-        if (FullDg) {
-            for (int t = threadIdx.x; t < ng; t += blockDim.x) {
-                int Q, P;
-                unpack_pair(t, Q, P);
-                out[t] = (P == Q) ? dg[P + np * P] : dg[P + np * Q] + dg[Q + np * P];
-            }
-        } else {
-            for (int t = threadIdx.x; t < ng; t += blockDim.x) out[t] = dg[t];
+    }
+    double* out = sd.partials + (size_t)blockIdx.x * (ng + 2 * n * n);
+    if (FullDg) {
+        for (int t = threadIdx.x; t < ng; t += blockDim.x) {
+            int Q, P;
+            unpack_pair(t, Q, P);
+            out[t] = (P == Q) ? dg[P + np * P] : dg[P + np * Q] + dg[Q + np * P];
         }
-        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-            out[ng + t] = ds[t];
-            out[ng + n * n + t] = dh[t];
-        }
+    } else {
+        for (int t = threadIdx.x; t < ng; t += blockDim.x) out[t] = dg[t];
     }
 }
 
@@ -436,32 +655,77 @@ struct LaneShape {
     static constexpr bool backward = forward && M0 <= 5;
 };
 
-size_t pairs_lane_lds_bytes(int n, int ng, bool backward) {
-    const size_t np = (size_t)n * (n + 1) / 2;
-    size_t doubles = np * np + 2 * (size_t)n * n;
-    if (backward) doubles += (pairs_full_dg(n, ng) ? np * np : (size_t)ng) + 2 * (size_t)n * n;
-    return doubles * sizeof(double);
-}
-
 namespace {
 
-template <bool B, bool F, int M0, int M1>
-hipError_t lane_launch(bool run, hipStream_t st, dim3 grid, size_t lds, const SlotDesc* slots, const TopoDesc& topo,
-                       int swap, double cond) {
+constexpr size_t kLaneLdsLimit = 160 * 1024;
+constexpr int kLaneWaves = 4;  // wavefronts per block (256 threads)
+
+size_t lane_fwd_doubles(int n) {
+    const size_t np = (size_t)n * (n + 1) / 2;
+    return np * np + 2 * (size_t)n * n;
+}
+
+size_t lane_bwd_f_doubles(int n) { return lane_fwd_doubles(n) + 2 * (size_t)n * n; }
+
+size_t lane_bwd_g_doubles(int n, int ng, const LaneBwdLayout& lay) {
+    const size_t np = (size_t)n * (n + 1) / 2;
+    return (size_t)n * n + (pairs_full_dg(n, ng) ? np * np : (size_t)ng) + (size_t)lay.nc * lay.nc +
+           (size_t)kLaneWaves * 32 * lay.np16;
+}
+
+// W and the MFMA staging rows when both fit in LDS, else neither (direct atomics)
+LaneBwdLayout lane_bwd_layout(int n, int ng) {
+    const int np = n * (n + 1) / 2;
+    const LaneBwdLayout fast{n * (n - 1) / 2, (np + 15) / 16 * 16};
+    if (lane_bwd_g_doubles(n, ng, fast) * sizeof(double) <= kLaneLdsLimit) return fast;
+    return {0, 0};
+}
+
+template <int M0, int M1>
+hipError_t lane_fwd(bool run, hipStream_t st, dim3 grid, size_t lds, const SlotDesc* slots, const TopoDesc& topo,
+                    int swap, double cond) {
     if (!run)
-        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_lane<B, F, M0, M1>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_pairs_lane<B, F, M0, M1>), grid, dim3(256), lds, st, slots, topo, swap, cond);
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_lane_fwd<M0, M1>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLaneLdsLimit);
+    hipLaunchKernelGGL((k_pairs_lane_fwd<M0, M1>), grid, dim3(64 * kLaneWaves), lds, st, slots, topo, swap, cond);
+    return hipSuccess;
+}
+
+template <bool F, bool Fast, int M0, int M1>
+hipError_t lane_bwd(bool run, hipStream_t st, dim3 grid, const SlotDesc* slots, const TopoDesc& topo, int swap,
+                    double cond, LaneBwdLayout lay) {
+    if (!run) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_lane_bwd_f<M0, M1>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLaneLdsLimit);
+        if (e != hipSuccess) return e;
+        return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_lane_bwd_g<F, Fast, M0, M1>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLaneLdsLimit);
+    }
+    const size_t lds_f = lane_bwd_f_doubles(topo.n) * sizeof(double);
+    const size_t lds_g = lane_bwd_g_doubles(topo.n, topo.ng, lay) * sizeof(double);
+    hipLaunchKernelGGL((k_pairs_lane_bwd_f<M0, M1>), grid, dim3(64 * kLaneWaves), lds_f, st, slots, topo, swap, cond);
+    hipLaunchKernelGGL((k_pairs_lane_bwd_g<F, Fast, M0, M1>), grid, dim3(64 * kLaneWaves), lds_g, st, slots, topo,
+                       swap, cond, lay);
     return hipSuccess;
 }
 
 template <int M0, int M1>
 hipError_t lane_dispatch_shape(bool run, bool backward, bool full_dg, hipStream_t st, dim3 grid, size_t lds,
-                               const SlotDesc* slots, const TopoDesc& topo, int swap, double cond) {
-    if (!backward) return lane_launch<false, false, M0, M1>(run, st, grid, lds, slots, topo, swap, cond);
+                               const SlotDesc* slots, const TopoDesc& topo, int swap, double cond,
+                               LaneBwdLayout lay) {
+    if (!backward) return lane_fwd<M0, M1>(run, st, grid, lds, slots, topo, swap, cond);
     if constexpr (LaneShape<M0, M1>::backward) {
-        return full_dg ? lane_launch<true, true, M0, M1>(run, st, grid, lds, slots, topo, swap, cond)
-                       : lane_launch<true, false, M0, M1>(run, st, grid, lds, slots, topo, swap, cond);
+        // configuration (run == false) prepares every variant
+        const bool fast = lay.nc != 0;
+        hipError_t e = hipSuccess;
+        if (!run || (full_dg && fast)) e = lane_bwd<true, true, M0, M1>(run, st, grid, slots, topo, swap, cond, lay);
+        if (e == hipSuccess && (!run || (full_dg && !fast)))
+            e = lane_bwd<true, false, M0, M1>(run, st, grid, slots, topo, swap, cond, lay);
+        if (e == hipSuccess && (!run || (!full_dg && fast)))
+            e = lane_bwd<false, true, M0, M1>(run, st, grid, slots, topo, swap, cond, lay);
+        if (e == hipSuccess && (!run || (!full_dg && !fast)))
+            e = lane_bwd<false, false, M0, M1>(run, st, grid, slots, topo, swap, cond, lay);
+        return e;
     }
     return hipErrorInvalidValue;
 }
@@ -489,6 +753,11 @@ bool for_lane_shape(int m0, int m1, Fn&& f) {
 
 }  // namespace
 
+size_t pairs_lane_lds_bytes(int n, int ng, bool backward) {
+    if (!backward) return lane_fwd_doubles(n) * sizeof(double);
+    return std::max(lane_bwd_f_doubles(n), lane_bwd_g_doubles(n, ng, lane_bwd_layout(n, ng))) * sizeof(double);
+}
+
 bool pairs_lane_supported(int n_alpha, int n_beta, bool backward) {
     const int m0 = n_alpha >= n_beta ? n_alpha : n_beta, m1 = n_alpha >= n_beta ? n_beta : n_alpha;
     bool ok = false;
@@ -501,14 +770,12 @@ hipError_t configure_pairs_lane() {
     for (int m0 = 1; m0 <= 6; ++m0)
         for (int m1 = 0; m1 <= m0; ++m1)
             for_lane_shape(m0, m1, [&]<int A, int B>() {
-                for (int b = 0; b < 2; ++b)
-                    for (int f = 0; f < 2; ++f) {
-                        if (b == 0 && f == 1) continue;
-                        if (b && !LaneShape<A, B>::backward) continue;
-                        const hipError_t e =
-                            lane_dispatch_shape<A, B>(false, b, f, nullptr, dim3(), 0, nullptr, TopoDesc{}, 0, 0.0);
-                        if (e != hipSuccess) err = e;
-                    }
+                for (int b = 0; b < 2; ++b) {
+                    if (b && !LaneShape<A, B>::backward) continue;
+                    const hipError_t e = lane_dispatch_shape<A, B>(false, b, true, nullptr, dim3(), 0, nullptr,
+                                                                   TopoDesc{}, 0, 0.0, LaneBwdLayout{});
+                    if (e != hipSuccess) err = e;
+                }
             });
     return err;
 }
@@ -518,10 +785,11 @@ void launch_pairs_lane(hipStream_t st, bool backward, int blocks, const SlotDesc
     const int swap = n_beta > n_alpha ? 1 : 0;
     const int m0 = swap ? n_beta : n_alpha, m1 = swap ? n_alpha : n_beta;
     const bool full = pairs_full_dg(topo.n, topo.ng);
+    const LaneBwdLayout lay = backward ? lane_bwd_layout(topo.n, topo.ng) : LaneBwdLayout{};
     const size_t lds = pairs_lane_lds_bytes(topo.n, topo.ng, backward);
     const dim3 grid(blocks, 1, count);
     for_lane_shape(m0, m1, [&]<int A, int B>() {
-        (void)lane_dispatch_shape<A, B>(true, backward, full, st, grid, lds, slots, topo, swap, cond_limit);
+        (void)lane_dispatch_shape<A, B>(true, backward, full, st, grid, lds, slots, topo, swap, cond_limit, lay);
     });
 }
 
