@@ -290,16 +290,19 @@ typedef double lane_double4 __attribute__((ext_vector_type(4)));
 
 // Opposite-spin adjoint of the kernel as outer products on the matrix cores:
 //   dg[P, Q] += sum_pairs cw * ga[P] * gb[Q],
-// ga / gb the alpha / beta transition densities scattered to packed pair indices. The wavefront's
-// 64 pairs go through in four chunks of 16: the chunk's lanes scatter their rows into zeroed LDS
-// rows (accumulating where two block entries share a packed index), every 16x16 tile of dg is
-// formed by four v_mfma_f64_16x16x4 steps (A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
-// D[i][j] in lane j + 16(i%4), element i/4) and added to the LDS accumulator, and the rows are
-// zeroed again. Called by all lanes of the wavefront.
-template <bool FullDg, int M0, int M1>
-__device__ void opposite_dg_mfma(double* bufA, double* bufB, int np16, int np, double* dg, bool live,
-                                 const LaneBlock<M0>& b0, const LaneBlock<M1>& b1,
-                                 const int (&P1)[LaneBlock<M1>::K][LaneBlock<M1>::K], double cw) {
+// ga / gb the alpha / beta transition densities scattered to packed pair indices (np <= 64, so at
+// most 4 x 4 tiles of 16). The wavefront's 64 pairs go through in four chunks of 16: the chunk's
+// lanes scatter their rows into zeroed LDS rows (accumulating where two block entries share a
+// packed index), every 16x16 tile is updated by four v_mfma_f64_16x16x4 steps (A[i][k] in lane
+// i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i%4), element i/4) into accumulators
+// that stay in registers for the whole kernel, and the rows are zeroed again. Called by all lanes.
+constexpr int kMfmaTiles = 4;
+typedef lane_double4 MfmaTiles[kMfmaTiles][kMfmaTiles];
+
+template <int M0, int M1>
+__device__ void opposite_dg_mfma(double* bufA, double* bufB, int np16, bool live, const LaneBlock<M0>& b0,
+                                 const LaneBlock<M1>& b1, const int (&P1)[LaneBlock<M1>::K][LaneBlock<M1>::K],
+                                 double cw, MfmaTiles& acc) {
     const int lane = threadIdx.x & 63;
     const int k4 = lane >> 4, i16 = lane & 15;
     const int tiles = np16 >> 4;
@@ -320,24 +323,23 @@ __device__ void opposite_dg_mfma(double* bufA, double* bufB, int np16, int np, d
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-#pragma unroll 1
-        for (int tp = 0; tp < tiles; ++tp)
-#pragma unroll 1
-            for (int tq = 0; tq < tiles; ++tq) {
-                lane_double4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const double av = bufA[(4 * kk + k4) * np16 + 16 * tp + i16];
-                    const double bv = bufB[(4 * kk + k4) * np16 + 16 * tq + i16];
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-                }
-                const int Q = 16 * tq + i16;
+        for (int kk = 0; kk < 4; ++kk) {
+            const double* ra = bufA + (4 * kk + k4) * np16 + i16;
+            const double* rb = bufB + (4 * kk + k4) * np16 + i16;
+            double av[kMfmaTiles], bv[kMfmaTiles];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int P = 16 * tp + 4 * v + k4;
-                    if (P < np && Q < np && acc[v] != 0.0) dg_add<FullDg>(dg, np, P, Q, acc[v]);
-                }
+            for (int t = 0; t < kMfmaTiles; ++t) {
+                av[t] = t < tiles ? ra[16 * t] : 0.0;
+                bv[t] = t < tiles ? rb[16 * t] : 0.0;
             }
+#pragma unroll
+            for (int tp = 0; tp < kMfmaTiles; ++tp)
+#pragma unroll
+                for (int tq = 0; tq < kMfmaTiles; ++tq)
+                    if (tp < tiles && tq < tiles)
+                        acc[tp][tq] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[tp], bv[tq], acc[tp][tq], 0, 0, 0);
+        }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (mine) {
@@ -353,6 +355,26 @@ __device__ void opposite_dg_mfma(double* bufA, double* bufB, int np16, int np, d
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// adds a wavefront's MFMA tiles into the block's dg accumulator
+template <bool FullDg>
+__device__ void flush_mfma_tiles(const MfmaTiles& acc, int np16, int np, double* dg) {
+    const int lane = threadIdx.x & 63;
+    const int k4 = lane >> 4, i16 = lane & 15;
+    const int tiles = np16 >> 4;
+#pragma unroll
+    for (int tp = 0; tp < kMfmaTiles; ++tp)
+#pragma unroll
+        for (int tq = 0; tq < kMfmaTiles; ++tq) {
+            if (!(tp < tiles && tq < tiles)) continue;
+            const int Q = 16 * tq + i16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int P = 16 * tp + 4 * v + k4;
+                if (P < np && Q < np && acc[tp][tq][v] != 0.0) dg_add<FullDg>(dg, np, P, Q, acc[tp][tq][v]);
+            }
+        }
 }
 
 // Optional LDS regions of the backward lane kernel (0 = not used).
@@ -583,6 +605,11 @@ __global__ void __launch_bounds__(256) LANE_WAVES
     // wave-uniform trip count: the MFMA update needs every lane of the wavefront
     const long long first = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)));
     const long long stride = (long long)gridDim.x * blockDim.x;
+    MfmaTiles tiles;
+#pragma unroll
+    for (int tp = 0; tp < kMfmaTiles; ++tp)
+#pragma unroll
+        for (int tq = 0; tq < kMfmaTiles; ++tq) tiles[tp][tq] = lane_double4{0.0, 0.0, 0.0, 0.0};
     for (long long tb = first; tb < n_pairs; tb += stride) {
         const long long t = tb + lane;
         int d = 0, e = 0;
@@ -608,7 +635,7 @@ __global__ void __launch_bounds__(256) LANE_WAVES
         auto pidx1 = [&](int r, int l) { return P1[r][l]; };
         const double cw = live ? b0.det * b1.det * w : 0.0;
         if constexpr (Fast) {
-            opposite_dg_mfma<FullDg, M0, M1>(bufA, bufB, lay.np16, np, dg, live, b0, b1, P1, cw);
+            opposite_dg_mfma<M0, M1>(bufA, bufB, lay.np16, live, b0, b1, P1, cw, tiles);
             if (live) {
                 same_spin_w<M0>(W, lay.nc, b0, cw);
                 same_spin_w<M1>(W, lay.nc, b1, cw);
@@ -631,6 +658,7 @@ __global__ void __launch_bounds__(256) LANE_WAVES
             }
         }
     }
+    if (Fast) flush_mfma_tiles<FullDg>(tiles, lay.np16, np, dg);
     __syncthreads();
     if (Fast) {
         w_to_dg<FullDg>(W, lay.nc, dg, np);
@@ -677,7 +705,8 @@ size_t lane_bwd_g_doubles(int n, int ng, const LaneBwdLayout& lay) {
 LaneBwdLayout lane_bwd_layout(int n, int ng) {
     const int np = n * (n + 1) / 2;
     const LaneBwdLayout fast{n * (n - 1) / 2, (np + 15) / 16 * 16};
-    if (lane_bwd_g_doubles(n, ng, fast) * sizeof(double) <= kLaneLdsLimit) return fast;
+    if (fast.np16 <= 16 * kMfmaTiles && lane_bwd_g_doubles(n, ng, fast) * sizeof(double) <= kLaneLdsLimit)
+        return fast;
     return {0, 0};
 }
 
