@@ -2,20 +2,24 @@
 """VBSCF energy+gradient throughput on one MI355X.
 
 Workload (medium config): synthetic RI problems n_bf=120, n_aux=360, n_act=10 VB orbitals with
-10 electrons (5 alpha / 5 beta), 42 covalent Rumer singlet structures expanded to 252
-determinants; orbitals resident in HBM. An evaluation is one full energy + analytic orbital
-gradient (orbital normalisation, RI transform, determinant pairs, structure H/S, generalised
-eigensolve, backward) through the engine's C ABI, FP64 throughout.
+10 electrons (5 alpha / 5 beta), 42 covalent Rumer structures expanded to 252 determinants;
+orbitals resident in HBM. An evaluation is one full energy + analytic orbital gradient (orbital
+normalisation, RI transform, determinant pairs, structure H/S, generalised eigensolve, backward)
+through the engine's C ABI, FP64 throughout.
 
-Two legs are timed:
-  * plain: one problem per launch sequence (the north star's plain FP64 GPU path);
+Three legs are timed:
+  * plain: one problem per launch sequence (the plain FP64 GPU path);
   * batched (value): a pool of --batch resident problems (seeds 0..K-1) evaluated by one launch
-    sequence per step; one step = K evaluations.
+    sequence per step; one step = K evaluations;
+  * served: Poisson arrivals (--serve-rate requests/s for --serve-seconds) over a 64-point
+    dissociation scan (inter-fragment h and B blocks scaled by exp(-0.5 r), r in [0.5, 4.0]),
+    continuously batched onto two in-flight submissions (submit / poll / wait); request latency
+    is completion minus arrival, host orbitals in and gradients out (PCIe included).
 
-Prints one JSON line: batched throughput (E+grad evaluations/s) beside the plain path's, p50/p99
-latency of both, the roofline of the dominant kernel of the batched leg (its duration from HIP
-events recorded on the engine stream during the timed region), parity against the FP64 CPU
-oracle and the oracle's own throughput on the host cores (cpu_baseline).
+Prints one JSON line: batched throughput (E+grad evaluations/s) beside the plain path's, request
+p50/p99 latency of the served leg, the roofline of the dominant pair kernel of the batched leg (its
+duration from HIP events recorded on the engine stream during the timed region), parity against
+the FP64 CPU oracle and the oracle's own throughput on the host cores (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -33,11 +37,12 @@ sys.path.insert(0, ROOT)
 METRIC = "VBSCF E+grad evals/s served vs plain FP64 GPU; p50/p99 latency; |ΔE|≤1e-8 Eh"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "medium_pmc_traffic.json")
 
 MEDIUM = dict(seed=0, n_bf=120, n_act=10, n_el=10, n_aux=360, structures="COV")
 
 
-def pair_flops(m_alpha: int, m_beta: int) -> float:
+def det_pair_flops(m_alpha: int, m_beta: int) -> float:
     """Algorithmic FP64 flops of one ordered determinant pair's forward matrix element:
     per spin block LU (2/3 m^3) + inverse (2 m^3) + same-spin two-electron terms (6 C(m,2)^2),
     plus the opposite-spin coupling (2 m_alpha^2 m_beta^2)."""
@@ -81,6 +86,75 @@ def cpu_baseline(cfg, sp, topo, gpu_energy, gpu_grad, seconds: float):
     return base, parity
 
 
+def serve(vb, ctx, topo, cfg, rate: float, seconds: float, replicas: int, max_batch: int, seed: int = 7):
+    """Poisson arrivals over a 64-point scan, continuously batched onto the pool's two lanes."""
+    import numpy as np
+
+    points = 64
+    rs = [0.5 + 3.5 * p / (points - 1) for p in range(points)]
+    pool = vb.Pool(ctx, topo, capacity=points * replicas)
+    orb, free = [], []
+    for p, r in enumerate(rs):
+        sp = vb.synthetic_problem(1000 + p, cfg["n_bf"], cfg["n_act"], cfg["n_aux"], scan_r=r)
+        for k in range(replicas):
+            pool.load_synthetic(p * replicas + k, sp)
+        orb.append(sp.C)
+        free.append(list(range(p * replicas, (p + 1) * replicas)))
+    # reference energy of every scan point (one batched launch, slot p*replicas)
+    ref_slots = [p * replicas for p in range(points)]
+    t = pool.submit(ref_slots, [orb[p] for p in range(points)], gradient=False)
+    e_ref, _ = pool.wait(t)
+
+    rng = np.random.default_rng(seed)
+    n_req = max(1, int(rate * seconds))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    point_of = rng.integers(0, points, n_req)
+    latency, energies_err, batches = [], 0.0, []
+    queue, inflight = [], {}  # inflight: ticket -> (requests, slots)
+    nxt = 0
+    t0 = time.perf_counter()
+    done = 0
+    while done < n_req:
+        now = time.perf_counter() - t0
+        while nxt < n_req and arrivals[nxt] <= now:
+            queue.append(nxt)
+            nxt += 1
+        for ticket in list(inflight):
+            if pool.poll(ticket):
+                e, _grads = pool.wait(ticket)
+                fin = time.perf_counter() - t0
+                reqs, slots = inflight.pop(ticket)
+                for q, s, ei in zip(reqs, slots, e):
+                    latency.append((fin - arrivals[q]) * 1e3)
+                    energies_err = max(energies_err, abs(ei - e_ref[point_of[q]]))
+                    free[point_of[q]].append(s)
+                done += len(reqs)
+        if queue and len(inflight) < 2:
+            reqs, slots, rest = [], [], []
+            for q in queue:
+                p = point_of[q]
+                if len(reqs) < max_batch and free[p]:
+                    reqs.append(q)
+                    slots.append(free[p].pop())
+                else:
+                    rest.append(q)
+            if reqs:
+                ticket = pool.submit(slots, [orb[point_of[q]] for q in reqs], gradient=True)
+                inflight[ticket] = (reqs, slots)
+                batches.append(len(reqs))
+                queue = rest
+        if not queue and not inflight and nxt < n_req:
+            time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 2e-4)))
+    makespan = time.perf_counter() - t0
+    pool.close()
+    return dict(requests=n_req, offered_rate=rate, throughput=n_req / makespan, seconds=makespan,
+                latency_ms=dict(p50=percentile(latency, 0.5), p99=percentile(latency, 0.99),
+                                max=max(latency)),
+                mean_batch=float(np.mean(batches)), launches=len(batches), max_abs_dE_vs_direct=energies_err,
+                scan=f"{points} points r in [0.5, 4.0], {replicas} resident replicas each",
+                io="host orbitals in, host gradients out (PCIe included)")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +162,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=128, help="resident problems evaluated per batched step")
     ap.add_argument("--plain-steps", type=int, default=300, help="timed single-problem evaluations")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the served leg")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the served leg (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle parity")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -119,7 +195,8 @@ def main():
     ctx = vb.Context(0)
     dev = torch.device("cuda:0")
     phases = ("orbital_prep_ms", "transform_ms", "ri_one_index_ms", "pair_forward_ms", "structure_ms", "eigensolve_ms",
-              "pair_backward_ms", "pair_backward_kernel_ms", "gradient_tail_ms", "total_ms")
+              "pair_backward_ms", "pair_backward_kernel_ms", "gradient_tail_ms", "total_ms", "pair_forward_kernel_ms",
+              "pair_backward_field_ms", "pair_backward_adjoint_ms")
 
     # ---- plain leg: one problem at a time ----
     prob = vb.Problem.from_synthetic(ctx, sp, topo)
@@ -176,25 +253,45 @@ def main():
                              grad_max_diff_slot0_vs_plain=float(np.abs(batch_grad0 - grad).max()))
     pool.close()
 
-    # roofline of the dominant single kernel of the batched leg (K problems per launch)
+    # ---- rooflines of the single kernels timed on the engine stream (K problems per launch) ----
     n_det = topo.n_determinants
     nbp = cfg["n_bf"] * (cfg["n_bf"] + 1) // 2
-    fwd_flops = K * n_det * n_det * pair_flops(topo.n_alpha, topo.n_beta)
-    candidates = {
-        "k_pairs<backward>": (mean["pair_backward_kernel_ms"], "mfma", 2.0 * fwd_flops, "TFLOP/s"),
-        "k_pairs<forward>": (mean["pair_forward_ms"], "mfma", fwd_flops, "TFLOP/s"),
+    f_det = K * n_det * n_det * det_pair_flops(topo.n_alpha, topo.n_beta)  # ordered pairs, forward
+    kernels = {
+        # backward ~ 2 F_det (two kernels over the same pairs): F blocks + S_act/h_act adjoints,
+        # and the packed-kernel adjoint, one F_det each
+        "k_pairs_lane_bwd_f": (mean["pair_backward_field_ms"], "mfma", f_det, "TFLOP/s"),
+        "k_pairs_lane_bwd_g": (mean["pair_backward_adjoint_ms"], "mfma", f_det, "TFLOP/s"),
+        "k_pairs_lane_fwd": (mean["pair_forward_kernel_ms"], "mfma", f_det, "TFLOP/s"),
         "k_ri_one_index": (mean["ri_one_index_ms"], "hbm",
                            K * 8.0 * cfg["n_aux"] * (nbp + cfg["n_act"] * cfg["n_bf"]), "GB/s"),
         "k_gep_lowest": (mean["eigensolve_ms"], "mfma", K * 10.0 * topo.n_structures ** 3, "TFLOP/s"),
     }
-    kname = max(candidates, key=lambda k: candidates[k][0])
-    ms, bound, work, unit = candidates[kname]
-    if unit == "GB/s":
-        achieved, peak = work / (ms * 1e-3) / 1e9, HBM_PEAK_GBS
-    else:
-        achieved, peak = work / (ms * 1e-3) / 1e12, FP64_PEAK_TFLOPS
-    roofline = dict(kernel=kname, bound=bound, achieved=achieved, peak=peak, unit=unit, frac=achieved / peak,
-                    traffic=None, kernel_ms=ms, algorithmic_work=work, problems_per_launch=K)
+    traffic_table = {}
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic_table = json.load(f).get("kernels", {})
+
+    def roof(name):
+        ms, bound, work, unit = kernels[name]
+        if unit == "GB/s":
+            achieved, peak = work / (ms * 1e-3) / 1e9, HBM_PEAK_GBS
+        else:
+            achieved, peak = work / (ms * 1e-3) / 1e12, FP64_PEAK_TFLOPS
+        tr = traffic_table.get(name)
+        traffic = None
+        if tr and tr.get("problems_per_launch"):
+            traffic = (tr["fetch_bytes"] + tr["write_bytes"]) * K / tr["problems_per_launch"]
+        return dict(kernel=name, bound=bound, achieved=achieved, peak=peak, unit=unit, frac=achieved / peak,
+                    traffic=traffic, kernel_ms=ms, algorithmic_work=work, problems_per_launch=K)
+
+    dominant = max(kernels, key=lambda k: kernels[k][0])
+    roofline = roof(dominant)
+    others = [roof(k) for k in kernels if k != dominant]
+
+    served = None
+    if args.serve_seconds > 0:
+        served = serve(vb, ctx, topo, cfg, args.serve_rate, args.serve_seconds, replicas=4, max_batch=K)
 
     value = K * args.steps / elapsed
     out = dict(metric=METRIC, value=value, unit="E+grad evals/s", n_gpus=1, steps=args.steps,
@@ -205,9 +302,14 @@ def main():
                                     "orbitals resident",
                            seed=cfg["seed"], n_bf=cfg["n_bf"], n_aux=cfg["n_aux"], n_act=cfg["n_act"],
                            n_structures=topo.n_structures, n_determinants=n_det, batch=K, path="batched-fp64"),
-               latency_ms=dict(p50=percentile(lat, 0.5), p99=percentile(lat, 0.99)),
-               plain_fp64=plain, speedup_vs_plain=value / plain["value"], batch_consistency=batch_consistency,
-               phase_ms={k: round(v, 5) for k, v in mean.items()}, energy=energy, roofline=roofline)
+               latency_ms=(served["latency_ms"] if served else dict(p50=percentile(lat, 0.5),
+                                                                    p99=percentile(lat, 0.99))),
+               latency_source=("served leg: request completion minus Poisson arrival" if served
+                               else "batched step"),
+               batch_step_ms=dict(p50=percentile(lat, 0.5), p99=percentile(lat, 0.99)),
+               plain_fp64=plain, speedup_vs_plain=value / plain["value"], served=served,
+               batch_consistency=batch_consistency, phase_ms={k: round(v, 5) for k, v in mean.items()},
+               energy=energy, roofline=roofline, roofline_other_kernels=others)
     if not args.no_cpu:
         base, parity = cpu_baseline(cfg, sp, topo, energy, grad, args.cpu_seconds)
         out["cpu_baseline"] = base

@@ -100,11 +100,14 @@ typedef struct vbe_flags {
 } vbe_flags;
 
 typedef struct vbe_phase_times {
-    /* device time per phase of the last evaluation, milliseconds (HIP events on the context stream);
-     * ri_one_index_ms and pair_backward_kernel_ms isolate single kernels inside transform_ms and
-     * pair_backward_ms */
+    /* device time per phase of the last evaluation, milliseconds (HIP events on the evaluation's
+     * stream). ri_one_index_ms isolates the one-index kernel inside transform_ms;
+     * pair_backward_kernel_ms the determinant-pair kernels inside pair_backward_ms. The last three
+     * isolate single pair kernels: the main forward pair kernel, and the two backward pair kernels
+     * (F blocks with the S_act/h_act adjoints; the packed-kernel adjoint). */
     float orbital_prep_ms, transform_ms, ri_one_index_ms, pair_forward_ms, structure_ms, eigensolve_ms,
         pair_backward_ms, pair_backward_kernel_ms, gradient_tail_ms, total_ms;
+    float pair_forward_kernel_ms, pair_backward_field_ms, pair_backward_adjoint_ms;
 } vbe_phase_times;
 
 /* ---- $CTRL method flag ------------------------------------------------------------------ */
@@ -239,6 +242,9 @@ int vbe_pool_eval_device(vbe_pool* pool, int32_t count, const int32_t* slots, co
  * submissions are in flight and a slot may belong to only one of them. */
 int vbe_pool_submit(vbe_pool* pool, int32_t count, const int32_t* slots, const double* C, int32_t want_gradient,
                     int32_t* ticket);
+/* Non-blocking completion test of a ticket (the vbe_poll of a serving loop): *done = 1 once its
+ * device work has finished, after which vbe_pool_wait returns without blocking. */
+int vbe_pool_poll(vbe_pool* pool, int32_t ticket, int32_t* done);
 /* Wait for a ticket; energies[count] and (if requested at submit) the concatenated gradients. */
 int vbe_pool_wait(vbe_pool* pool, int32_t ticket, double* energies, double* grads);
 /* Device time per phase of the pool's most recently completed launch. */

@@ -476,6 +476,9 @@ void finish(vbe_pool* p, Lane& lane) {
         t.pair_backward_kernel_ms = lane.with_grad ? elapsed(lane.ev[EV_BWD_BEGIN], lane.ev[EV_BWD_KERNEL]) : 0.f;
         t.gradient_tail_ms = lane.with_grad ? elapsed(lane.ev[EV_PAIR_BWD], lane.ev[EV_TAIL]) : 0.f;
         t.total_ms = elapsed(lane.ev[EV_START], lane.with_grad ? lane.ev[EV_TAIL] : lane.ev[EV_EIGEN]);
+        t.pair_forward_kernel_ms = elapsed(lane.ev[EV_TRANSFORM], lane.ev[EV_PAIR_FWD_MAIN]);
+        t.pair_backward_field_ms = lane.with_grad ? elapsed(lane.ev[EV_BWD_BEGIN], lane.ev[EV_BWD_FIELD]) : 0.f;
+        t.pair_backward_adjoint_ms = lane.with_grad ? elapsed(lane.ev[EV_BWD_FIELD], lane.ev[EV_BWD_ADJOINT]) : 0.f;
     }
     int64_t singular = 0;
     std::string err;
@@ -702,6 +705,24 @@ int vbe_pool_submit(vbe_pool* pool, int32_t count, const int32_t* slots, const d
         pool->last_lane = int(&lane - pool->lanes);
         *ticket = lane.ticket;
         return VBE_OK;
+    });
+}
+
+int vbe_pool_poll(vbe_pool* pool, int32_t ticket, int32_t* done) {
+    return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
+        if (!pool || !done) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        for (auto& lane : pool->lanes) {
+            if (!lane.busy || lane.ticket != ticket) continue;
+            const hipError_t q = hipEventQuery(lane.done);
+            if (q == hipErrorNotReady) {
+                *done = 0;
+                return VBE_OK;
+            }
+            check(q, "hipEventQuery");
+            *done = 1;
+            return VBE_OK;
+        }
+        throw StatusError(VBE_ERR_ARGUMENT, "unknown or already completed ticket");
     });
 }
 

@@ -1298,11 +1298,13 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
     hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4, 1, z), dim3(256), 0, st, slots, np);
     mark(EV_TRANSFORM);
     if (shape.lane_fwd) {
-        launch_pairs_lane(st, false, shape.lane_blocks_fwd, slots, count, topo, shape.n_alpha, shape.n_beta,
+        launch_pairs_lane(st, 0, shape.lane_blocks_fwd, slots, count, topo, shape.n_alpha, shape.n_beta,
                           shape.lane_cond_limit);
+        mark(EV_PAIR_FWD_MAIN);
         launch_pairs_group(st, false, true, shape.list_blocks, 0, slots, count, topo, shape.mmax);
     } else {
         launch_pairs_group(st, false, false, shape.pair_blocks_fwd, 0, slots, count, topo, shape.mmax);
+        mark(EV_PAIR_FWD_MAIN);
     }
     launch_pairs_singular(st, false, 0, slots, count, topo);
     mark(EV_PAIR_FWD);
@@ -1329,12 +1331,18 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
     mark(EV_BWD_BEGIN);
     int part_rows = shape.pair_blocks_bwd;
     if (shape.lane_bwd) {
-        launch_pairs_lane(st, true, shape.lane_blocks_bwd, slots, count, topo, shape.n_alpha, shape.n_beta,
+        launch_pairs_lane(st, 1, shape.lane_blocks_bwd, slots, count, topo, shape.n_alpha, shape.n_beta,
                           shape.lane_cond_limit);
+        mark(EV_BWD_FIELD);
+        launch_pairs_lane(st, 2, shape.lane_blocks_bwd, slots, count, topo, shape.n_alpha, shape.n_beta,
+                          shape.lane_cond_limit);
+        mark(EV_BWD_ADJOINT);
         launch_pairs_group(st, true, true, shape.list_blocks, shape.lane_blocks_bwd, slots, count, topo, shape.mmax);
         part_rows = shape.lane_blocks_bwd + shape.list_blocks;
     } else {
         launch_pairs_group(st, true, false, shape.pair_blocks_bwd, 0, slots, count, topo, shape.mmax);
+        mark(EV_BWD_FIELD);
+        mark(EV_BWD_ADJOINT);
     }
     launch_pairs_singular(st, true, part_rows, slots, count, topo);
     part_rows += kSingularBlocks;

@@ -30,8 +30,8 @@ enum DeviceFlag : int {
 
 // HIP events recorded between phases (optional)
 enum PhaseEvent : int {
-    EV_START, EV_PREP, EV_ONE_INDEX, EV_TRANSFORM, EV_PAIR_FWD, EV_STRUCT, EV_EIGEN, EV_BWD_BEGIN, EV_BWD_KERNEL,
-    EV_PAIR_BWD, EV_TAIL, EV_COUNT
+    EV_START, EV_PREP, EV_ONE_INDEX, EV_TRANSFORM, EV_PAIR_FWD_MAIN, EV_PAIR_FWD, EV_STRUCT, EV_EIGEN, EV_BWD_BEGIN,
+    EV_BWD_FIELD, EV_BWD_ADJOINT, EV_BWD_KERNEL, EV_PAIR_BWD, EV_TAIL, EV_COUNT
 };
 
 struct SlotDesc {
@@ -102,8 +102,9 @@ std::size_t pairs_singular_lds_bytes(int n, int ng, bool backward);
 hipError_t configure_pairs_singular();
 void launch_pairs_singular(hipStream_t st, bool backward, int part_offset, const SlotDesc* slots, int count,
                            const TopoDesc& topo);
-void launch_pairs_lane(hipStream_t st, bool backward, int blocks, const SlotDesc* slots, int count,
-                       const TopoDesc& topo, int n_alpha, int n_beta, double cond_limit);
+// which: 0 forward, 1 backward F blocks + S_act/h_act adjoints, 2 backward packed-kernel adjoint
+void launch_pairs_lane(hipStream_t st, int which, int blocks, const SlotDesc* slots, int count, const TopoDesc& topo,
+                       int n_alpha, int n_beta, double cond_limit);
 bool pairs_full_dg(int n, int ng);
 std::size_t gep_lds_bytes(int n);
 bool gep_in_lds(int n);

@@ -721,8 +721,8 @@ hipError_t lane_fwd(bool run, hipStream_t st, dim3 grid, size_t lds, const SlotD
 }
 
 template <bool F, bool Fast, int M0, int M1>
-hipError_t lane_bwd(bool run, hipStream_t st, dim3 grid, const SlotDesc* slots, const TopoDesc& topo, int swap,
-                    double cond, LaneBwdLayout lay) {
+hipError_t lane_bwd(bool run, int which, hipStream_t st, dim3 grid, const SlotDesc* slots, const TopoDesc& topo,
+                    int swap, double cond, LaneBwdLayout lay) {
     if (!run) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_lane_bwd_f<M0, M1>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLaneLdsLimit);
@@ -730,30 +730,35 @@ hipError_t lane_bwd(bool run, hipStream_t st, dim3 grid, const SlotDesc* slots, 
         return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_lane_bwd_g<F, Fast, M0, M1>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLaneLdsLimit);
     }
-    const size_t lds_f = lane_bwd_f_doubles(topo.n) * sizeof(double);
-    const size_t lds_g = lane_bwd_g_doubles(topo.n, topo.ng, lay) * sizeof(double);
-    hipLaunchKernelGGL((k_pairs_lane_bwd_f<M0, M1>), grid, dim3(64 * kLaneWaves), lds_f, st, slots, topo, swap, cond);
-    hipLaunchKernelGGL((k_pairs_lane_bwd_g<F, Fast, M0, M1>), grid, dim3(64 * kLaneWaves), lds_g, st, slots, topo,
-                       swap, cond, lay);
+    if (which == 1) {
+        const size_t lds_f = lane_bwd_f_doubles(topo.n) * sizeof(double);
+        hipLaunchKernelGGL((k_pairs_lane_bwd_f<M0, M1>), grid, dim3(64 * kLaneWaves), lds_f, st, slots, topo, swap,
+                           cond);
+    } else {
+        const size_t lds_g = lane_bwd_g_doubles(topo.n, topo.ng, lay) * sizeof(double);
+        hipLaunchKernelGGL((k_pairs_lane_bwd_g<F, Fast, M0, M1>), grid, dim3(64 * kLaneWaves), lds_g, st, slots, topo,
+                           swap, cond, lay);
+    }
     return hipSuccess;
 }
 
 template <int M0, int M1>
-hipError_t lane_dispatch_shape(bool run, bool backward, bool full_dg, hipStream_t st, dim3 grid, size_t lds,
+hipError_t lane_dispatch_shape(bool run, int which, bool full_dg, hipStream_t st, dim3 grid, size_t lds,
                                const SlotDesc* slots, const TopoDesc& topo, int swap, double cond,
                                LaneBwdLayout lay) {
-    if (!backward) return lane_fwd<M0, M1>(run, st, grid, lds, slots, topo, swap, cond);
+    if (which == 0) return lane_fwd<M0, M1>(run, st, grid, lds, slots, topo, swap, cond);
     if constexpr (LaneShape<M0, M1>::backward) {
         // configuration (run == false) prepares every variant
         const bool fast = lay.nc != 0;
         hipError_t e = hipSuccess;
-        if (!run || (full_dg && fast)) e = lane_bwd<true, true, M0, M1>(run, st, grid, slots, topo, swap, cond, lay);
+        if (!run || (full_dg && fast))
+            e = lane_bwd<true, true, M0, M1>(run, which, st, grid, slots, topo, swap, cond, lay);
         if (e == hipSuccess && (!run || (full_dg && !fast)))
-            e = lane_bwd<true, false, M0, M1>(run, st, grid, slots, topo, swap, cond, lay);
+            e = lane_bwd<true, false, M0, M1>(run, which, st, grid, slots, topo, swap, cond, lay);
         if (e == hipSuccess && (!run || (!full_dg && fast)))
-            e = lane_bwd<false, true, M0, M1>(run, st, grid, slots, topo, swap, cond, lay);
+            e = lane_bwd<false, true, M0, M1>(run, which, st, grid, slots, topo, swap, cond, lay);
         if (e == hipSuccess && (!run || (!full_dg && !fast)))
-            e = lane_bwd<false, false, M0, M1>(run, st, grid, slots, topo, swap, cond, lay);
+            e = lane_bwd<false, false, M0, M1>(run, which, st, grid, slots, topo, swap, cond, lay);
         return e;
     }
     return hipErrorInvalidValue;
@@ -799,9 +804,9 @@ hipError_t configure_pairs_lane() {
     for (int m0 = 1; m0 <= 6; ++m0)
         for (int m1 = 0; m1 <= m0; ++m1)
             for_lane_shape(m0, m1, [&]<int A, int B>() {
-                for (int b = 0; b < 2; ++b) {
-                    if (b && !LaneShape<A, B>::backward) continue;
-                    const hipError_t e = lane_dispatch_shape<A, B>(false, b, true, nullptr, dim3(), 0, nullptr,
+                for (int which = 0; which < 2; ++which) {
+                    if (which && !LaneShape<A, B>::backward) continue;
+                    const hipError_t e = lane_dispatch_shape<A, B>(false, which, true, nullptr, dim3(), 0, nullptr,
                                                                    TopoDesc{}, 0, 0.0, LaneBwdLayout{});
                     if (e != hipSuccess) err = e;
                 }
@@ -809,16 +814,16 @@ hipError_t configure_pairs_lane() {
     return err;
 }
 
-void launch_pairs_lane(hipStream_t st, bool backward, int blocks, const SlotDesc* slots, int count,
-                       const TopoDesc& topo, int n_alpha, int n_beta, double cond_limit) {
+void launch_pairs_lane(hipStream_t st, int which, int blocks, const SlotDesc* slots, int count, const TopoDesc& topo,
+                       int n_alpha, int n_beta, double cond_limit) {
     const int swap = n_beta > n_alpha ? 1 : 0;
     const int m0 = swap ? n_beta : n_alpha, m1 = swap ? n_alpha : n_beta;
     const bool full = pairs_full_dg(topo.n, topo.ng);
-    const LaneBwdLayout lay = backward ? lane_bwd_layout(topo.n, topo.ng) : LaneBwdLayout{};
-    const size_t lds = pairs_lane_lds_bytes(topo.n, topo.ng, backward);
+    const LaneBwdLayout lay = which ? lane_bwd_layout(topo.n, topo.ng) : LaneBwdLayout{};
+    const size_t lds = pairs_lane_lds_bytes(topo.n, topo.ng, which != 0);
     const dim3 grid(blocks, 1, count);
     for_lane_shape(m0, m1, [&]<int A, int B>() {
-        (void)lane_dispatch_shape<A, B>(true, backward, full, st, grid, lds, slots, topo, swap, cond_limit, lay);
+        (void)lane_dispatch_shape<A, B>(true, which, full, st, grid, lds, slots, topo, swap, cond_limit, lay);
     });
 }
 

@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "vbe_problem_release", "vbe_problem_device_bytes", "vbe_eval", "vbe_eval_device", "vbe_eval_forward",
     "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
     "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_eval_device",
-    "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_phase_times", "vbe_set_pair_condition_limit",
+    "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit",
 )
 
 
@@ -76,7 +76,8 @@ class Flags(C.Structure):
 class PhaseTimes(C.Structure):
     _fields_ = [(name, C.c_float) for name in (
         "orbital_prep_ms", "transform_ms", "ri_one_index_ms", "pair_forward_ms", "structure_ms", "eigensolve_ms",
-        "pair_backward_ms", "pair_backward_kernel_ms", "gradient_tail_ms", "total_ms")]
+        "pair_backward_ms", "pair_backward_kernel_ms", "gradient_tail_ms", "total_ms", "pair_forward_kernel_ms",
+        "pair_backward_field_ms", "pair_backward_adjoint_ms")]
 
 
 _lib = None
@@ -124,6 +125,7 @@ def library() -> C.CDLL:
         lib.vbe_pool_eval_device.argtypes = [P, I32, P, P, I32, P, P]
         lib.vbe_pool_submit.argtypes = [P, I32, P, P, I32, C.POINTER(I32)]
         lib.vbe_pool_wait.argtypes = [P, I32, P, P]
+        lib.vbe_pool_poll.argtypes = [P, I32, C.POINTER(I32)]
         lib.vbe_pool_phase_times.argtypes = [P, C.POINTER(PhaseTimes)]
         _lib = lib
     return _lib
@@ -418,6 +420,12 @@ class Pool:
         self._pending = getattr(self, "_pending", {})
         self._pending[t.value] = (list(sl), gradient)
         return t.value
+
+    def poll(self, ticket: int) -> bool:
+        """True once the ticket's device work has finished (wait() then does not block)."""
+        d = C.c_int32()
+        self.ctx.check(self.ctx._lib.vbe_pool_poll(self._h, ticket, C.byref(d)))
+        return bool(d.value)
 
     def wait(self, ticket: int):
         slots, gradient = self._pending.pop(ticket)
