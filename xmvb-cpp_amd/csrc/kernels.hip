@@ -267,34 +267,55 @@ __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict
     }
 }
 
-// L[A, (p,q)] = sum_mu T[A,q,mu] Cn[mu,p], q <= p. One thread per (A, packed (p,q)):
-// grid (ceil(n_aux/blockDim), n(n+1)/2).
-__global__ void k_ri_pair_factors(const SlotDesc* __restrict__ slots, int n) {
+// L[A, (p,q)] = sum_mu T[A,q,mu] Cn[mu,p], q <= p. grid (ceil(n_aux/64), ceil(n/4), slots): a
+// wavefront owns 64 aux rows and one q, so every T element is read once; Cn is staged in LDS as
+// [mu][16] chunks (zeros beyond n) and read as broadcasts; NQ = n rounded up to even.
+template <int NQ>
+__global__ void __launch_bounds__(256) k_ri_pair_factors(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
-    const double* __restrict__ T = sd.T;
+    if ((int)blockIdx.x * 64 >= n_aux) return;  // block-uniform
     const double* __restrict__ Cn = sd.Cn;
-    double* __restrict__ L = sd.L;
-    const int A = blockIdx.x * blockDim.x + threadIdx.x;
-    const int P = blockIdx.y;
-    if (A >= n_aux) return;
-    int p = (int)((sqrt(8.0 * P + 1.0) - 1.0) * 0.5);
-    while (p * (p + 1) / 2 > P) --p;
-    while ((p + 1) * (p + 2) / 2 <= P) ++p;
-    const int q = P - p * (p + 1) / 2;
-    const double* t = T + (size_t)A + (size_t)n_aux * q;
-    const double* c = Cn + (size_t)p * n_bf;
+    __shared__ double cT[kOneIndexChunk][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int A = blockIdx.x * 64 + lane;
+    const int Ac = A < n_aux ? A : n_aux - 1;
+    const int q = blockIdx.y * 4 + wave;
+    const int qc = q < n ? q : n - 1;
     const size_t step = (size_t)n_aux * n;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int mu = 0;
-    for (; mu + 3 < n_bf; mu += 4) {
-        a0 += t[step * mu] * c[mu];
-        a1 += t[step * (mu + 1)] * c[mu + 1];
-        a2 += t[step * (mu + 2)] * c[mu + 2];
-        a3 += t[step * (mu + 3)] * c[mu + 3];
+    const double* __restrict__ t = sd.T + (size_t)Ac + (size_t)n_aux * qc;
+    double acc[NQ];
+#pragma unroll
+    for (int p = 0; p < NQ; ++p) acc[p] = 0.0;
+    for (int c0 = 0; c0 < n_bf; c0 += kOneIndexChunk) {
+        const int len = min(kOneIndexChunk, n_bf - c0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kOneIndexChunk * 16; i += blockDim.x) {
+            const int v = i >> 4, p = i & 15;
+            cT[v][p] = (p < n && v < len) ? Cn[(size_t)p * n_bf + c0 + v] : 0.0;
+        }
+        __syncthreads();
+        int v = 0;
+        for (; v + 4 <= len; v += 4) {
+            double tv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) tv[u] = t[step * (c0 + v + u)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int p = 0; p < NQ; ++p) acc[p] += tv[u] * cT[v + u][p];
+        }
+        for (; v < len; ++v) {
+            const double tv = t[step * (c0 + v)];
+#pragma unroll
+            for (int p = 0; p < NQ; ++p) acc[p] += tv * cT[v][p];
+        }
     }
-    for (; mu < n_bf; ++mu) a0 += t[step * mu] * c[mu];
-    L[(size_t)A + (size_t)n_aux * P] = (a0 + a1) + (a2 + a3);
+    if (q >= n || A >= n_aux) return;
+    double* __restrict__ L = sd.L;
+#pragma unroll
+    for (int p = 0; p < NQ; ++p)
+        if (p >= q && p < n) L[(size_t)A + (size_t)n_aux * (p * (p + 1) / 2 + q)] = acc[p];
 }
 
 // packed G[(P,Q)] = sum_A L[A,P] L[A,Q]; one wavefront per packed entry.
@@ -1084,66 +1105,90 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
 // ---------------------------------------------------------------------------------------------
 
 // dL[A,P] = sum_Q M[P,Q] L[A,Q], M[P,Q] = dg(P,Q) off the diagonal, 2 dg(P,P) on it.
-__global__ void k_ri_pair_adjoint(const SlotDesc* __restrict__ slots, int np) {
+// grid (ceil(n_aux/64), ceil(np/8), slots): a thread owns one aux row and eight P, so each L
+// element is read np/8 times instead of np times; M entries are wave-uniform.
+__global__ void __launch_bounds__(64) k_ri_pair_adjoint(const SlotDesc* __restrict__ slots, int np) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_aux = sd.n_aux;
+    if ((int)blockIdx.x * 64 >= n_aux) return;  // block-uniform
     const double* __restrict__ L = sd.L;
     const double* __restrict__ dg = sd.dsum;
-    double* __restrict__ dL = sd.dL;
-    const int A = blockIdx.x * blockDim.x + threadIdx.x;
-    const int P = blockIdx.y;
-    if (A >= n_aux) return;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int A = blockIdx.x * 64 + threadIdx.x;
+    const int Ac = A < n_aux ? A : n_aux - 1;
+    const int P0 = blockIdx.y * 8;
+    double acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] = 0.0;
     for (int Q = 0; Q < np; ++Q) {
-        const double m = (P == Q) ? 2.0 * dg[pair_index(P, P)] : dg[pair_index(P, Q)];
-        acc[Q & 3] += m * L[(size_t)A + (size_t)n_aux * Q];
+        const double l = L[(size_t)Ac + (size_t)n_aux * Q];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int P = P0 + j;
+            if (P < np) acc[j] += (P == Q ? 2.0 : 1.0) * dg[pair_index(P, Q)] * l;
+        }
     }
-    dL[(size_t)A + (size_t)n_aux * P] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    if (A >= n_aux) return;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (P0 + j < np) sd.dL[(size_t)A + (size_t)n_aux * (P0 + j)] = acc[j];
 }
 
 // gcn[mu, p] = sum_i SC[mu,i] (ds+ds^T)[i,p] + HC[mu,i] (dh+dh^T)[i,p]
 //            + sum_q f_pq sum_A dL[A,(p,q)] T[A,q,mu],  f_pq = 2 if p == q else 1.
-// one block per AO index mu.
+// grid (ceil(n_bf/4), 1, slots): a block owns four AO indices mu, so each dL element is read once
+// per four mu; NQ = n rounded up to even.
+constexpr int kGradRows = 4;
+
+template <int NQ>
 __global__ void __launch_bounds__(256) k_orbital_gradient(const SlotDesc* __restrict__ slots, int n, int ng) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
-    if ((int)blockIdx.x >= n_bf) return;  // block-uniform
+    const int mu0 = blockIdx.x * kGradRows;
+    if (mu0 >= n_bf) return;  // block-uniform
     const double* __restrict__ T = sd.T;
     const double* __restrict__ dL = sd.dL;
-    const double* __restrict__ SC = sd.SC;
-    const double* __restrict__ HC = sd.HC;
-    const double* __restrict__ dsdh = sd.dsum;
-    double* __restrict__ gcn = sd.gcn;
-    __shared__ double part[4][MAX_ACT];
-    const int mu = blockIdx.x;
+    __shared__ double part[4][kGradRows][NQ];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double acc[MAX_ACT];
+    const double* tmu[kGradRows];
 #pragma unroll
-    for (int p = 0; p < MAX_ACT; ++p) acc[p] = 0.0;
-    // contraction over the contiguous (A, q) plane of T for this mu
-    const double* tmu = T + (size_t)n_aux * n * mu;
-    for (int k = threadIdx.x; k < n_aux * n; k += blockDim.x) {
-        const int q = k / n_aux, A = k - q * n_aux;
-        const double t = tmu[k];
+    for (int u = 0; u < kGradRows; ++u) tmu[u] = T + (size_t)n_aux * n * min(mu0 + u, n_bf - 1);
+    double acc[kGradRows][NQ];
 #pragma unroll
-        for (int p = 0; p < MAX_ACT; ++p)
-            if (p < n) acc[p] += (p == q ? 2.0 : 1.0) * dL[(size_t)A + (size_t)n_aux * pair_index(p, q)] * t;
-    }
+    for (int u = 0; u < kGradRows; ++u)
 #pragma unroll
-    for (int p = 0; p < MAX_ACT; ++p) {
-        const double s = wave_sum(acc[p]);
-        if (lane == 0 && p < n) part[wave][p] = s;
-    }
-    __syncthreads();
-    const double* ds = dsdh + ng;
-    const double* dh = dsdh + ng + n * n;
-    for (int p = threadIdx.x; p < n; p += blockDim.x) {
-        double g = (part[0][p] + part[1][p]) + (part[2][p] + part[3][p]);
-        for (int i = 0; i < n; ++i) {
-            g += SC[mu + (size_t)n_bf * i] * (ds[i + n * p] + ds[p + n * i]);
-            g += HC[mu + (size_t)n_bf * i] * (dh[i + n * p] + dh[p + n * i]);
+        for (int p = 0; p < NQ; ++p) acc[u][p] = 0.0;
+    for (int q = 0; q < n; ++q)
+        for (int A = threadIdx.x; A < n_aux; A += blockDim.x) {
+            double t[kGradRows];
+#pragma unroll
+            for (int u = 0; u < kGradRows; ++u) t[u] = tmu[u][(size_t)n_aux * q + A];
+#pragma unroll
+            for (int p = 0; p < NQ; ++p) {
+                if (p >= n) continue;
+                const double w = (p == q ? 2.0 : 1.0) * dL[(size_t)A + (size_t)n_aux * pair_index(p, q)];
+#pragma unroll
+                for (int u = 0; u < kGradRows; ++u) acc[u][p] += w * t[u];
+            }
         }
-        gcn[mu + (size_t)n_bf * p] = g;
+#pragma unroll
+    for (int u = 0; u < kGradRows; ++u)
+#pragma unroll
+        for (int p = 0; p < NQ; ++p) {
+            const double s = wave_sum(acc[u][p]);
+            if (lane == 0) part[wave][u][p] = s;
+        }
+    __syncthreads();
+    const double* ds = sd.dsum + ng;
+    const double* dh = ds + n * n;
+    for (int k = threadIdx.x; k < kGradRows * n; k += blockDim.x) {
+        const int u = k / n, p = k - u * n, mu = mu0 + u;
+        if (mu >= n_bf) continue;
+        double g = (part[0][u][p] + part[1][u][p]) + (part[2][u][p] + part[3][u][p]);
+        for (int i = 0; i < n; ++i) {
+            g += sd.SC[mu + (size_t)n_bf * i] * (ds[i + n * p] + ds[p + n * i]);
+            g += sd.HC[mu + (size_t)n_bf * i] * (dh[i + n * p] + dh[p + n * i]);
+        }
+        sd.gcn[mu + (size_t)n_bf * p] = g;
     }
 }
 
@@ -1294,7 +1339,18 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
         }
     }
     mark(EV_ONE_INDEX);
-    hipLaunchKernelGGL(k_ri_pair_factors, dim3((shape.max_n_aux + 63) / 64, np, z), dim3(64), 0, st, slots, n);
+    {
+        const dim3 grid((shape.max_n_aux + 63) / 64, (n + 3) / 4, z);
+        switch ((n + 1) / 2) {
+            case 1: hipLaunchKernelGGL(k_ri_pair_factors<2>, grid, dim3(256), 0, st, slots, n); break;
+            case 2: hipLaunchKernelGGL(k_ri_pair_factors<4>, grid, dim3(256), 0, st, slots, n); break;
+            case 3: hipLaunchKernelGGL(k_ri_pair_factors<6>, grid, dim3(256), 0, st, slots, n); break;
+            case 4: hipLaunchKernelGGL(k_ri_pair_factors<8>, grid, dim3(256), 0, st, slots, n); break;
+            case 5: hipLaunchKernelGGL(k_ri_pair_factors<10>, grid, dim3(256), 0, st, slots, n); break;
+            case 6: hipLaunchKernelGGL(k_ri_pair_factors<12>, grid, dim3(256), 0, st, slots, n); break;
+            default: hipLaunchKernelGGL(k_ri_pair_factors<14>, grid, dim3(256), 0, st, slots, n); break;
+        }
+    }
     hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4, 1, z), dim3(256), 0, st, slots, np);
     mark(EV_TRANSFORM);
     if (shape.lane_fwd) {
@@ -1350,8 +1406,19 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
     const int len = ng + 2 * n * n;
     hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, part_rows, len);
     mark(EV_PAIR_BWD);
-    hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((shape.max_n_aux + 63) / 64, np, z), dim3(64), 0, st, slots, np);
-    hipLaunchKernelGGL(k_orbital_gradient, dim3(shape.max_n_bf, 1, z), dim3(256), 0, st, slots, n, ng);
+    hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((shape.max_n_aux + 63) / 64, (np + 7) / 8, z), dim3(64), 0, st, slots, np);
+    {
+        const dim3 grid((shape.max_n_bf + kGradRows - 1) / kGradRows, 1, z);
+        switch ((n + 1) / 2) {
+            case 1: hipLaunchKernelGGL(k_orbital_gradient<2>, grid, dim3(256), 0, st, slots, n, ng); break;
+            case 2: hipLaunchKernelGGL(k_orbital_gradient<4>, grid, dim3(256), 0, st, slots, n, ng); break;
+            case 3: hipLaunchKernelGGL(k_orbital_gradient<6>, grid, dim3(256), 0, st, slots, n, ng); break;
+            case 4: hipLaunchKernelGGL(k_orbital_gradient<8>, grid, dim3(256), 0, st, slots, n, ng); break;
+            case 5: hipLaunchKernelGGL(k_orbital_gradient<10>, grid, dim3(256), 0, st, slots, n, ng); break;
+            case 6: hipLaunchKernelGGL(k_orbital_gradient<12>, grid, dim3(256), 0, st, slots, n, ng); break;
+            default: hipLaunchKernelGGL(k_orbital_gradient<14>, grid, dim3(256), 0, st, slots, n, ng); break;
+        }
+    }
     hipLaunchKernelGGL(k_normalization_pullback, dim3(n, 1, z), dim3(256), 0, st, slots);
     mark(EV_TAIL);
 }
