@@ -164,6 +164,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=300, help="timed single-problem evaluations")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the served leg")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the served leg (0: skip)")
+    ap.add_argument("--mixed-steps", type=int, default=30, help="timed batched steps with FP32 RI factors (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle parity")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -289,6 +290,31 @@ def main():
     roofline = roof(dominant)
     others = [roof(k) for k in kernels if k != dominant]
 
+    # ---- the same batch with FP32 RI factors (FP64 accumulation and everything downstream) ----
+    mixed = None
+    if args.mixed_steps > 0:
+        ctx32 = vb.Context(0)
+        ctx32.set_ri_precision(1)
+        pool32 = vb.Pool(ctx32, topo, capacity=K)
+        e64 = np.array([float(x) for x in energies])
+        for s in range(K):
+            sps = sp if s == 0 else vb.synthetic_problem(cfg["seed"] + s, cfg["n_bf"], cfg["n_act"], cfg["n_aux"])
+            pool32.load_synthetic(s, sps)
+        for _ in range(args.warmup):
+            pool32.evaluate_device(slots, cb_dev.data_ptr(), gb_dev.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.mixed_steps):
+            e32 = pool32.evaluate_device(slots, cb_dev.data_ptr(), gb_dev.data_ptr())
+        torch.cuda.synchronize()
+        el32 = time.perf_counter() - t0
+        mixed = dict(value=K * args.mixed_steps / el32, unit="E+grad evals/s", ri_storage="fp32",
+                     max_abs_dE_vs_fp64=float(np.abs(np.asarray(e32) - e64).max()),
+                     one_index_ms=pool32.phase_times()["ri_one_index_ms"],
+                     note="|dE| above 1e-8 Eh on some problems: not the headline path")
+        pool32.close()
+        ctx32.close()
+
     served = None
     if args.serve_seconds > 0:
         served = serve(vb, ctx, topo, cfg, args.serve_rate, args.serve_seconds, replicas=4, max_batch=K)
@@ -307,7 +333,7 @@ def main():
                latency_source=("served leg: request completion minus Poisson arrival" if served
                                else "batched step"),
                batch_step_ms=dict(p50=percentile(lat, 0.5), p99=percentile(lat, 0.99)),
-               plain_fp64=plain, speedup_vs_plain=value / plain["value"], served=served,
+               plain_fp64=plain, speedup_vs_plain=value / plain["value"], served=served, fp32_ri=mixed,
                batch_consistency=batch_consistency, phase_ms={k: round(v, 5) for k, v in mean.items()},
                energy=energy, roofline=roofline, roofline_other_kernels=others)
     if not args.no_cpu:

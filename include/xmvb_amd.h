@@ -58,7 +58,10 @@ typedef enum vbe_method {
 } vbe_method;
 
 typedef enum vbe_precision {
-    VBE_PRECISION_FP64 = 0 /* plain FP64 path; mixed-precision transforms are rejected for now */
+    VBE_PRECISION_FP64 = 0,    /* everything FP64 */
+    VBE_PRECISION_FP32_RI = 1  /* RI factors B stored in FP32 (half the memory and transform traffic),
+                                  the transform accumulating in FP64; determinant pairs, structure
+                                  matrices, metric Cholesky, eigensolve and gradient stay FP64 */
 } vbe_precision;
 
 typedef enum vbe_eigensolver {
@@ -174,6 +177,10 @@ int vbe_last_error(const vbe_context* ctx, char* buf, size_t len);
  * reference's pivot order, which keeps their rounding close to it. Applies to later evaluations.
  * No counterpart in the reference, which has a single (CPU) pair path. */
 int vbe_set_pair_condition_limit(vbe_context* ctx, double limit);
+/* Storage precision of the RI factors for problems and pools created after the call
+ * (vbe_precision; default VBE_PRECISION_FP64). Evaluations must then pass the same precision in
+ * vbe_flags. No counterpart in the reference, whose RI factors are FP64 only. */
+int vbe_set_ri_precision(vbe_context* ctx, int32_t precision);
 const char* vbe_global_error(void);
 
 /* Upload the molecule-static data of one problem: AO overlap S and core Hamiltonian h

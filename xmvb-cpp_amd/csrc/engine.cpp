@@ -50,6 +50,7 @@ struct vbe_context {
     uint64_t budget = 0;
     uint64_t in_use = 0;
     double pair_condition_limit = xmvb_amd::kLaneConditionDefault;
+    int ri_precision = VBE_PRECISION_FP64;
     std::string error;
 };
 
@@ -121,6 +122,7 @@ struct Lane {
 struct vbe_pool {
     vbe_context* ctx = nullptr;
     int n_act = 0, n_alpha = 0, n_beta = 0, capacity = 0;
+    int ri_precision = VBE_PRECISION_FP64;
     TopoDesc topo{};
     DeviceArena topo_arena;
     std::vector<Slot> slots;
@@ -227,6 +229,7 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     check(hipSetDevice(ctx->device), "hipSetDevice");
     auto p = std::make_unique<vbe_pool>();
     p->ctx = ctx;
+    p->ri_precision = ctx->ri_precision;
     p->n_act = n_act;
     p->n_alpha = n_alpha;
     p->n_beta = n_beta;
@@ -330,7 +333,8 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
         SlotDesc d{};
         d.S = A.alloc<double>((size_t)n_bf * n_bf);
         d.h = A.alloc<double>((size_t)n_bf * n_bf);
-        d.B = A.alloc<double>((size_t)n_aux * nbp);
+        if (p->ri_precision == VBE_PRECISION_FP32_RI) d.B32 = A.alloc<float>((size_t)n_aux * nbp);
+        else d.B = A.alloc<double>((size_t)n_aux * nbp);
         d.Cn = A.alloc<double>(n * n_bf);
         d.SC = A.alloc<double>(n * n_bf);
         d.HC = A.alloc<double>(n * n_bf);
@@ -371,7 +375,14 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
     hipStream_t st = p->lanes[0].stream;
     upload(const_cast<double*>(s.desc.S), S, (size_t)n_bf * n_bf, st);
     upload(const_cast<double*>(s.desc.h), h, (size_t)n_bf * n_bf, st);
-    upload(const_cast<double*>(s.desc.B), B, (size_t)n_aux * nbp, st);
+    if (p->ri_precision == VBE_PRECISION_FP32_RI) {
+        std::vector<float> b32((size_t)n_aux * nbp);
+        for (size_t k = 0; k < b32.size(); ++k) b32[k] = (float)B[k];
+        upload(const_cast<float*>(s.desc.B32), b32.data(), b32.size(), st);
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");  // b32 is a host temporary
+    } else {
+        upload(const_cast<double*>(s.desc.B), B, (size_t)n_aux * nbp, st);
+    }
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
     s.loaded = true;
 }
@@ -409,6 +420,7 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
                                                     std::min<long long>(lane_sweep, bwd_blocks_cap(p)));
     sh.list_blocks = xmvb_amd::kListBlocks;
     sh.lane_cond_limit = p->ctx->pair_condition_limit;
+    sh.b32 = p->ri_precision == VBE_PRECISION_FP32_RI;
     return sh;
 }
 
@@ -526,9 +538,12 @@ void ensure_stage(vbe_pool* p, Lane& l, size_t doubles) {
     l.stage_doubles = doubles;
 }
 
-void check_flags_args(const vbe_flags& flags) {
-    if (flags.precision != VBE_PRECISION_FP64)
-        throw StatusError(VBE_ERR_UNSUPPORTED, "only the FP64 precision path is implemented");
+void check_flags_args(const vbe_flags& flags, const vbe_pool* p) {
+    if (flags.precision != VBE_PRECISION_FP64 && flags.precision != VBE_PRECISION_FP32_RI)
+        throw StatusError(VBE_ERR_ARGUMENT, "unknown precision");
+    if (flags.precision != p->ri_precision)
+        throw StatusError(VBE_ERR_ARGUMENT, "precision differs from the one the problem was uploaded with "
+                                            "(vbe_set_ri_precision)");
     if (flags.eigensolver != VBE_EIGENSOLVER_DENSE)
         throw StatusError(VBE_ERR_UNSUPPORTED, "only EIGENSOLVER=DENSE is implemented");
 }
@@ -579,6 +594,14 @@ int vbe_context_create(int32_t device, uint64_t hbm_budget_bytes, vbe_context** 
 }
 
 void vbe_context_destroy(vbe_context* ctx) { delete ctx; }
+
+int vbe_set_ri_precision(vbe_context* ctx, int32_t precision) {
+    if (!ctx) return fail(nullptr, VBE_ERR_ARGUMENT, "null context");
+    if (precision != VBE_PRECISION_FP64 && precision != VBE_PRECISION_FP32_RI)
+        return fail(ctx, VBE_ERR_ARGUMENT, "unknown precision");
+    ctx->ri_precision = precision;
+    return VBE_OK;
+}
 
 int vbe_set_pair_condition_limit(vbe_context* ctx, double limit) {
     if (!ctx) return fail(nullptr, VBE_ERR_ARGUMENT, "null context");
@@ -776,7 +799,7 @@ int vbe_eval(vbe_context* ctx, vbe_problem* prob, const double* C_raw, const uin
     return guarded(ctx, [&]() -> int {
         if (!ctx || !prob || !C_raw || !E_total) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
         if (prob->pool->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
-        check_flags_args(flags);
+        check_flags_args(flags, prob->pool.get());
         if (flags.want_gradient && !grad_raw) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
         vbe_pool* p = prob->pool.get();
         Slot& s = p->slots[0];
@@ -801,7 +824,7 @@ int vbe_eval_device(vbe_context* ctx, vbe_problem* prob, const double* d_C_raw, 
     return guarded(ctx, [&]() -> int {
         if (!ctx || !prob || !d_C_raw || !E_total) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
         if (prob->pool->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
-        check_flags_args(flags);
+        check_flags_args(flags, prob->pool.get());
         if (flags.want_gradient && !d_grad_raw) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
         *E_total = eval_problem(prob, d_C_raw, d_support, flags.want_gradient != 0, d_grad_raw, true);
         return VBE_OK;
@@ -810,7 +833,7 @@ int vbe_eval_device(vbe_context* ctx, vbe_problem* prob, const double* d_C_raw, 
 
 int vbe_eval_forward(vbe_context* ctx, vbe_problem* prob, const double* C_raw, const uint8_t* support,
                      double* E_total) {
-    vbe_flags f{VBE_PRECISION_FP64, VBE_EIGENSOLVER_DENSE, 0};
+    vbe_flags f{prob ? prob->pool->ri_precision : VBE_PRECISION_FP64, VBE_EIGENSOLVER_DENSE, 0};
     return vbe_eval(ctx, prob, C_raw, support, f, E_total, nullptr, nullptr);
 }
 

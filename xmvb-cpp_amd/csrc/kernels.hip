@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "device.hpp"
 #include "kernels.hpp"
@@ -208,7 +209,7 @@ __global__ void k_active_matrices(const SlotDesc* __restrict__ slots, int n) {
 constexpr int kOneIndexChunk = 64;
 constexpr int kOneIndexRows = 4;  // AO rows per wavefront
 
-template <int NQ>
+template <int NQ, typename BT>
 __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
@@ -223,7 +224,7 @@ __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict
     int mu[kOneIndexRows];
 #pragma unroll
     for (int u = 0; u < kOneIndexRows; ++u) mu[u] = min(mu0 + u, n_bf - 1);
-    const double* __restrict__ Bp = sd.B + Ac;
+    const BT* __restrict__ Bp = (std::is_same_v<BT, float> ? (const BT*)sd.B32 : (const BT*)sd.B) + Ac;
     double acc[kOneIndexRows][NQ];
 #pragma unroll
     for (int u = 0; u < kOneIndexRows; ++u)
@@ -244,8 +245,8 @@ __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict
             double b1[kOneIndexRows], b2[kOneIndexRows];
 #pragma unroll
             for (int u = 0; u < kOneIndexRows; ++u) {
-                b1[u] = Bp[(size_t)n_aux * pair_index(mu[u], nu1)];
-                b2[u] = Bp[(size_t)n_aux * pair_index(mu[u], nu2)];
+                b1[u] = (double)Bp[(size_t)n_aux * pair_index(mu[u], nu1)];
+                b2[u] = (double)Bp[(size_t)n_aux * pair_index(mu[u], nu2)];
             }
             const double* c1 = cT[v];
             const double* c2 = cT[v + 1];  // row len (if reached) is all zeros
@@ -1328,14 +1329,19 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
     mark(EV_PREP);
     {
         const dim3 grid((shape.max_n_aux + 63) / 64, (shape.max_n_bf + 4 * kOneIndexRows - 1) / (4 * kOneIndexRows), z);
+        auto go = [&](auto nq) {
+            constexpr int NQ = decltype(nq)::value;
+            if (shape.b32) hipLaunchKernelGGL((k_ri_one_index<NQ, float>), grid, dim3(256), 0, st, slots, n);
+            else hipLaunchKernelGGL((k_ri_one_index<NQ, double>), grid, dim3(256), 0, st, slots, n);
+        };
         switch ((n + 1) / 2) {
-            case 1: hipLaunchKernelGGL(k_ri_one_index<2>, grid, dim3(256), 0, st, slots, n); break;
-            case 2: hipLaunchKernelGGL(k_ri_one_index<4>, grid, dim3(256), 0, st, slots, n); break;
-            case 3: hipLaunchKernelGGL(k_ri_one_index<6>, grid, dim3(256), 0, st, slots, n); break;
-            case 4: hipLaunchKernelGGL(k_ri_one_index<8>, grid, dim3(256), 0, st, slots, n); break;
-            case 5: hipLaunchKernelGGL(k_ri_one_index<10>, grid, dim3(256), 0, st, slots, n); break;
-            case 6: hipLaunchKernelGGL(k_ri_one_index<12>, grid, dim3(256), 0, st, slots, n); break;
-            default: hipLaunchKernelGGL(k_ri_one_index<14>, grid, dim3(256), 0, st, slots, n); break;
+            case 1: go(std::integral_constant<int, 2>{}); break;
+            case 2: go(std::integral_constant<int, 4>{}); break;
+            case 3: go(std::integral_constant<int, 6>{}); break;
+            case 4: go(std::integral_constant<int, 8>{}); break;
+            case 5: go(std::integral_constant<int, 10>{}); break;
+            case 6: go(std::integral_constant<int, 12>{}); break;
+            default: go(std::integral_constant<int, 14>{}); break;
         }
     }
     mark(EV_ONE_INDEX);

@@ -38,7 +38,8 @@ struct SlotDesc {
     // molecule-static data
     const double* S;
     const double* h;
-    const double* B;
+    const double* B;    // RI factors, FP64 storage (or null)
+    const float* B32;   // RI factors, FP32 storage (or null)
     int n_bf, n_aux;
     double e_nuc;
     // current evaluation: inputs and outputs
@@ -78,6 +79,7 @@ struct LaunchShape {
     int lane_blocks_fwd, lane_blocks_bwd;  // their blocks per slot
     int list_blocks;                       // lane-group blocks per slot for the deferred pairs
     double lane_cond_limit;                // lane kernel keeps blocks with max|X| max|X^-1| below this
+    bool b32;                              // RI factors stored in FP32
 };
 
 // largest condition bound under which a row-pivoted lane block is provably regular under the

@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "vbe_problem_release", "vbe_problem_device_bytes", "vbe_eval", "vbe_eval_device", "vbe_eval_forward",
     "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
     "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_eval_device",
-    "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit",
+    "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit", "vbe_set_ri_precision",
 )
 
 
@@ -106,6 +106,7 @@ def library() -> C.CDLL:
         lib.vbe_context_destroy.restype = None
         lib.vbe_last_error.argtypes = [P, C.c_char_p, C.c_size_t]
         lib.vbe_set_pair_condition_limit.argtypes = [P, D]
+        lib.vbe_set_ri_precision.argtypes = [P, I32]
         lib.vbe_problem_upload.argtypes = [P, C.POINTER(Dims), P, P, P, C.POINTER(TopologyView), D, C.POINTER(P)]
         lib.vbe_problem_release.argtypes = [P]
         lib.vbe_problem_release.restype = None
@@ -269,6 +270,12 @@ class Context:
         complete-pivoting path; 0 sends every pair there."""
         self.check(self._lib.vbe_set_pair_condition_limit(self._h, C.c_double(limit)))
 
+    def set_ri_precision(self, precision: int):
+        """0: FP64 RI factors; 1: FP32 RI factors (FP64 accumulation). Applies to problems and pools
+        created afterwards."""
+        self.check(self._lib.vbe_set_ri_precision(self._h, precision))
+        self.ri_precision = precision
+
     def close(self):
         if self._h:
             self._lib.vbe_context_destroy(self._h)
@@ -287,6 +294,7 @@ class Problem:
     def __init__(self, ctx: Context, n_bf: int, n_aux: int, topo: Topology, S: np.ndarray, h: np.ndarray,
                  B: np.ndarray, e_nuc: float = 0.0):
         self.ctx, self.topo = ctx, topo
+        self.precision = getattr(ctx, "ri_precision", 0)
         self.n_bf, self.n_act, self.n_aux = n_bf, topo.n_act, n_aux
         dims = Dims(n_bf, topo.n_act, topo.n_alpha, topo.n_beta, n_aux)
         self._S = np.asfortranarray(S, dtype=np.float64)
@@ -313,7 +321,7 @@ class Problem:
         sup = None if support is None else np.ascontiguousarray(support, dtype=np.uint8)
         e = C.c_double()
         g = np.zeros_like(Cm) if gradient else None
-        self.ctx.check(self.ctx._lib.vbe_eval(self.ctx._h, self._h, _ptr(Cm), _ptr(sup), Flags(0, 0, int(gradient)),
+        self.ctx.check(self.ctx._lib.vbe_eval(self.ctx._h, self._h, _ptr(Cm), _ptr(sup), Flags(self.precision, 0, int(gradient)),
                                               C.byref(e), None, _ptr(g)))
         return (e.value, g) if gradient else e.value
 
@@ -334,7 +342,7 @@ class Problem:
         e = C.c_double()
         self.ctx.check(self.ctx._lib.vbe_eval_device(self.ctx._h, self._h, C.c_void_p(C_ptr),
                                                      None if support_ptr is None else C.c_void_p(support_ptr),
-                                                     Flags(0, 0, int(gradient)), C.byref(e), C.c_void_p(grad_ptr)))
+                                                     Flags(self.precision, 0, int(gradient)), C.byref(e), C.c_void_p(grad_ptr)))
         return e.value
 
     def intermediates(self) -> dict:
