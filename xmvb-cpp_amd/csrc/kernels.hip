@@ -34,6 +34,11 @@ __device__ __forceinline__ double wave_bcast(double v, int src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// returns `old` with lane `dst` replaced by the wave-uniform value v (one compare and select)
+__device__ __forceinline__ double wave_put(double v, int dst, double old) {
+    return (int)(threadIdx.x & 63) == dst ? v : old;
+}
+
 // 64-bit DPP move (both halves with the same control)
 template <int Ctrl>
 __device__ __forceinline__ double dpp_f64(double v) {
@@ -817,9 +822,9 @@ __global__ void k_structure_left(const SlotDesc* __restrict__ slots, TopoDesc to
 
 // ---------------------------------------------------------------------------------------------
 // generalised eigenproblem H c = E S c, lowest root, one workgroup.
-// Cholesky S = L L^T, M = L^{-1} H L^{-T}, Householder tridiagonalisation, multisection on
-// Sturm counts for the lowest eigenvalue, inverse iteration on the tridiagonal,
-// back-transformation, c = L^{-T} z so that c^T S c = 1.
+// Cholesky S = L L^T, explicit L^{-1}, M = L^{-1} H L^{-T} by two triangular products,
+// Householder tridiagonalisation, multisection on Sturm counts for the lowest eigenvalue, inverse
+// iteration on the tridiagonal, back-transformation, c = L^{-T} z so that c^T S c = 1.
 // Storage: gep_work_doubles(n) = 3*n*n + 9*n doubles, in LDS when it fits (InLds) else in the
 // global workspace. A single wavefront is used for small n so that the block barriers are free.
 // ---------------------------------------------------------------------------------------------
@@ -858,13 +863,17 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
     __shared__ double lo_hi[2];
     __shared__ int first_true;
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int i = tid; i < n; i += nt)
-        for (int j = 0; j < n; ++j) {
-            Lm[i + n * j] = 0.5 * (Ss[i + n * j] + Ss[j + n * i]);
-            M[i + n * j] = 0.5 * (Hs[i + n * j] + Hs[j + n * i]);
-        }
+    // 2D thread layout for element-parallel updates: TX consecutive lanes share a row
+    const int TX = nt == 64 ? 8 : 16, TY = nt / TX, tx = tid % TX, ty = tid / TX;
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int i = idx % n, j = idx / n;
+        Lm[idx] = 0.5 * (Ss[i + n * j] + Ss[j + n * i]);
+        M[idx] = 0.5 * (Hs[i + n * j] + Hs[j + n * i]);
+        V[idx] = 0.0;
+    }
     __syncthreads();
-    // Cholesky (right-looking, lower); threads own rows, so no index division is needed
+    // Cholesky S = L L^T (right-looking; the lower triangle of Lm becomes L), trailing update
+    // spread over all threads element by element
     for (int k = 0; k < n; ++k) {
         const double a = Lm[k + n * k];
         if (!(a > 0.0)) {
@@ -875,56 +884,75 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
         __syncthreads();
         for (int i = k + tid; i < n; i += nt) Lm[i + n * k] = (i == k) ? dk : Lm[i + n * k] * rdk;
         __syncthreads();
-        for (int i = k + 1 + tid; i < n; i += nt) {
+        for (int i = k + 1 + ty; i < n; i += TY) {
             const double lik = Lm[i + n * k];
-            for (int j = k + 1; j <= i; ++j) Lm[i + n * j] -= lik * Lm[j + n * k];
+            for (int j = k + 1 + tx; j <= i; j += TX) Lm[i + n * j] -= lik * Lm[j + n * k];
         }
         __syncthreads();
     }
-    // M <- L^{-1} H by forward substitution, one column per thread (no barriers inside)
-    for (int j = tid; j < n; j += nt) {
-        for (int i = 0; i < n; ++i) {
-            double v0 = M[i + n * j], v1 = 0.0, v2 = 0.0, v3 = 0.0;
-            int k = 0;
-            for (; k + 3 < i; k += 4) {
-                v0 -= Lm[i + n * k] * M[k + n * j];
-                v1 -= Lm[i + n * (k + 1)] * M[k + 1 + n * j];
-                v2 -= Lm[i + n * (k + 2)] * M[k + 2 + n * j];
-                v3 -= Lm[i + n * (k + 3)] * M[k + 3 + n * j];
+    // Linv = L^{-1} into V (lower triangle), one row per sweep: V[i,j] = -(sum_{k=j}^{i-1} L[i,k] V[k,j]) / L[i,i]
+    for (int i = 0; i < n; ++i) {
+        const double rl = 1.0 / Lm[i + n * i];
+        for (int j = tid; j <= i; j += nt) {
+            if (j == i) {
+                V[i + n * i] = rl;
+            } else {
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                int k = j;
+                for (; k + 3 < i; k += 4) {
+                    s0 += Lm[i + n * k] * V[k + n * j];
+                    s1 += Lm[i + n * (k + 1)] * V[k + 1 + n * j];
+                    s2 += Lm[i + n * (k + 2)] * V[k + 2 + n * j];
+                    s3 += Lm[i + n * (k + 3)] * V[k + 3 + n * j];
+                }
+                for (; k < i; ++k) s0 += Lm[i + n * k] * V[k + n * j];
+                V[i + n * j] = -((s0 + s1) + (s2 + s3)) * rl;
             }
-            for (; k < i; ++k) v0 -= Lm[i + n * k] * M[k + n * j];
-            M[i + n * j] = ((v0 + v1) + (v2 + v3)) / Lm[i + n * i];
         }
+        __syncthreads();
+    }
+    // standard form A = Linv H Linv^T: T1 = Linv H into Lm (L is not needed any more), A into M.
+    // Consecutive lanes take consecutive columns of one T1 row so that the trip count (i + 1) is
+    // uniform across them; H is symmetric, so H[k,j] is read as H[j,k] (unit stride across lanes).
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int j = idx % n, i = idx / n;
+        double s0 = 0.0, s1 = 0.0;
+        int k = 0;
+        for (; k + 1 <= i; k += 2) {
+            s0 += V[i + n * k] * M[j + n * k];
+            s1 += V[i + n * (k + 1)] * M[j + n * (k + 1)];
+        }
+        for (; k <= i; ++k) s0 += V[i + n * k] * M[j + n * k];
+        Lm[i + n * j] = s0 + s1;
     }
     __syncthreads();
-    // M <- L^{-1} M^T (transpose staged through V), then symmetrise
-    for (int i = tid; i < n; i += nt)
-        for (int j = 0; j < n; ++j) V[i + n * j] = M[j + n * i];
-    __syncthreads();
-    for (int j = tid; j < n; j += nt) {
-        for (int i = 0; i < n; ++i) {
-            double v0 = V[i + n * j], v1 = 0.0, v2 = 0.0, v3 = 0.0;
-            int k = 0;
-            for (; k + 3 < i; k += 4) {
-                v0 -= Lm[i + n * k] * M[k + n * j];
-                v1 -= Lm[i + n * (k + 1)] * M[k + 1 + n * j];
-                v2 -= Lm[i + n * (k + 2)] * M[k + 2 + n * j];
-                v3 -= Lm[i + n * (k + 3)] * M[k + 3 + n * j];
-            }
-            for (; k < i; ++k) v0 -= Lm[i + n * k] * M[k + n * j];
-            M[i + n * j] = ((v0 + v1) + (v2 + v3)) / Lm[i + n * i];
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int i = idx % n, j = idx / n;
+        double s0 = 0.0, s1 = 0.0;
+        int k = 0;
+        for (; k + 1 <= j; k += 2) {
+            s0 += Lm[i + n * k] * V[j + n * k];
+            s1 += Lm[i + n * (k + 1)] * V[j + n * (k + 1)];
         }
+        for (; k <= j; ++k) s0 += Lm[i + n * k] * V[j + n * k];
+        M[idx] = s0 + s1;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += nt)
-        for (int j = 0; j < n; ++j) {
-            V[i + n * j] = 0.0;
-            if (i < j) {
-                const double s = 0.5 * (M[i + n * j] + M[j + n * i]);
-                M[i + n * j] = s;
-                M[j + n * i] = s;
-            }
+    // symmetrise A; keep Linv in Lm (for c = Linv^T z at the end) and clear V for the reflectors
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int i = idx % n, j = idx / n;
+        Lm[idx] = (i >= j) ? V[idx] : 0.0;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int i = idx % n, j = idx / n;
+        V[idx] = 0.0;
+        if (i < j) {
+            const double s2 = 0.5 * (M[i + n * j] + M[j + n * i]);
+            M[i + n * j] = s2;
+            M[j + n * i] = s2;
         }
+    }
     __syncthreads();
     // Householder tridiagonalisation: reflect rows/cols k+1..n-1 of column k
     for (int k = 0; k < n - 2; ++k) {
@@ -939,17 +967,17 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
         const double bk = (sigma > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
         if (tid == 0) beta[k] = bk;
         __syncthreads();
-        for (int i = m0 + tid; i < n; i += nt) {  // p = bk A v
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int j = m0;
-            for (; j + 3 < n; j += 4) {
-                a0 += M[i + n * j] * V[j + n * k];
-                a1 += M[i + n * (j + 1)] * V[j + 1 + n * k];
-                a2 += M[i + n * (j + 2)] * V[j + 2 + n * k];
-                a3 += M[i + n * (j + 3)] * V[j + 3 + n * k];
-            }
-            for (; j < n; ++j) a0 += M[i + n * j] * V[j + n * k];
-            y[i] = bk * ((a0 + a1) + (a2 + a3));
+        // p = bk A v: each row split over the TX lanes of a row group, summed across those lanes
+        for (int i0 = m0; i0 < n; i0 += TY) {
+            const int i = i0 + ty;
+            double a0 = 0.0;
+            if (i < n)
+                for (int j = m0 + tx; j < n; j += TX) a0 += M[i + n * j] * V[j + n * k];
+            a0 += dpp_f64<0xB1>(a0);
+            a0 += dpp_f64<0x4E>(a0);
+            a0 += dpp_f64<0x141>(a0);
+            if (TX == 16) a0 += dpp_f64<0x140>(a0);
+            if (i < n && tx == 0) y[i] = bk * a0;
         }
         __syncthreads();
         double kp = 0.0;
@@ -957,9 +985,9 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
         const double K = 0.5 * bk * block_sum(kp, red);
         for (int i = m0 + tid; i < n; i += nt) y[i] -= K * V[i + n * k];  // w = p - K v
         __syncthreads();
-        for (int i = m0 + tid; i < n; i += nt) {  // A -= v w^T + w v^T
+        for (int i = m0 + ty; i < n; i += TY) {  // A -= v w^T + w v^T
             const double vi = V[i + n * k], wi = y[i];
-            for (int j = m0; j < n; ++j) M[i + n * j] -= vi * y[j] + wi * V[j + n * k];
+            for (int j = m0 + tx; j < n; j += TX) M[i + n * j] -= vi * y[j] + wi * V[j + n * k];
         }
         if (bk != 0.0)
             for (int i = m0 + tid; i < n; i += nt) {
@@ -974,121 +1002,235 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
         ee[i] = (i + 1 < n) ? M[(i + 1) + n * i] : 0.0;
     }
     __syncthreads();
-    // lowest eigenvalue by multisection on Sturm counts
-    if (tid == 0) {
-        double lo = 1e300, hi = -1e300;
-        for (int i = 0; i < n; ++i) {
-            const double rad = (i > 0 ? fabs(ee[i - 1]) : 0.0) + (i + 1 < n ? fabs(ee[i]) : 0.0);
-            lo = fmin(lo, dd[i] - rad);
-            hi = fmax(hi, dd[i] + rad);
+    // Lowest eigenvalue by multisection on Sturm counts, then its eigenvector by inverse
+    // iteration on T - lambda I (LU with partial pivoting, LAPACK dgttrf/dgttrs scheme).
+    // For n <= 64 the tridiagonal lives in registers, lane i holding row i, and the serial
+    // recurrences read and write single lanes (readlane/writelane) instead of going through LDS.
+    if (n <= 64) {
+        const int ln = tid & 63;
+        const double d_r = ln < n ? dd[ln] : 0.0;
+        const double e_r = ln + 1 < n ? ee[ln] : 0.0;   // T[ln+1, ln]
+        const double e2_r = ln > 0 && ln < n ? ee[ln - 1] * ee[ln - 1] : 0.0;
+        if (tid < 64) {
+            const double e_prev = ln > 0 && ln < n ? fabs(ee[ln - 1]) : 0.0;
+            const double rad = e_prev + fabs(e_r);
+            const double lo = -wave_max(ln < n ? rad - d_r : -1e300);
+            const double hi = wave_max(ln < n ? d_r + rad : -1e300);
+            const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
+            if (tid == 0) {
+                lo_hi[0] = lo - pad;
+                lo_hi[1] = hi + pad;
+            }
         }
-        const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
-        lo_hi[0] = lo - pad;
-        lo_hi[1] = hi + pad;
-    }
-    __syncthreads();
-    for (int it = 0; it < 200; ++it) {
-        const double lo = lo_hi[0], hi = lo_hi[1];
-        if (!(hi - lo > 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)))) break;
-        if (tid == 0) first_true = nt;
         __syncthreads();
-        const double x = lo + (hi - lo) * (double)(tid + 1) / (double)nt;
-        int count = 0;
-        double q = 1.0;
-        for (int i = 0; i < n; ++i) {  // number of eigenvalues below x (LDL^T pivots)
-            const double e2 = i > 0 ? ee[i - 1] * ee[i - 1] : 0.0;
-            q = dd[i] - x - (i > 0 ? e2 / q : 0.0);
+        for (int it = 0; it < 200; ++it) {
+            const double lo = lo_hi[0], hi = lo_hi[1];
+            if (!(hi - lo > 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)))) break;
+            if (tid == 0) first_true = nt;
+            __syncthreads();
+            const double x = lo + (hi - lo) * (double)(tid + 1) / (double)nt;
+            int count = 0;
+            double q = wave_bcast(d_r, 0) - x;
             if (q == 0.0) q = -1e-300;
             if (q < 0.0) ++count;
-        }
-        if (count >= 1) atomicMin(&first_true, tid);
-        __syncthreads();
-        if (tid == 0) {
-            const int f = first_true;
-            lo_hi[0] = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)nt;
-            lo_hi[1] = f >= nt ? hi : lo + (hi - lo) * (double)(f + 1) / (double)nt;
-        }
-        __syncthreads();
-    }
-    const double lambda = 0.5 * (lo_hi[0] + lo_hi[1]);
-    // inverse iteration on T - lambda I (LU with partial pivoting, LAPACK dgttrf/dgttrs scheme)
-    if (tid == 0) {
-        for (int i = 0; i < n; ++i) {
-            tdg[i] = dd[i] - lambda;
-            tdu2[i] = 0.0;
-            if (i + 1 < n) {
-                tdl[i] = ee[i];
-                tdu[i] = ee[i];
+            for (int i = 1; i < n; ++i) {  // number of eigenvalues below x (LDL^T pivots)
+                q = wave_bcast(d_r, i) - x - wave_bcast(e2_r, i) / q;
+                if (q == 0.0) q = -1e-300;
+                if (q < 0.0) ++count;
             }
-        }
-        for (int i = 0; i + 1 < n; ++i) {
-            if (fabs(tdg[i]) >= fabs(tdl[i])) {
-                ipiv[i] = i;
-                if (tdg[i] == 0.0) tdg[i] = 1e-300;
-                const double f = tdl[i] / tdg[i];
-                tdl[i] = f;
-                tdg[i + 1] -= f * tdu[i];
-            } else {
-                ipiv[i] = i + 1;
-                const double f = tdg[i] / tdl[i];
-                tdg[i] = tdl[i];
-                tdl[i] = f;
-                const double t = tdu[i];
-                tdu[i] = tdg[i + 1];
-                tdg[i + 1] = t - f * tdg[i + 1];
-                if (i + 2 < n) {
-                    tdu2[i] = tdu[i + 1];
-                    tdu[i + 1] = -f * tdu[i + 1];
-                }
+            if (count >= 1) atomicMin(&first_true, tid);
+            __syncthreads();
+            if (tid == 0) {
+                const int f = first_true;
+                lo_hi[0] = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)nt;
+                lo_hi[1] = f >= nt ? hi : lo + (hi - lo) * (double)(f + 1) / (double)nt;
             }
+            __syncthreads();
         }
-        if (n > 0) {
-            ipiv[n - 1] = n - 1;
-            if (tdg[n - 1] == 0.0) tdg[n - 1] = 1e-300;
-        }
-        for (int i = 0; i < n; ++i) y[i] = 1.0;
-        for (int iter = 0; iter < 3; ++iter) {
+        const double lambda = 0.5 * (lo_hi[0] + lo_hi[1]);
+        if (tid < 64) {
+            double g = ln < n ? d_r - lambda : 1.0;  // diagonal of U
+            double l = e_r;                          // subdiagonal, then the multipliers
+            double u = e_r;                          // first superdiagonal of U
+            double u2 = 0.0;                         // second superdiagonal of U
+            int swp = 0;                             // lane i: rows i and i+1 interchanged
             for (int i = 0; i + 1 < n; ++i) {
-                if (ipiv[i] == i) {
-                    y[i + 1] -= tdl[i] * y[i];
+                double gi = wave_bcast(g, i);
+                const double li = wave_bcast(l, i), ui = wave_bcast(u, i), gi1 = wave_bcast(g, i + 1);
+                if (fabs(gi) >= fabs(li)) {
+                    if (gi == 0.0) gi = 1e-300;
+                    const double f = li / gi;
+                    g = wave_put(gi, i, g);
+                    l = wave_put(f, i, l);
+                    g = wave_put(gi1 - f * ui, i + 1, g);
                 } else {
-                    const double t = y[i];
-                    y[i] = y[i + 1];
-                    y[i + 1] = t - tdl[i] * y[i];
+                    const double f = gi / li;
+                    g = wave_put(li, i, g);
+                    l = wave_put(f, i, l);
+                    u = wave_put(gi1, i, u);
+                    g = wave_put(ui - f * gi1, i + 1, g);
+                    if (i + 2 < n) {
+                        const double ui1 = wave_bcast(u, i + 1);
+                        u2 = wave_put(ui1, i, u2);
+                        u = wave_put(-f * ui1, i + 1, u);
+                    }
+                    if (ln == i) swp = 1;
                 }
             }
-            for (int i = n - 1; i >= 0; --i) {
-                double v = y[i];
-                if (i + 1 < n) v -= tdu[i] * y[i + 1];
-                if (i + 2 < n) v -= tdu2[i] * y[i + 2];
-                y[i] = v / tdg[i];
+            if (wave_bcast(g, n - 1) == 0.0) g = wave_put(1e-300, n - 1, g);
+            double yv = ln < n ? 1.0 : 0.0;
+            for (int iter = 0; iter < 3; ++iter) {
+                for (int i = 0; i + 1 < n; ++i) {
+                    const double yi = wave_bcast(yv, i), yi1 = wave_bcast(yv, i + 1), li = wave_bcast(l, i);
+                    if (__builtin_amdgcn_readlane(swp, i) == 0) {
+                        yv = wave_put(yi1 - li * yi, i + 1, yv);
+                    } else {
+                        yv = wave_put(yi1, i, yv);
+                        yv = wave_put(yi - li * yi1, i + 1, yv);
+                    }
+                }
+                for (int i = n - 1; i >= 0; --i) {
+                    double v = wave_bcast(yv, i);
+                    if (i + 1 < n) v -= wave_bcast(u, i) * wave_bcast(yv, i + 1);
+                    if (i + 2 < n) v -= wave_bcast(u2, i) * wave_bcast(yv, i + 2);
+                    yv = wave_put(v / wave_bcast(g, i), i, yv);
+                }
+                yv *= 1.0 / sqrt(wave_sum(yv * yv));
             }
-            double nrm = 0.0;
-            for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
-            nrm = 1.0 / sqrt(nrm);
-            for (int i = 0; i < n; ++i) y[i] *= nrm;
+            for (int k = n - 3; k >= 0; --k) {  // back-transform z = H_0 H_1 ... H_{n-3} y
+                const double vk = ln > k && ln < n ? V[ln + n * k] : 0.0;
+                yv -= beta[k] * wave_sum(vk * yv) * vk;
+            }
+            if (ln < n) y[ln] = yv;
+            if (tid == 0) sh_scalar[1] = lambda;
         }
-        sh_scalar[1] = lambda;
+        __syncthreads();
+    } else {
+        // lowest eigenvalue by multisection on Sturm counts
+        if (tid == 0) {
+            double lo = 1e300, hi = -1e300;
+            for (int i = 0; i < n; ++i) {
+                const double rad = (i > 0 ? fabs(ee[i - 1]) : 0.0) + (i + 1 < n ? fabs(ee[i]) : 0.0);
+                lo = fmin(lo, dd[i] - rad);
+                hi = fmax(hi, dd[i] + rad);
+            }
+            const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
+            lo_hi[0] = lo - pad;
+            lo_hi[1] = hi + pad;
+        }
+        __syncthreads();
+        for (int it = 0; it < 200; ++it) {
+            const double lo = lo_hi[0], hi = lo_hi[1];
+            if (!(hi - lo > 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)))) break;
+            if (tid == 0) first_true = nt;
+            __syncthreads();
+            const double x = lo + (hi - lo) * (double)(tid + 1) / (double)nt;
+            int count = 0;
+            double q = 1.0;
+            for (int i = 0; i < n; ++i) {  // number of eigenvalues below x (LDL^T pivots)
+                const double e2 = i > 0 ? ee[i - 1] * ee[i - 1] : 0.0;
+                q = dd[i] - x - (i > 0 ? e2 / q : 0.0);
+                if (q == 0.0) q = -1e-300;
+                if (q < 0.0) ++count;
+            }
+            if (count >= 1) atomicMin(&first_true, tid);
+            __syncthreads();
+            if (tid == 0) {
+                const int f = first_true;
+                lo_hi[0] = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)nt;
+                lo_hi[1] = f >= nt ? hi : lo + (hi - lo) * (double)(f + 1) / (double)nt;
+            }
+            __syncthreads();
+        }
+        const double lambda = 0.5 * (lo_hi[0] + lo_hi[1]);
+        // inverse iteration on T - lambda I (LU with partial pivoting, LAPACK dgttrf/dgttrs scheme)
+        if (tid == 0) {
+            for (int i = 0; i < n; ++i) {
+                tdg[i] = dd[i] - lambda;
+                tdu2[i] = 0.0;
+                if (i + 1 < n) {
+                    tdl[i] = ee[i];
+                    tdu[i] = ee[i];
+                }
+            }
+            for (int i = 0; i + 1 < n; ++i) {
+                if (fabs(tdg[i]) >= fabs(tdl[i])) {
+                    ipiv[i] = i;
+                    if (tdg[i] == 0.0) tdg[i] = 1e-300;
+                    const double f = tdl[i] / tdg[i];
+                    tdl[i] = f;
+                    tdg[i + 1] -= f * tdu[i];
+                } else {
+                    ipiv[i] = i + 1;
+                    const double f = tdg[i] / tdl[i];
+                    tdg[i] = tdl[i];
+                    tdl[i] = f;
+                    const double t = tdu[i];
+                    tdu[i] = tdg[i + 1];
+                    tdg[i + 1] = t - f * tdg[i + 1];
+                    if (i + 2 < n) {
+                        tdu2[i] = tdu[i + 1];
+                        tdu[i + 1] = -f * tdu[i + 1];
+                    }
+                }
+            }
+            if (n > 0) {
+                ipiv[n - 1] = n - 1;
+                if (tdg[n - 1] == 0.0) tdg[n - 1] = 1e-300;
+            }
+            for (int i = 0; i < n; ++i) y[i] = 1.0;
+            for (int iter = 0; iter < 3; ++iter) {
+                for (int i = 0; i + 1 < n; ++i) {
+                    if (ipiv[i] == i) {
+                        y[i + 1] -= tdl[i] * y[i];
+                    } else {
+                        const double t = y[i];
+                        y[i] = y[i + 1];
+                        y[i + 1] = t - tdl[i] * y[i];
+                    }
+                }
+                for (int i = n - 1; i >= 0; --i) {
+                    double v = y[i];
+                    if (i + 1 < n) v -= tdu[i] * y[i + 1];
+                    if (i + 2 < n) v -= tdu2[i] * y[i + 2];
+                    y[i] = v / tdg[i];
+                }
+                double nrm = 0.0;
+                for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
+                nrm = 1.0 / sqrt(nrm);
+                for (int i = 0; i < n; ++i) y[i] *= nrm;
+            }
+            sh_scalar[1] = lambda;
+        }
+        __syncthreads();
+        // back-transform z = H_0 H_1 ... H_{n-3} y
+        for (int k = n - 3; k >= 0; --k) {
+            const int m0 = k + 1;
+            double part = 0.0;
+            for (int i = m0 + tid; i < n; i += nt) part += V[i + n * k] * y[i];
+            const double s = beta[k] * block_sum(part, red);
+            for (int i = m0 + tid; i < n; i += nt) y[i] -= s * V[i + n * k];
+            __syncthreads();
+        }
+    }
+    // c = Linv^T z (Lm holds Linv, lower triangle)
+    __syncthreads();
+    for (int i = tid; i < n; i += nt) {
+        double c0 = 0.0, c1 = 0.0;
+        int k = i;
+        for (; k + 1 < n; k += 2) {
+            c0 += Lm[k + n * i] * y[k];
+            c1 += Lm[k + 1 + n * i] * y[k + 1];
+        }
+        for (; k < n; ++k) c0 += Lm[k + n * i] * y[k];
+        tdg[i] = c0 + c1;  // tdg is free after the inverse iteration
     }
     __syncthreads();
-    // back-transform z = H_0 H_1 ... H_{n-3} y
-    for (int k = n - 3; k >= 0; --k) {
-        const int m0 = k + 1;
-        double part = 0.0;
-        for (int i = m0 + tid; i < n; i += nt) part += V[i + n * k] * y[i];
-        const double s = beta[k] * block_sum(part, red);
-        for (int i = m0 + tid; i < n; i += nt) y[i] -= s * V[i + n * k];
-        __syncthreads();
+    for (int i = tid; i < n; i += nt) {
+        y[i] = tdg[i];
+        coef[i] = tdg[i];
     }
-    // c = L^{-T} z (back substitution over the columns of L)
-    for (int k = n - 1; k >= 0; --k) {
-        if (tid == 0) y[k] /= Lm[k + n * k];
-        __syncthreads();
-        const double ck = y[k];
-        for (int i = tid; i < k; i += nt) y[i] -= Lm[k + n * i] * ck;
-        __syncthreads();
-    }
-    for (int i = tid; i < n; i += nt) coef[i] = y[i];
+    __syncthreads();
     if (tid == 0) {
         energy[0] = sh_scalar[1];
         energy[1] = sh_scalar[1] + e_nuc;
@@ -1374,7 +1516,7 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
     hipLaunchKernelGGL(k_structure_right, dim3((unsigned)((a + 255) / 256), 1, z), dim3(256), 0, st, slots, topo);
     hipLaunchKernelGGL(k_structure_left, dim3((unsigned)((b + 255) / 256), 1, z), dim3(256), 0, st, slots, topo);
     mark(EV_STRUCT);
-    const int threads = topo.n_str <= 64 ? 64 : 256;
+    const int threads = 256;
     if (gep_in_lds(topo.n_str))
         hipLaunchKernelGGL(k_gep_lowest<true>, dim3(1, 1, z), dim3(threads), gep_lds_bytes(topo.n_str), st, slots,
                            topo);
