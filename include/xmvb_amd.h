@@ -257,6 +257,54 @@ int vbe_pool_wait(vbe_pool* pool, int32_t ticket, double* energies, double* grad
 /* Device time per phase of the pool's most recently completed launch. */
 int vbe_pool_phase_times(const vbe_pool* pool, vbe_phase_times* times);
 
+/* ---- full convergence: projected L-BFGS on device-resident state -----------------------------
+ * Options of VbScfOptimizerOptions (src/vbscf/optimization/driver/options.hpp:15-108) that the
+ * P-LBFGS backend reads; vbe_optimizer_defaults fills the reference defaults. */
+typedef struct vbe_optimizer_options {
+    int32_t max_iterations;     /* accepted iterations (2000)                                */
+    int32_t history_size;       /* secant pairs kept (100)                                   */
+    double gradient_tolerance;  /* projected-gradient norm (1e-3)                            */
+    double energy_tolerance;    /* |E_k - E_k-1| (1e-7)                                      */
+    double initial_step_size;   /* first Armijo trial step, capped at 1 (1.0)                */
+    double minimum_step_size;   /* Armijo gives up below this step (1e-7)                    */
+    double armijo_constant;     /* sufficient-decrease constant (1e-4)                       */
+} vbe_optimizer_options;
+
+/* Termination reasons of run_plbfgs_backend (src/vbscf/optimization/backends/plbfgs.cpp:20-226). */
+typedef enum vbe_termination {
+    VBE_TERM_MAX_ITERATIONS = 0,       /* iteration limit reached                              */
+    VBE_TERM_INITIAL_TOLERANCE = 1,    /* "plbfgs_initial_tolerance"                           */
+    VBE_TERM_CONVERGED = 2,            /* "plbfgs_projected_gradient_tolerance"                */
+    VBE_TERM_LINE_SEARCH_FAILED = 3,   /* "plbfgs_line_search_failed"                          */
+    VBE_TERM_LINE_SEARCH_STALLED = 4,  /* "plbfgs_line_search_stalled"                         */
+    VBE_TERM_NON_DESCENT = 5,          /* "plbfgs_non_descent_direction"                       */
+    VBE_TERM_EMPTY_SPACE = 6           /* "nonredundant_space_empty"                           */
+} vbe_termination;
+
+typedef struct vbe_optimizer_result {
+    double energy;                  /* E_total at the returned orbitals                        */
+    double gradient_norm;           /* projected-gradient norm there (the convergence measure) */
+    int32_t iterations;             /* accepted iterations                                     */
+    int32_t energy_evaluations;     /* forward evaluations (Armijo trials included)            */
+    int32_t gradient_evaluations;   /* backward evaluations                                    */
+    int32_t converged;
+    int32_t termination;            /* vbe_termination                                         */
+} vbe_optimizer_result;
+
+void vbe_optimizer_defaults(vbe_optimizer_options* options);
+
+/* Converge the orbitals of one problem: run_plbfgs_backend (plbfgs.cpp:20-226) with the Armijo
+ * backtracking of try_armijo_backtracking_nonredundant_direction (globalization/line_search.cpp:
+ * 33-95) and transported secant pairs (preconditioners/transported_lbfgs.cpp). Orbitals, gradients
+ * and the secant history stay on the device; the host only reads scalars (energies, directional
+ * derivatives, norms) to take the line-search decisions. C_raw (n_act x n_bf, in/out): start point
+ * in, converged orbitals (each normalised to c^T S c = 1) out; support as in vbe_eval; grad_raw
+ * (optional) receives the gradient at the returned orbitals; energy_trace (optional, trace_capacity
+ * entries) the energy after each accepted iteration, starting with the initial point. */
+int vbe_optimize(vbe_context* ctx, vbe_problem* prob, double* C_raw, const uint8_t* support,
+                 const vbe_optimizer_options* options, vbe_optimizer_result* result, double* grad_raw,
+                 double* energy_trace, int32_t trace_capacity);
+
 #ifdef __cplusplus
 }
 #endif

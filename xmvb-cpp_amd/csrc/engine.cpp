@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -17,6 +18,7 @@
 #include "../../include/xmvb_amd.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "optimizer.hpp"
 
 namespace {
 
@@ -88,6 +90,7 @@ struct Slot {
     double e_nuc = 0.0;
     std::unique_ptr<DeviceArena> arena;
     SlotDesc desc{};           // workspace + static pointers; per-launch fields filled at launch
+    std::vector<double> S_host;  // AO overlap (the optimiser's chart metric)
     double* C_buf = nullptr;   // slot-owned orbitals / gradient / support for host-buffer evaluations
     double* grad_buf = nullptr;
     uint8_t* support_buf = nullptr;
@@ -321,6 +324,7 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
             throw StatusError(VBE_ERR_ARGUMENT, "slot is in use by a pending submission");
     check(hipSetDevice(p->ctx->device), "hipSetDevice");
     Slot& s = p->slots[slot];
+    s.S_host.assign(S, S + (size_t)n_bf * n_bf);
     const TopoDesc& T = p->topo;
     const size_t n = T.n, np = T.np, nd = T.n_det, ns = T.n_str;
     const size_t nbp = (size_t)n_bf * (n_bf + 1) / 2;
@@ -876,6 +880,50 @@ int vbe_last_intermediates(vbe_context* ctx, vbe_problem* prob, double* H, doubl
         get(h_act, d.hact, n * n);
         get(g_packed, d.G, p->topo.ng);
         check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        return VBE_OK;
+    });
+}
+
+void vbe_optimizer_defaults(vbe_optimizer_options* options) {
+    if (options) xmvb_amd::optimizer_defaults(options);
+}
+
+int vbe_optimize(vbe_context* ctx, vbe_problem* prob, double* C_raw, const uint8_t* support,
+                 const vbe_optimizer_options* options, vbe_optimizer_result* result, double* grad_raw,
+                 double* energy_trace, int32_t trace_capacity) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !prob || !C_raw || !options || !result) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        if (prob->pool->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
+        if (trace_capacity < 0) throw StatusError(VBE_ERR_ARGUMENT, "negative trace capacity");
+        vbe_pool* p = prob->pool.get();
+        Slot& s = p->slots[0];
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        xmvb_amd::OptimizerHooks hooks;
+        hooks.stream = p->lanes[0].stream;
+        hooks.n_act = p->n_act;
+        hooks.n_bf = s.n_bf;
+        hooks.S_ao = s.S_host.data();
+        hooks.energy = [&](const double* d_C, const uint8_t* d_support, double* d_grad) -> double {
+            try {
+                return eval_problem(prob, d_C, d_support, false, d_grad, true);
+            } catch (const StatusError& e) {
+                // a trial point whose orbitals or structure metric break down is a rejected trial
+                if (e.status == VBE_ERR_NUMERIC || e.status == VBE_ERR_METRIC)
+                    return std::numeric_limits<double>::quiet_NaN();
+                throw;
+            }
+        };
+        hooks.gradient = [&](const double* d_C, const uint8_t* d_support, double* d_grad) {
+            eval_problem(prob, d_C, d_support, true, d_grad, false);
+        };
+        try {
+            xmvb_amd::run_plbfgs(hooks, support, C_raw, *options, *result, grad_raw, energy_trace, trace_capacity);
+        } catch (const std::invalid_argument& e) {
+            throw StatusError(VBE_ERR_ARGUMENT, e.what());
+        } catch (const std::runtime_error& e) {
+            if (dynamic_cast<const StatusError*>(&e) || dynamic_cast<const HipError*>(&e)) throw;
+            throw StatusError(VBE_ERR_NUMERIC, e.what());
+        }
         return VBE_OK;
     });
 }

@@ -44,7 +44,12 @@ EXPORTED_SYMBOLS = (
     "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
     "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_eval_device",
     "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit", "vbe_set_ri_precision",
+    "vbe_optimizer_defaults", "vbe_optimize",
 )
+
+TERMINATIONS = {0: "max_iterations", 1: "plbfgs_initial_tolerance", 2: "plbfgs_projected_gradient_tolerance",
+                3: "plbfgs_line_search_failed", 4: "plbfgs_line_search_stalled", 5: "plbfgs_non_descent_direction",
+                6: "nonredundant_space_empty"}
 
 
 class VbeError(RuntimeError):
@@ -78,6 +83,18 @@ class PhaseTimes(C.Structure):
         "orbital_prep_ms", "transform_ms", "ri_one_index_ms", "pair_forward_ms", "structure_ms", "eigensolve_ms",
         "pair_backward_ms", "pair_backward_kernel_ms", "gradient_tail_ms", "total_ms", "pair_forward_kernel_ms",
         "pair_backward_field_ms", "pair_backward_adjoint_ms")]
+
+
+class OptimizerOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("history_size", C.c_int32), ("gradient_tolerance", C.c_double),
+                ("energy_tolerance", C.c_double), ("initial_step_size", C.c_double),
+                ("minimum_step_size", C.c_double), ("armijo_constant", C.c_double)]
+
+
+class OptimizerResult(C.Structure):
+    _fields_ = [("energy", C.c_double), ("gradient_norm", C.c_double), ("iterations", C.c_int32),
+                ("energy_evaluations", C.c_int32), ("gradient_evaluations", C.c_int32), ("converged", C.c_int32),
+                ("termination", C.c_int32)]
 
 
 _lib = None
@@ -128,6 +145,9 @@ def library() -> C.CDLL:
         lib.vbe_pool_wait.argtypes = [P, I32, P, P]
         lib.vbe_pool_poll.argtypes = [P, I32, C.POINTER(I32)]
         lib.vbe_pool_phase_times.argtypes = [P, C.POINTER(PhaseTimes)]
+        lib.vbe_optimizer_defaults.argtypes = [C.POINTER(OptimizerOptions)]
+        lib.vbe_optimizer_defaults.restype = None
+        lib.vbe_optimize.argtypes = [P, P, P, P, C.POINTER(OptimizerOptions), C.POINTER(OptimizerResult), P, P, I32]
         _lib = lib
     return _lib
 
@@ -344,6 +364,28 @@ class Problem:
                                                      None if support_ptr is None else C.c_void_p(support_ptr),
                                                      Flags(self.precision, 0, int(gradient)), C.byref(e), C.c_void_p(grad_ptr)))
         return e.value
+
+    def optimize(self, Cm: np.ndarray, support: np.ndarray | None = None, **options) -> dict:
+        """Full P-LBFGS convergence from orbitals Cm; options override the VbScfOptimizerOptions
+        defaults (max_iterations, history_size, gradient_tolerance, energy_tolerance,
+        initial_step_size, minimum_step_size, armijo_constant)."""
+        Cm = self._orbitals(Cm).copy()
+        sup = None if support is None else np.ascontiguousarray(support, dtype=np.uint8)
+        opt = OptimizerOptions()
+        self.ctx._lib.vbe_optimizer_defaults(C.byref(opt))
+        for k, v in options.items():
+            if not hasattr(opt, k):
+                raise TypeError(f"unknown optimizer option {k}")
+            setattr(opt, k, v)
+        res = OptimizerResult()
+        g = np.zeros_like(Cm)
+        trace = np.zeros(opt.max_iterations + 1)
+        self.ctx.check(self.ctx._lib.vbe_optimize(self.ctx._h, self._h, _ptr(Cm), _ptr(sup), C.byref(opt), C.byref(res),
+                                                  _ptr(g), _ptr(trace), len(trace)))
+        out = {name: getattr(res, name) for name, _ in OptimizerResult._fields_}
+        out["termination"] = TERMINATIONS.get(res.termination, res.termination)
+        out.update(C=Cm, grad=g, trace=trace[:min(len(trace), res.iterations + 1)].copy())
+        return out
 
     def intermediates(self) -> dict:
         ns, n = self.topo.n_structures, self.n_act
