@@ -14,7 +14,10 @@ Three legs are timed:
   * served: Poisson arrivals (--serve-rate requests/s for --serve-seconds) over a 64-point
     dissociation scan (inter-fragment h and B blocks scaled by exp(-0.5 r), r in [0.5, 4.0]),
     continuously batched onto two in-flight submissions (submit / poll / wait); request latency
-    is completion minus arrival, host orbitals in and gradients out (PCIe included).
+    is completion minus arrival, host orbitals in and gradients out (PCIe included);
+  * converge: --converge full-convergence requests (projected L-BFGS on device-resident state,
+    reference optimizer defaults) at scan points, reported as latency per request and per
+    iteration next to the plain evaluation time.
 
 Prints one JSON line: batched throughput (E+grad evaluations/s) beside the plain path's, request
 p50/p99 latency of the served leg, the roofline of the dominant pair kernel of the batched leg (its
@@ -165,6 +168,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the served leg")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the served leg (0: skip)")
     ap.add_argument("--mixed-steps", type=int, default=30, help="timed batched steps with FP32 RI factors (0: skip)")
+    ap.add_argument("--converge", type=int, default=3, help="full-convergence requests timed (0: skip)")
+    ap.add_argument("--converge-iterations", type=int, default=500, help="iteration cap per convergence request")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle parity")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -319,6 +324,28 @@ def main():
     if args.serve_seconds > 0:
         served = serve(vb, ctx, topo, cfg, args.serve_rate, args.serve_seconds, replicas=4, max_batch=K)
 
+    converge = None
+    if args.converge > 0:
+        runs = []
+        for j in range(args.converge):
+            r = 1.0 + 1.5 * j / max(1, args.converge - 1)
+            spr = vb.synthetic_problem(cfg["seed"], cfg["n_bf"], cfg["n_act"], cfg["n_aux"], r)
+            pr = vb.Problem.from_synthetic(ctx, spr, topo)
+            pr.optimize(spr.C, max_iterations=2)  # first-use allocations outside the timing
+            t0 = time.perf_counter()
+            res = pr.optimize(spr.C, max_iterations=args.converge_iterations)
+            dt = time.perf_counter() - t0
+            runs.append(dict(r=r, iterations=res["iterations"], termination=res["termination"],
+                             energy_start=float(res["trace"][0]), energy=res["energy"],
+                             gradient_norm=res["gradient_norm"], forward_evaluations=res["energy_evaluations"],
+                             backward_evaluations=res["gradient_evaluations"], ms=dt * 1e3,
+                             ms_per_iteration=dt * 1e3 / max(1, res["iterations"])))
+            pr.close()
+        converge = dict(runs=runs, max_iterations=args.converge_iterations,
+                        options="reference optimizer defaults: gradient_tolerance 1e-3, energy_tolerance 1e-7, "
+                                "history 100, Armijo 1e-4",
+                        plain_eval_ms=plain["ms_per_eval"])
+
     value = K * args.steps / elapsed
     out = dict(metric=METRIC, value=value, unit="E+grad evals/s", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=elapsed * 1e3 / args.steps, higher_is_better=True, scaling="weak",
@@ -334,6 +361,7 @@ def main():
                                else "batched step"),
                batch_step_ms=dict(p50=percentile(lat, 0.5), p99=percentile(lat, 0.99)),
                plain_fp64=plain, speedup_vs_plain=value / plain["value"], served=served, fp32_ri=mixed,
+               converge=converge,
                batch_consistency=batch_consistency, phase_ms={k: round(v, 5) for k, v in mean.items()},
                energy=energy, roofline=roofline, roofline_other_kernels=others)
     if not args.no_cpu:

@@ -916,6 +916,15 @@ int vbe_optimize(vbe_context* ctx, vbe_problem* prob, double* C_raw, const uint8
         hooks.gradient = [&](const double* d_C, const uint8_t* d_support, double* d_grad) {
             eval_problem(prob, d_C, d_support, true, d_grad, false);
         };
+        hooks.energy_gradient = [&](const double* d_C, const uint8_t* d_support, double* d_grad) -> double {
+            try {
+                return eval_problem(prob, d_C, d_support, true, d_grad, true);
+            } catch (const StatusError& e) {
+                if (e.status == VBE_ERR_NUMERIC || e.status == VBE_ERR_METRIC)
+                    return std::numeric_limits<double>::quiet_NaN();
+                throw;
+            }
+        };
         try {
             xmvb_amd::run_plbfgs(hooks, support, C_raw, *options, *result, grad_raw, energy_trace, trace_capacity);
         } catch (const std::invalid_argument& e) {

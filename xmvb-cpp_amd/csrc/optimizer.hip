@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <limits>
 #include <map>
 #include <stdexcept>
@@ -493,12 +494,20 @@ void run_plbfgs(const OptimizerHooks& h, const uint8_t* support, double* C_host,
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     };
     double gnorm = 0.0, cmax = 0.0;
-    auto chart = [&](int at) {  // normalise and project at pts[at]; sets gnorm / cmax
+    auto launch_chart = [&](int at) {
         const size_t lds = sizeof(double) * 2 * n_bf;
         hipLaunchKernelGGL(k_opt_point, dim3(n_act), dim3(kPointThreads), lds, st, n_bf, d_S, d_Sinv, d_pat,
                            d_support, pts[at], grs[at], d_Sc, d_R, d_stats);
         hip_check(hipGetLastError(), "k_opt_point");
-        read(stats, d_stats, stats.size());
+        hip_check(hipMemcpyAsync(stats.data(), d_stats, sizeof(double) * stats.size(), hipMemcpyDeviceToHost, st),
+                  "D2H");
+    };
+    // normalise and project at pts[at] (sets gnorm / cmax); with `then`, enqueue more work before
+    // the one synchronisation
+    auto chart = [&](int at, const std::function<void()>& then = {}) {
+        launch_chart(at);
+        if (then) then();
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         gnorm = 0.0;
         cmax = 0.0;
         for (int i = 0; i < n_act; ++i) {
@@ -522,10 +531,16 @@ void run_plbfgs(const OptimizerHooks& h, const uint8_t* support, double* C_host,
         tinf = out[1];
     };
     const double zero_step = 128.0 * kEps;
+    // The first trial of a line search runs forward and backward in one launch sequence when the
+    // previous search accepted its first trial (the common case once the secant model is built),
+    // which saves a synchronisation and a launch gap; a rejected speculative trial costs one
+    // backward. Which trials are accepted does not depend on it.
+    bool speculate = false;
     auto armijo = [&](int at, double E0, double step0, double& Et) -> bool {
         // try_armijo_backtracking_nonredundant_direction (globalization/line_search.cpp:33-95)
         if (!std::isfinite(dd) || dd >= 0.0) return false;
         double step = std::max(opt.minimum_step_size, step0);
+        bool first = true;
         while (step >= opt.minimum_step_size) {
             if (step * tinf <= zero_step * std::max(1.0, cmax)) {
                 step *= 0.5;
@@ -534,15 +549,23 @@ void run_plbfgs(const OptimizerHooks& h, const uint8_t* support, double* C_host,
             hipLaunchKernelGGL(k_opt_trial, dim3((N + 255) / 256), dim3(256), 0, st, N, pts[at], d_T, step,
                                pts[trial]);
             hip_check(hipGetLastError(), "k_opt_trial");
-            Et = h.energy(pts[trial], d_support, grs[trial]);
+            const bool with_grad = first && speculate;
+            Et = with_grad ? h.energy_gradient(pts[trial], d_support, grs[trial])
+                           : h.energy(pts[trial], d_support, grs[trial]);
             ++res.energy_evaluations;
+            if (with_grad) ++res.gradient_evaluations;
             if (std::isfinite(Et) && Et <= E0 + opt.armijo_constant * step * dd) {
-                h.gradient(pts[trial], d_support, grs[trial]);
-                ++res.gradient_evaluations;
+                if (!with_grad) {
+                    h.gradient(pts[trial], d_support, grs[trial]);
+                    ++res.gradient_evaluations;
+                }
+                speculate = first;
                 return true;
             }
+            first = false;
             step *= 0.5;
         }
+        speculate = false;
         return false;
     };
 
@@ -596,16 +619,15 @@ void run_plbfgs(const OptimizerHooks& h, const uint8_t* support, double* C_host,
         std::swap(prev, cur);   // prev = the point we stepped from
         std::swap(cur, trial);  // cur = the accepted trial
         E = Et;
-        chart(cur);
         const int slot = cap > 0 ? (newest + 1) % cap : 0;
-        auto append = [&]() {
+        auto append = [&]() {  // enqueue only; the chart's synchronisation completes it
             hipLaunchKernelGGL(k_opt_append, dim3(1), dim3(kBlockThreads), 0, st, n_act, n_bf, d_S, d_Sinv, d_pat,
                                pts[prev], grs[prev], pts[cur], grs[cur], d_Sc, d_hs + (size_t)slot * N,
                                d_hy + (size_t)slot * N, d_hstat + (size_t)slot * n_act * 3, d_out);
             hip_check(hipGetLastError(), "k_opt_append");
-            read(out, d_out, 2);
+            hip_check(hipMemcpyAsync(out.data(), d_out, sizeof(double) * 2, hipMemcpyDeviceToHost, st), "D2H");
         };
-        append();  // also measures the accepted step
+        chart(cur, append);  // the candidate pair also measures the accepted step
         // a step that changed nothing or made no progress: Armijo steepest descent from the
         // previous point with a fresh history (plbfgs.cpp:146-191)
         const double energy_scale = std::max(1.0, std::fabs(reference_energy));
@@ -629,8 +651,7 @@ void run_plbfgs(const OptimizerHooks& h, const uint8_t* support, double* C_host,
             std::swap(prev, cur);
             std::swap(cur, trial);
             E = Et;
-            chart(cur);
-            append();
+            chart(cur, append);
         }
         ++res.iterations;
         record(E);

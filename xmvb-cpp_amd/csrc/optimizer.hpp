@@ -21,6 +21,8 @@ struct OptimizerHooks {
     const double* S_ao = nullptr;  // host AO overlap, n_bf x n_bf
     std::function<double(const double* d_C, const uint8_t* d_support, double* d_grad)> energy;
     std::function<void(const double* d_C, const uint8_t* d_support, double* d_grad)> gradient;
+    // forward and backward in one launch sequence (NaN as for `energy`)
+    std::function<double(const double* d_C, const uint8_t* d_support, double* d_grad)> energy_gradient;
 };
 
 void optimizer_defaults(vbe_optimizer_options* o);
