@@ -305,6 +305,16 @@ int vbe_optimize(vbe_context* ctx, vbe_problem* prob, double* C_raw, const uint8
                  const vbe_optimizer_options* options, vbe_optimizer_result* result, double* grad_raw,
                  double* energy_trace, int32_t trace_capacity);
 
+/* The same optimisation for `count` resident slots of a pool at once (no reference counterpart:
+ * the reference converges one problem per process). The slots advance in lockstep, each phase of
+ * every slot's iteration (chart, search direction, Armijo trial, secant append) being one batched
+ * launch, and every round of trials one launch sequence of the pool; trials are evaluated with
+ * their gradient. Each slot follows exactly the iterates it would follow alone. C_raw, support
+ * (or NULL) and grad_raw (or NULL): concatenated n_act x n_bf blocks of the listed slots in list
+ * order; results[count]. The pool must have no submission in flight. */
+int vbe_pool_optimize(vbe_pool* pool, int32_t count, const int32_t* slots, double* C_raw, const uint8_t* support,
+                      const vbe_optimizer_options* options, vbe_optimizer_result* results, double* grad_raw);
+
 #ifdef __cplusplus
 }
 #endif

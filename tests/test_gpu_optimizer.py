@@ -83,3 +83,50 @@ def test_optimizer_rejects_bad_options(ctx):
     r = prob.optimize(sp.C, gradient_tolerance=1e9)
     assert r["termination"] == "plbfgs_initial_tolerance" and r["iterations"] == 0
     prob.close()
+
+
+def test_pool_optimize_follows_each_problem_alone(ctx):
+    """Several slots (different seeds and basis sizes) converged together in lockstep reach the
+    point each reaches alone, with the same iteration count and stopping reason."""
+    print("case pool optimize COV n_act=4 slots seeds 2,3,5", flush=True)
+    topo = vb.Topology.build("COV", 4, 4, 1)
+    cases = [(2, 16, 48), (3, 20, 40), (5, 16, 32)]
+    pool = vb.Pool(ctx, topo, capacity=4)
+    sps = []
+    for slot, (seed, n_bf, n_aux) in enumerate(cases):
+        sp = vb.synthetic_problem(seed, n_bf, 4, n_aux)
+        pool.load_synthetic(slot, sp)
+        sps.append(sp)
+    opts = dict(gradient_tolerance=1e-6, energy_tolerance=1e-12, max_iterations=30)
+    res = pool.optimize([0, 1, 2], [sp.C for sp in sps], **opts)
+    for slot, (sp, r) in enumerate(zip(sps, res)):
+        prob = vb.Problem.from_synthetic(ctx, sp, topo)
+        alone = prob.optimize(sp.C, **opts)
+        prob.close()
+        print(f"slot {slot}: pool {r['iterations']} it E={r['energy']:.12f} {r['termination']}; alone "
+              f"{alone['iterations']} it E={alone['energy']:.12f}", flush=True)
+        assert r["termination"] == alone["termination"]
+        assert r["iterations"] == alone["iterations"]
+        assert r["energy_evaluations"] == alone["energy_evaluations"]
+        assert abs(r["energy"] - alone["energy"]) <= 1e-9
+        assert np.abs(r["C"] - alone["C"]).max() <= 1e-6
+        assert np.abs(r["grad"] - alone["grad"]).max() <= 1e-6
+    pool.close()
+
+
+def test_pool_optimize_medium_batch(ctx):
+    """Eight medium problems (n_bf=120, n_act=10) converging together, checked against the oracle's
+    energy and gradient at each returned point."""
+    print("case pool optimize medium x8", flush=True)
+    topo = vb.Topology.build("COV", 10, 10, 1)
+    pool = vb.Pool(ctx, topo, capacity=8)
+    sps = [vb.synthetic_problem(s, 120, 10, 360) for s in range(8)]
+    for s, sp in enumerate(sps):
+        pool.load_synthetic(s, sp)
+    res = pool.optimize(list(range(8)), [sp.C for sp in sps], max_iterations=25)
+    for s in (0, 5):
+        chk = oracle.evaluate(120, 360, topo, sps[s].S, sps[s].h, sps[s].B, res[s]["C"])
+        assert res[s]["iterations"] == 25
+        assert abs(chk["energy"] - res[s]["energy"]) <= 1e-8
+        assert np.abs(chk["grad"] - res[s]["grad"]).max() <= 1e-7
+    pool.close()

@@ -475,8 +475,9 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Wait for a lane, record phase times, raise on device flags.
-void finish(vbe_pool* p, Lane& lane) {
+// Wait for a lane, record phase times, raise on device flags (or, with slot_status, report each
+// listed slot's status instead of raising).
+void finish(vbe_pool* p, Lane& lane, std::vector<int>* slot_status = nullptr) {
     check(hipStreamSynchronize(lane.stream), "hipStreamSynchronize");
     lane.busy = false;
     using namespace xmvb_amd;
@@ -499,24 +500,26 @@ void finish(vbe_pool* p, Lane& lane) {
     int64_t singular = 0;
     std::string err;
     int status = VBE_OK;
+    if (slot_status) slot_status->assign(lane.slots.size(), VBE_OK);
     for (size_t i = 0; i < lane.slots.size(); ++i) {
         const int* f = lane.h_flags + FLAG_COUNT * i;
         singular += f[FLAG_SINGULAR_PAIRS];
-        if (status != VBE_OK) continue;
+        const int si = f[FLAG_NORMALIZATION] || f[FLAG_ACTIVE_OVERLAP] ? VBE_ERR_NUMERIC
+                       : f[FLAG_METRIC]                              ? VBE_ERR_METRIC
+                                                                     : VBE_OK;
+        if (slot_status) (*slot_status)[i] = si;
+        if (status != VBE_OK || si == VBE_OK) continue;
+        status = si;
         const std::string where = lane.slots.size() > 1 ? " (slot " + std::to_string(lane.slots[i]) + ")" : "";
-        if (f[FLAG_NORMALIZATION]) {
-            status = VBE_ERR_NUMERIC;
+        if (f[FLAG_NORMALIZATION])
             err = "orbital normalization failed for orbital " + std::to_string(f[FLAG_NORMALIZATION] - 1) + where;
-        } else if (f[FLAG_ACTIVE_OVERLAP]) {
-            status = VBE_ERR_NUMERIC;
+        else if (f[FLAG_ACTIVE_OVERLAP])
             err = "active orbital overlap matrix is not positive definite" + where;
-        } else if (f[FLAG_METRIC]) {
-            status = VBE_ERR_METRIC;
+        else
             err = "structure metric is not positive definite" + where;
-        }
     }
     p->last_singular = singular;
-    if (status != VBE_OK) throw StatusError(status, err);
+    if (status != VBE_OK && !slot_status) throw StatusError(status, err);
 }
 
 Lane& acquire_lane(vbe_pool* p) {
@@ -570,6 +573,51 @@ double eval_problem(vbe_problem* prob, const double* d_C, const uint8_t* d_suppo
     finish(p, lane);
     prob->forward_valid = true;
     return lane.h_energy[1];
+}
+
+// P-LBFGS over the listed slots of a pool (a single problem is a pool of one, passed as prob so
+// that energy-only trials can keep their forward state for the gradient).
+void optimize_slots(vbe_pool* p, vbe_problem* prob, const std::vector<int>& slots,
+                    std::vector<xmvb_amd::OptProblem>& batch, const vbe_optimizer_options& options) {
+    check(hipSetDevice(p->ctx->device), "hipSetDevice");
+    Lane& lane = p->lanes[0];
+    xmvb_amd::OptimizerHooks hooks;
+    hooks.stream = lane.stream;
+    hooks.n_act = p->n_act;
+    hooks.evaluate = [&](const std::vector<int>& which, const std::vector<const double*>& C,
+                         const std::vector<const uint8_t*>& sup, const std::vector<double*>& G, bool with_grad,
+                         std::vector<double>& E) {
+        std::vector<SlotInput> in;
+        for (size_t j = 0; j < which.size(); ++j) in.push_back({slots[which[j]], C[j], sup[j], G[j]});
+        if (prob) prob->forward_valid = false;
+        lane.busy = true;
+        try {
+            enqueue(p, lane, in, with_grad, true);
+        } catch (...) {
+            lane.busy = false;
+            throw;
+        }
+        std::vector<int> status;
+        finish(p, lane, &status);
+        if (prob) prob->forward_valid = true;
+        for (size_t j = 0; j < which.size(); ++j)
+            E[j] = status[j] == VBE_OK ? lane.h_energy[2 * j + 1] : std::numeric_limits<double>::quiet_NaN();
+    };
+    hooks.complete = [&](int, const double* C, double* G) {
+        if (!prob) throw StatusError(VBE_ERR_ARGUMENT, "energy-only trials need a single problem");
+        eval_problem(prob, C, lane.h_desc[0].support, true, G, false);
+    };
+    try {
+        xmvb_amd::run_plbfgs(hooks, batch, options);
+    } catch (const std::invalid_argument& e) {
+        throw StatusError(VBE_ERR_ARGUMENT, e.what());
+    } catch (const StatusError&) {
+        throw;
+    } catch (const HipError&) {
+        throw;
+    } catch (const std::runtime_error& e) {
+        throw StatusError(VBE_ERR_NUMERIC, e.what());
+    }
 }
 
 }  // namespace
@@ -896,43 +944,50 @@ int vbe_optimize(vbe_context* ctx, vbe_problem* prob, double* C_raw, const uint8
         if (prob->pool->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
         if (trace_capacity < 0) throw StatusError(VBE_ERR_ARGUMENT, "negative trace capacity");
         vbe_pool* p = prob->pool.get();
-        Slot& s = p->slots[0];
-        check(hipSetDevice(ctx->device), "hipSetDevice");
-        xmvb_amd::OptimizerHooks hooks;
-        hooks.stream = p->lanes[0].stream;
-        hooks.n_act = p->n_act;
-        hooks.n_bf = s.n_bf;
-        hooks.S_ao = s.S_host.data();
-        hooks.energy = [&](const double* d_C, const uint8_t* d_support, double* d_grad) -> double {
-            try {
-                return eval_problem(prob, d_C, d_support, false, d_grad, true);
-            } catch (const StatusError& e) {
-                // a trial point whose orbitals or structure metric break down is a rejected trial
-                if (e.status == VBE_ERR_NUMERIC || e.status == VBE_ERR_METRIC)
-                    return std::numeric_limits<double>::quiet_NaN();
-                throw;
-            }
-        };
-        hooks.gradient = [&](const double* d_C, const uint8_t* d_support, double* d_grad) {
-            eval_problem(prob, d_C, d_support, true, d_grad, false);
-        };
-        hooks.energy_gradient = [&](const double* d_C, const uint8_t* d_support, double* d_grad) -> double {
-            try {
-                return eval_problem(prob, d_C, d_support, true, d_grad, true);
-            } catch (const StatusError& e) {
-                if (e.status == VBE_ERR_NUMERIC || e.status == VBE_ERR_METRIC)
-                    return std::numeric_limits<double>::quiet_NaN();
-                throw;
-            }
-        };
-        try {
-            xmvb_amd::run_plbfgs(hooks, support, C_raw, *options, *result, grad_raw, energy_trace, trace_capacity);
-        } catch (const std::invalid_argument& e) {
-            throw StatusError(VBE_ERR_ARGUMENT, e.what());
-        } catch (const std::runtime_error& e) {
-            if (dynamic_cast<const StatusError*>(&e) || dynamic_cast<const HipError*>(&e)) throw;
-            throw StatusError(VBE_ERR_NUMERIC, e.what());
+        xmvb_amd::OptProblem op;
+        op.n_bf = p->slots[0].n_bf;
+        op.S_ao = p->slots[0].S_host.data();
+        op.support = support;
+        op.C = C_raw;
+        op.grad = grad_raw;
+        op.trace = energy_trace;
+        op.trace_capacity = trace_capacity;
+        op.result = result;
+        std::vector<xmvb_amd::OptProblem> batch{op};
+        optimize_slots(p, prob, {0}, batch, *options);
+        return VBE_OK;
+    });
+}
+
+int vbe_pool_optimize(vbe_pool* pool, int32_t count, const int32_t* slots, double* C_raw, const uint8_t* support,
+                      const vbe_optimizer_options* options, vbe_optimizer_result* results, double* grad_raw) {
+    return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
+        if (!pool || !slots || !C_raw || !options || !results || count <= 0)
+            throw StatusError(VBE_ERR_ARGUMENT, "null argument or empty slot list");
+        if (count > pool->capacity) throw StatusError(VBE_ERR_ARGUMENT, "more slots than the pool holds");
+        for (const auto& l : pool->lanes)
+            if (l.busy) throw StatusError(VBE_ERR_ARGUMENT, "the pool has submissions in flight; wait for them first");
+        std::vector<int> list(slots, slots + count);
+        std::vector<char> seen(pool->capacity, 0);
+        std::vector<xmvb_amd::OptProblem> batch(count);
+        size_t off = 0;
+        for (int j = 0; j < count; ++j) {
+            const int sl = list[j];
+            if (sl < 0 || sl >= pool->capacity || !pool->slots[sl].loaded)
+                throw StatusError(VBE_ERR_ARGUMENT, "slot not loaded");
+            if (seen[sl]) throw StatusError(VBE_ERR_ARGUMENT, "a slot appears twice");
+            seen[sl] = 1;
+            const Slot& s = pool->slots[sl];
+            auto& op = batch[j];
+            op.n_bf = s.n_bf;
+            op.S_ao = s.S_host.data();
+            op.support = support ? support + off : nullptr;
+            op.C = C_raw + off;
+            op.grad = grad_raw ? grad_raw + off : nullptr;
+            op.result = results + j;
+            off += (size_t)pool->n_act * s.n_bf;
         }
+        optimize_slots(pool, nullptr, list, batch, *options);
         return VBE_OK;
     });
 }

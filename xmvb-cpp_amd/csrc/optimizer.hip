@@ -26,12 +26,16 @@
 #include "optimizer.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <exception>
 #include <functional>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace xmvb_amd {
@@ -102,20 +106,58 @@ __device__ double block_largest(double v, double* red) {
 
 __device__ __forceinline__ bool on_support(const uint8_t* support, size_t at) { return !support || support[at]; }
 
-// Chart at an accepted point, one block per orbital i: normalise c_i in the S metric (the
-// gradient of the scale-invariant energy scales inversely), form S c_i on the support, the
-// projected gradient r_i = g_i - (S c_i)(c_i^T g_i) and its dual norm sqrt(r_i^T S_i^-1 r_i).
-// stats[4 i + 0..3] = dual norm, max |c_i|, norm before normalisation (<= 0: failure), c_i^T g_i.
-__global__ void __launch_bounds__(kPointThreads)
-k_opt_point(int n_bf, const double* __restrict__ S, const double* __restrict__ Sinv, const int* __restrict__ pat,
-            const uint8_t* __restrict__ support, double* __restrict__ C, double* __restrict__ G,
-            double* __restrict__ Sc, double* __restrict__ R, double* __restrict__ stats) {
+// Per-problem view of the optimiser's device state for one batched launch (blockIdx.z indexes the
+// descriptor array). cur / prev / trial rotate on the host, so descriptors are rebuilt per launch.
+struct OptDesc {
+    int n_bf;
+    int N;                      // n_act * n_bf
+    const double* S;            // AO overlap, n_bf x n_bf
+    const double* Sinv;         // support-restricted inverse overlaps, one n_bf x n_bf per pattern
+    const int* pat;             // pattern of each orbital
+    const uint8_t* support;     // N mask or null
+    double* C;                  // current point (normalised in place by the chart)
+    double* G;
+    const double* Cp;           // previous point (secant append)
+    const double* Gp;
+    double* Ct;                 // trial point (trial kernel output)
+    double* Sc;                 // S c_i on the support
+    double* R;                  // projected gradient
+    double* T;                  // search direction
+    double* q;                  // two-loop work vectors
+    double* z;
+    double* stats;              // kOrbStats * n_act chart statistics
+    double* out;                // kOutSlots direction / append scalars
+    double* hs;                 // secant ring: steps, gradient changes, per-orbital statistics
+    double* hy;
+    double* hstat;
+    int cap;                    // ring capacity (>= 1)
+    int count;                  // pairs used by this direction (newest first)
+    int newest;                 // ring slot of the newest pair
+    int slot;                   // ring slot the append writes
+    double step;                // trial step
+};
+
+constexpr int kOutSlots = 8;  // [0] g.t [1] max|t| [2] pairs used [3] gamma [4] bad | [5] accepted [6] max|s|
+
+// Chart at an accepted point, one block per orbital i (grid x = orbital, z = problem): normalise
+// c_i in the S metric (the gradient of the scale-invariant energy scales inversely), form S c_i on
+// the support, the projected gradient r_i = g_i - (S c_i)(c_i^T g_i) and its dual norm
+// sqrt(r_i^T S_i^-1 r_i). stats[4 i + 0..3] = dual norm, max |c_i|, norm before normalisation
+// (<= 0: failure), c_i^T g_i.
+__global__ void __launch_bounds__(kPointThreads) k_opt_point(const OptDesc* __restrict__ descs) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int n_bf = o.n_bf;
     extern __shared__ double lds[];
     double* cs = lds;          // c_i, n_bf
     double* rs = lds + n_bf;   // r_i, n_bf
     __shared__ double red[16];
     const int i = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const size_t row = (size_t)i * n_bf;
+    const double* __restrict__ S = o.S;
+    const uint8_t* support = o.support;
+    double* __restrict__ C = o.C;
+    double* __restrict__ G = o.G;
+    double* __restrict__ Sc = o.Sc;
     for (int k = tid; k < n_bf; k += nt) cs[k] = on_support(support, row + k) ? C[row + k] : 0.0;
     __syncthreads();
     double part = 0.0;
@@ -128,7 +170,7 @@ k_opt_point(int n_bf, const double* __restrict__ S, const double* __restrict__ S
     }
     const double rho2 = block_total(part, red);
     if (!(rho2 > 0.0) || !isfinite(rho2)) {
-        if (tid == 0) stats[kOrbStats * i + 2] = -1.0;
+        if (tid == 0) o.stats[kOrbStats * i + 2] = -1.0;
         return;  // uniform
     }
     const double rho = sqrt(rho2), inv = 1.0 / rho;
@@ -146,11 +188,11 @@ k_opt_point(int n_bf, const double* __restrict__ S, const double* __restrict__ S
     const double cmax_all = block_largest(cmax, red);
     for (int k = tid; k < n_bf; k += nt) {
         const double r = G[row + k] - Sc[row + k] * beta;
-        R[row + k] = r;
+        o.R[row + k] = r;
         rs[k] = r;
     }
     __syncthreads();
-    const double* P = Sinv + (size_t)pat[i] * n_bf * n_bf;
+    const double* P = o.Sinv + (size_t)o.pat[i] * n_bf * n_bf;
     double npart = 0.0;
     for (int k = tid; k < n_bf; k += nt) {
         double a = 0.0;
@@ -159,28 +201,28 @@ k_opt_point(int n_bf, const double* __restrict__ S, const double* __restrict__ S
     }
     const double nu2 = block_total(npart, red);
     if (tid == 0) {
-        stats[kOrbStats * i + 0] = sqrt(fmax(nu2, 0.0));
-        stats[kOrbStats * i + 1] = cmax_all;
-        stats[kOrbStats * i + 2] = rho;
-        stats[kOrbStats * i + 3] = beta;
+        o.stats[kOrbStats * i + 0] = sqrt(fmax(nu2, 0.0));
+        o.stats[kOrbStats * i + 1] = cmax_all;
+        o.stats[kOrbStats * i + 2] = rho;
+        o.stats[kOrbStats * i + 3] = beta;
     }
 }
 
 // Search direction t = -H r by the two-loop recursion (TransportedReducedLbfgsPreconditioner::
 // apply, transported_lbfgs.cpp:41-86) over the secant pairs projected into the current chart,
 // with the scaled-identity initial inverse gamma S^-1 (gamma = s'y'/y'y' of the newest kept pair)
-// and the try_add_pair curvature filter (transported_lbfgs.cpp:14-32). Pairs are addressed by age
-// (0 = newest) in a ring of `cap` slots whose newest slot is `newest`.
-// out: [0] g . t, [1] max |t|, [2] pairs used, [3] gamma, [4] 1 if a recursion coefficient was
-// not finite.
-__global__ void __launch_bounds__(kBlockThreads)
-k_opt_direction(int n_act, int n_bf, const double* __restrict__ C, const double* __restrict__ Sc,
-                const double* __restrict__ G, const double* __restrict__ R, const double* __restrict__ Sinv,
-                const int* __restrict__ pat, const double* __restrict__ hs, const double* __restrict__ hy,
-                const double* __restrict__ hstat, int cap, int count, int newest, double* __restrict__ q,
-                double* __restrict__ z, double* __restrict__ T, double* __restrict__ out) {
+// and the try_add_pair curvature filter (transported_lbfgs.cpp:14-32). One block per problem;
+// pairs are addressed by age (0 = newest) in the ring.
+__global__ void __launch_bounds__(kBlockThreads) k_opt_direction(const OptDesc* __restrict__ descs, int n_act) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int n_bf = o.n_bf, N = o.N, cap = o.cap, count = o.count, newest = o.newest;
+    const double* __restrict__ C = o.C;
+    const double* __restrict__ Sc = o.Sc;
+    const double* __restrict__ hs = o.hs;
+    const double* __restrict__ hy = o.hy;
+    double* __restrict__ q = o.q;
+    double* __restrict__ z = o.z;
     extern __shared__ double lds[];
-    const int N = n_act * n_bf;
     double* sig = lds;                          // (S c_i)^T s_a,i     count x n_act
     double* bet = sig + (size_t)count * n_act;  // c_i^T y_a,i         count x n_act
     double* alp = bet + (size_t)count * n_act;  // first-loop coefficients
@@ -207,7 +249,7 @@ k_opt_direction(int n_act, int n_bf, const double* __restrict__ C, const double*
         const int slot = (newest - a + cap) % cap;
         double ss = 0.0, yy = 0.0, sy = 0.0;
         for (int i = 0; i < n_act; ++i) {
-            const double* st = hstat + ((size_t)slot * n_act + i) * 3;
+            const double* st = o.hstat + ((size_t)slot * n_act + i) * 3;
             const double s_ = sig[a * n_act + i], b_ = bet[a * n_act + i];
             ss += st[0] - s_ * s_;
             yy += st[1] - b_ * b_;
@@ -219,7 +261,7 @@ k_opt_direction(int n_act, int n_bf, const double* __restrict__ C, const double*
         rho[a] = ok ? 1.0 / sy : 0.0;
         gam[a] = ok ? sy / yy : 0.0;
     }
-    for (int e = tid; e < N; e += nt) q[e] = R[e];
+    for (int e = tid; e < N; e += nt) q[e] = o.R[e];
     __syncthreads();
     int used = 0;
     double gamma = 1.0;
@@ -244,7 +286,7 @@ k_opt_direction(int n_act, int n_bf, const double* __restrict__ C, const double*
     __syncthreads();
     for (int e = tid; e < N; e += nt) {  // z = gamma S_i^-1 q_i
         const int i = e / n_bf, k = e - i * n_bf;
-        const double* P = Sinv + (size_t)pat[i] * n_bf * n_bf;
+        const double* P = o.Sinv + (size_t)o.pat[i] * n_bf * n_bf;
         const double* qi = q + (size_t)i * n_bf;
         double acc = 0.0;
         for (int j = 0; j < n_bf; ++j) acc += P[k + (size_t)n_bf * j] * qi[j];
@@ -266,65 +308,65 @@ k_opt_direction(int n_act, int n_bf, const double* __restrict__ C, const double*
     double dd = 0.0, tinf = 0.0;
     for (int e = tid; e < N; e += nt) {
         const double t = -z[e];
-        T[e] = t;
-        dd += G[e] * t;
+        o.T[e] = t;
+        dd += o.G[e] * t;
         tinf = fmax(tinf, fabs(t));
     }
     dd = block_total(dd, red);
     tinf = block_largest(tinf, red);
     if (tid == 0) {
-        out[0] = dd;
-        out[1] = tinf;
-        out[2] = used;
-        out[3] = gamma;
-        out[4] = bad;
+        o.out[0] = dd;
+        o.out[1] = tinf;
+        o.out[2] = used;
+        o.out[3] = gamma;
+        o.out[4] = bad;
     }
 }
 
-// Armijo trial point C + step * T.
-__global__ void k_opt_trial(int N, const double* __restrict__ C, const double* __restrict__ T, double step,
-                            double* __restrict__ Ct) {
+// Armijo trial points C + step T (grid x over elements, z = problem).
+__global__ void k_opt_trial(const OptDesc* __restrict__ descs) {
+    const OptDesc& o = descs[blockIdx.z];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < N) Ct[e] = C[e] + step * T[e];
+    if (e < o.N) o.Ct[e] = o.C[e] + o.step * o.T[e];
 }
 
 // Candidate secant pair (append_projected_secant_pair, transported_lbfgs.cpp:147-176) between the
 // previous and the new accepted point (both normalised): s = C - Cp, y = G - Gp are written to
 // the ring slot with their per-orbital chart-independent statistics s^T S s, y^T S^-1 y, s^T y;
-// the curvature test runs in the new chart. out: [0] 1 if accepted, [1] max |s|.
-__global__ void __launch_bounds__(kBlockThreads)
-k_opt_append(int n_act, int n_bf, const double* __restrict__ S, const double* __restrict__ Sinv,
-             const int* __restrict__ pat, const double* __restrict__ Cp, const double* __restrict__ Gp,
-             const double* __restrict__ C, const double* __restrict__ G, const double* __restrict__ Sc,
-             double* __restrict__ hs, double* __restrict__ hy, double* __restrict__ hstat, double* __restrict__ out) {
+// the curvature test runs in the new chart. out[5] = 1 if accepted, out[6] = max |s|.
+__global__ void __launch_bounds__(kBlockThreads) k_opt_append(const OptDesc* __restrict__ descs, int n_act) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int n_bf = o.n_bf, N = o.N;
+    double* __restrict__ hs = o.hs + (size_t)o.slot * N;
+    double* __restrict__ hy = o.hy + (size_t)o.slot * N;
+    double* __restrict__ hstat = o.hstat + (size_t)o.slot * n_act * 3;
     __shared__ double red[16];
     __shared__ double orb[16 * 5];
-    const int N = n_act * n_bf;
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
     double sinf = 0.0;
     for (int e = tid; e < N; e += nt) {
-        const double s = C[e] - Cp[e];
+        const double s = o.C[e] - o.Cp[e];
         hs[e] = s;
-        hy[e] = G[e] - Gp[e];
+        hy[e] = o.G[e] - o.Gp[e];
         sinf = fmax(sinf, fabs(s));
     }
     sinf = block_largest(sinf, red);  // its barriers also publish hs / hy to the block
     for (int i = wave; i < n_act; i += nw) {
         const double* s = hs + (size_t)i * n_bf;
         const double* y = hy + (size_t)i * n_bf;
-        const double* P = Sinv + (size_t)pat[i] * n_bf * n_bf;
+        const double* P = o.Sinv + (size_t)o.pat[i] * n_bf * n_bf;
         double sss = 0.0, ysy = 0.0, sy = 0.0, sg = 0.0, bt = 0.0;
         for (int k = lane; k < n_bf; k += 64) {
             double a = 0.0, b = 0.0;
             for (int j = 0; j < n_bf; ++j) {
-                a += S[k + (size_t)n_bf * j] * s[j];
+                a += o.S[k + (size_t)n_bf * j] * s[j];
                 b += P[k + (size_t)n_bf * j] * y[j];
             }
             sss += s[k] * a;
             ysy += y[k] * b;
             sy += s[k] * y[k];
-            sg += Sc[(size_t)i * n_bf + k] * s[k];
-            bt += C[(size_t)i * n_bf + k] * y[k];
+            sg += o.Sc[(size_t)i * n_bf + k] * s[k];
+            bt += o.C[(size_t)i * n_bf + k] * y[k];
         }
         sss = wave_total(sss);
         ysy = wave_total(ysy);
@@ -346,33 +388,18 @@ k_opt_append(int n_act, int n_bf, const double* __restrict__ S, const double* __
     if (tid == 0) {
         double ss = 0.0, yy = 0.0, sy = 0.0;
         for (int i = 0; i < n_act; ++i) {
-            const double* o = orb + i * 5;
-            ss += o[0] - o[3] * o[3];
-            yy += o[1] - o[4] * o[4];
-            sy += o[2] - o[3] * o[4];
+            const double* w = orb + i * 5;
+            ss += w[0] - w[3] * w[3];
+            yy += w[1] - w[4] * w[4];
+            sy += w[2] - w[3] * w[4];
         }
         const double sn = sqrt(fmax(ss, 0.0)), yn = sqrt(fmax(yy, 0.0));
         const bool ok = sn > 0.0 && yn > 0.0 && isfinite(sn) && isfinite(yn) && isfinite(sy) &&
                         sy > 1.4901161193847656e-08 * sn * yn;
-        out[0] = ok ? 1.0 : 0.0;
-        out[1] = sinf;
+        o.out[5] = ok ? 1.0 : 0.0;
+        o.out[6] = sinf;
     }
 }
-
-// Device buffers of one optimisation, freed on scope exit.
-struct DeviceBuffers {
-    std::vector<void*> ptrs;
-    template <typename T>
-    T* alloc(size_t count) {
-        void* p = nullptr;
-        hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
-        ptrs.push_back(p);
-        return static_cast<T*>(p);
-    }
-    ~DeviceBuffers() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-};
 
 // Support-restricted inverse overlap of one support pattern, zero-padded to n_bf x n_bf
 // (column-major): Cholesky of S[idx, idx], then the inverse column by column.
@@ -384,7 +411,8 @@ std::vector<double> restricted_inverse(const double* S, int n_bf, const std::vec
             double v = S[idx[a] + (size_t)n_bf * idx[b]];
             for (int k = 0; k < b; ++k) v -= L[(size_t)a * m + k] * L[(size_t)b * m + k];
             if (a == b) {
-                if (!(v > 0.0)) throw std::runtime_error("AO overlap restricted to an orbital support is not positive definite");
+                if (!(v > 0.0))
+                    throw std::runtime_error("AO overlap restricted to an orbital support is not positive definite");
                 L[(size_t)a * m + a] = std::sqrt(v);
             } else {
                 L[(size_t)a * m + b] = v / L[(size_t)b * m + b];
@@ -407,6 +435,65 @@ std::vector<double> restricted_inverse(const double* S, int n_bf, const std::vec
     return out;
 }
 
+// One allocation for the device state of every problem of a batch, freed on scope exit.
+struct DeviceBlock {
+    void* base = nullptr;
+    size_t used = 0, size = 0;
+    explicit DeviceBlock(size_t bytes) : size(bytes) { hip_check(hipMalloc(&base, std::max<size_t>(bytes, 1)), "hipMalloc"); }
+    ~DeviceBlock() {
+        if (base) (void)hipFree(base);
+    }
+    template <typename T>
+    T* take(size_t count) {
+        const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~size_t(255);
+        if (used + bytes > size) throw std::logic_error("optimizer device block overflow");
+        T* p = reinterpret_cast<T*>(static_cast<char*>(base) + used);
+        used += bytes;
+        return p;
+    }
+    static size_t bytes_for(size_t count, size_t elem) { return (std::max<size_t>(count, 1) * elem + 255) & ~size_t(255); }
+};
+
+// Host side of one problem: the control flow of run_plbfgs_backend (plbfgs.cpp:20-226) unrolled
+// into phases, so that a batch advances all its problems with one launch per phase.
+struct Problem {
+    enum Phase { Direction, Trial, Rechart, Done };
+    OptProblem* p = nullptr;
+    int n_bf = 0, N = 0;
+    long long reduced_size = 0;
+    std::vector<int> pat;
+    std::vector<std::vector<int>> patterns;
+    // device state
+    double *S = nullptr, *Sinv = nullptr, *Sc = nullptr, *R = nullptr, *T = nullptr, *q = nullptr, *z = nullptr;
+    double *stats = nullptr, *out = nullptr, *hs = nullptr, *hy = nullptr, *hstat = nullptr;
+    int* d_pat = nullptr;
+    uint8_t* d_support = nullptr;
+    double* pts[3] = {};
+    double* grs[3] = {};
+    int cur = 0, prev = 1, trial = 2;
+    // host copies of the scalars read back after a phase
+    const double* h_stats = nullptr;
+    const double* h_out = nullptr;
+    // control state
+    Phase phase = Direction;
+    int count = 0, newest = 0, slot = 0, use = 0;
+    bool stall_mode = false, first_trial = false, speculate = false, with_grad = false;
+    double E = 0.0, Et = 0.0, previous_energy = 0.0, reference_energy = 0.0;
+    double gnorm = 0.0, cmax = 0.0, gnorm_before = 0.0, cmax_before = 0.0;
+    double dd = 0.0, tinf = 0.0, step = 0.0;
+    int n_trace = 0;
+    vbe_optimizer_result res{};
+    void record(double e) {
+        if (p->trace && n_trace < p->trace_capacity) p->trace[n_trace] = e;
+        ++n_trace;
+    }
+    void finish(int termination, bool converged) {
+        res.termination = termination;
+        res.converged = converged ? 1 : 0;
+        phase = Done;
+    }
+};
+
 }  // namespace
 
 void optimizer_defaults(vbe_optimizer_options* o) {
@@ -419,258 +506,433 @@ void optimizer_defaults(vbe_optimizer_options* o) {
     o->armijo_constant = 1.0e-4;
 }
 
-void run_plbfgs(const OptimizerHooks& h, const uint8_t* support, double* C_host, const vbe_optimizer_options& opt,
-                vbe_optimizer_result& res, double* grad_host, double* trace, int trace_capacity) {
+void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, const vbe_optimizer_options& opt) {
     if (opt.max_iterations < 0 || opt.history_size < 0) throw std::invalid_argument("negative iteration or history count");
     if (!(opt.gradient_tolerance >= 0.0) || !(opt.energy_tolerance >= 0.0) || !(opt.minimum_step_size > 0.0) ||
         !(opt.initial_step_size > 0.0) || !(opt.armijo_constant > 0.0 && opt.armijo_constant < 1.0))
         throw std::invalid_argument("optimizer tolerances and step sizes must be positive (Armijo constant in (0,1))");
-    const int n_act = h.n_act, n_bf = h.n_bf, N = n_act * n_bf;
+    const int n_act = h.n_act;
     if (n_act > 16) throw std::invalid_argument("at most 16 orbitals per optimisation");
+    const int np = (int)problems.size();
+    if (np == 0) return;
+    const bool single = np == 1;
     hipStream_t st = h.stream;
     constexpr double kEps = std::numeric_limits<double>::epsilon();
-
-    // support patterns and their restricted inverse overlaps
-    std::map<std::vector<uint8_t>, int> pattern_of;
-    std::vector<std::vector<int>> patterns;
-    std::vector<int> pat(n_act);
-    long long reduced_size = 0;
-    for (int i = 0; i < n_act; ++i) {
-        std::vector<uint8_t> mask(n_bf, 1);
-        if (support)
-            for (int k = 0; k < n_bf; ++k) mask[k] = support[(size_t)i * n_bf + k] ? 1 : 0;
-        auto it = pattern_of.find(mask);
-        if (it == pattern_of.end()) {
-            std::vector<int> idx;
-            for (int k = 0; k < n_bf; ++k)
-                if (mask[k]) idx.push_back(k);
-            if (idx.empty()) throw std::invalid_argument("an orbital has an empty support");
-            it = pattern_of.emplace(mask, (int)patterns.size()).first;
-            patterns.push_back(idx);
-        }
-        pat[i] = it->second;
-        reduced_size += (long long)patterns[pat[i]].size() - 1;
-    }
-    std::vector<double> sinv_host((size_t)patterns.size() * n_bf * n_bf);
-    for (size_t p = 0; p < patterns.size(); ++p) {
-        const std::vector<double> inv = restricted_inverse(h.S_ao, n_bf, patterns[p]);
-        std::copy(inv.begin(), inv.end(), sinv_host.begin() + p * (size_t)n_bf * n_bf);
-    }
-
-    const int cap = opt.history_size;
-    DeviceBuffers buf;
-    double* d_S = buf.alloc<double>((size_t)n_bf * n_bf);
-    double* d_Sinv = buf.alloc<double>(sinv_host.size());
-    int* d_pat = buf.alloc<int>(n_act);
-    uint8_t* d_support = support ? buf.alloc<uint8_t>(N) : nullptr;
-    double* pts[3] = {buf.alloc<double>(N), buf.alloc<double>(N), buf.alloc<double>(N)};  // orbitals
-    double* grs[3] = {buf.alloc<double>(N), buf.alloc<double>(N), buf.alloc<double>(N)};  // gradients
-    double* d_Sc = buf.alloc<double>(N);
-    double* d_R = buf.alloc<double>(N);
-    double* d_T = buf.alloc<double>(N);
-    double* d_q = buf.alloc<double>(N);
-    double* d_z = buf.alloc<double>(N);
-    double* d_stats = buf.alloc<double>((size_t)kOrbStats * n_act);
-    double* d_out = buf.alloc<double>(8);
-    double* d_hs = buf.alloc<double>((size_t)std::max(cap, 1) * N);
-    double* d_hy = buf.alloc<double>((size_t)std::max(cap, 1) * N);
-    double* d_hstat = buf.alloc<double>((size_t)std::max(cap, 1) * n_act * 3);
-
-    std::vector<double> c0(C_host, C_host + N);
-    if (support)
-        for (int e = 0; e < N; ++e)
-            if (!support[e]) c0[e] = 0.0;
-    hip_check(hipMemcpyAsync(d_S, h.S_ao, sizeof(double) * n_bf * n_bf, hipMemcpyHostToDevice, st), "H2D");
-    hip_check(hipMemcpyAsync(d_Sinv, sinv_host.data(), sizeof(double) * sinv_host.size(), hipMemcpyHostToDevice, st),
-              "H2D");
-    hip_check(hipMemcpyAsync(d_pat, pat.data(), sizeof(int) * n_act, hipMemcpyHostToDevice, st), "H2D");
-    if (support) hip_check(hipMemcpyAsync(d_support, support, N, hipMemcpyHostToDevice, st), "H2D");
-    hip_check(hipMemcpyAsync(pts[0], c0.data(), sizeof(double) * N, hipMemcpyHostToDevice, st), "H2D");
-
-    int cur = 0, prev = 1, trial = 2;  // rotating indices into pts / grs
-    std::vector<double> stats((size_t)kOrbStats * n_act), out(8);
-    auto read = [&](std::vector<double>& dst, const double* src, size_t n) {
-        hip_check(hipMemcpyAsync(dst.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost, st), "D2H");
-        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-    };
-    double gnorm = 0.0, cmax = 0.0;
-    auto launch_chart = [&](int at) {
-        const size_t lds = sizeof(double) * 2 * n_bf;
-        hipLaunchKernelGGL(k_opt_point, dim3(n_act), dim3(kPointThreads), lds, st, n_bf, d_S, d_Sinv, d_pat,
-                           d_support, pts[at], grs[at], d_Sc, d_R, d_stats);
-        hip_check(hipGetLastError(), "k_opt_point");
-        hip_check(hipMemcpyAsync(stats.data(), d_stats, sizeof(double) * stats.size(), hipMemcpyDeviceToHost, st),
-                  "D2H");
-    };
-    // normalise and project at pts[at] (sets gnorm / cmax); with `then`, enqueue more work before
-    // the one synchronisation
-    auto chart = [&](int at, const std::function<void()>& then = {}) {
-        launch_chart(at);
-        if (then) then();
-        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        gnorm = 0.0;
-        cmax = 0.0;
-        for (int i = 0; i < n_act; ++i) {
-            if (!(stats[kOrbStats * i + 2] > 0.0))
-                throw std::runtime_error("orbital " + std::to_string(i) + " has no positive norm on its support");
-            gnorm = std::max(gnorm, stats[kOrbStats * i + 0]);
-            cmax = std::max(cmax, stats[kOrbStats * i + 1]);
-        }
-    };
-    int count = 0, newest = cap - 1;
-    double dd = 0.0, tinf = 0.0;
-    auto direction = [&](int at, int use) {  // search direction at pts[at] with the newest `use` pairs
-        const size_t lds = sizeof(double) * ((size_t)2 * use * n_act + 3 * (size_t)use);
-        hipLaunchKernelGGL(k_opt_direction, dim3(1), dim3(kBlockThreads), lds, st, n_act, n_bf, pts[at], d_Sc,
-                           grs[at], d_R, d_Sinv, d_pat, d_hs, d_hy, d_hstat, std::max(cap, 1), use,
-                           std::max(newest, 0), d_q, d_z, d_T, d_out);
-        hip_check(hipGetLastError(), "k_opt_direction");
-        read(out, d_out, 5);
-        if (out[4] != 0.0) throw std::runtime_error("transported L-BFGS recursion produced a non-finite coefficient");
-        dd = out[0];
-        tinf = out[1];
-    };
     const double zero_step = 128.0 * kEps;
-    // The first trial of a line search runs forward and backward in one launch sequence when the
-    // previous search accepted its first trial (the common case once the secant model is built),
-    // which saves a synchronisation and a launch gap; a rejected speculative trial costs one
-    // backward. Which trials are accepted does not depend on it.
-    bool speculate = false;
-    auto armijo = [&](int at, double E0, double step0, double& Et) -> bool {
-        // try_armijo_backtracking_nonredundant_direction (globalization/line_search.cpp:33-95)
-        if (!std::isfinite(dd) || dd >= 0.0) return false;
-        double step = std::max(opt.minimum_step_size, step0);
-        bool first = true;
-        while (step >= opt.minimum_step_size) {
-            if (step * tinf <= zero_step * std::max(1.0, cmax)) {
-                step *= 0.5;
+    const int cap = std::max(opt.history_size, 1);
+
+    // support patterns, restricted inverse overlaps (host, once per distinct S and pattern)
+    std::vector<Problem> P(np);
+    std::vector<std::vector<double>> sinv_host(np);
+    size_t bytes = 0;
+    for (int k = 0; k < np; ++k) {
+        Problem& s = P[k];
+        s.p = &problems[k];
+        s.n_bf = s.p->n_bf;
+        s.N = n_act * s.n_bf;
+        s.pat.resize(n_act);
+        std::map<std::vector<uint8_t>, int> pattern_of;
+        for (int i = 0; i < n_act; ++i) {
+            std::vector<uint8_t> mask(s.n_bf, 1);
+            if (s.p->support)
+                for (int b = 0; b < s.n_bf; ++b) mask[b] = s.p->support[(size_t)i * s.n_bf + b] ? 1 : 0;
+            auto it = pattern_of.find(mask);
+            if (it == pattern_of.end()) {
+                std::vector<int> idx;
+                for (int b = 0; b < s.n_bf; ++b)
+                    if (mask[b]) idx.push_back(b);
+                if (idx.empty()) throw std::invalid_argument("an orbital has an empty support");
+                it = pattern_of.emplace(mask, (int)s.patterns.size()).first;
+                s.patterns.push_back(idx);
+            }
+            s.pat[i] = it->second;
+            s.reduced_size += (long long)s.patterns[s.pat[i]].size() - 1;
+        }
+        const size_t nn = (size_t)s.n_bf * s.n_bf;
+        sinv_host[k].resize(s.patterns.size() * nn);
+        bytes += DeviceBlock::bytes_for(nn, 8) + DeviceBlock::bytes_for(sinv_host[k].size(), 8) +
+                 DeviceBlock::bytes_for(n_act, 4) + (s.p->support ? DeviceBlock::bytes_for(s.N, 1) : 0) +
+                 11 * DeviceBlock::bytes_for(s.N, 8) + DeviceBlock::bytes_for((size_t)kOrbStats * n_act, 8) +
+                 DeviceBlock::bytes_for(kOutSlots, 8) + 2 * DeviceBlock::bytes_for((size_t)cap * s.N, 8) +
+                 DeviceBlock::bytes_for((size_t)cap * n_act * 3, 8);
+    }
+    {  // the inverses, spread over host threads
+        std::vector<std::pair<int, int>> jobs;
+        for (int k = 0; k < np; ++k)
+            for (int t = 0; t < (int)P[k].patterns.size(); ++t) jobs.emplace_back(k, t);
+        std::atomic<size_t> next{0};
+        std::exception_ptr failure;
+        std::mutex failure_lock;
+        auto work = [&]() {
+            for (size_t j; (j = next.fetch_add(1)) < jobs.size();) {
+                const auto [k, t] = jobs[j];
+                try {
+                    const std::vector<double> inv = restricted_inverse(problems[k].S_ao, P[k].n_bf, P[k].patterns[t]);
+                    std::copy(inv.begin(), inv.end(), sinv_host[k].begin() + (size_t)t * P[k].n_bf * P[k].n_bf);
+                } catch (...) {
+                    std::lock_guard<std::mutex> g(failure_lock);
+                    if (!failure) failure = std::current_exception();
+                }
+            }
+        };
+        const int n_threads = (int)std::min<size_t>(jobs.size(), std::max(1u, std::min(8u, std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < n_threads; ++t) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+        if (failure) std::rethrow_exception(failure);
+    }
+
+    DeviceBlock dev(bytes);
+    const size_t scal_per = (size_t)kOrbStats * n_act + kOutSlots;
+    double* h_scal = nullptr;  // pinned read-back of every problem's stats and out
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_scal), sizeof(double) * scal_per * np, hipHostMallocDefault),
+              "hipHostMalloc");
+    OptDesc* h_desc = nullptr;
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_desc), sizeof(OptDesc) * np, hipHostMallocDefault),
+              "hipHostMalloc");
+    struct PinnedFree {
+        void* a;
+        void* b;
+        ~PinnedFree() {
+            (void)hipHostFree(a);
+            (void)hipHostFree(b);
+        }
+    } pinned_free{h_scal, h_desc};
+    DeviceBlock scal_dev(DeviceBlock::bytes_for(scal_per * np, 8) + DeviceBlock::bytes_for((size_t)np, sizeof(OptDesc)));
+    double* d_scal = scal_dev.take<double>(scal_per * np);
+    OptDesc* d_desc = scal_dev.take<OptDesc>(np);
+    std::vector<double> c0;
+    for (int k = 0; k < np; ++k) {
+        Problem& s = P[k];
+        const size_t nn = (size_t)s.n_bf * s.n_bf;
+        s.S = dev.take<double>(nn);
+        s.Sinv = dev.take<double>(sinv_host[k].size());
+        s.d_pat = dev.take<int>(n_act);
+        s.d_support = s.p->support ? dev.take<uint8_t>(s.N) : nullptr;
+        for (int b = 0; b < 3; ++b) {
+            s.pts[b] = dev.take<double>(s.N);
+            s.grs[b] = dev.take<double>(s.N);
+        }
+        s.Sc = dev.take<double>(s.N);
+        s.R = dev.take<double>(s.N);
+        s.T = dev.take<double>(s.N);
+        s.q = dev.take<double>(s.N);
+        s.z = dev.take<double>(s.N);
+        s.stats = d_scal + scal_per * k;
+        s.out = s.stats + (size_t)kOrbStats * n_act;
+        s.h_stats = h_scal + scal_per * k;
+        s.h_out = s.h_stats + (size_t)kOrbStats * n_act;
+        s.hs = dev.take<double>((size_t)cap * s.N);
+        s.hy = dev.take<double>((size_t)cap * s.N);
+        s.hstat = dev.take<double>((size_t)cap * n_act * 3);
+        s.newest = cap - 1;
+        c0.assign(s.p->C, s.p->C + s.N);
+        if (s.p->support)
+            for (int e = 0; e < s.N; ++e)
+                if (!s.p->support[e]) c0[e] = 0.0;
+        hip_check(hipMemcpyAsync(s.S, s.p->S_ao, sizeof(double) * nn, hipMemcpyHostToDevice, st), "H2D");
+        hip_check(hipMemcpyAsync(s.Sinv, sinv_host[k].data(), sizeof(double) * sinv_host[k].size(),
+                                 hipMemcpyHostToDevice, st), "H2D");
+        hip_check(hipMemcpyAsync(s.d_pat, s.pat.data(), sizeof(int) * n_act, hipMemcpyHostToDevice, st), "H2D");
+        if (s.d_support) hip_check(hipMemcpyAsync(s.d_support, s.p->support, s.N, hipMemcpyHostToDevice, st), "H2D");
+        hip_check(hipMemcpyAsync(s.pts[0], c0.data(), sizeof(double) * s.N, hipMemcpyHostToDevice, st), "H2D");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");  // c0 is reused
+    }
+
+    int max_n_bf = 0, max_N = 0;
+    for (auto& s : P) {
+        max_n_bf = std::max(max_n_bf, s.n_bf);
+        max_N = std::max(max_N, s.N);
+    }
+    auto describe = [&](const std::vector<int>& list) {
+        for (size_t j = 0; j < list.size(); ++j) {
+            Problem& s = P[list[j]];
+            OptDesc d{};
+            d.n_bf = s.n_bf;
+            d.N = s.N;
+            d.S = s.S;
+            d.Sinv = s.Sinv;
+            d.pat = s.d_pat;
+            d.support = s.d_support;
+            d.C = s.pts[s.cur];
+            d.G = s.grs[s.cur];
+            d.Cp = s.pts[s.prev];
+            d.Gp = s.grs[s.prev];
+            d.Ct = s.pts[s.trial];
+            d.Sc = s.Sc;
+            d.R = s.R;
+            d.T = s.T;
+            d.q = s.q;
+            d.z = s.z;
+            d.stats = s.stats;
+            d.out = s.out;
+            d.hs = s.hs;
+            d.hy = s.hy;
+            d.hstat = s.hstat;
+            d.cap = cap;
+            d.count = s.use;
+            d.newest = s.newest;
+            d.slot = s.slot;
+            d.step = s.step;
+            h_desc[j] = d;
+        }
+        hip_check(hipMemcpyAsync(d_desc, h_desc, sizeof(OptDesc) * list.size(), hipMemcpyHostToDevice, st), "H2D");
+    };
+    auto read_back = [&]() {
+        hip_check(hipMemcpyAsync(h_scal, d_scal, sizeof(double) * scal_per * np, hipMemcpyDeviceToHost, st), "D2H");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    };
+    // chart (+ optional secant append) of the listed problems at their current point
+    auto chart = [&](const std::vector<int>& list, bool append) {
+        if (list.empty()) return;
+        describe(list);
+        hipLaunchKernelGGL(k_opt_point, dim3(n_act, 1, (unsigned)list.size()), dim3(kPointThreads),
+                           sizeof(double) * 2 * max_n_bf, st, d_desc);
+        hip_check(hipGetLastError(), "k_opt_point");
+        if (append) {
+            hipLaunchKernelGGL(k_opt_append, dim3(1, 1, (unsigned)list.size()), dim3(kBlockThreads), 0, st, d_desc,
+                               n_act);
+            hip_check(hipGetLastError(), "k_opt_append");
+        }
+        read_back();
+        for (int k : list) {
+            Problem& s = P[k];
+            s.gnorm = 0.0;
+            s.cmax = 0.0;
+            for (int i = 0; i < n_act; ++i) {
+                if (!(s.h_stats[kOrbStats * i + 2] > 0.0))
+                    throw std::runtime_error("orbital " + std::to_string(i) + " has no positive norm on its support");
+                s.gnorm = std::max(s.gnorm, s.h_stats[kOrbStats * i + 0]);
+                s.cmax = std::max(s.cmax, s.h_stats[kOrbStats * i + 1]);
+            }
+        }
+    };
+    auto direction = [&](const std::vector<int>& list) {  // uses each problem's `use` newest pairs
+        if (list.empty()) return;
+        describe(list);
+        int max_use = 0;
+        for (int k : list) max_use = std::max(max_use, P[k].use);
+        const size_t lds = sizeof(double) * ((size_t)2 * max_use * n_act + 3 * (size_t)max_use);
+        hipLaunchKernelGGL(k_opt_direction, dim3(1, 1, (unsigned)list.size()), dim3(kBlockThreads), lds, st, d_desc,
+                           n_act);
+        hip_check(hipGetLastError(), "k_opt_direction");
+        read_back();
+        for (int k : list) {
+            Problem& s = P[k];
+            if (s.h_out[4] != 0.0)
+                throw std::runtime_error("transported L-BFGS recursion produced a non-finite coefficient");
+            s.dd = s.h_out[0];
+            s.tinf = s.h_out[1];
+        }
+    };
+    auto evaluate = [&](const std::vector<int>& list, bool trial_point, bool with_grad) {
+        std::vector<const double*> Cs;
+        std::vector<const uint8_t*> sup;
+        std::vector<double*> Gs;
+        for (int k : list) {
+            Problem& s = P[k];
+            const int b = trial_point ? s.trial : s.cur;
+            Cs.push_back(s.pts[b]);
+            sup.push_back(s.d_support);
+            Gs.push_back(s.grs[b]);
+        }
+        std::vector<double> E(list.size());
+        h.evaluate(list, Cs, sup, Gs, with_grad, E);
+        return E;
+    };
+    auto descent = [](const Problem& s) { return std::isfinite(s.dd) && s.dd < 0.0; };
+    auto zero_direction = [&](const Problem& s) { return s.tinf <= zero_step * std::max(1.0, s.cmax); };
+
+    // starting point: energy, gradient and chart
+    std::vector<int> all(np);
+    for (int k = 0; k < np; ++k) all[k] = k;
+    {
+        const std::vector<double> E = evaluate(all, false, true);
+        for (int k = 0; k < np; ++k) {
+            if (!std::isfinite(E[k])) throw std::runtime_error("the starting orbitals cannot be evaluated");
+            P[k].E = E[k];
+            P[k].res.energy_evaluations = 1;
+            P[k].res.gradient_evaluations = 1;
+        }
+        chart(all, false);
+        for (auto& s : P) {
+            s.record(s.E);
+            s.previous_energy = s.E;
+            s.res.termination = VBE_TERM_MAX_ITERATIONS;
+        }
+    }
+
+    for (;;) {
+        // -- search directions: the iteration's opening checks, then t = -H r with fallbacks
+        std::vector<int> dir;
+        for (int k = 0; k < np; ++k) {
+            Problem& s = P[k];
+            if (s.phase != Problem::Direction) continue;
+            if (!s.stall_mode) {
+                if (s.res.iterations >= opt.max_iterations) {
+                    s.finish(VBE_TERM_MAX_ITERATIONS, false);
+                    continue;
+                }
+                if (s.reduced_size == 0) {
+                    s.finish(VBE_TERM_EMPTY_SPACE, false);
+                    continue;
+                }
+                s.gnorm_before = s.gnorm;
+                if (s.res.iterations == 0 && s.gnorm < opt.gradient_tolerance) {
+                    s.finish(VBE_TERM_INITIAL_TOLERANCE, true);
+                    continue;
+                }
+            }
+            s.use = s.stall_mode || opt.history_size == 0 ? 0 : s.count;
+            dir.push_back(k);
+        }
+        direction(dir);
+        std::vector<int> redo;
+        for (int k : dir) {
+            Problem& s = P[k];
+            if (s.stall_mode) continue;
+            if (!descent(s) || zero_direction(s)) {  // drop the history and take steepest descent
+                s.count = 0;
+                if (s.use > 0) {
+                    s.use = 0;
+                    redo.push_back(k);
+                }
+            }
+        }
+        direction(redo);
+        for (int k : dir) {
+            Problem& s = P[k];
+            if (s.stall_mode) {
+                // Armijo steepest descent from the previous point (plbfgs.cpp:166-191)
+                s.step = std::max(opt.minimum_step_size,
+                                  std::min(std::min(1.0, opt.initial_step_size), 1.0 / std::max(1.0, s.gnorm_before)));
+                if (!descent(s)) {
+                    s.E = s.reference_energy;
+                    s.finish(VBE_TERM_LINE_SEARCH_STALLED, false);
+                    continue;
+                }
+            } else {
+                if (!descent(s)) {
+                    s.finish(VBE_TERM_NON_DESCENT, false);
+                    continue;
+                }
+                s.reference_energy = s.E;
+                s.cmax_before = s.cmax;
+                s.step = std::max(opt.minimum_step_size, std::min(1.0, opt.initial_step_size));
+            }
+            s.first_trial = true;
+            s.phase = Problem::Trial;
+        }
+
+        // -- one Armijo trial per problem in a line search (line_search.cpp:33-95)
+        std::vector<int> tri;
+        for (int k = 0; k < np; ++k) {
+            Problem& s = P[k];
+            if (s.phase != Problem::Trial) continue;
+            while (s.step >= opt.minimum_step_size && s.step * s.tinf <= zero_step * std::max(1.0, s.cmax))
+                s.step *= 0.5;
+            if (s.step < opt.minimum_step_size) {
+                s.speculate = false;
+                if (s.stall_mode) {
+                    s.E = s.reference_energy;
+                    s.finish(VBE_TERM_LINE_SEARCH_STALLED, false);
+                } else {
+                    s.finish(VBE_TERM_LINE_SEARCH_FAILED, false);
+                }
                 continue;
             }
-            hipLaunchKernelGGL(k_opt_trial, dim3((N + 255) / 256), dim3(256), 0, st, N, pts[at], d_T, step,
-                               pts[trial]);
-            hip_check(hipGetLastError(), "k_opt_trial");
-            const bool with_grad = first && speculate;
-            Et = with_grad ? h.energy_gradient(pts[trial], d_support, grs[trial])
-                           : h.energy(pts[trial], d_support, grs[trial]);
-            ++res.energy_evaluations;
-            if (with_grad) ++res.gradient_evaluations;
-            if (std::isfinite(Et) && Et <= E0 + opt.armijo_constant * step * dd) {
+            tri.push_back(k);
+        }
+        bool any_active = false;
+        for (auto& s : P) any_active |= s.phase != Problem::Done;
+        if (!any_active) break;
+        if (tri.empty()) continue;
+        describe(tri);
+        hipLaunchKernelGGL(k_opt_trial, dim3((max_N + 255) / 256, 1, (unsigned)tri.size()), dim3(256), 0, st, d_desc);
+        hip_check(hipGetLastError(), "k_opt_trial");
+        // a batch always evaluates trials with their gradient; one problem alone runs the first
+        // trial of a line search forward + backward when the previous search accepted its first
+        // trial, otherwise energy only, with the backward after acceptance
+        const bool with_grad = !single || (P[tri[0]].first_trial && P[tri[0]].speculate);
+        const std::vector<double> E = evaluate(tri, true, with_grad);
+        std::vector<int> post;
+        for (size_t j = 0; j < tri.size(); ++j) {
+            Problem& s = P[tri[j]];
+            ++s.res.energy_evaluations;
+            if (with_grad) ++s.res.gradient_evaluations;
+            s.Et = E[j];
+            const double E0 = s.stall_mode ? s.reference_energy : s.E;
+            if (std::isfinite(s.Et) && s.Et <= E0 + opt.armijo_constant * s.step * s.dd) {
                 if (!with_grad) {
-                    h.gradient(pts[trial], d_support, grs[trial]);
-                    ++res.gradient_evaluations;
+                    h.complete(tri[j], s.pts[s.trial], s.grs[s.trial]);
+                    ++s.res.gradient_evaluations;
                 }
-                speculate = first;
-                return true;
+                s.speculate = s.first_trial;
+                post.push_back(tri[j]);
+            } else {
+                s.first_trial = false;
+                s.step *= 0.5;
             }
-            first = false;
-            step *= 0.5;
         }
-        speculate = false;
-        return false;
-    };
 
-    res = vbe_optimizer_result{};
-    res.termination = VBE_TERM_MAX_ITERATIONS;
-    int n_trace = 0;
-    auto record = [&](double e) {
-        if (trace && n_trace < trace_capacity) trace[n_trace] = e;
-        ++n_trace;
-    };
-    double E = h.energy(pts[cur], d_support, grs[cur]);
-    ++res.energy_evaluations;
-    if (!std::isfinite(E)) throw std::runtime_error("the starting orbitals cannot be evaluated");
-    h.gradient(pts[cur], d_support, grs[cur]);
-    ++res.gradient_evaluations;
-    chart(cur);
-    record(E);
-    double previous_energy = E;
-
-    for (int iteration = 0; iteration < opt.max_iterations; ++iteration) {
-        if (reduced_size == 0) {
-            res.termination = VBE_TERM_EMPTY_SPACE;
-            break;
+        // -- accepted points: chart, candidate secant pair, stall test, convergence
+        for (int k : post) {
+            Problem& s = P[k];
+            std::swap(s.prev, s.cur);   // prev = the point stepped from
+            std::swap(s.cur, s.trial);  // cur = the accepted trial
+            s.E = s.Et;
+            s.slot = (s.newest + 1) % cap;
         }
-        const double gnorm_before = gnorm;
-        if (iteration == 0 && gnorm < opt.gradient_tolerance) {
-            res.converged = 1;
-            res.termination = VBE_TERM_INITIAL_TOLERANCE;
-            break;
-        }
-        direction(cur, count);
-        if (!(std::isfinite(dd) && dd < 0.0)) {  // not a descent direction: drop the history
-            count = 0;
-            direction(cur, 0);
-        }
-        if (!(std::isfinite(dd) && dd < 0.0) || tinf <= zero_step * std::max(1.0, cmax)) {
-            count = 0;
-            direction(cur, 0);
-        }
-        if (!(std::isfinite(dd) && dd < 0.0)) {
-            res.termination = VBE_TERM_NON_DESCENT;
-            break;
-        }
-        const double reference_energy = E;
-        const double cmax_before = cmax;
-        double Et = 0.0;
-        if (!armijo(cur, E, std::min(1.0, opt.initial_step_size), Et)) {
-            res.termination = VBE_TERM_LINE_SEARCH_FAILED;
-            break;
-        }
-        std::swap(prev, cur);   // prev = the point we stepped from
-        std::swap(cur, trial);  // cur = the accepted trial
-        E = Et;
-        const int slot = cap > 0 ? (newest + 1) % cap : 0;
-        auto append = [&]() {  // enqueue only; the chart's synchronisation completes it
-            hipLaunchKernelGGL(k_opt_append, dim3(1), dim3(kBlockThreads), 0, st, n_act, n_bf, d_S, d_Sinv, d_pat,
-                               pts[prev], grs[prev], pts[cur], grs[cur], d_Sc, d_hs + (size_t)slot * N,
-                               d_hy + (size_t)slot * N, d_hstat + (size_t)slot * n_act * 3, d_out);
-            hip_check(hipGetLastError(), "k_opt_append");
-            hip_check(hipMemcpyAsync(out.data(), d_out, sizeof(double) * 2, hipMemcpyDeviceToHost, st), "D2H");
-        };
-        chart(cur, append);  // the candidate pair also measures the accepted step
-        // a step that changed nothing or made no progress: Armijo steepest descent from the
-        // previous point with a fresh history (plbfgs.cpp:146-191)
-        const double energy_scale = std::max(1.0, std::fabs(reference_energy));
-        const bool no_progress = std::isfinite(reference_energy) && std::isfinite(E) &&
-                                 std::fabs(E - reference_energy) <= 4096.0 * kEps * energy_scale &&
-                                 !(gnorm_before < opt.gradient_tolerance) && gnorm >= 0.9 * gnorm_before;
-        const bool stalled = out[1] <= zero_step * std::max(1.0, cmax_before) || no_progress;
-        if (stalled && gnorm_before >= opt.gradient_tolerance) {
-            std::swap(cur, trial);  // discard the stalled point; trial slot is free again
-            std::swap(cur, prev);   // back at the previous point
-            chart(cur);
-            count = 0;
-            direction(cur, 0);
-            const double sd_step = std::max(opt.minimum_step_size,
-                                            std::min(std::min(1.0, opt.initial_step_size), 1.0 / std::max(1.0, gnorm)));
-            if (!armijo(cur, reference_energy, sd_step, Et)) {
-                E = reference_energy;
-                res.termination = VBE_TERM_LINE_SEARCH_STALLED;
-                break;
+        chart(post, true);
+        std::vector<int> rechart;
+        for (int k : post) {
+            Problem& s = P[k];
+            if (!s.stall_mode) {
+                const double scale = std::max(1.0, std::fabs(s.reference_energy));
+                const bool no_progress = std::isfinite(s.reference_energy) && std::isfinite(s.E) &&
+                                         std::fabs(s.E - s.reference_energy) <= 4096.0 * kEps * scale &&
+                                         !(s.gnorm_before < opt.gradient_tolerance) && s.gnorm >= 0.9 * s.gnorm_before;
+                const bool stalled = s.h_out[6] <= zero_step * std::max(1.0, s.cmax_before) || no_progress;
+                if (stalled && s.gnorm_before >= opt.gradient_tolerance) {
+                    std::swap(s.cur, s.trial);  // discard the stalled point
+                    std::swap(s.cur, s.prev);   // back at the point stepped from
+                    s.stall_mode = true;
+                    s.count = 0;
+                    s.phase = Problem::Rechart;
+                    rechart.push_back(k);
+                    continue;
+                }
             }
-            std::swap(prev, cur);
-            std::swap(cur, trial);
-            E = Et;
-            chart(cur, append);
+            s.stall_mode = false;
+            ++s.res.iterations;
+            s.record(s.E);
+            const double energy_change = s.E - s.previous_energy;
+            s.previous_energy = s.E;
+            if (std::fabs(energy_change) < opt.energy_tolerance && s.gnorm < opt.gradient_tolerance) {
+                s.finish(VBE_TERM_CONVERGED, true);
+                continue;
+            }
+            if (opt.history_size > 0 && s.h_out[5] != 0.0) {
+                s.newest = s.slot;
+                s.count = std::min(s.count + 1, cap);
+            }
+            s.phase = Problem::Direction;
         }
-        ++res.iterations;
-        record(E);
-        const double energy_change = E - previous_energy;
-        previous_energy = E;
-        if (std::fabs(energy_change) < opt.energy_tolerance && gnorm < opt.gradient_tolerance) {
-            res.converged = 1;
-            res.termination = VBE_TERM_CONVERGED;
-            break;
-        }
-        if (cap > 0 && out[0] != 0.0) {
-            newest = slot;
-            count = std::min(count + 1, cap);
-        }
+        chart(rechart, false);
+        for (int k : rechart) P[k].phase = Problem::Direction;
     }
-    res.energy = E;
-    res.gradient_norm = gnorm;
-    hip_check(hipMemcpyAsync(C_host, pts[cur], sizeof(double) * N, hipMemcpyDeviceToHost, st), "D2H");
-    if (grad_host) hip_check(hipMemcpyAsync(grad_host, grs[cur], sizeof(double) * N, hipMemcpyDeviceToHost, st), "D2H");
+
+    // results: orbitals, gradients, energies
+    for (int k = 0; k < np; ++k) {
+        Problem& s = P[k];
+        s.res.energy = s.E;
+        s.res.gradient_norm = s.gnorm;
+        *s.p->result = s.res;
+        hip_check(hipMemcpyAsync(s.p->C, s.pts[s.cur], sizeof(double) * s.N, hipMemcpyDeviceToHost, st), "D2H");
+        if (s.p->grad)
+            hip_check(hipMemcpyAsync(s.p->grad, s.grs[s.cur], sizeof(double) * s.N, hipMemcpyDeviceToHost, st), "D2H");
+    }
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 

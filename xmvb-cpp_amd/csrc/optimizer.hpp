@@ -5,30 +5,44 @@
 
 #include <cstdint>
 #include <functional>
+#include <vector>
 
 #include "../../include/xmvb_amd.h"
 
 namespace xmvb_amd {
 
-// What the optimiser needs from one resident problem. `energy` runs a forward (energy-only)
-// evaluation of device orbitals d_C and retains it, with d_grad as the gradient destination of
-// the later `gradient` call; it returns NaN when the trial point is numerically invalid
-// (normalisation or metric failure), which the line search treats as a rejected trial.
+// One optimisation of a batch: host start orbitals in, converged orbitals out.
+struct OptProblem {
+    int n_bf = 0;
+    const double* S_ao = nullptr;      // host AO overlap, n_bf x n_bf
+    const uint8_t* support = nullptr;  // host n_act x n_bf mask or null
+    double* C = nullptr;               // host n_act x n_bf, in/out
+    double* grad = nullptr;            // host n_act x n_bf or null
+    double* trace = nullptr;           // energy after each accepted iteration (optional)
+    int trace_capacity = 0;
+    vbe_optimizer_result* result = nullptr;
+};
+
+// What the optimiser needs from the engine. `evaluate` runs one launch sequence over the listed
+// problems (indices into the batch) at device orbitals C[j] (support masks sup[j], device or null),
+// writing energies (NaN where the point is numerically invalid: normalisation or metric failure,
+// which the line search treats as a rejected trial) and, with with_grad, gradients into G[j].
+// Without with_grad (only for a batch of one) the forward stays on the device and `complete`
+// turns it into the gradient.
 struct OptimizerHooks {
     hipStream_t stream = nullptr;
     int n_act = 0;
-    int n_bf = 0;
-    const double* S_ao = nullptr;  // host AO overlap, n_bf x n_bf
-    std::function<double(const double* d_C, const uint8_t* d_support, double* d_grad)> energy;
-    std::function<void(const double* d_C, const uint8_t* d_support, double* d_grad)> gradient;
-    // forward and backward in one launch sequence (NaN as for `energy`)
-    std::function<double(const double* d_C, const uint8_t* d_support, double* d_grad)> energy_gradient;
+    std::function<void(const std::vector<int>& which, const std::vector<const double*>& C,
+                       const std::vector<const uint8_t*>& sup, const std::vector<double*>& G, bool with_grad,
+                       std::vector<double>& E)>
+        evaluate;
+    std::function<void(int which, const double* C, double* G)> complete;
 };
 
 void optimizer_defaults(vbe_optimizer_options* o);
 
-// Throws std::invalid_argument for bad options and std::runtime_error for failures.
-void run_plbfgs(const OptimizerHooks& hooks, const uint8_t* support, double* C, const vbe_optimizer_options& opt,
-                vbe_optimizer_result& result, double* grad, double* energy_trace, int trace_capacity);
+// Converges every problem of the batch; problems advance in lockstep, one batched launch per
+// phase. Throws std::invalid_argument for bad options and std::runtime_error for failures.
+void run_plbfgs(const OptimizerHooks& hooks, std::vector<OptProblem>& problems, const vbe_optimizer_options& opt);
 
 }  // namespace xmvb_amd

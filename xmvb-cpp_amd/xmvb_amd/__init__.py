@@ -44,7 +44,7 @@ EXPORTED_SYMBOLS = (
     "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
     "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_eval_device",
     "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit", "vbe_set_ri_precision",
-    "vbe_optimizer_defaults", "vbe_optimize",
+    "vbe_optimizer_defaults", "vbe_optimize", "vbe_pool_optimize",
 )
 
 TERMINATIONS = {0: "max_iterations", 1: "plbfgs_initial_tolerance", 2: "plbfgs_projected_gradient_tolerance",
@@ -148,6 +148,7 @@ def library() -> C.CDLL:
         lib.vbe_optimizer_defaults.argtypes = [C.POINTER(OptimizerOptions)]
         lib.vbe_optimizer_defaults.restype = None
         lib.vbe_optimize.argtypes = [P, P, P, P, C.POINTER(OptimizerOptions), C.POINTER(OptimizerResult), P, P, I32]
+        lib.vbe_pool_optimize.argtypes = [P, I32, P, P, P, C.POINTER(OptimizerOptions), P, P]
         _lib = lib
     return _lib
 
@@ -493,6 +494,35 @@ class Pool:
 
     def evaluate(self, slots, orbitals, gradient: bool = True):
         return self.wait(self.submit(slots, orbitals, gradient))
+
+    def optimize(self, slots, orbitals, supports=None, **options) -> list:
+        """P-LBFGS for several slots at once (lockstep, one batched launch per phase); orbitals and
+        supports: lists of (n_act, n_bf_slot) arrays in slot order. Returns one dict per slot."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        flat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(o, dtype=np.float64).ravel()
+                                                    for o in orbitals]))
+        sup = None if supports is None else np.ascontiguousarray(
+            np.concatenate([np.ascontiguousarray(m, dtype=np.uint8).ravel() for m in supports]))
+        opt = OptimizerOptions()
+        self.ctx._lib.vbe_optimizer_defaults(C.byref(opt))
+        for k, v in options.items():
+            if not hasattr(opt, k):
+                raise TypeError(f"unknown optimizer option {k}")
+            setattr(opt, k, v)
+        res = (OptimizerResult * len(sl))()
+        g = np.zeros_like(flat)
+        self.ctx.check(self.ctx._lib.vbe_pool_optimize(self._h, len(sl), _ptr(sl), _ptr(flat), _ptr(sup), C.byref(opt),
+                                                       C.cast(res, C.c_void_p), _ptr(g)))
+        out, at = [], 0
+        for j, s in enumerate(sl):
+            sz = self.n_act * self.n_bf[s]
+            d = {name: getattr(res[j], name) for name, _ in OptimizerResult._fields_}
+            d["termination"] = TERMINATIONS.get(d["termination"], d["termination"])
+            d.update(C=flat[at:at + sz].reshape(self.n_act, self.n_bf[s]).copy(),
+                     grad=g[at:at + sz].reshape(self.n_act, self.n_bf[s]).copy())
+            out.append(d)
+            at += sz
+        return out
 
     def phase_times(self) -> dict:
         t = PhaseTimes()
