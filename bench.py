@@ -170,6 +170,9 @@ def main():
     ap.add_argument("--mixed-steps", type=int, default=30, help="timed batched steps with FP32 RI factors (0: skip)")
     ap.add_argument("--converge", type=int, default=3, help="full-convergence requests timed (0: skip)")
     ap.add_argument("--converge-iterations", type=int, default=500, help="iteration cap per convergence request")
+    ap.add_argument("--converge-batch", type=int, default=128,
+                    help="scan-point problems converged together in one pool (0: skip)")
+    ap.add_argument("--converge-batch-iterations", type=int, default=100, help="iteration cap in the batched run")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle parity")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -344,7 +347,32 @@ def main():
         converge = dict(runs=runs, max_iterations=args.converge_iterations,
                         options="reference optimizer defaults: gradient_tolerance 1e-3, energy_tolerance 1e-7, "
                                 "history 100, Armijo 1e-4",
-                        plain_eval_ms=plain["ms_per_eval"])
+                        plain_eval_ms=plain["ms_per_eval"],
+                        iterations_per_s_single=sum(x["iterations"] for x in runs) / (sum(x["ms"] for x in runs) * 1e-3))
+        if args.converge_batch > 0:
+            # scan points r in [0.5, 4.0], several replicas each, converged in lockstep in one pool
+            KB = args.converge_batch
+            poolc = vb.Pool(ctx, topo, capacity=KB)
+            starts = []
+            for slot in range(KB):
+                r = 0.5 + 3.5 * (slot % 64) / 63.0
+                spr = vb.synthetic_problem(cfg["seed"] + slot // 64, cfg["n_bf"], cfg["n_act"], cfg["n_aux"], r)
+                poolc.load_synthetic(slot, spr)
+                starts.append(spr.C)
+            poolc.optimize(list(range(KB)), starts, max_iterations=1)  # first-use allocations
+            t0 = time.perf_counter()
+            resb = poolc.optimize(list(range(KB)), starts, max_iterations=args.converge_batch_iterations)
+            dtb = time.perf_counter() - t0
+            its = sum(x["iterations"] for x in resb)
+            term = {}
+            for x in resb:
+                term[x["termination"]] = term.get(x["termination"], 0) + 1
+            converge["batched"] = dict(
+                problems=KB, max_iterations=args.converge_batch_iterations, seconds=dtb, accepted_iterations=its,
+                forward_evaluations=sum(x["energy_evaluations"] for x in resb), iterations_per_s=its / dtb,
+                terminations=term, speedup_vs_single=(its / dtb) / converge["iterations_per_s_single"],
+                workload="64-point scan x 2 seeds, medium size, lockstep rounds")
+            poolc.close()
 
     value = K * args.steps / elapsed
     out = dict(metric=METRIC, value=value, unit="E+grad evals/s", n_gpus=1, steps=args.steps,
