@@ -65,7 +65,9 @@ typedef enum vbe_precision {
 } vbe_precision;
 
 typedef enum vbe_eigensolver {
-    VBE_EIGENSOLVER_DENSE = 0 /* EIGENSOLVER=DENSE (linalg/eigensolver.cpp:365-387) */
+    VBE_EIGENSOLVER_DENSE = 0,    /* EIGENSOLVER=DENSE (linalg/eigensolver.cpp:365-387) */
+    VBE_EIGENSOLVER_DAVIDSON = 1  /* lowest root by Davidson on the resident H and S
+                                     (linalg/eigensolver.cpp:412-435, davidson_core.hpp:271-730) */
 } vbe_eigensolver;
 
 typedef struct vbe_context vbe_context;
@@ -122,7 +124,7 @@ typedef struct vbe_phase_times {
 int vbe_parse_ctrl_method(const char* ctrl_line, int32_t* method);
 
 /* Reject option combinations HIPVBSCF does not implement, in the manner of run_cuvbscf
- * (src/cli/cuvbscf.cpp:38-50): eigensolver must be DENSE, two-electron mode must be RI (ri=1),
+ * (src/cli/cuvbscf.cpp:38-50): eigensolver DENSE or DAVIDSON, two-electron mode must be RI (ri=1),
  * exactly one state, no inactive (core) electrons. Returns VBE_OK or VBE_ERR_UNSUPPORTED and
  * writes the reason to vbe_global_error(). */
 int vbe_check_hipvbscf_options(int32_t eigensolver, int32_t ri, int32_t n_states, int32_t n_inactive_electrons);
@@ -181,6 +183,15 @@ int vbe_set_pair_condition_limit(vbe_context* ctx, double limit);
  * (vbe_precision; default VBE_PRECISION_FP64). Evaluations must then pass the same precision in
  * vbe_flags. No counterpart in the reference, whose RI factors are FP64 only. */
 int vbe_set_ri_precision(vbe_context* ctx, int32_t precision);
+
+/* Structure eigensolver of problems and pools created afterwards (vbe_eval / vbe_eval_device select
+ * it per call through vbe_flags.eigensolver, which then stays with the problem) and the Davidson
+ * accuracy: energy_tolerance (1e-7), relative residual tolerance (1e-3) and CI stationarity
+ * tolerance ||(H - E S) c|| / ||c|| (1e-7), the StructureSolveAccuracy defaults
+ * (src/vbscf/core/contracts/eigensolver.hpp:23-45); reuse_roots (1) starts each slot from its
+ * previous eigenvector (src/vbscf/derivatives/gradient/active_space/evaluator.cpp:233-246). */
+int vbe_set_eigensolver(vbe_context* ctx, int32_t eigensolver, double energy_tolerance, double residual_tolerance,
+                        double stationarity_tolerance, int32_t reuse_roots);
 const char* vbe_global_error(void);
 
 /* Upload the molecule-static data of one problem: AO overlap S and core Hamiltonian h

@@ -99,8 +99,8 @@ int vbe_parse_ctrl_method(const char* ctrl_line, int32_t* method) {
 }
 
 int vbe_check_hipvbscf_options(int32_t eigensolver, int32_t ri, int32_t n_states, int32_t n_inactive_electrons) {
-    if (eigensolver != VBE_EIGENSOLVER_DENSE) {
-        set_global_error("HIPVBSCF supports EIGENSOLVER=DENSE only");
+    if (eigensolver != VBE_EIGENSOLVER_DENSE && eigensolver != VBE_EIGENSOLVER_DAVIDSON) {
+        set_global_error("HIPVBSCF supports EIGENSOLVER=DENSE and EIGENSOLVER=DAVIDSON only");
         return VBE_ERR_UNSUPPORTED;
     }
     if (!ri) {

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -53,6 +54,11 @@ struct vbe_context {
     uint64_t in_use = 0;
     double pair_condition_limit = xmvb_amd::kLaneConditionDefault;
     int ri_precision = VBE_PRECISION_FP64;
+    // structure eigensolver of pools created afterwards; Davidson accuracy (StructureSolveAccuracy
+    // defaults, src/vbscf/core/contracts/eigensolver.hpp:23-45) and root recycling
+    int eigensolver = VBE_EIGENSOLVER_DENSE;
+    double dav_energy_tol = 1.0e-7, dav_residual_tol = 1.0e-3, dav_stationarity_tol = 1.0e-7;
+    bool dav_reuse_roots = true;
     std::string error;
 };
 
@@ -91,6 +97,7 @@ struct Slot {
     std::unique_ptr<DeviceArena> arena;
     SlotDesc desc{};           // workspace + static pointers; per-launch fields filled at launch
     std::vector<double> S_host;  // AO overlap (the optimiser's chart metric)
+    bool has_root = false;     // desc.coef holds this slot's last structure eigenvector
     double* C_buf = nullptr;   // slot-owned orbitals / gradient / support for host-buffer evaluations
     double* grad_buf = nullptr;
     uint8_t* support_buf = nullptr;
@@ -126,6 +133,7 @@ struct vbe_pool {
     vbe_context* ctx = nullptr;
     int n_act = 0, n_alpha = 0, n_beta = 0, capacity = 0;
     int ri_precision = VBE_PRECISION_FP64;
+    int eigensolver = VBE_EIGENSOLVER_DENSE;
     TopoDesc topo{};
     DeviceArena topo_arena;
     std::vector<Slot> slots;
@@ -233,6 +241,7 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     auto p = std::make_unique<vbe_pool>();
     p->ctx = ctx;
     p->ri_precision = ctx->ri_precision;
+    p->eigensolver = ctx->eigensolver;
     p->n_act = n_act;
     p->n_alpha = n_alpha;
     p->n_beta = n_beta;
@@ -325,6 +334,7 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
     check(hipSetDevice(p->ctx->device), "hipSetDevice");
     Slot& s = p->slots[slot];
     s.S_host.assign(S, S + (size_t)n_bf * n_bf);
+    s.has_root = false;
     const TopoDesc& T = p->topo;
     const size_t n = T.n, np = T.np, nd = T.n_det, ns = T.n_str;
     const size_t nbp = (size_t)n_bf * (n_bf + 1) / 2;
@@ -425,6 +435,10 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
     sh.list_blocks = xmvb_amd::kListBlocks;
     sh.lane_cond_limit = p->ctx->pair_condition_limit;
     sh.b32 = p->ri_precision == VBE_PRECISION_FP32_RI;
+    sh.davidson = p->eigensolver == VBE_EIGENSOLVER_DAVIDSON;
+    sh.dav_energy_tol = p->ctx->dav_energy_tol;
+    sh.dav_residual_tol = p->ctx->dav_residual_tol;
+    sh.dav_stationarity_tol = p->ctx->dav_stationarity_tol;
     return sh;
 }
 
@@ -448,6 +462,7 @@ void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool wit
             d.grad = in[i].grad;
             d.energy = lane.d_energy + 2 * i;
             d.flags = lane.d_flags + FLAG_COUNT * i;
+            d.seeded = p->slots[s].has_root && p->ctx->dav_reuse_roots ? 1 : 0;
             lane.h_desc[i] = d;
         }
         upload(lane.d_desc, lane.h_desc, count, st);
@@ -504,10 +519,11 @@ void finish(vbe_pool* p, Lane& lane, std::vector<int>* slot_status = nullptr) {
     for (size_t i = 0; i < lane.slots.size(); ++i) {
         const int* f = lane.h_flags + FLAG_COUNT * i;
         singular += f[FLAG_SINGULAR_PAIRS];
-        const int si = f[FLAG_NORMALIZATION] || f[FLAG_ACTIVE_OVERLAP] ? VBE_ERR_NUMERIC
-                       : f[FLAG_METRIC]                              ? VBE_ERR_METRIC
-                                                                     : VBE_OK;
+        const int si = f[FLAG_NORMALIZATION] || f[FLAG_ACTIVE_OVERLAP] || f[xmvb_amd::FLAG_EIGEN] ? VBE_ERR_NUMERIC
+                       : f[FLAG_METRIC]                                                        ? VBE_ERR_METRIC
+                                                                                               : VBE_OK;
         if (slot_status) (*slot_status)[i] = si;
+        if (lane.events_valid) p->slots[lane.slots[i]].has_root = si == VBE_OK;
         if (status != VBE_OK || si == VBE_OK) continue;
         status = si;
         const std::string where = lane.slots.size() > 1 ? " (slot " + std::to_string(lane.slots[i]) + ")" : "";
@@ -515,6 +531,13 @@ void finish(vbe_pool* p, Lane& lane, std::vector<int>* slot_status = nullptr) {
             err = "orbital normalization failed for orbital " + std::to_string(f[FLAG_NORMALIZATION] - 1) + where;
         else if (f[FLAG_ACTIVE_OVERLAP])
             err = "active orbital overlap matrix is not positive definite" + where;
+        else if (f[xmvb_amd::FLAG_EIGEN])
+            err = std::string("Davidson stopped without meeting its tolerances (") +
+                  (f[xmvb_amd::FLAG_EIGEN] == 1   ? "initial vectors dependent in the overlap metric"
+                   : f[xmvb_amd::FLAG_EIGEN] == 2 ? "complete basis cannot meet the residual"
+                   : f[xmvb_amd::FLAG_EIGEN] == 3 ? "correction space became linearly dependent"
+                                                  : "iteration limit") +
+                  ")" + where;
         else
             err = "structure metric is not positive definite" + where;
     }
@@ -551,8 +574,8 @@ void check_flags_args(const vbe_flags& flags, const vbe_pool* p) {
     if (flags.precision != p->ri_precision)
         throw StatusError(VBE_ERR_ARGUMENT, "precision differs from the one the problem was uploaded with "
                                             "(vbe_set_ri_precision)");
-    if (flags.eigensolver != VBE_EIGENSOLVER_DENSE)
-        throw StatusError(VBE_ERR_UNSUPPORTED, "only EIGENSOLVER=DENSE is implemented");
+    if (flags.eigensolver != VBE_EIGENSOLVER_DENSE && flags.eigensolver != VBE_EIGENSOLVER_DAVIDSON)
+        throw StatusError(VBE_ERR_ARGUMENT, "unknown eigensolver");
 }
 
 // Evaluate slot 0 of a single problem synchronously; C/support/grad are device pointers.
@@ -652,6 +675,22 @@ int vbe_set_ri_precision(vbe_context* ctx, int32_t precision) {
     if (precision != VBE_PRECISION_FP64 && precision != VBE_PRECISION_FP32_RI)
         return fail(ctx, VBE_ERR_ARGUMENT, "unknown precision");
     ctx->ri_precision = precision;
+    return VBE_OK;
+}
+
+int vbe_set_eigensolver(vbe_context* ctx, int32_t eigensolver, double energy_tolerance, double residual_tolerance,
+                        double stationarity_tolerance, int32_t reuse_roots) {
+    if (!ctx) return VBE_ERR_ARGUMENT;
+    if (eigensolver != VBE_EIGENSOLVER_DENSE && eigensolver != VBE_EIGENSOLVER_DAVIDSON)
+        return fail(ctx, VBE_ERR_ARGUMENT, "unknown eigensolver");
+    if (!(energy_tolerance > 0.0) || !(residual_tolerance > 0.0) || !(stationarity_tolerance > 0.0) ||
+        !std::isfinite(energy_tolerance) || !std::isfinite(residual_tolerance) || !std::isfinite(stationarity_tolerance))
+        return fail(ctx, VBE_ERR_ARGUMENT, "Davidson accuracy tolerances must be positive");
+    ctx->eigensolver = eigensolver;
+    ctx->dav_energy_tol = energy_tolerance;
+    ctx->dav_residual_tol = residual_tolerance;
+    ctx->dav_stationarity_tol = stationarity_tolerance;
+    ctx->dav_reuse_roots = reuse_roots != 0;
     return VBE_OK;
 }
 
@@ -852,6 +891,7 @@ int vbe_eval(vbe_context* ctx, vbe_problem* prob, const double* C_raw, const uin
         if (!ctx || !prob || !C_raw || !E_total) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
         if (prob->pool->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
         check_flags_args(flags, prob->pool.get());
+        prob->pool->eigensolver = flags.eigensolver;
         if (flags.want_gradient && !grad_raw) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
         vbe_pool* p = prob->pool.get();
         Slot& s = p->slots[0];
@@ -877,6 +917,7 @@ int vbe_eval_device(vbe_context* ctx, vbe_problem* prob, const double* d_C_raw, 
         if (!ctx || !prob || !d_C_raw || !E_total) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
         if (prob->pool->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
         check_flags_args(flags, prob->pool.get());
+        prob->pool->eigensolver = flags.eigensolver;
         if (flags.want_gradient && !d_grad_raw) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
         *E_total = eval_problem(prob, d_C_raw, d_support, flags.want_gradient != 0, d_grad_raw, true);
         return VBE_OK;
@@ -885,7 +926,8 @@ int vbe_eval_device(vbe_context* ctx, vbe_problem* prob, const double* d_C_raw, 
 
 int vbe_eval_forward(vbe_context* ctx, vbe_problem* prob, const double* C_raw, const uint8_t* support,
                      double* E_total) {
-    vbe_flags f{prob ? prob->pool->ri_precision : VBE_PRECISION_FP64, VBE_EIGENSOLVER_DENSE, 0};
+    vbe_flags f{prob ? prob->pool->ri_precision : VBE_PRECISION_FP64,
+                prob ? prob->pool->eigensolver : VBE_EIGENSOLVER_DENSE, 0};
     return vbe_eval(ctx, prob, C_raw, support, f, E_total, nullptr, nullptr);
 }
 

@@ -1517,7 +1517,9 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
     hipLaunchKernelGGL(k_structure_left, dim3((unsigned)((b + 255) / 256), 1, z), dim3(256), 0, st, slots, topo);
     mark(EV_STRUCT);
     const int threads = 256;
-    if (gep_in_lds(topo.n_str))
+    if (shape.davidson)
+        launch_davidson(st, slots, z, topo, shape);
+    else if (gep_in_lds(topo.n_str))
         hipLaunchKernelGGL(k_gep_lowest<true>, dim3(1, 1, z), dim3(threads), gep_lds_bytes(topo.n_str), st, slots,
                            topo);
     else

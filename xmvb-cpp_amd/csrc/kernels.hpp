@@ -25,6 +25,7 @@ enum DeviceFlag : int {
     FLAG_SINGULAR_PAIRS = 2,  // pairs with a rank-deficient overlap block (SlotDesc::singular)
     FLAG_METRIC = 3,          // structure metric not positive definite
     FLAG_DEFERRED = 4,        // pairs left by the lane kernel for the lane-group kernel (SlotDesc::defer)
+    FLAG_EIGEN = 5,           // Davidson stopped without meeting its tolerances (1 + reason)
     FLAG_COUNT = 8
 };
 
@@ -54,6 +55,7 @@ struct SlotDesc {
     double *partials, *dsum, *dL, *gcn;
     int* defer;     // deferred unordered pair indices (capacity: all pairs)
     int* singular;  // pairs with a rank-deficient block (capacity: all pairs)
+    int seeded;     // coef holds this slot's previous structure eigenvector (Davidson start)
 };
 
 struct TopoDesc {
@@ -80,6 +82,8 @@ struct LaunchShape {
     int list_blocks;                       // lane-group blocks per slot for the deferred pairs
     double lane_cond_limit;                // lane kernel keeps blocks with max|X| max|X^-1| below this
     bool b32;                              // RI factors stored in FP32
+    bool davidson;                         // structure eigenproblem by Davidson instead of dense
+    double dav_energy_tol, dav_residual_tol, dav_stationarity_tol;
 };
 
 // largest condition bound under which a row-pivoted lane block is provably regular under the
@@ -95,6 +99,13 @@ constexpr int kListBlocks = 4;
 constexpr int kSingularBlocks = 4;
 
 inline std::size_t gep_work_doubles(int n) { return 3 * (std::size_t)n * n + 9 * (std::size_t)n; }
+
+// Davidson subspace limit for one root: min(n, max(32, 8 n_roots)) (make_davidson_options,
+// linalg/eigensolver.cpp:321-361)
+constexpr int kDavidsonMaxSubspace = 32;
+// lowest structure root by block Davidson on the resident H and S (davidson.hip); workspace
+// 3 n min(n, 32) + 9 n doubles of SlotDesc::gep
+void launch_davidson(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape);
 
 int pair_group(int mmax);
 bool pairs_lane_supported(int n_alpha, int n_beta, bool backward);

@@ -44,7 +44,7 @@ EXPORTED_SYMBOLS = (
     "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
     "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_eval_device",
     "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit", "vbe_set_ri_precision",
-    "vbe_optimizer_defaults", "vbe_optimize", "vbe_pool_optimize",
+    "vbe_optimizer_defaults", "vbe_optimize", "vbe_pool_optimize", "vbe_set_eigensolver",
 )
 
 TERMINATIONS = {0: "max_iterations", 1: "plbfgs_initial_tolerance", 2: "plbfgs_projected_gradient_tolerance",
@@ -124,6 +124,7 @@ def library() -> C.CDLL:
         lib.vbe_last_error.argtypes = [P, C.c_char_p, C.c_size_t]
         lib.vbe_set_pair_condition_limit.argtypes = [P, D]
         lib.vbe_set_ri_precision.argtypes = [P, I32]
+        lib.vbe_set_eigensolver.argtypes = [P, I32, D, D, D, I32]
         lib.vbe_problem_upload.argtypes = [P, C.POINTER(Dims), P, P, P, C.POINTER(TopologyView), D, C.POINTER(P)]
         lib.vbe_problem_release.argtypes = [P]
         lib.vbe_problem_release.restype = None
@@ -291,6 +292,14 @@ class Context:
         complete-pivoting path; 0 sends every pair there."""
         self.check(self._lib.vbe_set_pair_condition_limit(self._h, C.c_double(limit)))
 
+    def set_eigensolver(self, eigensolver: int, energy_tolerance: float = 1e-7, residual_tolerance: float = 1e-3,
+                        stationarity_tolerance: float = 1e-7, reuse_roots: bool = True):
+        """0: dense GEP; 1: Davidson (tolerances as StructureSolveAccuracy). Applies to problems and
+        pools created afterwards; Problem.evaluate's `eigensolver` argument selects it per call."""
+        self.check(self._lib.vbe_set_eigensolver(self._h, eigensolver, energy_tolerance, residual_tolerance,
+                                                 stationarity_tolerance, int(reuse_roots)))
+        self.eigensolver = eigensolver
+
     def set_ri_precision(self, precision: int):
         """0: FP64 RI factors; 1: FP32 RI factors (FP64 accumulation). Applies to problems and pools
         created afterwards."""
@@ -316,6 +325,7 @@ class Problem:
                  B: np.ndarray, e_nuc: float = 0.0):
         self.ctx, self.topo = ctx, topo
         self.precision = getattr(ctx, "ri_precision", 0)
+        self.eigensolver = getattr(ctx, "eigensolver", 0)
         self.n_bf, self.n_act, self.n_aux = n_bf, topo.n_act, n_aux
         dims = Dims(n_bf, topo.n_act, topo.n_alpha, topo.n_beta, n_aux)
         self._S = np.asfortranarray(S, dtype=np.float64)
@@ -342,7 +352,7 @@ class Problem:
         sup = None if support is None else np.ascontiguousarray(support, dtype=np.uint8)
         e = C.c_double()
         g = np.zeros_like(Cm) if gradient else None
-        self.ctx.check(self.ctx._lib.vbe_eval(self.ctx._h, self._h, _ptr(Cm), _ptr(sup), Flags(self.precision, 0, int(gradient)),
+        self.ctx.check(self.ctx._lib.vbe_eval(self.ctx._h, self._h, _ptr(Cm), _ptr(sup), Flags(self.precision, self.eigensolver, int(gradient)),
                                               C.byref(e), None, _ptr(g)))
         return (e.value, g) if gradient else e.value
 
@@ -363,7 +373,7 @@ class Problem:
         e = C.c_double()
         self.ctx.check(self.ctx._lib.vbe_eval_device(self.ctx._h, self._h, C.c_void_p(C_ptr),
                                                      None if support_ptr is None else C.c_void_p(support_ptr),
-                                                     Flags(self.precision, 0, int(gradient)), C.byref(e), C.c_void_p(grad_ptr)))
+                                                     Flags(self.precision, self.eigensolver, int(gradient)), C.byref(e), C.c_void_p(grad_ptr)))
         return e.value
 
     def optimize(self, Cm: np.ndarray, support: np.ndarray | None = None, **options) -> dict:
