@@ -129,6 +129,22 @@ int vbe_parse_ctrl_method(const char* ctrl_line, int32_t* method);
  * writes the reason to vbe_global_error(). */
 int vbe_check_hipvbscf_options(int32_t eigensolver, int32_t ri, int32_t n_states, int32_t n_inactive_electrons);
 
+/* ---- AO integrals for small decks (host) ---------------------------------------------------
+ * Replaces the libcint materialised provider (src/integrals/libcint/materialized_provider.cpp) for
+ * the basis data embedded in the library: hydrogen in 6-31G / 6-31G* / 6-31G** (basis/6-31gss.gbs,
+ * Cartesian functions, per atom: s(3G), s, then p x, y, z for **). McMurchie-Davidson integrals. */
+int vbe_gaussian_basis_size(const char* basis, int32_t n_atoms, const int32_t* Z, int32_t* n_bf);
+/* S, T, V (n_bf x n_bf column-major, any may be NULL), two-electron integrals over packed pairs
+ * eri_packed[PQ + npair RS] (npair = n_bf(n_bf+1)/2, pair = hi(hi+1)/2+lo; NULL to skip), the atom
+ * of each function and the nuclear repulsion energy. Coordinates in bohr (the deck loader converts
+ * angstrom with 1/0.529177249, src/input/loading/loader.cpp:42). */
+int vbe_ao_integrals(const char* basis, int32_t n_atoms, const int32_t* Z, const double* xyz_bohr, double* S,
+                     double* T, double* V, double* eri_packed, int32_t* atom_of_function, double* e_nuc);
+/* Pivoted Cholesky M ~= B^T B of a symmetric positive semidefinite n x n matrix (B: max_rank x n,
+ * column-major; *rank rows used), stopping when the largest remaining diagonal is <= tolerance:
+ * turns exact two-electron integrals into the engine's RI-shaped factors. */
+int vbe_pivoted_cholesky(int32_t n, const double* M, double tolerance, int32_t max_rank, int32_t* rank, double* B);
+
 /* ---- topology (host) ---------------------------------------------------------------------- */
 
 /* Closed-form structure count for a structure-class keyword ("COV", "FULL", "ION", "COV ION(1)"),

@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_eval_device",
     "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit", "vbe_set_ri_precision",
     "vbe_optimizer_defaults", "vbe_optimize", "vbe_pool_optimize", "vbe_set_eigensolver",
+    "vbe_gaussian_basis_size", "vbe_ao_integrals", "vbe_pivoted_cholesky",
 )
 
 TERMINATIONS = {0: "max_iterations", 1: "plbfgs_initial_tolerance", 2: "plbfgs_projected_gradient_tolerance",
@@ -125,6 +126,9 @@ def library() -> C.CDLL:
         lib.vbe_set_pair_condition_limit.argtypes = [P, D]
         lib.vbe_set_ri_precision.argtypes = [P, I32]
         lib.vbe_set_eigensolver.argtypes = [P, I32, D, D, D, I32]
+        lib.vbe_gaussian_basis_size.argtypes = [C.c_char_p, I32, P, C.POINTER(I32)]
+        lib.vbe_ao_integrals.argtypes = [C.c_char_p, I32, P, P, P, P, P, P, P, C.POINTER(D)]
+        lib.vbe_pivoted_cholesky.argtypes = [I32, P, D, I32, C.POINTER(I32), P]
         lib.vbe_problem_upload.argtypes = [P, C.POINTER(Dims), P, P, P, C.POINTER(TopologyView), D, C.POINTER(P)]
         lib.vbe_problem_release.argtypes = [P]
         lib.vbe_problem_release.restype = None
@@ -268,6 +272,67 @@ def synthetic_problem(seed: int, n_bf: int, n_act: int, n_aux: int, scan_r: floa
     _check_global(library().vbe_synthetic_problem(seed, n_bf, n_act, n_aux, scan_r, _ptr(S), _ptr(h), _ptr(B),
                                                   _ptr(Cm)))
     return SyntheticProblem(n_bf, n_act, n_aux, S, h, B, Cm)
+
+
+BOHR_PER_ANGSTROM = 1.0 / 0.529177249  # src/input/loading/loader.cpp:42
+
+
+def ao_integrals(basis: str, Z, xyz_bohr, eri: bool = True) -> dict:
+    """AO integrals of a molecule from the embedded basis data (hydrogen, 6-31G family):
+    S, T, V, h = T + V, packed-pair two-electron integrals, atom of each function, E_nuc."""
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1)
+    n = C.c_int32()
+    _check_global(library().vbe_gaussian_basis_size(basis.encode(), len(Z), _ptr(Z), C.byref(n)))
+    nb = n.value
+    npair = nb * (nb + 1) // 2
+    out = dict(S=np.zeros((nb, nb), order="F"), T=np.zeros((nb, nb), order="F"), V=np.zeros((nb, nb), order="F"),
+               eri=np.zeros((npair, npair), order="F") if eri else None, atom=np.zeros(nb, np.int32))
+    e = C.c_double()
+    _check_global(library().vbe_ao_integrals(basis.encode(), len(Z), _ptr(Z), _ptr(xyz), _ptr(out["S"]),
+                                             _ptr(out["T"]), _ptr(out["V"]), _ptr(out["eri"]), _ptr(out["atom"]),
+                                             C.byref(e)))
+    out["h"] = out["T"] + out["V"]
+    out["e_nuc"] = e.value
+    out["n_bf"] = nb
+    return out
+
+
+def pivoted_cholesky(M: np.ndarray, tolerance: float = 1e-13) -> np.ndarray:
+    """B (rank x n) with M ~= B^T B (pivoted Cholesky, largest remaining diagonal first)."""
+    M = np.asfortranarray(M, dtype=np.float64)
+    n = M.shape[0]
+    B = np.zeros((n, n), order="F")
+    r = C.c_int32()
+    _check_global(library().vbe_pivoted_cholesky(n, _ptr(M), tolerance, n, C.byref(r), _ptr(B)))
+    return np.asfortranarray(B[: r.value, :])
+
+
+def h2_deck(distance_angstrom: float = 0.74, basis: str = "6-31G**") -> dict:
+    """The H2 deck: two hydrogens on the z axis, two VB orbitals each expanded only on its own atom's
+    basis functions (the $FRAG/$ORB assignment of an HAO deck), STR=FULL NAO=2 NAE=2 singlet
+    (1 covalent + 2 ionic structures), exact two-electron integrals written as pivoted-Cholesky
+    factors (rank <= npair, remaining diagonal <= 1e-13). Starting orbitals: the lowest root of the
+    core Hamiltonian restricted to each atom's functions. Returns S, h, B (n_aux x npair), n_aux,
+    e_nuc, support, C0 and the topology."""
+    R = distance_angstrom * BOHR_PER_ANGSTROM
+    ints = ao_integrals(basis, [1, 1], [[0.0, 0.0, 0.0], [0.0, 0.0, R]])
+    n = ints["n_bf"]
+    B = pivoted_cholesky(ints["eri"], 1e-13)
+    support = np.zeros((2, n), np.uint8)
+    C0 = np.zeros((2, n))
+    for k in range(2):
+        idx = np.flatnonzero(ints["atom"] == k)
+        support[k, idx] = 1
+        Sk = ints["S"][np.ix_(idx, idx)]
+        hk = ints["h"][np.ix_(idx, idx)]
+        L = np.linalg.cholesky(Sk)
+        Li = np.linalg.inv(L)
+        w, U = np.linalg.eigh(Li @ hk @ Li.T)
+        C0[k, idx] = Li.T @ U[:, 0]
+    topo = Topology.build("FULL", 2, 2, 1)
+    return dict(S=ints["S"], h=ints["h"], B=np.ascontiguousarray(B.ravel(order="F")), n_aux=B.shape[0], n_bf=n,
+                e_nuc=ints["e_nuc"], support=support, C0=C0, topo=topo, eri=ints["eri"], atom=ints["atom"])
 
 
 class Context:
