@@ -680,7 +680,7 @@ __global__ void __launch_bounds__(256) LANE_WAVES
 template <int M0, int M1>
 struct LaneShape {
     static constexpr bool forward = M0 >= M1 && M0 <= 6 && M0 - M1 <= 2;
-    static constexpr bool backward = forward && M0 <= 5;
+    static constexpr bool backward = forward && M0 <= 6;
 };
 
 namespace {
