@@ -14,7 +14,9 @@ Three legs are timed:
   * served: Poisson arrivals (--serve-rate requests/s for --serve-seconds) over a 64-point
     dissociation scan (inter-fragment h and B blocks scaled by exp(-0.5 r), r in [0.5, 4.0]),
     continuously batched onto two in-flight submissions (submit / poll / wait); request latency
-    is completion minus arrival, host orbitals in and gradients out (PCIe included);
+    is completion minus arrival, host orbitals in and gradients out (PCIe included); run once with
+    E+grad requests only and once with a second Poisson stream of full-convergence requests
+    (P-LBFGS, converged in lockstep on their own pool and host thread) sharing the GPU;
   * converge: --converge full-convergence requests (projected L-BFGS on device-resident state,
     reference optimizer defaults) at scan points, reported as latency per request and per
     iteration next to the plain evaluation time.
@@ -89,8 +91,14 @@ def cpu_baseline(cfg, sp, topo, gpu_energy, gpu_grad, seconds: float):
     return base, parity
 
 
-def serve(vb, ctx, topo, cfg, rate: float, seconds: float, replicas: int, max_batch: int, seed: int = 7):
-    """Poisson arrivals over a 64-point scan, continuously batched onto the pool's two lanes."""
+def serve(vb, ctx, topo, cfg, rate: float, seconds: float, replicas: int, max_batch: int, seed: int = 7,
+          converge_rate: float = 0.0, converge_iterations: int = 150):
+    """Poisson arrivals over a 64-point scan, continuously batched onto the pool's two lanes. With
+    converge_rate > 0 a second Poisson stream of full-convergence requests (P-LBFGS from the scan
+    point's start orbitals) runs beside it on its own pool and host thread: requests that have
+    arrived are converged together in lockstep, one batch at a time."""
+    import threading
+
     import numpy as np
 
     points = 64
@@ -108,6 +116,47 @@ def serve(vb, ctx, topo, cfg, rate: float, seconds: float, replicas: int, max_ba
     t = pool.submit(ref_slots, [orb[p] for p in range(points)], gradient=False)
     e_ref, _ = pool.wait(t)
 
+    conv = None
+    if converge_rate > 0:
+        cpool = vb.Pool(ctx, topo, capacity=points)
+        for p, r in enumerate(rs):
+            cpool.load_synthetic(p, vb.synthetic_problem(1000 + p, cfg["n_bf"], cfg["n_act"], cfg["n_aux"], scan_r=r))
+        crng = np.random.default_rng(seed + 1)
+        n_conv = max(1, int(converge_rate * seconds))
+        c_arrivals = np.cumsum(crng.exponential(1.0 / converge_rate, n_conv))
+        c_point = crng.integers(0, points, n_conv)
+        conv = dict(latency=[], iterations=[], batches=[], terminations={})
+
+    def converge_worker(t0):
+        try:
+            converge_loop(t0)
+        except BaseException as exc:  # re-raised by the serving thread after join
+            conv["error"] = exc
+
+    def converge_loop(t0):
+        pending, nxt_c = [], 0
+        while nxt_c < n_conv or pending:
+            now = time.perf_counter() - t0
+            while nxt_c < n_conv and c_arrivals[nxt_c] <= now:
+                pending.append(nxt_c)
+                nxt_c += 1
+            if not pending:
+                time.sleep(max(0.0, min(c_arrivals[nxt_c] - now, 5e-3)))
+                continue
+            batch, seen, rest = [], set(), []
+            for q in pending:  # one request per scan point per batch
+                (rest if c_point[q] in seen else batch).append(q)
+                seen.add(c_point[q])
+            res = cpool.optimize([int(c_point[q]) for q in batch], [orb[c_point[q]] for q in batch],
+                                 max_iterations=converge_iterations)
+            fin = time.perf_counter() - t0
+            for q, r in zip(batch, res):
+                conv["latency"].append((fin - c_arrivals[q]) * 1e3)
+                conv["iterations"].append(r["iterations"])
+                conv["terminations"][r["termination"]] = conv["terminations"].get(r["termination"], 0) + 1
+            conv["batches"].append(len(batch))
+            pending = rest
+
     rng = np.random.default_rng(seed)
     n_req = max(1, int(rate * seconds))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
@@ -116,6 +165,10 @@ def serve(vb, ctx, topo, cfg, rate: float, seconds: float, replicas: int, max_ba
     queue, inflight = [], {}  # inflight: ticket -> (requests, slots)
     nxt = 0
     t0 = time.perf_counter()
+    worker = None
+    if conv is not None:
+        worker = threading.Thread(target=converge_worker, args=(t0,))
+        worker.start()
     done = 0
     while done < n_req:
         now = time.perf_counter() - t0
@@ -149,13 +202,25 @@ def serve(vb, ctx, topo, cfg, rate: float, seconds: float, replicas: int, max_ba
         if not queue and not inflight and nxt < n_req:
             time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 2e-4)))
     makespan = time.perf_counter() - t0
+    converge = None
+    if worker is not None:
+        worker.join()
+        cpool.close()
+        if "error" in conv:
+            raise conv["error"]
+        converge = dict(requests=n_conv, offered_rate=converge_rate, max_iterations=converge_iterations,
+                        latency_ms=dict(p50=percentile(conv["latency"], 0.5), p99=percentile(conv["latency"], 0.99),
+                                        max=max(conv["latency"])),
+                        mean_iterations=float(np.mean(conv["iterations"])), batches=len(conv["batches"]),
+                        mean_batch=float(np.mean(conv["batches"])), terminations=conv["terminations"],
+                        makespan_s=time.perf_counter() - t0)
     pool.close()
     return dict(requests=n_req, offered_rate=rate, throughput=n_req / makespan, seconds=makespan,
                 latency_ms=dict(p50=percentile(latency, 0.5), p99=percentile(latency, 0.99),
                                 max=max(latency)),
                 mean_batch=float(np.mean(batches)), launches=len(batches), max_abs_dE_vs_direct=energies_err,
                 scan=f"{points} points r in [0.5, 4.0], {replicas} resident replicas each",
-                io="host orbitals in, host gradients out (PCIe included)")
+                io="host orbitals in, host gradients out (PCIe included)", convergence_requests=converge)
 
 
 def main():
@@ -163,10 +228,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100, help="timed batched steps (K evaluations each)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--batch", type=int, default=128, help="resident problems evaluated per batched step")
+    ap.add_argument("--batch", type=int, default=512, help="resident problems evaluated per batched step")
+    ap.add_argument("--distinct", type=int, default=128,
+                    help="distinct seeded problems behind the batch (slot s holds problem s mod this)")
     ap.add_argument("--plain-steps", type=int, default=300, help="timed single-problem evaluations")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the served leg")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the served leg (0: skip)")
+    ap.add_argument("--serve-converge-rate", type=float, default=4.0,
+                    help="Poisson rate of full-convergence requests mixed into the served leg (0: none)")
+    ap.add_argument("--serve-converge-iterations", type=int, default=150,
+                    help="iteration cap of a served convergence request")
     ap.add_argument("--mixed-steps", type=int, default=30, help="timed batched steps with FP32 RI factors (0: skip)")
     ap.add_argument("--converge", type=int, default=3, help="full-convergence requests timed (0: skip)")
     ap.add_argument("--converge-iterations", type=int, default=500, help="iteration cap per convergence request")
@@ -235,8 +306,15 @@ def main():
     K = args.batch
     pool = vb.Pool(ctx, topo, capacity=K)
     orbitals = []
+    # distinct problems generated on host threads (the generator releases the GIL); each slot holds
+    # its own device copy of problem s mod D
+    from concurrent.futures import ThreadPoolExecutor
+    D = max(1, min(K, args.distinct))
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        distinct = [sp] + list(ex.map(lambda s: vb.synthetic_problem(cfg["seed"] + s, cfg["n_bf"], cfg["n_act"],
+                                                                     cfg["n_aux"]), range(1, D)))
     for s in range(K):
-        sps = sp if s == 0 else vb.synthetic_problem(cfg["seed"] + s, cfg["n_bf"], cfg["n_act"], cfg["n_aux"])
+        sps = distinct[s % D]
         pool.load_synthetic(s, sps)
         orbitals.append(sps.C)
     cb_dev = torch.from_numpy(np.concatenate([o.ravel() for o in orbitals])).to(dev)
@@ -308,8 +386,7 @@ def main():
         pool32 = vb.Pool(ctx32, topo, capacity=K)
         e64 = np.array([float(x) for x in energies])
         for s in range(K):
-            sps = sp if s == 0 else vb.synthetic_problem(cfg["seed"] + s, cfg["n_bf"], cfg["n_act"], cfg["n_aux"])
-            pool32.load_synthetic(s, sps)
+            pool32.load_synthetic(s, distinct[s % D])
         for _ in range(args.warmup):
             pool32.evaluate_device(slots, cb_dev.data_ptr(), gb_dev.data_ptr())
         torch.cuda.synchronize()
@@ -328,6 +405,10 @@ def main():
     served = None
     if args.serve_seconds > 0:
         served = serve(vb, ctx, topo, cfg, args.serve_rate, args.serve_seconds, replicas=4, max_batch=K)
+        if args.serve_converge_rate > 0:
+            served["with_convergence_requests"] = serve(
+                vb, ctx, topo, cfg, args.serve_rate, args.serve_seconds, replicas=4, max_batch=K,
+                converge_rate=args.serve_converge_rate, converge_iterations=args.serve_converge_iterations)
 
     converge = None
     if args.converge > 0:
@@ -422,8 +503,8 @@ def main():
                warmup=args.warmup, ms_per_step=elapsed * 1e3 / args.steps, higher_is_better=True, scaling="weak",
                vs_baseline=None, dtype="f64", data="synthetic",
                config=dict(workload="medium: n_bf=120 n_aux=360 n_act=10 (5a/5b) 42 covalent Rumer structures, "
-                                    f"252 determinants; {K} resident problems per batched step, FP64 throughout, "
-                                    "orbitals resident",
+                                    f"252 determinants; {K} resident problems per batched step ({D} distinct "
+                                    "seeded problems, each slot its own device copy), FP64 throughout, orbitals resident",
                            seed=cfg["seed"], n_bf=cfg["n_bf"], n_aux=cfg["n_aux"], n_act=cfg["n_act"],
                            n_structures=topo.n_structures, n_determinants=n_det, batch=K, path="batched-fp64"),
                latency_ms=(served["latency_ms"] if served else dict(p50=percentile(lat, 0.5),

@@ -130,3 +130,19 @@ def test_pool_optimize_medium_batch(ctx):
         assert abs(chk["energy"] - res[s]["energy"]) <= 1e-8
         assert np.abs(chk["grad"] - res[s]["grad"]).max() <= 1e-7
     pool.close()
+
+
+def test_pool_optimize_single_slot(ctx):
+    """A pool batch of one slot evaluates its trials with gradients (no retained single-problem
+    forward to complete) and still follows the iterates of the problem alone."""
+    topo = vb.Topology.build("COV", 4, 4, 1)
+    sp = vb.synthetic_problem(2, 16, 4, 48)
+    pool = vb.Pool(ctx, topo, capacity=2)
+    pool.load_synthetic(1, sp)
+    r = pool.optimize([1], [sp.C], max_iterations=20)[0]
+    prob = vb.Problem.from_synthetic(ctx, sp, topo)
+    alone = prob.optimize(sp.C, max_iterations=20)
+    prob.close()
+    assert r["iterations"] == alone["iterations"] == 20
+    assert abs(r["energy"] - alone["energy"]) <= 1e-9
+    pool.close()

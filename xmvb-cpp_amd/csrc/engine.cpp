@@ -626,6 +626,7 @@ void optimize_slots(vbe_pool* p, vbe_problem* prob, const std::vector<int>& slot
         for (size_t j = 0; j < which.size(); ++j)
             E[j] = status[j] == VBE_OK ? lane.h_energy[2 * j + 1] : std::numeric_limits<double>::quiet_NaN();
     };
+    hooks.energy_only_trials = prob != nullptr;
     hooks.complete = [&](int, const double* C, double* G) {
         if (!prob) throw StatusError(VBE_ERR_ARGUMENT, "energy-only trials need a single problem");
         eval_problem(prob, C, lane.h_desc[0].support, true, G, false);

@@ -515,7 +515,7 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
     if (n_act > 16) throw std::invalid_argument("at most 16 orbitals per optimisation");
     const int np = (int)problems.size();
     if (np == 0) return;
-    const bool single = np == 1;
+    const bool single = np == 1 && h.energy_only_trials;
     hipStream_t st = h.stream;
     constexpr double kEps = std::numeric_limits<double>::epsilon();
     const double zero_step = 128.0 * kEps;

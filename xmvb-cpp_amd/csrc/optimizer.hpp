@@ -37,6 +37,7 @@ struct OptimizerHooks {
                        std::vector<double>& E)>
         evaluate;
     std::function<void(int which, const double* C, double* G)> complete;
+    bool energy_only_trials = false;  // `complete` is available (a single problem, not a pool)
 };
 
 void optimizer_defaults(vbe_optimizer_options* o);
