@@ -12,6 +12,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -288,6 +289,15 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     T.ng = T.np * (T.np + 1) / 2;
     T.n_det = nd;
     T.n_str = ns;
+    {
+        // a multiplier coprime with the pair count near pairs / golden ratio: lanes of one wavefront
+        // then take pairs of different determinants, which spreads the backward kernels' LDS atomic
+        // targets (consecutive pairs share their ket determinant and hit the same rows)
+        const long long pairs = (long long)nd * (nd + 1) / 2;
+        long long m = std::max<long long>(1, (long long)(0.6180339887498949 * (double)pairs)) | 1;
+        while (std::gcd(m, pairs) != 1) m += 2;
+        T.pair_mul = pairs > 1 && pairs < (1LL << 32) ? m : 1;  // the kernels form t * pair_mul in 64 bits
+    }
     auto* amask = A.alloc<uint64_t>(t.n_alpha_strings);
     auto* bmask = A.alloc<uint64_t>(t.n_beta_strings);
     auto* det_a = A.alloc<int>(nd);

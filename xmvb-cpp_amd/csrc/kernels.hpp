@@ -68,6 +68,7 @@ struct TopoDesc {
     const int* ts;
     const double* tc;
     const long long* soff;  // structure -> determinants CSR
+    long long pair_mul;     // backward pair order: pair t is taken as (t * pair_mul) mod pairs (coprime)
     const int* sdet;
     const double* scoef;
 };

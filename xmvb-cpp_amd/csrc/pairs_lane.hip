@@ -499,7 +499,8 @@ __global__ void __launch_bounds__(256) LANE_WAVES
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_pairs;
          t += (long long)gridDim.x * blockDim.x) {
         int d, e;
-        unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
+        unpack_pair((long long)(((unsigned long long)t * (unsigned long long)topo.pair_mul) % (unsigned long long)n_pairs), d,
+                    e);  // d: left (bra) determinant, e: right (ket)
         const double w = sd.dcoef[d] * sd.dcoef[e] * (d == e ? 1.0 : 2.0);
         if (w == 0.0) continue;
         LaneBlock<M0> b0;
@@ -614,7 +615,9 @@ __global__ void __launch_bounds__(256) LANE_WAVES
         const long long t = tb + lane;
         int d = 0, e = 0;
         bool live = t < n_pairs;
-        if (live) unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
+        if (live)
+            unpack_pair((long long)(((unsigned long long)t * (unsigned long long)topo.pair_mul) % (unsigned long long)n_pairs),
+                        d, e);  // d: left (bra) determinant, e: right (ket)
         const double w = live ? sd.dcoef[d] * sd.dcoef[e] * (d == e ? 1.0 : 2.0) : 0.0;
         live = live && w != 0.0;
         LaneBlock<M0> b0;
