@@ -91,136 +91,128 @@ def cpu_baseline(cfg, sp, topo, gpu_energy, gpu_grad, seconds: float):
     return base, parity
 
 
-def serve(vb, ctx, topo, cfg, rate: float, seconds: float, replicas: int, max_batch: int, seed: int = 7,
+def serve(vb, ctx, scans, rate: float, seconds: float, max_batch: int, replicas: int = 1, seed: int = 7,
           converge_rate: float = 0.0, converge_iterations: int = 150):
-    """Poisson arrivals over a 64-point scan, continuously batched onto the pool's two lanes. With
-    converge_rate > 0 a second Poisson stream of full-convergence requests (P-LBFGS from the scan
-    point's start orbitals) runs beside it on its own pool and host thread: requests that have
-    arrived are converged together in lockstep, one batch at a time."""
-    import threading
-
+    """Poisson arrivals of E+grad requests over the points of one or more bond-stretch scans,
+    served by the engine's request loop (xmvb_amd.serving.Server: continuous batching per pool,
+    admission under the context's HBM budget). scans: list of (name, topology, [SyntheticProblem
+    per point]); each point is registered `replicas` times so that requests for one point can run
+    in both of a pool's launches. With converge_rate > 0 a second Poisson stream of
+    full-convergence requests (P-LBFGS from the point's start orbitals) goes to a
+    ConvergenceWorker on its own host thread, sharing the device and the budget."""
     import numpy as np
 
-    points = 64
-    rs = [0.5 + 3.5 * p / (points - 1) for p in range(points)]
-    pool = vb.Pool(ctx, topo, capacity=points * replicas)
-    orb, free = [], []
-    for p, r in enumerate(rs):
-        sp = vb.synthetic_problem(1000 + p, cfg["n_bf"], cfg["n_act"], cfg["n_aux"], scan_r=r)
-        for k in range(replicas):
-            pool.load_synthetic(p * replicas + k, sp)
-        orb.append(sp.C)
-        free.append(list(range(p * replicas, (p + 1) * replicas)))
-    # reference energy of every scan point (one batched launch, slot p*replicas)
-    ref_slots = [p * replicas for p in range(points)]
-    t = pool.submit(ref_slots, [orb[p] for p in range(points)], gradient=False)
-    e_ref, _ = pool.wait(t)
+    from xmvb_amd import serving
 
-    conv = None
+    def loader(sp):
+        return lambda: (sp.n_bf, sp.n_aux, sp.S, sp.h, sp.B)
+
+    n_points = sum(len(sps) for _, _, sps in scans)
+    server = serving.Server(ctx, capacity=max(len(sps) for _, _, sps in scans) * replicas, max_batch=max_batch)
+    for si, (_, topo, sps) in enumerate(scans):
+        for p, sp in enumerate(sps):
+            for k in range(replicas):
+                server.register((si, p, k), topo, loader(sp))
+    # direct reference energies: every point of a scan in one batched launch of a pool of its own
+    e_ref = {}
+    for si, (_, topo, sps) in enumerate(scans):
+        ref_pool = vb.Pool(ctx, topo, capacity=len(sps))
+        for p, sp in enumerate(sps):
+            ref_pool.load_synthetic(p, sp)
+        e, _ = ref_pool.evaluate(list(range(len(sps))), [sp.C for sp in sps], gradient=False)
+        ref_pool.close()
+        for p in range(len(sps)):
+            e_ref[(si, p)] = e[p]
+    # admission before the timed window: one request per registered problem
+    for key in list(server.res.problems):
+        si, p, _k = key
+        server.submit(key, scans[si][2][p].C)
+    server.drain()
+    loads_warm = server.loads
+
+    worker = None
     if converge_rate > 0:
-        cpool = vb.Pool(ctx, topo, capacity=points)
-        for p, r in enumerate(rs):
-            cpool.load_synthetic(p, vb.synthetic_problem(1000 + p, cfg["n_bf"], cfg["n_act"], cfg["n_aux"], scan_r=r))
+        worker = serving.ConvergenceWorker(ctx, capacity=max(len(sps) for _, _, sps in scans),
+                                           max_iterations=converge_iterations)
+        for si, (_, topo, sps) in enumerate(scans):
+            for p, sp in enumerate(sps):
+                worker.register((si, p), topo, loader(sp))
         crng = np.random.default_rng(seed + 1)
         n_conv = max(1, int(converge_rate * seconds))
         c_arrivals = np.cumsum(crng.exponential(1.0 / converge_rate, n_conv))
-        c_point = crng.integers(0, points, n_conv)
-        conv = dict(latency=[], iterations=[], batches=[], terminations={})
-
-    def converge_worker(t0):
-        try:
-            converge_loop(t0)
-        except BaseException as exc:  # re-raised by the serving thread after join
-            conv["error"] = exc
-
-    def converge_loop(t0):
-        pending, nxt_c = [], 0
-        while nxt_c < n_conv or pending:
-            now = time.perf_counter() - t0
-            while nxt_c < n_conv and c_arrivals[nxt_c] <= now:
-                pending.append(nxt_c)
-                nxt_c += 1
-            if not pending:
-                time.sleep(max(0.0, min(c_arrivals[nxt_c] - now, 5e-3)))
-                continue
-            batch, seen, rest = [], set(), []
-            for q in pending:  # one request per scan point per batch
-                (rest if c_point[q] in seen else batch).append(q)
-                seen.add(c_point[q])
-            res = cpool.optimize([int(c_point[q]) for q in batch], [orb[c_point[q]] for q in batch],
-                                 max_iterations=converge_iterations)
-            fin = time.perf_counter() - t0
-            for q, r in zip(batch, res):
-                conv["latency"].append((fin - c_arrivals[q]) * 1e3)
-                conv["iterations"].append(r["iterations"])
-                conv["terminations"][r["termination"]] = conv["terminations"].get(r["termination"], 0) + 1
-            conv["batches"].append(len(batch))
-            pending = rest
+        c_scan = crng.integers(0, len(scans), n_conv)
+        c_pick = [(int(si), int(crng.integers(len(scans[si][2])))) for si in c_scan]
 
     rng = np.random.default_rng(seed)
     n_req = max(1, int(rate * seconds))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
-    point_of = rng.integers(0, points, n_req)
-    latency, energies_err, batches = [], 0.0, []
-    queue, inflight = [], {}  # inflight: ticket -> (requests, slots)
-    nxt = 0
+    scan_of = rng.integers(0, len(scans), n_req)
+    pick = [(int(si), int(rng.integers(len(scans[si][2])))) for si in scan_of]
+    reqs, launches0 = [], server.launches
+    nxt, nxt_c, done = 0, 0, 0
     t0 = time.perf_counter()
-    worker = None
-    if conv is not None:
-        worker = threading.Thread(target=converge_worker, args=(t0,))
+    if worker is not None:
         worker.start()
-    done = 0
     while done < n_req:
         now = time.perf_counter() - t0
         while nxt < n_req and arrivals[nxt] <= now:
-            queue.append(nxt)
+            si, p = pick[nxt]
+            k = min(range(replicas), key=lambda j: server.res.problems[(si, p, j)].pending)
+            reqs.append(server.submit((si, p, k), scans[si][2][p].C, arrival=t0 + arrivals[nxt]))
             nxt += 1
-        for ticket in list(inflight):
-            if pool.poll(ticket):
-                e, _grads = pool.wait(ticket)
-                fin = time.perf_counter() - t0
-                reqs, slots = inflight.pop(ticket)
-                for q, s, ei in zip(reqs, slots, e):
-                    latency.append((fin - arrivals[q]) * 1e3)
-                    energies_err = max(energies_err, abs(ei - e_ref[point_of[q]]))
-                    free[point_of[q]].append(s)
-                done += len(reqs)
-        if queue and len(inflight) < 2:
-            reqs, slots, rest = [], [], []
-            for q in queue:
-                p = point_of[q]
-                if len(reqs) < max_batch and free[p]:
-                    reqs.append(q)
-                    slots.append(free[p].pop())
-                else:
-                    rest.append(q)
-            if reqs:
-                ticket = pool.submit(slots, [orb[point_of[q]] for q in reqs], gradient=True)
-                inflight[ticket] = (reqs, slots)
-                batches.append(len(reqs))
-                queue = rest
-        if not queue and not inflight and nxt < n_req:
+        if worker is not None:
+            while nxt_c < n_conv and c_arrivals[nxt_c] <= now:
+                si, p = c_pick[nxt_c]
+                worker.submit((si, p), scans[si][2][p].C, arrival=t0 + c_arrivals[nxt_c])
+                nxt_c += 1
+        done += len(server.pump())
+        if not server.busy() and nxt < n_req:
             time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 2e-4)))
     makespan = time.perf_counter() - t0
     converge = None
     if worker is not None:
-        worker.join()
-        cpool.close()
-        if "error" in conv:
-            raise conv["error"]
+        while nxt_c < n_conv:  # the rest of the convergence stream, at its own arrival times
+            now = time.perf_counter() - t0
+            if c_arrivals[nxt_c] <= now:
+                si, p = c_pick[nxt_c]
+                worker.submit((si, p), scans[si][2][p].C, arrival=t0 + c_arrivals[nxt_c])
+                nxt_c += 1
+            else:
+                time.sleep(min(c_arrivals[nxt_c] - now, 5e-3))
+        worker.finish()
+        cl = [r.latency * 1e3 for r in worker.completed]
+        its = [r.result["iterations"] for r in worker.completed]
+        term = {}
+        for r in worker.completed:
+            term[r.result["termination"]] = term.get(r.result["termination"], 0) + 1
         converge = dict(requests=n_conv, offered_rate=converge_rate, max_iterations=converge_iterations,
-                        latency_ms=dict(p50=percentile(conv["latency"], 0.5), p99=percentile(conv["latency"], 0.99),
-                                        max=max(conv["latency"])),
-                        mean_iterations=float(np.mean(conv["iterations"])), batches=len(conv["batches"]),
-                        mean_batch=float(np.mean(conv["batches"])), terminations=conv["terminations"],
+                        latency_ms=dict(p50=percentile(cl, 0.5), p99=percentile(cl, 0.99), max=max(cl)),
+                        mean_iterations=float(np.mean(its)), batches=len(worker.rounds),
+                        mean_batch=float(np.mean(worker.rounds)), terminations=term,
                         makespan_s=time.perf_counter() - t0)
-    pool.close()
-    return dict(requests=n_req, offered_rate=rate, throughput=n_req / makespan, seconds=makespan,
-                latency_ms=dict(p50=percentile(latency, 0.5), p99=percentile(latency, 0.99),
-                                max=max(latency)),
-                mean_batch=float(np.mean(batches)), launches=len(batches), max_abs_dE_vs_direct=energies_err,
-                scan=f"{points} points r in [0.5, 4.0], {replicas} resident replicas each",
-                io="host orbitals in, host gradients out (PCIe included)", convergence_requests=converge)
+        worker.close()
+    lat = [r.latency * 1e3 for r in reqs]
+    err = max(abs(r.energy - e_ref[(r.key[0], r.key[1])]) for r in reqs)
+    per_scan = {}
+    for si, (name, topo, sps) in enumerate(scans):
+        ls = [r.latency * 1e3 for r in reqs if r.key[0] == si]
+        if ls:
+            per_scan[name] = dict(requests=len(ls), n_act=topo.n_act, n_bf=sps[0].n_bf,
+                                  p50=percentile(ls, 0.5), p99=percentile(ls, 0.99))
+    used, budget = ctx.memory()
+    sizes = server.batch_sizes[len(server.batch_sizes) - (server.launches - launches0):]
+    out = dict(requests=n_req, offered_rate=rate, throughput=n_req / makespan, seconds=makespan,
+               latency_ms=dict(p50=percentile(lat, 0.5), p99=percentile(lat, 0.99), max=max(lat)),
+               mean_batch=float(np.mean(sizes)), launches=server.launches - launches0,
+               max_abs_dE_vs_direct=float(err),
+               scan=f"{len(scans)} scan(s) x {len(scans[0][2])} points r in [0.5, 4.0], {replicas} resident "
+                    f"replica(s) per point ({n_points * replicas} problems)",
+               admission=dict(budget_gib=budget / 2**30, resident_gib=used / 2**30, loads=server.loads,
+                              loads_before_timing=loads_warm, evictions=server.evictions),
+               per_scan=per_scan, io="host orbitals in, host gradients out (PCIe included)",
+               convergence_requests=converge)
+    server.close()
+    return out
 
 
 def main():
@@ -238,6 +230,8 @@ def main():
                     help="Poisson rate of full-convergence requests mixed into the served leg (0: none)")
     ap.add_argument("--serve-converge-iterations", type=int, default=150,
                     help="iteration cap of a served convergence request")
+    ap.add_argument("--serve-mixed", type=int, default=1,
+                    help="also serve the mixed-size scans (one per n_act in {4,...,12}) under a 200 GB budget")
     ap.add_argument("--mixed-steps", type=int, default=30, help="timed batched steps with FP32 RI factors (0: skip)")
     ap.add_argument("--converge", type=int, default=3, help="full-convergence requests timed (0: skip)")
     ap.add_argument("--converge-iterations", type=int, default=500, help="iteration cap per convergence request")
@@ -405,11 +399,37 @@ def main():
 
     served = None
     if args.serve_seconds > 0:
-        served = serve(vb, ctx, topo, cfg, args.serve_rate, args.serve_seconds, replicas=4, max_batch=K)
+        rs = [0.5 + 3.5 * p / 63 for p in range(64)]
+        scan = [vb.synthetic_problem(1000 + p, cfg["n_bf"], cfg["n_act"], cfg["n_aux"], scan_r=r)
+                for p, r in enumerate(rs)]
+        medium_scans = [("medium", topo, scan)]
+        served = serve(vb, ctx, medium_scans, args.serve_rate, args.serve_seconds, max_batch=K, replicas=4)
         if args.serve_converge_rate > 0:
             served["with_convergence_requests"] = serve(
-                vb, ctx, topo, cfg, args.serve_rate, args.serve_seconds, replicas=4, max_batch=K,
+                vb, ctx, medium_scans, args.serve_rate, args.serve_seconds, max_batch=K, replicas=4,
                 converge_rate=args.serve_converge_rate, converge_iterations=args.serve_converge_iterations)
+        del scan, medium_scans
+    served_mixed = None
+    if args.serve_seconds > 0 and args.serve_mixed:
+        # the serving form: one 64-point scan per active space n_act in {4,...,12} (n_bf drawn from
+        # [40, 200], n_aux = 3 n_bf, one seed per scan, r scales the off-diagonal blocks), E+grad
+        # requests spread over all scans and sizes, full-convergence requests beside them, all under
+        # a 200 GB HBM budget
+        from concurrent.futures import ThreadPoolExecutor
+        srng = np.random.default_rng(2000)
+        rs = [0.5 + 3.5 * p / 63 for p in range(64)]
+        specs = [(na, int(srng.integers(40, 201))) for na in (4, 6, 8, 10, 12)]
+        with ThreadPoolExecutor(max_workers=16) as ex:
+            mixed_scans = [(f"n_act={na} n_bf={nb}", vb.Topology.build("COV", na, na, 1),
+                            list(ex.map(lambda r, na=na, nb=nb: vb.synthetic_problem(30000 + na, nb, na, 3 * nb,
+                                                                                     scan_r=r), rs)))
+                           for na, nb in specs]
+        ctx_s = vb.Context(0, hbm_budget_bytes=200 * 10**9)
+        served_mixed = serve(vb, ctx_s, mixed_scans, args.serve_rate, args.serve_seconds, max_batch=64,
+                             replicas=2, converge_rate=args.serve_converge_rate,
+                             converge_iterations=args.serve_converge_iterations)
+        ctx_s.close()
+        del mixed_scans
 
     converge = None
     if args.converge > 0:
@@ -540,7 +560,8 @@ def main():
                latency_source=("served leg: request completion minus Poisson arrival" if served
                                else "batched step"),
                batch_step_ms=dict(p50=percentile(lat, 0.5), p99=percentile(lat, 0.99)),
-               plain_fp64=plain, speedup_vs_plain=value / plain["value"], served=served, fp32_ri=mixed,
+               plain_fp64=plain, speedup_vs_plain=value / plain["value"], served=served,
+               served_mixed_sizes=served_mixed, fp32_ri=mixed,
                converge=converge, mixed_batch=mixed_batch,
                batch_consistency=batch_consistency, phase_ms={k: round(v, 5) for k, v in mean.items()},
                energy=energy, roofline=roofline, roofline_other_kernels=others)

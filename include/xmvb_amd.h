@@ -269,6 +269,13 @@ void vbe_pool_destroy(vbe_pool* pool);
 int vbe_pool_load(vbe_pool* pool, int32_t slot, int32_t n_bf, int32_t n_aux, const double* S_ao, const double* h_ao,
                   const double* B_packed, double e_nuc);
 int vbe_pool_device_bytes(const vbe_pool* pool, uint64_t* bytes);
+/* Release the device storage of one slot (its budget returns to the context); the slot can be
+ * loaded again later. Rejected (VBE_ERR_ARGUMENT) while a pending submission holds the slot.
+ * With vbe_context_memory this is what an admission policy needs to keep the resident problems
+ * under the context's HBM budget. */
+int vbe_pool_unload(vbe_pool* pool, int32_t slot);
+/* Bytes of device storage currently charged to the context, and its budget. */
+int vbe_context_memory(const vbe_context* ctx, uint64_t* in_use, uint64_t* budget);
 /* Synchronous batched evaluation with device-resident orbitals / gradients; energies[count] on the host. */
 int vbe_pool_eval_device(vbe_pool* pool, int32_t count, const int32_t* slots, const double* d_C,
                          int32_t want_gradient, double* energies, double* d_grad);

@@ -1,0 +1,174 @@
+"""The serving loop (xmvb_amd.serving) on the device: continuous batching over pools of different
+active spaces, admission under an HBM budget with eviction, and full-convergence requests on a
+worker thread.
+
+Each served result must equal the same problem evaluated alone (|dE| <= 1e-10, gradient 1e-9:
+the per-problem arithmetic is the same, only the block partition of the adjoint sums can change
+with the launch), whatever it was batched with and however often its problem was evicted and
+loaded again; convergence requests must follow the iterates of the same problem optimised alone."""
+import numpy as np
+import pytest
+
+import xmvb_amd as vb
+from xmvb_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+def _loader(sp):
+    return lambda: (sp.n_bf, sp.n_aux, sp.S, sp.h, sp.B)
+
+
+def _alone(ctx, topo, sp, gradient=True):
+    pool = vb.Pool(ctx, topo, capacity=1)
+    pool.load_synthetic(0, sp)
+    e, g = pool.evaluate([0], [sp.C], gradient=gradient)
+    pool.close()
+    return e[0], (g[0] if g is not None else None)
+
+
+def test_pool_unload_returns_budget_and_reloads():
+    ctx = vb.Context(0)
+    topo = vb.Topology.build("COV", 6, 6, 1)
+    sp = vb.synthetic_problem(31, 36, 6, 72)
+    pool = vb.Pool(ctx, topo, capacity=2)
+    base = ctx.memory()[0]
+    pool.load_synthetic(1, sp)
+    held = ctx.memory()[0] - base
+    assert held > 8 * sp.B.size  # B and the slot's workspace are charged
+    print("case unload/reload slot", flush=True)
+    e1, g1 = pool.evaluate([1], [sp.C])
+    after_eval = ctx.memory()[0]  # launches may size the pool's own per-launch tables
+    pool.unload(1)
+    assert ctx.memory()[0] == after_eval - held
+    with pytest.raises(vb.VbeError):
+        pool.evaluate([1], [sp.C])
+    pool.load_synthetic(1, sp)
+    e2, g2 = pool.evaluate([1], [sp.C])
+    # the adjoint sums use LDS atomics, so the gradient may differ from run to run in the last bits
+    assert abs(e1[0] - e2[0]) <= 1e-12
+    assert float(np.abs(g1[0] - g2[0]).max()) <= 1e-12
+    pool.close()
+    ctx.close()
+
+
+def test_budget_refusal_leaves_slot_unloaded():
+    topo = vb.Topology.build("COV", 4, 4, 1)
+    big = vb.synthetic_problem(41, 40, 4, 120)
+    probe = vb.Context(0)
+    pool = vb.Pool(probe, topo, capacity=1)
+    before = probe.memory()[0]
+    pool.load_synthetic(0, big)
+    per_slot = probe.memory()[0] - before
+    topo_bytes = before
+    e_ref, _ = pool.evaluate([0], [big.C])
+    pool.close()
+    probe.close()
+    ctx = vb.Context(0, hbm_budget_bytes=topo_bytes + per_slot + per_slot // 2)
+    pool = vb.Pool(ctx, topo, capacity=2)
+    pool.load_synthetic(0, big)
+    print("case budget refusal", flush=True)
+    with pytest.raises(vb.VbeError) as err:
+        pool.load_synthetic(1, big)
+    assert err.value.status == 6
+    used, budget = ctx.memory()
+    assert used <= budget
+    with pytest.raises(vb.VbeError):
+        pool.evaluate([1], [big.C])  # never loaded: rejected, not evaluated with stale pointers
+    e, _ = pool.evaluate([0], [big.C])
+    assert abs(e[0] - e_ref[0]) <= 1e-10
+    pool.close()
+    ctx.close()
+
+
+def test_served_mix_matches_alone():
+    ctx = vb.Context(0)
+    rng = np.random.default_rng(5)
+    server = serving.Server(ctx, capacity=16, max_batch=8)
+    probs = {}
+    for n_act in (4, 6, 8):
+        topo = vb.Topology.build("COV", n_act, n_act, 1)
+        for k in range(3):
+            n_bf = int(rng.integers(20, 48))
+            sp = vb.synthetic_problem(500 + 10 * n_act + k, n_bf, n_act, 2 * n_bf)
+            probs[(n_act, k)] = (topo, sp)
+            server.register((n_act, k), topo, _loader(sp))
+    keys = list(probs)
+    print("case served mix of 3 active spaces, 9 problems, 120 requests", flush=True)
+    reqs = []
+    for i in range(120):
+        key = keys[int(rng.integers(len(keys)))]
+        reqs.append(server.submit(key, probs[key][1].C, gradient=bool(i % 7)))
+        if i % 5 == 0:
+            server.pump()
+    server.drain()
+    assert server.launches < len(reqs)  # requests were batched
+    ref = {key: _alone(ctx, *probs[key]) for key in keys}
+    for r in reqs:
+        e0, g0 = ref[r.key]
+        assert abs(r.energy - e0) <= 1e-10, (r.key, r.energy, e0)
+        if r.gradient:
+            assert float(np.abs(r.grad - g0).max()) <= 1e-9
+        else:
+            assert r.grad is None
+    server.close()
+    ctx.close()
+
+
+def test_admission_under_budget_evicts_and_reloads():
+    topo = vb.Topology.build("COV", 6, 6, 1)
+    sps = [vb.synthetic_problem(700 + k, 40, 6, 120) for k in range(5)]
+    probe = vb.Context(0)
+    pool = vb.Pool(probe, topo, capacity=1)
+    before = probe.memory()[0]
+    pool.load_synthetic(0, sps[0])
+    per_slot = probe.memory()[0] - before
+    pool.close()
+    probe.close()
+    # room for the pool's topology tables and two resident problems, not three
+    ctx = vb.Context(0, hbm_budget_bytes=before + 2 * per_slot + per_slot // 2)
+    server = serving.Server(ctx, capacity=8, max_batch=4)
+    for k, sp in enumerate(sps):
+        server.register(k, topo, _loader(sp))
+    rng = np.random.default_rng(9)
+    print("case admission under a two-problem budget, 5 problems, 60 requests", flush=True)
+    reqs = []
+    for i in range(60):
+        k = int(rng.integers(5))
+        reqs.append(server.submit(k, sps[k].C))
+        server.pump()
+        used, budget = ctx.memory()
+        assert used <= budget
+        assert server.res.resident() <= 2
+    server.drain()
+    assert server.evictions > 0 and server.loads > 5
+    ref_ctx = vb.Context(0)
+    ref = [_alone(ref_ctx, topo, sp) for sp in sps]
+    for r in reqs:
+        assert abs(r.energy - ref[r.key][0]) <= 1e-10
+        assert float(np.abs(r.grad - ref[r.key][1]).max()) <= 1e-9
+    ref_ctx.close()
+    server.close()
+    ctx.close()
+
+
+def test_convergence_worker_matches_single_problem_optimisation():
+    ctx = vb.Context(0)
+    topo = vb.Topology.build("COV", 4, 4, 1)
+    sps = [vb.synthetic_problem(900 + k, 24, 4, 48) for k in range(3)]
+    worker = serving.ConvergenceWorker(ctx, capacity=4, max_iterations=40)
+    for k, sp in enumerate(sps):
+        worker.register(k, topo, _loader(sp))
+    worker.start()
+    print("case convergence worker, 3 problems", flush=True)
+    reqs = [worker.submit(k, sps[k].C) for k in range(3)]
+    worker.finish()
+    for k, r in enumerate(reqs):
+        prob = vb.Problem(ctx, sps[k].n_bf, sps[k].n_aux, topo, sps[k].S, sps[k].h, sps[k].B)
+        alone = prob.optimize(sps[k].C, max_iterations=40)
+        assert r.result["iterations"] == alone["iterations"]
+        assert r.result["termination"] == alone["termination"]
+        assert abs(r.result["energy"] - alone["energy"]) <= 1e-9
+        prob.close()
+    worker.close()
+    ctx.close()

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -61,6 +62,9 @@ struct vbe_context {
     double dav_energy_tol = 1.0e-7, dav_residual_tol = 1.0e-3, dav_stationarity_tol = 1.0e-7;
     bool dav_reuse_roots = true;
     std::string error;
+    // guards in_use and error: pools of one context may be driven from several host threads
+    // (for example a serving loop beside a convergence worker), one thread per pool
+    std::mutex mu;
 };
 
 namespace {
@@ -73,19 +77,30 @@ struct DeviceArena {
     template <typename T>
     T* alloc(size_t count) {
         const size_t nbytes = std::max<size_t>(count, 1) * sizeof(T);
-        if (ctx->budget && ctx->in_use + nbytes > ctx->budget)
-            throw StatusError(VBE_ERR_MEMORY, "HBM budget exceeded while allocating problem storage");
+        {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            if (ctx->budget && ctx->in_use + nbytes > ctx->budget)
+                throw StatusError(VBE_ERR_MEMORY, "HBM budget exceeded while allocating problem storage");
+            ctx->in_use += nbytes;  // reserved before the allocation, returned if it fails
+        }
         void* p = nullptr;
-        check(hipMalloc(&p, nbytes), "hipMalloc");
+        const hipError_t e = hipMalloc(&p, nbytes);
+        if (e != hipSuccess) {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            ctx->in_use -= std::min(ctx->in_use, (uint64_t)nbytes);
+        }
+        check(e, "hipMalloc");
         ptrs.push_back(p);
         bytes += nbytes;
-        ctx->in_use += nbytes;
         return static_cast<T*>(p);
     }
     void release() {
         for (void* p : ptrs) (void)hipFree(p);
         ptrs.clear();
-        if (ctx) ctx->in_use -= std::min(ctx->in_use, bytes);
+        if (ctx) {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            ctx->in_use -= std::min(ctx->in_use, bytes);
+        }
         bytes = 0;
     }
     ~DeviceArena() { release(); }
@@ -168,7 +183,10 @@ struct vbe_problem {
 namespace {
 
 int fail(vbe_context* ctx, int status, const std::string& what) {
-    if (ctx) ctx->error = what;
+    if (ctx) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->error = what;
+    }
     xmvb_amd::set_global_error(what);
     return status;
 }
@@ -349,11 +367,18 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
     const size_t n = T.n, np = T.np, nd = T.n_det, ns = T.n_str;
     const size_t nbp = (size_t)n_bf * (n_bf + 1) / 2;
     const bool realloc = !s.arena || s.n_bf != n_bf || s.n_aux != n_aux;
+    // the slot is unusable until the upload below has completed; a failed (re)allocation, e.g.
+    // over the HBM budget, leaves it unloaded with no storage rather than with stale pointers
+    s.loaded = false;
     if (realloc) {
         s.arena.reset();
-        s.arena = std::make_unique<DeviceArena>();
-        s.arena->ctx = p->ctx;
-        auto& A = *s.arena;
+        s.desc = SlotDesc{};
+        s.C_buf = s.grad_buf = nullptr;
+        s.support_buf = nullptr;
+        s.n_bf = s.n_aux = 0;
+        auto arena = std::make_unique<DeviceArena>();
+        arena->ctx = p->ctx;
+        auto& A = *arena;
         SlotDesc d{};
         d.S = A.alloc<double>((size_t)n_bf * n_bf);
         d.h = A.alloc<double>((size_t)n_bf * n_bf);
@@ -385,9 +410,13 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
         d.gcn = A.alloc<double>(n * n_bf);
         d.defer = A.alloc<int>((size_t)pair_count(p));
         d.singular = A.alloc<int>((size_t)pair_count(p));
-        s.C_buf = A.alloc<double>(n * n_bf);
-        s.grad_buf = A.alloc<double>(n * n_bf);
-        s.support_buf = A.alloc<uint8_t>(n * n_bf);
+        double* C_buf = A.alloc<double>(n * n_bf);
+        double* grad_buf = A.alloc<double>(n * n_bf);
+        uint8_t* support_buf = A.alloc<uint8_t>(n * n_bf);
+        s.arena = std::move(arena);
+        s.C_buf = C_buf;
+        s.grad_buf = grad_buf;
+        s.support_buf = support_buf;
         s.desc = d;
     }
     s.n_bf = n_bf;
@@ -715,7 +744,13 @@ int vbe_set_pair_condition_limit(vbe_context* ctx, double limit) {
 
 int vbe_last_error(const vbe_context* ctx, char* buf, size_t len) {
     if (!buf || len == 0) return VBE_ERR_ARGUMENT;
-    const std::string e = ctx ? ctx->error : std::string(vbe_global_error());
+    std::string e;
+    if (ctx) {
+        std::lock_guard<std::mutex> g(const_cast<vbe_context*>(ctx)->mu);
+        e = ctx->error;
+    } else {
+        e = vbe_global_error();
+    }
     const size_t n = std::min(len - 1, e.size());
     std::memcpy(buf, e.data(), n);
     buf[n] = '\0';
@@ -742,6 +777,38 @@ int vbe_pool_load(vbe_pool* pool, int32_t slot, int32_t n_bf, int32_t n_aux, con
         load_slot(pool, slot, n_bf, n_aux, S_ao, h_ao, B_packed, e_nuc);
         return VBE_OK;
     });
+}
+
+int vbe_pool_unload(vbe_pool* pool, int32_t slot) {
+    return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
+        if (!pool) throw StatusError(VBE_ERR_ARGUMENT, "null pool");
+        if (slot < 0 || slot >= pool->capacity) throw StatusError(VBE_ERR_ARGUMENT, "slot index out of range");
+        for (auto& l : pool->lanes)
+            if (l.busy && std::find(l.slots.begin(), l.slots.end(), slot) != l.slots.end())
+                throw StatusError(VBE_ERR_ARGUMENT, "slot is in use by a pending submission");
+        check(hipSetDevice(pool->ctx->device), "hipSetDevice");
+        // launches that read the slot were ordered on the lanes' streams and have completed
+        // (no pending submission holds it); a synchronous evaluation returns only when done
+        Slot& s = pool->slots[slot];
+        s.arena.reset();
+        s.desc = SlotDesc{};
+        s.C_buf = s.grad_buf = nullptr;
+        s.support_buf = nullptr;
+        s.S_host.clear();
+        s.S_host.shrink_to_fit();
+        s.n_bf = s.n_aux = 0;
+        s.has_root = false;
+        s.loaded = false;
+        return VBE_OK;
+    });
+}
+
+int vbe_context_memory(const vbe_context* ctx, uint64_t* in_use, uint64_t* budget) {
+    if (!ctx || !in_use || !budget) return VBE_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> g(const_cast<vbe_context*>(ctx)->mu);
+    *in_use = ctx->in_use;
+    *budget = ctx->budget;
+    return VBE_OK;
 }
 
 int vbe_pool_device_bytes(const vbe_pool* pool, uint64_t* bytes) {

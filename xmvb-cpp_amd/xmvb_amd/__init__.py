@@ -42,7 +42,8 @@ EXPORTED_SYMBOLS = (
     "vbe_context_create", "vbe_context_destroy", "vbe_last_error", "vbe_global_error", "vbe_problem_upload",
     "vbe_problem_release", "vbe_problem_device_bytes", "vbe_eval", "vbe_eval_device", "vbe_eval_forward",
     "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
-    "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_eval_device",
+    "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_unload",
+    "vbe_context_memory", "vbe_pool_eval_device",
     "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit", "vbe_set_ri_precision",
     "vbe_optimizer_defaults", "vbe_optimize", "vbe_pool_optimize", "vbe_set_eigensolver",
     "vbe_gaussian_basis_size", "vbe_ao_integrals", "vbe_pivoted_cholesky",
@@ -145,6 +146,8 @@ def library() -> C.CDLL:
         lib.vbe_pool_destroy.restype = None
         lib.vbe_pool_load.argtypes = [P, I32, I32, I32, P, P, P, D]
         lib.vbe_pool_device_bytes.argtypes = [P, C.POINTER(U64)]
+        lib.vbe_pool_unload.argtypes = [P, I32]
+        lib.vbe_context_memory.argtypes = [P, C.POINTER(U64), C.POINTER(U64)]
         lib.vbe_pool_eval_device.argtypes = [P, I32, P, P, I32, P, P]
         lib.vbe_pool_submit.argtypes = [P, I32, P, P, I32, C.POINTER(I32)]
         lib.vbe_pool_wait.argtypes = [P, I32, P, P]
@@ -352,6 +355,12 @@ class Context:
         if rc != 0:
             raise VbeError(rc, self.error())
 
+    def memory(self) -> tuple:
+        """(bytes of device storage charged to this context, its HBM budget)."""
+        used, budget = C.c_uint64(), C.c_uint64()
+        self.check(self._lib.vbe_context_memory(self._h, C.byref(used), C.byref(budget)))
+        return used.value, budget.value
+
     def set_pair_condition_limit(self, limit: float):
         """Pairs whose spin blocks have max|X| max|X^-1| >= limit (0..1e9) take the
         complete-pivoting path; 0 sends every pair there."""
@@ -518,7 +527,10 @@ class Pool:
         S = np.asfortranarray(S, dtype=np.float64)
         h = np.asfortranarray(h, dtype=np.float64)
         B = np.ascontiguousarray(B, dtype=np.float64)
-        self.ctx.check(self.ctx._lib.vbe_pool_load(self._h, slot, n_bf, n_aux, _ptr(S), _ptr(h), _ptr(B), e_nuc))
+        rc = self.ctx._lib.vbe_pool_load(self._h, slot, n_bf, n_aux, _ptr(S), _ptr(h), _ptr(B), e_nuc)
+        if rc == 6:  # VBE_ERR_MEMORY: the engine leaves the slot unloaded
+            self.n_bf[slot] = 0
+        self.ctx.check(rc)
         self.n_bf[slot] = n_bf
 
     def load_synthetic(self, slot: int, sp: "SyntheticProblem", e_nuc: float = 0.0):
@@ -608,6 +620,11 @@ class Pool:
         b = C.c_uint64()
         self.ctx._lib.vbe_pool_device_bytes(self._h, C.byref(b))
         return b.value
+
+    def unload(self, slot: int):
+        """Release one slot's device storage (its bytes return to the context budget)."""
+        self.ctx.check(self.ctx._lib.vbe_pool_unload(self._h, int(slot)))
+        self.n_bf[int(slot)] = 0
 
     def close(self):
         if self._h:
