@@ -110,7 +110,7 @@ void launch_davidson(hipStream_t st, const SlotDesc* slots, int count, const Top
 
 int pair_group(int mmax);
 bool pairs_lane_supported(int n_alpha, int n_beta, bool backward);
-std::size_t pairs_lane_lds_bytes(int n, int ng, bool backward);
+std::size_t pairs_lane_lds_bytes(int n, int ng, bool backward, int m0);
 hipError_t configure_pairs_lane();
 std::size_t pairs_singular_lds_bytes(int n, int ng, bool backward);
 hipError_t configure_pairs_singular();

@@ -296,13 +296,19 @@ typedef double lane_double4 __attribute__((ext_vector_type(4)));
 // packed index), every 16x16 tile is updated by four v_mfma_f64_16x16x4 steps (A[i][k] in lane
 // i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i%4), element i/4) into accumulators
 // that stay in registers for the whole kernel, and the rows are zeroed again. Called by all lanes.
-constexpr int kMfmaTiles = 4;
-typedef lane_double4 MfmaTiles[kMfmaTiles][kMfmaTiles];
+constexpr int kMfmaTiles = 4;  // tiles per side kept for the whole kernel (np16 <= 64)
+constexpr int kMfmaTilesWide = 5;  // 6-electron blocks: np16 <= 80, tiles flushed every iteration
 
-template <int M0, int M1>
-__device__ void opposite_dg_mfma(double* bufA, double* bufB, int np16, bool live, const LaneBlock<M0>& b0,
+// Only tiles (tp, tq) with tp <= tq are used: the packed adjoint needs M + M^T of the outer-product
+// sum M = sum_pairs u0 u1^T, so a diagonal tile accumulates A_p B_p^T and an off-diagonal one
+// A_p B_q^T + B_p A_q^T (its transpose partner), TN(TN+1)/2 accumulators instead of TN^2.
+template <int TN>
+using MfmaTilesN = lane_double4[TN][TN];
+
+template <int M0, int M1, int TN>
+__device__ __forceinline__ void opposite_dg_mfma(double* bufA, double* bufB, int np16, bool live, const LaneBlock<M0>& b0,
                                  const LaneBlock<M1>& b1, const int (&P1)[LaneBlock<M1>::K][LaneBlock<M1>::K],
-                                 double cw, MfmaTiles& acc) {
+                                 double cw, MfmaTilesN<TN>& acc) {
     const int lane = threadIdx.x & 63;
     const int k4 = lane >> 4, i16 = lane & 15;
     const int tiles = np16 >> 4;
@@ -327,18 +333,21 @@ __device__ void opposite_dg_mfma(double* bufA, double* bufB, int np16, bool live
         for (int kk = 0; kk < 4; ++kk) {
             const double* ra = bufA + (4 * kk + k4) * np16 + i16;
             const double* rb = bufB + (4 * kk + k4) * np16 + i16;
-            double av[kMfmaTiles], bv[kMfmaTiles];
+            double av[TN], bv[TN];
 #pragma unroll
-            for (int t = 0; t < kMfmaTiles; ++t) {
+            for (int t = 0; t < TN; ++t) {
                 av[t] = t < tiles ? ra[16 * t] : 0.0;
                 bv[t] = t < tiles ? rb[16 * t] : 0.0;
             }
 #pragma unroll
-            for (int tp = 0; tp < kMfmaTiles; ++tp)
+            for (int tp = 0; tp < TN; ++tp)
 #pragma unroll
-                for (int tq = 0; tq < kMfmaTiles; ++tq)
-                    if (tp < tiles && tq < tiles)
+                for (int tq = tp; tq < TN; ++tq)
+                    if (tq < tiles) {
                         acc[tp][tq] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[tp], bv[tq], acc[tp][tq], 0, 0, 0);
+                        if (tq != tp)
+                            acc[tp][tq] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[tp], av[tq], acc[tp][tq], 0, 0, 0);
+                    }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -357,30 +366,32 @@ __device__ void opposite_dg_mfma(double* bufA, double* bufB, int np16, bool live
     }
 }
 
-// adds a wavefront's MFMA tiles into the block's dg accumulator
-template <bool FullDg>
-__device__ void flush_mfma_tiles(const MfmaTiles& acc, int np16, int np, double* dg) {
+// adds a wavefront's MFMA tiles (tp <= tq) into the block's dg accumulator and, if asked, clears them
+template <bool FullDg, int TN>
+__device__ __forceinline__ void flush_mfma_tiles(MfmaTilesN<TN>& acc, int np16, int np, double* dg, bool clear) {
     const int lane = threadIdx.x & 63;
     const int k4 = lane >> 4, i16 = lane & 15;
     const int tiles = np16 >> 4;
 #pragma unroll
-    for (int tp = 0; tp < kMfmaTiles; ++tp)
+    for (int tp = 0; tp < TN; ++tp)
 #pragma unroll
-        for (int tq = 0; tq < kMfmaTiles; ++tq) {
-            if (!(tp < tiles && tq < tiles)) continue;
+        for (int tq = tp; tq < TN; ++tq) {
+            if (!(tq < tiles)) continue;
             const int Q = 16 * tq + i16;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int P = 16 * tp + 4 * v + k4;
                 if (P < np && Q < np && acc[tp][tq][v] != 0.0) dg_add<FullDg>(dg, np, P, Q, acc[tp][tq][v]);
             }
+            if (clear) acc[tp][tq] = lane_double4{0.0, 0.0, 0.0, 0.0};
         }
 }
 
 // Optional LDS regions of the backward lane kernel (0 = not used).
 struct LaneBwdLayout {
-    int nc;    // n(n-1)/2: compound dimension of the same-spin W accumulator
-    int np16;  // np rounded up to 16: row length of the MFMA staging rows (16 rows x 2 per wavefront)
+    int nc;       // n(n-1)/2: compound dimension of the same-spin W accumulator (0: no W, direct atomics)
+    int np16;     // np rounded up to 16: row length of the MFMA staging rows (16 rows x 2 per wavefront)
+    int full_dg;  // 1: the kernel adjoint is held as the full np x np matrix in LDS, 0: packed
 };
 
 __device__ __forceinline__ void load_pair_tables(const SlotDesc& sd, int n, int np, double* G2, double* sact,
@@ -606,11 +617,15 @@ __global__ void __launch_bounds__(256) LANE_WAVES
     // wave-uniform trip count: the MFMA update needs every lane of the wavefront
     const long long first = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)));
     const long long stride = (long long)gridDim.x * blockDim.x;
-    MfmaTiles tiles;
+    // 6-electron blocks need np16 = 80 (5 x 5 tiles); their tiles live only for one iteration and
+    // are flushed into dg after it, the rest keep theirs in registers for the whole kernel
+    constexpr int TN = M0 >= 6 ? kMfmaTilesWide : kMfmaTiles;
+    constexpr bool FlushEach = M0 >= 6;
+    MfmaTilesN<TN> tiles;
 #pragma unroll
-    for (int tp = 0; tp < kMfmaTiles; ++tp)
+    for (int tp = 0; tp < TN; ++tp)
 #pragma unroll
-        for (int tq = 0; tq < kMfmaTiles; ++tq) tiles[tp][tq] = lane_double4{0.0, 0.0, 0.0, 0.0};
+        for (int tq = 0; tq < TN; ++tq) tiles[tp][tq] = lane_double4{0.0, 0.0, 0.0, 0.0};
     for (long long tb = first; tb < n_pairs; tb += stride) {
         const long long t = tb + lane;
         int d = 0, e = 0;
@@ -638,11 +653,16 @@ __global__ void __launch_bounds__(256) LANE_WAVES
         auto pidx1 = [&](int r, int l) { return P1[r][l]; };
         const double cw = live ? b0.det * b1.det * w : 0.0;
         if constexpr (Fast) {
-            opposite_dg_mfma<M0, M1>(bufA, bufB, lay.np16, live, b0, b1, P1, cw, tiles);
-            if (live) {
+            if (FlushEach && live) {  // pair state dies before the wide tiles come alive
                 same_spin_w<M0>(W, lay.nc, b0, cw);
                 same_spin_w<M1>(W, lay.nc, b1, cw);
             }
+            opposite_dg_mfma<M0, M1, TN>(bufA, bufB, lay.np16, live, b0, b1, P1, cw, tiles);
+            if (!FlushEach && live) {
+                same_spin_w<M0>(W, lay.nc, b0, cw);
+                same_spin_w<M1>(W, lay.nc, b1, cw);
+            }
+            if (FlushEach) flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, true);
         } else if (live) {
             same_spin_dg<FullDg, M0>(dg, np, b0, pidx0, cw);
             same_spin_dg<FullDg, M1>(dg, np, b1, pidx1, cw);
@@ -661,7 +681,7 @@ __global__ void __launch_bounds__(256) LANE_WAVES
             }
         }
     }
-    if (Fast) flush_mfma_tiles<FullDg>(tiles, lay.np16, np, dg);
+    if (Fast && !FlushEach) flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, false);
     __syncthreads();
     if (Fast) {
         w_to_dg<FullDg>(W, lay.nc, dg, np);
@@ -700,17 +720,22 @@ size_t lane_bwd_f_doubles(int n) { return lane_fwd_doubles(n) + 2 * (size_t)n * 
 
 size_t lane_bwd_g_doubles(int n, int ng, const LaneBwdLayout& lay) {
     const size_t np = (size_t)n * (n + 1) / 2;
-    return (size_t)n * n + (pairs_full_dg(n, ng) ? np * np : (size_t)ng) + (size_t)lay.nc * lay.nc +
+    return (size_t)n * n + (lay.full_dg ? np * np : (size_t)ng) + (size_t)lay.nc * lay.nc +
            (size_t)kLaneWaves * 32 * lay.np16;
 }
 
-// W and the MFMA staging rows when both fit in LDS, else neither (direct atomics)
-LaneBwdLayout lane_bwd_layout(int n, int ng) {
+// W and the MFMA staging rows when both fit in LDS, else neither (direct atomics). The adjoint is
+// held as the full np x np matrix when that fits too, else packed (n = 11, 12 with 6-electron
+// blocks). m0: the larger spin block, which sets how many MFMA tiles the kernel can carry.
+LaneBwdLayout lane_bwd_layout(int n, int ng, int m0) {
     const int np = n * (n + 1) / 2;
-    const LaneBwdLayout fast{n * (n - 1) / 2, (np + 15) / 16 * 16};
-    if (fast.np16 <= 16 * kMfmaTiles && lane_bwd_g_doubles(n, ng, fast) * sizeof(double) <= kLaneLdsLimit)
-        return fast;
-    return {0, 0};
+    const int tn = m0 >= 6 ? kMfmaTilesWide : kMfmaTiles;
+    const int full = pairs_full_dg(n, ng) ? 1 : 0;
+    for (const int f : {full, 0}) {
+        const LaneBwdLayout fast{n * (n - 1) / 2, (np + 15) / 16 * 16, f};
+        if (fast.np16 <= 16 * tn && lane_bwd_g_doubles(n, ng, fast) * sizeof(double) <= kLaneLdsLimit) return fast;
+    }
+    return {0, 0, full};
 }
 
 template <int M0, int M1>
@@ -790,9 +815,9 @@ bool for_lane_shape(int m0, int m1, Fn&& f) {
 
 }  // namespace
 
-size_t pairs_lane_lds_bytes(int n, int ng, bool backward) {
+size_t pairs_lane_lds_bytes(int n, int ng, bool backward, int m0) {
     if (!backward) return lane_fwd_doubles(n) * sizeof(double);
-    return std::max(lane_bwd_f_doubles(n), lane_bwd_g_doubles(n, ng, lane_bwd_layout(n, ng))) * sizeof(double);
+    return std::max(lane_bwd_f_doubles(n), lane_bwd_g_doubles(n, ng, lane_bwd_layout(n, ng, m0))) * sizeof(double);
 }
 
 bool pairs_lane_supported(int n_alpha, int n_beta, bool backward) {
@@ -821,9 +846,9 @@ void launch_pairs_lane(hipStream_t st, int which, int blocks, const SlotDesc* sl
                        int n_alpha, int n_beta, double cond_limit) {
     const int swap = n_beta > n_alpha ? 1 : 0;
     const int m0 = swap ? n_beta : n_alpha, m1 = swap ? n_alpha : n_beta;
-    const bool full = pairs_full_dg(topo.n, topo.ng);
-    const LaneBwdLayout lay = which ? lane_bwd_layout(topo.n, topo.ng) : LaneBwdLayout{};
-    const size_t lds = pairs_lane_lds_bytes(topo.n, topo.ng, which != 0);
+    const LaneBwdLayout lay = which ? lane_bwd_layout(topo.n, topo.ng, m0) : LaneBwdLayout{};
+    const bool full = lay.full_dg != 0;
+    const size_t lds = pairs_lane_lds_bytes(topo.n, topo.ng, which != 0, m0);
     const dim3 grid(blocks, 1, count);
     for_lane_shape(m0, m1, [&]<int A, int B>() {
         (void)lane_dispatch_shape<A, B>(true, which, full, st, grid, lds, slots, topo, swap, cond_limit, lay);

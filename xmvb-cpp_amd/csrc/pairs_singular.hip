@@ -472,7 +472,13 @@ __global__ void __launch_bounds__(256) k_pairs_singular(const SlotDesc* __restri
 
 }  // namespace
 
-std::size_t pairs_singular_lds_bytes(int n, int ng, bool backward) { return pairs_lane_lds_bytes(n, ng, backward); }
+// G2 (np x np), S_act, h_act; backward adds the kernel adjoint (full np x np or packed) and ds, dh
+std::size_t pairs_singular_lds_bytes(int n, int ng, bool backward) {
+    const std::size_t np = (std::size_t)n * (n + 1) / 2, nn = (std::size_t)n * n;
+    std::size_t doubles = np * np + 2 * nn;
+    if (backward) doubles += (pairs_full_dg(n, ng) ? np * np : (std::size_t)ng) + 2 * nn;
+    return doubles * sizeof(double);
+}
 
 hipError_t configure_pairs_singular() {
     const int limit = 160 * 1024;
