@@ -124,7 +124,8 @@ typedef struct vbe_phase_times {
 int vbe_parse_ctrl_method(const char* ctrl_line, int32_t* method);
 
 /* Reject option combinations HIPVBSCF does not implement, in the manner of run_cuvbscf
- * (src/cli/cuvbscf.cpp:38-50): eigensolver DENSE or DAVIDSON, two-electron mode must be RI (ri=1),
+ * (src/cli/cuvbscf.cpp:38-50): eigensolver DENSE or DAVIDSON, two-electron integrals given as
+ * RI-shaped factors (ri=1: INT=RI factors, or exact integrals factorised by vbe_pivoted_cholesky),
  * exactly one state, no inactive (core) electrons. Returns VBE_OK or VBE_ERR_UNSUPPORTED and
  * writes the reason to vbe_global_error(). */
 int vbe_check_hipvbscf_options(int32_t eigensolver, int32_t ri, int32_t n_states, int32_t n_inactive_electrons);

@@ -104,7 +104,8 @@ int vbe_check_hipvbscf_options(int32_t eigensolver, int32_t ri, int32_t n_states
         return VBE_ERR_UNSUPPORTED;
     }
     if (!ri) {
-        set_global_error("HIPVBSCF supports INT=RI two-electron integrals only");
+        set_global_error("HIPVBSCF takes two-electron integrals as RI-shaped factors (INT=RI, or exact integrals "
+                         "factorised by vbe_pivoted_cholesky)");
         return VBE_ERR_UNSUPPORTED;
     }
     if (n_states != 1) {
