@@ -241,6 +241,7 @@ def main():
     ap.add_argument("--mixed-batch", type=int, default=4096,
                     help="problems of the uneven-size batching workload evaluated per step (0: skip)")
     ap.add_argument("--mixed-steps-batching", type=int, default=3, help="timed steps of the batching workload")
+    ap.add_argument("--autotune", type=int, default=1, help="autotune the pair-kernel launch shape of each pool")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle parity")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -315,6 +316,8 @@ def main():
     cb_dev = torch.from_numpy(np.concatenate([o.ravel() for o in orbitals])).to(dev)
     gb_dev = torch.zeros_like(cb_dev)
     slots = list(range(K))
+    # launch shape: the pool's autotuner picks the pair-kernel block targets on these slots
+    tuned = pool.autotune(slots, orbitals, repeats=2) if args.autotune else None
     torch.cuda.synchronize()
     for _ in range(args.warmup):
         pool.evaluate_device(slots, cb_dev.data_ptr(), gb_dev.data_ptr())
@@ -520,6 +523,12 @@ def main():
                 del made
         gen_s = time.perf_counter() - tg
 
+        tuned_buckets = {}
+        if args.autotune:
+            for na, pl in pools.items():  # on the bucket's own launch (the target is per launch)
+                t = pl.autotune(list(range(len(orbs[na]))), orbs[na], repeats=1)
+                tuned_buckets[str(na)] = dict(fwd_target=t["fwd_target"], bwd_target=t["bwd_target"])
+
         def mixed_step():
             tickets = {na: pools[na].submit(list(range(len(orbs[na]))), orbs[na]) for na in pools}
             return {na: pools[na].wait(t)[0] for na, t in tickets.items()}
@@ -536,7 +545,8 @@ def main():
         dtm = (time.perf_counter() - t0) / msteps
         same = max(float(np.max(np.abs(again[na] - first[na]))) for na in first)
         for na, pl in pools.items():
-            per_bucket[str(na)] = dict(problems=len(orbs[na]), last_launch_ms=round(pl.phase_times()["total_ms"], 3))
+            per_bucket[str(na)] = dict(problems=len(orbs[na]), last_launch_ms=round(pl.phase_times()["total_ms"], 3),
+                                       launch_shape=tuned_buckets.get(str(na)))
             pl.close()
         mixed_batch = dict(problems=nb_total, value=nb_total / dtm, unit="E+grad evals/s",
                            ms_per_step=dtm * 1e3, steps=msteps, buckets=per_bucket,
@@ -554,7 +564,9 @@ def main():
                                     f"252 determinants; {K} resident problems per batched step ({D} distinct "
                                     "seeded problems, each slot its own device copy), FP64 throughout, orbitals resident",
                            seed=cfg["seed"], n_bf=cfg["n_bf"], n_aux=cfg["n_aux"], n_act=cfg["n_act"],
-                           n_structures=topo.n_structures, n_determinants=n_det, batch=K, path="batched-fp64"),
+                           n_structures=topo.n_structures, n_determinants=n_det, batch=K, path="batched-fp64",
+                           launch_shape=(dict(fwd_target=tuned["fwd_target"], bwd_target=tuned["bwd_target"],
+                                              tuned_ms=tuned["table"]) if tuned else "default")),
                latency_ms=(served["latency_ms"] if served else dict(p50=percentile(lat, 0.5),
                                                                     p99=percentile(lat, 0.99))),
                latency_source=("served leg: request completion minus Poisson arrival" if served

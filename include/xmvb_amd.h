@@ -291,6 +291,16 @@ int vbe_pool_poll(vbe_pool* pool, int32_t ticket, int32_t* done);
 int vbe_pool_wait(vbe_pool* pool, int32_t ticket, double* energies, double* grads);
 /* Device time per phase of the pool's most recently completed launch. */
 int vbe_pool_phase_times(const vbe_pool* pool, vbe_phase_times* times);
+/* Launch-shape autotuner (no reference counterpart): evaluates the listed slots (host orbitals C,
+ * as for vbe_pool_submit; E+grad) at each candidate pair-kernel block target per launch
+ * (256 ... 8192, VBE_AUTOTUNE_CANDIDATES of them), `repeats` times each, and keeps for the pool the
+ * target with the shortest forward pair phase and, separately, the shortest backward pair phase
+ * (device times of the pool's phase events). The choice applies to later launches of the pool;
+ * results differ between shapes only by the summation order of the adjoint partial sums.
+ * table (optional, VBE_AUTOTUNE_CANDIDATES x 2): best forward and backward ms per candidate. */
+#define VBE_AUTOTUNE_CANDIDATES 6
+int vbe_pool_autotune(vbe_pool* pool, int32_t count, const int32_t* slots, const double* C, int32_t repeats,
+                      int32_t* fwd_target, int32_t* bwd_target, double* table);
 
 /* ---- full convergence: projected L-BFGS on device-resident state -----------------------------
  * Options of VbScfOptimizerOptions (src/vbscf/optimization/driver/options.hpp:15-108) that the

@@ -99,3 +99,23 @@ def test_medium_batch_matches_single_problem(ctx):
     assert abs(e[5] - ref["energy"]) <= 1e-8 and np.abs(g[5] - ref["grad"]).max() <= 1e-7
     prob.close()
     pool.close()
+
+
+def test_autotune_keeps_results(ctx):
+    topo = vb.Topology.build("COV", 8, 8, 1)
+    K = 8
+    pool = vb.Pool(ctx, topo, capacity=K)
+    sps = [vb.synthetic_problem(300 + s, 40, 8, 120) for s in range(K)]
+    for s, sp in enumerate(sps):
+        pool.load_synthetic(s, sp)
+    slots = list(range(K))
+    e0, g0 = pool.evaluate(slots, [sp.C for sp in sps])
+    print("case autotune 8 slots", flush=True)
+    tuned = pool.autotune(slots, [sp.C for sp in sps], repeats=1)
+    assert tuned["fwd_target"] in tuned["table"] and tuned["bwd_target"] in tuned["table"]
+    assert all(v["fwd_ms"] > 0.0 and v["bwd_ms"] > 0.0 for v in tuned["table"].values())
+    e1, g1 = pool.evaluate(slots, [sp.C for sp in sps])
+    for s in range(K):
+        assert abs(e1[s] - e0[s]) <= 1e-10
+        assert float(np.abs(g1[s] - g0[s]).max()) <= 1e-10
+    pool.close()

@@ -43,10 +43,9 @@ void check(hipError_t e, const char* what) {
 }
 
 constexpr int kLanes = 2;              // streams of a pool (in-flight submissions)
-constexpr int kPairBlocksFwd = 4096;   // target forward pair blocks per launch (all slots)
-constexpr int kPairBlocksBwd = 2048;   // target backward pair blocks per launch (all slots)
 constexpr int kMaxBwdBlocksPerSlot = 512;
-constexpr int kLaneBlocks = 2048;      // target one-pair-per-lane blocks per launch (all slots)
+constexpr int kLaneBlocks = 2048;      // default one-pair-per-lane blocks per launch (all slots)
+constexpr int kAutotuneTargets[VBE_AUTOTUNE_CANDIDATES] = {256, 512, 1024, 2048, 4096, 8192};
 
 }  // namespace
 
@@ -158,6 +157,10 @@ struct vbe_pool {
     int last_lane = 0;
     int64_t last_singular = 0;
     vbe_phase_times times{};
+    // pair-kernel blocks per launch (all listed slots together), chosen by vbe_pool_autotune:
+    // forward one-pair-per-lane blocks (the lane-group kernel takes twice this), backward blocks
+    int fwd_target = kLaneBlocks;
+    int bwd_target = kLaneBlocks;
 
     ~vbe_pool() {
         for (auto& l : lanes) {
@@ -459,17 +462,18 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
     const long long count = (long long)in.size();
     // at least ~4 sweeps of 4 wavefronts' pair groups per forward block
     const long long per_sweep = 4LL * (64 / xmvb_amd::pair_group(sh.mmax));
-    sh.pair_blocks_fwd = (int)std::clamp<long long>((kPairBlocksFwd + count - 1) / count, 1,
+    const long long fwd_group = 2LL * p->fwd_target, bwd_group = p->bwd_target;
+    sh.pair_blocks_fwd = (int)std::clamp<long long>((fwd_group + count - 1) / count, 1,
                                                     std::max<long long>(1, (pairs + 4 * per_sweep - 1) / (4 * per_sweep)));
-    sh.pair_blocks_bwd = (int)std::clamp<long long>((kPairBlocksBwd + count - 1) / count, 1, bwd_blocks_cap(p));
+    sh.pair_blocks_bwd = (int)std::clamp<long long>((bwd_group + count - 1) / count, 1, bwd_blocks_cap(p));
     // one pair per lane: 256 pairs per block sweep
     sh.n_alpha = p->n_alpha;
     sh.n_beta = p->n_beta;
     sh.lane_fwd = xmvb_amd::pairs_lane_supported(p->n_alpha, p->n_beta, false);
     sh.lane_bwd = xmvb_amd::pairs_lane_supported(p->n_alpha, p->n_beta, true);
     const long long lane_sweep = std::max<long long>(1, (pairs + 255) / 256);
-    sh.lane_blocks_fwd = (int)std::clamp<long long>((kLaneBlocks + count - 1) / count, 1, lane_sweep);
-    sh.lane_blocks_bwd = (int)std::clamp<long long>((kLaneBlocks + count - 1) / count, 1,
+    sh.lane_blocks_fwd = (int)std::clamp<long long>((p->fwd_target + count - 1) / count, 1, lane_sweep);
+    sh.lane_blocks_bwd = (int)std::clamp<long long>((p->bwd_target + count - 1) / count, 1,
                                                     std::min<long long>(lane_sweep, bwd_blocks_cap(p)));
     sh.list_blocks = xmvb_amd::kListBlocks;
     sh.lane_cond_limit = p->ctx->pair_condition_limit;
@@ -932,6 +936,60 @@ int vbe_pool_wait(vbe_pool* pool, int32_t ticket, double* energies, double* grad
         }
         throw StatusError(VBE_ERR_ARGUMENT, "unknown or already completed ticket");
     });
+}
+
+int vbe_pool_autotune(vbe_pool* pool, int32_t count, const int32_t* slots, const double* C, int32_t repeats,
+                      int32_t* fwd_target, int32_t* bwd_target, double* table) {
+    if (!pool || !slots || !C || count <= 0 || repeats <= 0 || !fwd_target || !bwd_target) {
+        if (pool) fail(pool->ctx, VBE_ERR_ARGUMENT, "vbe_pool_autotune: invalid argument");
+        return VBE_ERR_ARGUMENT;
+    }
+    size_t total = 0;
+    for (int i = 0; i < count; ++i) {
+        if (slots[i] < 0 || slots[i] >= pool->capacity || !pool->slots[slots[i]].loaded)
+            return fail(pool->ctx, VBE_ERR_ARGUMENT, "slot not loaded");
+        total += (size_t)pool->n_act * pool->slots[slots[i]].n_bf;
+    }
+    std::vector<double> e(count), g(total);
+    auto run = [&](double& t_fwd, double& t_bwd) -> int {
+        int32_t ticket = 0;
+        int rc = vbe_pool_submit(pool, count, slots, C, 1, &ticket);
+        if (rc != VBE_OK) return rc;
+        rc = vbe_pool_wait(pool, ticket, e.data(), g.data());
+        t_fwd = pool->times.pair_forward_ms;
+        t_bwd = pool->times.pair_backward_ms;
+        return rc;
+    };
+    const int saved_f = pool->fwd_target, saved_b = pool->bwd_target;
+    double tf = 0.0, tb = 0.0;
+    int rc = run(tf, tb);  // warm-up at the current shape
+    double best_f = 1e300, best_b = 1e300;
+    int pick_f = saved_f, pick_b = saved_b;
+    for (int c = 0; c < VBE_AUTOTUNE_CANDIDATES && rc == VBE_OK; ++c) {
+        pool->fwd_target = pool->bwd_target = kAutotuneTargets[c];
+        double mf = 1e300, mb = 1e300;
+        for (int r = 0; r < repeats && rc == VBE_OK; ++r) {
+            rc = run(tf, tb);
+            mf = std::min(mf, tf);
+            mb = std::min(mb, tb);
+        }
+        if (table) {
+            table[2 * c] = mf;
+            table[2 * c + 1] = mb;
+        }
+        if (mf < best_f) best_f = mf, pick_f = kAutotuneTargets[c];
+        if (mb < best_b) best_b = mb, pick_b = kAutotuneTargets[c];
+    }
+    if (rc != VBE_OK) {
+        pool->fwd_target = saved_f;
+        pool->bwd_target = saved_b;
+        return rc;
+    }
+    pool->fwd_target = pick_f;
+    pool->bwd_target = pick_b;
+    *fwd_target = pick_f;
+    *bwd_target = pick_b;
+    return VBE_OK;
 }
 
 int vbe_pool_phase_times(const vbe_pool* pool, vbe_phase_times* times) {

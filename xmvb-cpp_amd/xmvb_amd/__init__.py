@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "vbe_problem_release", "vbe_problem_device_bytes", "vbe_eval", "vbe_eval_device", "vbe_eval_forward",
     "vbe_complete_gradient", "vbe_last_intermediates", "vbe_last_phase_times", "vbe_last_singular_pairs",
     "vbe_pool_create", "vbe_pool_destroy", "vbe_pool_load", "vbe_pool_device_bytes", "vbe_pool_unload",
-    "vbe_context_memory", "vbe_pool_eval_device",
+    "vbe_context_memory", "vbe_pool_autotune", "vbe_pool_eval_device",
     "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit", "vbe_set_ri_precision",
     "vbe_optimizer_defaults", "vbe_optimize", "vbe_pool_optimize", "vbe_set_eigensolver",
     "vbe_gaussian_basis_size", "vbe_ao_integrals", "vbe_pivoted_cholesky",
@@ -147,6 +147,7 @@ def library() -> C.CDLL:
         lib.vbe_pool_load.argtypes = [P, I32, I32, I32, P, P, P, D]
         lib.vbe_pool_device_bytes.argtypes = [P, C.POINTER(U64)]
         lib.vbe_pool_unload.argtypes = [P, I32]
+        lib.vbe_pool_autotune.argtypes = [P, I32, P, P, I32, C.POINTER(I32), C.POINTER(I32), P]
         lib.vbe_context_memory.argtypes = [P, C.POINTER(U64), C.POINTER(U64)]
         lib.vbe_pool_eval_device.argtypes = [P, I32, P, P, I32, P, P]
         lib.vbe_pool_submit.argtypes = [P, I32, P, P, I32, C.POINTER(I32)]
@@ -620,6 +621,19 @@ class Pool:
         b = C.c_uint64()
         self.ctx._lib.vbe_pool_device_bytes(self._h, C.byref(b))
         return b.value
+
+    def autotune(self, slots, orbitals, repeats: int = 2) -> dict:
+        """Time the candidate pair-kernel block targets on these slots and keep the fastest for
+        later launches; returns the choice and the per-candidate forward/backward pair-phase ms."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(o, dtype=np.float64).ravel() for o in orbitals]))
+        f, b = C.c_int32(), C.c_int32()
+        table = np.zeros(2 * 6)
+        self.ctx.check(self.ctx._lib.vbe_pool_autotune(self._h, len(sl), _ptr(sl), _ptr(flat), int(repeats),
+                                                       C.byref(f), C.byref(b), _ptr(table)))
+        cands = [256, 512, 1024, 2048, 4096, 8192]
+        return dict(fwd_target=f.value, bwd_target=b.value,
+                    table={c: dict(fwd_ms=table[2 * k], bwd_ms=table[2 * k + 1]) for k, c in enumerate(cands)})
 
     def unload(self, slot: int):
         """Release one slot's device storage (its bytes return to the context budget)."""
