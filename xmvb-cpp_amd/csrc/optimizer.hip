@@ -43,6 +43,8 @@ namespace {
 
 constexpr int kPointThreads = 256;
 constexpr int kBlockThreads = 1024;
+constexpr int kDirThreads = 256;  // two-loop block
+constexpr int kDirRegs = 8;       // elements per thread the two-loop keeps in registers (N <= 2048)
 constexpr int kOrbStats = 4;  // per orbital: dual norm, max |c|, norm before normalisation, c^T g
 
 void hip_check(hipError_t e, const char* what) {
@@ -130,6 +132,7 @@ struct OptDesc {
     double* hs;                 // secant ring: steps, gradient changes, per-orbital statistics
     double* hy;
     double* hstat;
+    double* proj;               // chart projections of the stored pairs: sig then bet, cap x n_act each
     int cap;                    // ring capacity (>= 1)
     int count;                  // pairs used by this direction (newest first)
     int newest;                 // ring slot of the newest pair
@@ -213,7 +216,24 @@ __global__ void __launch_bounds__(kPointThreads) k_opt_point(const OptDesc* __re
 // with the scaled-identity initial inverse gamma S^-1 (gamma = s'y'/y'y' of the newest kept pair)
 // and the try_add_pair curvature filter (transported_lbfgs.cpp:14-32). One block per problem;
 // pairs are addressed by age (0 = newest) in the ring.
-__global__ void __launch_bounds__(kBlockThreads) k_opt_direction(const OptDesc* __restrict__ descs, int n_act) {
+// Chart projections of the stored pairs for the two-loop: sig[a,i] = (S c_i)^T s_a,i and
+// bet[a,i] = c_i^T y_a,i (a = age, newest first), one wavefront per dot product over the orbital's
+// n_bf coefficients, spread over many blocks (grid x over tasks, y = which, z = problem).
+__global__ void __launch_bounds__(256) k_opt_project(const OptDesc* __restrict__ descs, int n_act) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int task = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (task >= o.count * n_act) return;  // wave-uniform
+    const int which = blockIdx.y, a = task / n_act, i = task % n_act, n_bf = o.n_bf;
+    const int slot = (o.newest - a + o.cap) % o.cap;
+    const double* v = (which == 0 ? o.hs : o.hy) + (size_t)slot * o.N + (size_t)i * n_bf;
+    const double* w = (which == 0 ? o.Sc : o.C) + (size_t)i * n_bf;
+    double part = 0.0;
+    for (int k = lane; k < n_bf; k += 64) part += v[k] * w[k];
+    part = wave_total(part);
+    if (lane == 0) o.proj[(size_t)which * o.cap * n_act + task] = part;
+}
+
+__global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __restrict__ descs, int n_act) {
     const OptDesc& o = descs[blockIdx.z];
     const int n_bf = o.n_bf, N = o.N, cap = o.cap, count = o.count, newest = o.newest;
     const double* __restrict__ C = o.C;
@@ -230,19 +250,12 @@ __global__ void __launch_bounds__(kBlockThreads) k_opt_direction(const OptDesc* 
     double* gam = rho + count;                  // s'^T y' / y'^T y'
     __shared__ double red[16];
     __shared__ int bad;
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    const int tid = threadIdx.x, nt = blockDim.x;
     if (tid == 0) bad = 0;
-    // projection coefficients of every stored pair in this chart, one wavefront per dot product
-    const int tasks = 2 * count * n_act;
-    for (int task = wave; task < tasks; task += nw) {
-        const int which = task / (count * n_act), rem = task % (count * n_act), a = rem / n_act, i = rem % n_act;
-        const int slot = (newest - a + cap) % cap;
-        const double* v = (which == 0 ? hs : hy) + (size_t)slot * N + (size_t)i * n_bf;
-        const double* w = (which == 0 ? Sc : C) + (size_t)i * n_bf;
-        double part = 0.0;
-        for (int k = lane; k < n_bf; k += 64) part += v[k] * w[k];
-        part = wave_total(part);
-        if (lane == 0) (which == 0 ? sig : bet)[a * n_act + i] = part;
+    // projection coefficients of the stored pairs in this chart (k_opt_project)
+    for (int t = tid; t < count * n_act; t += nt) {
+        sig[t] = o.proj[t];
+        bet[t] = o.proj[(size_t)cap * n_act + t];
     }
     __syncthreads();
     for (int a = tid; a < count; a += nt) {
@@ -261,10 +274,116 @@ __global__ void __launch_bounds__(kBlockThreads) k_opt_direction(const OptDesc* 
         rho[a] = ok ? 1.0 / sy : 0.0;
         gam[a] = ok ? sy / yy : 0.0;
     }
-    for (int e = tid; e < N; e += nt) q[e] = o.R[e];
     __syncthreads();
     int used = 0;
     double gamma = 1.0;
+    if (N <= kDirRegs * nt) {
+        // register path: a thread's elements e = tid + m nt stay in registers (q, z, C, Sc), and the
+        // next pair's s and y are loaded while the current pair's block reduction runs; the
+        // arithmetic per element and its order are those of the memory path below
+        double qr[kDirRegs], cr[kDirRegs], scr[kDirRegs], sr[kDirRegs], yr[kDirRegs], sn[kDirRegs], yn[kDirRegs];
+        int orb[kDirRegs];
+#pragma unroll
+        for (int m = 0; m < kDirRegs; ++m) {
+            const int e = tid + m * nt;
+            const bool in = e < N;
+            qr[m] = in ? o.R[e] : 0.0;
+            cr[m] = in ? C[e] : 0.0;
+            scr[m] = in ? Sc[e] : 0.0;
+            orb[m] = in ? e / n_bf : 0;
+        }
+        auto load_pair = [&](int a, double* sv, double* yv) {
+            const int slot = (newest - a + cap) % cap;
+            const double* s = hs + (size_t)slot * N;
+            const double* y = hy + (size_t)slot * N;
+#pragma unroll
+            for (int m = 0; m < kDirRegs; ++m) {
+                const int e = tid + m * nt;
+                sv[m] = e < N ? s[e] : 0.0;
+                yv[m] = e < N ? y[e] : 0.0;
+            }
+        };
+        // pairs in use, newest to oldest
+        int first = 0;
+        while (first < count && rho[first] == 0.0) ++first;
+        if (first < count) load_pair(first, sn, yn);
+        bool have_newest = false;
+        for (int a = first; a < count;) {
+#pragma unroll
+            for (int m = 0; m < kDirRegs; ++m) {
+                sr[m] = sn[m];
+                yr[m] = yn[m];
+            }
+            int next = a + 1;
+            while (next < count && rho[next] == 0.0) ++next;
+            if (next < count) load_pair(next, sn, yn);
+            if (!have_newest) {
+                gamma = gam[a];
+                have_newest = true;
+            }
+            ++used;
+            double part = 0.0;
+#pragma unroll
+            for (int m = 0; m < kDirRegs; ++m)
+                if (tid + m * nt < N) part += (sr[m] - cr[m] * sig[a * n_act + orb[m]]) * qr[m];
+            const double alpha = rho[a] * block_total(part, red);
+            if (!isfinite(alpha) && tid == 0) bad = 1;
+            if (tid == 0) alp[a] = alpha;
+#pragma unroll
+            for (int m = 0; m < kDirRegs; ++m)
+                if (tid + m * nt < N) qr[m] -= alpha * (yr[m] - scr[m] * bet[a * n_act + orb[m]]);
+            a = next;
+        }
+#pragma unroll
+        for (int m = 0; m < kDirRegs; ++m)
+            if (tid + m * nt < N) q[tid + m * nt] = qr[m];
+        __syncthreads();
+        double zr[kDirRegs];
+#pragma unroll
+        for (int m = 0; m < kDirRegs; ++m) {  // z = gamma S_i^-1 q_i
+            const int e = tid + m * nt;
+            zr[m] = 0.0;
+            if (e < N) {
+                const int i = orb[m], k = e - i * n_bf;
+                const double* P = o.Sinv + (size_t)o.pat[i] * n_bf * n_bf;
+                const double* qi = q + (size_t)i * n_bf;
+                double acc = 0.0;
+                for (int j = 0; j < n_bf; ++j) acc += P[k + (size_t)n_bf * j] * qi[j];
+                zr[m] = gamma * acc;
+            }
+        }
+        // pairs in use, oldest to newest
+        int last = count - 1;
+        while (last >= 0 && rho[last] == 0.0) --last;
+        if (last >= 0) load_pair(last, sn, yn);
+        for (int a = last; a >= 0;) {
+#pragma unroll
+            for (int m = 0; m < kDirRegs; ++m) {
+                sr[m] = sn[m];
+                yr[m] = yn[m];
+            }
+            int next = a - 1;
+            while (next >= 0 && rho[next] == 0.0) --next;
+            if (next >= 0) load_pair(next, sn, yn);
+            double part = 0.0;
+#pragma unroll
+            for (int m = 0; m < kDirRegs; ++m)
+                if (tid + m * nt < N) part += (yr[m] - scr[m] * bet[a * n_act + orb[m]]) * zr[m];
+            const double b = rho[a] * block_total(part, red);
+            if (!isfinite(b) && tid == 0) bad = 1;
+            const double coef = alp[a] - b;
+#pragma unroll
+            for (int m = 0; m < kDirRegs; ++m)
+                if (tid + m * nt < N) zr[m] += (sr[m] - cr[m] * sig[a * n_act + orb[m]]) * coef;
+            a = next;
+        }
+#pragma unroll
+        for (int m = 0; m < kDirRegs; ++m)
+            if (tid + m * nt < N) z[tid + m * nt] = zr[m];
+        __syncthreads();
+    } else {
+    for (int e = tid; e < N; e += nt) q[e] = o.R[e];
+    __syncthreads();
     bool have_newest = false;
     for (int a = 0; a < count; ++a) {  // newest to oldest
         if (rho[a] == 0.0) continue;
@@ -304,6 +423,8 @@ __global__ void __launch_bounds__(kBlockThreads) k_opt_direction(const OptDesc* 
         if (!isfinite(b) && tid == 0) bad = 1;
         const double coef = alp[a] - b;
         for (int e = tid; e < N; e += nt) z[e] += (s[e] - C[e] * sig[a * n_act + e / n_bf]) * coef;
+    }
+    __syncthreads();
     }
     double dd = 0.0, tinf = 0.0;
     for (int e = tid; e < N; e += nt) {
@@ -465,7 +586,7 @@ struct Problem {
     std::vector<std::vector<int>> patterns;
     // device state
     double *S = nullptr, *Sinv = nullptr, *Sc = nullptr, *R = nullptr, *T = nullptr, *q = nullptr, *z = nullptr;
-    double *stats = nullptr, *out = nullptr, *hs = nullptr, *hy = nullptr, *hstat = nullptr;
+    double *stats = nullptr, *out = nullptr, *hs = nullptr, *hy = nullptr, *hstat = nullptr, *proj = nullptr;
     int* d_pat = nullptr;
     uint8_t* d_support = nullptr;
     double* pts[3] = {};
@@ -554,7 +675,7 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
                  DeviceBlock::bytes_for(n_act, 4) + (s.p->support ? DeviceBlock::bytes_for(s.N, 1) : 0) +
                  11 * DeviceBlock::bytes_for(s.N, 8) + DeviceBlock::bytes_for((size_t)kOrbStats * n_act, 8) +
                  DeviceBlock::bytes_for(kOutSlots, 8) + 2 * DeviceBlock::bytes_for((size_t)cap * s.N, 8) +
-                 DeviceBlock::bytes_for((size_t)cap * n_act * 3, 8);
+                 DeviceBlock::bytes_for((size_t)cap * n_act * 3, 8) + DeviceBlock::bytes_for((size_t)cap * n_act * 2, 8);
     }
     {  // the inverses, spread over host threads
         std::vector<std::pair<int, int>> jobs;
@@ -626,6 +747,7 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
         s.hs = dev.take<double>((size_t)cap * s.N);
         s.hy = dev.take<double>((size_t)cap * s.N);
         s.hstat = dev.take<double>((size_t)cap * n_act * 3);
+        s.proj = dev.take<double>((size_t)cap * n_act * 2);
         s.newest = cap - 1;
         c0.assign(s.p->C, s.p->C + s.N);
         if (s.p->support)
@@ -670,6 +792,7 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
             d.hs = s.hs;
             d.hy = s.hy;
             d.hstat = s.hstat;
+            d.proj = s.proj;
             d.cap = cap;
             d.count = s.use;
             d.newest = s.newest;
@@ -714,7 +837,12 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
         int max_use = 0;
         for (int k : list) max_use = std::max(max_use, P[k].use);
         const size_t lds = sizeof(double) * ((size_t)2 * max_use * n_act + 3 * (size_t)max_use);
-        hipLaunchKernelGGL(k_opt_direction, dim3(1, 1, (unsigned)list.size()), dim3(kBlockThreads), lds, st, d_desc,
+        if (max_use > 0) {
+            hipLaunchKernelGGL(k_opt_project, dim3((unsigned)((max_use * n_act + 3) / 4), 2, (unsigned)list.size()),
+                               dim3(256), 0, st, d_desc, n_act);
+            hip_check(hipGetLastError(), "k_opt_project");
+        }
+        hipLaunchKernelGGL(k_opt_direction, dim3(1, 1, (unsigned)list.size()), dim3(kDirThreads), lds, st, d_desc,
                            n_act);
         hip_check(hipGetLastError(), "k_opt_direction");
         read_back();
