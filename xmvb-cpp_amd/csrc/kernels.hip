@@ -828,8 +828,109 @@ __global__ void k_structure_left(const SlotDesc* __restrict__ slots, TopoDesc to
 // Storage: gep_work_doubles(n) = 3*n*n + 9*n doubles, in LDS when it fits (InLds) else in the
 // global workspace. A single wavefront is used for small n so that the block barriers are free.
 // ---------------------------------------------------------------------------------------------
-template <bool InLds>
-__global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__ slots, TopoDesc topo) {
+// Front half of the eigensolver for n_str whose matrices live in global memory, over many
+// workgroups: k_gep_chol (one workgroup per problem: symmetrised S and H, Cholesky S = L L^T in
+// place), k_gep_linv (L^-1 by independent columns, one lane per column: X1 holds it row-major and
+// V column-major), k_gep_hlt (U = H L^-T into X2) and k_gep_lul (A = L^-1 U into M), after which
+// k_gep_lowest<false, false> continues from the symmetrisation of A. Workspace layout as in
+// k_gep_lowest (Lm, M, V, 9 n vectors) followed by X1, X2.
+__device__ __forceinline__ size_t gep_core_dev(int n) { return 3 * (size_t)n * n + 9 * (size_t)n; }
+
+__global__ void __launch_bounds__(1024) k_gep_chol(const SlotDesc* __restrict__ slots, TopoDesc topo) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n = topo.n_str;
+    double* __restrict__ Lm = sd.gep;
+    double* __restrict__ M = Lm + (size_t)n * n;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int TX = 16, TY = nt / TX, tx = tid % TX, ty = tid / TX;
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int i = idx % n, j = idx / n;
+        Lm[idx] = 0.5 * (sd.Ss[i + n * j] + sd.Ss[j + n * i]);
+        M[idx] = 0.5 * (sd.Hs[i + n * j] + sd.Hs[j + n * i]);
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const double a = Lm[k + n * k];
+        if (!(a > 0.0)) {
+            if (tid == 0) sd.flags[FLAG_METRIC] = 1;
+            return;  // uniform; the later kernels run on meaningless values the host discards
+        }
+        const double dk = sqrt(a), rdk = 1.0 / dk;
+        __syncthreads();
+        for (int i = k + tid; i < n; i += nt) Lm[i + n * k] = (i == k) ? dk : Lm[i + n * k] * rdk;
+        __syncthreads();
+        for (int j = k + 1 + ty; j < n; j += TY) {
+            const double ljk = Lm[j + n * k];
+            for (int i = j + tx; i < n; i += TX) Lm[i + n * j] -= Lm[i + n * k] * ljk;
+        }
+        __syncthreads();
+    }
+}
+
+// X = L^-1, lane j = column j: X[i][j] = (delta_ij - sum_{k=j}^{i-1} L[i][k] X[k][j]) / L[i][i]
+__global__ void __launch_bounds__(64) k_gep_linv(const SlotDesc* __restrict__ slots, int n) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ L = sd.gep;
+    double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
+    double* __restrict__ X1 = sd.gep + gep_core_dev(n);
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    if (j >= n) return;
+    for (int i = 0; i < j; ++i) {
+        X1[j + (size_t)n * i] = 0.0;
+        V[i + (size_t)n * j] = 0.0;
+    }
+    for (int i = j; i < n; ++i) {
+        double s0 = (i == j) ? 1.0 : 0.0, s1 = 0.0;
+        int k = j;
+        for (; k + 1 < i; k += 2) {
+            s0 -= L[i + (size_t)n * k] * X1[j + (size_t)n * k];
+            s1 -= L[i + (size_t)n * (k + 1)] * X1[j + (size_t)n * (k + 1)];
+        }
+        for (; k < i; ++k) s0 -= L[i + (size_t)n * k] * X1[j + (size_t)n * k];
+        const double x = (s0 + s1) / L[i + (size_t)n * i];
+        X1[j + (size_t)n * i] = x;  // row-major: X1[j + n i] = X[i][j]
+        V[i + (size_t)n * j] = x;   // column-major
+    }
+}
+
+// U = H L^-T: U[i][j] = sum_{k<=j} H[i][k] X[j][k] (thread = row i, grid y = column j)
+__global__ void __launch_bounds__(256) k_gep_hlt(const SlotDesc* __restrict__ slots, int n) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ H = sd.gep + (size_t)n * n;  // M holds the symmetrised H
+    const double* __restrict__ X1 = sd.gep + gep_core_dev(n);
+    double* __restrict__ X2 = sd.gep + gep_core_dev(n) + (size_t)n * n;
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    double s0 = 0.0, s1 = 0.0;
+    int k = 0;
+    for (; k + 1 <= j; k += 2) {
+        s0 += H[i + (size_t)n * k] * X1[k + (size_t)n * j];
+        s1 += H[i + (size_t)n * (k + 1)] * X1[k + 1 + (size_t)n * j];
+    }
+    for (; k <= j; ++k) s0 += H[i + (size_t)n * k] * X1[k + (size_t)n * j];
+    X2[i + (size_t)n * j] = s0 + s1;
+}
+
+// A = L^-1 U: A[i][j] = sum_{k<=i} X[i][k] U[k][j] into M (thread = row i, grid y = column j)
+__global__ void __launch_bounds__(256) k_gep_lul(const SlotDesc* __restrict__ slots, int n) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    double* __restrict__ A = sd.gep + (size_t)n * n;
+    const double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
+    const double* __restrict__ X2 = sd.gep + gep_core_dev(n) + (size_t)n * n;
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    double s0 = 0.0, s1 = 0.0;
+    int k = 0;
+    for (; k + 1 <= i; k += 2) {
+        s0 += V[i + (size_t)n * k] * X2[k + (size_t)n * j];
+        s1 += V[i + (size_t)n * (k + 1)] * X2[k + 1 + (size_t)n * j];
+    }
+    for (; k <= i; ++k) s0 += V[i + (size_t)n * k] * X2[k + (size_t)n * j];
+    A[i + (size_t)n * j] = s0 + s1;
+}
+
+template <bool InLds, bool Front = true>
+__global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDesc* __restrict__ slots, TopoDesc topo) {
     const SlotDesc& sd = slots[blockIdx.z];
     const double* __restrict__ Hs = sd.Hs;
     const double* __restrict__ Ss = sd.Ss;
@@ -865,6 +966,7 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
     const int tid = threadIdx.x, nt = blockDim.x;
     // 2D thread layout for element-parallel updates: TX consecutive lanes share a row
     const int TX = nt == 64 ? 8 : 16, TY = nt / TX, tx = tid % TX, ty = tid / TX;
+    if constexpr (Front) {
     for (int idx = tid; idx < n * n; idx += nt) {
         const int i = idx % n, j = idx / n;
         Lm[idx] = 0.5 * (Ss[i + n * j] + Ss[j + n * i]);
@@ -884,9 +986,9 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
         __syncthreads();
         for (int i = k + tid; i < n; i += nt) Lm[i + n * k] = (i == k) ? dk : Lm[i + n * k] * rdk;
         __syncthreads();
-        for (int i = k + 1 + ty; i < n; i += TY) {
-            const double lik = Lm[i + n * k];
-            for (int j = k + 1 + tx; j <= i; j += TX) Lm[i + n * j] -= lik * Lm[j + n * k];
+        for (int j = k + 1 + ty; j < n; j += TY) {  // column j, rows i >= j: consecutive lanes, consecutive i
+            const double ljk = Lm[j + n * k];
+            for (int i = j + tx; i < n; i += TX) Lm[i + n * j] -= Lm[i + n * k] * ljk;
         }
         __syncthreads();
     }
@@ -938,6 +1040,7 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
         M[idx] = s0 + s1;
     }
     __syncthreads();
+    }
     // symmetrise A; keep Linv in Lm (for c = Linv^T z at the end) and clear V for the reflectors
     for (int idx = tid; idx < n * n; idx += nt) {
         const int i = idx % n, j = idx / n;
@@ -971,8 +1074,8 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
         for (int i0 = m0; i0 < n; i0 += TY) {
             const int i = i0 + ty;
             double a0 = 0.0;
-            if (i < n)
-                for (int j = m0 + tx; j < n; j += TX) a0 += M[i + n * j] * V[j + n * k];
+            if (i < n)  // A is symmetric: row i read as column i, consecutive across the lanes
+                for (int j = m0 + tx; j < n; j += TX) a0 += M[j + n * i] * V[j + n * k];
             a0 += dpp_f64<0xB1>(a0);
             a0 += dpp_f64<0x4E>(a0);
             a0 += dpp_f64<0x141>(a0);
@@ -985,9 +1088,9 @@ __global__ void __launch_bounds__(256) k_gep_lowest(const SlotDesc* __restrict__
         const double K = 0.5 * bk * block_sum(kp, red);
         for (int i = m0 + tid; i < n; i += nt) y[i] -= K * V[i + n * k];  // w = p - K v
         __syncthreads();
-        for (int i = m0 + ty; i < n; i += TY) {  // A -= v w^T + w v^T
+        for (int i = m0 + ty; i < n; i += TY) {  // A -= v w^T + w v^T, column i, consecutive rows j
             const double vi = V[i + n * k], wi = y[i];
-            for (int j = m0 + tx; j < n; j += TX) M[i + n * j] -= vi * y[j] + wi * V[j + n * k];
+            for (int j = m0 + tx; j < n; j += TX) M[j + n * i] -= V[j + n * k] * wi + y[j] * vi;
         }
         if (bk != 0.0)
             for (int i = m0 + tid; i < n; i += nt) {
@@ -1453,7 +1556,7 @@ hipError_t configure_kernels(int n, int ng, int n_str, int mmax) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGepLdsLimit);
 }
 
-size_t gep_lds_bytes(int n) { return gep_work_doubles(n) * sizeof(double); }
+size_t gep_lds_bytes(int n) { return gep_core_doubles(n) * sizeof(double); }
 
 bool gep_in_lds(int n) { return gep_lds_bytes(n) <= kGepLdsLimit; }
 
@@ -1522,8 +1625,14 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
     else if (gep_in_lds(topo.n_str))
         hipLaunchKernelGGL(k_gep_lowest<true>, dim3(1, 1, z), dim3(threads), gep_lds_bytes(topo.n_str), st, slots,
                            topo);
-    else
-        hipLaunchKernelGGL(k_gep_lowest<false>, dim3(1, 1, z), dim3(threads), 0, st, slots, topo);
+    else {
+        const int ns = topo.n_str;
+        hipLaunchKernelGGL(k_gep_chol, dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
+        hipLaunchKernelGGL(k_gep_linv, dim3((unsigned)((ns + 63) / 64), 1, z), dim3(64), 0, st, slots, ns);
+        hipLaunchKernelGGL(k_gep_hlt, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
+        hipLaunchKernelGGL(k_gep_lul, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
+        hipLaunchKernelGGL((k_gep_lowest<false, false>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
+    }
     mark(EV_EIGEN);
 }
 

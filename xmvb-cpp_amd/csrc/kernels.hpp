@@ -99,7 +99,10 @@ constexpr int kListBlocks = 4;
 // blocks per slot of the singular-pair kernel (partial rows after the pair kernels' rows)
 constexpr int kSingularBlocks = 4;
 
-inline std::size_t gep_work_doubles(int n) { return 3 * (std::size_t)n * n + 9 * (std::size_t)n; }
+// dense eigensolver workspace: 3 n^2 + 9 n doubles (in LDS for small n), plus two n x n scratch
+// matrices for the multi-workgroup front half of the large-n path
+inline std::size_t gep_core_doubles(int n) { return 3 * (std::size_t)n * n + 9 * (std::size_t)n; }
+inline std::size_t gep_work_doubles(int n) { return gep_core_doubles(n) + 2 * (std::size_t)n * n; }
 
 // Davidson subspace limit for one root: min(n, max(32, 8 n_roots)) (make_davidson_options,
 // linalg/eigensolver.cpp:321-361)
