@@ -119,3 +119,23 @@ def test_autotune_keeps_results(ctx):
         assert abs(e1[s] - e0[s]) <= 1e-10
         assert float(np.abs(g1[s] - g0[s]).max()) <= 1e-10
     pool.close()
+
+
+def test_large_structure_space_batched_matches_alone(ctx):
+    """132 structures: the eigensolve runs from global memory, per step over many workgroups when
+    a launch holds few problems and one workgroup per problem in larger batches. Both forms must
+    give the same energies and gradients (the tridiagonalisation's sums are ordered differently,
+    hence the 1e-9 / 1e-8 tolerances)."""
+    topo = vb.Topology.build("COV", 12, 12, 1)
+    K = 10
+    pool = vb.Pool(ctx, topo, capacity=K)
+    sps = [vb.synthetic_problem(300 + s, 48, 12, 96) for s in range(K)]
+    for s, sp in enumerate(sps):
+        pool.load_synthetic(s, sp)
+    print("case 132 structures, 10 slots alone vs batched", flush=True)
+    e_all, g_all = pool.evaluate(list(range(K)), [sp.C for sp in sps])
+    for s in (0, 4, 9):
+        e1, g1 = pool.evaluate([s], [sps[s].C])
+        assert abs(e_all[s] - e1[0]) <= 1e-9
+        assert np.abs(g_all[s] - g1[0]).max() <= 1e-8
+    pool.close()

@@ -832,8 +832,9 @@ __global__ void k_structure_left(const SlotDesc* __restrict__ slots, TopoDesc to
 // workgroups: k_gep_chol (one workgroup per problem: symmetrised S and H, Cholesky S = L L^T in
 // place), k_gep_linv (L^-1 by independent columns, one lane per column: X1 holds it row-major and
 // V column-major), k_gep_hlt (U = H L^-T into X2) and k_gep_lul (A = L^-1 U into M), after which
-// k_gep_lowest<false, false> continues from the symmetrisation of A. Workspace layout as in
-// k_gep_lowest (Lm, M, V, 9 n vectors) followed by X1, X2.
+// k_gep_lowest<false, 3> continues from the symmetrisation of A (launches of few problems replace
+// the Cholesky and the Householder stage by the per-step kernels further down). Workspace layout
+// as in k_gep_lowest (Lm, M, V, 9 n vectors) followed by X1, X2.
 __device__ __forceinline__ size_t gep_core_dev(int n) { return 3 * (size_t)n * n + 9 * (size_t)n; }
 
 __global__ void __launch_bounds__(1024) k_gep_chol(const SlotDesc* __restrict__ slots, TopoDesc topo) {
@@ -929,7 +930,120 @@ __global__ void __launch_bounds__(256) k_gep_lul(const SlotDesc* __restrict__ sl
     A[i + (size_t)n * j] = s0 + s1;
 }
 
-template <bool InLds, bool Front = true>
+// Per-step Cholesky of the large-n path for launches of few problems: k_gep_load symmetrises S
+// and H into the workspace (many blocks), k_chol_scale (one block) takes column k
+// (L[k][k] = sqrt(a), L[i][k] /= L[k][k]; a <= 0 sets the metric flag and the solve goes on with
+// values the host discards), k_chol_update (four columns per block, rows on consecutive lanes)
+// applies L[i][j] -= L[i][k] L[j][k] for k < j <= i.
+__global__ void __launch_bounds__(256) k_gep_load(const SlotDesc* __restrict__ slots, int n) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    double* __restrict__ Lm = sd.gep;
+    double* __restrict__ M = Lm + (size_t)n * n;
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    Lm[idx] = 0.5 * (sd.Ss[i + (size_t)n * j] + sd.Ss[j + (size_t)n * i]);
+    M[idx] = 0.5 * (sd.Hs[i + (size_t)n * j] + sd.Hs[j + (size_t)n * i]);
+}
+
+__global__ void __launch_bounds__(256) k_chol_scale(const SlotDesc* __restrict__ slots, int n, int k) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    double* __restrict__ L = sd.gep;
+    const double a = L[k + (size_t)n * k];
+    if (!(a > 0.0) && threadIdx.x == 0) sd.flags[FLAG_METRIC] = 1;
+    const double dk = sqrt(a), rdk = 1.0 / dk;
+    __syncthreads();  // every thread has read the diagonal before it is replaced
+    for (int i = k + threadIdx.x; i < n; i += blockDim.x)
+        L[i + (size_t)n * k] = (i == k) ? dk : L[i + (size_t)n * k] * rdk;
+}
+
+__global__ void __launch_bounds__(256) k_chol_update(const SlotDesc* __restrict__ slots, int n, int k) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    double* __restrict__ L = sd.gep;
+    const int j = k + 1 + (int)blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (j >= n) return;  // wave-uniform
+    const double ljk = L[j + (size_t)n * k];
+    for (int i = j + lane; i < n; i += 64) L[i + (size_t)n * j] -= L[i + (size_t)n * k] * ljk;
+}
+
+// Householder step k of the large-n path, over many blocks. k_hh_matvec: every block forms
+// v_k from column k of A itself (block 0 stores it and beta_k), then p = beta_k A v for four rows
+// per block (one wavefront per row; A symmetric, so row i is read as column i). k_hh_update:
+// K = beta_k v^T p / 2, w = p - K v, and A -= v w^T + w v^T on four columns per block; block 0
+// writes the reduced column and row k. Workspace as in k_gep_lowest (y holds p, then w).
+__global__ void __launch_bounds__(256) k_hh_matvec(const SlotDesc* __restrict__ slots, int n, int k) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ M = sd.gep + (size_t)n * n;
+    double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
+    double* __restrict__ y = sd.gep + 3 * (size_t)n * n + 2 * (size_t)n;
+    double* __restrict__ beta = y + n;
+    extern __shared__ double vloc[];  // n
+    __shared__ double red[16];
+    const int m0 = k + 1, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
+    double part = 0.0;
+    for (int i = m0 + tid; i < n; i += nt) {
+        const double x = M[i + (size_t)n * k];
+        part += x * x;
+    }
+    const double sigma = block_sum(part, red);
+    const double x0 = M[m0 + (size_t)n * k];
+    const double alpha = sigma > 0.0 ? (x0 >= 0.0 ? -sqrt(sigma) : sqrt(sigma)) : 0.0;
+    const double vtv = sigma - 2.0 * alpha * x0 + alpha * alpha;
+    const double bk = (sigma > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
+    for (int i = tid; i < n; i += nt) vloc[i] = i >= m0 ? M[i + (size_t)n * k] - (i == m0 ? alpha : 0.0) : 0.0;
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        for (int i = m0 + tid; i < n; i += nt) V[i + (size_t)n * k] = vloc[i];
+        if (tid == 0) beta[k] = bk;
+    }
+    const int i = m0 + (int)blockIdx.x * 4 + wave;
+    if (i < n) {  // wave-uniform
+        double a = 0.0;
+        for (int j = m0 + lane; j < n; j += 64) a += M[j + (size_t)n * i] * vloc[j];
+        a = wave_sum(a);
+        if (lane == 0) y[i] = bk * a;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_hh_update(const SlotDesc* __restrict__ slots, int n, int k) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    double* __restrict__ M = sd.gep + (size_t)n * n;
+    const double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
+    const double* __restrict__ y = sd.gep + 3 * (size_t)n * n + 2 * (size_t)n;
+    const double bk = y[n + k];  // beta[k]
+    extern __shared__ double loc[];  // v (n), w (n)
+    double* vloc = loc;
+    double* wloc = loc + n;
+    __shared__ double red[16];
+    const int m0 = k + 1, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
+    double part = 0.0;
+    for (int i = m0 + tid; i < n; i += nt) part += V[i + (size_t)n * k] * y[i];
+    const double K = 0.5 * bk * block_sum(part, red);
+    const double alpha = M[m0 + (size_t)n * k] - V[m0 + (size_t)n * k];  // v_m0 = x0 - alpha
+    for (int i = tid; i < n; i += nt) {
+        const double v = i >= m0 ? V[i + (size_t)n * k] : 0.0;
+        vloc[i] = v;
+        wloc[i] = i >= m0 ? y[i] - K * v : 0.0;
+    }
+    __syncthreads();
+    const int c = m0 + (int)blockIdx.x * 4 + wave;
+    if (c < n) {  // wave-uniform: column c, rows j >= m0 on consecutive lanes
+        const double vc = vloc[c], wc = wloc[c];
+        for (int j = m0 + lane; j < n; j += 64) M[j + (size_t)n * c] -= vloc[j] * wc + wloc[j] * vc;
+    }
+    if (blockIdx.x == 0 && bk != 0.0)
+        for (int i = m0 + tid; i < n; i += nt) {
+            const double e = (i == m0) ? alpha : 0.0;
+            M[i + (size_t)n * k] = e;
+            M[k + (size_t)n * i] = e;
+        }
+}
+
+// Part 0: the whole solve (matrices in LDS). Large-n path (matrices in global memory, after the
+// multi-workgroup front half): Part 3 is the rest on one workgroup per problem (batched launches);
+// for launches of few problems, Part 1 is the symmetrisation and Part 2 the rest after the
+// per-step Householder kernels.
+template <bool InLds, int Part = 0>
 __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDesc* __restrict__ slots, TopoDesc topo) {
     const SlotDesc& sd = slots[blockIdx.z];
     const double* __restrict__ Hs = sd.Hs;
@@ -966,7 +1080,7 @@ __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDes
     const int tid = threadIdx.x, nt = blockDim.x;
     // 2D thread layout for element-parallel updates: TX consecutive lanes share a row
     const int TX = nt == 64 ? 8 : 16, TY = nt / TX, tx = tid % TX, ty = tid / TX;
-    if constexpr (Front) {
+    if constexpr (Part == 0) {
     for (int idx = tid; idx < n * n; idx += nt) {
         const int i = idx % n, j = idx / n;
         Lm[idx] = 0.5 * (Ss[i + n * j] + Ss[j + n * i]);
@@ -1042,6 +1156,7 @@ __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDes
     __syncthreads();
     }
     // symmetrise A; keep Linv in Lm (for c = Linv^T z at the end) and clear V for the reflectors
+    if constexpr (Part != 2) {
     for (int idx = tid; idx < n * n; idx += nt) {
         const int i = idx % n, j = idx / n;
         Lm[idx] = (i >= j) ? V[idx] : 0.0;
@@ -1057,7 +1172,10 @@ __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDes
         }
     }
     __syncthreads();
+    }
+    if constexpr (Part == 1) return;
     // Householder tridiagonalisation: reflect rows/cols k+1..n-1 of column k
+    if constexpr (Part == 0 || Part == 3) {
     for (int k = 0; k < n - 2; ++k) {
         const int m0 = k + 1;
         double part = 0.0;
@@ -1099,6 +1217,7 @@ __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDes
                 M[k + n * i] = v;
             }
         __syncthreads();
+    }
     }
     for (int i = tid; i < n; i += nt) {
         dd[i] = M[i + n * i];
@@ -1467,6 +1586,8 @@ __global__ void k_normalization_pullback(const SlotDesc* __restrict__ slots) {
 // backward, the adjoint accumulators (full np x np when that still fits, else packed).
 constexpr size_t kPairLdsLimit = 160 * 1024;
 constexpr size_t kGepLdsLimit = 144 * 1024;
+constexpr unsigned kGepPerStepMax = 8;  // problems per launch up to which the large-n solve runs per step
+constexpr int kCholPerStepMinN = 256;   // below it one workgroup factors S faster than n launch pairs
 
 // lanes per determinant pair for blocks of at most `mmax` electrons of one spin
 int pair_group(int mmax) { return mmax <= 4 ? 16 : (mmax == 5 ? 32 : 64); }
@@ -1626,12 +1747,39 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
         hipLaunchKernelGGL(k_gep_lowest<true>, dim3(1, 1, z), dim3(threads), gep_lds_bytes(topo.n_str), st, slots,
                            topo);
     else {
+        // matrices in global memory: few problems per launch take the per-step Cholesky and
+        // Householder kernels (many blocks each); batched launches keep one workgroup per problem
+        // for those two stages, which the number of problems already parallelises
         const int ns = topo.n_str;
-        hipLaunchKernelGGL(k_gep_chol, dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
+        const bool per_step = z <= kGepPerStepMax;
+        if (per_step && ns >= kCholPerStepMinN) {
+            hipLaunchKernelGGL(k_gep_load, dim3((unsigned)(((long long)ns * ns + 255) / 256), 1, z), dim3(256), 0, st,
+                               slots, ns);
+            for (int k = 0; k < ns; ++k) {
+                hipLaunchKernelGGL(k_chol_scale, dim3(1, 1, z), dim3(256), 0, st, slots, ns, k);
+                if (k + 1 < ns)
+                    hipLaunchKernelGGL(k_chol_update, dim3((unsigned)((ns - (k + 1) + 3) / 4), 1, z), dim3(256), 0,
+                                       st, slots, ns, k);
+            }
+        } else {
+            hipLaunchKernelGGL(k_gep_chol, dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
+        }
         hipLaunchKernelGGL(k_gep_linv, dim3((unsigned)((ns + 63) / 64), 1, z), dim3(64), 0, st, slots, ns);
         hipLaunchKernelGGL(k_gep_hlt, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
         hipLaunchKernelGGL(k_gep_lul, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
-        hipLaunchKernelGGL((k_gep_lowest<false, false>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
+        if (per_step) {
+            hipLaunchKernelGGL((k_gep_lowest<false, 1>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
+            for (int k = 0; k + 2 < ns; ++k) {
+                const unsigned rows = (unsigned)((ns - (k + 1) + 3) / 4);
+                hipLaunchKernelGGL(k_hh_matvec, dim3(rows, 1, z), dim3(256), (size_t)ns * sizeof(double), st, slots,
+                                   ns, k);
+                hipLaunchKernelGGL(k_hh_update, dim3(rows, 1, z), dim3(256), 2 * (size_t)ns * sizeof(double), st,
+                                   slots, ns, k);
+            }
+            hipLaunchKernelGGL((k_gep_lowest<false, 2>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
+        } else {
+            hipLaunchKernelGGL((k_gep_lowest<false, 3>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
+        }
     }
     mark(EV_EIGEN);
 }
