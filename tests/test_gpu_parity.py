@@ -120,6 +120,9 @@ def test_normalisation_failure_is_reported(ctx):
     ("COV", 6, 4, 3, 24, 48, 22),   # triplet: 3x3 alpha, 1x1 beta blocks
     ("COV", 7, 5, 2, 28, 56, 23),   # doublet: 3x3 alpha, 2x2 beta blocks
     ("COV", 10, 10, 1, 40, 80, 24),  # 5x5 blocks
+    ("FULL", 8, 14, 1, 32, 64, 25),  # 7x7 / 7x7 blocks, 36 structures
+    ("FULL", 8, 13, 2, 32, 64, 26),  # doublet: 7x7 alpha, 6x6 beta blocks
+    ("FULL", 8, 12, 3, 32, 64, 27),  # triplet: 7x7 alpha, 5x5 beta blocks
 ])
 @pytest.mark.parametrize("limit", [0.0, 1.0e3, None])
 def test_pair_condition_limit(kw, n_act, n_el, mult, n_bf, n_aux, seed, limit):

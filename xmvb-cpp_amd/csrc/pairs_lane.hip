@@ -699,11 +699,11 @@ __global__ void __launch_bounds__(256) LANE_WAVES
     }
 }
 
-// spin-block shapes with a lane kernel: M0 >= M1, M0 <= 6, M0 - M1 <= 2
+// spin-block shapes with a lane kernel: M0 >= M1, M0 <= 7, M0 - M1 <= 2
 template <int M0, int M1>
 struct LaneShape {
-    static constexpr bool forward = M0 >= M1 && M0 <= 6 && M0 - M1 <= 2;
-    static constexpr bool backward = forward && M0 <= 6;
+    static constexpr bool forward = M0 >= M1 && M0 <= 7 && M0 - M1 <= 2;
+    static constexpr bool backward = forward;
 };
 
 namespace {
@@ -808,7 +808,7 @@ bool for_lane_shape(int m0, int m1, Fn&& f) {
     XMVB_LANE(1, 0); XMVB_LANE(1, 1); XMVB_LANE(2, 0); XMVB_LANE(2, 1); XMVB_LANE(2, 2);
     XMVB_LANE(3, 1); XMVB_LANE(3, 2); XMVB_LANE(3, 3); XMVB_LANE(4, 2); XMVB_LANE(4, 3);
     XMVB_LANE(4, 4); XMVB_LANE(5, 3); XMVB_LANE(5, 4); XMVB_LANE(5, 5); XMVB_LANE(6, 4);
-    XMVB_LANE(6, 5); XMVB_LANE(6, 6);
+    XMVB_LANE(6, 5); XMVB_LANE(6, 6); XMVB_LANE(7, 5); XMVB_LANE(7, 6); XMVB_LANE(7, 7);
 #undef XMVB_LANE
     return hit;
 }
@@ -829,7 +829,7 @@ bool pairs_lane_supported(int n_alpha, int n_beta, bool backward) {
 
 hipError_t configure_pairs_lane() {
     hipError_t err = hipSuccess;
-    for (int m0 = 1; m0 <= 6; ++m0)
+    for (int m0 = 1; m0 <= 7; ++m0)
         for (int m1 = 0; m1 <= m0; ++m1)
             for_lane_shape(m0, m1, [&]<int A, int B>() {
                 for (int which = 0; which < 2; ++which) {
