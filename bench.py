@@ -232,6 +232,8 @@ def main():
                     help="iteration cap of a served convergence request")
     ap.add_argument("--serve-mixed", type=int, default=1,
                     help="also serve the mixed-size scans (one per n_act in {4,...,12}) under a 200 GB budget")
+    ap.add_argument("--precision-steps", type=int, default=5,
+                    help="timed evaluations of the precision form, FP64 and FP32 RI (0: skip)")
     ap.add_argument("--mixed-steps", type=int, default=30, help="timed batched steps with FP32 RI factors (0: skip)")
     ap.add_argument("--converge", type=int, default=3, help="full-convergence requests timed (0: skip)")
     ap.add_argument("--converge-iterations", type=int, default=500, help="iteration cap per convergence request")
@@ -399,6 +401,37 @@ def main():
                      note="|dE| above 1e-8 Eh on some problems: not the headline path")
         pool32.close()
         ctx32.close()
+
+    # ---- the precision form: one problem at n_bf=400, n_aux=1200, n_act=14 (429 structures), FP64
+    # and with FP32 RI factors; |dE| and the gradient max-norm error of the FP32 run against FP64 ----
+    precision = None
+    if args.precision_steps > 0:
+        psp = vb.synthetic_problem(0, 400, 14, 1200)
+        ptopo = vb.Topology.build("COV", 14, 14, 1)
+        runs = {}
+        for label, ri in (("fp64", 0), ("fp32_ri", 1)):
+            pctx = vb.Context(0)
+            if ri:
+                pctx.set_ri_precision(1)
+            pprob = vb.Problem.from_synthetic(pctx, psp, ptopo)
+            pe, pg = pprob.evaluate(psp.C)  # warm-up
+            times = []
+            for _ in range(args.precision_steps):
+                t0 = time.perf_counter()
+                pe, pg = pprob.evaluate(psp.C)
+                times.append((time.perf_counter() - t0) * 1e3)
+            runs[label] = dict(energy=float(pe), grad=pg, ms=percentile(times, 0.5),
+                               phase_ms={k: round(v, 3) for k, v in pprob.phase_times().items()})
+            pprob.close()
+            pctx.close()
+        precision = dict(workload="precision form: n_bf=400 n_aux=1200 n_act=14 (7a/7b), 429 covalent Rumer "
+                                  "structures, 3432 determinants, one problem, host orbitals in",
+                         fp64_ms_per_eval=runs["fp64"]["ms"], fp32_ri_ms_per_eval=runs["fp32_ri"]["ms"],
+                         energy_fp64=runs["fp64"]["energy"],
+                         abs_dE_fp32_ri_vs_fp64=abs(runs["fp32_ri"]["energy"] - runs["fp64"]["energy"]),
+                         grad_max_err_fp32_ri_vs_fp64=float(np.abs(runs["fp32_ri"]["grad"] - runs["fp64"]["grad"]).max()),
+                         phase_ms_fp64=runs["fp64"]["phase_ms"])
+        del psp, runs
 
     served = None
     if args.serve_seconds > 0:
@@ -573,7 +606,7 @@ def main():
                                else "batched step"),
                batch_step_ms=dict(p50=percentile(lat, 0.5), p99=percentile(lat, 0.99)),
                plain_fp64=plain, speedup_vs_plain=value / plain["value"], served=served,
-               served_mixed_sizes=served_mixed, fp32_ri=mixed,
+               served_mixed_sizes=served_mixed, fp32_ri=mixed, precision_form=precision,
                converge=converge, mixed_batch=mixed_batch,
                batch_consistency=batch_consistency, phase_ms={k: round(v, 5) for k, v in mean.items()},
                energy=energy, roofline=roofline, roofline_other_kernels=others)
