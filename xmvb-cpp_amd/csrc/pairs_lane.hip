@@ -46,8 +46,11 @@ __device__ __forceinline__ void occupied(uint32_t mask, int* out) {
     }
 }
 
-// In-place Gauss-Jordan inversion with row pivoting; the row interchanges are undone as column
-// interchanges in reverse order at the end. Returns whether the block is kept on the lane path.
+// In-place Gauss-Jordan inversion with row pivoting. A row interchange of X is an interchange of
+// two ket orbitals, so it is applied to R as well and never undone: the result is the inverse of
+// the row-permuted block, paired with the permuted R, which every consumer sums over in any order
+// (same-spin terms are symmetric under r <-> r2; the compound W index sorts the pair itself).
+// det keeps the sign of the unpermuted block. Returns whether the block stays on the lane path.
 template <int M>
 __device__ __forceinline__ bool lane_invert(const double* __restrict__ sact, int n, double cond_limit, LaneBlock<M>& b) {
     if constexpr (M == 0) {
@@ -63,7 +66,6 @@ __device__ __forceinline__ bool lane_invert(const double* __restrict__ sact, int
                 a[i][j] = sact[b.R[i] + n * b.L[j]];
                 amax = fmax(amax, fabs(a[i][j]));
             }
-        int piv[M];
         double det = 1.0;
         bool ok = true;
 #pragma unroll
@@ -77,7 +79,6 @@ __device__ __forceinline__ bool lane_invert(const double* __restrict__ sact, int
                 best = g ? v : best;
                 p = g ? i : p;
             }
-            piv[k] = p;
 #pragma unroll
             for (int i = k + 1; i < M; ++i) {
                 const bool s = p == i;
@@ -87,6 +88,9 @@ __device__ __forceinline__ bool lane_invert(const double* __restrict__ sact, int
                     a[k][j] = s ? a[i][j] : t;
                     a[i][j] = s ? t : a[i][j];
                 }
+                const int tr = b.R[k];
+                b.R[k] = s ? b.R[i] : tr;
+                b.R[i] = s ? tr : b.R[i];
             }
             const double pv = a[k][k];
             ok = ok && best > 0.0;
@@ -104,18 +108,6 @@ __device__ __forceinline__ bool lane_invert(const double* __restrict__ sact, int
                 for (int j = 0; j < M; ++j) a[i][j] -= f * a[k][j];
             }
         }
-#pragma unroll
-        for (int k = M - 1; k >= 0; --k)
-#pragma unroll
-            for (int c = k + 1; c < M; ++c) {
-                const bool s = piv[k] == c;
-#pragma unroll
-                for (int i = 0; i < M; ++i) {
-                    const double t = a[i][k];
-                    a[i][k] = s ? a[i][c] : t;
-                    a[i][c] = s ? t : a[i][c];
-                }
-            }
         double ymax = 0.0;
 #pragma unroll
         for (int i = 0; i < M; ++i)
@@ -248,12 +240,15 @@ __device__ __forceinline__ void same_spin_w(double* W, int nc, const LaneBlock<M
     for (int r = 0; r < M; ++r)
 #pragma unroll
         for (int r2 = r + 1; r2 < M; ++r2) {
-            const int row = compound_index(b.R[r], b.R[r2]) * nc;
+            // R is in pivot order: sort the ket pair, the minor changes sign with it
+            const bool up = b.R[r] < b.R[r2];
+            const int row = compound_index(up ? b.R[r] : b.R[r2], up ? b.R[r2] : b.R[r]) * nc;
+            const double cws = up ? cw : -cw;
 #pragma unroll
             for (int l = 0; l < M; ++l)
 #pragma unroll
                 for (int l2 = l + 1; l2 < M; ++l2) {
-                    const double v = cw * (b.a[l][r] * b.a[l2][r2] - b.a[l2][r] * b.a[l][r2]);
+                    const double v = cws * (b.a[l][r] * b.a[l2][r2] - b.a[l2][r] * b.a[l][r2]);
                     atomicAdd(&W[row + compound_index(b.L[l], b.L[l2])], v);
                 }
         }
