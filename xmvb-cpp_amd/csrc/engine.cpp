@@ -310,14 +310,40 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     T.ng = T.np * (T.np + 1) / 2;
     T.n_det = nd;
     T.n_str = ns;
+    const long long pairs = (long long)nd * (nd + 1) / 2;
+    // spin-flip partners: determinant (a, b) <-> the determinant with strings (b, a), when every
+    // determinant has one (equal electron counts and a flip-closed determinant set)
+    std::vector<int> flip, lane_pairs;
+    if (n_alpha == n_beta && pairs < (1LL << 31)) {
+        std::vector<std::pair<std::pair<uint64_t, uint64_t>, int>> key(nd);
+        for (int d = 0; d < nd; ++d) key[d] = {{t.alpha_masks[t.det_alpha[d]], t.beta_masks[t.det_beta[d]]}, d};
+        std::sort(key.begin(), key.end());
+        flip.resize(nd);
+        for (int d = 0; d < nd && !flip.empty(); ++d) {
+            const std::pair<uint64_t, uint64_t> want{t.beta_masks[t.det_beta[d]], t.alpha_masks[t.det_alpha[d]]};
+            auto it = std::lower_bound(key.begin(), key.end(), std::make_pair(want, -1));
+            if (it == key.end() || it->first != want) flip.clear();
+            else flip[d] = it->second;
+        }
+        if (!flip.empty()) {
+            lane_pairs.reserve((size_t)(pairs / 2 + nd));
+            for (long long e = 0; e < nd; ++e)
+                for (long long d = 0; d <= e; ++d) {
+                    const long long lo = std::min(flip[d], flip[e]), hi = std::max(flip[d], flip[e]);
+                    const long long tt = e * (e + 1) / 2 + d;
+                    if (tt <= hi * (hi + 1) / 2 + lo) lane_pairs.push_back((int)tt);
+                }
+        }
+    }
+    T.n_lane_pairs = flip.empty() ? pairs : (long long)lane_pairs.size();
     {
-        // a multiplier coprime with the pair count near pairs / golden ratio: lanes of one wavefront
+        // a multiplier coprime with the work count near count / golden ratio: lanes of one wavefront
         // then take pairs of different determinants, which spreads the backward kernels' LDS atomic
         // targets (consecutive pairs share their ket determinant and hit the same rows)
-        const long long pairs = (long long)nd * (nd + 1) / 2;
-        long long m = std::max<long long>(1, (long long)(0.6180339887498949 * (double)pairs)) | 1;
-        while (std::gcd(m, pairs) != 1) m += 2;
-        T.pair_mul = pairs > 1 && pairs < (1LL << 32) ? m : 1;  // the kernels form t * pair_mul in 64 bits
+        const long long work = T.n_lane_pairs;
+        long long m = std::max<long long>(1, (long long)(0.6180339887498949 * (double)work)) | 1;
+        while (std::gcd(m, work) != 1) m += 2;
+        T.pair_mul = work > 1 && work < (1LL << 32) ? m : 1;  // the kernels form i * pair_mul in 64 bits
     }
     auto* amask = A.alloc<uint64_t>(t.n_alpha_strings);
     auto* bmask = A.alloc<uint64_t>(t.n_beta_strings);
@@ -329,6 +355,8 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     auto* d_soff = A.alloc<long long>(ns + 1);
     auto* d_sdet = A.alloc<int>(n_terms);
     auto* d_scoef = A.alloc<double>(n_terms);
+    int* d_flip = flip.empty() ? nullptr : A.alloc<int>(nd);
+    int* d_lane_pairs = flip.empty() ? nullptr : A.alloc<int>(lane_pairs.size());
     upload(amask, t.alpha_masks, t.n_alpha_strings, st);
     upload(bmask, t.beta_masks, t.n_beta_strings, st);
     upload(det_a, t.det_alpha, nd, st);
@@ -339,6 +367,12 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     upload(d_soff, soff.data(), soff.size(), st);
     upload(d_sdet, sdet.data(), sdet.size(), st);
     upload(d_scoef, scoef.data(), scoef.size(), st);
+    if (d_flip) {
+        upload(d_flip, flip.data(), flip.size(), st);
+        upload(d_lane_pairs, lane_pairs.data(), lane_pairs.size(), st);
+    }
+    T.flip = d_flip;
+    T.lane_pairs = d_lane_pairs;
     T.amask = amask;
     T.bmask = bmask;
     T.det_a = det_a;
@@ -471,7 +505,7 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
     sh.n_beta = p->n_beta;
     sh.lane_fwd = xmvb_amd::pairs_lane_supported(p->n_alpha, p->n_beta, false);
     sh.lane_bwd = xmvb_amd::pairs_lane_supported(p->n_alpha, p->n_beta, true);
-    const long long lane_sweep = std::max<long long>(1, (pairs + 255) / 256);
+    const long long lane_sweep = std::max<long long>(1, (p->topo.n_lane_pairs + 255) / 256);
     sh.lane_blocks_fwd = (int)std::clamp<long long>((p->fwd_target + count - 1) / count, 1, lane_sweep);
     sh.lane_blocks_bwd = (int)std::clamp<long long>((p->bwd_target + count - 1) / count, 1,
                                                     std::min<long long>(lane_sweep, bwd_blocks_cap(p)));

@@ -68,9 +68,17 @@ struct TopoDesc {
     const int* ts;
     const double* tc;
     const long long* soff;  // structure -> determinants CSR
-    long long pair_mul;     // backward pair order: pair t is taken as (t * pair_mul) mod pairs (coprime)
+    long long pair_mul;     // backward lane order: work item i is taken as (i * pair_mul) mod n_lane_pairs (coprime)
     const int* sdet;
     const double* scoef;
+    // Spin-flip pairing (n_alpha == n_beta and a determinant set closed under exchanging the alpha
+    // and beta strings): flip[d] is that partner of determinant d. Pairs {d, e} and {flip d, flip e}
+    // have the same two blocks with the spins exchanged, hence the same H, S and the same adjoint
+    // contributions; the lane kernels evaluate the one with the smaller packed index (listed in
+    // lane_pairs) for both. flip == nullptr: every pair is evaluated (lane_pairs == nullptr).
+    const int* flip;
+    const int* lane_pairs;
+    long long n_lane_pairs;  // work items of the lane kernels
 };
 
 struct LaunchShape {

@@ -228,6 +228,25 @@ __device__ __forceinline__ SpinTables spin_tables(const TopoDesc& topo, int swap
     return t;
 }
 
+// Packed index of the spin-flip partner pair of {d, e} (t itself when the topology has no
+// flip pairing or the pair is its own partner); fd, fe: the partner determinants.
+__device__ __forceinline__ long long flip_partner(const TopoDesc& topo, long long t, int d, int e, int& fd, int& fe) {
+    if (!topo.flip) {
+        fd = d;
+        fe = e;
+        return t;
+    }
+    const int a = topo.flip[d], b = topo.flip[e];
+    fd = a < b ? a : b;
+    fe = a < b ? b : a;
+    return (long long)fe * (fe + 1) / 2 + fd;
+}
+
+// Work item i of the lane kernels -> packed pair index.
+__device__ __forceinline__ long long lane_pair(const TopoDesc& topo, long long i) {
+    return topo.lane_pairs ? (long long)topo.lane_pairs[i] : i;
+}
+
 // Same-spin adjoint of the kernel in compound space: every same-spin kernel term of a pair is
 // [(R_r L_l|R_r2 L_l2) - (R_r L_l2|R_r2 L_l)] times a 2x2 minor of D with r < r2, l < l2, so the
 // adjoint is carried by W[(p,s),(q,t)] (p = R_r < s = R_r2, q = L_l < t = L_l2), one atomic per
@@ -415,11 +434,13 @@ __global__ void __launch_bounds__(256) LANE_WAVES
     load_pair_tables(sd, n, np, G2, sact, hact);
     __syncthreads();
     const SpinTables tab = spin_tables(topo, swap_spins);
-    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_pairs;
-         t += (long long)gridDim.x * blockDim.x) {
-        int d, e;
+    const long long n_work = topo.n_lane_pairs;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_work;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long t = lane_pair(topo, i);
+        int d, e, fd, fe;
         unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
+        const long long tf = flip_partner(topo, t, d, e, fd, fe);
         LaneBlock<M0> b0;
         LaneBlock<M1> b1;
         occupied<M0>((uint32_t)tab.mask0[tab.det0[e]], b0.R);
@@ -428,9 +449,10 @@ __global__ void __launch_bounds__(256) LANE_WAVES
         occupied<M1>((uint32_t)tab.mask1[tab.det1[d]], b1.L);
         const bool ok0 = lane_invert<M0>(sact, n, cond_limit, b0);
         const bool ok1 = lane_invert<M1>(sact, n, cond_limit, b1);
-        if (!(ok0 && ok1)) {
-            const int slot = atomicAdd(&sd.flags[FLAG_DEFERRED], 1);
+        if (!(ok0 && ok1)) {  // the partner pair has the same blocks: deferred with it
+            const int slot = atomicAdd(&sd.flags[FLAG_DEFERRED], tf != t ? 2 : 1);
             sd.defer[slot] = (int)t;
+            if (tf != t) sd.defer[slot + 1] = (int)tf;
             continue;
         }
         // spin-1 packed indices are kept (the opposite-spin loop reuses each one M0^2 times);
@@ -471,6 +493,12 @@ __global__ void __launch_bounds__(256) LANE_WAVES
         sd.Hdet[(size_t)e + (size_t)n_det * d] = H;
         sd.Sdet[(size_t)d + (size_t)n_det * e] = c;
         sd.Sdet[(size_t)e + (size_t)n_det * d] = c;
+        if (tf != t) {
+            sd.Hdet[(size_t)fd + (size_t)n_det * fe] = H;
+            sd.Hdet[(size_t)fe + (size_t)n_det * fd] = H;
+            sd.Sdet[(size_t)fd + (size_t)n_det * fe] = c;
+            sd.Sdet[(size_t)fe + (size_t)n_det * fd] = c;
+        }
     }
 }
 
@@ -488,7 +516,7 @@ template <int M0, int M1>
 __global__ void __launch_bounds__(256) LANE_WAVES
     k_pairs_lane_bwd_f(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const int n = topo.n, ng = topo.ng, n_det = topo.n_det;
+    const int n = topo.n, ng = topo.ng;
     const int np = n * (n + 1) / 2;
     extern __shared__ double lds[];
     double* G2 = lds;
@@ -500,14 +528,16 @@ __global__ void __launch_bounds__(256) LANE_WAVES
     for (int t = threadIdx.x; t < 2 * n * n; t += blockDim.x) ds[t] = 0.0;
     __syncthreads();
     const SpinTables tab = spin_tables(topo, swap_spins);
-    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
+    const long long n_work = topo.n_lane_pairs;
     const double E = sd.energy[0];
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_pairs;
-         t += (long long)gridDim.x * blockDim.x) {
-        int d, e;
-        unpack_pair((long long)(((unsigned long long)t * (unsigned long long)topo.pair_mul) % (unsigned long long)n_pairs), d,
-                    e);  // d: left (bra) determinant, e: right (ket)
-        const double w = sd.dcoef[d] * sd.dcoef[e] * (d == e ? 1.0 : 2.0);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_work;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long t =
+            lane_pair(topo, (long long)(((unsigned long long)i * (unsigned long long)topo.pair_mul) % (unsigned long long)n_work));
+        int d, e, fd, fe;
+        unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
+        const long long tf = flip_partner(topo, t, d, e, fd, fe);
+        const double w = (sd.dcoef[d] * sd.dcoef[e] + (tf != t ? sd.dcoef[fd] * sd.dcoef[fe] : 0.0)) * (d == e ? 1.0 : 2.0);
         if (w == 0.0) continue;
         LaneBlock<M0> b0;
         LaneBlock<M1> b1;
@@ -590,7 +620,7 @@ __global__ void __launch_bounds__(256) LANE_WAVES
     k_pairs_lane_bwd_g(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit,
                        LaneBwdLayout lay) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const int n = topo.n, ng = topo.ng, n_det = topo.n_det;
+    const int n = topo.n, ng = topo.ng;
     const int np = n * (n + 1) / 2;
     extern __shared__ double lds[];
     double* sact = lds;
@@ -607,7 +637,7 @@ __global__ void __launch_bounds__(256) LANE_WAVES
     }
     __syncthreads();
     const SpinTables tab = spin_tables(topo, swap_spins);
-    const long long n_pairs = (long long)n_det * (n_det + 1) / 2;
+    const long long n_work = topo.n_lane_pairs;
     const int lane = threadIdx.x & 63;
     // wave-uniform trip count: the MFMA update needs every lane of the wavefront
     const long long first = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)));
@@ -621,14 +651,19 @@ __global__ void __launch_bounds__(256) LANE_WAVES
     for (int tp = 0; tp < TN; ++tp)
 #pragma unroll
         for (int tq = 0; tq < TN; ++tq) tiles[tp][tq] = lane_double4{0.0, 0.0, 0.0, 0.0};
-    for (long long tb = first; tb < n_pairs; tb += stride) {
-        const long long t = tb + lane;
+    for (long long tb = first; tb < n_work; tb += stride) {
+        const long long i = tb + lane;
         int d = 0, e = 0;
-        bool live = t < n_pairs;
-        if (live)
-            unpack_pair((long long)(((unsigned long long)t * (unsigned long long)topo.pair_mul) % (unsigned long long)n_pairs),
-                        d, e);  // d: left (bra) determinant, e: right (ket)
-        const double w = live ? sd.dcoef[d] * sd.dcoef[e] * (d == e ? 1.0 : 2.0) : 0.0;
+        bool live = i < n_work;
+        double w = 0.0;
+        if (live) {
+            const long long t = lane_pair(
+                topo, (long long)(((unsigned long long)i * (unsigned long long)topo.pair_mul) % (unsigned long long)n_work));
+            int fd, fe;
+            unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
+            const long long tf = flip_partner(topo, t, d, e, fd, fe);
+            w = (sd.dcoef[d] * sd.dcoef[e] + (tf != t ? sd.dcoef[fd] * sd.dcoef[fe] : 0.0)) * (d == e ? 1.0 : 2.0);
+        }
         live = live && w != 0.0;
         LaneBlock<M0> b0;
         LaneBlock<M1> b1;
