@@ -1500,12 +1500,94 @@ __global__ void __launch_bounds__(64) k_ri_pair_adjoint(const SlotDesc* __restri
 
 // gcn[mu, p] = sum_i SC[mu,i] (ds+ds^T)[i,p] + HC[mu,i] (dh+dh^T)[i,p]
 //            + sum_q f_pq sum_A dL[A,(p,q)] T[A,q,mu],  f_pq = 2 if p == q else 1.
-// grid (ceil(n_bf/4), 1, slots): a block owns four AO indices mu, so each dL element is read once
-// per four mu; NQ = n rounded up to even.
+// grid (ceil(n_bf / (4 R)), 1, slots), R = grad_wave_rows<NQ>() (batched launches): wavefront w of
+// a block owns the R AO indices mu0 + R w .. and its lanes take 64 consecutive aux indices A, so every T element
+// is read once, with unit stride across the lanes. The weights f_pq dL[A,(p,q)] of a 64-aux chunk
+// and QC values of q are staged in LDS as [q][p][a] (a = lane: no bank conflicts; at most 64 KB)
+// and serve the block's 4 R rows. NQ = n rounded up to even.
+template <int NQ>
+constexpr int grad_wave_rows() { return NQ <= 10 ? 8 : 4; }
+template <int NQ>
+constexpr int grad_q_chunk() { return NQ <= 10 ? NQ : NQ / 2; }
+constexpr unsigned kGradFewProblems = 16;  // up to this many problems per launch: k_orbital_gradient_few
+
+template <int NQ, int R>
+__global__ void __launch_bounds__(256) k_orbital_gradient(const SlotDesc* __restrict__ slots, int n, int ng) {
+    constexpr int QC = grad_q_chunk<NQ>();
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n_bf = sd.n_bf, n_aux = sd.n_aux;
+    const int mu_blk = blockIdx.x * 4 * R;
+    if (mu_blk >= n_bf) return;  // block-uniform
+    const double* __restrict__ T = sd.T;
+    const double* __restrict__ dL = sd.dL;
+    __shared__ double wq[QC * NQ * 64];  // [q - q0][p][a]
+    __shared__ double part[4 * R][NQ];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mu0 = mu_blk + wave * R;
+    const double* tmu[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) tmu[u] = T + (size_t)n_aux * n * min(mu0 + u, n_bf - 1);
+    double acc[R][NQ];
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+#pragma unroll
+        for (int p = 0; p < NQ; ++p) acc[u][p] = 0.0;
+    for (int A0 = 0; A0 < n_aux; A0 += 64)
+    for (int q0 = 0; q0 < n; q0 += QC) {
+        const int nq = min(QC, n - q0);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < nq * n * 64; idx += blockDim.x) {
+            const int a = idx & 63, qp = idx >> 6, qq = qp / n, p = qp - qq * n, q = q0 + qq;
+            const int A = A0 + a;
+            wq[(qq * NQ + p) * 64 + a] =
+                A < n_aux ? (p == q ? 2.0 : 1.0) * dL[(size_t)A + (size_t)n_aux * pair_index(p, q)] : 0.0;
+        }
+        __syncthreads();
+        const int A = min(A0 + lane, n_aux - 1);  // past n_aux the weights are zero
+        for (int qq = 0; qq < nq; ++qq) {
+            const int q = q0 + qq;
+            double t[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u) t[u] = tmu[u][(size_t)n_aux * q + A];
+            const double* w = wq + qq * NQ * 64 + lane;
+#pragma unroll
+            for (int p = 0; p < NQ; ++p) {
+                if (p >= n) continue;
+                const double wp = w[p * 64];
+#pragma unroll
+                for (int u = 0; u < R; ++u) acc[u][p] += wp * t[u];
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+#pragma unroll
+        for (int p = 0; p < NQ; ++p) {
+            const double s = wave_sum(acc[u][p]);
+            if (lane == 0) part[wave * R + u][p] = s;
+        }
+    __syncthreads();
+    const double* ds = sd.dsum + ng;
+    const double* dh = ds + n * n;
+    for (int k = threadIdx.x; k < 4 * R * n; k += blockDim.x) {
+        const int u = k / n, p = k - u * n, mu = mu_blk + u;
+        if (mu >= n_bf) continue;
+        double g = part[u][p];
+        for (int i = 0; i < n; ++i) {
+            g += sd.SC[mu + (size_t)n_bf * i] * (ds[i + n * p] + ds[p + n * i]);
+            g += sd.HC[mu + (size_t)n_bf * i] * (dh[i + n * p] + dh[p + n * i]);
+        }
+        sd.gcn[mu + (size_t)n_bf * p] = g;
+    }
+}
+
+// The same for launches of few problems, where blocks are scarce: grid (ceil(n_bf/4), 1, slots), a
+// block owns four AO indices mu and its threads run over A with no staging (each dL element is
+// read once per four mu).
 constexpr int kGradRows = 4;
 
 template <int NQ>
-__global__ void __launch_bounds__(256) k_orbital_gradient(const SlotDesc* __restrict__ slots, int n, int ng) {
+__global__ void __launch_bounds__(256) k_orbital_gradient_few(const SlotDesc* __restrict__ slots, int n, int ng) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
     const int mu0 = blockIdx.x * kGradRows;
@@ -1815,15 +1897,26 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
     mark(EV_PAIR_BWD);
     hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((shape.max_n_aux + 63) / 64, (np + 7) / 8, z), dim3(64), 0, st, slots, np);
     {
-        const dim3 grid((shape.max_n_bf + kGradRows - 1) / kGradRows, 1, z);
+        auto grad = [&]<int NQ>() {
+            if (z <= kGradFewProblems) {
+                hipLaunchKernelGGL(k_orbital_gradient_few<NQ>,
+                                   dim3((unsigned)((shape.max_n_bf + kGradRows - 1) / kGradRows), 1, z), dim3(256), 0,
+                                   st, slots, n, ng);
+            } else {
+                const int rows = 4 * grad_wave_rows<NQ>();
+                hipLaunchKernelGGL((k_orbital_gradient<NQ, grad_wave_rows<NQ>()>),
+                                   dim3((unsigned)((shape.max_n_bf + rows - 1) / rows), 1, z), dim3(256), 0, st, slots,
+                                   n, ng);
+            }
+        };
         switch ((n + 1) / 2) {
-            case 1: hipLaunchKernelGGL(k_orbital_gradient<2>, grid, dim3(256), 0, st, slots, n, ng); break;
-            case 2: hipLaunchKernelGGL(k_orbital_gradient<4>, grid, dim3(256), 0, st, slots, n, ng); break;
-            case 3: hipLaunchKernelGGL(k_orbital_gradient<6>, grid, dim3(256), 0, st, slots, n, ng); break;
-            case 4: hipLaunchKernelGGL(k_orbital_gradient<8>, grid, dim3(256), 0, st, slots, n, ng); break;
-            case 5: hipLaunchKernelGGL(k_orbital_gradient<10>, grid, dim3(256), 0, st, slots, n, ng); break;
-            case 6: hipLaunchKernelGGL(k_orbital_gradient<12>, grid, dim3(256), 0, st, slots, n, ng); break;
-            default: hipLaunchKernelGGL(k_orbital_gradient<14>, grid, dim3(256), 0, st, slots, n, ng); break;
+            case 1: grad.template operator()<2>(); break;
+            case 2: grad.template operator()<4>(); break;
+            case 3: grad.template operator()<6>(); break;
+            case 4: grad.template operator()<8>(); break;
+            case 5: grad.template operator()<10>(); break;
+            case 6: grad.template operator()<12>(); break;
+            default: grad.template operator()<14>(); break;
         }
     }
     hipLaunchKernelGGL(k_normalization_pullback, dim3(n, 1, z), dim3(256), 0, st, slots);
