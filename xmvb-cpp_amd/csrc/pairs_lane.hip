@@ -25,9 +25,12 @@
 
 namespace xmvb_amd {
 
-#ifndef LANE_WAVES
-#define LANE_WAVES
-#endif
+// Wavefronts per SIMD the compiler must fit (register budget 512 / w per lane). Two for the
+// forward up to 6-electron blocks and for the field kernel up to 5: measured faster than one
+// wavefront with more registers (medium batch: forward 2.66 -> 2.22 ms, field 4.92 -> 3.82 ms);
+// the 6-electron field kernel and every adjoint kernel are slower when held to two.
+constexpr int lane_fwd_waves(int m0) { return m0 <= 6 ? 2 : 1; }
+constexpr int lane_field_waves(int m0) { return m0 <= 5 ? 2 : 1; }
 
 template <int M>
 struct LaneBlock {
@@ -422,7 +425,7 @@ __device__ __forceinline__ void load_pair_tables(const SlotDesc& sd, int n, int 
 
 // Forward. LDS: full np x np kernel G2, S_act, h_act.
 template <int M0, int M1>
-__global__ void __launch_bounds__(256) LANE_WAVES
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_fwd_waves(M0))))
     k_pairs_lane_fwd(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n = topo.n, n_det = topo.n_det;
@@ -513,7 +516,7 @@ __global__ void __launch_bounds__(256) LANE_WAVES
 
 // LDS: G2, S_act, h_act, ds, dh.
 template <int M0, int M1>
-__global__ void __launch_bounds__(256) LANE_WAVES
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_field_waves(M0))))
     k_pairs_lane_bwd_f(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n = topo.n, ng = topo.ng;
@@ -616,7 +619,7 @@ __global__ void __launch_bounds__(256) LANE_WAVES
 // (per wavefront 2 x 16 x np16).
 // Fast: W and the MFMA update (when both fit in LDS); otherwise direct LDS atomics into dg.
 template <bool FullDg, bool Fast, int M0, int M1>
-__global__ void __launch_bounds__(256) LANE_WAVES
+__global__ void __launch_bounds__(256)
     k_pairs_lane_bwd_g(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit,
                        LaneBwdLayout lay) {
     const SlotDesc& sd = slots[blockIdx.z];
