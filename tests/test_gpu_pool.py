@@ -139,3 +139,24 @@ def test_large_structure_space_batched_matches_alone(ctx):
         assert abs(e_all[s] - e1[0]) <= 1e-9
         assert np.abs(g_all[s] - g1[0]).max() <= 1e-8
     pool.close()
+
+
+@pytest.mark.parametrize("n_act,n_bf", [(10, 120), (12, 60), (14, 40)])
+def test_batched_tail_kernels_match_single(ctx, n_act, n_bf):
+    """More than 16 problems per launch switch the Gram product of the active factors and the
+    orbital-gradient contraction to their batched (LDS-tiled) kernels; their results must equal
+    the one-problem kernels' up to summation order (n_act 12 and 14 take the q-chunked gradient
+    staging)."""
+    topo = vb.Topology.build("COV", n_act, n_act, 1) if n_act <= 12 else vb.Topology.build("COV", n_act, 2, 1)
+    K = 20
+    pool = vb.Pool(ctx, topo, capacity=K)
+    sps = [vb.synthetic_problem(400 + s, n_bf + (s % 3), n_act, 3 * n_bf) for s in range(K)]
+    for s, sp in enumerate(sps):
+        pool.load_synthetic(s, sp)
+    print(f"case batched tail n_act={n_act} K={K}", flush=True)
+    e_all, g_all = pool.evaluate(list(range(K)), [sp.C for sp in sps])
+    for s in (0, 7, 19):
+        e1, g1 = pool.evaluate([s], [sps[s].C])
+        assert abs(e_all[s] - e1[0]) <= 1e-10 * max(1.0, abs(e1[0]))
+        assert np.abs(g_all[s] - g1[0]).max() <= 1e-9
+    pool.close()

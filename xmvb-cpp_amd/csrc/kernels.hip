@@ -344,6 +344,63 @@ __global__ void k_ri_pair_gram(const SlotDesc* __restrict__ slots, int np) {
     if (lane == 0) G[wave] = acc;
 }
 
+// The same for batched launches: one block per problem; L is staged in LDS by chunks of 64 aux
+// rows as [a][P] (row length kGramLd, zeros past np), and each thread accumulates 2x2 tiles
+// (P0, P0+1) x (Q0, Q0+1), P0 >= Q0, over the chunks, so each L element is read from memory once.
+constexpr int kGramChunk = 64;
+constexpr int kGramLd = MAX_ACT * (MAX_ACT + 1) / 2 + 3;  // np <= 105: room for the 2-wide tiles
+constexpr int kGramTilesPerThread = 6;                    // 53 * 54 / 2 tile pairs at np = 105
+constexpr unsigned kGramFewProblems = 16;                 // up to this many problems: k_ri_pair_gram
+
+__global__ void __launch_bounds__(256) k_ri_pair_gram_tiled(const SlotDesc* __restrict__ slots, int np) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n_aux = sd.n_aux;
+    const double* __restrict__ L = sd.L;
+    __shared__ double Ls[kGramChunk * kGramLd];
+    const int nt = (np + 1) / 2, ntp = nt * (nt + 1) / 2;
+    int p0[kGramTilesPerThread], q0[kGramTilesPerThread];
+    double acc[kGramTilesPerThread][4];
+#pragma unroll
+    for (int k = 0; k < kGramTilesPerThread; ++k) {
+        const int t = threadIdx.x + 256 * k;
+        int tq = 0, tp = 0;
+        if (t < ntp) unpack_pair(t, tq, tp);  // tq <= tp
+        p0[k] = 2 * tp;
+        q0[k] = 2 * tq;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[k][j] = 0.0;
+    }
+    for (int A0 = 0; A0 < n_aux; A0 += kGramChunk) {
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < kGramChunk * kGramLd; idx += blockDim.x) {
+            const int a = idx % kGramChunk, P = idx / kGramChunk, A = A0 + a;
+            Ls[a * kGramLd + P] = (A < n_aux && P < np) ? L[(size_t)A + (size_t)n_aux * P] : 0.0;
+        }
+        __syncthreads();
+        for (int a = 0; a < kGramChunk; ++a) {
+            const double* row = Ls + a * kGramLd;
+#pragma unroll
+            for (int k = 0; k < kGramTilesPerThread; ++k) {
+                if (threadIdx.x + 256 * k >= ntp) break;  // later k are past the tile count too
+                const double x0 = row[p0[k]], x1 = row[p0[k] + 1], y0 = row[q0[k]], y1 = row[q0[k] + 1];
+                acc[k][0] += x0 * y0;
+                acc[k][1] += x0 * y1;
+                acc[k][2] += x1 * y0;
+                acc[k][3] += x1 * y1;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kGramTilesPerThread; ++k) {
+        if (threadIdx.x + 256 * k >= ntp) break;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int P = p0[k] + (j >> 1), Q = q0[k] + (j & 1);
+            if (P < np && Q <= P) sd.G[pair_index(P, Q)] = acc[k][j];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // determinant pairs: one wavefront per unordered determinant pair (d <= e)
 // ---------------------------------------------------------------------------------------------
@@ -1805,7 +1862,10 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
             default: hipLaunchKernelGGL(k_ri_pair_factors<14>, grid, dim3(256), 0, st, slots, n); break;
         }
     }
-    hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4, 1, z), dim3(256), 0, st, slots, np);
+    if (z <= kGramFewProblems)
+        hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4, 1, z), dim3(256), 0, st, slots, np);
+    else
+        hipLaunchKernelGGL(k_ri_pair_gram_tiled, dim3(1, 1, z), dim3(256), 0, st, slots, np);
     mark(EV_TRANSFORM);
     if (shape.lane_fwd) {
         launch_pairs_lane(st, 0, shape.lane_blocks_fwd, slots, count, topo, shape.n_alpha, shape.n_beta,
