@@ -987,11 +987,12 @@ __global__ void __launch_bounds__(256) k_gep_lul(const SlotDesc* __restrict__ sl
     A[i + (size_t)n * j] = s0 + s1;
 }
 
-// Per-step Cholesky of the large-n path for launches of few problems: k_gep_load symmetrises S
-// and H into the workspace (many blocks), k_chol_scale (one block) takes column k
-// (L[k][k] = sqrt(a), L[i][k] /= L[k][k]; a <= 0 sets the metric flag and the solve goes on with
-// values the host discards), k_chol_update (four columns per block, rows on consecutive lanes)
-// applies L[i][j] -= L[i][k] L[j][k] for k < j <= i.
+// Blocked Cholesky of the large-n path for launches of few problems: k_gep_load symmetrises S
+// and H into the workspace (many blocks); then per panel of kCholPanel columns j0.. j0+jb-1:
+// k_chol_diag factors the diagonal tile in LDS (one block; a pivot a <= 0 sets the metric flag
+// and the solve goes on with values the host discards), k_chol_trsm solves the rows below it
+// against the tile (one thread per row), k_chol_syrk subtracts the panel's product from the
+// trailing lower triangle (16 x 16 tiles). About 3 n / kCholPanel launches instead of 2 n.
 __global__ void __launch_bounds__(256) k_gep_load(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
     double* __restrict__ Lm = sd.gep;
@@ -1003,24 +1004,84 @@ __global__ void __launch_bounds__(256) k_gep_load(const SlotDesc* __restrict__ s
     M[idx] = 0.5 * (sd.Hs[i + (size_t)n * j] + sd.Hs[j + (size_t)n * i]);
 }
 
-__global__ void __launch_bounds__(256) k_chol_scale(const SlotDesc* __restrict__ slots, int n, int k) {
+constexpr int kCholPanel = 32;
+
+__global__ void __launch_bounds__(256) k_chol_diag(const SlotDesc* __restrict__ slots, int n, int j0, int jb) {
     const SlotDesc& sd = slots[blockIdx.z];
     double* __restrict__ L = sd.gep;
-    const double a = L[k + (size_t)n * k];
-    if (!(a > 0.0) && threadIdx.x == 0) sd.flags[FLAG_METRIC] = 1;
-    const double dk = sqrt(a), rdk = 1.0 / dk;
-    __syncthreads();  // every thread has read the diagonal before it is replaced
-    for (int i = k + threadIdx.x; i < n; i += blockDim.x)
-        L[i + (size_t)n * k] = (i == k) ? dk : L[i + (size_t)n * k] * rdk;
+    __shared__ double t[kCholPanel][kCholPanel + 1];  // t[i][j], lower triangle
+    for (int idx = threadIdx.x; idx < jb * jb; idx += blockDim.x) {
+        const int i = idx % jb, j = idx / jb;
+        t[i][j] = L[(size_t)(j0 + i) + (size_t)n * (j0 + j)];
+    }
+    __syncthreads();
+    for (int k = 0; k < jb; ++k) {
+        const double a = t[k][k];
+        if (!(a > 0.0) && threadIdx.x == 0) sd.flags[FLAG_METRIC] = 1;
+        const double dk = sqrt(a), rdk = 1.0 / dk;
+        __syncthreads();  // every thread has read the pivot before it is replaced
+        for (int i = k + threadIdx.x; i < jb; i += blockDim.x) t[i][k] = (i == k) ? dk : t[i][k] * rdk;
+        __syncthreads();
+        const int m = jb - k - 1;
+        for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
+            const int i = k + 1 + idx % m, j = k + 1 + idx / m;
+            if (j <= i) t[i][j] -= t[i][k] * t[j][k];
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < jb * jb; idx += blockDim.x) {
+        const int i = idx % jb, j = idx / jb;
+        if (j <= i) L[(size_t)(j0 + i) + (size_t)n * (j0 + j)] = t[i][j];
+    }
 }
 
-__global__ void __launch_bounds__(256) k_chol_update(const SlotDesc* __restrict__ slots, int n, int k) {
+// rows i >= j0 + jb: L[i][j0..j0+jb) = A[i][j0..j0+jb) L_tile^-T (forward substitution per row)
+__global__ void __launch_bounds__(256) k_chol_trsm(const SlotDesc* __restrict__ slots, int n, int j0, int jb) {
     const SlotDesc& sd = slots[blockIdx.z];
     double* __restrict__ L = sd.gep;
-    const int j = k + 1 + (int)blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (j >= n) return;  // wave-uniform
-    const double ljk = L[j + (size_t)n * k];
-    for (int i = j + lane; i < n; i += 64) L[i + (size_t)n * j] -= L[i + (size_t)n * k] * ljk;
+    __shared__ double t[kCholPanel][kCholPanel + 1];
+    for (int idx = threadIdx.x; idx < jb * jb; idx += blockDim.x) {
+        const int i = idx % jb, j = idx / jb;
+        t[i][j] = j <= i ? L[(size_t)(j0 + i) + (size_t)n * (j0 + j)] : 0.0;
+    }
+    __syncthreads();
+    const int i = j0 + jb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x[kCholPanel];
+#pragma unroll
+    for (int j = 0; j < kCholPanel; ++j) {
+        if (j < jb) {
+            double v = L[(size_t)i + (size_t)n * (j0 + j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) v -= x[k] * t[j][k];
+            x[j] = v / t[j][j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kCholPanel; ++j)
+        if (j < jb) L[(size_t)i + (size_t)n * (j0 + j)] = x[j];
+}
+
+// trailing update: A[i][j] -= sum_k L[i][j0+k] L[j][j0+k] for j0 + jb <= j <= i < n, 16 x 16 tiles
+__global__ void __launch_bounds__(256) k_chol_syrk(const SlotDesc* __restrict__ slots, int n, int j0, int jb) {
+    const int ti = blockIdx.x, tj = blockIdx.y;
+    if (tj > ti) return;  // block-uniform: tiles above the diagonal
+    const SlotDesc& sd = slots[blockIdx.z];
+    double* __restrict__ L = sd.gep;
+    __shared__ double li[16][kCholPanel + 1], lj[16][kCholPanel + 1];
+    const int r0 = j0 + jb + 16 * ti, c0 = j0 + jb + 16 * tj;
+    for (int idx = threadIdx.x; idx < 16 * jb; idx += blockDim.x) {
+        const int r = idx % 16, k = idx / 16;
+        li[r][k] = r0 + r < n ? L[(size_t)(r0 + r) + (size_t)n * (j0 + k)] : 0.0;
+        lj[r][k] = c0 + r < n ? L[(size_t)(c0 + r) + (size_t)n * (j0 + k)] : 0.0;
+    }
+    __syncthreads();
+    const int r = threadIdx.x % 16, c = threadIdx.x / 16;
+    const int i = r0 + r, j = c0 + c;
+    if (i >= n || j > i) return;
+    double acc = 0.0;
+    for (int k = 0; k < jb; ++k) acc += li[r][k] * lj[c][k];
+    L[(size_t)i + (size_t)n * j] -= acc;
 }
 
 // Householder step k of the large-n path, over many blocks. k_hh_matvec: every block forms
@@ -1897,11 +1958,15 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
         if (per_step && ns >= kCholPerStepMinN) {
             hipLaunchKernelGGL(k_gep_load, dim3((unsigned)(((long long)ns * ns + 255) / 256), 1, z), dim3(256), 0, st,
                                slots, ns);
-            for (int k = 0; k < ns; ++k) {
-                hipLaunchKernelGGL(k_chol_scale, dim3(1, 1, z), dim3(256), 0, st, slots, ns, k);
-                if (k + 1 < ns)
-                    hipLaunchKernelGGL(k_chol_update, dim3((unsigned)((ns - (k + 1) + 3) / 4), 1, z), dim3(256), 0,
-                                       st, slots, ns, k);
+            for (int j0 = 0; j0 < ns; j0 += kCholPanel) {
+                const int jb = std::min(kCholPanel, ns - j0), rest = ns - j0 - jb;
+                hipLaunchKernelGGL(k_chol_diag, dim3(1, 1, z), dim3(256), 0, st, slots, ns, j0, jb);
+                if (rest > 0) {
+                    hipLaunchKernelGGL(k_chol_trsm, dim3((unsigned)((rest + 255) / 256), 1, z), dim3(256), 0, st, slots,
+                                       ns, j0, jb);
+                    const unsigned tiles = (unsigned)((rest + 15) / 16);
+                    hipLaunchKernelGGL(k_chol_syrk, dim3(tiles, tiles, z), dim3(256), 0, st, slots, ns, j0, jb);
+                }
             }
         } else {
             hipLaunchKernelGGL(k_gep_chol, dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
