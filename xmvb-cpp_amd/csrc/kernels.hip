@@ -1084,83 +1084,129 @@ __global__ void __launch_bounds__(256) k_chol_syrk(const SlotDesc* __restrict__ 
     L[(size_t)i + (size_t)n * j] -= acc;
 }
 
-// Householder step k of the large-n path, over many blocks. k_hh_matvec: every block forms
-// v_k from column k of A itself (block 0 stores it and beta_k), then p = beta_k A v for four rows
-// per block (one wavefront per row; A symmetric, so row i is read as column i). k_hh_update:
-// K = beta_k v^T p / 2, w = p - K v, and A -= v w^T + w v^T on four columns per block; block 0
-// writes the reduced column and row k. Workspace as in k_gep_lowest (y holds p, then w).
-__global__ void __launch_bounds__(256) k_hh_matvec(const SlotDesc* __restrict__ slots, int n, int k) {
+// Householder step k with the previous step's rank-2 update applied lazily, one launch per step.
+// Step k - 1 left p_{k-1} = beta_{k-1} A_{k-1} v_{k-1} (in pbuf[(k-1) & 1]); every block forms
+// w_{k-1} = p_{k-1} - K v_{k-1} (K = beta_{k-1} v_{k-1}^T p_{k-1} / 2) and v_k from column k of
+// A_k = A_{k-1} - v w^T - w v^T (corrected on the fly; column k itself is never written back).
+// Wavefront i (i > k) then reads column i once: applies the step k-1 update to rows >= k+1, writes
+// it back and accumulates p_i = beta_k (A_k v_k)_i. Block 0 stores v_k, beta_k, alpha_k, the
+// corrected diagonal A[k][k], and the reduced column and row k - 1 (alpha_{k-1} on the
+// subdiagonal, zeros below), which no block reads during this step. Each column is read and
+// written by its own wavefront only. k_hh_finish applies the last step to the trailing 2 x 2
+// block and writes the last reduced column. Workspace: M, V as in k_gep_lowest; p in y / tdl by
+// parity of k, alpha_k in tdu.
+__device__ __forceinline__ void hh_pending(const double* __restrict__ V, const double* __restrict__ pbuf,
+                                           const double* __restrict__ beta, int n, int k, double* vprev,
+                                           double* wprev, double* red) {
+    // v_{k-1} and w_{k-1} into LDS (zeros above row k and for k == 0)
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double part = 0.0;
+    for (int i = tid; i < n; i += nt) {
+        const double v = (k > 0 && i >= k) ? V[i + (size_t)n * (k - 1)] : 0.0;
+        vprev[i] = v;
+        if (k > 0 && i >= k) part += v * pbuf[i];
+    }
+    const double K = k > 0 ? 0.5 * beta[k - 1] * block_sum(part, red) : 0.0;
+    for (int i = tid; i < n; i += nt) wprev[i] = (k > 0 && i >= k) ? pbuf[i] - K * vprev[i] : 0.0;
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_hh_step(const SlotDesc* __restrict__ slots, int n, int k) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ M = sd.gep + (size_t)n * n;
+    double* __restrict__ M = sd.gep + (size_t)n * n;
     double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
-    double* __restrict__ y = sd.gep + 3 * (size_t)n * n + 2 * (size_t)n;
-    double* __restrict__ beta = y + n;
-    extern __shared__ double vloc[];  // n
+    double* __restrict__ vec = sd.gep + 3 * (size_t)n * n;  // dd, ee, y, beta, tdl, tdu, ...
+    double* __restrict__ beta = vec + 3 * (size_t)n;
+    double* __restrict__ alphas = vec + 5 * (size_t)n;
+    const double* __restrict__ pprev = vec + ((k - 1) & 1 ? 4 : 2) * (size_t)n;
+    double* __restrict__ pnow = vec + (k & 1 ? 4 : 2) * (size_t)n;
+    extern __shared__ double loc[];  // v_k, v_{k-1}, w_{k-1}: 3 n
+    double* vloc = loc;
+    double* vprev = loc + n;
+    double* wprev = loc + 2 * n;
     __shared__ double red[16];
     const int m0 = k + 1, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
+    hh_pending(V, pprev, beta, n, k, vprev, wprev, red);
+    const double vk = vprev[k], wk = wprev[k];
     double part = 0.0;
     for (int i = m0 + tid; i < n; i += nt) {
-        const double x = M[i + (size_t)n * k];
+        const double x = M[i + (size_t)n * k] - vprev[i] * wk - wprev[i] * vk;
+        vloc[i] = x;
         part += x * x;
     }
-    const double sigma = block_sum(part, red);
-    const double x0 = M[m0 + (size_t)n * k];
+    const double sigma = block_sum(part, red);  // ends with a barrier: vloc is complete
+    const double x0 = vloc[m0];
     const double alpha = sigma > 0.0 ? (x0 >= 0.0 ? -sqrt(sigma) : sqrt(sigma)) : 0.0;
     const double vtv = sigma - 2.0 * alpha * x0 + alpha * alpha;
     const double bk = (sigma > 0.0 && vtv > 0.0) ? 2.0 / vtv : 0.0;
-    for (int i = tid; i < n; i += nt) vloc[i] = i >= m0 ? M[i + (size_t)n * k] - (i == m0 ? alpha : 0.0) : 0.0;
+    __syncthreads();  // every thread has read x0 before it becomes v_m0
+    if (tid == 0) vloc[m0] = x0 - alpha;
+    for (int i = tid; i < m0; i += nt) vloc[i] = 0.0;
     __syncthreads();
     if (blockIdx.x == 0) {
         for (int i = m0 + tid; i < n; i += nt) V[i + (size_t)n * k] = vloc[i];
-        if (tid == 0) beta[k] = bk;
+        if (tid == 0) {
+            beta[k] = bk;
+            alphas[k] = alpha;
+            M[k + (size_t)n * k] -= 2.0 * vk * wk;
+        }
+        if (k > 0)
+            for (int i = k + tid; i < n; i += nt) {
+                const double e = (i == k) ? alphas[k - 1] : 0.0;
+                M[i + (size_t)n * (k - 1)] = e;
+                M[(k - 1) + (size_t)n * i] = e;
+            }
     }
     const int i = m0 + (int)blockIdx.x * 4 + wave;
-    if (i < n) {  // wave-uniform
+    if (i < n) {  // wave-uniform: column i, rows j >= m0 on consecutive lanes
+        const double vi = vprev[i], wi = wprev[i];
         double a = 0.0;
-        for (int j = m0 + lane; j < n; j += 64) a += M[j + (size_t)n * i] * vloc[j];
+        for (int j = m0 + lane; j < n; j += 64) {
+            const double mji = M[j + (size_t)n * i] - vprev[j] * wi - wprev[j] * vi;
+            M[j + (size_t)n * i] = mji;
+            a += mji * vloc[j];
+        }
         a = wave_sum(a);
-        if (lane == 0) y[i] = bk * a;
+        if (lane == 0) pnow[i] = bk * a;
     }
 }
 
-__global__ void __launch_bounds__(256) k_hh_update(const SlotDesc* __restrict__ slots, int n, int k) {
+__global__ void __launch_bounds__(256) k_hh_finish(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
     double* __restrict__ M = sd.gep + (size_t)n * n;
     const double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
-    const double* __restrict__ y = sd.gep + 3 * (size_t)n * n + 2 * (size_t)n;
-    const double bk = y[n + k];  // beta[k]
-    extern __shared__ double loc[];  // v (n), w (n)
-    double* vloc = loc;
-    double* wloc = loc + n;
+    double* __restrict__ vec = sd.gep + 3 * (size_t)n * n;
+    const double* __restrict__ beta = vec + 3 * (size_t)n;
+    const double* __restrict__ alphas = vec + 5 * (size_t)n;
+    const int k = n - 2;  // the step whose update is pending is k - 1 = n - 3
+    const double* __restrict__ pprev = vec + ((k - 1) & 1 ? 4 : 2) * (size_t)n;
+    extern __shared__ double loc[];  // v_{k-1}, w_{k-1}: 2 n
+    double* vprev = loc;
+    double* wprev = loc + n;
     __shared__ double red[16];
-    const int m0 = k + 1, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
-    double part = 0.0;
-    for (int i = m0 + tid; i < n; i += nt) part += V[i + (size_t)n * k] * y[i];
-    const double K = 0.5 * bk * block_sum(part, red);
-    const double alpha = M[m0 + (size_t)n * k] - V[m0 + (size_t)n * k];  // v_m0 = x0 - alpha
-    for (int i = tid; i < n; i += nt) {
-        const double v = i >= m0 ? V[i + (size_t)n * k] : 0.0;
-        vloc[i] = v;
-        wloc[i] = i >= m0 ? y[i] - K * v : 0.0;
+    hh_pending(V, pprev, beta, n, k, vprev, wprev, red);
+    const int tid = threadIdx.x;
+    if (tid < 4) {  // the trailing 2 x 2 block (rows and columns n-2, n-1)
+        const int j = k + (tid & 1), c = k + (tid >> 1);
+        M[j + (size_t)n * c] -= vprev[j] * wprev[c] + wprev[j] * vprev[c];
     }
-    __syncthreads();
-    const int c = m0 + (int)blockIdx.x * 4 + wave;
-    if (c < n) {  // wave-uniform: column c, rows j >= m0 on consecutive lanes
-        const double vc = vloc[c], wc = wloc[c];
-        for (int j = m0 + lane; j < n; j += 64) M[j + (size_t)n * c] -= vloc[j] * wc + wloc[j] * vc;
-    }
-    if (blockIdx.x == 0 && bk != 0.0)
-        for (int i = m0 + tid; i < n; i += nt) {
-            const double e = (i == m0) ? alpha : 0.0;
-            M[i + (size_t)n * k] = e;
-            M[k + (size_t)n * i] = e;
+    for (int i = k + tid; i < n; i += blockDim.x) {
+        const double e = (i == k) ? alphas[k - 1] : 0.0;
+        if (i > k) {
+            M[i + (size_t)n * (k - 1)] = e;
+            M[(k - 1) + (size_t)n * i] = e;
         }
+    }
+    if (tid == 0) {
+        M[k + (size_t)n * (k - 1)] = alphas[k - 1];
+        M[(k - 1) + (size_t)n * k] = alphas[k - 1];
+    }
 }
 
 // Part 0: the whole solve (matrices in LDS). Large-n path (matrices in global memory, after the
 // multi-workgroup front half): Part 3 is the rest on one workgroup per problem (batched launches);
 // for launches of few problems, Part 1 is the symmetrisation and Part 2 the rest after the
-// per-step Householder kernels.
+// per-step Householder kernels (k_hh_step, k_hh_finish).
 template <bool InLds, int Part = 0>
 __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDesc* __restrict__ slots, TopoDesc topo) {
     const SlotDesc& sd = slots[blockIdx.z];
@@ -1978,11 +2024,10 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
             hipLaunchKernelGGL((k_gep_lowest<false, 1>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
             for (int k = 0; k + 2 < ns; ++k) {
                 const unsigned rows = (unsigned)((ns - (k + 1) + 3) / 4);
-                hipLaunchKernelGGL(k_hh_matvec, dim3(rows, 1, z), dim3(256), (size_t)ns * sizeof(double), st, slots,
+                hipLaunchKernelGGL(k_hh_step, dim3(rows, 1, z), dim3(256), 3 * (size_t)ns * sizeof(double), st, slots,
                                    ns, k);
-                hipLaunchKernelGGL(k_hh_update, dim3(rows, 1, z), dim3(256), 2 * (size_t)ns * sizeof(double), st,
-                                   slots, ns, k);
             }
+            hipLaunchKernelGGL(k_hh_finish, dim3(1, 1, z), dim3(256), 2 * (size_t)ns * sizeof(double), st, slots, ns);
             hipLaunchKernelGGL((k_gep_lowest<false, 2>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
         } else {
             hipLaunchKernelGGL((k_gep_lowest<false, 3>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
