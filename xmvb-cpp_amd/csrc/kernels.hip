@@ -1675,9 +1675,9 @@ template <int NQ>
 constexpr int grad_q_chunk() { return NQ <= 10 ? NQ : NQ / 2; }
 constexpr unsigned kGradFewProblems = 16;  // up to this many problems per launch: k_orbital_gradient_few
 
-template <int NQ, int R>
+template <int NQ>
 __global__ void __launch_bounds__(256) k_orbital_gradient(const SlotDesc* __restrict__ slots, int n, int ng) {
-    constexpr int QC = grad_q_chunk<NQ>();
+    constexpr int R = grad_wave_rows<NQ>(), QC = grad_q_chunk<NQ>();
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
     const int mu_blk = blockIdx.x * 4 * R;
@@ -2074,7 +2074,7 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
                                    st, slots, n, ng);
             } else {
                 const int rows = 4 * grad_wave_rows<NQ>();
-                hipLaunchKernelGGL((k_orbital_gradient<NQ, grad_wave_rows<NQ>()>),
+                hipLaunchKernelGGL(k_orbital_gradient<NQ>,
                                    dim3((unsigned)((shape.max_n_bf + rows - 1) / rows), 1, z), dim3(256), 0, st, slots,
                                    n, ng);
             }
