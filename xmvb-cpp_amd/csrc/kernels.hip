@@ -1634,27 +1634,31 @@ __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDes
 // ---------------------------------------------------------------------------------------------
 
 // dL[A,P] = sum_Q M[P,Q] L[A,Q], M[P,Q] = dg(P,Q) off the diagonal, 2 dg(P,P) on it.
-// grid (ceil(n_aux/64), ceil(np/8), slots): a thread owns one aux row and eight P, so each L
-// element is read np/8 times instead of np times; M entries are wave-uniform.
-__global__ void __launch_bounds__(64) k_ri_pair_adjoint(const SlotDesc* __restrict__ slots, int np) {
+// grid (ceil(n_aux/256), ceil(np/8), slots): a thread owns one aux row and eight P, so each L
+// element is read np/8 times instead of np times; the block's eight rows of M are staged in LDS
+// once and read as broadcasts.
+__global__ void __launch_bounds__(256) k_ri_pair_adjoint(const SlotDesc* __restrict__ slots, int np) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_aux = sd.n_aux;
-    if ((int)blockIdx.x * 64 >= n_aux) return;  // block-uniform
+    if ((int)blockIdx.x * 256 >= n_aux) return;  // block-uniform
     const double* __restrict__ L = sd.L;
     const double* __restrict__ dg = sd.dsum;
-    const int A = blockIdx.x * 64 + threadIdx.x;
-    const int Ac = A < n_aux ? A : n_aux - 1;
+    __shared__ double m[8][kGramLd];
     const int P0 = blockIdx.y * 8;
+    for (int idx = threadIdx.x; idx < 8 * np; idx += blockDim.x) {
+        const int j = idx / np, Q = idx - j * np, P = P0 + j;
+        m[j][Q] = P < np ? (P == Q ? 2.0 : 1.0) * dg[pair_index(P, Q)] : 0.0;
+    }
+    __syncthreads();
+    const int A = blockIdx.x * 256 + threadIdx.x;
+    const int Ac = A < n_aux ? A : n_aux - 1;
     double acc[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[j] = 0.0;
     for (int Q = 0; Q < np; ++Q) {
         const double l = L[(size_t)Ac + (size_t)n_aux * Q];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int P = P0 + j;
-            if (P < np) acc[j] += (P == Q ? 2.0 : 1.0) * dg[pair_index(P, Q)] * l;
-        }
+        for (int j = 0; j < 8; ++j) acc[j] += m[j][Q] * l;
     }
     if (A >= n_aux) return;
 #pragma unroll
@@ -2065,7 +2069,8 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
     const int len = ng + 2 * n * n;
     hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, part_rows, len);
     mark(EV_PAIR_BWD);
-    hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((shape.max_n_aux + 63) / 64, (np + 7) / 8, z), dim3(64), 0, st, slots, np);
+    hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((shape.max_n_aux + 255) / 256, (np + 7) / 8, z), dim3(256), 0, st, slots,
+                       np);
     {
         auto grad = [&]<int NQ>() {
             if (z <= kGradFewProblems) {
