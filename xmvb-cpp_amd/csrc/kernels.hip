@@ -213,6 +213,7 @@ __global__ void k_active_matrices(const SlotDesc* __restrict__ slots, int n) {
 // beyond n, so NQ (n rounded up to even) accumulators per row need no guards.
 constexpr int kOneIndexChunk = 64;
 constexpr int kOneIndexRows = 4;  // AO rows per wavefront
+constexpr int kOneIndexNu = 4;    // nu per inner step (loads in flight per row); divides kOneIndexChunk
 
 template <int NQ, typename BT>
 __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict__ slots, int n) {
@@ -244,22 +245,28 @@ __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict
         }
         __syncthreads();
         if (!rows) continue;
-        for (int v = 0; v < len; v += 2) {
-            // two nu per step (the second is a zero-weight repeat of the first past the chunk end)
-            const int nu1 = c0 + v, nu2 = c0 + min(v + 1, len - 1);
-            double b1[kOneIndexRows], b2[kOneIndexRows];
+        for (int v = 0; v < len; v += kOneIndexNu) {
+            // kOneIndexNu nu per step (past the chunk end: zero-weight repeats of its last nu; the
+            // staged rows from len on are zeros, and v + kOneIndexNu <= kOneIndexChunk)
+            double b[kOneIndexNu][kOneIndexRows];
 #pragma unroll
-            for (int u = 0; u < kOneIndexRows; ++u) {
-                b1[u] = (double)Bp[(size_t)n_aux * pair_index(mu[u], nu1)];
-                b2[u] = (double)Bp[(size_t)n_aux * pair_index(mu[u], nu2)];
+            for (int t = 0; t < kOneIndexNu; ++t) {
+                const int nu = c0 + min(v + t, len - 1);
+#pragma unroll
+                for (int u = 0; u < kOneIndexRows; ++u) b[t][u] = (double)Bp[(size_t)n_aux * pair_index(mu[u], nu)];
             }
-            const double* c1 = cT[v];
-            const double* c2 = cT[v + 1];  // row len (if reached) is all zeros
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                const double x1 = c1[q], x2 = c2[q];
+                double x[kOneIndexNu];
 #pragma unroll
-                for (int u = 0; u < kOneIndexRows; ++u) acc[u][q] += b1[u] * x1 + b2[u] * x2;
+                for (int t = 0; t < kOneIndexNu; ++t) x[t] = cT[v + t][q];
+#pragma unroll
+                for (int u = 0; u < kOneIndexRows; ++u) {
+                    double s = acc[u][q];
+#pragma unroll
+                    for (int t = 0; t < kOneIndexNu; ++t) s += b[t][u] * x[t];
+                    acc[u][q] = s;
+                }
             }
         }
     }
