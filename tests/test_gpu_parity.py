@@ -1,6 +1,6 @@
 """HIP engine vs the FP64 CPU oracle, called through the C ABI (needs GPU 0).
 
-Tolerances: energies |dE| <= 1e-8 Eh (the FP64 path is expected near 1e-12),
+Tolerances: energies |dE| <= 1e-8 Eh (the energy bar for every result; the FP64 path is expected near 1e-12),
 gradient max-norm 1e-7, structure H/S relative 1e-10.
 """
 import numpy as np
