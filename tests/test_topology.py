@@ -128,3 +128,50 @@ def test_invalid_class_rejected():
         vb.Topology.build("BOGUS", 4, 4, 1)
     with pytest.raises(vb.VbeError):
         vb.count_structures("ION(5)", 4, 4, 1)
+
+
+def _flip_pairs_brute(topo):
+    """Unordered determinant pairs {d, e} with one representative per spin-flip couple
+    {d, e} / {flip d, flip e} (the smaller packed index), by enumeration."""
+    nd = len(topo.det_alpha)
+    key = {(int(topo.alpha_masks[topo.det_alpha[d]]), int(topo.beta_masks[topo.det_beta[d]])): d for d in range(nd)}
+    flip = [key.get((int(topo.beta_masks[topo.det_beta[d]]), int(topo.alpha_masks[topo.det_alpha[d]])))
+            for d in range(nd)]
+    if topo.n_alpha != topo.n_beta or any(f is None for f in flip):
+        return None, nd * (nd + 1) // 2
+    kept = 0
+    for e in range(nd):
+        for d in range(e + 1):
+            lo, hi = min(flip[d], flip[e]), max(flip[d], flip[e])
+            kept += e * (e + 1) // 2 + d <= hi * (hi + 1) // 2 + lo
+    return flip, kept
+
+
+@pytest.mark.parametrize("kw,n,ne,mult", [("COV", 6, 6, 1), ("COV", 8, 8, 1), ("FULL", 2, 2, 1), ("FULL", 4, 4, 1),
+                                          ("FULL", 8, 14, 1), ("FULL", 6, 6, 1), ("COV", 6, 4, 3)])
+def test_spin_flip_pairing(kw, n, ne, mult):
+    """The spin-flip partner map is an involution that exchanges the strings, every couple of
+    pairs {d, e} / {flip d, flip e} has exactly one representative, and the closed-form count the
+    benchmark uses for the evaluated pairs equals the enumeration (the lane kernels' work list
+    follows the same rule)."""
+    import bench
+
+    topo = vb.Topology.build(kw, n, ne, mult)
+    flip, kept = _flip_pairs_brute(topo)
+    assert bench.lane_pairs_evaluated(topo) == kept
+    if flip is None:
+        assert topo.n_alpha != topo.n_beta
+        return
+    nd = len(flip)
+    for d in range(nd):
+        assert flip[flip[d]] == d
+        assert topo.alpha_masks[topo.det_alpha[flip[d]]] == topo.beta_masks[topo.det_beta[d]]
+    # couples: each unordered pair maps to a representative, each representative covers 1 or 2 pairs
+    covered = {}
+    for e in range(nd):
+        for d in range(e + 1):
+            lo, hi = min(flip[d], flip[e]), max(flip[d], flip[e])
+            rep = min(e * (e + 1) // 2 + d, hi * (hi + 1) // 2 + lo)
+            covered[rep] = covered.get(rep, 0) + 1
+    assert len(covered) == kept and set(covered.values()) <= {1, 2}
+    assert sum(covered.values()) == nd * (nd + 1) // 2
