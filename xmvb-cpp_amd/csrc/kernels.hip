@@ -894,7 +894,7 @@ __global__ void k_structure_left(const SlotDesc* __restrict__ slots, TopoDesc to
 // ---------------------------------------------------------------------------------------------
 // Front half of the eigensolver for n_str whose matrices live in global memory, over many
 // workgroups: k_gep_chol (one workgroup per problem: symmetrised S and H, Cholesky S = L L^T in
-// place), k_gep_linv (L^-1 by independent columns, one lane per column: X1 holds it row-major and
+// place), k_gep_linv (L^-1 by independent columns, one wavefront per column: X1 holds it row-major and
 // V column-major), k_gep_hlt (U = H L^-T into X2) and k_gep_lul (A = L^-1 U into M), after which
 // k_gep_lowest<false, 3> continues from the symmetrisation of A (launches of few problems replace
 // the Cholesky and the Householder stage by the per-step kernels further down). Workspace layout
@@ -932,8 +932,42 @@ __global__ void __launch_bounds__(1024) k_gep_chol(const SlotDesc* __restrict__ 
     }
 }
 
-// X = L^-1, lane j = column j: X[i][j] = (delta_ij - sum_{k=j}^{i-1} L[i][k] X[k][j]) / L[i][i]
-__global__ void __launch_bounds__(64) k_gep_linv(const SlotDesc* __restrict__ slots, int n) {
+// X = L^-1 by columns, one wavefront per column j: X[i][j] = (delta_ij - sum_{k=j}^{i-1} L[i][k]
+// X[k][j]) / L[i][i] for i = j..n-1 in order, each sum split over the 64 lanes, the column kept in
+// LDS as it grows. grid (ceil(n/4), 1, slots), 256 threads, dynamic LDS 4 n doubles
+// (n <= kLinvWaveMaxN; larger n take k_gep_linv_lane).
+constexpr int kLinvWaveMaxN = 2048;
+
+__global__ void __launch_bounds__(256) k_gep_linv(const SlotDesc* __restrict__ slots, int n) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const double* __restrict__ L = sd.gep;
+    double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
+    double* __restrict__ X1 = sd.gep + gep_core_dev(n);
+    extern __shared__ double xs[];  // four columns of n
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = blockIdx.x * 4 + wave;
+    if (j >= n) return;  // wave-uniform; the kernel has no block barrier
+    double* x = xs + (size_t)wave * n;
+    for (int i = lane; i < j; i += 64) {
+        X1[j + (size_t)n * i] = 0.0;
+        V[i + (size_t)n * j] = 0.0;
+    }
+    for (int i = j; i < n; ++i) {
+        double s = 0.0;
+        for (int k = j + lane; k < i; k += 64) s += L[i + (size_t)n * k] * x[k];
+        s = wave_sum(s);
+        const double xi = ((i == j) ? 1.0 - s : -s) / L[i + (size_t)n * i];
+        if (lane == 0) {
+            x[i] = xi;
+            X1[j + (size_t)n * i] = xi;  // row-major: X1[j + n i] = X[i][j]
+            V[i + (size_t)n * j] = xi;   // column-major
+        }
+        wave_sync();  // x[i] is visible to every lane before row i + 1
+    }
+}
+
+// The same with one lane per column (no LDS; for n above kLinvWaveMaxN).
+__global__ void __launch_bounds__(64) k_gep_linv_lane(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
     const double* __restrict__ L = sd.gep;
     double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
@@ -2028,7 +2062,11 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
         } else {
             hipLaunchKernelGGL(k_gep_chol, dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
         }
-        hipLaunchKernelGGL(k_gep_linv, dim3((unsigned)((ns + 63) / 64), 1, z), dim3(64), 0, st, slots, ns);
+        if (ns <= kLinvWaveMaxN)
+            hipLaunchKernelGGL(k_gep_linv, dim3((unsigned)((ns + 3) / 4), 1, z), dim3(256), 4 * (size_t)ns * sizeof(double),
+                               st, slots, ns);
+        else
+            hipLaunchKernelGGL(k_gep_linv_lane, dim3((unsigned)((ns + 63) / 64), 1, z), dim3(64), 0, st, slots, ns);
         hipLaunchKernelGGL(k_gep_hlt, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
         hipLaunchKernelGGL(k_gep_lul, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
         if (per_step) {
