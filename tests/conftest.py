@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# PyTorch (device buffers for the device-pointer entry points) carries its own HIP runtime; loading
+# it before the engine library makes the process use one runtime for both, as bench.py does
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "xmvb-cpp_amd"), os.path.join(ROOT, "tests")):
     if p not in sys.path:

@@ -75,6 +75,19 @@ def test_slot_conflict_is_rejected(ctx):
     t = pool.submit([0], [sp.C])
     with pytest.raises(vb.VbeError):
         pool.submit([0, 1], [sp.C, sp.C])  # slot 0 still pending
+    # a synchronous batched evaluation of a slot that the pending submission holds is rejected too
+    import torch
+
+    c_dev = torch.from_numpy(np.concatenate([sp.C.ravel(), sp.C.ravel()])).to("cuda:0")
+    g_dev = torch.zeros_like(c_dev)
+    torch.cuda.synchronize()
+    with pytest.raises(vb.VbeError) as err:
+        pool.evaluate_device([0], c_dev.data_ptr(), g_dev.data_ptr())
+    assert err.value.status == 1 and "pending submission" in str(err.value)
+    with pytest.raises(vb.VbeError):
+        pool.load_synthetic(0, sp)
+    e_free = pool.evaluate_device([1], c_dev.data_ptr(), g_dev.data_ptr())  # the other slot is free
+    assert np.isfinite(e_free[0])
     pool.wait(t)
     with pytest.raises(vb.VbeError):
         pool.evaluate([0, 0], [sp.C, sp.C])  # duplicate slot in one launch

@@ -389,13 +389,22 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     return p;
 }
 
+// Every path that starts work on slots (evaluation, submission, load, unload) rejects a slot that a
+// pending submission on either lane still holds: the two launches would share its workspace.
+void check_slots_free(const vbe_pool* p, const int* slots, int count) {
+    for (const auto& l : p->lanes)
+        if (l.busy)
+            for (int i = 0; i < count; ++i)
+                if (std::find(l.slots.begin(), l.slots.end(), slots[i]) != l.slots.end())
+                    throw StatusError(VBE_ERR_ARGUMENT, "slot " + std::to_string(slots[i]) +
+                                                            " is in use by a pending submission");
+}
+
 void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, const double* h, const double* B,
                double e_nuc) {
     if (slot < 0 || slot >= p->capacity) throw StatusError(VBE_ERR_ARGUMENT, "slot index out of range");
     if (n_bf <= 0 || n_aux <= 0 || !S || !h || !B) throw StatusError(VBE_ERR_ARGUMENT, "invalid slot data");
-    for (auto& l : p->lanes)
-        if (l.busy && std::find(l.slots.begin(), l.slots.end(), slot) != l.slots.end())
-            throw StatusError(VBE_ERR_ARGUMENT, "slot is in use by a pending submission");
+    check_slots_free(p, &slot, 1);
     check(hipSetDevice(p->ctx->device), "hipSetDevice");
     Slot& s = p->slots[slot];
     s.S_host.assign(S, S + (size_t)n_bf * n_bf);
@@ -821,9 +830,7 @@ int vbe_pool_unload(vbe_pool* pool, int32_t slot) {
     return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
         if (!pool) throw StatusError(VBE_ERR_ARGUMENT, "null pool");
         if (slot < 0 || slot >= pool->capacity) throw StatusError(VBE_ERR_ARGUMENT, "slot index out of range");
-        for (auto& l : pool->lanes)
-            if (l.busy && std::find(l.slots.begin(), l.slots.end(), slot) != l.slots.end())
-                throw StatusError(VBE_ERR_ARGUMENT, "slot is in use by a pending submission");
+        check_slots_free(pool, &slot, 1);
         check(hipSetDevice(pool->ctx->device), "hipSetDevice");
         // launches that read the slot were ordered on the lanes' streams and have completed
         // (no pending submission holds it); a synchronous evaluation returns only when done
@@ -865,6 +872,7 @@ int vbe_pool_eval_device(vbe_pool* pool, int32_t count, const int32_t* slots, co
         if (want_gradient && !d_grad) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
         check(hipSetDevice(pool->ctx->device), "hipSetDevice");
         Lane& lane = acquire_lane(pool);
+        check_slots_free(pool, slots, count);
         std::vector<SlotInput> in;
         size_t off = 0;
         for (int i = 0; i < count; ++i) {
@@ -895,11 +903,7 @@ int vbe_pool_submit(vbe_pool* pool, int32_t count, const int32_t* slots, const d
         if (count > pool->capacity) throw StatusError(VBE_ERR_ARGUMENT, "more slots than the pool capacity");
         check(hipSetDevice(pool->ctx->device), "hipSetDevice");
         Lane& lane = acquire_lane(pool);
-        for (const auto& other : pool->lanes)
-            if (other.busy)
-                for (int i = 0; i < count; ++i)
-                    if (std::find(other.slots.begin(), other.slots.end(), slots[i]) != other.slots.end())
-                        throw StatusError(VBE_ERR_ARGUMENT, "slot is in use by a pending submission");
+        check_slots_free(pool, slots, count);
         size_t total = 0;
         for (int i = 0; i < count; ++i) {
             if (slots[i] < 0 || slots[i] >= pool->capacity || !pool->slots[slots[i]].loaded)
