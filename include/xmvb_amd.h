@@ -235,6 +235,20 @@ int vbe_eval(vbe_context* ctx, vbe_problem* prob, const double* C_raw, const uin
 int vbe_eval_device(vbe_context* ctx, vbe_problem* prob, const double* d_C_raw, const uint8_t* d_support,
                     vbe_flags flags, double* E_total, double* d_grad_raw);
 
+/* No reference counterpart: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same inputs, outputs and results
+ * as vbe_eval_device, but the whole per-evaluation workspace (orbital intermediates, one-index
+ * factors, packed kernel, determinant and structure matrices, eigensolver work, backward partial
+ * rows, pair lists) lives in a static device array of VBE_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends, and the call makes no device allocation. Problems whose workspace exceeds that
+ * maximum are rejected with VBE_ERR_MEMORY (the medium and batching-form sizes fit, n_act = 14 with
+ * 429 structures does not). Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU race. Davidson does not start from the problem's previous root here, and the call discards a
+ * retained forward state (vbe_complete_gradient then needs a new vbe_eval_forward). */
+#define VBE_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int vbe_eval_device_noalloc(vbe_context* ctx, vbe_problem* prob, const double* d_C_raw, const uint8_t* d_support,
+                            vbe_flags flags, double* E_total, double* d_grad_raw);
+
 /* Two-phase contract of VbScfObjective (src/vbscf/optimization/objective/function.cpp:89-275):
  * vbe_eval_forward evaluates the energy of trial orbitals and keeps the forward state on the
  * device; vbe_complete_gradient turns the retained forward state into the gradient. A new

@@ -16,11 +16,13 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/xmvb_amd.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "noalloc.hpp"
 #include "optimizer.hpp"
 
 namespace {
@@ -137,6 +139,7 @@ struct Lane {
     bool busy = false;
     bool with_grad = false;
     bool events_valid = false;
+    bool scratch = false;  // the launch ran on a workspace outside the slots (no-allocation entry point)
     int ticket = -1;
     std::vector<int> slots;
     std::vector<size_t> grad_offsets;
@@ -400,6 +403,47 @@ void check_slots_free(const vbe_pool* p, const int* slots, int count) {
                                                             " is in use by a pending submission");
 }
 
+// Per-evaluation workspace of one slot of a pool, carved from one block of device memory (arrays
+// aligned to 256 bytes). With base == nullptr nothing is assigned and only the size is returned.
+// The slot's own arena and the no-allocation entry point's static scratch use this same layout.
+size_t carve_workspace(const vbe_pool* p, int n_bf, int n_aux, unsigned char* base, SlotDesc& d) {
+    const TopoDesc& T = p->topo;
+    const size_t n = T.n, np = T.np, nd = T.n_det, ns = T.n_str;
+    size_t off = 0;
+    auto take = [&](auto*& ptr, size_t count) {
+        using E = std::remove_pointer_t<std::remove_reference_t<decltype(ptr)>>;
+        off = (off + 255) & ~size_t(255);
+        if (base) ptr = reinterpret_cast<E*>(base + off);
+        off += std::max<size_t>(count, 1) * sizeof(E);
+    };
+    take(d.Cn, n * n_bf);
+    take(d.SC, n * n_bf);
+    take(d.HC, n * n_bf);
+    take(d.nu2, n);
+    take(d.sact, n * n);
+    take(d.hact, n * n);
+    take(d.T, (size_t)n_aux * n * n_bf);
+    take(d.L, (size_t)n_aux * np);
+    take(d.G, (size_t)T.ng);
+    take(d.Hdet, nd * nd);
+    take(d.Sdet, nd * nd);
+    take(d.Yh, nd * ns);
+    take(d.Ys, nd * ns);
+    take(d.Hs, ns * ns);
+    take(d.Ss, ns * ns);
+    take(d.gep, xmvb_amd::gep_work_doubles((int)ns));
+    take(d.coef, ns);
+    take(d.dcoef, nd);
+    const size_t len = (size_t)T.ng + 2 * n * n;
+    take(d.partials, (size_t)(bwd_blocks_cap(p) + xmvb_amd::kListBlocks + xmvb_amd::kSingularBlocks) * len);
+    take(d.dsum, len);
+    take(d.dL, (size_t)n_aux * np);
+    take(d.gcn, n * n_bf);
+    take(d.defer, (size_t)pair_count(p));
+    take(d.singular, (size_t)pair_count(p));
+    return (off + 255) & ~size_t(255);
+}
+
 void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, const double* h, const double* B,
                double e_nuc) {
     if (slot < 0 || slot >= p->capacity) throw StatusError(VBE_ERR_ARGUMENT, "slot index out of range");
@@ -409,8 +453,7 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
     Slot& s = p->slots[slot];
     s.S_host.assign(S, S + (size_t)n_bf * n_bf);
     s.has_root = false;
-    const TopoDesc& T = p->topo;
-    const size_t n = T.n, np = T.np, nd = T.n_det, ns = T.n_str;
+    const size_t n = p->topo.n;
     const size_t nbp = (size_t)n_bf * (n_bf + 1) / 2;
     const bool realloc = !s.arena || s.n_bf != n_bf || s.n_aux != n_aux;
     // the slot is unusable until the upload below has completed; a failed (re)allocation, e.g.
@@ -430,32 +473,10 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
         d.h = A.alloc<double>((size_t)n_bf * n_bf);
         if (p->ri_precision == VBE_PRECISION_FP32_RI) d.B32 = A.alloc<float>((size_t)n_aux * nbp);
         else d.B = A.alloc<double>((size_t)n_aux * nbp);
-        d.Cn = A.alloc<double>(n * n_bf);
-        d.SC = A.alloc<double>(n * n_bf);
-        d.HC = A.alloc<double>(n * n_bf);
-        d.nu2 = A.alloc<double>(n);
-        d.sact = A.alloc<double>(n * n);
-        d.hact = A.alloc<double>(n * n);
-        d.T = A.alloc<double>((size_t)n_aux * n * n_bf);
-        d.L = A.alloc<double>((size_t)n_aux * np);
-        d.G = A.alloc<double>(T.ng);
-        d.Hdet = A.alloc<double>(nd * nd);
-        d.Sdet = A.alloc<double>(nd * nd);
-        d.Yh = A.alloc<double>(nd * ns);
-        d.Ys = A.alloc<double>(nd * ns);
-        d.Hs = A.alloc<double>(ns * ns);
-        d.Ss = A.alloc<double>(ns * ns);
-        d.gep = A.alloc<double>(xmvb_amd::gep_work_doubles((int)ns));
-        d.coef = A.alloc<double>(ns);
-        d.dcoef = A.alloc<double>(nd);
-        const size_t len = (size_t)T.ng + 2 * n * n;
-        d.partials =
-            A.alloc<double>((size_t)(bwd_blocks_cap(p) + xmvb_amd::kListBlocks + xmvb_amd::kSingularBlocks) * len);
-        d.dsum = A.alloc<double>(len);
-        d.dL = A.alloc<double>((size_t)n_aux * np);
-        d.gcn = A.alloc<double>(n * n_bf);
-        d.defer = A.alloc<int>((size_t)pair_count(p));
-        d.singular = A.alloc<int>((size_t)pair_count(p));
+        // the per-evaluation workspace as one block, in the layout the no-allocation entry point
+        // reproduces in its static scratch
+        unsigned char* ws = A.alloc<unsigned char>(carve_workspace(p, n_bf, n_aux, nullptr, d));
+        carve_workspace(p, n_bf, n_aux, ws, d);
         double* C_buf = A.alloc<double>(n * n_bf);
         double* grad_buf = A.alloc<double>(n * n_bf);
         uint8_t* support_buf = A.alloc<uint8_t>(n * n_bf);
@@ -529,9 +550,12 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
 }
 
 // Enqueue forward and/or backward for a set of slots on a lane, then the D2H of energies and flags.
-void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool with_grad, bool forward) {
+// ws (optional, single-slot launches): a descriptor whose workspace replaces the slot's own.
+void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool with_grad, bool forward,
+             const SlotDesc* ws = nullptr) {
     const int count = (int)in.size();
     if (count <= 0 || count > p->capacity) throw StatusError(VBE_ERR_ARGUMENT, "invalid number of slots");
+    if (ws && count != 1) throw StatusError(VBE_ERR_ARGUMENT, "a replacement workspace serves one slot");
     hipStream_t st = lane.stream;
     const LaunchShape sh = shape_for(p, in);
     if (forward) {
@@ -542,13 +566,14 @@ void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool wit
                 throw StatusError(VBE_ERR_ARGUMENT, "slot not loaded");
             if (seen[s]) throw StatusError(VBE_ERR_ARGUMENT, "a slot appears twice in one launch");
             seen[s] = 1;
-            SlotDesc d = p->slots[s].desc;
+            SlotDesc d = ws ? *ws : p->slots[s].desc;
             d.C = in[i].C;
             d.support = in[i].support;
             d.grad = in[i].grad;
             d.energy = lane.d_energy + 2 * i;
             d.flags = lane.d_flags + FLAG_COUNT * i;
-            d.seeded = p->slots[s].has_root && p->ctx->dav_reuse_roots ? 1 : 0;
+            // a foreign workspace holds no previous eigenvector of this slot
+            d.seeded = !ws && p->slots[s].has_root && p->ctx->dav_reuse_roots ? 1 : 0;
             lane.h_desc[i] = d;
         }
         upload(lane.d_desc, lane.h_desc, count, st);
@@ -568,6 +593,7 @@ void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool wit
     for (const auto& x : in) lane.slots.push_back(x.slot);
     lane.with_grad = with_grad;
     lane.events_valid = forward;
+    lane.scratch = ws != nullptr;
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -609,7 +635,7 @@ void finish(vbe_pool* p, Lane& lane, std::vector<int>* slot_status = nullptr) {
                        : f[FLAG_METRIC]                                                        ? VBE_ERR_METRIC
                                                                                                : VBE_OK;
         if (slot_status) (*slot_status)[i] = si;
-        if (lane.events_valid) p->slots[lane.slots[i]].has_root = si == VBE_OK;
+        if (lane.events_valid && !lane.scratch) p->slots[lane.slots[i]].has_root = si == VBE_OK;
         if (status != VBE_OK || si == VBE_OK) continue;
         status = si;
         const std::string where = lane.slots.size() > 1 ? " (slot " + std::to_string(lane.slots[i]) + ")" : "";
@@ -731,6 +757,51 @@ void optimize_slots(vbe_pool* p, vbe_problem* prob, const std::vector<int>& slot
 }
 
 }  // namespace
+
+namespace xmvb_amd {
+
+int context_device(const vbe_context* ctx) { return ctx ? ctx->device : -1; }
+
+int report_error(vbe_context* ctx, int status, const char* what) { return fail(ctx, status, what); }
+
+int eval_device_in_scratch(vbe_context* ctx, vbe_problem* prob, const double* d_C, const uint8_t* d_support,
+                           vbe_flags flags, double* E_total, double* d_grad, unsigned char* scratch,
+                           size_t scratch_bytes) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !prob || !d_C || !E_total || !scratch) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        vbe_pool* p = prob->pool.get();
+        if (p->ctx != ctx) throw StatusError(VBE_ERR_ARGUMENT, "problem does not belong to this context");
+        check_flags_args(flags, p);
+        if (flags.want_gradient && !d_grad) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
+        const Slot& s = p->slots[0];
+        if (!s.loaded) throw StatusError(VBE_ERR_ARGUMENT, "problem not loaded");
+        SlotDesc ws = s.desc;
+        const size_t need = carve_workspace(p, s.n_bf, s.n_aux, nullptr, ws);
+        if (need > scratch_bytes)
+            throw StatusError(VBE_ERR_MEMORY, "evaluation workspace of " + std::to_string(need) +
+                                                  " bytes exceeds the static scratch maximum of " +
+                                                  std::to_string(scratch_bytes) + " bytes");
+        carve_workspace(p, s.n_bf, s.n_aux, scratch, ws);
+        p->eigensolver = flags.eigensolver;
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        Lane& lane = p->lanes[0];
+        std::vector<SlotInput> in{{0, d_C, d_support, flags.want_gradient ? d_grad : nullptr}};
+        // the problem's own workspace no longer holds the last forward
+        prob->forward_valid = false;
+        lane.busy = true;
+        try {
+            enqueue(p, lane, in, flags.want_gradient != 0, true, &ws);
+        } catch (...) {
+            lane.busy = false;
+            throw;
+        }
+        finish(p, lane);
+        *E_total = lane.h_energy[1];
+        return VBE_OK;
+    });
+}
+
+}  // namespace xmvb_amd
 
 extern "C" {
 

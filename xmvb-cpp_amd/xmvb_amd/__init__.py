@@ -46,7 +46,7 @@ EXPORTED_SYMBOLS = (
     "vbe_context_memory", "vbe_pool_autotune", "vbe_pool_eval_device",
     "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit", "vbe_set_ri_precision",
     "vbe_optimizer_defaults", "vbe_optimize", "vbe_pool_optimize", "vbe_set_eigensolver",
-    "vbe_gaussian_basis_size", "vbe_ao_integrals", "vbe_pivoted_cholesky",
+    "vbe_gaussian_basis_size", "vbe_ao_integrals", "vbe_pivoted_cholesky", "vbe_eval_device_noalloc",
 )
 
 TERMINATIONS = {0: "max_iterations", 1: "plbfgs_initial_tolerance", 2: "plbfgs_projected_gradient_tolerance",
@@ -136,6 +136,7 @@ def library() -> C.CDLL:
         lib.vbe_problem_device_bytes.argtypes = [P, C.POINTER(U64)]
         lib.vbe_eval.argtypes = [P, P, P, P, Flags, C.POINTER(D), P, P]
         lib.vbe_eval_device.argtypes = [P, P, P, P, Flags, C.POINTER(D), P]
+        lib.vbe_eval_device_noalloc.argtypes = [P, P, P, P, Flags, C.POINTER(D), P]
         lib.vbe_eval_forward.argtypes = [P, P, P, P, C.POINTER(D)]
         lib.vbe_complete_gradient.argtypes = [P, P, P]
         lib.vbe_last_intermediates.argtypes = [P, P, P, P, P, P, P, P]
@@ -449,6 +450,17 @@ class Problem:
         self.ctx.check(self.ctx._lib.vbe_eval_device(self.ctx._h, self._h, C.c_void_p(C_ptr),
                                                      None if support_ptr is None else C.c_void_p(support_ptr),
                                                      Flags(self.precision, self.eigensolver, int(gradient)), C.byref(e), C.c_void_p(grad_ptr)))
+        return e.value
+
+    def evaluate_device_noalloc(self, C_ptr: int, grad_ptr: int, gradient: bool = True,
+                                support_ptr: int | None = None):
+        """evaluate_device with the per-evaluation workspace in the library's static device scratch
+        (vbe_eval_device_noalloc: no device allocation in the call)."""
+        e = C.c_double()
+        self.ctx.check(self.ctx._lib.vbe_eval_device_noalloc(self.ctx._h, self._h, C.c_void_p(C_ptr),
+                                                             None if support_ptr is None else C.c_void_p(support_ptr),
+                                                             Flags(self.precision, self.eigensolver, int(gradient)),
+                                                             C.byref(e), C.c_void_p(grad_ptr)))
         return e.value
 
     def optimize(self, Cm: np.ndarray, support: np.ndarray | None = None, **options) -> dict:
