@@ -375,15 +375,24 @@ def main():
     lane_pairs = lane_pairs_evaluated(topo)
     nbp = cfg["n_bf"] * (cfg["n_bf"] + 1) // 2
     f_det = K * n_det * n_det * det_pair_flops(topo.n_alpha, topo.n_beta)  # ordered pairs, forward
+    # (duration, roofline bound, algorithmic work, unit, what actually limits the kernel). The FP64
+    # determinant-pair and eigensolver kernels are priced against the FP64 peak ("mfma": on MI355X
+    # the FP64 vector and matrix peaks are the same 78.6 TFLOP/s) but are limited by something else,
+    # named in "limiter": LDS gathers of the np x np kernel and FP64 LDS atomics for the pair
+    # kernels, serial recurrences and block barriers for the eigensolver.
     kernels = {
         # backward ~ 2 F_det (two kernels over the same pairs): F blocks + S_act/h_act adjoints,
         # and the packed-kernel adjoint, one F_det each
-        "k_pairs_lane_bwd_f": (mean["pair_backward_field_ms"], "mfma", f_det, "TFLOP/s"),
-        "k_pairs_lane_bwd_g": (mean["pair_backward_adjoint_ms"], "mfma", f_det, "TFLOP/s"),
-        "k_pairs_lane_fwd": (mean["pair_forward_kernel_ms"], "mfma", f_det, "TFLOP/s"),
+        "k_pairs_lane_bwd_f": (mean["pair_backward_field_ms"], "mfma", f_det, "TFLOP/s",
+                               "LDS gathers of the kernel and FP64 LDS atomics (VALU about a third busy)"),
+        "k_pairs_lane_bwd_g": (mean["pair_backward_adjoint_ms"], "mfma", f_det, "TFLOP/s",
+                               "FP64 LDS atomics of the same-spin adjoint"),
+        "k_pairs_lane_fwd": (mean["pair_forward_kernel_ms"], "mfma", f_det, "TFLOP/s",
+                             "LDS gathers of the kernel"),
         "k_ri_one_index": (mean["ri_one_index_ms"], "hbm",
-                           K * 8.0 * cfg["n_aux"] * (nbp + cfg["n_act"] * cfg["n_bf"]), "GB/s"),
-        "k_gep_lowest": (mean["eigensolve_ms"], "mfma", K * 10.0 * topo.n_structures ** 3, "TFLOP/s"),
+                           K * 8.0 * cfg["n_aux"] * (nbp + cfg["n_act"] * cfg["n_bf"]), "GB/s", "HBM"),
+        "k_gep_lowest": (mean["eigensolve_ms"], "mfma", K * 10.0 * topo.n_structures ** 3, "TFLOP/s",
+                         "latency: serial recurrences and block barriers"),
     }
     traffic_table = {}
     if os.path.exists(TRAFFIC_FILE):
@@ -391,7 +400,7 @@ def main():
             traffic_table = json.load(f).get("kernels", {})
 
     def roof(name):
-        ms, bound, work, unit = kernels[name]
+        ms, bound, work, unit, limiter = kernels[name]
         if unit == "GB/s":
             achieved, peak = work / (ms * 1e-3) / 1e9, HBM_PEAK_GBS
         else:
@@ -401,7 +410,12 @@ def main():
         if tr and tr.get("problems_per_launch"):
             traffic = (tr["fetch_bytes"] + tr["write_bytes"]) * K / tr["problems_per_launch"]
         out = dict(kernel=name, bound=bound, achieved=achieved, peak=peak, unit=unit, frac=achieved / peak,
-                   traffic=traffic, kernel_ms=ms, algorithmic_work=work, problems_per_launch=K)
+                   traffic=traffic, kernel_ms=ms, algorithmic_work=work, problems_per_launch=K, limiter=limiter,
+                   timing="HIP events recorded on the evaluating stream around this kernel, mean over the timed steps")
+        if traffic is not None:
+            out["traffic_source"] = ("profiles/medium_pmc_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                                     "of this kernel in a separate run of the same launch shape (not measured "
+                                     "in this run), scaled to this launch's problem count")
         if name.startswith("k_pairs_lane"):
             # the algorithmic work counts every ordered determinant pair; the kernel evaluates one
             # unordered pair of each spin-flip couple, and one evaluation does the work of the
@@ -416,27 +430,36 @@ def main():
     roofline = roof(dominant)
     others = [roof(k) for k in kernels if k != dominant]
 
-    # ---- the same batch with FP32 RI factors (FP64 accumulation and everything downstream) ----
+    # ---- the same batch with compensated FP32 RI factors (FP32 + an 8-bit correction of its rounding,
+    # decoded exactly to FP64; FP64 arithmetic throughout): the mixed-precision engine ----
     mixed = None
     if args.mixed_steps > 0:
         ctx32 = vb.Context(0)
-        ctx32.set_ri_precision(1)
+        ctx32.set_ri_precision(2)
         pool32 = vb.Pool(ctx32, topo, capacity=K)
         e64 = np.array([float(x) for x in energies])
+        g64 = gb_dev.cpu().numpy()
         for s in range(K):
             pool32.load_synthetic(s, distinct[s % D])
+        gb32 = torch.zeros_like(cb_dev)
         for _ in range(args.warmup):
-            pool32.evaluate_device(slots, cb_dev.data_ptr(), gb_dev.data_ptr())
+            pool32.evaluate_device(slots, cb_dev.data_ptr(), gb32.data_ptr())
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.mixed_steps):
-            e32 = pool32.evaluate_device(slots, cb_dev.data_ptr(), gb_dev.data_ptr())
+            e32 = pool32.evaluate_device(slots, cb_dev.data_ptr(), gb32.data_ptr())
         torch.cuda.synchronize()
         el32 = time.perf_counter() - t0
-        mixed = dict(value=K * args.mixed_steps / el32, unit="E+grad evals/s", ri_storage="fp32",
+        g32 = gb32.cpu().numpy()
+        mixed = dict(value=K * args.mixed_steps / el32, unit="E+grad evals/s",
+                     ri_storage="compensated fp32 (fp32 + 8-bit correction of its rounding error, 5 bytes per "
+                                "element, at most 2^-31 relative), FP64 arithmetic",
+                     problems_compared=K, distinct_problems=D,
                      max_abs_dE_vs_fp64=float(np.abs(np.asarray(e32) - e64).max()),
+                     grad_max_err_vs_fp64=float(np.abs(g32 - g64).max()),
                      one_index_ms=pool32.phase_times()["ri_one_index_ms"],
-                     note="|dE| above 1e-8 Eh on some problems: not the headline path")
+                     phase_ms={k: round(v, 5) for k, v in pool32.phase_times().items()},
+                     speedup_vs_plain=K * args.mixed_steps / el32 / plain["value"])
         pool32.close()
         ctx32.close()
 
@@ -447,10 +470,10 @@ def main():
         psp = vb.synthetic_problem(0, 400, 14, 1200)
         ptopo = vb.Topology.build("COV", 14, 14, 1)
         runs = {}
-        for label, ri in (("fp64", 0), ("fp32_ri", 1)):
+        for label, ri in (("fp64", 0), ("fp32c_ri", 2)):
             pctx = vb.Context(0)
             if ri:
-                pctx.set_ri_precision(1)
+                pctx.set_ri_precision(ri)
             pprob = vb.Problem.from_synthetic(pctx, psp, ptopo)
             pe, pg = pprob.evaluate(psp.C)  # warm-up
             times = []
@@ -464,10 +487,10 @@ def main():
             pctx.close()
         precision = dict(workload="precision form: n_bf=400 n_aux=1200 n_act=14 (7a/7b), 429 covalent Rumer "
                                   "structures, 3432 determinants, one problem, host orbitals in",
-                         fp64_ms_per_eval=runs["fp64"]["ms"], fp32_ri_ms_per_eval=runs["fp32_ri"]["ms"],
+                         fp64_ms_per_eval=runs["fp64"]["ms"], fp32c_ri_ms_per_eval=runs["fp32c_ri"]["ms"],
                          energy_fp64=runs["fp64"]["energy"],
-                         abs_dE_fp32_ri_vs_fp64=abs(runs["fp32_ri"]["energy"] - runs["fp64"]["energy"]),
-                         grad_max_err_fp32_ri_vs_fp64=float(np.abs(runs["fp32_ri"]["grad"] - runs["fp64"]["grad"]).max()),
+                         abs_dE_fp32c_ri_vs_fp64=abs(runs["fp32c_ri"]["energy"] - runs["fp64"]["energy"]),
+                         grad_max_err_fp32c_ri_vs_fp64=float(np.abs(runs["fp32c_ri"]["grad"] - runs["fp64"]["grad"]).max()),
                          phase_ms_fp64=runs["fp64"]["phase_ms"])
         del psp, runs
 
@@ -644,7 +667,7 @@ def main():
                                else "batched step"),
                batch_step_ms=dict(p50=percentile(lat, 0.5), p99=percentile(lat, 0.99)),
                plain_fp64=plain, speedup_vs_plain=value / plain["value"], served=served,
-               served_mixed_sizes=served_mixed, fp32_ri=mixed, precision_form=precision,
+               served_mixed_sizes=served_mixed, mixed_precision=mixed, precision_form=precision,
                converge=converge, mixed_batch=mixed_batch,
                batch_consistency=batch_consistency, phase_ms={k: round(v, 5) for k, v in mean.items()},
                energy=energy, roofline=roofline, roofline_other_kernels=others)

@@ -58,10 +58,17 @@ typedef enum vbe_method {
 } vbe_method;
 
 typedef enum vbe_precision {
-    VBE_PRECISION_FP64 = 0,    /* everything FP64 */
-    VBE_PRECISION_FP32_RI = 1  /* RI factors B stored in FP32 (half the memory and transform traffic),
-                                  the transform accumulating in FP64; determinant pairs, structure
-                                  matrices, metric Cholesky, eigensolve and gradient stay FP64 */
+    VBE_PRECISION_FP64 = 0,     /* everything FP64 */
+    VBE_PRECISION_FP32_RI = 1,  /* RI factors B stored in plain FP32 (half the memory and transform
+                                   traffic), the transform accumulating in FP64. The FP32 rounding of B
+                                   moves energies by up to ~1e-7 Eh: this mode does NOT meet the 1e-8 Eh
+                                   bar and is kept only as a comparison point */
+    VBE_PRECISION_FP32C_RI = 2  /* compensated FP32 storage of B: hi = fp32(B) plus an 8-bit correction
+                                   k of its rounding error, B = hi + k 2^(e(hi) - 158) (e(hi) the biased
+                                   exponent of hi), 5 bytes per element and at most 2^-31 relative error;
+                                   decoded exactly to FP64, the transform accumulating in FP64. Meets
+                                   the 1e-8 Eh bar. Determinant pairs, structure matrices, metric
+                                   Cholesky, eigensolve and gradient stay FP64 in every mode */
 } vbe_precision;
 
 typedef enum vbe_eigensolver {

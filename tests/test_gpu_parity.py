@@ -203,11 +203,37 @@ def test_singular_pairs(kw, n_frag_bf, a, b, n_aux, limit):
     c.close()
 
 
+@pytest.mark.parametrize("n_act,n_bf,n_aux,seed", [(6, 40, 120, 3), (10, 120, 360, 0), (10, 120, 360, 1),
+                                                   (10, 120, 360, 2), (4, 240, 96, 4)])
+def test_fp32c_ri_storage(n_act, n_bf, n_aux, seed):
+    """Compensated FP32 storage of the RI factors (FP32 plus an 8-bit correction of its rounding
+    error, 2^-32 relative) with FP64 arithmetic: the energy bar |dE| <= 1e-8 Eh and gradient 1e-7
+    against the FP64 oracle. n_bf = 240 is above the tiled transform's range, so that case runs the
+    flat-layout kernel on the same storage format."""
+    print(f"case fp32c RI n_act={n_act} n_bf={n_bf} n_aux={n_aux} seed={seed}", flush=True)
+    sp = vb.synthetic_problem(seed, n_bf, n_act, n_aux)
+    topo = vb.Topology.build("COV", n_act, n_act, 1)
+    c = vb.Context(0)
+    c.set_ri_precision(2)
+    prob = vb.Problem.from_synthetic(c, sp, topo)
+    e, g = prob.evaluate(sp.C)
+    ref = oracle.evaluate(n_bf, n_aux, topo, sp.S, sp.h, sp.B, sp.C)
+    de, dg = abs(e - ref["energy"]), float(np.abs(g - ref["grad"]).max())
+    print(f"fp32c RI: dE={de:.2e} dg={dg:.2e}", flush=True)
+    assert de <= E_TOL
+    assert dg <= G_TOL
+    prob.precision = 0  # evaluation flags must match the upload precision
+    with pytest.raises(vb.VbeError):
+        prob.evaluate(sp.C)
+    prob.close()
+    c.close()
+
+
 @pytest.mark.parametrize("n_act,n_bf,n_aux,seed", [(6, 40, 120, 3), (10, 120, 360, 0)])
-def test_fp32_ri_storage(n_act, n_bf, n_aux, seed):
-    """RI factors stored in FP32 (FP64 accumulation): energy and gradient against the FP64 oracle.
-    FP32 rounding of B moves energies by ~1e-8 Eh (measured 0.7e-8..3.3e-8 Eh on these synthetic
-    problems), so this mode is checked at 1e-7 Eh / 1e-6 and is not the headline path."""
+def test_fp32_ri_storage_is_a_comparison_mode(n_act, n_bf, n_aux, seed):
+    """Plain FP32 storage of the RI factors (FP64 accumulation) is kept only as a comparison
+    point: its rounding of B moves energies by ~1e-8..1e-7 Eh, above the 1e-8 Eh bar, so it is checked
+    here at 1e-6 Eh / 1e-5 and never reported as a result of the engine."""
     sp = vb.synthetic_problem(seed, n_bf, n_act, n_aux)
     topo = vb.Topology.build("COV", n_act, n_act, 1)
     c = vb.Context(0)
@@ -215,12 +241,7 @@ def test_fp32_ri_storage(n_act, n_bf, n_aux, seed):
     prob = vb.Problem.from_synthetic(c, sp, topo)
     e, g = prob.evaluate(sp.C)
     ref = oracle.evaluate(n_bf, n_aux, topo, sp.S, sp.h, sp.B, sp.C)
-    print(f"fp32 RI n_act={n_act} n_bf={n_bf}: dE={abs(e - ref['energy']):.2e} "
-          f"dg={np.abs(g - ref['grad']).max():.2e}", flush=True)
-    assert abs(e - ref["energy"]) <= 1e-7
-    assert np.abs(g - ref["grad"]).max() <= 1e-6
-    prob.precision = 0  # evaluation flags must match the upload precision
-    with pytest.raises(vb.VbeError):
-        prob.evaluate(sp.C)
+    assert abs(e - ref["energy"]) <= 1e-6
+    assert np.abs(g - ref["grad"]).max() <= 1e-5
     prob.close()
     c.close()

@@ -37,6 +37,26 @@ def test_heterogeneous_slots_match_oracle(ctx):
     pool.close()
 
 
+def test_tiled_and_flat_factor_layouts_in_one_launch(ctx):
+    """Slots with n_bf up to 224 keep their RI factors aux-tiled (one-read transform), larger ones in
+    the flat layout; a launch over both runs both transform kernels."""
+    topo = vb.Topology.build("COV", 4, 4, 1)
+    dims = [(40, 96), (240, 72), (33, 41)]
+    pool = vb.Pool(ctx, topo, capacity=len(dims))
+    sps = []
+    for s, (n_bf, n_aux) in enumerate(dims):
+        sp = vb.synthetic_problem(300 + s, n_bf, 4, n_aux)
+        pool.load_synthetic(s, sp)
+        sps.append(sp)
+    print("case tiled + flat RI layouts in one pool launch", flush=True)
+    e, g = pool.evaluate([0, 1, 2], [sp.C for sp in sps])
+    for s, sp in enumerate(sps):
+        ref = oracle.evaluate(sp.n_bf, sp.n_aux, topo, sp.S, sp.h, sp.B, sp.C)
+        assert abs(e[s] - ref["energy"]) <= 1e-8
+        assert np.abs(g[s] - ref["grad"]).max() <= 1e-7
+    pool.close()
+
+
 def test_alone_and_in_the_mix_agree(ctx):
     topo = vb.Topology.build("COV", 8, 8, 1)
     K = 6

@@ -32,6 +32,9 @@ def test_pool_unload_returns_budget_and_reloads():
     topo = vb.Topology.build("COV", 6, 6, 1)
     sp = vb.synthetic_problem(31, 36, 6, 72)
     pool = vb.Pool(ctx, topo, capacity=2)
+    # the first load also sizes the pool's staging buffer for the RI factor layout; the second
+    # load's bytes are the slot's own
+    pool.load_synthetic(0, sp)
     base = ctx.memory()[0]
     pool.load_synthetic(1, sp)
     held = ctx.memory()[0] - base
@@ -119,14 +122,16 @@ def test_admission_under_budget_evicts_and_reloads():
     topo = vb.Topology.build("COV", 6, 6, 1)
     sps = [vb.synthetic_problem(700 + k, 40, 6, 120) for k in range(5)]
     probe = vb.Context(0)
-    pool = vb.Pool(probe, topo, capacity=1)
+    pool = vb.Pool(probe, topo, capacity=2)
     before = probe.memory()[0]
     pool.load_synthetic(0, sps[0])
-    per_slot = probe.memory()[0] - before
+    first = probe.memory()[0] - before  # the slot and the pool's RI staging buffer
+    pool.load_synthetic(1, sps[1])
+    per_slot = probe.memory()[0] - before - first
     pool.close()
     probe.close()
-    # room for the pool's topology tables and two resident problems, not three
-    ctx = vb.Context(0, hbm_budget_bytes=before + 2 * per_slot + per_slot // 2)
+    # room for the pool's topology tables, its staging buffer and two resident problems, not three
+    ctx = vb.Context(0, hbm_budget_bytes=before + first + per_slot + per_slot // 2)
     server = serving.Server(ctx, capacity=8, max_batch=4)
     for k, sp in enumerate(sps):
         server.register(k, topo, _loader(sp))

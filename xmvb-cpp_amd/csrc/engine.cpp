@@ -154,6 +154,9 @@ struct vbe_pool {
     int eigensolver = VBE_EIGENSOLVER_DENSE;
     TopoDesc topo{};
     DeviceArena topo_arena;
+    DeviceArena ri_stage_arena;  // staging of flat RI factors for re-layout at load
+    double* ri_stage = nullptr;
+    size_t ri_stage_doubles = 0;
     std::vector<Slot> slots;
     Lane lanes[kLanes];
     int next_ticket = 0;
@@ -417,6 +420,7 @@ size_t carve_workspace(const vbe_pool* p, int n_bf, int n_aux, unsigned char* ba
         off += std::max<size_t>(count, 1) * sizeof(E);
     };
     take(d.Cn, n * n_bf);
+    take(d.CnT, (size_t)(n_bf + 31) / 32 * 32 * xmvb_amd::kCnStride);
     take(d.SC, n * n_bf);
     take(d.HC, n * n_bf);
     take(d.nu2, n);
@@ -442,6 +446,24 @@ size_t carve_workspace(const vbe_pool* p, int n_bf, int n_aux, unsigned char* ba
     take(d.defer, (size_t)pair_count(p));
     take(d.singular, (size_t)pair_count(p));
     return (off + 255) & ~size_t(255);
+}
+
+// aux rows of the tiled RI layout (padded to whole tiles)
+size_t ri_tiled_aux(int n_aux) {
+    return (size_t)(n_aux + xmvb_amd::kRiTileAux - 1) / xmvb_amd::kRiTileAux * xmvb_amd::kRiTileAux;
+}
+
+// The pool's device staging buffer for flat RI factors on their way to a slot's layout; it only
+// grows (a larger problem replaces it) and is charged to the context budget.
+double* ensure_ri_stage(vbe_pool* p, size_t doubles) {
+    if (p->ri_stage_doubles >= doubles) return p->ri_stage;
+    p->ri_stage_arena.release();
+    p->ri_stage_arena.ctx = p->ctx;
+    p->ri_stage = nullptr;
+    p->ri_stage_doubles = 0;
+    p->ri_stage = p->ri_stage_arena.alloc<double>(doubles);
+    p->ri_stage_doubles = doubles;
+    return p->ri_stage;
 }
 
 void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, const double* h, const double* B,
@@ -471,8 +493,17 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
         SlotDesc d{};
         d.S = A.alloc<double>((size_t)n_bf * n_bf);
         d.h = A.alloc<double>((size_t)n_bf * n_bf);
-        if (p->ri_precision == VBE_PRECISION_FP32_RI) d.B32 = A.alloc<float>((size_t)n_aux * nbp);
-        else d.B = A.alloc<double>((size_t)n_aux * nbp);
+        // aux-tiled RI factors for the one-read transform where its bands cover n_bf (plain FP32
+        // storage keeps the flat layout); tiles are padded with zero aux rows
+        const bool tiled = p->ri_precision != VBE_PRECISION_FP32_RI && n_bf <= xmvb_amd::kRiTiledMaxBf;
+        const size_t nb_elems = (tiled ? ri_tiled_aux(n_aux) : (size_t)n_aux) * nbp;
+        if (p->ri_precision == VBE_PRECISION_FP64) {
+            d.B = A.alloc<double>(nb_elems);
+        } else {
+            d.B32 = A.alloc<float>(nb_elems);
+            if (p->ri_precision == VBE_PRECISION_FP32C_RI) d.Blo = A.alloc<signed char>(nb_elems);
+        }
+        d.b_tiled = tiled ? 1 : 0;
         // the per-evaluation workspace as one block, in the layout the no-allocation entry point
         // reproduces in its static scratch
         unsigned char* ws = A.alloc<unsigned char>(carve_workspace(p, n_bf, n_aux, nullptr, d));
@@ -495,13 +526,23 @@ void load_slot(vbe_pool* p, int slot, int n_bf, int n_aux, const double* S, cons
     hipStream_t st = p->lanes[0].stream;
     upload(const_cast<double*>(s.desc.S), S, (size_t)n_bf * n_bf, st);
     upload(const_cast<double*>(s.desc.h), h, (size_t)n_bf * n_bf, st);
+    const size_t flat = (size_t)n_aux * nbp;
     if (p->ri_precision == VBE_PRECISION_FP32_RI) {
-        std::vector<float> b32((size_t)n_aux * nbp);
+        std::vector<float> b32(flat);
         for (size_t k = 0; k < b32.size(); ++k) b32[k] = (float)B[k];
         upload(const_cast<float*>(s.desc.B32), b32.data(), b32.size(), st);
         check(hipStreamSynchronize(st), "hipStreamSynchronize");  // b32 is a host temporary
+    } else if (p->ri_precision == VBE_PRECISION_FP64 && !s.desc.b_tiled) {
+        upload(const_cast<double*>(s.desc.B), B, flat, st);
     } else {
-        upload(const_cast<double*>(s.desc.B), B, (size_t)n_aux * nbp, st);
+        // the flat factors go to the pool's staging buffer, a kernel writes the slot's layout
+        // (tiling and, for compensated storage, the FP32 + correction encoding)
+        double* stage = ensure_ri_stage(p, flat);
+        upload(stage, B, flat, st);
+        xmvb_amd::launch_ri_relayout(st, stage, n_aux, (long long)nbp, s.desc.b_tiled != 0,
+                                     const_cast<double*>(s.desc.B), const_cast<float*>(s.desc.B32),
+                                     const_cast<signed char*>(s.desc.Blo));
+        check(hipGetLastError(), "RI factor layout kernel");
     }
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
     s.loaded = true;
@@ -542,6 +583,16 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
     sh.list_blocks = xmvb_amd::kListBlocks;
     sh.lane_cond_limit = p->ctx->pair_condition_limit;
     sh.b32 = p->ri_precision == VBE_PRECISION_FP32_RI;
+    sh.b32c = p->ri_precision == VBE_PRECISION_FP32C_RI;
+    for (const auto& x : in) {
+        const Slot& sl = p->slots[x.slot];
+        if (sl.desc.b_tiled) {
+            sh.any_tiled = true;
+            sh.max_n_bf_tiled = std::max(sh.max_n_bf_tiled, sl.n_bf);
+        } else {
+            sh.any_flat = true;
+        }
+    }
     sh.davidson = p->eigensolver == VBE_EIGENSOLVER_DAVIDSON;
     sh.dav_energy_tol = p->ctx->dav_energy_tol;
     sh.dav_residual_tol = p->ctx->dav_residual_tol;
@@ -681,7 +732,8 @@ void ensure_stage(vbe_pool* p, Lane& l, size_t doubles) {
 }
 
 void check_flags_args(const vbe_flags& flags, const vbe_pool* p) {
-    if (flags.precision != VBE_PRECISION_FP64 && flags.precision != VBE_PRECISION_FP32_RI)
+    if (flags.precision != VBE_PRECISION_FP64 && flags.precision != VBE_PRECISION_FP32_RI &&
+        flags.precision != VBE_PRECISION_FP32C_RI)
         throw StatusError(VBE_ERR_ARGUMENT, "unknown precision");
     if (flags.precision != p->ri_precision)
         throw StatusError(VBE_ERR_ARGUMENT, "precision differs from the one the problem was uploaded with "
@@ -830,7 +882,7 @@ void vbe_context_destroy(vbe_context* ctx) { delete ctx; }
 
 int vbe_set_ri_precision(vbe_context* ctx, int32_t precision) {
     if (!ctx) return fail(nullptr, VBE_ERR_ARGUMENT, "null context");
-    if (precision != VBE_PRECISION_FP64 && precision != VBE_PRECISION_FP32_RI)
+    if (precision != VBE_PRECISION_FP64 && precision != VBE_PRECISION_FP32_RI && precision != VBE_PRECISION_FP32C_RI)
         return fail(ctx, VBE_ERR_ARGUMENT, "unknown precision");
     ctx->ri_precision = precision;
     return VBE_OK;
@@ -929,7 +981,7 @@ int vbe_context_memory(const vbe_context* ctx, uint64_t* in_use, uint64_t* budge
 
 int vbe_pool_device_bytes(const vbe_pool* pool, uint64_t* bytes) {
     if (!pool || !bytes) return VBE_ERR_ARGUMENT;
-    uint64_t b = pool->topo_arena.bytes;
+    uint64_t b = pool->topo_arena.bytes + pool->ri_stage_arena.bytes;
     for (const auto& s : pool->slots)
         if (s.arena) b += s.arena->bytes;
     *bytes = b;

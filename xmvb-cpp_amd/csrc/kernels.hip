@@ -97,7 +97,6 @@ __global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
     double* __restrict__ HC = sd.HC;
     double* __restrict__ nu2 = sd.nu2;
     int* __restrict__ flags = sd.flags;
-    (void)n_act;
     extern __shared__ double lds[];
     double* c = lds;             // n_bf
     double* red = lds + n_bf;    // 16
@@ -132,10 +131,19 @@ __global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
     const bool ok = isfinite(q) && q > 2.220446049250313e-16;
     const double f = ok ? 1.0 / sqrt(q) : 0.0;
     if (!ok && threadIdx.x == 0) atomicMax(&flags[FLAG_NORMALIZATION], k + 1);
+    double* __restrict__ CnT = sd.CnT;
     for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
         Cn[(size_t)k * n_bf + mu] = c[mu] * f;
+        CnT[(size_t)mu * kCnStride + k] = c[mu] * f;
         SC[mu + (size_t)n_bf * k] *= f;
         HC[mu + (size_t)n_bf * k] *= f;
+    }
+    if (k == 0) {  // the zeros of CnT: orbitals past n_act, rows past n_bf up to a multiple of 32
+        const int rows = (n_bf + 31) / 32 * 32;
+        for (int t = threadIdx.x; t < rows * kCnStride; t += blockDim.x) {
+            const int mu = t / kCnStride, q = t - mu * kCnStride;
+            if (mu >= n_bf || q >= n_act) CnT[t] = 0.0;
+        }
     }
     if (threadIdx.x == 0) nu2[k] = q;
 }
@@ -215,10 +223,30 @@ constexpr int kOneIndexChunk = 64;
 constexpr int kOneIndexRows = 4;  // AO rows per wavefront
 constexpr int kOneIndexNu = 4;    // nu per inner step (loads in flight per row); divides kOneIndexChunk
 
+// storage of the flat factors read by k_ri_one_index: FP64, FP32, compensated FP32 (FP32 + an
+// 8-bit correction in units of 2^(e(hi) - 158))
+struct RiFp32c {};
+
+template <typename BT>
+__device__ __forceinline__ double flat_b(const SlotDesc& sd, size_t at) {
+    if constexpr (std::is_same_v<BT, double>) {
+        return sd.B[at];
+    } else if constexpr (std::is_same_v<BT, float>) {
+        return (double)sd.B32[at];
+    } else {
+        const float hi = sd.B32[at];
+        unsigned e = (__float_as_uint(hi) >> 23) & 0xffu;
+        e = e ? e : 1u;
+        const double scale = __hiloint2double((int)((e + 1023u - 158u) << 20), 0);
+        return fma((double)sd.Blo[at], scale, (double)hi);
+    }
+}
+
 template <int NQ, typename BT>
 __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
+    if (sd.b_tiled) return;  // read by k_ri_one_index_tiled
     if ((int)blockIdx.y * 4 * kOneIndexRows >= n_bf || (int)blockIdx.x * 64 >= n_aux) return;  // block-uniform
     const double* __restrict__ Cn = sd.Cn;
     __shared__ double cT[kOneIndexChunk][16];
@@ -230,7 +258,6 @@ __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict
     int mu[kOneIndexRows];
 #pragma unroll
     for (int u = 0; u < kOneIndexRows; ++u) mu[u] = min(mu0 + u, n_bf - 1);
-    const BT* __restrict__ Bp = (std::is_same_v<BT, float> ? (const BT*)sd.B32 : (const BT*)sd.B) + Ac;
     double acc[kOneIndexRows][NQ];
 #pragma unroll
     for (int u = 0; u < kOneIndexRows; ++u)
@@ -253,7 +280,8 @@ __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict
             for (int t = 0; t < kOneIndexNu; ++t) {
                 const int nu = c0 + min(v + t, len - 1);
 #pragma unroll
-                for (int u = 0; u < kOneIndexRows; ++u) b[t][u] = (double)Bp[(size_t)n_aux * pair_index(mu[u], nu)];
+                for (int u = 0; u < kOneIndexRows; ++u)
+                    b[t][u] = flat_b<BT>(sd, (size_t)Ac + (size_t)n_aux * pair_index(mu[u], nu));
             }
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
@@ -1963,6 +1991,7 @@ hipError_t configure_kernels(int n, int ng, int n_str, int mmax) {
         if ((e = set_pairs_lds<true, false, true>(group, kPairLdsLimit)) != hipSuccess) return e;
     }
     if ((e = configure_pairs_lane()) != hipSuccess) return e;
+    if ((e = configure_ri_tiled()) != hipSuccess) return e;
     if ((e = configure_pairs_singular()) != hipSuccess) return e;
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gep_lowest<true>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGepLdsLimit);
@@ -1984,11 +2013,14 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
                        slots, n);
     hipLaunchKernelGGL(k_active_matrices, dim3(1, 1, z), dim3(256), 0, st, slots, n);
     mark(EV_PREP);
-    {
+    if (shape.any_tiled)
+        launch_ri_one_index_tiled(st, slots, count, n, shape.max_n_bf_tiled, shape.max_n_aux, shape.b32c);
+    if (shape.any_flat) {
         const dim3 grid((shape.max_n_aux + 63) / 64, (shape.max_n_bf + 4 * kOneIndexRows - 1) / (4 * kOneIndexRows), z);
         auto go = [&](auto nq) {
             constexpr int NQ = decltype(nq)::value;
             if (shape.b32) hipLaunchKernelGGL((k_ri_one_index<NQ, float>), grid, dim3(256), 0, st, slots, n);
+            else if (shape.b32c) hipLaunchKernelGGL((k_ri_one_index<NQ, RiFp32c>), grid, dim3(256), 0, st, slots, n);
             else hipLaunchKernelGGL((k_ri_one_index<NQ, double>), grid, dim3(256), 0, st, slots, n);
         };
         switch ((n + 1) / 2) {

@@ -39,8 +39,10 @@ struct SlotDesc {
     // molecule-static data
     const double* S;
     const double* h;
-    const double* B;    // RI factors, FP64 storage (or null)
-    const float* B32;   // RI factors, FP32 storage (or null)
+    const double* B;         // RI factors, FP64 storage (or null)
+    const float* B32;        // RI factors, FP32 storage or the FP32 part of compensated storage (or null)
+    const signed char* Blo;  // compensated storage: 8-bit correction of each FP32 element (or null)
+    int b_tiled;             // B, B32, Blo in the aux-tiled layout (ri_tiled.hip) instead of n_aux x npair
     int n_bf, n_aux;
     double e_nuc;
     // current evaluation: inputs and outputs
@@ -51,6 +53,7 @@ struct SlotDesc {
     int* flags;      // FLAG_COUNT ints
     // workspace
     double *Cn, *SC, *HC, *nu2, *sact, *hact, *T, *L, *G;
+    double* CnT;  // Cn as [mu][kCnStride] for rows mu < n_bf rounded up to 32, zeros past n and n_bf
     double *Hdet, *Sdet, *Yh, *Ys, *Hs, *Ss, *gep, *coef, *dcoef;
     double *partials, *dsum, *dL, *gcn;
     int* defer;     // deferred unordered pair indices (capacity: all pairs)
@@ -91,9 +94,26 @@ struct LaunchShape {
     int list_blocks;                       // lane-group blocks per slot for the deferred pairs
     double lane_cond_limit;                // lane kernel keeps blocks with max|X| max|X^-1| below this
     bool b32;                              // RI factors stored in FP32
+    bool b32c;                             // RI factors stored as compensated FP32 (FP32 + 8-bit correction)
+    bool any_tiled, any_flat;              // slots of the launch with the aux-tiled / the flat layout
+    int max_n_bf_tiled;                    // largest n_bf among the slots with the aux-tiled layout
     bool davidson;                         // structure eigenproblem by Davidson instead of dense
     double dav_energy_tol, dav_residual_tol, dav_stationarity_tol;
 };
+
+// Aux-tiled RI factors (ri_tiled.hip): tiles of kRiTileAux aux indices, used for n_bf up to
+// 32 * kRiTiledMaxBands (the one-index transform then reads every packed element once).
+constexpr int kRiTileAux = 8;
+constexpr int kCnStride = 16;  // row length of SlotDesc::CnT (>= MAX_ACT)
+constexpr int kRiTiledMaxBands = 7;
+constexpr int kRiTiledMaxBf = 32 * kRiTiledMaxBands;
+hipError_t configure_ri_tiled();
+void launch_ri_one_index_tiled(hipStream_t st, const SlotDesc* slots, int count, int n, int max_n_bf, int max_n_aux,
+                               bool compensated);
+// Writes flat FP64 factors (n_aux x npair) in a slot's storage: FP64 tiled (B), or compensated FP32
+// (B32 + Blo) tiled or flat, whichever pointers are non-null.
+void launch_ri_relayout(hipStream_t st, const double* flat, int n_aux, long long npair, bool tiled, double* B,
+                        float* B32, signed char* Blo);
 
 // largest condition bound under which a row-pivoted lane block is provably regular under the
 // complete-pivoting rank rule (see pairs_lane.hip)

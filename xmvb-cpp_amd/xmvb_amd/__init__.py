@@ -377,8 +377,9 @@ class Context:
         self.eigensolver = eigensolver
 
     def set_ri_precision(self, precision: int):
-        """0: FP64 RI factors; 1: FP32 RI factors (FP64 accumulation). Applies to problems and pools
-        created afterwards."""
+        """0: FP64 RI factors; 1: plain FP32 RI factors (FP64 accumulation; a comparison mode that
+        misses the 1e-8 Eh bar); 2: compensated FP32 RI factors (FP32 plus an 8-bit correction,
+        decoded exactly to FP64; within the bar). Applies to problems and pools created afterwards."""
         self.check(self._lib.vbe_set_ri_precision(self._h, precision))
         self.ri_precision = precision
 
