@@ -10,7 +10,7 @@
 // independent check fixes each part; see tests/):
 //   topology            - reference count fixtures (tests/input/test_generated_structures.cpp:84-141),
 //                         Catalan/2^(n/2) closed forms, S^2 = 0 per structure, product-vs-oracle bit-exact
-//   resolve/cofactors   - brute-force Laplace minors (tests/test_oracle_algebra.py), nullity cases
+//   resolve/cofactors   - brute-force Laplace minors (tests/test_oracle_pinning.py), nullity cases
 //   same/opposite spin  - orthonormal-basis FCI projected through the exterior map (independent formula)
 //   GEP                 - scipy.linalg.eigh on the same H, S
 //   RI transform        - dense numpy einsum at small sizes
