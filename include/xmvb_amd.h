@@ -73,8 +73,8 @@ typedef enum vbe_precision {
 
 typedef enum vbe_eigensolver {
     VBE_EIGENSOLVER_DENSE = 0,    /* EIGENSOLVER=DENSE (linalg/eigensolver.cpp:365-387) */
-    VBE_EIGENSOLVER_DAVIDSON = 1  /* lowest root by Davidson on the resident H and S
-                                     (linalg/eigensolver.cpp:412-435, davidson_core.hpp:271-730) */
+    VBE_EIGENSOLVER_DAVIDSON = 1  /* lowest root by Davidson in the metric's factored coordinates
+                                     (evaluator.cpp:339-391, davidson_core.hpp:271-730) */
 } vbe_eigensolver;
 
 typedef struct vbe_context vbe_context;
@@ -210,10 +210,15 @@ int vbe_set_ri_precision(vbe_context* ctx, int32_t precision);
 
 /* Structure eigensolver of problems and pools created afterwards (vbe_eval / vbe_eval_device select
  * it per call through vbe_flags.eigensolver, which then stays with the problem) and the Davidson
- * accuracy: energy_tolerance (1e-7), relative residual tolerance (1e-3) and CI stationarity
- * tolerance ||(H - E S) c|| / ||c|| (1e-7), the StructureSolveAccuracy defaults
- * (src/vbscf/core/contracts/eigensolver.hpp:23-45); reuse_roots (1) starts each slot from its
- * previous eigenvector (src/vbscf/derivatives/gradient/active_space/evaluator.cpp:233-246). */
+ * accuracy. Davidson runs in the metric's factored coordinates y = L^T c, S = L L^T, as the
+ * reference's solve_factored_structure_roots (src/vbscf/derivatives/gradient/active_space/
+ * evaluator.cpp:339-391) with its physical stopping (StructureSolveAccuracy::
+ * configure_physical_stopping, src/vbscf/core/contracts/eigensolver.hpp:51-56): Ritz energy stable
+ * to min(energy_tolerance, 1e-8) Eh and stationarity residual ||A y - E y|| / ||y|| at most the
+ * square root of that. stationarity_tolerance can only tighten the latter (pass 1.0 for the
+ * reference's bound); residual_tolerance (the reference's gradient_tolerance, 1e-3) is validated
+ * and kept, the physical stopping does not use it. reuse_roots (1) starts each slot from its
+ * previous eigenvector (evaluator.cpp:233-246). */
 int vbe_set_eigensolver(vbe_context* ctx, int32_t eigensolver, double energy_tolerance, double residual_tolerance,
                         double stationarity_tolerance, int32_t reuse_roots);
 const char* vbe_global_error(void);

@@ -1,10 +1,14 @@
-"""Davidson structure eigensolver (EIGENSOLVER=DAVIDSON) against the FP64 oracle (dense GEP) and
-against the engine's dense path, through the C ABI.
+"""Davidson structure eigensolver (EIGENSOLVER=DAVIDSON) in the metric's factored coordinates against
+the FP64 oracle (dense GEP) and against the engine's dense path, through the C ABI.
 
-With the reference's default accuracy (energy 1e-7, relative residual 1e-3, CI stationarity
-||(H - E S) c|| / ||c|| <= 1e-7) the energy error is second order in the residual, so |dE| <= 1e-8
-Eh holds; the eigenvector, and with it the gradient, is only first-order accurate (~1e-7 / gap).
-With tightened tolerances the gradient meets the dense path's 1e-7 bar."""
+The reference's factored solve (solve_factored_structure_roots, evaluator.cpp:339-391) stops when
+the Ritz energy is stable to min(energy_tolerance, 1e-8) Eh and the stationarity residual
+||A y - E y|| / ||y|| (A = L^-1 H L^-T, identity metric) is at most the square root of that, 1e-4
+at the defaults. The energy error is second order in that residual, so |dE| <= 1e-8 Eh holds at the
+defaults; the eigenvector, and with it the gradient, is first-order accurate only (residual / gap).
+With tightened tolerances (energy 1e-12, stationarity 1e-9) the energy meets 1e-10 and the
+gradient the dense path's 1e-7 bar. The cases include the 42-, 132- and 429-structure covalent
+spaces (n_act 10, 12, 14), whose metrics are ill-conditioned (cond S up to ~1e6)."""
 import numpy as np
 import pytest
 
@@ -13,7 +17,7 @@ import xmvb_amd as vb
 
 pytestmark = pytest.mark.gpu
 
-TIGHT = dict(energy_tolerance=1e-12, residual_tolerance=1e-10, stationarity_tolerance=1e-11)
+TIGHT = dict(energy_tolerance=1e-12, stationarity_tolerance=1e-9)
 
 
 @pytest.mark.parametrize("kw,n_act,n_el,n_bf,n_aux,seed", [
@@ -21,7 +25,9 @@ TIGHT = dict(energy_tolerance=1e-12, residual_tolerance=1e-10, stationarity_tole
     ("COV", 4, 4, 16, 48, 2),      # 2 structures
     ("COV", 6, 6, 30, 90, 4),      # 5 structures
     ("COV", 8, 8, 40, 120, 6),     # 14 structures
-    ("FULL", 6, 6, 24, 60, 5),     # 175 structures: restarts of the 32-vector basis
+    ("COV", 10, 10, 40, 80, 3),    # 42 structures
+    ("COV", 12, 12, 48, 96, 13),   # 132 structures, cond S ~ 8e4: restarts of the 32-vector basis
+    ("FULL", 6, 6, 24, 60, 5),     # 175 structures
 ])
 @pytest.mark.parametrize("tight", [False, True])
 def test_davidson_matches_oracle(kw, n_act, n_el, n_bf, n_aux, seed, tight):
@@ -36,13 +42,13 @@ def test_davidson_matches_oracle(kw, n_act, n_el, n_bf, n_aux, seed, tight):
     de, dg = abs(e - ref["energy"]), np.abs(g - ref["grad"]).max()
     print(f"n_str={topo.n_structures} dE={de:.2e} dg={dg:.2e}", flush=True)
     assert de <= (1e-10 if tight else 1e-8)
-    assert dg <= (1e-7 if tight else 1e-4)
+    assert dg <= (1e-7 if tight else 1e-3)
     # S-normalised eigenvector, same ray as the dense solution
     mid = prob.intermediates()
     c, cref = mid["coef"], ref["coef"]
     assert abs(c @ mid["S"] @ c - 1.0) <= 1e-10
     if tight:
-        assert np.abs(np.sign(c @ cref) * c - cref).max() <= 1e-7
+        assert np.abs(np.sign(c @ cref) * c - cref).max() <= 1e-6
     # a second evaluation starts from the recycled root and agrees
     e2, g2 = prob.evaluate(sp.C)
     assert abs(e2 - e) <= (1e-11 if tight else 1e-8)
@@ -50,22 +56,51 @@ def test_davidson_matches_oracle(kw, n_act, n_el, n_bf, n_aux, seed, tight):
     ctx.close()
 
 
-def test_davidson_budget_exhaustion_is_reported():
-    """132 covalent structures with an ill-conditioned metric (cond S ~ 8e4): the Jacobi-preconditioned
-    iteration does not reach the 1e-7 Eh stationarity bound within the reference's 2 n iterations,
-    which the reference reports as an error (davidson_core.hpp:722-729); so does the engine."""
+def test_davidson_429_structures_matches_dense():
+    """n_act 14, 429 covalent structures (cond S ~ 2e6): the Davidson energy against the oracle's
+    dense GEP, the gradient against the engine's dense path (itself pinned to the oracle). Each
+    Davidson run starts from the reference's initial block (a fresh problem has no root to reuse)."""
+    sp = vb.synthetic_problem(5, 28, 14, 56)
+    topo = vb.Topology.build("COV", 14, 14, 1)
     ctx = vb.Context(0)
-    ctx.set_eigensolver(1)
-    sp = vb.synthetic_problem(13, 48, 12, 96)
-    topo = vb.Topology.build("COV", 12, 12, 1)
+    dense = vb.Problem.from_synthetic(ctx, sp, topo)
+    print("case davidson 429 structures dense reference", flush=True)
+    e_dense, g_dense = dense.evaluate(sp.C)
+    dense.close()
+    results = {}
+    for name, tol in (("tight", TIGHT), ("default", {})):
+        ctx.set_eigensolver(1, **tol)
+        prob = vb.Problem.from_synthetic(ctx, sp, topo)
+        print(f"case davidson 429 structures {name}", flush=True)
+        results[name] = prob.evaluate(sp.C)
+        prob.close()
+    e, g = results["tight"]
+    assert abs(e - e_dense) <= 1e-10
+    assert np.abs(g - g_dense).max() <= 1e-7
+    e1, g1 = results["default"]
+    assert abs(e1 - e_dense) <= 1e-8
+    assert np.abs(g1 - g_dense).max() <= 1e-3
+    ref = oracle.evaluate(28, 56, topo, sp.S, sp.h, sp.B, sp.C, gradient=False)
+    assert abs(e - ref["energy"]) <= 1e-10
+    ctx.close()
+
+
+def test_davidson_budget_exhaustion_is_reported():
+    """A stationarity bound below what FP64 can reach on 42 structures: the iteration runs out of
+    its 2 n budget, which the reference reports as an error (davidson_core.hpp:722-729); so does
+    the engine, and the problem stays usable."""
+    ctx = vb.Context(0)
+    ctx.set_eigensolver(1, energy_tolerance=1e-12, stationarity_tolerance=1e-30)
+    sp = vb.synthetic_problem(3, 40, 10, 80)
+    topo = vb.Topology.build("COV", 10, 10, 1)
     prob = vb.Problem.from_synthetic(ctx, sp, topo)
+    print("case davidson budget exhaustion", flush=True)
     with pytest.raises(vb.VbeError) as err:
         prob.evaluate(sp.C)
     assert err.value.status == 2 and "Davidson" in str(err.value)
-    # the problem stays usable: the dense solver evaluates it
     prob.eigensolver = 0
     e, _ = prob.evaluate(sp.C)
-    ref = oracle.evaluate(48, 96, topo, sp.S, sp.h, sp.B, sp.C, gradient=False)
+    ref = oracle.evaluate(40, 80, topo, sp.S, sp.h, sp.B, sp.C, gradient=False)
     assert abs(e - ref["energy"]) <= 1e-8
     prob.close()
     ctx.close()

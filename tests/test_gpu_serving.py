@@ -177,3 +177,47 @@ def test_convergence_worker_matches_single_problem_optimisation():
         prob.close()
     worker.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("n_act,n_bf", [(10, 120), (12, 48)])
+def test_convergence_requests_in_a_mix_follow_their_alone_iterates(n_act, n_bf):
+    """Full-convergence requests converged together in the worker's lockstep rounds while E+grad
+    requests of another active space run on the server's pools: every request follows the iterates
+    it follows alone (pinned launch shapes and fixed-order adjoint sums give the same bits), so the
+    iteration count and termination agree and the energies agree to the last bit, over 160
+    iterations."""
+    ctx = vb.Context(0)
+    topo = vb.Topology.build("COV", n_act, n_act, 1)
+    sps = [vb.synthetic_problem(1200 + k, n_bf, n_act, 3 * n_bf, scan_r=0.8 + 0.4 * k) for k in range(3)]
+    worker = serving.ConvergenceWorker(ctx, capacity=4, max_iterations=160)
+    for k, sp in enumerate(sps):
+        worker.register(k, topo, _loader(sp))
+    server = serving.Server(ctx, capacity=8, max_batch=8)
+    etopo = vb.Topology.build("COV", 6, 6, 1)
+    esps = [vb.synthetic_problem(1300 + k, 40, 6, 120) for k in range(4)]
+    for k, sp in enumerate(esps):
+        server.register(("e", k), etopo, _loader(sp))
+    print(f"case convergence requests in a mix n_act={n_act} n_bf={n_bf}", flush=True)
+    reqs = [worker.submit(k, sps[k].C) for k in range(3)]
+    worker.start()
+    served = []
+    for i in range(200):
+        k = i % 4
+        served.append(server.submit(("e", k), esps[k].C))
+        server.pump()
+    server.drain()
+    worker.finish()
+    assert all(r.energy is not None for r in served)
+    for k, r in enumerate(reqs):
+        prob = vb.Problem.from_synthetic(ctx, sps[k], topo)
+        alone = prob.optimize(sps[k].C, max_iterations=160)
+        prob.close()
+        print(f"request {k}: {r.result['iterations']} iterations ({r.result['termination']}), "
+              f"alone {alone['iterations']}, dE={abs(r.result['energy'] - alone['energy']):.1e}", flush=True)
+        assert r.result["iterations"] == alone["iterations"]
+        assert r.result["termination"] == alone["termination"]
+        assert r.result["energy"] == alone["energy"]
+    assert max(r.result["iterations"] for r in reqs) >= 150
+    worker.close()
+    server.close()
+    ctx.close()

@@ -681,7 +681,7 @@ int run(const Options& o) {
     vbe_context* ctx = nullptr;
     check(vbe_context_create(0, 0, &ctx), "device context");
     check_ctx(ctx,
-              vbe_set_eigensolver(ctx, p.eigensolver, 1e-7, 1e-3, 1e-7, 1), "eigensolver");
+              vbe_set_eigensolver(ctx, p.eigensolver, 1e-7, 1e-3, 1.0, 1), "eigensolver");
     vbe_dims dims{p.n_bf, p.n_act, p.n_alpha, p.n_beta, p.n_aux};
     vbe_problem* prob = nullptr;
     check_ctx(ctx, vbe_problem_upload(ctx, &dims, p.S.data(), p.h.data(), p.B.data(), &view, p.e_nuc, &prob),

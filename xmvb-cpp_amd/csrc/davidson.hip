@@ -1,26 +1,38 @@
-// Lowest root of the structure eigenproblem H c = E S c by Davidson iteration on the resident
-// dense H and S: GeneralizedEigensolver::solve_davidson with n_roots = 1
-// (linalg/davidson_core.hpp:271-730, linalg/eigensolver.cpp:295-307,321-361,412-435).
+// Lowest root of the structure eigenproblem H c = E S c by Davidson iteration in the factored
+// coordinates of the structure metric, the reference's solve_factored_structure_roots
+// (derivatives/gradient/active_space/evaluator.cpp:339-391) with n_roots = 1 and the Davidson core
+// of linalg/davidson_core.hpp:271-560 in its identity-overlap form.
 //
-// One 256-thread workgroup per problem slot runs the whole iteration: dense H/S actions (each
-// thread owns rows), the S-orthonormal basis extension with two classical Gram-Schmidt passes in
-// the S metric (append_s_orthonormal_block, davidson_core.hpp:101-180), the correction
-// -r_i / (H_ii - E S_ii) with the reference's floor (eigensolver.cpp:225-240), its projection
-// against the basis with the rounding bound of project_davidson_corrections (davidson_core.hpp:
-// 186-213), and the BackwardError stopping test (davidson_core.hpp:440-475). The projected matrix
-// (at most 32 x 32) is solved by the first wavefront alone: Householder tridiagonalisation in LDS,
-// multisection on Sturm counts for the two lowest eigenvalues (the second gives the gap of the
-// stopping test), inverse iteration for the lowest eigenvector, back-transformation.
+// With S = L L^T (the metric's Cholesky factor; the reference keeps its upper factor U = L^T) and
+// y = L^T c the problem becomes A y = E y, A = L^-1 H L^-T, with the identity metric. The reference
+// applies A matrix-free as U^-T H U^-1 y; here H and S are resident, so the front half of the
+// eigensolve (launch_gep_front: Cholesky, L^-1, A) forms A once and every Davidson action is one
+// dense product A y. Options as structure_solve_options with the physical stopping of
+// StructureSolveAccuracy::configure_physical_stopping (core/contracts/eigensolver.hpp:51-56): energy
+// tolerance min(energy_tolerance, 1e-8), stationarity tolerance its square root; at most 2 n
+// iterations and min(n, 32) basis vectors (make_davidson_options, linalg/eigensolver.cpp:321-361).
+//
+// One 256-thread workgroup per problem slot runs the iteration: orthonormal basis extension with two
+// classical Gram-Schmidt passes and the 64-epsilon dependence floor (append_s_orthonormal_block,
+// davidson_core.hpp:101-180, identity metric), the projected matrix (at most 32 x 32) solved by the
+// first wavefront (Householder in LDS, multisection on Sturm counts for the two lowest eigenvalues,
+// inverse iteration, back-transform), the EnergyAndResidual stopping test (davidson_core.hpp:
+// 452-487: stationarity residual |r| / |y| at most the tolerance and the Ritz energy stable to the
+// energy tolerance, or resolved at the first iteration, or the basis complete), the identity
+// preconditioner of the factored solve (the correction is the residual itself), projected against
+// the basis with the rounding bound of project_davidson_corrections (davidson_core.hpp:186-213).
 //
 // Start: the slot's previous eigenvector when it has one (root recycling,
-// gradient/active_space/evaluator.cpp:233-246), else the reference's initial block: the unit
-// vector at the smallest H_ii/S_ii and a deterministic xorshift +-1 vector.
-// Restart (basis full): the basis shrinks to the current Ritz vector and the previous one (the
-// reference keeps the 16 lowest Ritz vectors); this changes iteration counts, not the stopping test.
+// gradient/active_space/evaluator.cpp:233-246), mapped as y = L^T c, else the reference's initial
+// block (make_davidson_initial_vectors: the unit vector at the smallest H_ii/S_ii and a
+// deterministic xorshift +-1 vector) mapped the same way. Restart (basis full): the basis shrinks to
+// the current Ritz vector and the previous one (the reference keeps the 16 lowest Ritz vectors);
+// this changes iteration counts, not the stopping test. The result c = L^-T y satisfies c^T S c = 1.
 
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <cmath>
 
 namespace xmvb_amd {
 namespace {
@@ -246,100 +258,91 @@ __device__ void dav_small_lowest(double* A, int m, double* Vh, double* hb, doubl
 }
 
 struct DavidsonParams {
-    double energy_tol, residual_tol, stationarity_tol;
+    double energy_tol, stationarity_tol;
 };
 
+// gep layout after launch_gep_front: L at 0, A at n^2, L^-1 (column-major) at 2 n^2; the basis and
+// its vectors from 3 n^2 on (2 n ms + 6 n <= 2 n^2 + 9 n doubles, inside gep_work_doubles(n))
 __global__ void __launch_bounds__(kDavThreads)
-k_davidson(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonParams prm) {
+k_davidson_factored(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonParams prm) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n = topo.n_str;
+    if (sd.flags[FLAG_METRIC]) return;  // the metric's Cholesky failed: the host reports it
     const int ms = n < MS ? n : MS;
+    const double* __restrict__ Lc = sd.gep;                       // L, lower triangle
+    const double* __restrict__ A = sd.gep + (size_t)n * n;        // L^-1 H L^-T
+    const double* __restrict__ Li = sd.gep + 2 * (size_t)n * n;   // L^-1, column-major
+    double* V = sd.gep + 3 * (size_t)n * n;  // n x ms orthonormal basis
+    double* AV = V + (size_t)n * ms;         // A V
+    double* root = AV + (size_t)n * ms;      // Ritz vector and its image
+    double* aroot = root + n;
+    double* pv = aroot + n;                  // candidate / correction and its image
+    double* ap = pv + n;
+    double* prv = ap + n;                    // previous Ritz vector and its image (restart)
+    double* aprv = prv + n;
     const double* __restrict__ H = sd.Hs;
     const double* __restrict__ S = sd.Ss;
-    double* W = sd.gep;
-    double* V = W;                       // n x ms, S-orthonormal basis
-    double* HV = V + (size_t)n * ms;     // H V
-    double* SV = HV + (size_t)n * ms;    // S V
-    double* root = SV + (size_t)n * ms;  // Ritz vector and its images
-    double* hroot = root + n;
-    double* sroot = hroot + n;
-    double* pv = sroot + n;              // candidate / correction and its images
-    double* hp = pv + n;
-    double* sp = hp + n;
-    double* prv = sp + n;                // previous Ritz vector and its images (restart)
-    double* hprv = prv + n;
-    double* sprv = hprv + n;
-    __shared__ double P[MS * MS];        // projected Hamiltonian V^T H V
-    __shared__ double A[MS * MS];        // its working copy for the small eigensolver
+    __shared__ double P[MS * MS];        // projected matrix V^T A V
+    __shared__ double Aw[MS * MS];       // its working copy for the small eigensolver
     __shared__ double Vh[MS * MS];
     __shared__ double hb[MS], ys[MS], yprev[MS], coef[MS], bnorm[MS], lam[2];
     __shared__ double red[16], red32[4 * MS];
-    __shared__ int state[2];             // [0] active dimension, [1] failure reason
+    __shared__ int state[2];             // [0] active dimension, [1] scratch index
     const int tid = threadIdx.x, nt = blockDim.x;
     constexpr double eps = 2.220446049250313e-16;
 
-    // apply H and S to pv into hp, sp (rows owned by each thread)
+    // ap = A pv (rows owned by each thread; A is symmetric, read by columns for unit stride)
     auto action = [&]() {
         __syncthreads();
         for (int i = tid; i < n; i += nt) {
-            double a = 0.0, b = 0.0;
-            for (int j = 0; j < n; ++j) {
-                const double x = pv[j];
-                a += H[i + (size_t)n * j] * x;
-                b += S[i + (size_t)n * j] * x;
+            double a0 = 0.0, a1 = 0.0;
+            int j = 0;
+            for (; j + 1 < n; j += 2) {
+                a0 += A[i + (size_t)n * j] * pv[j];
+                a1 += A[i + (size_t)n * (j + 1)] * pv[j + 1];
             }
-            hp[i] = a;
-            sp[i] = b;
+            if (j < n) a0 += A[i + (size_t)n * j] * pv[j];
+            ap[i] = a0 + a1;
         }
         __syncthreads();
     };
-    // append_s_orthonormal_block for the single candidate (pv, hp, sp); returns whether it was added
+    // append_s_orthonormal_block for the single candidate (pv, ap), identity metric
     auto append = [&]() -> bool {
         const int active = state[0];
-        double t[3] = {0.0, 0.0, 0.0};
-        for (int i = tid; i < n; i += nt) {
-            t[0] += pv[i] * sp[i];
-            t[1] += pv[i] * pv[i];
-            t[2] += sp[i] * sp[i];
-        }
-        dav_block_sums(t, 3, red);
-        const double initial_scale = fmax(fabs(t[0]), sqrt(t[1]) * sqrt(t[2]));
+        double t[1] = {0.0};
+        for (int i = tid; i < n; i += nt) t[0] += pv[i] * pv[i];
+        dav_block_sums(t, 1, red);
+        const double initial_scale = t[0];
         for (int pass = 0; pass < 2 && active > 0; ++pass) {
-            dav_project(V, n, active, sp, coef, red32);
+            dav_project(V, n, active, pv, coef, red32);
             dav_subtract(pv, V, n, active, coef);
-            dav_subtract(hp, HV, n, active, coef);
-            dav_subtract(sp, SV, n, active, coef);
+            dav_subtract(ap, AV, n, active, coef);
             __syncthreads();
         }
-        double u[3] = {0.0, 0.0, 0.0};
-        for (int i = tid; i < n; i += nt) {
-            u[0] += pv[i] * sp[i];
-            u[1] += pv[i] * pv[i];
-            u[2] += sp[i] * sp[i];
-        }
-        dav_block_sums(u, 3, red);
-        const double metric2 = u[0], vnorm = sqrt(u[1]), scale = vnorm * sqrt(u[2]);
+        double u[1] = {0.0};
+        for (int i = tid; i < n; i += nt) u[0] += pv[i] * pv[i];
+        dav_block_sums(u, 1, red);
+        const double metric2 = u[0];
         const double threshold = 64.0 * eps * fmax(initial_scale, 2.2250738585072014e-308);
-        if (!isfinite(metric2) || metric2 <= threshold || metric2 <= 0.0 || !isfinite(scale)) return false;
+        if (!isfinite(metric2) || metric2 <= threshold || metric2 <= 0.0) return false;
         const double inv = 1.0 / sqrt(metric2);
         for (int i = tid; i < n; i += nt) {
             V[i + (size_t)n * active] = pv[i] * inv;
-            HV[i + (size_t)n * active] = hp[i] * inv;
-            SV[i + (size_t)n * active] = sp[i] * inv;
+            AV[i + (size_t)n * active] = ap[i] * inv;
         }
         __syncthreads();
         if (tid == 0) {
-            bnorm[active] = vnorm * inv;
+            bnorm[active] = 1.0;
             state[0] = active + 1;
         }
         __syncthreads();
         return true;
     };
-    // projected Hamiltonian columns [first, active): V^T (H V[:, c]), symmetrised diagonal block
+    // projected matrix columns [first, active): V^T (A V[:, c]), symmetrised block
     auto update_projected = [&](int first) {
         const int active = state[0];
         for (int c = first; c < active; ++c) {
-            dav_project(V, n, active, HV + (size_t)n * c, coef, red32);
+            dav_project(V, n, active, AV + (size_t)n * c, coef, red32);
             if (tid < active) {
                 P[tid + MS * c] = coef[tid];
                 if (tid < first) P[c + MS * tid] = coef[tid];
@@ -355,22 +358,26 @@ k_davidson(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonParams prm
                 }
         __syncthreads();
     };
+    // pv = L^T x (x in prv): seeds and recycled roots enter the factored coordinates
+    auto to_factored = [&]() {
+        __syncthreads();
+        for (int i = tid; i < n; i += nt) {
+            double a = 0.0;
+            for (int k = i; k < n; ++k) a += Lc[k + (size_t)n * i] * prv[k];
+            pv[i] = a;
+        }
+        __syncthreads();
+    };
 
     if (tid == 0) {
         state[0] = 0;
         state[1] = 0;
     }
     __syncthreads();
-    // initial block: previous root, or e_j at min H_jj / S_jj and an xorshift +-1 vector
-    int n_initial;
-    if (sd.seeded) {
-        n_initial = 1;
-    } else {
-        n_initial = ms < 2 ? ms : 2;
-    }
+    const int n_initial = sd.seeded ? 1 : (ms < 2 ? ms : 2);
     for (int c = 0; c < n_initial; ++c) {
         if (sd.seeded) {
-            for (int i = tid; i < n; i += nt) pv[i] = sd.coef[i];
+            for (int i = tid; i < n; i += nt) prv[i] = sd.coef[i];
         } else if (c == 0) {
             // argmin_j H_jj / S_jj (ties: lowest index), by the first wavefront
             if (tid < 64) {
@@ -386,13 +393,11 @@ k_davidson(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonParams prm
                 const double lowest = -dav_wave_max(-best);
                 int cand = best == lowest ? at : 0x7fffffff;
                 for (int o = 32; o >= 1; o >>= 1) cand = min(cand, __shfl_xor(cand, o));
-                if (tid == 0) state[1] = cand;  // temporary: index of the unit vector
+                if (tid == 0) state[1] = cand;
             }
             __syncthreads();
             const int jmin = state[1];
-            for (int i = tid; i < n; i += nt) pv[i] = i == jmin ? 1.0 : 0.0;
-            __syncthreads();
-            if (tid == 0) state[1] = 0;
+            for (int i = tid; i < n; i += nt) prv[i] = i == jmin ? 1.0 : 0.0;
         } else {
             if (tid == 0) {
                 unsigned long long st = 0x9e3779b97f4a7c15ULL;
@@ -400,15 +405,16 @@ k_davidson(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonParams prm
                     st ^= st << 13;
                     st ^= st >> 7;
                     st ^= st << 17;
-                    pv[i] = (st & 1ULL) == 0ULL ? -1.0 : 1.0;
+                    prv[i] = (st & 1ULL) == 0ULL ? -1.0 : 1.0;
                 }
             }
         }
+        to_factored();
         action();
         append();
     }
     if (state[0] < 1) {
-        if (tid == 0) sd.flags[FLAG_EIGEN] = 1;  // initial vectors dependent in the S metric
+        if (tid == 0) sd.flags[FLAG_EIGEN] = 1;  // initial vectors dependent
         return;
     }
     update_projected(0);
@@ -418,55 +424,48 @@ k_davidson(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonParams prm
     const int max_iterations = 2 * n;
     for (int iteration = 1; iteration <= max_iterations; ++iteration) {
         const int active = state[0];
-        // lowest projected eigenpair (first wavefront), then the Ritz vector and its images
         if (tid < 64) {
-            for (int e = tid; e < MS * MS; e += 64) A[e] = P[e];
+            for (int e = tid; e < MS * MS; e += 64) Aw[e] = P[e];
             __builtin_amdgcn_wave_barrier();
-            dav_small_lowest(A, active, Vh, hb, ys, lam);
+            dav_small_lowest(Aw, active, Vh, hb, ys, lam);
         }
         __syncthreads();
         const double lambda = lam[0];
-        const double gap = active > 1 ? fmax(lam[1] - lam[0], eps) : eps;
         for (int i = tid; i < n; i += nt) {
-            double a = 0.0, b = 0.0, c = 0.0;
+            double a = 0.0, b = 0.0;
             for (int j = 0; j < active; ++j) {
-                const double yj = ys[j];
-                a += V[i + (size_t)n * j] * yj;
-                b += HV[i + (size_t)n * j] * yj;
-                c += SV[i + (size_t)n * j] * yj;
+                a += V[i + (size_t)n * j] * ys[j];
+                b += AV[i + (size_t)n * j] * ys[j];
             }
             root[i] = a;
-            hroot[i] = b;
-            sroot[i] = c;
+            aroot[i] = b;
         }
-        double t[4] = {0.0, 0.0, 0.0, 0.0};
+        double t[2] = {0.0, 0.0};
         for (int i = tid; i < n; i += nt) {
-            const double r = hroot[i] - lambda * sroot[i];
+            const double r = aroot[i] - lambda * root[i];
             t[0] += r * r;
-            t[1] += hroot[i] * hroot[i];
-            t[2] += sroot[i] * sroot[i];
-            t[3] += root[i] * root[i];
+            t[1] += root[i] * root[i];
         }
-        dav_block_sums(t, 4, red);
-        const double rnorm = sqrt(t[0]);
-        const double relative = rnorm / fmax(1.0, sqrt(t[1]) + fabs(lambda) * sqrt(t[2]));
-        const double stationarity = rnorm / sqrt(t[3]);
-        const double escale = fmax(1.0, fabs(lambda));
-        const double spectral = sqrt((prm.energy_tol / escale) * (gap / escale));
+        dav_block_sums(t, 2, red);
+        const double stationarity = sqrt(t[0]) / sqrt(t[1]);
         const bool stable = have_prev && fabs(lambda - lam_prev) <= prm.energy_tol;
-        const bool independent = relative <= prm.energy_tol / escale;
-        const double required = fmin(prm.residual_tol, spectral);
-        if (relative <= required && stationarity <= prm.stationarity_tol && (stable || independent)) {
-            for (int i = tid; i < n; i += nt) sd.coef[i] = root[i];
+        const bool initial_resolved = iteration == 1 && stationarity <= prm.energy_tol;
+        if (stationarity <= prm.stationarity_tol && (stable || initial_resolved || active == n)) {
+            // c = L^-T y / |y| (c^T S c = 1), the energy, the determinant coefficients
+            const double inv = 1.0 / sqrt(t[1]);
+            for (int i = tid; i < n; i += nt) {
+                double a = 0.0;
+                for (int k = i; k < n; ++k) a += Li[k + (size_t)n * i] * root[k];
+                sd.coef[i] = a * inv;
+            }
             if (tid == 0) {
                 sd.energy[0] = lambda;
                 sd.energy[1] = lambda + sd.e_nuc;
             }
             __syncthreads();
-            // determinant coefficients a_d = sum_terms t_dI c_I for the backward weights
             for (int d = tid; d < topo.n_det; d += nt) {
                 double acc = 0.0;
-                for (long long q = topo.off[d]; q < topo.off[d + 1]; ++q) acc += topo.tc[q] * root[topo.ts[q]];
+                for (long long q = topo.off[d]; q < topo.off[d + 1]; ++q) acc += topo.tc[q] * sd.coef[topo.ts[q]];
                 sd.dcoef[d] = acc;
             }
             return;
@@ -477,48 +476,38 @@ k_davidson(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonParams prm
         }
         lam_prev = lambda;
         have_prev = true;
-        // correction -r_i / (H_ii - E S_ii) with the reference's floor
-        for (int i = tid; i < n; i += nt) {
-            const double hd = H[i + (size_t)n * i], sdg = S[i + (size_t)n * i];
-            double den = hd - lambda * sdg;
-            const double floor = 1.4901161193847656e-08 * fmax(1.0, fmax(fabs(hd), fabs(lambda * sdg)));
-            if (fabs(den) < floor) den = copysign(floor, den == 0.0 ? 1.0 : den);
-            pv[i] = -(hroot[i] - lambda * sroot[i]) / den;
-        }
+        // identity preconditioner: the correction is the residual
+        for (int i = tid; i < n; i += nt) pv[i] = aroot[i] - lambda * root[i];
         __syncthreads();
         if (active + 1 > ms) {
             // restart: the basis becomes the current Ritz vector and the previous one
             for (int i = tid; i < n; i += nt) {
-                double a = 0.0, b = 0.0, c = 0.0;
+                double a = 0.0, b = 0.0;
                 for (int j = 0; j < active; ++j) {
-                    const double yj = yprev[j];
-                    a += V[i + (size_t)n * j] * yj;
-                    b += HV[i + (size_t)n * j] * yj;
-                    c += SV[i + (size_t)n * j] * yj;
+                    a += V[i + (size_t)n * j] * yprev[j];
+                    b += AV[i + (size_t)n * j] * yprev[j];
                 }
                 prv[i] = a;
-                hprv[i] = b;
-                sprv[i] = c;
+                aprv[i] = b;
             }
             __syncthreads();
+            const double rn = 1.0 / sqrt(t[1]);
             for (int i = tid; i < n; i += nt) {
-                V[i] = root[i];
-                HV[i] = hroot[i];
-                SV[i] = sroot[i];
+                V[i] = root[i] * rn;
+                AV[i] = aroot[i] * rn;
             }
             if (tid == 0) {
                 state[0] = 1;
-                bnorm[0] = sqrt(t[3]);
+                bnorm[0] = 1.0;
             }
             __syncthreads();
-            // the previous Ritz vector, S-orthogonalised against the current one (correction
-            // kept in pv: move it aside through the candidate arrays)
+            // the previous Ritz vector, orthogonalised against the current one (the correction is
+            // kept aside in prv meanwhile)
             for (int i = tid; i < n; i += nt) {
                 const double c = pv[i];
                 pv[i] = prv[i];
                 prv[i] = c;
-                hp[i] = hprv[i];
-                sp[i] = sprv[i];
+                ap[i] = aprv[i];
             }
             __syncthreads();
             append();
@@ -533,7 +522,7 @@ k_davidson(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonParams prm
             for (int i = tid; i < n; i += nt) t0[0] += pv[i] * pv[i];
             dav_block_sums(t0, 1, red);
             const double initial_norm = sqrt(t0[0]);
-            dav_project(SV, n, act, pv, coef, red32);
+            dav_project(V, n, act, pv, coef, red32);
             double terms = 0.0;
             for (int j = 0; j < act; ++j) terms += fabs(coef[j]) * bnorm[j];
             const double ops = (double)(2 * act + 1) * eps;
@@ -568,8 +557,12 @@ k_davidson(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonParams prm
 
 void launch_davidson(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo,
                      const LaunchShape& shape) {
-    const DavidsonParams prm{shape.dav_energy_tol, shape.dav_residual_tol, shape.dav_stationarity_tol};
-    hipLaunchKernelGGL(k_davidson, dim3(1, 1, count), dim3(kDavThreads), 0, st, slots, topo, prm);
+    // StructureSolveAccuracy::configure_physical_stopping
+    // (the caller's stationarity tolerance can only tighten the square-root bound)
+    const double etol = std::min(shape.dav_energy_tol, 1.0e-8);
+    const DavidsonParams prm{etol, std::min(std::sqrt(etol), shape.dav_stationarity_tol)};
+    launch_gep_front(st, slots, (unsigned)count, topo, shape.pinned || count <= 8);
+    hipLaunchKernelGGL(k_davidson_factored, dim3(1, 1, count), dim3(kDavThreads), 0, st, slots, topo, prm);
 }
 
 }  // namespace xmvb_amd

@@ -60,7 +60,7 @@ struct vbe_context {
     // structure eigensolver of pools created afterwards; Davidson accuracy (StructureSolveAccuracy
     // defaults, src/vbscf/core/contracts/eigensolver.hpp:23-45) and root recycling
     int eigensolver = VBE_EIGENSOLVER_DENSE;
-    double dav_energy_tol = 1.0e-7, dav_residual_tol = 1.0e-3, dav_stationarity_tol = 1.0e-7;
+    double dav_energy_tol = 1.0e-7, dav_residual_tol = 1.0e-3, dav_stationarity_tol = 1.0;
     bool dav_reuse_roots = true;
     std::string error;
     // guards in_use and error: pools of one context may be driven from several host threads
@@ -445,6 +445,10 @@ size_t carve_workspace(const vbe_pool* p, int n_bf, int n_aux, unsigned char* ba
     take(d.gcn, n * n_bf);
     take(d.defer, (size_t)pair_count(p));
     take(d.singular, (size_t)pair_count(p));
+    take(d.defer_flag, (size_t)T.n_lane_pairs);
+    take(d.singular_flag, (size_t)pair_count(p));
+    take(d.chunk_count,
+         (size_t)(std::max<long long>(T.n_lane_pairs, pair_count(p)) / xmvb_amd::kCompactChunk + 2));
     return (off + 255) & ~size_t(255);
 }
 
@@ -556,8 +560,9 @@ struct SlotInput {
     double* grad;            // device gradient output
 };
 
-LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
+LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in, bool pinned) {
     LaunchShape sh{};
+    sh.pinned = pinned;
     for (const auto& x : in) {
         sh.max_n_bf = std::max(sh.max_n_bf, p->slots[x.slot].n_bf);
         sh.max_n_aux = std::max(sh.max_n_aux, p->slots[x.slot].n_aux);
@@ -570,7 +575,8 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
     const long long fwd_group = 2LL * p->fwd_target, bwd_group = p->bwd_target;
     sh.pair_blocks_fwd = (int)std::clamp<long long>((fwd_group + count - 1) / count, 1,
                                                     std::max<long long>(1, (pairs + 4 * per_sweep - 1) / (4 * per_sweep)));
-    sh.pair_blocks_bwd = (int)std::clamp<long long>((bwd_group + count - 1) / count, 1, bwd_blocks_cap(p));
+    sh.pair_blocks_bwd = pinned ? bwd_blocks_cap(p)
+                                : (int)std::clamp<long long>((bwd_group + count - 1) / count, 1, bwd_blocks_cap(p));
     // one pair per lane: 256 pairs per block sweep
     sh.n_alpha = p->n_alpha;
     sh.n_beta = p->n_beta;
@@ -578,9 +584,13 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
     sh.lane_bwd = xmvb_amd::pairs_lane_supported(p->n_alpha, p->n_beta, true);
     const long long lane_sweep = std::max<long long>(1, (p->topo.n_lane_pairs + 255) / 256);
     sh.lane_blocks_fwd = (int)std::clamp<long long>((p->fwd_target + count - 1) / count, 1, lane_sweep);
-    sh.lane_blocks_bwd = (int)std::clamp<long long>((p->bwd_target + count - 1) / count, 1,
-                                                    std::min<long long>(lane_sweep, bwd_blocks_cap(p)));
-    sh.list_blocks = xmvb_amd::kListBlocks;
+    sh.lane_blocks_bwd = pinned ? (int)std::min<long long>(lane_sweep, bwd_blocks_cap(p))
+                                : (int)std::clamp<long long>((p->bwd_target + count - 1) / count, 1,
+                                                             std::min<long long>(lane_sweep, bwd_blocks_cap(p)));
+    // pinned shapes keep the full count so that a problem's partial rows do not depend on its batch
+    sh.list_blocks = pinned ? xmvb_amd::kListBlocks
+                            : (int)std::clamp<long long>(256 / std::max(count, 1LL), xmvb_amd::kListBlocksBatched,
+                                                         xmvb_amd::kListBlocks);
     sh.lane_cond_limit = p->ctx->pair_condition_limit;
     sh.b32 = p->ri_precision == VBE_PRECISION_FP32_RI;
     sh.b32c = p->ri_precision == VBE_PRECISION_FP32C_RI;
@@ -603,12 +613,12 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in) {
 // Enqueue forward and/or backward for a set of slots on a lane, then the D2H of energies and flags.
 // ws (optional, single-slot launches): a descriptor whose workspace replaces the slot's own.
 void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool with_grad, bool forward,
-             const SlotDesc* ws = nullptr) {
+             const SlotDesc* ws = nullptr, bool pinned = false) {
     const int count = (int)in.size();
     if (count <= 0 || count > p->capacity) throw StatusError(VBE_ERR_ARGUMENT, "invalid number of slots");
     if (ws && count != 1) throw StatusError(VBE_ERR_ARGUMENT, "a replacement workspace serves one slot");
     hipStream_t st = lane.stream;
-    const LaunchShape sh = shape_for(p, in);
+    const LaunchShape sh = shape_for(p, in, pinned);
     if (forward) {
         std::vector<char> seen(p->capacity, 0);
         for (int i = 0; i < count; ++i) {
@@ -752,7 +762,8 @@ double eval_problem(vbe_problem* prob, const double* d_C, const uint8_t* d_suppo
     if (forward) prob->forward_valid = false;
     lane.busy = true;
     try {
-        enqueue(p, lane, in, want_grad, forward);
+        // a single problem's launch shape is the pinned one (the same bits as in any pinned batch)
+        enqueue(p, lane, in, want_grad, forward, nullptr, true);
     } catch (...) {
         lane.busy = false;
         throw;
@@ -779,7 +790,8 @@ void optimize_slots(vbe_pool* p, vbe_problem* prob, const std::vector<int>& slot
         if (prob) prob->forward_valid = false;
         lane.busy = true;
         try {
-            enqueue(p, lane, in, with_grad, true);
+            // pinned: each slot follows the iterates it would follow alone, whatever the batch
+            enqueue(p, lane, in, with_grad, true, nullptr, true);
         } catch (...) {
             lane.busy = false;
             throw;

@@ -17,6 +17,7 @@
 //   k_normalization_pullback  backpropagate_normalization_to_raw_slots (orbitals/pullback/operator.cpp:139-215)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -596,6 +597,8 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
     const int* __restrict__ det_b = topo.det_b;
     int* __restrict__ flags = sd.flags;
     const long long n_pairs = Listed ? (long long)flags[FLAG_DEFERRED] : (long long)n_det * (n_det + 1) / 2;
+    // a list block with no pair stays idle and writes no partial row (k_reduce_partials skips it)
+    if (Listed && (long long)blockIdx.x * (blockDim.x / GROUP) >= n_pairs) return;
 
     extern __shared__ double lds[];
     double* G2 = lds;
@@ -686,10 +689,10 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
             }
             gs.D[s][lig] = dens[s];
         }
-        if (!Backward && live && singular && lig == 0) {
-            // rank-deficient under the complete-pivoting rule: evaluated by k_pairs_singular
-            const int at = atomicAdd(&flags[FLAG_SINGULAR_PAIRS], 1);
-            sd.singular[at] = (int)tp;
+        if (!Backward && ti < n_pairs && lig == 0) {
+            // rank-deficient under the complete-pivoting rule: evaluated by k_pairs_singular, listed in
+            // entry order by k_compact_pairs
+            sd.singular_flag[ti] = live && singular ? 1 : 0;
         }
         live = live && !singular;
         wave_sync();
@@ -848,18 +851,161 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
     }
 }
 
-// sum block partials in a fixed order: out[i] = sum_b partials[b*len + i], one wavefront per i
-__global__ void k_reduce_partials(const SlotDesc* __restrict__ slots, int n_blocks, int len) {
+// sum block partials in a fixed order: out[i] = sum_b partials[b*len + i], one wavefront per i.
+// Rows [list_begin, list_begin + list_rows) belong to the deferred-list blocks, which take
+// list_per_row pairs per sweep; a row whose block had no pair was not written and is skipped.
+__global__ void k_reduce_partials(const SlotDesc* __restrict__ slots, int n_blocks, int len, int list_begin,
+                                  int list_rows, int list_per_row) {
     const SlotDesc& sd = slots[blockIdx.z];
     const double* __restrict__ partials = sd.partials;
     double* __restrict__ out = sd.dsum;
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (i >= len) return;
+    const long long deferred = list_rows > 0 ? (long long)sd.flags[FLAG_DEFERRED] : 0;
     double acc = 0.0;
-    for (int b = lane; b < n_blocks; b += 64) acc += partials[(size_t)b * len + i];
+    for (int b = lane; b < n_blocks; b += 64) {
+        const int j = b - list_begin;
+        if (j >= 0 && j < list_rows && (long long)j * list_per_row >= deferred) continue;
+        acc += partials[(size_t)b * len + i];
+    }
     acc = wave_sum(acc);
     if (lane == 0) out[i] = acc;
+}
+
+// Ordered compaction of a pair list: the flagged entries of a per-entry byte array, listed in entry
+// order. Deferred == true: entries are the lane kernels' work items and each flagged item lists its
+// pair and, when it has one, the spin-flip partner pair (the lane kernels defer the two together);
+// the count goes to FLAG_DEFERRED. Deferred == false: entries are the lane-group kernel's pairs (the
+// deferred list, or every pair when the shape has no lane kernel) and each flagged entry lists its
+// pair; the count goes to FLAG_SINGULAR_PAIRS. Three launches: per-chunk counts, their exclusive
+// scan (one block per slot), and the scatter, each chunk of kCompactChunk entries by one block.
+template <bool Deferred>
+struct CompactView {
+    const uint8_t* flag;
+    int* out;
+    long long entries;
+    __device__ CompactView(const SlotDesc& sd, const TopoDesc& topo, int listed) {
+        flag = Deferred ? sd.defer_flag : sd.singular_flag;
+        out = Deferred ? sd.defer : sd.singular;
+        entries = Deferred ? topo.n_lane_pairs
+                  : listed ? (long long)sd.flags[FLAG_DEFERRED]
+                           : (long long)topo.n_det * (topo.n_det + 1) / 2;
+    }
+    // pairs listed for entry i (0, 1 or 2)
+    __device__ int emit(const SlotDesc& sd, const TopoDesc& topo, int listed, long long i, int& first,
+                        int& second) const {
+        if (i >= entries || !flag[i]) return 0;
+        if (Deferred) {
+            const long long t = topo.lane_pairs ? (long long)topo.lane_pairs[i] : i;
+            long long tf = t;
+            if (topo.flip) {
+                int d, e;
+                unpack_pair(t, d, e);
+                const int a = topo.flip[d], b = topo.flip[e];
+                const long long lo = a < b ? a : b, hi = a < b ? b : a;
+                tf = hi * (hi + 1) / 2 + lo;
+            }
+            first = (int)t;
+            second = (int)tf;
+            return tf != t ? 2 : 1;
+        }
+        first = listed ? sd.defer[i] : (int)i;
+        return 1;
+    }
+};
+
+// wave-level exclusive prefix of per-lane counts (0..2) and the block's per-wave totals (LDS)
+__device__ __forceinline__ int compact_prefix(int count, int* wave_total, int& block_before_wave, int& block_total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const unsigned long long one = __ballot(count >= 1), two = __ballot(count == 2);
+    const int before = __popcll(one & below) + __popcll(two & below);
+    if (lane == 0) wave_total[wave] = __popcll(one) + __popcll(two);
+    __syncthreads();
+    block_before_wave = 0;
+    block_total = 0;
+    for (int w = 0; w < nw; ++w) {
+        if (w < wave) block_before_wave += wave_total[w];
+        block_total += wave_total[w];
+    }
+    __syncthreads();
+    return before;
+}
+
+template <bool Deferred>
+__global__ void __launch_bounds__(kCompactChunk) k_compact_count(const SlotDesc* __restrict__ slots, TopoDesc topo,
+                                                                 int listed) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const CompactView<Deferred> v(sd, topo, listed);
+    __shared__ int wave_total[kCompactChunk / 64];
+    const long long chunks = (v.entries + kCompactChunk - 1) / kCompactChunk;
+    for (long long c = blockIdx.x; c < chunks; c += gridDim.x) {
+        int first, second;
+        const int count = v.emit(sd, topo, listed, c * kCompactChunk + threadIdx.x, first, second);
+        int bw, total;
+        (void)compact_prefix(count, wave_total, bw, total);
+        if (threadIdx.x == 0) sd.chunk_count[c] = total;
+    }
+}
+
+// exclusive scan of the chunk counts in place (one block per slot), total to the slot's flag
+template <bool Deferred>
+__global__ void __launch_bounds__(1024) k_compact_scan(const SlotDesc* __restrict__ slots, TopoDesc topo, int listed) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const CompactView<Deferred> v(sd, topo, listed);
+    const long long chunks = (v.entries + kCompactChunk - 1) / kCompactChunk;
+    __shared__ int part[1024];
+    long long carry = 0;
+    for (long long base = 0; base < chunks; base += blockDim.x) {
+        const long long c = base + threadIdx.x;
+        const int x = c < chunks ? sd.chunk_count[c] : 0;
+        part[threadIdx.x] = x;
+        __syncthreads();
+        // Hillis-Steele inclusive scan over the block
+        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+            const int y = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += y;
+            __syncthreads();
+        }
+        if (c < chunks) sd.chunk_count[c] = (int)(carry + part[threadIdx.x] - x);
+        carry += part[blockDim.x - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sd.flags[Deferred ? FLAG_DEFERRED : FLAG_SINGULAR_PAIRS] = (int)carry;
+}
+
+template <bool Deferred>
+__global__ void __launch_bounds__(kCompactChunk) k_compact_scatter(const SlotDesc* __restrict__ slots, TopoDesc topo,
+                                                                   int listed) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const CompactView<Deferred> v(sd, topo, listed);
+    __shared__ int wave_total[kCompactChunk / 64];
+    const long long chunks = (v.entries + kCompactChunk - 1) / kCompactChunk;
+    for (long long c = blockIdx.x; c < chunks; c += gridDim.x) {
+        int first = 0, second = 0;
+        const int count = v.emit(sd, topo, listed, c * kCompactChunk + threadIdx.x, first, second);
+        int bw, total;
+        const int before = compact_prefix(count, wave_total, bw, total);
+        const long long at = (long long)sd.chunk_count[c] + bw + before;
+        if (count >= 1) v.out[at] = first;
+        if (count == 2) v.out[at + 1] = second;
+    }
+}
+
+// The deferred (Deferred) or rank-deficient list of every slot of a launch, in entry order. The
+// singular list's entry count is read on the device (the deferred count), so it runs after the
+// deferred list's kernels on the same stream.
+template <bool Deferred>
+void compact_pairs(hipStream_t st, const SlotDesc* slots, unsigned z, const TopoDesc& topo, int listed,
+                   long long max_entries) {
+    const unsigned blocks = (unsigned)std::max<long long>(
+        1, std::min<long long>(kCompactBlocks, (max_entries + kCompactChunk - 1) / kCompactChunk));
+    hipLaunchKernelGGL(k_compact_count<Deferred>, dim3(blocks, 1, z), dim3(kCompactChunk), 0, st, slots, topo, listed);
+    hipLaunchKernelGGL(k_compact_scan<Deferred>, dim3(1, 1, z), dim3(1024), 0, st, slots, topo, listed);
+    hipLaunchKernelGGL(k_compact_scatter<Deferred>, dim3(blocks, 1, z), dim3(kCompactChunk), 0, st, slots, topo,
+                       listed);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1954,11 +2100,13 @@ void launch_pairs_variant(hipStream_t st, int group, dim3 grid, int threads, siz
 }
 
 // group kernel over all pairs, or over the deferred list after the lane kernel
-void launch_pairs_group(hipStream_t st, bool backward, bool listed, int blocks, int part_offset, const SlotDesc* slots,
-                        int count, const TopoDesc& topo, int mmax) {
+int launch_pairs_group(hipStream_t st, bool backward, bool listed, int blocks, int part_offset, const SlotDesc* slots,
+                       int count, const TopoDesc& topo, int mmax) {
     const int n = topo.n, ng = topo.ng;
     const int group = pair_group(mmax);
-    const int waves = pairs_waves(n, ng, backward, group);
+    // the backward accumulates into the block's LDS with atomics: one wavefront per block keeps their
+    // order that of the wavefront's own instructions, hence the same bits on every run
+    const int waves = backward ? 1 : pairs_waves(n, ng, backward, group);
     const size_t lds = pairs_lds_bytes(n, ng, backward, waves, group);
     const dim3 grid(blocks, 1, count);
     const int thr = 64 * waves;
@@ -1972,6 +2120,7 @@ void launch_pairs_group(hipStream_t st, bool backward, bool listed, int blocks, 
         if (listed) launch_pairs_variant<true, false, true>(st, group, grid, thr, lds, slots, topo, part_offset);
         else launch_pairs_variant<true, false, false>(st, group, grid, thr, lds, slots, topo, part_offset);
     }
+    return thr / group;  // pairs per block sweep
 }
 
 // Raise the dynamic-LDS ceiling of every LDS-heavy kernel variant to its limit. The attribute is
@@ -2000,6 +2149,36 @@ hipError_t configure_kernels(int n, int ng, int n_str, int mmax) {
 size_t gep_lds_bytes(int n) { return gep_core_doubles(n) * sizeof(double); }
 
 bool gep_in_lds(int n) { return gep_lds_bytes(n) <= kGepLdsLimit; }
+
+// Front half of the eigensolve in global memory: Cholesky S = L L^T (blocked over many workgroups
+// when per_step and the space is large, else one workgroup per problem), L^-1 and the standard-form
+// matrix A = L^-1 H L^-T. Workspace (SlotDesc::gep): L at 0, A at n^2, L^-1 column-major at 2 n^2.
+void launch_gep_front(hipStream_t st, const SlotDesc* slots, unsigned z, const TopoDesc& topo, bool per_step) {
+    const int ns = topo.n_str;
+    if (per_step && ns >= kCholPerStepMinN) {
+        hipLaunchKernelGGL(k_gep_load, dim3((unsigned)(((long long)ns * ns + 255) / 256), 1, z), dim3(256), 0, st,
+                           slots, ns);
+        for (int j0 = 0; j0 < ns; j0 += kCholPanel) {
+            const int jb = std::min(kCholPanel, ns - j0), rest = ns - j0 - jb;
+            hipLaunchKernelGGL(k_chol_diag, dim3(1, 1, z), dim3(256), 0, st, slots, ns, j0, jb);
+            if (rest > 0) {
+                hipLaunchKernelGGL(k_chol_trsm, dim3((unsigned)((rest + 255) / 256), 1, z), dim3(256), 0, st, slots, ns,
+                                   j0, jb);
+                const unsigned tiles = (unsigned)((rest + 15) / 16);
+                hipLaunchKernelGGL(k_chol_syrk, dim3(tiles, tiles, z), dim3(256), 0, st, slots, ns, j0, jb);
+            }
+        }
+    } else {
+        hipLaunchKernelGGL(k_gep_chol, dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
+    }
+    if (ns <= kLinvWaveMaxN)
+        hipLaunchKernelGGL(k_gep_linv, dim3((unsigned)((ns + 3) / 4), 1, z), dim3(256), 4 * (size_t)ns * sizeof(double),
+                           st, slots, ns);
+    else
+        hipLaunchKernelGGL(k_gep_linv_lane, dim3((unsigned)((ns + 63) / 64), 1, z), dim3(64), 0, st, slots, ns);
+    hipLaunchKernelGGL(k_gep_hlt, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
+    hipLaunchKernelGGL(k_gep_lul, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
+}
 
 void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape,
                     hipEvent_t* ev) {
@@ -2046,7 +2225,7 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
             default: hipLaunchKernelGGL(k_ri_pair_factors<14>, grid, dim3(256), 0, st, slots, n); break;
         }
     }
-    if (z <= kGramFewProblems)
+    if (shape.pinned || z <= kGramFewProblems)
         hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4, 1, z), dim3(256), 0, st, slots, np);
     else
         hipLaunchKernelGGL(k_ri_pair_gram_tiled, dim3(1, 1, z), dim3(256), 0, st, slots, np);
@@ -2055,10 +2234,14 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
         launch_pairs_lane(st, 0, shape.lane_blocks_fwd, slots, count, topo, shape.n_alpha, shape.n_beta,
                           shape.lane_cond_limit);
         mark(EV_PAIR_FWD_MAIN);
+        const long long all_pairs = (long long)topo.n_det * (topo.n_det + 1) / 2;
+        compact_pairs<true>(st, slots, z, topo, 1, topo.n_lane_pairs);
         launch_pairs_group(st, false, true, shape.list_blocks, 0, slots, count, topo, shape.mmax);
+        compact_pairs<false>(st, slots, z, topo, 1, all_pairs);
     } else {
         launch_pairs_group(st, false, false, shape.pair_blocks_fwd, 0, slots, count, topo, shape.mmax);
         mark(EV_PAIR_FWD_MAIN);
+        compact_pairs<false>(st, slots, z, topo, 0, (long long)topo.n_det * (topo.n_det + 1) / 2);
     }
     launch_pairs_singular(st, false, 0, slots, count, topo);
     mark(EV_PAIR_FWD);
@@ -2077,30 +2260,8 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
         // Householder kernels (many blocks each); batched launches keep one workgroup per problem
         // for those two stages, which the number of problems already parallelises
         const int ns = topo.n_str;
-        const bool per_step = z <= kGepPerStepMax;
-        if (per_step && ns >= kCholPerStepMinN) {
-            hipLaunchKernelGGL(k_gep_load, dim3((unsigned)(((long long)ns * ns + 255) / 256), 1, z), dim3(256), 0, st,
-                               slots, ns);
-            for (int j0 = 0; j0 < ns; j0 += kCholPanel) {
-                const int jb = std::min(kCholPanel, ns - j0), rest = ns - j0 - jb;
-                hipLaunchKernelGGL(k_chol_diag, dim3(1, 1, z), dim3(256), 0, st, slots, ns, j0, jb);
-                if (rest > 0) {
-                    hipLaunchKernelGGL(k_chol_trsm, dim3((unsigned)((rest + 255) / 256), 1, z), dim3(256), 0, st, slots,
-                                       ns, j0, jb);
-                    const unsigned tiles = (unsigned)((rest + 15) / 16);
-                    hipLaunchKernelGGL(k_chol_syrk, dim3(tiles, tiles, z), dim3(256), 0, st, slots, ns, j0, jb);
-                }
-            }
-        } else {
-            hipLaunchKernelGGL(k_gep_chol, dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
-        }
-        if (ns <= kLinvWaveMaxN)
-            hipLaunchKernelGGL(k_gep_linv, dim3((unsigned)((ns + 3) / 4), 1, z), dim3(256), 4 * (size_t)ns * sizeof(double),
-                               st, slots, ns);
-        else
-            hipLaunchKernelGGL(k_gep_linv_lane, dim3((unsigned)((ns + 63) / 64), 1, z), dim3(64), 0, st, slots, ns);
-        hipLaunchKernelGGL(k_gep_hlt, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
-        hipLaunchKernelGGL(k_gep_lul, dim3((unsigned)((ns + 255) / 256), (unsigned)ns, z), dim3(256), 0, st, slots, ns);
+        const bool per_step = shape.pinned || z <= kGepPerStepMax;
+        launch_gep_front(st, slots, z, topo, per_step);
         if (per_step) {
             hipLaunchKernelGGL((k_gep_lowest<false, 1>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
             for (int k = 0; k + 2 < ns; ++k) {
@@ -2125,7 +2286,7 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
         if (ev) (void)hipEventRecord(ev[k], st);
     };
     mark(EV_BWD_BEGIN);
-    int part_rows = shape.pair_blocks_bwd;
+    int part_rows = shape.pair_blocks_bwd, list_begin = 0, list_rows = 0, list_per_row = 1;
     if (shape.lane_bwd) {
         launch_pairs_lane(st, 1, shape.lane_blocks_bwd, slots, count, topo, shape.n_alpha, shape.n_beta,
                           shape.lane_cond_limit);
@@ -2133,7 +2294,10 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
         launch_pairs_lane(st, 2, shape.lane_blocks_bwd, slots, count, topo, shape.n_alpha, shape.n_beta,
                           shape.lane_cond_limit);
         mark(EV_BWD_ADJOINT);
-        launch_pairs_group(st, true, true, shape.list_blocks, shape.lane_blocks_bwd, slots, count, topo, shape.mmax);
+        list_per_row = launch_pairs_group(st, true, true, shape.list_blocks, shape.lane_blocks_bwd, slots, count, topo,
+                                          shape.mmax);
+        list_begin = shape.lane_blocks_bwd;
+        list_rows = shape.list_blocks;
         part_rows = shape.lane_blocks_bwd + shape.list_blocks;
     } else {
         launch_pairs_group(st, true, false, shape.pair_blocks_bwd, 0, slots, count, topo, shape.mmax);
@@ -2144,13 +2308,14 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
     part_rows += kSingularBlocks;
     mark(EV_BWD_KERNEL);
     const int len = ng + 2 * n * n;
-    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, part_rows, len);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, part_rows, len,
+                       list_begin, list_rows, list_per_row);
     mark(EV_PAIR_BWD);
     hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((shape.max_n_aux + 255) / 256, (np + 7) / 8, z), dim3(256), 0, st, slots,
                        np);
     {
         auto grad = [&]<int NQ>() {
-            if (z <= kGradFewProblems) {
+            if (shape.pinned || z <= kGradFewProblems) {
                 hipLaunchKernelGGL(k_orbital_gradient_few<NQ>,
                                    dim3((unsigned)((shape.max_n_bf + kGradRows - 1) / kGradRows), 1, z), dim3(256), 0,
                                    st, slots, n, ng);

@@ -58,6 +58,9 @@ struct SlotDesc {
     double *partials, *dsum, *dL, *gcn;
     int* defer;     // deferred unordered pair indices (capacity: all pairs)
     int* singular;  // pairs with a rank-deficient block (capacity: all pairs)
+    uint8_t* defer_flag;     // per lane work item: deferred by the lane kernel (listed into defer in order)
+    uint8_t* singular_flag;  // per lane-group entry: rank deficient (listed into singular in order)
+    int* chunk_count;        // list compaction: entries listed per chunk of kCompactChunk, then offsets
     int seeded;     // coef holds this slot's previous structure eigenvector (Davidson start)
 };
 
@@ -98,6 +101,11 @@ struct LaunchShape {
     bool any_tiled, any_flat;              // slots of the launch with the aux-tiled / the flat layout
     int max_n_bf_tiled;                    // largest n_bf among the slots with the aux-tiled layout
     bool davidson;                         // structure eigenproblem by Davidson instead of dense
+    // Pinned launch shape: every choice that changes a slot's summation order (blocks per slot of the
+    // backward pair kernels, the few-problem or batched form of the Gram product, gradient tail and
+    // large eigensolve) is made independently of how many slots the launch holds, so a slot's
+    // energy and gradient are the same bits alone and in any batch.
+    bool pinned;
     double dav_energy_tol, dav_residual_tol, dav_stationarity_tol;
 };
 
@@ -122,10 +130,22 @@ constexpr double kLaneConditionMax = 1.0e9;
 // their rounding matches the complete-pivoting LU closely (typically well under 1% of all pairs)
 constexpr double kLaneConditionDefault = 1.0e5;
 
-// blocks per slot that evaluate the deferred-pair list (and own partial rows after the lane kernel's)
-constexpr int kListBlocks = 4;
+// blocks per slot that evaluate the deferred-pair list (and own partial rows after the lane kernel's):
+// at most kListBlocks; kListBlocksBatched in launches of many problems. A list block whose first
+// pair lies beyond the list does no work and writes no partial row (the reduction skips it).
+constexpr int kListBlocks = 64;
+constexpr int kListBlocksBatched = 4;
+// entries per chunk of the ordered pair-list compaction; chunks handled per launch per slot at most
+constexpr int kCompactChunk = 256;
+constexpr int kCompactBlocks = 64;
+
 // blocks per slot of the singular-pair kernel (partial rows after the pair kernels' rows)
 constexpr int kSingularBlocks = 4;
+
+// Front half of the eigensolve in global memory (Cholesky of S, L^-1, A = L^-1 H L^-T; SlotDesc::gep
+// holds L at 0, A at n^2, L^-1 column-major at 2 n^2); per_step spreads the Cholesky of large spaces
+// over many workgroups.
+void launch_gep_front(hipStream_t st, const SlotDesc* slots, unsigned z, const TopoDesc& topo, bool per_step);
 
 // dense eigensolver workspace: 3 n^2 + 9 n doubles (in LDS for small n), plus two n x n scratch
 // matrices for the multi-workgroup front half of the large-n path
@@ -135,8 +155,8 @@ inline std::size_t gep_work_doubles(int n) { return gep_core_doubles(n) + 2 * (s
 // Davidson subspace limit for one root: min(n, max(32, 8 n_roots)) (make_davidson_options,
 // linalg/eigensolver.cpp:321-361)
 constexpr int kDavidsonMaxSubspace = 32;
-// lowest structure root by block Davidson on the resident H and S (davidson.hip); workspace
-// 3 n min(n, 32) + 9 n doubles of SlotDesc::gep
+// lowest structure root by Davidson in the metric's factored coordinates (davidson.hip): the
+// eigensolver's front half forms A = L^-1 H L^-T, the iteration works from 3 n^2 on in SlotDesc::gep
 void launch_davidson(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape);
 
 int pair_group(int mmax);

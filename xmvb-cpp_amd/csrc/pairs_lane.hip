@@ -289,18 +289,64 @@ __device__ __forceinline__ void compound_pair(int i, int& p, int& s) {
     p = i - s * (s - 1) / 2;
 }
 
-// W[(p,s),(q,t)] -> +W at ((p,q),(s,t)) and -W at ((p,t),(s,q)); block-cooperative
+// Adjoint of the ordered kernel entry (X, Y), X and Y packed orbital pairs, from the same-spin
+// compound accumulator: W[(p,s),(q,t)] (p < s, q < t, summed over `copies` copies of nc x nc)
+// enters with + at (X, Y) = ((p,q),(s,t)) and with - at ((p,t),(s,q)). Terms are taken in a fixed
+// order.
+__device__ __forceinline__ double w_terms(const double* W, int copies, int nc, int X, int Y) {
+    int x0, x1, y0, y1;
+    unpack_pair(X, x0, x1);
+    unpack_pair(Y, y0, y1);
+    const int xa[2] = {x0, x1}, ya[2] = {y0, y1};
+    const int nx = x0 == x1 ? 1 : 2, ny = y0 == y1 ? 1 : 2;
+    double acc = 0.0;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            const int u = xa[i], u2 = xa[1 - i], v = ya[j], v2 = ya[1 - j];
+            // + term: p = u, q = u2, s = v, t = v2
+            if (u < v && u2 < v2) {
+                const int at = compound_index(u, v) * nc + compound_index(u2, v2);
+                for (int c = 0; c < copies; ++c) acc += W[c * nc * nc + at];
+            }
+            // - term: p = u, t = u2, s = v, q = v2
+            if (u < v && v2 < u2) {
+                const int at = compound_index(u, v) * nc + compound_index(v2, u2);
+                for (int c = 0; c < copies; ++c) acc -= W[c * nc * nc + at];
+            }
+        }
+    return acc;
+}
+
+// dg += the expansion of W, every adjoint element computed by one thread (no atomics);
+// block-cooperative. FullDg: dg[X * np + Y] ordered; packed: dg[pair_index(X, Y)] takes both orders.
 template <bool FullDg>
-__device__ void w_to_dg(const double* W, int nc, double* dg, int np) {
-    for (int idx = threadIdx.x; idx < nc * nc; idx += blockDim.x) {
-        const double v = W[idx];
-        if (v == 0.0) continue;
-        int p, s, q, t;
-        compound_pair(idx / nc, p, s);
-        compound_pair(idx % nc, q, t);
-        dg_add<FullDg>(dg, np, pair_index(p, q), pair_index(s, t), v);
-        dg_add<FullDg>(dg, np, pair_index(p, t), pair_index(s, q), -v);
+__device__ void w_gather(const double* W, int copies, int nc, double* dg, int np) {
+    const int targets = FullDg ? np * np : np * (np + 1) / 2;
+    for (int idx = threadIdx.x; idx < targets; idx += blockDim.x) {
+        double acc;
+        if (FullDg) {
+            acc = w_terms(W, copies, nc, idx / np, idx % np);
+        } else {
+            int Y, X;
+            unpack_pair(idx, Y, X);
+            acc = w_terms(W, copies, nc, X, Y);
+            if (X != Y) acc += w_terms(W, copies, nc, Y, X);
+        }
+        dg[idx] += acc;
     }
+}
+
+// Runs body() for each wavefront of the block in turn, wavefront 0 first, with a block barrier
+// between turns, so that accumulations into shared LDS arrive in a fixed wavefront order. Every
+// thread of the block must reach the call (block-uniform control flow).
+template <typename F>
+__device__ __forceinline__ void wave_turns(F&& body) {
+    const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
+    for (int t = 0; t < nw; ++t) {
+        __syncthreads();
+        if (w == t) body();
+    }
+    __syncthreads();
 }
 
 typedef double lane_double4 __attribute__((ext_vector_type(4)));
@@ -409,6 +455,7 @@ struct LaneBwdLayout {
     int nc;       // n(n-1)/2: compound dimension of the same-spin W accumulator (0: no W, direct atomics)
     int np16;     // np rounded up to 16: row length of the MFMA staging rows (16 rows x 2 per wavefront)
     int full_dg;  // 1: the kernel adjoint is held as the full np x np matrix in LDS, 0: packed
+    int wcopies;  // copies of W: one per wavefront (no ordering needed) or 1 (wavefronts take turns)
 };
 
 __device__ __forceinline__ void load_pair_tables(const SlotDesc& sd, int n, int np, double* G2, double* sact,
@@ -452,12 +499,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
         occupied<M1>((uint32_t)tab.mask1[tab.det1[d]], b1.L);
         const bool ok0 = lane_invert<M0>(sact, n, cond_limit, b0);
         const bool ok1 = lane_invert<M1>(sact, n, cond_limit, b1);
-        if (!(ok0 && ok1)) {  // the partner pair has the same blocks: deferred with it
-            const int slot = atomicAdd(&sd.flags[FLAG_DEFERRED], tf != t ? 2 : 1);
-            sd.defer[slot] = (int)t;
-            if (tf != t) sd.defer[slot + 1] = (int)tf;
-            continue;
-        }
+        // deferred work items (the partner pair has the same blocks and goes with it) are flagged
+        // and listed in work-item order by k_compact_pairs, so the list does not depend on timing
+        sd.defer_flag[i] = ok0 && ok1 ? 0 : 1;
+        if (!(ok0 && ok1)) continue;
         // spin-1 packed indices are kept (the opposite-spin loop reuses each one M0^2 times);
         // spin-0 indices are recomputed where needed, which costs VALU only
         int P1[LaneBlock<M1>::K][LaneBlock<M1>::K];
@@ -514,7 +559,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
 //                       cores; it needs no kernel values at all.
 // Both use the same wave-uniform pair loop, so a pair is skipped (zero weight, deferred) by both.
 
-// LDS: G2, S_act, h_act, ds, dh.
+// LDS: G2, S_act, h_act, then ds, dh per wavefront: each wavefront accumulates its pairs into its
+// own copy, in its own instruction order, and the copies are summed in wavefront order at the end,
+// so the block's partial row does not depend on how the wavefronts interleave.
 template <int M0, int M1>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_field_waves(M0))))
     k_pairs_lane_bwd_f(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit) {
@@ -525,10 +572,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
     double* G2 = lds;
     double* sact = G2 + np * np;
     double* hact = sact + n * n;
-    double* ds = hact + n * n;
+    double* ds_all = hact + n * n;  // one (ds, dh) copy per wavefront
+    double* ds = ds_all + (threadIdx.x >> 6) * 2 * n * n;
     double* dh = ds + n * n;
     load_pair_tables(sd, n, np, G2, sact, hact);
-    for (int t = threadIdx.x; t < 2 * n * n; t += blockDim.x) ds[t] = 0.0;
+    for (int t = threadIdx.x; t < 2 * n * n * (int)(blockDim.x >> 6); t += blockDim.x) ds_all[t] = 0.0;
     __syncthreads();
     const SpinTables tab = spin_tables(topo, swap_spins);
     const long long n_work = topo.n_lane_pairs;
@@ -612,12 +660,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
     }
     __syncthreads();
     double* out = sd.partials + (size_t)blockIdx.x * (ng + 2 * n * n) + ng;
-    for (int t = threadIdx.x; t < 2 * n * n; t += blockDim.x) out[t] = ds[t];
+    const int nw = blockDim.x >> 6;
+    for (int t = threadIdx.x; t < 2 * n * n; t += blockDim.x) {
+        double v = 0.0;
+        for (int w = 0; w < nw; ++w) v += ds_all[w * 2 * n * n + t];
+        out[t] = v;
+    }
 }
 
-// LDS: S_act, dg (full np x np or packed), then the optional W (nc x nc) and MFMA staging rows
-// (per wavefront 2 x 16 x np16).
+// LDS: S_act, dg (full np x np or packed), then the optional W (wcopies x nc x nc) and MFMA staging
+// rows (per wavefront 2 x 16 x np16).
 // Fast: W and the MFMA update (when both fit in LDS); otherwise direct LDS atomics into dg.
+// Fixed summation order: every wavefront runs the same number of iterations (block-uniform loop);
+// W is either private per wavefront (summed in wavefront order when it is expanded) or shared and
+// updated by the wavefronts in turn; the MFMA tiles and the direct atomics enter dg in turn; the
+// expansion of W into dg gathers per element. With a given launch shape the partial rows are
+// therefore the same bits on every run.
 template <bool FullDg, bool Fast, int M0, int M1>
 __global__ void __launch_bounds__(256)
     k_pairs_lane_bwd_g(const SlotDesc* __restrict__ slots, TopoDesc topo, int swap_spins, double cond_limit,
@@ -628,22 +686,22 @@ __global__ void __launch_bounds__(256)
     extern __shared__ double lds[];
     double* sact = lds;
     double* dg = sact + n * n;
-    double* W = dg + (FullDg ? np * np : ng);
-    double* bufs = W + lay.nc * lay.nc;
+    double* W_all = dg + (FullDg ? np * np : ng);
+    double* bufs = W_all + lay.wcopies * lay.nc * lay.nc;
     const int wave = threadIdx.x >> 6;
+    const bool private_w = lay.wcopies > 1;
+    double* W = W_all + (private_w ? wave * lay.nc * lay.nc : 0);
     double* bufA = bufs + (size_t)wave * 32 * lay.np16;
     double* bufB = bufA + 16 * lay.np16;
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) sact[t] = sd.sact[t];
     {
-        const int nzero = (FullDg ? np * np : ng) + lay.nc * lay.nc + (blockDim.x >> 6) * 32 * lay.np16;
+        const int nzero =
+            (FullDg ? np * np : ng) + lay.wcopies * lay.nc * lay.nc + (blockDim.x >> 6) * 32 * lay.np16;
         for (int t = threadIdx.x; t < nzero; t += blockDim.x) dg[t] = 0.0;
     }
     __syncthreads();
     const SpinTables tab = spin_tables(topo, swap_spins);
     const long long n_work = topo.n_lane_pairs;
-    const int lane = threadIdx.x & 63;
-    // wave-uniform trip count: the MFMA update needs every lane of the wavefront
-    const long long first = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)));
     const long long stride = (long long)gridDim.x * blockDim.x;
     // 6-electron blocks need np16 = 80 (5 x 5 tiles); their tiles live only for one iteration and
     // are flushed into dg after it, the rest keep theirs in registers for the whole kernel
@@ -654,8 +712,10 @@ __global__ void __launch_bounds__(256)
     for (int tp = 0; tp < TN; ++tp)
 #pragma unroll
         for (int tq = 0; tq < TN; ++tq) tiles[tp][tq] = lane_double4{0.0, 0.0, 0.0, 0.0};
-    for (long long tb = first; tb < n_work; tb += stride) {
-        const long long i = tb + lane;
+    // block-uniform trip count: every wavefront takes part in every turn, and the MFMA update needs
+    // every lane of its wavefront
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < n_work; base += stride) {
+        const long long i = base + threadIdx.x;
         int d = 0, e = 0;
         bool live = i < n_work;
         double w = 0.0;
@@ -686,38 +746,45 @@ __global__ void __launch_bounds__(256)
         auto pidx1 = [&](int r, int l) { return P1[r][l]; };
         const double cw = live ? b0.det * b1.det * w : 0.0;
         if constexpr (Fast) {
-            if (FlushEach && live) {  // pair state dies before the wide tiles come alive
-                same_spin_w<M0>(W, lay.nc, b0, cw);
-                same_spin_w<M1>(W, lay.nc, b1, cw);
+            auto same_spin = [&]() {
+                if (live) {
+                    same_spin_w<M0>(W, lay.nc, b0, cw);
+                    same_spin_w<M1>(W, lay.nc, b1, cw);
+                }
+            };
+            if (FlushEach) {  // pair state dies before the wide tiles come alive
+                if (private_w) same_spin();
+                else wave_turns(same_spin);
             }
             opposite_dg_mfma<M0, M1, TN>(bufA, bufB, lay.np16, live, b0, b1, P1, cw, tiles);
-            if (!FlushEach && live) {
-                same_spin_w<M0>(W, lay.nc, b0, cw);
-                same_spin_w<M1>(W, lay.nc, b1, cw);
+            if (!FlushEach) {
+                if (private_w) same_spin();
+                else wave_turns(same_spin);
             }
-            if (FlushEach) flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, true);
+            if (FlushEach) wave_turns([&]() { flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, true); });
         } else if (live) {
+            // no room for W or a copy of dg per wavefront (n_act 13-14): shared LDS atomics, whose
+            // order follows the wavefronts' interleaving (results reproducible to rounding only);
+            // taking turns here would serialise the ~4,000 atomics per 7x7 pair
             same_spin_dg<FullDg, M0>(dg, np, b0, pidx0, cw);
             same_spin_dg<FullDg, M1>(dg, np, b1, pidx1, cw);
-            {
 #pragma unroll
-                for (int r = 0; r < M0; ++r)
+            for (int r = 0; r < M0; ++r)
 #pragma unroll
-                    for (int l = 0; l < M0; ++l) {
-                        const int P = pidx0(r, l);
-                        const double cwD0 = cw * b0.a[l][r];
+                for (int l = 0; l < M0; ++l) {
+                    const int P = pidx0(r, l);
+                    const double cwD0 = cw * b0.a[l][r];
 #pragma unroll
-                        for (int r2 = 0; r2 < M1; ++r2)
+                    for (int r2 = 0; r2 < M1; ++r2)
 #pragma unroll
-                            for (int l2 = 0; l2 < M1; ++l2) dg_add<FullDg>(dg, np, P, P1[r2][l2], cwD0 * b1.a[l2][r2]);
-                    }
-            }
+                        for (int l2 = 0; l2 < M1; ++l2) dg_add<FullDg>(dg, np, P, P1[r2][l2], cwD0 * b1.a[l2][r2]);
+                }
         }
     }
-    if (Fast && !FlushEach) flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, false);
+    if (Fast && !FlushEach) wave_turns([&]() { flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, false); });
     __syncthreads();
     if (Fast) {
-        w_to_dg<FullDg>(W, lay.nc, dg, np);
+        w_gather<FullDg>(W_all, lay.wcopies, lay.nc, dg, np);
         __syncthreads();
     }
     double* out = sd.partials + (size_t)blockIdx.x * (ng + 2 * n * n);
@@ -749,26 +816,28 @@ size_t lane_fwd_doubles(int n) {
     return np * np + 2 * (size_t)n * n;
 }
 
-size_t lane_bwd_f_doubles(int n) { return lane_fwd_doubles(n) + 2 * (size_t)n * n; }
+size_t lane_bwd_f_doubles(int n) { return lane_fwd_doubles(n) + (size_t)kLaneWaves * 2 * n * n; }
 
 size_t lane_bwd_g_doubles(int n, int ng, const LaneBwdLayout& lay) {
     const size_t np = (size_t)n * (n + 1) / 2;
-    return (size_t)n * n + (lay.full_dg ? np * np : (size_t)ng) + (size_t)lay.nc * lay.nc +
+    return (size_t)n * n + (lay.full_dg ? np * np : (size_t)ng) + (size_t)lay.wcopies * lay.nc * lay.nc +
            (size_t)kLaneWaves * 32 * lay.np16;
 }
 
 // W and the MFMA staging rows when both fit in LDS, else neither (direct atomics). The adjoint is
 // held as the full np x np matrix when that fits too, else packed (n = 11, 12 with 6-electron
 // blocks). m0: the larger spin block, which sets how many MFMA tiles the kernel can carry.
-LaneBwdLayout lane_bwd_layout(int n, int ng, int m0) {
+LaneBwdLayout lane_bwd_layout(int n, int ng, int m0) {  // W copies: one per wavefront when they fit
     const int np = n * (n + 1) / 2;
     const int tn = m0 >= 6 ? kMfmaTilesWide : kMfmaTiles;
     const int full = pairs_full_dg(n, ng) ? 1 : 0;
-    for (const int f : {full, 0}) {
-        const LaneBwdLayout fast{n * (n - 1) / 2, (np + 15) / 16 * 16, f};
-        if (fast.np16 <= 16 * tn && lane_bwd_g_doubles(n, ng, fast) * sizeof(double) <= kLaneLdsLimit) return fast;
-    }
-    return {0, 0, full};
+    for (const int f : {full, 0})
+        for (const int copies : {kLaneWaves, 1}) {
+            const LaneBwdLayout fast{n * (n - 1) / 2, (np + 15) / 16 * 16, f, copies};
+            if (fast.np16 <= 16 * tn && lane_bwd_g_doubles(n, ng, fast) * sizeof(double) <= kLaneLdsLimit)
+                return fast;
+        }
+    return {0, 0, full, 1};
 }
 
 template <int M0, int M1>

@@ -496,12 +496,14 @@ void launch_pairs_singular(hipStream_t st, bool backward, int part_offset, const
                            const TopoDesc& topo) {
     const dim3 grid(kSingularBlocks, 1, count);
     const size_t lds = pairs_singular_lds_bytes(topo.n, topo.ng, backward);
+    // the backward accumulates with LDS atomics: one wavefront per block keeps their order that of
+    // the wavefront's own instructions, hence the same bits on every run
     if (!backward)
         hipLaunchKernelGGL((k_pairs_singular<false, false>), grid, dim3(256), lds, st, slots, topo, part_offset);
     else if (pairs_full_dg(topo.n, topo.ng))
-        hipLaunchKernelGGL((k_pairs_singular<true, true>), grid, dim3(256), lds, st, slots, topo, part_offset);
+        hipLaunchKernelGGL((k_pairs_singular<true, true>), grid, dim3(64), lds, st, slots, topo, part_offset);
     else
-        hipLaunchKernelGGL((k_pairs_singular<true, false>), grid, dim3(256), lds, st, slots, topo, part_offset);
+        hipLaunchKernelGGL((k_pairs_singular<true, false>), grid, dim3(64), lds, st, slots, topo, part_offset);
 }
 
 }  // namespace xmvb_amd

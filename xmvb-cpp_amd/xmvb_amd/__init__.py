@@ -369,8 +369,10 @@ class Context:
         self.check(self._lib.vbe_set_pair_condition_limit(self._h, C.c_double(limit)))
 
     def set_eigensolver(self, eigensolver: int, energy_tolerance: float = 1e-7, residual_tolerance: float = 1e-3,
-                        stationarity_tolerance: float = 1e-7, reuse_roots: bool = True):
-        """0: dense GEP; 1: Davidson (tolerances as StructureSolveAccuracy). Applies to problems and
+                        stationarity_tolerance: float = 1.0, reuse_roots: bool = True):
+        """0: dense GEP; 1: Davidson in the metric's factored coordinates with the reference's physical
+        stopping: Ritz energy stable to min(energy_tolerance, 1e-8), stationarity residual at most
+        the square root of that, or stationarity_tolerance when smaller. Applies to problems and
         pools created afterwards; Problem.evaluate's `eigensolver` argument selects it per call."""
         self.check(self._lib.vbe_set_eigensolver(self._h, eigensolver, energy_tolerance, residual_tolerance,
                                                  stationarity_tolerance, int(reuse_roots)))
