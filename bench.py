@@ -193,10 +193,11 @@ def serve(vb, ctx, scans, rate: float, seconds: float, max_batch: int, replicas:
                 si, p = c_pick[nxt_c]
                 worker.submit((si, p), scans[si][2][p].C, arrival=t0 + c_arrivals[nxt_c])
                 nxt_c += 1
-        done += len(server.pump())
-        if not server.busy() and nxt < n_req:
-            time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 2e-4)))
-    makespan = time.perf_counter() - t0
+        # the engine's dispatcher launches on its own thread; this loop only submits on time and
+        # collects, waiting inside the engine until the next arrival
+        gap = arrivals[nxt] - (time.perf_counter() - t0) if nxt < n_req else 2e-3
+        done += len(server.pump(wait_us=int(max(0.0, min(gap, 2e-3)) * 1e6)))
+    makespan = max(r.done for r in reqs) - t0
     converge = None
     if worker is not None:
         while nxt_c < n_conv:  # the rest of the convergence stream, at its own arrival times
@@ -220,6 +221,10 @@ def serve(vb, ctx, scans, rate: float, seconds: float, max_batch: int, replicas:
                         makespan_s=time.perf_counter() - t0)
         worker.close()
     lat = [r.latency * 1e3 for r in reqs]
+    queue_ms = [r.queue_ms for r in reqs]
+    device_ms = [r.device_ms for r in reqs]
+    host_ms = [r.latency * 1e3 - r.queue_ms - r.device_ms for r in reqs]
+    assert all(r.status == 0 for r in reqs)
     err = max(abs(r.energy - e_ref[(r.key[0], r.key[1])]) for r in reqs)
     per_scan = {}
     for si, (name, topo, sps) in enumerate(scans):
@@ -228,10 +233,17 @@ def serve(vb, ctx, scans, rate: float, seconds: float, max_batch: int, replicas:
             per_scan[name] = dict(requests=len(ls), n_act=topo.n_act, n_bf=sps[0].n_bf,
                                   p50=percentile(ls, 0.5), p99=percentile(ls, 0.99))
     used, budget = ctx.memory()
-    sizes = server.batch_sizes[len(server.batch_sizes) - (server.launches - launches0):]
+    launches = server.launches - launches0
     out = dict(requests=n_req, offered_rate=rate, throughput=n_req / makespan, seconds=makespan,
                latency_ms=dict(p50=percentile(lat, 0.5), p99=percentile(lat, 0.99), max=max(lat)),
-               mean_batch=float(np.mean(sizes)), launches=server.launches - launches0,
+               latency_parts_ms=dict(
+                   queue=dict(p50=percentile(queue_ms, 0.5), p99=percentile(queue_ms, 0.99)),
+                   device=dict(p50=percentile(device_ms, 0.5), p99=percentile(device_ms, 0.99)),
+                   host=dict(p50=percentile(host_ms, 0.5), p99=percentile(host_ms, 0.99)),
+                   note="queue: arrival to launch (waiting for a free lane of the pool or for the slot); "
+                        "device: the launch on the device (HIP events); host: the rest (copies, stream "
+                        "order behind other launches, completion polling)"),
+               mean_batch=n_req / max(1, launches), launches=launches,
                max_abs_dE_vs_direct=float(err),
                scan=f"{len(scans)} scan(s) x {len(scans[0][2])} points r in [0.5, 4.0], {replicas} resident "
                     f"replica(s) per point ({n_points * replicas} problems)",

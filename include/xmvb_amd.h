@@ -386,6 +386,47 @@ int vbe_optimize(vbe_context* ctx, vbe_problem* prob, double* C_raw, const uint8
 int vbe_pool_optimize(vbe_pool* pool, int32_t count, const int32_t* slots, double* C_raw, const uint8_t* support,
                       const vbe_optimizer_options* options, vbe_optimizer_result* results, double* grad_raw);
 
+/* ---- request scheduler (no reference counterpart: the reference evaluates one problem per
+ * process, src/cli/run.cpp; SURVEY's asynchronous vbe_submit / vbe_poll for serving) ----------
+ * A native dispatcher thread batches E(+grad) requests for loaded pool slots continuously: when a
+ * pool has a free launch lane, every queued request of that pool whose slot is not in flight (up to
+ * max_batch, one gradient flag per launch) goes into one launch. Pools' lanes run on their context's
+ * shared streams (at most four per context). Times are CLOCK_MONOTONIC seconds (Python's
+ * time.perf_counter on Linux). The caller keeps each request's gradient buffer valid until the
+ * request is collected, and must not unload a slot that has requests queued or in flight. */
+typedef struct vbe_server vbe_server;
+
+typedef struct vbe_request_info {
+    int64_t id;
+    int32_t status;      /* VBE_OK or the launch's status for this slot */
+    int32_t batch_size;  /* requests in the launch that served it */
+    double energy;       /* E_total */
+    double t_arrival;    /* as passed to vbe_server_submit (its submit time when <= 0) */
+    double t_submit;     /* accepted into the queue */
+    double t_dispatch;   /* launched */
+    double t_complete;   /* results copied out */
+    double device_ms;    /* device time of its launch (forward and backward, HIP events) */
+} vbe_request_info;
+
+typedef struct vbe_server_counters {
+    int64_t queued, in_flight_launches, launches, launched_requests, completed, uncollected;
+} vbe_server_counters;
+
+/* max_inflight_per_pool: launches of one pool in flight at once, 1..4 (its launch lanes). */
+int vbe_server_create(vbe_context* ctx, int32_t max_batch, int32_t max_inflight_per_pool, vbe_server** out);
+/* Completes every queued and in-flight request, stops the dispatcher and frees the server. */
+void vbe_server_destroy(vbe_server* server);
+/* Queue one request for slot `slot` of `pool`: C (n_act x n_bf of the slot, host, copied at the
+ * call), want_gradient, grad_out (n_act x n_bf host buffer, written on completion; NULL without
+ * gradient), arrival (the request's arrival time for latency accounting, <= 0: now). */
+int vbe_server_submit(vbe_server* server, vbe_pool* pool, int32_t slot, const double* C, int32_t want_gradient,
+                      double* grad_out, double arrival, int64_t* id);
+/* Completed requests, oldest first, up to max_count; waits up to wait_us microseconds for one when
+ * none is ready. *count may be 0. */
+int vbe_server_collect(vbe_server* server, int32_t max_count, int32_t wait_us, vbe_request_info* out,
+                       int32_t* count);
+int vbe_server_stats(vbe_server* server, vbe_server_counters* counters);
+
 #ifdef __cplusplus
 }
 #endif

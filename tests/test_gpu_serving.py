@@ -118,6 +118,42 @@ def test_served_mix_matches_alone():
     ctx.close()
 
 
+@pytest.mark.parametrize("n_act,n_bf", [(10, 200), (12, 120)])
+def test_served_scan_at_serving_sizes(n_act, n_bf):
+    """Points of a bond-stretch scan (scan_r 0.5 to 4.0, n_bf up to 200, n_aux = 3 n_bf) served
+    through the native scheduler: every result equals the point evaluated alone, and the reported
+    times are consistent (queue + device <= latency, device time positive)."""
+    ctx = vb.Context(0)
+    topo = vb.Topology.build("COV", n_act, n_act, 1)
+    rs = (0.5, 1.5, 2.5, 4.0)
+    sps = [vb.synthetic_problem(1500 + k, n_bf - 12 * k, n_act, 3 * (n_bf - 12 * k), scan_r=r)
+           for k, r in enumerate(rs)]
+    server = serving.Server(ctx, capacity=8, max_batch=16)
+    for k, sp in enumerate(sps):
+        server.register(k, topo, _loader(sp))
+    print(f"case served scan n_act={n_act} n_bf<={n_bf}, 48 requests", flush=True)
+    reqs = []
+    for i in range(48):
+        k = i % len(sps)
+        reqs.append(server.submit(k, sps[k].C, gradient=bool(i % 5)))
+        if i % 3 == 0:
+            server.pump()
+    server.drain()
+    ref = [_alone(ctx, topo, sp) for sp in sps]
+    for r in reqs:
+        assert r.status == 0
+        e0, g0 = ref[r.key]
+        assert abs(r.energy - e0) <= 1e-10 * max(1.0, abs(e0)), (r.key, r.energy, e0)
+        if r.gradient:
+            assert float(np.abs(r.grad - g0).max()) <= 1e-9
+        else:
+            assert r.grad is None
+        assert r.device_ms > 0.0 and r.queue_ms >= 0.0 and 1 <= r.batch <= 16
+        assert r.queue_ms + r.device_ms <= r.latency * 1e3 + 1e-3
+    server.close()
+    ctx.close()
+
+
 def test_admission_under_budget_evicts_and_reloads():
     topo = vb.Topology.build("COV", 6, 6, 1)
     sps = [vb.synthetic_problem(700 + k, 40, 6, 120) for k in range(5)]

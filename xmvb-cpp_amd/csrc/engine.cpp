@@ -24,6 +24,7 @@
 #include "kernels.hpp"
 #include "noalloc.hpp"
 #include "optimizer.hpp"
+#include "server.hpp"
 
 namespace {
 
@@ -44,7 +45,8 @@ void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-constexpr int kLanes = 2;              // streams of a pool (in-flight submissions)
+constexpr int kLanes = 4;              // launch lanes of a pool (in-flight submissions)
+constexpr int kContextStreams = 4;     // HIP streams of a context, shared by all of its pools
 constexpr int kMaxBwdBlocksPerSlot = 512;
 constexpr int kLaneBlocks = 2048;      // default one-pair-per-lane blocks per launch (all slots)
 constexpr int kAutotuneTargets[VBE_AUTOTUNE_CANDIDATES] = {256, 512, 1024, 2048, 4096, 8192};
@@ -63,9 +65,33 @@ struct vbe_context {
     double dav_energy_tol = 1.0e-7, dav_residual_tol = 1.0e-3, dav_stationarity_tol = 1.0;
     bool dav_reuse_roots = true;
     std::string error;
-    // guards in_use and error: pools of one context may be driven from several host threads
-    // (for example a serving loop beside a convergence worker), one thread per pool
+    // guards in_use, error and the stream cursor: pools of one context may be driven from several
+    // host threads (for example the request scheduler beside a convergence worker)
     std::mutex mu;
+    // The context's streams (created with its first pool). Every pool's launch lanes take their
+    // streams from this set, so however many pools are resident, the device sees at most
+    // kContextStreams queues of this context, one per hardware queue; launches of different pools
+    // on one stream run in submission order. Completion is tracked per lane with events, never by
+    // synchronizing a shared stream.
+    std::vector<hipStream_t> streams;
+    int next_stream = 0;
+    hipStream_t take_stream() {
+        std::lock_guard<std::mutex> g(mu);
+        if (streams.empty()) {
+            for (int k = 0; k < kContextStreams; ++k) {
+                hipStream_t st = nullptr;
+                check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+                streams.push_back(st);
+            }
+        }
+        return streams[next_stream++ % streams.size()];
+    }
+    ~vbe_context() {
+        for (hipStream_t st : streams) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    }
 };
 
 namespace {
@@ -168,9 +194,13 @@ struct vbe_pool {
     int fwd_target = kLaneBlocks;
     int bwd_target = kLaneBlocks;
 
+    // serializes host access to the pool's lanes and slots between the caller's thread and the
+    // request scheduler's dispatcher thread
+    std::recursive_mutex mu;
+
     ~vbe_pool() {
         for (auto& l : lanes) {
-            if (l.stream) (void)hipStreamSynchronize(l.stream);
+            if (l.done) (void)hipEventSynchronize(l.done);  // the lane's last launch (streams are shared)
             for (auto& e : l.ev)
                 if (e) (void)hipEventDestroy(e);
             if (l.done) (void)hipEventDestroy(l.done);
@@ -179,7 +209,6 @@ struct vbe_pool {
             if (l.h_flags) (void)hipHostFree(l.h_flags);
             if (l.h_C) (void)hipHostFree(l.h_C);
             if (l.h_grad) (void)hipHostFree(l.h_grad);
-            if (l.stream) (void)hipStreamDestroy(l.stream);
         }
     }
 };
@@ -276,7 +305,7 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     p->capacity = capacity;
     p->topo_arena.ctx = ctx;
     for (auto& l : p->lanes) {
-        check(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking), "hipStreamCreate");
+        l.stream = ctx->take_stream();
         for (auto& e : l.ev) check(hipEventCreate(&e), "hipEventCreate");
         check(hipEventCreateWithFlags(&l.done, hipEventDisableTiming), "hipEventCreate");
         check(hipHostMalloc(reinterpret_cast<void**>(&l.h_desc), sizeof(SlotDesc) * capacity, hipHostMallocDefault),
@@ -650,6 +679,8 @@ void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool wit
           "energy D2H");
     check(hipMemcpyAsync(lane.h_flags, lane.d_flags, sizeof(int) * FLAG_COUNT * count, hipMemcpyDeviceToHost, st),
           "flags D2H");
+    // the lane's completion point (the stream may carry other pools' launches after it)
+    check(hipEventRecord(lane.done, st), "hipEventRecord");
     lane.slots.clear();
     for (const auto& x : in) lane.slots.push_back(x.slot);
     lane.with_grad = with_grad;
@@ -666,7 +697,7 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 // Wait for a lane, record phase times, raise on device flags (or, with slot_status, report each
 // listed slot's status instead of raising).
 void finish(vbe_pool* p, Lane& lane, std::vector<int>* slot_status = nullptr) {
-    check(hipStreamSynchronize(lane.stream), "hipStreamSynchronize");
+    check(hipEventSynchronize(lane.done), "hipEventSynchronize");
     lane.busy = false;
     using namespace xmvb_amd;
     if (lane.events_valid) {
@@ -865,6 +896,130 @@ int eval_device_in_scratch(vbe_context* ctx, vbe_problem* prob, const double* d_
     });
 }
 
+// ---- request-scheduler hooks ------------------------------------------------------------------
+
+int pool_device(const vbe_pool* p) { return p->ctx->device; }
+
+int context_streams(vbe_context* ctx, void** out, int max_streams) {
+    (void)ctx->take_stream();  // creates the set on first use
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const int n = std::min<int>(max_streams, (int)ctx->streams.size());
+    for (int k = 0; k < n; ++k) out[k] = ctx->streams[k];
+    return n;
+}
+
+int pool_slot_dims(vbe_pool* p, int slot, int* n_act, int* n_bf, std::string* err) {
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
+    if (slot < 0 || slot >= p->capacity || !p->slots[slot].loaded) {
+        *err = "slot " + std::to_string(slot) + " is not loaded";
+        return VBE_ERR_ARGUMENT;
+    }
+    *n_act = p->n_act;
+    *n_bf = p->slots[slot].n_bf;
+    return VBE_OK;
+}
+
+bool pool_slot_busy(vbe_pool* p, int slot) {
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
+    for (const auto& l : p->lanes)
+        if (l.busy && std::find(l.slots.begin(), l.slots.end(), slot) != l.slots.end()) return true;
+    return false;
+}
+
+int pool_launch(vbe_pool* p, int count, const int* slots, const double* const* C, bool gradient, void* stream,
+                int* lane_out, std::string* err) {
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
+    *lane_out = -1;
+    Lane* lane = nullptr;
+    for (int k = 0; k < kLanes && !lane; ++k)
+        if (!p->lanes[(p->last_lane + 1 + k) % kLanes].busy) lane = &p->lanes[(p->last_lane + 1 + k) % kLanes];
+    if (!lane) return VBE_OK;
+    try {
+        check(hipSetDevice(p->ctx->device), "hipSetDevice");
+        check_slots_free(p, slots, count);
+        // an idle lane may change streams: its events are recorded afresh on the new one
+        if (stream) lane->stream = static_cast<hipStream_t>(stream);
+        size_t total = 0;
+        for (int i = 0; i < count; ++i) {
+            if (slots[i] < 0 || slots[i] >= p->capacity || !p->slots[slots[i]].loaded)
+                throw StatusError(VBE_ERR_ARGUMENT, "slot not loaded");
+            total += (size_t)p->n_act * p->slots[slots[i]].n_bf;
+        }
+        ensure_stage(p, *lane, total);
+        std::vector<SlotInput> in;
+        lane->grad_offsets.clear();
+        size_t off = 0;
+        for (int i = 0; i < count; ++i) {
+            const size_t len = (size_t)p->n_act * p->slots[slots[i]].n_bf;
+            std::memcpy(lane->h_C + off, C[i], len * sizeof(double));
+            in.push_back({slots[i], lane->d_C + off, nullptr, gradient ? lane->d_grad + off : nullptr});
+            lane->grad_offsets.push_back(off);
+            off += len;
+        }
+        lane->grad_offsets.push_back(total);
+        upload(lane->d_C, lane->h_C, total, lane->stream);
+        lane->busy = true;
+        try {
+            enqueue(p, *lane, in, gradient, true);
+            if (gradient) {
+                check(hipMemcpyAsync(lane->h_grad, lane->d_grad, total * sizeof(double), hipMemcpyDeviceToHost,
+                                     lane->stream),
+                      "gradient D2H");
+                check(hipEventRecord(lane->done, lane->stream), "hipEventRecord");
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(lane->stream);
+            lane->busy = false;
+            throw;
+        }
+        lane->ticket = -1;  // owned by the scheduler, not a caller ticket
+        p->last_lane = int(lane - p->lanes);
+        *lane_out = p->last_lane;
+        return VBE_OK;
+    } catch (const StatusError& e) {
+        *err = e.what();
+        return e.status;
+    } catch (const std::exception& e) {
+        *err = e.what();
+        return VBE_ERR_DEVICE;
+    }
+}
+
+int pool_launch_query(vbe_pool* p, int lane, std::string* err) {
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
+    const hipError_t q = hipEventQuery(p->lanes[lane].done);
+    if (q == hipErrorNotReady) return 0;
+    if (q != hipSuccess) {
+        *err = std::string("hipEventQuery: ") + hipGetErrorString(q);
+        return -VBE_ERR_DEVICE;
+    }
+    return 1;
+}
+
+int pool_launch_collect(vbe_pool* p, int lane_index, double* energies, double* const* grads, int* status,
+                        float* device_ms, std::string* err) {
+    std::lock_guard<std::recursive_mutex> lk(p->mu);
+    Lane& lane = p->lanes[lane_index];
+    try {
+        std::vector<int> st;
+        const size_t count = lane.slots.size();
+        finish(p, lane, &st);
+        *device_ms = p->times.total_ms;
+        for (size_t i = 0; i < count; ++i) {
+            energies[i] = lane.h_energy[2 * i + 1];
+            status[i] = st[i];
+            if (lane.with_grad && grads && grads[i])
+                std::memcpy(grads[i], lane.h_grad + lane.grad_offsets[i],
+                            (lane.grad_offsets[i + 1] - lane.grad_offsets[i]) * sizeof(double));
+        }
+        return VBE_OK;
+    } catch (const std::exception& e) {
+        lane.busy = false;
+        *err = e.what();
+        return VBE_ERR_DEVICE;
+    }
+}
+
 }  // namespace xmvb_amd
 
 extern "C" {
@@ -956,6 +1111,7 @@ int vbe_pool_load(vbe_pool* pool, int32_t slot, int32_t n_bf, int32_t n_aux, con
                   const double* B_packed, double e_nuc) {
     return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
         if (!pool) throw StatusError(VBE_ERR_ARGUMENT, "null pool");
+        std::lock_guard<std::recursive_mutex> lk(pool->mu);
         load_slot(pool, slot, n_bf, n_aux, S_ao, h_ao, B_packed, e_nuc);
         return VBE_OK;
     });
@@ -964,6 +1120,7 @@ int vbe_pool_load(vbe_pool* pool, int32_t slot, int32_t n_bf, int32_t n_aux, con
 int vbe_pool_unload(vbe_pool* pool, int32_t slot) {
     return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
         if (!pool) throw StatusError(VBE_ERR_ARGUMENT, "null pool");
+        std::lock_guard<std::recursive_mutex> lk(pool->mu);
         if (slot < 0 || slot >= pool->capacity) throw StatusError(VBE_ERR_ARGUMENT, "slot index out of range");
         check_slots_free(pool, &slot, 1);
         check(hipSetDevice(pool->ctx->device), "hipSetDevice");
@@ -1004,6 +1161,7 @@ int vbe_pool_eval_device(vbe_pool* pool, int32_t count, const int32_t* slots, co
                          int32_t want_gradient, double* energies, double* d_grad) {
     return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
         if (!pool || !slots || !d_C || !energies || count <= 0) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(pool->mu);
         if (want_gradient && !d_grad) throw StatusError(VBE_ERR_ARGUMENT, "null gradient buffer");
         check(hipSetDevice(pool->ctx->device), "hipSetDevice");
         Lane& lane = acquire_lane(pool);
@@ -1035,6 +1193,7 @@ int vbe_pool_submit(vbe_pool* pool, int32_t count, const int32_t* slots, const d
                     int32_t* ticket) {
     return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
         if (!pool || !slots || !C || !ticket || count <= 0) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(pool->mu);
         if (count > pool->capacity) throw StatusError(VBE_ERR_ARGUMENT, "more slots than the pool capacity");
         check(hipSetDevice(pool->ctx->device), "hipSetDevice");
         Lane& lane = acquire_lane(pool);
@@ -1080,6 +1239,7 @@ int vbe_pool_submit(vbe_pool* pool, int32_t count, const int32_t* slots, const d
 int vbe_pool_poll(vbe_pool* pool, int32_t ticket, int32_t* done) {
     return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
         if (!pool || !done) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(pool->mu);
         for (auto& lane : pool->lanes) {
             if (!lane.busy || lane.ticket != ticket) continue;
             const hipError_t q = hipEventQuery(lane.done);
@@ -1098,6 +1258,7 @@ int vbe_pool_poll(vbe_pool* pool, int32_t ticket, int32_t* done) {
 int vbe_pool_wait(vbe_pool* pool, int32_t ticket, double* energies, double* grads) {
     return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
         if (!pool || !energies) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(pool->mu);
         for (auto& lane : pool->lanes) {
             if (!lane.busy || lane.ticket != ticket) continue;
             const size_t count = lane.slots.size();
@@ -1117,6 +1278,7 @@ int vbe_pool_autotune(vbe_pool* pool, int32_t count, const int32_t* slots, const
         if (pool) fail(pool->ctx, VBE_ERR_ARGUMENT, "vbe_pool_autotune: invalid argument");
         return VBE_ERR_ARGUMENT;
     }
+    std::lock_guard<std::recursive_mutex> lk(pool->mu);
     size_t total = 0;
     for (int i = 0; i < count; ++i) {
         if (slots[i] < 0 || slots[i] >= pool->capacity || !pool->slots[slots[i]].loaded)
@@ -1315,6 +1477,7 @@ int vbe_pool_optimize(vbe_pool* pool, int32_t count, const int32_t* slots, doubl
     return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
         if (!pool || !slots || !C_raw || !options || !results || count <= 0)
             throw StatusError(VBE_ERR_ARGUMENT, "null argument or empty slot list");
+        std::lock_guard<std::recursive_mutex> lk(pool->mu);
         if (count > pool->capacity) throw StatusError(VBE_ERR_ARGUMENT, "more slots than the pool holds");
         for (const auto& l : pool->lanes)
             if (l.busy) throw StatusError(VBE_ERR_ARGUMENT, "the pool has submissions in flight; wait for them first");

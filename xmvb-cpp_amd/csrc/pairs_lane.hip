@@ -456,6 +456,8 @@ struct LaneBwdLayout {
     int np16;     // np rounded up to 16: row length of the MFMA staging rows (16 rows x 2 per wavefront)
     int full_dg;  // 1: the kernel adjoint is held as the full np x np matrix in LDS, 0: packed
     int wcopies;  // copies of W: one per wavefront (no ordering needed) or 1 (wavefronts take turns)
+    int dcopies;  // copies of the kernel adjoint: one per wavefront or 1 (MFMA flushes take turns)
+    int waves;    // wavefronts per block
 };
 
 __device__ __forceinline__ void load_pair_tables(const SlotDesc& sd, int n, int np, double* G2, double* sact,
@@ -685,19 +687,20 @@ __global__ void __launch_bounds__(256)
     const int np = n * (n + 1) / 2;
     extern __shared__ double lds[];
     double* sact = lds;
-    double* dg = sact + n * n;
-    double* W_all = dg + (FullDg ? np * np : ng);
+    const int dglen = FullDg ? np * np : ng;
+    double* dg_all = sact + n * n;
+    double* W_all = dg_all + lay.dcopies * dglen;
     double* bufs = W_all + lay.wcopies * lay.nc * lay.nc;
     const int wave = threadIdx.x >> 6;
-    const bool private_w = lay.wcopies > 1;
+    const bool private_w = lay.wcopies > 1, private_dg = lay.dcopies > 1;
+    double* dg = dg_all + (private_dg ? wave * dglen : 0);
     double* W = W_all + (private_w ? wave * lay.nc * lay.nc : 0);
     double* bufA = bufs + (size_t)wave * 32 * lay.np16;
     double* bufB = bufA + 16 * lay.np16;
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) sact[t] = sd.sact[t];
     {
-        const int nzero =
-            (FullDg ? np * np : ng) + lay.wcopies * lay.nc * lay.nc + (blockDim.x >> 6) * 32 * lay.np16;
-        for (int t = threadIdx.x; t < nzero; t += blockDim.x) dg[t] = 0.0;
+        const int nzero = lay.dcopies * dglen + lay.wcopies * lay.nc * lay.nc + (blockDim.x >> 6) * 32 * lay.np16;
+        for (int t = threadIdx.x; t < nzero; t += blockDim.x) dg_all[t] = 0.0;
     }
     __syncthreads();
     const SpinTables tab = spin_tables(topo, swap_spins);
@@ -761,7 +764,10 @@ __global__ void __launch_bounds__(256)
                 if (private_w) same_spin();
                 else wave_turns(same_spin);
             }
-            if (FlushEach) wave_turns([&]() { flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, true); });
+            if (FlushEach) {
+                if (private_dg) flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, true);
+                else wave_turns([&]() { flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, true); });
+            }
         } else if (live) {
             // no room for W or a copy of dg per wavefront (n_act 13-14): shared LDS atomics, whose
             // order follows the wavefronts' interleaving (results reproducible to rounding only);
@@ -781,8 +787,20 @@ __global__ void __launch_bounds__(256)
                 }
         }
     }
-    if (Fast && !FlushEach) wave_turns([&]() { flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, false); });
+    if (Fast && !FlushEach) {
+        if (private_dg) flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, false);
+        else wave_turns([&]() { flush_mfma_tiles<FullDg, TN>(tiles, lay.np16, np, dg, false); });
+    }
     __syncthreads();
+    dg = dg_all;
+    if (private_dg) {  // the wavefronts' copies, summed in wavefront order
+        for (int t = threadIdx.x; t < dglen; t += blockDim.x) {
+            double v = dg_all[t];
+            for (int w = 1; w < lay.dcopies; ++w) v += dg_all[(size_t)w * dglen + t];
+            dg_all[t] = v;
+        }
+        __syncthreads();
+    }
     if (Fast) {
         w_gather<FullDg>(W_all, lay.wcopies, lay.nc, dg, np);
         __syncthreads();
@@ -820,24 +838,31 @@ size_t lane_bwd_f_doubles(int n) { return lane_fwd_doubles(n) + (size_t)kLaneWav
 
 size_t lane_bwd_g_doubles(int n, int ng, const LaneBwdLayout& lay) {
     const size_t np = (size_t)n * (n + 1) / 2;
-    return (size_t)n * n + (lay.full_dg ? np * np : (size_t)ng) + (size_t)lay.wcopies * lay.nc * lay.nc +
-           (size_t)kLaneWaves * 32 * lay.np16;
+    return (size_t)n * n + (size_t)lay.dcopies * (lay.full_dg ? np * np : (size_t)ng) +
+           (size_t)lay.wcopies * lay.nc * lay.nc + (size_t)lay.waves * 32 * lay.np16;
 }
 
 // W and the MFMA staging rows when both fit in LDS, else neither (direct atomics). The adjoint is
 // held as the full np x np matrix when that fits too, else packed (n = 11, 12 with 6-electron
 // blocks). m0: the larger spin block, which sets how many MFMA tiles the kernel can carry.
-LaneBwdLayout lane_bwd_layout(int n, int ng, int m0) {  // W copies: one per wavefront when they fit
+// Preference: four wavefronts with a private W each (the MFMA tiles enter the shared adjoint once,
+// in turns, at the end); else two wavefronts with a private W and a private adjoint each (6-electron
+// blocks, whose tiles are flushed after every iteration: no turns at all); else four wavefronts
+// taking turns on one W and one adjoint.
+LaneBwdLayout lane_bwd_layout(int n, int ng, int m0) {
     const int np = n * (n + 1) / 2;
     const int tn = m0 >= 6 ? kMfmaTilesWide : kMfmaTiles;
     const int full = pairs_full_dg(n, ng) ? 1 : 0;
-    for (const int f : {full, 0})
-        for (const int copies : {kLaneWaves, 1}) {
-            const LaneBwdLayout fast{n * (n - 1) / 2, (np + 15) / 16 * 16, f, copies};
-            if (fast.np16 <= 16 * tn && lane_bwd_g_doubles(n, ng, fast) * sizeof(double) <= kLaneLdsLimit)
-                return fast;
-        }
-    return {0, 0, full, 1};
+    const int nc = n * (n - 1) / 2, np16 = (np + 15) / 16 * 16;
+    if (np16 <= 16 * tn) {
+        const int shapes[3][3] = {{kLaneWaves, kLaneWaves, 1}, {2, 2, 2}, {kLaneWaves, 1, 1}};  // waves, W, dg copies
+        for (const auto& sh : shapes)
+            for (const int f : {full, 0}) {
+                const LaneBwdLayout fast{nc, np16, f, sh[1], sh[2], sh[0]};
+                if (lane_bwd_g_doubles(n, ng, fast) * sizeof(double) <= kLaneLdsLimit) return fast;
+            }
+    }
+    return {0, 0, full, 1, 1, kLaneWaves};
 }
 
 template <int M0, int M1>
@@ -866,7 +891,7 @@ hipError_t lane_bwd(bool run, int which, hipStream_t st, dim3 grid, const SlotDe
                            cond);
     } else {
         const size_t lds_g = lane_bwd_g_doubles(topo.n, topo.ng, lay) * sizeof(double);
-        hipLaunchKernelGGL((k_pairs_lane_bwd_g<F, Fast, M0, M1>), grid, dim3(64 * kLaneWaves), lds_g, st, slots, topo,
+        hipLaunchKernelGGL((k_pairs_lane_bwd_g<F, Fast, M0, M1>), grid, dim3(64 * lay.waves), lds_g, st, slots, topo,
                            swap, cond, lay);
     }
     return hipSuccess;

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -47,6 +48,7 @@ EXPORTED_SYMBOLS = (
     "vbe_pool_submit", "vbe_pool_wait", "vbe_pool_poll", "vbe_pool_phase_times", "vbe_set_pair_condition_limit", "vbe_set_ri_precision",
     "vbe_optimizer_defaults", "vbe_optimize", "vbe_pool_optimize", "vbe_set_eigensolver",
     "vbe_gaussian_basis_size", "vbe_ao_integrals", "vbe_pivoted_cholesky", "vbe_eval_device_noalloc",
+    "vbe_server_create", "vbe_server_destroy", "vbe_server_submit", "vbe_server_collect", "vbe_server_stats",
 )
 
 TERMINATIONS = {0: "max_iterations", 1: "plbfgs_initial_tolerance", 2: "plbfgs_projected_gradient_tolerance",
@@ -91,6 +93,17 @@ class OptimizerOptions(C.Structure):
     _fields_ = [("max_iterations", C.c_int32), ("history_size", C.c_int32), ("gradient_tolerance", C.c_double),
                 ("energy_tolerance", C.c_double), ("initial_step_size", C.c_double),
                 ("minimum_step_size", C.c_double), ("armijo_constant", C.c_double)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("batch_size", C.c_int32), ("energy", C.c_double),
+                ("t_arrival", C.c_double), ("t_submit", C.c_double), ("t_dispatch", C.c_double),
+                ("t_complete", C.c_double), ("device_ms", C.c_double)]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [(name, C.c_int64) for name in ("queued", "in_flight_launches", "launches", "launched_requests",
+                                                "completed", "uncollected")]
 
 
 class OptimizerResult(C.Structure):
@@ -159,6 +172,12 @@ def library() -> C.CDLL:
         lib.vbe_optimizer_defaults.restype = None
         lib.vbe_optimize.argtypes = [P, P, P, P, C.POINTER(OptimizerOptions), C.POINTER(OptimizerResult), P, P, I32]
         lib.vbe_pool_optimize.argtypes = [P, I32, P, P, P, C.POINTER(OptimizerOptions), P, P]
+        lib.vbe_server_create.argtypes = [P, I32, I32, C.POINTER(P)]
+        lib.vbe_server_destroy.argtypes = [P]
+        lib.vbe_server_destroy.restype = None
+        lib.vbe_server_submit.argtypes = [P, P, I32, P, I32, P, D, C.POINTER(I64)]
+        lib.vbe_server_collect.argtypes = [P, I32, I32, C.POINTER(RequestInfo), C.POINTER(I32)]
+        lib.vbe_server_stats.argtypes = [P, C.POINTER(ServerCounters)]
         _lib = lib
     return _lib
 
@@ -346,6 +365,7 @@ class Context:
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
         self._lib = library()
         self._h = C.c_void_p()
+        self._children = weakref.WeakSet()  # problems, pools and schedulers: closed before the context
         _check_global(self._lib.vbe_context_create(device, hbm_budget_bytes, C.byref(self._h)))
 
     def error(self) -> str:
@@ -387,6 +407,9 @@ class Context:
 
     def close(self):
         if self._h:
+            # schedulers first (they drive pools), then pools and problems
+            for child in sorted(list(self._children), key=lambda c: not isinstance(c, Scheduler)):
+                child.close()
             self._lib.vbe_context_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -412,6 +435,7 @@ class Problem:
         self._B = np.ascontiguousarray(B, dtype=np.float64)
         self._view = topo.view()
         self._h = C.c_void_p()
+        ctx._children.add(self)
         ctx.check(ctx._lib.vbe_problem_upload(ctx._h, C.byref(dims), _ptr(self._S), _ptr(self._hm), _ptr(self._B),
                                               C.byref(self._view), e_nuc, C.byref(self._h)))
 
@@ -536,6 +560,7 @@ class Pool:
         self._view = topo.view()
         self._h = C.c_void_p()
         self.n_bf = [0] * capacity
+        ctx._children.add(self)
         ctx.check(ctx._lib.vbe_pool_create(ctx._h, C.byref(self._view), topo.n_act, topo.n_alpha, topo.n_beta,
                                            capacity, C.byref(self._h)))
 
@@ -658,6 +683,64 @@ class Pool:
     def close(self):
         if self._h:
             self.ctx._lib.vbe_pool_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scheduler:
+    """Native request scheduler over resident pool slots (vbe_server_*): a dispatcher thread in the
+    engine batches queued E(+grad) requests per pool continuously, on the context's shared
+    streams. `submit` queues a request for a loaded slot; `collect` returns completed requests as
+    dicts with the energy, status, batch size and times (CLOCK_MONOTONIC seconds, comparable with
+    time.perf_counter); gradients land in the array passed to submit."""
+
+    def __init__(self, ctx: Context, max_batch: int = 64, max_inflight_per_pool: int = 4):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        self._grads = {}  # request id -> gradient array kept alive until collected
+        ctx._children.add(self)
+        ctx.check(ctx._lib.vbe_server_create(ctx._h, max_batch, max_inflight_per_pool, C.byref(self._h)))
+        self._buf = (RequestInfo * 256)()
+
+    def submit(self, pool: "Pool", slot: int, Cm: np.ndarray, grad: np.ndarray | None = None,
+               arrival: float = 0.0) -> int:
+        Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+        if Cm.size != pool.n_act * pool.n_bf[slot]:
+            raise ValueError("orbitals do not match the slot's n_act x n_bf")
+        if grad is not None and (grad.dtype != np.float64 or not grad.flags.c_contiguous or grad.size != Cm.size):
+            raise ValueError("gradient buffer must be a C-contiguous float64 array of the orbitals' size")
+        rid = C.c_int64()
+        self.ctx.check(self.ctx._lib.vbe_server_submit(self._h, pool._h, slot, _ptr(Cm), int(grad is not None),
+                                                       _ptr(grad), float(arrival), C.byref(rid)))
+        if grad is not None:
+            self._grads[rid.value] = grad
+        return rid.value
+
+    def collect(self, wait_us: int = 0) -> list:
+        n = C.c_int32()
+        self.ctx.check(self.ctx._lib.vbe_server_collect(self._h, len(self._buf), int(wait_us), self._buf,
+                                                        C.byref(n)))
+        out = []
+        for i in range(n.value):
+            r = self._buf[i]
+            out.append(dict(id=r.id, status=r.status, batch_size=r.batch_size, energy=r.energy,
+                            t_arrival=r.t_arrival, t_submit=r.t_submit, t_dispatch=r.t_dispatch,
+                            t_complete=r.t_complete, device_ms=r.device_ms, grad=self._grads.pop(r.id, None)))
+        return out
+
+    def counters(self) -> dict:
+        c = ServerCounters()
+        self.ctx.check(self.ctx._lib.vbe_server_stats(self._h, C.byref(c)))
+        return {name: getattr(c, name) for name, _ in ServerCounters._fields_}
+
+    def close(self):
+        if self._h:
+            self.ctx._lib.vbe_server_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

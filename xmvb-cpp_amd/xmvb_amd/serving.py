@@ -2,17 +2,17 @@
 
 No reference counterpart: the reference evaluates one problem per process (`cli/run.cpp`), so this
 module is the engine's own request loop. Requests name a registered problem. Problems that share
-a structure topology (same active space and structure set, any n_bf / n_aux) live in one pool;
-each pool takes up to two submissions at once (its two streams), and pools of different
-topologies run concurrently on the device.
+a structure topology (same active space and structure set, any n_bf / n_aux) live in one pool.
 
-Admission: a problem becomes resident (its S, h, B and workspace uploaded into a pool slot) when a
-request for it is first dispatched, provided the pool has a free slot and the context's HBM budget
-has room (`Context(hbm_budget_bytes=...)`; the engine refuses an allocation over it). Otherwise the
-least recently used resident problem with no queued or running request is unloaded to make room;
-if there is none, the request waits until completions free slots or memory. Requests that arrive
-while others run join the next launch of their pool (continuous batching), so a result does not
-depend on which requests it was batched with beyond rounding.
+E+grad requests go to the engine's native scheduler (`xmvb_amd.Scheduler`, vbe_server_*): its
+dispatcher thread batches every queued request of a pool whose slot is not in flight into one
+launch as soon as one of the pool's launch lanes is free, on the context's shared streams (at most
+four), and stamps each request with its queue, device and host times. This module only keeps the
+residency: a problem becomes resident (its S, h, B and workspace uploaded into a pool slot) when a
+request for it arrives, provided the pool has a free slot and the context's HBM budget has room
+(`Context(hbm_budget_bytes=...)`; the engine refuses an allocation over it). Otherwise the least
+recently used resident problem with no queued or running request is unloaded to make room; if there
+is none, the request waits here until completions free slots or memory.
 """
 from __future__ import annotations
 
@@ -20,15 +20,20 @@ import collections
 import threading
 import time
 
-from . import Pool, VbeError
+import numpy as np
+
+from . import Pool, Scheduler, VbeError
 
 VBE_ERR_MEMORY = 6
 
 
 class Request:
-    """One request; `energy`, `grad` (E+grad) or `result` (convergence) are set on completion."""
+    """One request; `energy`, `grad` (E+grad) or `result` (convergence) are set on completion, with
+    `status` (0 on success) and, for E+grad requests, `queue_ms` (arrival to launch), `device_ms`
+    (its launch on the device) and `batch` (requests in that launch)."""
 
-    __slots__ = ("id", "key", "C", "gradient", "arrival", "done", "energy", "grad", "result", "options")
+    __slots__ = ("id", "key", "C", "gradient", "arrival", "done", "energy", "grad", "result", "options", "status",
+                 "queue_ms", "device_ms", "batch")
 
     def __init__(self, rid, key, C, gradient, arrival, options=None):
         self.id, self.key, self.C, self.gradient, self.arrival = rid, key, C, gradient, arrival
@@ -37,6 +42,9 @@ class Request:
         self.grad = None
         self.result = None
         self.options = options or {}
+        self.status = None
+        self.queue_ms = self.device_ms = None
+        self.batch = None
 
     @property
     def latency(self) -> float:
@@ -58,8 +66,6 @@ class _PoolState:
     def __init__(self, ctx, topo, capacity):
         self.pool = Pool(ctx, topo, capacity)
         self.free_slots = list(range(capacity - 1, -1, -1))
-        self.inflight = {}    # ticket -> list of requests
-        self.running = set()  # slots held by in-flight submissions
 
 
 class _Residency:
@@ -138,91 +144,73 @@ class _Residency:
 
 
 class Server:
-    """E+grad requests, continuously batched per pool (at most two submissions per pool in flight).
+    """E+grad requests over registered problems, batched by the native scheduler.
 
-    Single-threaded: call `submit` and `pump` from one thread. `pump` collects finished launches,
-    admits queued requests' problems and starts new launches; it returns the requests completed
-    since the previous call."""
+    Call `submit` and `pump` from one thread. `submit` admits the problem if needed and hands the
+    request to the scheduler at once (or holds it until admission is possible); `pump` returns the
+    requests completed since the previous call (waiting up to wait_us for one) and retries held
+    admissions."""
 
-    def __init__(self, ctx, capacity: int = 256, max_batch: int = 64):
+    def __init__(self, ctx, capacity: int = 256, max_batch: int = 64, max_inflight_per_pool: int = 4):
         self.res = _Residency(ctx, capacity)
-        self.max_batch = max_batch
-        self.queue = collections.deque()
-        self.launches = 0
-        self.batch_sizes = []
+        self.sched = Scheduler(ctx, max_batch=max_batch, max_inflight_per_pool=max_inflight_per_pool)
+        self.held = collections.deque()  # waiting for admission
+        self.inflight = {}               # scheduler id -> request
         self._next_id = 0
 
     def register(self, key, topology, loader, e_nuc=0.0):
         self.res.register(key, topology, loader, e_nuc)
+
+    def _send(self, r) -> bool:
+        p = self.res.problems[r.key]
+        if p.slot is None and not self.res.admit(p):
+            return False
+        ps = p.home
+        grad = np.empty((ps.pool.n_act, ps.pool.n_bf[p.slot])) if r.gradient else None
+        sid = self.sched.submit(ps.pool, p.slot, r.C, grad, arrival=r.arrival)
+        self.inflight[sid] = r
+        return True
 
     def submit(self, key, C, gradient: bool = True, arrival: float | None = None) -> Request:
         p = self.res.problems[key]
         r = Request(self._next_id, key, C, gradient, time.perf_counter() if arrival is None else arrival)
         self._next_id += 1
         p.pending += 1
-        self.queue.append(r)
+        if self.held or not self._send(r):
+            self.held.append(r)
         return r
 
     def busy(self) -> bool:
-        return bool(self.queue) or any(ps.inflight for ps in self.res.pools.values())
+        return bool(self.held) or bool(self.inflight)
 
-    def pump(self) -> list:
+    def pump(self, wait_us: int = 0) -> list:
         done = []
-        for ps in self.res.pools.values():
-            for ticket in list(ps.inflight):
-                if not ps.pool.poll(ticket):
-                    continue
-                e, grads = ps.pool.wait(ticket)
-                now = time.perf_counter()
-                for i, r in enumerate(ps.inflight.pop(ticket)):
-                    p = self.res.problems[r.key]
-                    r.energy = float(e[i])
-                    r.grad = grads[i] if grads is not None else None
-                    r.done = now
-                    p.pending -= 1
-                    p.last_used = now
-                    ps.running.discard(p.slot)
-                    done.append(r)
-        if self.queue:
-            self._dispatch()
-        return done
-
-    def _dispatch(self):
-        batches = {}   # id(pool state) -> (pool state, requests, gradient flag)
-        rest = collections.deque()
-        for r in self.queue:
+        for info in self.sched.collect(wait_us if self.inflight else 0):
+            r = self.inflight.pop(info["id"])
             p = self.res.problems[r.key]
-            ps = p.home
-            b = batches.get(id(ps))
-            if len(ps.inflight) >= 2 or (b and len(b[1]) >= self.max_batch):
-                rest.append(r)
-                continue
-            if b and b[2] != r.gradient:  # one gradient flag per launch; the other kind waits
-                rest.append(r)
-                continue
-            if p.slot is None and not self.res.admit(p):
-                rest.append(r)
-                continue
-            if p.slot in ps.running or (b and any(q.key == r.key for q in b[1])):
-                rest.append(r)  # the problem's slot is busy in this or a running launch
-                continue
-            if b is None:
-                b = batches[id(ps)] = (ps, [], r.gradient)
-            b[1].append(r)
-        for ps, reqs, gradient in batches.values():
-            slots = [self.res.problems[r.key].slot for r in reqs]
-            ticket = ps.pool.submit(slots, [r.C for r in reqs], gradient=gradient)
-            ps.inflight[ticket] = reqs
-            ps.running.update(slots)
-            self.launches += 1
-            self.batch_sizes.append(len(reqs))
-        self.queue = rest
+            r.status = info["status"]
+            r.energy = info["energy"]
+            r.grad = info["grad"]
+            r.done = info["t_complete"]
+            r.queue_ms = (info["t_dispatch"] - r.arrival) * 1e3
+            r.device_ms = info["device_ms"]
+            r.batch = info["batch_size"]
+            p.pending -= 1
+            p.last_used = r.done
+            done.append(r)
+        while self.held and self._send(self.held[0]):
+            self.held.popleft()
+        return done
 
     def drain(self) -> list:
         out = []
         while self.busy():
-            out += self.pump()
+            out += self.pump(wait_us=2000)
         return out
+
+    @property
+    def launches(self) -> int:
+        return self.sched.counters()["launches"]
 
     @property
     def loads(self) -> int:
@@ -233,6 +221,7 @@ class Server:
         return self.res.evictions
 
     def close(self):
+        self.sched.close()
         self.res.close()
 
 
