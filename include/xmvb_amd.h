@@ -23,8 +23,10 @@
  * src/cli/main.cpp:17-20). The text of the last error of a context is available through
  * vbe_last_error(); errors raised before a context exists go to vbe_global_error().
  *
- * Threading: a problem or pool owns its HIP streams; calls on one problem or pool must not
- * overlap. Different problems, pools and contexts are independent.
+ * Threading: the problems and pools of a context launch on the context's four shared HIP streams
+ * (completion is tracked per launch with events). Calls on one pool are serialised by the pool's
+ * lock, so the request scheduler's dispatcher thread (vbe_server_*) and the caller's loads and
+ * unloads may interleave; calls on one problem must not overlap. Contexts are independent.
  */
 #ifndef XMVB_AMD_H
 #define XMVB_AMD_H
@@ -133,7 +135,7 @@ int vbe_parse_ctrl_method(const char* ctrl_line, int32_t* method);
 /* Reject option combinations HIPVBSCF does not implement, in the manner of run_cuvbscf
  * (src/cli/cuvbscf.cpp:38-50): eigensolver DENSE or DAVIDSON, two-electron integrals given as
  * RI-shaped factors (ri=1: INT=RI factors, or exact integrals factorised by vbe_pivoted_cholesky),
- * exactly one state, no inactive (core) electrons. Returns VBE_OK or VBE_ERR_UNSUPPORTED and
+ * 1 to 8 averaged states (more than one with the dense eigensolver), no inactive (core) electrons. Returns VBE_OK or VBE_ERR_UNSUPPORTED and
  * writes the reason to vbe_global_error(). */
 int vbe_check_hipvbscf_options(int32_t eigensolver, int32_t ri, int32_t n_states, int32_t n_inactive_electrons);
 
@@ -223,6 +225,15 @@ int vbe_set_eigensolver(vbe_context* ctx, int32_t eigensolver, double energy_tol
                         double stationarity_tolerance, int32_t reuse_roots);
 const char* vbe_global_error(void);
 
+/* Selected structure states of problems and pools created afterwards: n_states (1..8) distinct
+ * root indices (ascending eigenvalue; NULL: 0..n_states-1) and weights (NULL: equal), normalised
+ * as mcscf::normalize_weights (src/mcscf/states/weights.cpp:7-33: non-negative, positive sum).
+ * The evaluation's energy is then E11 + sum_s w_s lambda_s + E_nuc and its gradient that of this
+ * average (workflow/evaluator.cpp:22-32,65-111; the deck's NSTATE / WSTATE(1), equal weights,
+ * src/input/deck/model.cpp:167-209). Default: the lowest root alone. More than one state, or a
+ * state other than the lowest, takes the dense eigensolver (the Davidson path rejects it). */
+int vbe_set_states(vbe_context* ctx, int32_t n_states, const int32_t* state_indices, const double* weights);
+
 /* Upload the molecule-static data of one problem: AO overlap S and core Hamiltonian h
  * (n_bf x n_bf), RI factors B, the structure topology and E_nuc. Host buffers are copied; the
  * problem stays resident in HBM until vbe_problem_release. Replaces the one-time upload of the
@@ -277,6 +288,11 @@ int vbe_last_intermediates(vbe_context* ctx, vbe_problem* prob, double* H, doubl
 
 /* Device time per phase of the last evaluation. */
 int vbe_last_phase_times(const vbe_problem* prob, vbe_phase_times* times);
+/* Total energies E11 + lambda_s + E_nuc of the selected states at the problem's last evaluation
+ * (VbScfResult::selected_state_total_energies, workflow/evaluator.cpp:98-106); E_states[n_states]. */
+int vbe_last_state_energies(vbe_context* ctx, vbe_problem* prob, double* E_states);
+/* The same for a pool slot after its last evaluation (pools with more than one selected state). */
+int vbe_pool_state_energies(vbe_pool* pool, int32_t slot, double* E_states);
 
 /* Count of determinant pairs whose overlap block was rank deficient in the last evaluation. */
 int vbe_last_singular_pairs(const vbe_problem* prob, int64_t* count);

@@ -34,6 +34,8 @@ def library() -> C.CDLL:
         lib.oracle_gep.argtypes = [C.c_int, P, P, P, P]
         lib.oracle_eval.argtypes = ([C.c_int] * 3 + [P] * 5 + [C.c_int, C.c_int, P, C.c_int, P, C.c_int] + [P] * 5 +
                                     [C.c_double, C.c_int, P] + [P] * 9)
+        lib.oracle_eval_states.argtypes = ([C.c_int] * 3 + [P] * 5 + [C.c_int, C.c_int, P, C.c_int, P, C.c_int] +
+                                           [P] * 5 + [C.c_double, C.c_int, C.c_int, P, P] + [P] * 11)
         _lib = lib
     return _lib
 
@@ -106,8 +108,11 @@ def gep(H, S):
     return w, x
 
 
-def evaluate(n_bf, n_aux, topo, S, h, B, Cm, support=None, e_nuc=0.0, gradient=True, want_det=False):
-    """One FP64 E(+grad) evaluation. topo: object/dict with the expansion arrays and n_act."""
+def evaluate(n_bf, n_aux, topo, S, h, B, Cm, support=None, e_nuc=0.0, gradient=True, want_det=False,
+             states=None, weights=None):
+    """One FP64 E(+grad) evaluation. topo: object/dict with the expansion arrays and n_act.
+    states/weights: selected structure roots and their (unnormalised) weights for a state-averaged
+    energy and gradient; default the lowest root alone. out["state_energies"]: E_s + E_nuc."""
     get = (lambda k: topo[k]) if isinstance(topo, dict) else (lambda k: getattr(topo, k))
     n_act = int(get("n_act"))
     am, bm = np.ascontiguousarray(get("alpha_masks"), np.uint64), np.ascontiguousarray(get("beta_masks"), np.uint64)
@@ -127,10 +132,21 @@ def evaluate(n_bf, n_aux, topo, S, h, B, Cm, support=None, e_nuc=0.0, gradient=T
                h_act=np.zeros((n_act, n_act), order="F"), g_packed=np.zeros(np_ * (np_ + 1) // 2),
                H_det=np.zeros((nd, nd), order="F") if want_det else None,
                S_det=np.zeros((nd, nd), order="F") if want_det else None)
-    _err(library().oracle_eval(n_bf, n_act, n_aux, _p(Sf), _p(hf), _p(Bc), _p(Cc), _p(sup), ns, len(am), _p(am),
-                               len(bm), _p(bm), nd, _p(da), _p(db), _p(off), _p(ts), _p(tc), e_nuc, int(gradient),
-                               _p(e), _p(out["grad"]), _p(out["H"]), _p(out["S"]), _p(out["coef"]),
-                               _p(out["S_act"]), _p(out["h_act"]), _p(out["g_packed"]), _p(out["H_det"]),
-                               _p(out["S_det"])))
+    if states is None:
+        _err(library().oracle_eval(n_bf, n_act, n_aux, _p(Sf), _p(hf), _p(Bc), _p(Cc), _p(sup), ns, len(am), _p(am),
+                                   len(bm), _p(bm), nd, _p(da), _p(db), _p(off), _p(ts), _p(tc), e_nuc, int(gradient),
+                                   _p(e), _p(out["grad"]), _p(out["H"]), _p(out["S"]), _p(out["coef"]),
+                                   _p(out["S_act"]), _p(out["h_act"]), _p(out["g_packed"]), _p(out["H_det"]),
+                                   _p(out["S_det"])))
+    else:
+        si = np.ascontiguousarray(states, np.int32)
+        sw = np.ascontiguousarray(np.ones(len(si)) if weights is None else weights, np.float64)
+        out["state_energies"] = np.zeros(len(si))
+        _err(library().oracle_eval_states(n_bf, n_act, n_aux, _p(Sf), _p(hf), _p(Bc), _p(Cc), _p(sup), ns, len(am),
+                                          _p(am), len(bm), _p(bm), nd, _p(da), _p(db), _p(off), _p(ts), _p(tc), e_nuc,
+                                          int(gradient), len(si), _p(si), _p(sw), _p(e), _p(out["grad"]),
+                                          _p(out["H"]), _p(out["S"]), _p(out["coef"]), _p(out["S_act"]),
+                                          _p(out["h_act"]), _p(out["g_packed"]), _p(out["H_det"]), _p(out["S_det"]),
+                                          _p(out["state_energies"])))
     out["energy"] = float(e[0])
     return out

@@ -37,7 +37,7 @@ def _restricted_inverse(S, mask):
 
 def plbfgs(n_bf, n_aux, topo, S, h, B, C0, support=None, max_iterations=2000, history_size=100,
            gradient_tolerance=1e-3, energy_tolerance=1e-7, initial_step_size=1.0, minimum_step_size=1e-7,
-           armijo_constant=1e-4):
+           armijo_constant=1e-4, states=None, weights=None):
     S = np.asarray(S, dtype=np.float64)
     n_act = C0.shape[0]
     masks = np.ones((n_act, n_bf), bool) if support is None else np.asarray(support).astype(bool)
@@ -49,13 +49,15 @@ def plbfgs(n_bf, n_aux, topo, S, h, B, C0, support=None, max_iterations=2000, hi
     def energy(Cm):
         counts["energy_evaluations"] += 1
         try:
-            return evaluate(n_bf, n_aux, topo, S, h, B, Cm, support=sup, gradient=False)["energy"]
+            return evaluate(n_bf, n_aux, topo, S, h, B, Cm, support=sup, gradient=False, states=states,
+                            weights=weights)["energy"]
         except RuntimeError:
             return float("nan")
 
     def gradient(Cm):
         counts["gradient_evaluations"] += 1
-        return evaluate(n_bf, n_aux, topo, S, h, B, Cm, support=sup, gradient=True)["grad"]
+        return evaluate(n_bf, n_aux, topo, S, h, B, Cm, support=sup, gradient=True, states=states,
+                        weights=weights)["grad"]
 
     def chart(Cm, Gm):
         Sc = np.where(masks, Cm @ S, 0.0)

@@ -1243,12 +1243,17 @@ struct Problem {
     const int32_t* ts;
     const double* tc;
     double enuc;
+    // selected states and their weights (normalised here); n_states == 0: the lowest root alone
+    int n_states = 0;
+    const int32_t* sidx = nullptr;
+    const double* sw = nullptr;
 };
 
 struct Out {
     double energy = 0;
     Vec grad, H, Sm, coef, sact, hact, g, cn, hdet, sdet;
     Vec eigenvalues;
+    Vec state_energies;  // E11 + lambda_s + E_nuc per selected state
 };
 
 int evaluate(const Problem& p, Out& o, bool want_grad, std::string& err) {
@@ -1454,11 +1459,41 @@ int evaluate(const Problem& p, Out& o, bool want_grad, std::string& err) {
         err = "structure metric is not positive definite";
         return 3;
     }
-    // E = E11 + sum_s w_s lambda_s + E_nuc with one state of weight 1, no inactive core (E11 = 0)
-    // (derivatives/gradient/active_space/evaluator.cpp:1243-1248)
-    o.energy = w[0] + p.enuc;
+    // E = E11 + sum_s w_s lambda_s + E_nuc over the selected states with normalised weights, no
+    // inactive core (E11 = 0) (workflow/evaluator.cpp:22-32,65-111; mcscf/states/weights.cpp:18-33;
+    // derivatives/gradient/active_space/evaluator.cpp:1243-1248). Without a selection: the lowest
+    // root with weight 1.
+    const bool single = p.n_states == 0;
+    std::vector<int> sel = single ? std::vector<int>{0} : std::vector<int>(p.sidx, p.sidx + p.n_states);
+    Vec omega(sel.size(), 1.0);
+    if (!single) {
+        double wsum = 0.0;
+        for (int k = 0; k < p.n_states; ++k) {
+            if (!(p.sw[k] >= 0.0)) {
+                err = "state_average_weights must be non-negative";
+                return 1;
+            }
+            if (sel[k] < 0 || sel[k] >= ns) {
+                err = "selected state index is out of range";
+                return 1;
+            }
+            wsum += p.sw[k];
+        }
+        if (!(wsum > 0.0)) {
+            err = "state_average_weights must sum to a positive value";
+            return 1;
+        }
+        for (int k = 0; k < p.n_states; ++k) omega[k] = p.sw[k] / wsum;
+    }
+    double eavg = 0.0;
+    o.state_energies.assign(sel.size(), 0.0);
+    for (size_t k = 0; k < sel.size(); ++k) {
+        eavg += omega[k] * w[sel[k]];
+        o.state_energies[k] = w[sel[k]] + p.enuc;
+    }
+    o.energy = single ? w[0] + p.enuc : eavg + p.enuc;
     o.eigenvalues = w;
-    o.coef.assign(x.begin(), x.begin() + ns);
+    o.coef.assign(x.begin() + (size_t)ns * sel[0], x.begin() + (size_t)ns * (sel[0] + 1));
     o.H = Hs;
     o.Sm = Ss;
     o.sact = sact;
@@ -1470,10 +1505,20 @@ int evaluate(const Problem& p, Out& o, bool want_grad, std::string& err) {
     if (!want_grad) return 0;
 
     // ---- gradient: Hellmann-Feynman over determinant pairs, weights a_d a_e ----
+    // (selected states: sum_s w_s a^s_d a^s_e for dH and sum_s w_s lambda_s a^s_d a^s_e for dS, the
+    // state-averaged form of the selected-state determinant coefficient matrices,
+    // structures/assembly/selected_coefficients.hpp:47-66)
     const double E = w[0];
     Vec a(nd, 0.0);
     for (int d = 0; d < nd; ++d)
         for (int64_t t = p.off[d]; t < p.off[d + 1]; ++t) a[d] += p.tc[t] * o.coef[p.ts[t]];
+    const size_t nsel = sel.size();
+    Vec as(single ? 0 : nsel * nd, 0.0);
+    if (!single)
+        for (size_t k = 0; k < nsel; ++k)
+            for (int d = 0; d < nd; ++d)
+                for (int64_t t = p.off[d]; t < p.off[d + 1]; ++t)
+                    as[k * nd + d] += p.tc[t] * x[p.ts[t] + (size_t)ns * sel[k]];
     int nthreads = 1;
 #ifdef _OPENMP
     nthreads = omp_get_max_threads();
@@ -1488,11 +1533,25 @@ int evaluate(const Problem& p, Out& o, bool want_grad, std::string& err) {
 #endif
 #pragma omp for schedule(dynamic, 1)
         for (int d = 0; d < nd; ++d) {
-            if (a[d] == 0.0) continue;
+            if (single) {
+                if (a[d] == 0.0) continue;
+                for (int e = 0; e < nd; ++e) {
+                    const double wgt = a[d] * a[e];
+                    if (wgt == 0.0) continue;
+                    pair_backward(p.am[p.da[d]], p.am[p.da[e]], p.bm[p.db[d]], p.bm[p.db[e]], I, wgt, -E * wgt,
+                                  part[tid]);
+                }
+                continue;
+            }
             for (int e = 0; e < nd; ++e) {
-                const double wgt = a[d] * a[e];
-                if (wgt == 0.0) continue;
-                pair_backward(p.am[p.da[d]], p.am[p.da[e]], p.bm[p.db[d]], p.bm[p.db[e]], I, wgt, -E * wgt, part[tid]);
+                double wh = 0.0, wsv = 0.0;
+                for (size_t k = 0; k < nsel; ++k) {
+                    const double c2 = omega[k] * as[k * nd + d] * as[k * nd + e];
+                    wh += c2;
+                    wsv -= w[sel[k]] * c2;
+                }
+                if (wh == 0.0 && wsv == 0.0) continue;
+                pair_backward(p.am[p.da[d]], p.am[p.da[e]], p.bm[p.db[d]], p.bm[p.db[e]], I, wh, wsv, part[tid]);
             }
         }
     }
@@ -1713,15 +1772,17 @@ int oracle_gep(int n, const double* h, const double* s, double* w, double* x) {
 //   C, grad, support: orbital-major [k*n_bf + mu]; S, h: n_bf x n_bf; B: naux x n_bf(n_bf+1)/2 column-major.
 //   out_H, out_S: n_str^2; out_coef: n_str; out_sact/out_hact: n_act^2; out_g: packed kernel;
 //   out_hdet/out_sdet: n_det^2.
-int oracle_eval(int n_bf, int n_act, int naux, const double* S, const double* h, const double* B, const double* C,
-                const uint8_t* support, int n_str, int n_astr, const uint64_t* amask, int n_bstr, const uint64_t* bmask,
-                int n_det, const int32_t* det_a, const int32_t* det_b, const int64_t* off, const int32_t* term_s,
-                const double* term_c, double e_nuc, int want_grad, double* out_energy, double* out_grad, double* out_H,
-                double* out_S, double* out_coef, double* out_sact, double* out_hact, double* out_g, double* out_hdet,
-                double* out_sdet) {
+int oracle_eval_states(int n_bf, int n_act, int naux, const double* S, const double* h, const double* B,
+                       const double* C, const uint8_t* support, int n_str, int n_astr, const uint64_t* amask,
+                       int n_bstr, const uint64_t* bmask, int n_det, const int32_t* det_a, const int32_t* det_b,
+                       const int64_t* off, const int32_t* term_s, const double* term_c, double e_nuc, int want_grad,
+                       int n_states, const int32_t* state_idx, const double* state_w, double* out_energy,
+                       double* out_grad, double* out_H, double* out_S, double* out_coef, double* out_sact,
+                       double* out_hact, double* out_g, double* out_hdet, double* out_sdet,
+                       double* out_state_energies) {
     try {
         orc::Problem p{n_bf, n_act, naux, S, h, B, C, support, n_str, n_astr, n_bstr, n_det, amask, bmask,
-                       det_a, det_b, off, term_s, term_c, e_nuc};
+                       det_a, det_b, off, term_s, term_c, e_nuc, n_states, state_idx, state_w};
         orc::Out o;
         std::string err;
         const int rc = orc::evaluate(p, o, want_grad != 0, err);
@@ -1742,11 +1803,24 @@ int oracle_eval(int n_bf, int n_act, int naux, const double* S, const double* h,
         put(o.g, out_g);
         put(o.hdet, out_hdet);
         put(o.sdet, out_sdet);
+        put(o.state_energies, out_state_energies);
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();
         return 1;
     }
+}
+
+int oracle_eval(int n_bf, int n_act, int naux, const double* S, const double* h, const double* B, const double* C,
+                const uint8_t* support, int n_str, int n_astr, const uint64_t* amask, int n_bstr, const uint64_t* bmask,
+                int n_det, const int32_t* det_a, const int32_t* det_b, const int64_t* off, const int32_t* term_s,
+                const double* term_c, double e_nuc, int want_grad, double* out_energy, double* out_grad, double* out_H,
+                double* out_S, double* out_coef, double* out_sact, double* out_hact, double* out_g, double* out_hdet,
+                double* out_sdet) {
+    return oracle_eval_states(n_bf, n_act, naux, S, h, B, C, support, n_str, n_astr, amask, n_bstr, bmask, n_det,
+                              det_a, det_b, off, term_s, term_c, e_nuc, want_grad, 0, nullptr, nullptr, out_energy,
+                              out_grad, out_H, out_S, out_coef, out_sact, out_hact, out_g, out_hdet, out_sdet,
+                              nullptr);
 }
 
 }  // extern "C"

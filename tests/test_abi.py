@@ -49,7 +49,9 @@ def test_ctrl_method_flag_rejections(line, status):
 def test_hipvbscf_option_rejections():
     vb.check_hipvbscf_options(eigensolver=0, ri=True, n_states=1, n_inactive_electrons=0)
     vb.check_hipvbscf_options(eigensolver=1, ri=True, n_states=1, n_inactive_electrons=0)
-    for kwargs in (dict(eigensolver=2), dict(ri=False), dict(n_states=2), dict(n_inactive_electrons=2)):
+    vb.check_hipvbscf_options(eigensolver=0, ri=True, n_states=3, n_inactive_electrons=0)  # NSTATE=3, dense
+    for kwargs in (dict(eigensolver=2), dict(ri=False), dict(n_states=2, eigensolver=1), dict(n_states=0),
+                   dict(n_states=9), dict(n_inactive_electrons=2)):
         with pytest.raises(vb.VbeError) as e:
             vb.check_hipvbscf_options(**kwargs)
         assert e.value.status == 5

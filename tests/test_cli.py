@@ -72,7 +72,10 @@ def test_oeo_support_is_every_function(tmp_path):
     ({"INT=LIBCINT": "INT=RI"}, "auxiliary basis"),
     ({"BASIS=6-31G**": "BASIS=CC-PVDZ"}, "not available"),
     ({"EIGENSOLVER=DENSE": "EIGENSOLVER=LANCZOS"}, "unsupported EIGENSOLVER value"),
-    ({"NAE=2": "NAE=2 NSTATE=2"}, "single-state"),
+    ({"NAE=2": "NAE=2 NSTATE=9"}, "1 to 8 averaged states"),
+    ({"EIGENSOLVER=DENSE": "EIGENSOLVER=DAVIDSON NSTATE=2"}, "needs EIGENSOLVER=DENSE"),
+    ({"NAE=2": "NAE=2 WSTATE(1)=0.5,0.25"}, "only equal positive WSTATE weights"),
+    ({"NAE=2": "NAE=2 WSTATE(2)=0.5,0.5"}, "supports only WSTATE(1)"),
     ({"FRGTYP=ATOM": "FRGTYP=SAO"}, "FRGTYP=ATOM only"),
     ({"HIPVBSCF": "HIPVBSCF CASSCF"}, "only one electronic method"),
     ({"HIPVBSCF": "HIPVBSCF TBVBSCF"}, "no longer supported"),

@@ -74,3 +74,26 @@ def test_restart_from_orbital_file(tmp_path, oracle_run):
     assert "guess: READ" in out
     assert int(dg["scf_iterations"]) <= 1
     assert abs(float(dg["total_energy_hartree"]) - e1) <= 1e-9
+
+
+def test_h2_state_average_matches_oracle(tmp_path):
+    """NSTATE=2 (equal weights) on the H2 deck with the dense eigensolver: the state-averaged
+    P-LBFGS run equals the oracle's on the same integrals and start, and both state energies are
+    reported."""
+    d = vb.h2_deck()
+    ref = plbfgs(d["n_bf"], d["n_aux"], d["topo"], d["S"], d["h"], d["B"], d["C0"], support=d["support"],
+                 states=[0, 1], weights=[1.0, 1.0])
+    deck = deck_variant(H2_DECK, str(tmp_path / "h2sa.xmi"), {"NAE=2": "NAE=2 NSTATE=2"})
+    print("case cli h2 NSTATE=2", flush=True)
+    rc, out, err = run_cli(deck, timeout=300)
+    assert rc == 0, out[-2000:] + err
+    dg = diagnostics(out)
+    e_ref = ref["energy"] + d["e_nuc"]
+    print(f"cli E={dg['total_energy_hartree']} ({dg['scf_iterations']} it); oracle E={e_ref:.12f} "
+          f"({ref['iterations']} it)", flush=True)
+    assert dg["states"] == "2" and dg["converged"] == "1"
+    assert int(dg["scf_iterations"]) == ref["iterations"]
+    assert abs(float(dg["total_energy_hartree"]) - e_ref) <= 1e-10
+    es = [float(m) for m in re.findall(r"State \d+ total energy\s+(\S+) Eh", out)]
+    assert len(es) == 2 and es[0] < es[1]
+    assert abs(0.5 * (es[0] + es[1]) - e_ref) <= 1e-9

@@ -330,6 +330,7 @@ struct Problem {
     int eigensolver = VBE_EIGENSOLVER_DAVIDSON;
     int max_iterations = 2000;
     int multiplicity = 1;
+    int n_states = 1;
 };
 
 std::vector<std::vector<int>> build_supports(const Deck& d, const Problem& p, const std::string& orbtyp,
@@ -580,7 +581,34 @@ Problem build_problem(const Deck& d, const Options& o) {
     const std::string itmax = get(d, "ITMAX", "");
     if (!itmax.empty() && parse_int(itmax) > 0) p.max_iterations = parse_int(itmax);
     if (o.max_iterations >= 0) p.max_iterations = o.max_iterations;
-    const int n_states = parse_int(get(d, "NSTATE", "1"));
+    // NSTATE and WSTATE(1)=w,w,... (equal weights only), as apply_ctrl_assignment
+    // (src/input/deck/model.cpp:167-209): the state count is the larger of the two
+    int n_states = parse_int(get(d, "NSTATE", "1"));
+    if (n_states <= 0) throw std::invalid_argument("NSTATE must be positive");
+    for (const auto& [key, value] : d.assign) {
+        if (key.rfind("WSTATE(", 0) != 0 || key.back() != ')') continue;
+        if (parse_int(key.substr(7, key.size() - 8)) != 1)
+            throw std::invalid_argument("equal-weight state averaging supports only WSTATE(1)");
+        std::vector<double> w;
+        size_t begin = 0;
+        for (;;) {
+            const size_t comma = value.find(',', begin);
+            const std::string tok = value.substr(begin, comma == std::string::npos ? std::string::npos : comma - begin);
+            if (tok.find_first_not_of(" \t") == std::string::npos) throw std::invalid_argument("WSTATE contains an empty weight");
+            std::string f = tok;  // Fortran exponent letters D/d read as E
+            for (char& ch : f)
+                if (ch == 'D' || ch == 'd') ch = 'E';
+            w.push_back(std::stod(f));
+            if (comma == std::string::npos) break;
+            begin = comma + 1;
+        }
+        if (!(w.front() > 0.0)) throw std::invalid_argument("WSTATE weights must be positive");
+        for (double x : w)
+            if (!(x > 0.0) || std::abs(x - w.front()) > 1e-12 * std::max({1.0, std::abs(w.front()), std::abs(x)}))
+                throw std::invalid_argument("only equal positive WSTATE weights are supported");
+        n_states = std::max(n_states, (int)w.size());
+    }
+    p.n_states = n_states;
     if (upper(get(d, "INT", "")) == "RI")
         throw std::invalid_argument("INT=RI needs an auxiliary basis, which this build does not carry; use exact integrals");
     // geometry
@@ -682,6 +710,7 @@ int run(const Options& o) {
     check(vbe_context_create(0, 0, &ctx), "device context");
     check_ctx(ctx,
               vbe_set_eigensolver(ctx, p.eigensolver, 1e-7, 1e-3, 1.0, 1), "eigensolver");
+    if (p.n_states > 1) check_ctx(ctx, vbe_set_states(ctx, p.n_states, nullptr, nullptr), "states");
     vbe_dims dims{p.n_bf, p.n_act, p.n_alpha, p.n_beta, p.n_aux};
     vbe_problem* prob = nullptr;
     check_ctx(ctx, vbe_problem_upload(ctx, &dims, p.S.data(), p.h.data(), p.B.data(), &view, p.e_nuc, &prob),
@@ -720,6 +749,15 @@ int run(const Options& o) {
         std::vector<double> Hs((size_t)ns * ns), Ss((size_t)ns * ns), c(ns);
         check_ctx(ctx, vbe_last_intermediates(ctx, prob, Hs.data(), Ss.data(), c.data(), nullptr, nullptr, nullptr),
                   "structure matrices");
+        if (p.n_states > 1) {
+            std::vector<double> es(p.n_states);
+            check_ctx(ctx, vbe_last_state_energies(ctx, prob, es.data()), "state energies");
+            std::cout << "\n State-averaged over " << p.n_states << " states (equal weights)\n";
+            for (int k = 0; k < p.n_states; ++k)
+                std::cout << " State " << k + 1 << " total energy " << std::fixed << std::setprecision(12) << es[k]
+                          << " Eh\n";
+            std::cout << " Coefficients below: state 1\n" << std::defaultfloat << std::setprecision(16);
+        }
         std::cout << "\n Structure   Coefficient        Coulson-Chirgwin weight\n";
         for (int I = 0; I < ns; ++I) {
             double sc = 0.0;
@@ -736,7 +774,7 @@ int run(const Options& o) {
         std::cout << std::defaultfloat << std::setprecision(16);
     }
     std::cout << "\n XMVB-CPP RUN DIAGNOSTICS\n\n"
-              << "method=hipvbscf_plbfgs\nstructures=" << sizes[0] << "\ndeterminants=" << sizes[3]
+              << "method=hipvbscf_plbfgs\nstates=" << p.n_states << "\nstructures=" << sizes[0] << "\ndeterminants=" << sizes[3]
               << "\nbasis_functions=" << p.n_bf << "\nactive_orbitals=" << p.n_act << "\nfactor_rank=" << p.n_aux
               << "\n";
     if (!orbital_path.empty()) std::cout << "orbital_output=" << orbital_path << "\n";

@@ -108,8 +108,13 @@ int vbe_check_hipvbscf_options(int32_t eigensolver, int32_t ri, int32_t n_states
                          "factorised by vbe_pivoted_cholesky)");
         return VBE_ERR_UNSUPPORTED;
     }
-    if (n_states != 1) {
-        set_global_error("HIPVBSCF currently supports single-state calculations");
+    if (n_states < 1 || n_states > 8) {
+        set_global_error("HIPVBSCF supports 1 to 8 averaged states (NSTATE)");
+        return VBE_ERR_UNSUPPORTED;
+    }
+    if (n_states > 1 && eigensolver != VBE_EIGENSOLVER_DENSE) {
+        set_global_error("state-averaged HIPVBSCF (NSTATE > 1) needs EIGENSOLVER=DENSE: the engine's Davidson finds "
+                         "the lowest structure root only");
         return VBE_ERR_UNSUPPORTED;
     }
     if (n_inactive_electrons != 0) {

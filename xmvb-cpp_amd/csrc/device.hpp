@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "kernels.hpp"
+
 namespace xmvb_amd {
 
 // packed index of the unordered pair (a, b): hi(hi+1)/2 + lo
@@ -18,6 +20,28 @@ __device__ __forceinline__ void unpack_pair(long long t, int& d, int& e) {
     while ((x + 1) * (x + 2) / 2 <= t) ++x;
     d = (int)(t - x * (x + 1) / 2);
     e = (int)x;
+}
+
+// Weights of one unordered determinant pair {d, e} (and of its spin-flip partner {fd, fe} when
+// flip) in the backward: the dH weight wH = sum_s w_s a^s_d a^s_e (x mult: 2 off the diagonal)
+// and the dS weight wS = -sum_s w_s lambda_s a^s_d a^s_e, so the pair's adjoint is that of
+// wH H_de + wS S_de. One state: wH = a_d a_e mult, wS = -lambda wH.
+__device__ __forceinline__ void pair_weights(const SlotDesc& sd, const TopoDesc& topo, int d, int e, int fd, int fe,
+                                             bool flip, double mult, double& wH, double& wS) {
+    if (topo.n_states <= 1) {
+        wH = (sd.dcoef[d] * sd.dcoef[e] + (flip ? sd.dcoef[fd] * sd.dcoef[fe] : 0.0)) * mult;
+        wS = -sd.energy[0] * wH;
+        return;
+    }
+    double h = 0.0, s = 0.0;
+    for (int k = 0; k < topo.n_states; ++k) {
+        const double* a = sd.dcoef + (size_t)k * topo.n_det;
+        const double c = topo.state_w[k] * ((a[d] * a[e] + (flip ? a[fd] * a[fe] : 0.0)) * mult);
+        h += c;
+        s -= sd.state_energy[k] * c;
+    }
+    wH = h;
+    wS = s;
 }
 
 }  // namespace xmvb_amd

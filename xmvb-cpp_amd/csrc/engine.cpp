@@ -64,6 +64,10 @@ struct vbe_context {
     int eigensolver = VBE_EIGENSOLVER_DENSE;
     double dav_energy_tol = 1.0e-7, dav_residual_tol = 1.0e-3, dav_stationarity_tol = 1.0;
     bool dav_reuse_roots = true;
+    // selected structure states of problems and pools created afterwards (normalised weights)
+    int n_states = 1;
+    int state_idx[xmvb_amd::kMaxStates] = {0};
+    double state_w[xmvb_amd::kMaxStates] = {1.0};
     std::string error;
     // guards in_use, error and the stream cursor: pools of one context may be driven from several
     // host threads (for example the request scheduler beside a convergence worker)
@@ -418,6 +422,15 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     T.soff = d_soff;
     T.sdet = d_sdet;
     T.scoef = d_scoef;
+    T.n_states = ctx->n_states;
+    for (int k = 0; k < xmvb_amd::kMaxStates; ++k) {
+        T.state_idx[k] = k < ctx->n_states ? ctx->state_idx[k] : 0;
+        T.state_w[k] = k < ctx->n_states ? ctx->state_w[k] : 0.0;
+    }
+    for (int k = 0; k < T.n_states; ++k)
+        if (T.state_idx[k] >= ns)
+            throw StatusError(VBE_ERR_ARGUMENT, "selected state " + std::to_string(T.state_idx[k]) +
+                                                    " is out of range for " + std::to_string(ns) + " structures");
     check(xmvb_amd::configure_kernels(n_act, T.ng, ns, std::max(n_alpha, n_beta)), "hipFuncSetAttribute");
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
     p->slots.resize(capacity);
@@ -465,8 +478,9 @@ size_t carve_workspace(const vbe_pool* p, int n_bf, int n_aux, unsigned char* ba
     take(d.Hs, ns * ns);
     take(d.Ss, ns * ns);
     take(d.gep, xmvb_amd::gep_work_doubles((int)ns));
-    take(d.coef, ns);
-    take(d.dcoef, nd);
+    take(d.coef, ns * (size_t)T.n_states);
+    take(d.dcoef, nd * (size_t)T.n_states);
+    take(d.state_energy, (size_t)xmvb_amd::kMaxStates);
     const size_t len = (size_t)T.ng + 2 * n * n;
     take(d.partials, (size_t)(bwd_blocks_cap(p) + xmvb_amd::kListBlocks + xmvb_amd::kSingularBlocks) * len);
     take(d.dsum, len);
@@ -633,6 +647,9 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in, bool 
         }
     }
     sh.davidson = p->eigensolver == VBE_EIGENSOLVER_DAVIDSON;
+    if (sh.davidson && (p->topo.n_states > 1 || p->topo.state_idx[0] != 0))
+        throw StatusError(VBE_ERR_UNSUPPORTED, "the Davidson eigensolver finds the lowest structure root only; "
+                                               "state-averaged evaluations take the dense eigensolver");
     sh.dav_energy_tol = p->ctx->dav_energy_tol;
     sh.dav_residual_tol = p->ctx->dav_residual_tol;
     sh.dav_stationarity_tol = p->ctx->dav_stationarity_tol;
@@ -1071,6 +1088,33 @@ int vbe_set_eigensolver(vbe_context* ctx, int32_t eigensolver, double energy_tol
     return VBE_OK;
 }
 
+int vbe_set_states(vbe_context* ctx, int32_t n_states, const int32_t* state_indices, const double* weights) {
+    if (!ctx) return fail(nullptr, VBE_ERR_ARGUMENT, "null context");
+    if (n_states < 1 || n_states > xmvb_amd::kMaxStates)
+        return fail(ctx, VBE_ERR_ARGUMENT, "n_states must lie in [1, " + std::to_string(xmvb_amd::kMaxStates) + "]");
+    // mcscf::normalize_weights (src/mcscf/states/weights.cpp:7-33): non-negative, positive sum
+    double sum = 0.0;
+    for (int k = 0; k < n_states; ++k) {
+        const double w = weights ? weights[k] : 1.0;
+        if (!(w >= 0.0) || !std::isfinite(w)) return fail(ctx, VBE_ERR_ARGUMENT, "state_average_weights must be non-negative");
+        sum += w;
+    }
+    if (!(sum > 0.0)) return fail(ctx, VBE_ERR_ARGUMENT, "state_average_weights must sum to a positive value");
+    for (int k = 0; k < n_states; ++k) {
+        const int idx = state_indices ? state_indices[k] : k;
+        if (idx < 0) return fail(ctx, VBE_ERR_ARGUMENT, "selected state index is out of range");
+        for (int j = 0; j < k; ++j)
+            if ((state_indices ? state_indices[j] : j) == idx)
+                return fail(ctx, VBE_ERR_ARGUMENT, "selected states must be distinct");
+    }
+    ctx->n_states = n_states;
+    for (int k = 0; k < xmvb_amd::kMaxStates; ++k) {
+        ctx->state_idx[k] = k < n_states ? (state_indices ? state_indices[k] : k) : 0;
+        ctx->state_w[k] = k < n_states ? (weights ? weights[k] : 1.0) / sum : 0.0;
+    }
+    return VBE_OK;
+}
+
 int vbe_set_pair_condition_limit(vbe_context* ctx, double limit) {
     if (!ctx) return fail(nullptr, VBE_ERR_ARGUMENT, "null context");
     if (!(limit >= 0.0 && limit <= xmvb_amd::kLaneConditionMax))
@@ -1502,6 +1546,47 @@ int vbe_pool_optimize(vbe_pool* pool, int32_t count, const int32_t* slots, doubl
             off += (size_t)pool->n_act * s.n_bf;
         }
         optimize_slots(pool, nullptr, list, batch, *options);
+        return VBE_OK;
+    });
+}
+
+namespace {
+
+// total energies (E_nuc added) of the selected states of a slot's last evaluation
+void state_energies(vbe_pool* p, int slot, double e_total, double* out) {
+    const auto& T = p->topo;
+    if (T.n_states == 1) {
+        out[0] = e_total;
+        return;
+    }
+    const Slot& s = p->slots[slot];
+    check(hipSetDevice(p->ctx->device), "hipSetDevice");
+    check(hipMemcpy(out, s.desc.state_energy, sizeof(double) * T.n_states, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    for (int k = 0; k < T.n_states; ++k) out[k] += s.e_nuc;
+}
+
+}  // namespace
+
+int vbe_last_state_energies(vbe_context* ctx, vbe_problem* prob, double* E_states) {
+    return guarded(ctx, [&]() -> int {
+        if (!ctx || !prob || !E_states) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        vbe_pool* p = prob->pool.get();
+        if (!prob->forward_valid) throw StatusError(VBE_ERR_ARGUMENT, "no evaluation of this problem to report");
+        state_energies(p, 0, p->lanes[0].h_energy[1], E_states);
+        return VBE_OK;
+    });
+}
+
+int vbe_pool_state_energies(vbe_pool* pool, int32_t slot, double* E_states) {
+    return guarded(pool ? pool->ctx : nullptr, [&]() -> int {
+        if (!pool || !E_states) throw StatusError(VBE_ERR_ARGUMENT, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(pool->mu);
+        if (slot < 0 || slot >= pool->capacity || !pool->slots[slot].loaded)
+            throw StatusError(VBE_ERR_ARGUMENT, "slot not loaded");
+        check_slots_free(pool, &slot, 1);
+        if (pool->topo.n_states == 1)
+            throw StatusError(VBE_ERR_ARGUMENT, "one selected state: its energy is the evaluation's energy");
+        state_energies(pool, slot, 0.0, E_states);
         return VBE_OK;
     });
 }

@@ -639,7 +639,6 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
     GroupSlab<GROUP>& gs = slabs[wave * GPW + grp];
     const long long first = ((long long)blockIdx.x * waves_per_block + wave) * GPW;
     const long long stride = (long long)gridDim.x * waves_per_block * GPW;
-    const double E = Backward ? sd.energy[0] : 0.0;
     // every string of a topology has the same electron count per spin
     const int m[2] = {__builtin_popcountll(amask[0]), __builtin_popcountll(bmask[0])};
 
@@ -650,10 +649,10 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
         // unordered pair t = e(e+1)/2 + d, d <= e; d is the left (bra) determinant
         const long long tp = live ? (Listed ? (long long)sd.defer[ti] : ti) : 0;
         if (live) unpack_pair(tp, d, ee);
-        double weight = 0.0;
+        double wH = 0.0, wS = 0.0;
         if (Backward) {
-            weight = live ? sd.dcoef[d] * sd.dcoef[ee] * (d == ee ? 1.0 : 2.0) : 0.0;
-            live = live && weight != 0.0;
+            if (live) pair_weights(sd, topo, d, ee, d, ee, false, d == ee ? 1.0 : 2.0, wH, wS);
+            live = live && (wH != 0.0 || wS != 0.0);
         }
         const uint64_t mL[2] = {amask[det_a[d]], bmask[det_b[d]]};
         const uint64_t mR[2] = {amask[det_a[ee]], bmask[det_b[ee]]};
@@ -758,8 +757,7 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
             wave_sync();
             continue;
         }
-        // ---- backward: F_pair = w (H - E S) ----
-        const double wH = weight, wS = -E * weight;
+        // ---- backward: F_pair = wH H + wS S (one state: w (H - E S)) ----
         const double c = Dtot;
         const double cw = c * wH;
         wave_sync();
@@ -1844,6 +1842,185 @@ __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDes
     }
 }
 
+// Selected states (state-averaged evaluations, workflow/evaluator.cpp:22-32,65-111): after the
+// global-memory dense solve (k_gep_lowest<false, 2 or 3>), whose workspace still holds L^-1 (Lm),
+// the Householder vectors and scalars (V, beta) and the tridiagonal (dd, ee), one workgroup per
+// problem finds eigenvalue state_idx[k] of the tridiagonal for each selected state by multisection
+// on Sturm counts (the smallest x with at least state_idx[k] + 1 eigenvalues below it), its vector
+// by inverse iteration (LAPACK dgttrf/dgttrs scheme; three iterations from a fixed +-1 start, each
+// orthogonalised against the vectors of the states already found, as dstein does within a
+// cluster), back-transforms it, maps it to c = L^-T z (c^T S c = 1), expands determinant
+// coefficients and forms E = sum_k w_k lambda_k. The tridiagonal vectors live in the workspace
+// past the core arrays (free after the front half). State indices are distinct, so
+// n_states <= n_str.
+__global__ void __launch_bounds__(256) k_gep_states(const SlotDesc* __restrict__ slots, TopoDesc topo) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const int n = topo.n_str, S = topo.n_states, n_det = topo.n_det;
+    if (sd.flags[FLAG_METRIC]) return;
+    double* __restrict__ work = sd.gep;
+    const double* Lm = work;            // L^-1, lower triangle
+    const double* V = work + 2 * (size_t)n * n;
+    const double* dd = work + 3 * (size_t)n * n;
+    const double* ee = dd + n;
+    const double* beta = ee + 2 * (size_t)n;
+    double* tdl = const_cast<double*>(beta) + n;
+    double* tdu = tdl + n;
+    double* tdu2 = tdu + n;
+    double* tdg = tdu2 + n;
+    int* ipiv = reinterpret_cast<int*>(tdg + n);
+    double* Z = work + gep_core_dev(n);  // n_states tridiagonal vectors of n
+    __shared__ double red[16];
+    __shared__ double lo_hi[2];
+    __shared__ double lam_sh[kMaxStates];
+    __shared__ int first_true;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int k = 0; k < S; ++k) {
+        const int target = topo.state_idx[k] + 1;
+        if (tid == 0) {  // Gershgorin interval
+            double lo = 1e300, hi = -1e300;
+            for (int i = 0; i < n; ++i) {
+                const double rad = (i > 0 ? fabs(ee[i - 1]) : 0.0) + (i + 1 < n ? fabs(ee[i]) : 0.0);
+                lo = fmin(lo, dd[i] - rad);
+                hi = fmax(hi, dd[i] + rad);
+            }
+            const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
+            lo_hi[0] = lo - pad;
+            lo_hi[1] = hi + pad;
+        }
+        __syncthreads();
+        for (int it = 0; it < 200; ++it) {
+            const double lo = lo_hi[0], hi = lo_hi[1];
+            if (!(hi - lo > 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)))) break;
+            if (tid == 0) first_true = nt;
+            __syncthreads();
+            const double x = lo + (hi - lo) * (double)(tid + 1) / (double)nt;
+            int count = 0;
+            double q = 1.0;
+            for (int i = 0; i < n; ++i) {  // eigenvalues below x (LDL^T pivots)
+                const double e2 = i > 0 ? ee[i - 1] * ee[i - 1] : 0.0;
+                q = dd[i] - x - (i > 0 ? e2 / q : 0.0);
+                if (q == 0.0) q = -1e-300;
+                if (q < 0.0) ++count;
+            }
+            if (count >= target) atomicMin(&first_true, tid);
+            __syncthreads();
+            if (tid == 0) {
+                const int f = first_true;
+                lo_hi[0] = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)nt;
+                lo_hi[1] = f >= nt ? hi : lo + (hi - lo) * (double)(f + 1) / (double)nt;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const double lambda = 0.5 * (lo_hi[0] + lo_hi[1]);
+            lam_sh[k] = lambda;
+            for (int i = 0; i < n; ++i) {
+                tdg[i] = dd[i] - lambda;
+                tdu2[i] = 0.0;
+                if (i + 1 < n) {
+                    tdl[i] = ee[i];
+                    tdu[i] = ee[i];
+                }
+            }
+            for (int i = 0; i + 1 < n; ++i) {
+                if (fabs(tdg[i]) >= fabs(tdl[i])) {
+                    ipiv[i] = i;
+                    if (tdg[i] == 0.0) tdg[i] = 1e-300;
+                    const double f = tdl[i] / tdg[i];
+                    tdl[i] = f;
+                    tdg[i + 1] -= f * tdu[i];
+                } else {
+                    ipiv[i] = i + 1;
+                    const double f = tdg[i] / tdl[i];
+                    tdg[i] = tdl[i];
+                    tdl[i] = f;
+                    const double t = tdu[i];
+                    tdu[i] = tdg[i + 1];
+                    tdg[i + 1] = t - f * tdg[i + 1];
+                    if (i + 2 < n) {
+                        tdu2[i] = tdu[i + 1];
+                        tdu[i + 1] = -f * tdu[i + 1];
+                    }
+                }
+            }
+            ipiv[n - 1] = n - 1;
+            if (tdg[n - 1] == 0.0) tdg[n - 1] = 1e-300;
+            double* y = Z + (size_t)k * n;
+            unsigned long long st = 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL * (unsigned long long)k;
+            for (int i = 0; i < n; ++i) {
+                st ^= st << 13;
+                st ^= st >> 7;
+                st ^= st << 17;
+                y[i] = (st & 1ULL) ? 1.0 : -1.0;
+            }
+            for (int iter = 0; iter < 3; ++iter) {
+                for (int i = 0; i + 1 < n; ++i) {
+                    if (ipiv[i] == i) {
+                        y[i + 1] -= tdl[i] * y[i];
+                    } else {
+                        const double t = y[i];
+                        y[i] = y[i + 1];
+                        y[i + 1] = t - tdl[i] * y[i];
+                    }
+                }
+                for (int i = n - 1; i >= 0; --i) {
+                    double v = y[i];
+                    if (i + 1 < n) v -= tdu[i] * y[i + 1];
+                    if (i + 2 < n) v -= tdu2[i] * y[i + 2];
+                    y[i] = v / tdg[i];
+                }
+                for (int j = 0; j < k; ++j) {  // against the states already found
+                    const double* zj = Z + (size_t)j * n;
+                    double dot = 0.0;
+                    for (int i = 0; i < n; ++i) dot += zj[i] * y[i];
+                    for (int i = 0; i < n; ++i) y[i] -= dot * zj[i];
+                }
+                double nrm = 0.0;
+                for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
+                nrm = 1.0 / sqrt(nrm);
+                for (int i = 0; i < n; ++i) y[i] *= nrm;
+            }
+        }
+        __syncthreads();
+    }
+    double eavg = 0.0;
+    for (int k = 0; k < S; ++k) {
+        double* y = Z + (size_t)k * n;
+        for (int j = n - 3; j >= 0; --j) {  // z = H_0 ... H_{n-3} y
+            const int m0 = j + 1;
+            double part = 0.0;
+            for (int i = m0 + tid; i < n; i += nt) part += V[i + (size_t)n * j] * y[i];
+            const double sv = beta[j] * block_sum(part, red);
+            for (int i = m0 + tid; i < n; i += nt) y[i] -= sv * V[i + (size_t)n * j];
+            __syncthreads();
+        }
+        double* c = sd.coef + (size_t)k * n;
+        for (int i = tid; i < n; i += nt) {  // c = L^-T z
+            double c0 = 0.0, c1 = 0.0;
+            int m = i;
+            for (; m + 1 < n; m += 2) {
+                c0 += Lm[m + (size_t)n * i] * y[m];
+                c1 += Lm[m + 1 + (size_t)n * i] * y[m + 1];
+            }
+            for (; m < n; ++m) c0 += Lm[m + (size_t)n * i] * y[m];
+            c[i] = c0 + c1;
+        }
+        __syncthreads();
+        double* a = sd.dcoef + (size_t)k * n_det;
+        for (int d = tid; d < n_det; d += nt) {
+            double acc = 0.0;
+            for (long long t = topo.off[d]; t < topo.off[d + 1]; ++t) acc += topo.tc[t] * c[topo.ts[t]];
+            a[d] = acc;
+        }
+        eavg += topo.state_w[k] * lam_sh[k];
+    }
+    if (tid < S) sd.state_energy[tid] = lam_sh[tid];
+    if (tid == 0) {
+        sd.energy[0] = eavg;
+        sd.energy[1] = eavg + sd.e_nuc;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // gradient tail
 // ---------------------------------------------------------------------------------------------
@@ -2250,9 +2427,12 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
     hipLaunchKernelGGL(k_structure_left, dim3((unsigned)((b + 255) / 256), 1, z), dim3(256), 0, st, slots, topo);
     mark(EV_STRUCT);
     const int threads = 256;
+    // one selected state that is the lowest root: the lowest-root solvers; otherwise the
+    // global-memory dense solve followed by the selected-states kernel
+    const bool states = topo.n_states > 1 || topo.state_idx[0] != 0;
     if (shape.davidson)
         launch_davidson(st, slots, z, topo, shape);
-    else if (gep_in_lds(topo.n_str))
+    else if (gep_in_lds(topo.n_str) && !states)
         hipLaunchKernelGGL(k_gep_lowest<true>, dim3(1, 1, z), dim3(threads), gep_lds_bytes(topo.n_str), st, slots,
                            topo);
     else {
@@ -2274,6 +2454,7 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
         } else {
             hipLaunchKernelGGL((k_gep_lowest<false, 3>), dim3(1, 1, z), dim3(1024), 0, st, slots, topo);
         }
+        if (states) hipLaunchKernelGGL(k_gep_states, dim3(1, 1, z), dim3(256), 0, st, slots, topo);
     }
     mark(EV_EIGEN);
 }

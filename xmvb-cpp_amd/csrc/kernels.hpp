@@ -15,6 +15,8 @@ namespace xmvb_amd {
 
 // Largest active space handled by the LDS-resident determinant-pair kernels.
 constexpr int MAX_ACT = 14;
+// Largest number of selected structure states of a state-averaged evaluation.
+constexpr int kMaxStates = 8;
 // Largest occupied block (alpha or beta electrons): one element per lane of a 64-wide wavefront.
 constexpr int MAX_SPIN_BLOCK = 8;
 
@@ -61,6 +63,7 @@ struct SlotDesc {
     uint8_t* defer_flag;     // per lane work item: deferred by the lane kernel (listed into defer in order)
     uint8_t* singular_flag;  // per lane-group entry: rank deficient (listed into singular in order)
     int* chunk_count;        // list compaction: entries listed per chunk of kCompactChunk, then offsets
+    double* state_energy;    // eigenvalue of each selected state (kMaxStates; n_states > 1 only)
     int seeded;     // coef holds this slot's previous structure eigenvector (Davidson start)
 };
 
@@ -85,6 +88,13 @@ struct TopoDesc {
     const int* flip;
     const int* lane_pairs;
     long long n_lane_pairs;  // work items of the lane kernels
+    // Selected structure states (roots by index, ascending eigenvalue) and their normalised weights.
+    // One state: dcoef holds its determinant coefficients and energy[0] its eigenvalue. Several:
+    // dcoef holds n_states blocks of n_det, coef n_states blocks of n_str, state_energy the
+    // eigenvalues and energy[0] their weighted sum.
+    int n_states;
+    int state_idx[kMaxStates];
+    double state_w[kMaxStates];
 };
 
 struct LaunchShape {

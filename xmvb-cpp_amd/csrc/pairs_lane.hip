@@ -582,7 +582,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
     __syncthreads();
     const SpinTables tab = spin_tables(topo, swap_spins);
     const long long n_work = topo.n_lane_pairs;
-    const double E = sd.energy[0];
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_work;
          i += (long long)gridDim.x * blockDim.x) {
         const long long t =
@@ -590,8 +589,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
         int d, e, fd, fe;
         unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
         const long long tf = flip_partner(topo, t, d, e, fd, fe);
-        const double w = (sd.dcoef[d] * sd.dcoef[e] + (tf != t ? sd.dcoef[fd] * sd.dcoef[fe] : 0.0)) * (d == e ? 1.0 : 2.0);
-        if (w == 0.0) continue;
+        double wH, wS;
+        pair_weights(sd, topo, d, e, fd, fe, tf != t, d == e ? 1.0 : 2.0, wH, wS);
+        if (wH == 0.0 && wS == 0.0) continue;
         LaneBlock<M0> b0;
         LaneBlock<M1> b1;
         occupied<M0>((uint32_t)tab.mask0[tab.det0[e]], b0.R);
@@ -609,7 +609,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
         auto pidx0 = [&](int r, int l) { return pair_index(b0.R[r], b0.L[l]); };
         auto pidx1 = [&](int r, int l) { return P1[r][l]; };
         const double c = b0.det * b1.det;
-        const double wH = w, wS = -E * w;
         double F0[LaneBlock<M0>::K][LaneBlock<M0>::K];
         double F1[LaneBlock<M1>::K][LaneBlock<M1>::K];
         double phi_h = 0.0;
@@ -728,7 +727,8 @@ __global__ void __launch_bounds__(256)
             int fd, fe;
             unpack_pair(t, d, e);  // d: left (bra) determinant, e: right (ket)
             const long long tf = flip_partner(topo, t, d, e, fd, fe);
-            w = (sd.dcoef[d] * sd.dcoef[e] + (tf != t ? sd.dcoef[fd] * sd.dcoef[fe] : 0.0)) * (d == e ? 1.0 : 2.0);
+            double wS;
+            pair_weights(sd, topo, d, e, fd, fe, tf != t, d == e ? 1.0 : 2.0, w, wS);  // the adjoint of G needs wH only
         }
         live = live && w != 0.0;
         LaneBlock<M0> b0;

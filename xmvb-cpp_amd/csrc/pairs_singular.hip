@@ -407,14 +407,13 @@ __global__ void __launch_bounds__(256) k_pairs_singular(const SlotDesc* __restri
         }
     }
     __syncthreads();
-    const double E = Backward ? sd.energy[0] : 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         int d, e;
         unpack_pair(sd.singular[i], d, e);
-        double w = 1.0;
+        double w = 1.0, wS = 0.0;
         if (Backward) {
-            w = sd.dcoef[d] * sd.dcoef[e] * (d == e ? 1.0 : 2.0);
-            if (w == 0.0) continue;
+            pair_weights(sd, topo, d, e, d, e, false, d == e ? 1.0 : 2.0, w, wS);
+            if (w == 0.0 && wS == 0.0) continue;
         }
         Chart ch[2];
         const uint64_t mL[2] = {topo.amask[topo.det_a[d]], topo.bmask[topo.det_b[d]]};
@@ -439,7 +438,7 @@ __global__ void __launch_bounds__(256) k_pairs_singular(const SlotDesc* __restri
                     H += scale * c * pair_phi(G2, hact, n, np, ch, D);
                     S += scale * c;
                 } else {
-                    pair_adjoint<FullDg>(G2, hact, n, np, ch, D, c, w * scale, -E * w * scale, dg, ds, dh);
+                    pair_adjoint<FullDg>(G2, hact, n, np, ch, D, c, w * scale, wS * scale, dg, ds, dh);
                 }
             }
         }

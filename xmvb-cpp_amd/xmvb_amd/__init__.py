@@ -49,6 +49,7 @@ EXPORTED_SYMBOLS = (
     "vbe_optimizer_defaults", "vbe_optimize", "vbe_pool_optimize", "vbe_set_eigensolver",
     "vbe_gaussian_basis_size", "vbe_ao_integrals", "vbe_pivoted_cholesky", "vbe_eval_device_noalloc",
     "vbe_server_create", "vbe_server_destroy", "vbe_server_submit", "vbe_server_collect", "vbe_server_stats",
+    "vbe_set_states", "vbe_last_state_energies", "vbe_pool_state_energies",
 )
 
 TERMINATIONS = {0: "max_iterations", 1: "plbfgs_initial_tolerance", 2: "plbfgs_projected_gradient_tolerance",
@@ -172,6 +173,9 @@ def library() -> C.CDLL:
         lib.vbe_optimizer_defaults.restype = None
         lib.vbe_optimize.argtypes = [P, P, P, P, C.POINTER(OptimizerOptions), C.POINTER(OptimizerResult), P, P, I32]
         lib.vbe_pool_optimize.argtypes = [P, I32, P, P, P, C.POINTER(OptimizerOptions), P, P]
+        lib.vbe_set_states.argtypes = [P, I32, P, P]
+        lib.vbe_last_state_energies.argtypes = [P, P, P]
+        lib.vbe_pool_state_energies.argtypes = [P, I32, P]
         lib.vbe_server_create.argtypes = [P, I32, I32, C.POINTER(P)]
         lib.vbe_server_destroy.argtypes = [P]
         lib.vbe_server_destroy.restype = None
@@ -398,6 +402,17 @@ class Context:
                                                  stationarity_tolerance, int(reuse_roots)))
         self.eigensolver = eigensolver
 
+    def set_states(self, states=(0,), weights=None):
+        """Selected structure states (root indices, distinct, at most 8) and their weights (default
+        equal; normalised) for problems and pools created afterwards: the energy and gradient are
+        those of the weighted average of the states' energies."""
+        st = np.ascontiguousarray(states, dtype=np.int32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        if w is not None and len(w) != len(st):
+            raise ValueError("one weight per state")
+        self.check(self._lib.vbe_set_states(self._h, len(st), _ptr(st), _ptr(w)))
+        self.n_states = len(st)
+
     def set_ri_precision(self, precision: int):
         """0: FP64 RI factors; 1: plain FP32 RI factors (FP64 accumulation; a comparison mode that
         misses the 1e-8 Eh bar); 2: compensated FP32 RI factors (FP32 plus an 8-bit correction,
@@ -428,6 +443,7 @@ class Problem:
         self.ctx, self.topo = ctx, topo
         self.precision = getattr(ctx, "ri_precision", 0)
         self.eigensolver = getattr(ctx, "eigensolver", 0)
+        self.n_states = getattr(ctx, "n_states", 1)
         self.n_bf, self.n_act, self.n_aux = n_bf, topo.n_act, n_aux
         dims = Dims(n_bf, topo.n_act, topo.n_alpha, topo.n_beta, n_aux)
         self._S = np.asfortranarray(S, dtype=np.float64)
@@ -523,6 +539,12 @@ class Problem:
                                                                                       "h_act", "g_packed"))))
         return out
 
+    def state_energies(self) -> np.ndarray:
+        """Total energies of the selected states at the last evaluation."""
+        out = np.zeros(getattr(self, "n_states", 1))
+        self.ctx.check(self.ctx._lib.vbe_last_state_energies(self.ctx._h, self._h, _ptr(out)))
+        return out
+
     def phase_times(self) -> dict:
         t = PhaseTimes()
         self.ctx._lib.vbe_last_phase_times(self._h, C.byref(t))
@@ -560,6 +582,7 @@ class Pool:
         self._view = topo.view()
         self._h = C.c_void_p()
         self.n_bf = [0] * capacity
+        self.n_states = getattr(ctx, "n_states", 1)
         ctx._children.add(self)
         ctx.check(ctx._lib.vbe_pool_create(ctx._h, C.byref(self._view), topo.n_act, topo.n_alpha, topo.n_beta,
                                            capacity, C.byref(self._h)))
@@ -650,6 +673,13 @@ class Pool:
                      grad=g[at:at + sz].reshape(self.n_act, self.n_bf[s]).copy())
             out.append(d)
             at += sz
+        return out
+
+    def state_energies(self, slot: int) -> np.ndarray:
+        """Total energies of the selected states of a slot at its last evaluation (pools created
+        with more than one selected state)."""
+        out = np.zeros(self.n_states)
+        self.ctx.check(self.ctx._lib.vbe_pool_state_energies(self._h, slot, _ptr(out)))
         return out
 
     def phase_times(self) -> dict:
