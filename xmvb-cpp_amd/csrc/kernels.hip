@@ -68,6 +68,15 @@ __device__ __forceinline__ double wave_sum(double v) {
     return (wave_bcast(v, 0) + wave_bcast(v, 16)) + (wave_bcast(v, 32) + wave_bcast(v, 48));
 }
 
+// a / b to a few ulp: v_rcp_f64 refined by two Newton steps. Used where only a small relative
+// error matters (Sturm counts, the pivots of a tridiagonal LU), not for results.
+__device__ __forceinline__ double fast_div(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return a * r;
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
     // wavefront sum, then across waves through LDS (red: >= 16 doubles); a single-wave block
     // needs no barrier
@@ -1606,96 +1615,126 @@ __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDes
     // For n <= 64 the tridiagonal lives in registers, lane i holding row i, and the serial
     // recurrences read and write single lanes (readlane/writelane) instead of going through LDS.
     if (n <= 64) {
-        const int ln = tid & 63;
-        const double d_r = ln < n ? dd[ln] : 0.0;
-        const double e_r = ln + 1 < n ? ee[ln] : 0.0;   // T[ln+1, ln]
-        const double e2_r = ln > 0 && ln < n ? ee[ln - 1] * ee[ln - 1] : 0.0;
+        // The first wavefront alone, with no block barrier: multisection on Sturm counts (LDL^T pivot
+        // recurrence, 64 points per step, read through readlane) to a relative width of 1e-10, then
+        // two steps of inverse iteration on T - sigma I whose serial recurrences run in every lane
+        // on wave-uniform values read from LDS (only their arithmetic is on the critical path), and
+        // the eigenvalue as the Rayleigh quotient of that vector.
+        __shared__ double t_d[64], t_e[64], t_f[64], t_g[64], t_u[64], t_u2[64], t_sw[64], t_y[64], t_b[64];
         if (tid < 64) {
-            const double e_prev = ln > 0 && ln < n ? fabs(ee[ln - 1]) : 0.0;
+            const int ln = tid;
+            const bool row = ln < n;
+            const double d_r = row ? dd[ln] : 0.0;
+            const double e_r = ln + 1 < n ? ee[ln] : 0.0;  // T[ln+1, ln]
+            const double e2_r = ln > 0 && row ? ee[ln - 1] * ee[ln - 1] : 0.0;
+            const double e_prev = ln > 0 && row ? fabs(ee[ln - 1]) : 0.0;
             const double rad = e_prev + fabs(e_r);
-            const double lo = -wave_max(ln < n ? rad - d_r : -1e300);
-            const double hi = wave_max(ln < n ? d_r + rad : -1e300);
+            double lo = -wave_max(row ? rad - d_r : -1e300);
+            double hi = wave_max(row ? d_r + rad : -1e300);
             const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
-            if (tid == 0) {
-                lo_hi[0] = lo - pad;
-                lo_hi[1] = hi + pad;
-            }
-        }
-        __syncthreads();
-        for (int it = 0; it < 200; ++it) {
-            const double lo = lo_hi[0], hi = lo_hi[1];
-            if (!(hi - lo > 4.0 * 2.220446049250313e-16 * fmax(fabs(lo), fabs(hi)))) break;
-            if (tid == 0) first_true = nt;
-            __syncthreads();
-            const double x = lo + (hi - lo) * (double)(tid + 1) / (double)nt;
-            int count = 0;
-            double q = wave_bcast(d_r, 0) - x;
-            if (q == 0.0) q = -1e-300;
-            if (q < 0.0) ++count;
-            for (int i = 1; i < n; ++i) {  // number of eigenvalues below x (LDL^T pivots)
-                q = wave_bcast(d_r, i) - x - wave_bcast(e2_r, i) / q;
+            lo -= pad;
+            hi += pad;
+            for (int it = 0; it < 64; ++it) {
+                if (!(hi - lo > 1e-10 * fmax(fabs(lo), fabs(hi)))) break;
+                const double x = lo + (hi - lo) * (double)(ln + 1) / 64.0;
+                int count = 0;
+                double q = wave_bcast(d_r, 0) - x;
                 if (q == 0.0) q = -1e-300;
-                if (q < 0.0) ++count;
-            }
-            if (count >= 1) atomicMin(&first_true, tid);
-            __syncthreads();
-            if (tid == 0) {
-                const int f = first_true;
-                lo_hi[0] = f == 0 ? lo : lo + (hi - lo) * (double)f / (double)nt;
-                lo_hi[1] = f >= nt ? hi : lo + (hi - lo) * (double)(f + 1) / (double)nt;
-            }
-            __syncthreads();
-        }
-        const double lambda = 0.5 * (lo_hi[0] + lo_hi[1]);
-        if (tid < 64) {
-            double g = ln < n ? d_r - lambda : 1.0;  // diagonal of U
-            double l = e_r;                          // subdiagonal, then the multipliers
-            double u = e_r;                          // first superdiagonal of U
-            double u2 = 0.0;                         // second superdiagonal of U
-            int swp = 0;                             // lane i: rows i and i+1 interchanged
-            for (int i = 0; i + 1 < n; ++i) {
-                double gi = wave_bcast(g, i);
-                const double li = wave_bcast(l, i), ui = wave_bcast(u, i), gi1 = wave_bcast(g, i + 1);
-                if (fabs(gi) >= fabs(li)) {
-                    if (gi == 0.0) gi = 1e-300;
-                    const double f = li / gi;
-                    g = wave_put(gi, i, g);
-                    l = wave_put(f, i, l);
-                    g = wave_put(gi1 - f * ui, i + 1, g);
-                } else {
-                    const double f = gi / li;
-                    g = wave_put(li, i, g);
-                    l = wave_put(f, i, l);
-                    u = wave_put(gi1, i, u);
-                    g = wave_put(ui - f * gi1, i + 1, g);
-                    if (i + 2 < n) {
-                        const double ui1 = wave_bcast(u, i + 1);
-                        u2 = wave_put(ui1, i, u2);
-                        u = wave_put(-f * ui1, i + 1, u);
-                    }
-                    if (ln == i) swp = 1;
+                count += q < 0.0;
+                for (int i = 1; i < n; ++i) {  // eigenvalues below x (LDL^T pivots)
+                    q = wave_bcast(d_r, i) - x - fast_div(wave_bcast(e2_r, i), q);
+                    if (q == 0.0) q = -1e-300;
+                    count += q < 0.0;
                 }
+                const uint64_t hit = __ballot(count >= 1);
+                const int f = hit ? __builtin_ctzll(hit) : 64;
+                const double nlo = f == 0 ? lo : lo + (hi - lo) * (double)f / 64.0;
+                const double nhi = f >= 64 ? hi : lo + (hi - lo) * (double)(f + 1) / 64.0;
+                lo = nlo;
+                hi = nhi;
             }
-            if (wave_bcast(g, n - 1) == 0.0) g = wave_put(1e-300, n - 1, g);
-            double yv = ln < n ? 1.0 : 0.0;
-            for (int iter = 0; iter < 3; ++iter) {
+            const double sig = 0.5 * (lo + hi);
+            if (row) {
+                t_d[ln] = d_r - sig;
+                t_e[ln] = e_r;
+            }
+            wave_sync();
+            {  // LU of T - sig I with partial pivoting (LAPACK dgttrf), every lane on the same values
+                double gi = t_d[0], ui = t_e[0];
                 for (int i = 0; i + 1 < n; ++i) {
-                    const double yi = wave_bcast(yv, i), yi1 = wave_bcast(yv, i + 1), li = wave_bcast(l, i);
-                    if (__builtin_amdgcn_readlane(swp, i) == 0) {
-                        yv = wave_put(yi1 - li * yi, i + 1, yv);
+                    const double li = t_e[i], gi1 = t_d[i + 1], ui1 = i + 2 < n ? t_e[i + 1] : 0.0;
+                    double fk, gk, uk, u2k, swk, gn, un;
+                    if (fabs(gi) >= fabs(li)) {
+                        if (gi == 0.0) gi = 1e-300;
+                        fk = fast_div(li, gi);
+                        gk = gi;
+                        uk = ui;
+                        u2k = 0.0;
+                        swk = 0.0;
+                        gn = gi1 - fk * ui;
+                        un = ui1;
                     } else {
-                        yv = wave_put(yi1, i, yv);
-                        yv = wave_put(yi - li * yi1, i + 1, yv);
+                        fk = fast_div(gi, li);
+                        gk = li;
+                        uk = gi1;
+                        u2k = ui1;
+                        swk = 1.0;
+                        gn = ui - fk * gi1;
+                        un = -fk * ui1;
                     }
+                    if (ln == i) {
+                        t_f[i] = fk;
+                        t_g[i] = gk;
+                        t_u[i] = uk;
+                        t_u2[i] = u2k;
+                        t_sw[i] = swk;
+                    }
+                    gi = gn;
+                    ui = un;
                 }
+                if (gi == 0.0) gi = 1e-300;
+                if (ln == n - 1) {
+                    t_g[n - 1] = gi;
+                    t_u[n - 1] = 0.0;
+                    t_u2[n - 1] = 0.0;
+                }
+            }
+            wave_sync();
+            if (row) t_g[ln] = 1.0 / t_g[ln];  // pivot reciprocals, all lanes at once
+            double yv = row ? 1.0 : 0.0;
+            for (int iter = 0; iter < 2; ++iter) {  // dgttrs from the all-ones vector, twice
+                if (row) t_y[ln] = yv;
+                wave_sync();
+                double cur = t_y[0], yo = 0.0;
+                for (int i = 0; i + 1 < n; ++i) {
+                    const double yn = t_y[i + 1], li = t_f[i];
+                    double out;
+                    if (t_sw[i] == 0.0) {
+                        out = cur;
+                        cur = yn - li * cur;
+                    } else {
+                        out = yn;
+                        cur = cur - li * yn;
+                    }
+                    if (ln == i) yo = out;
+                }
+                if (ln == n - 1) yo = cur;
+                if (row) t_b[ln] = yo;
+                wave_sync();
+                double c1 = 0.0, c2 = 0.0;  // y_{i+1}, y_{i+2}
                 for (int i = n - 1; i >= 0; --i) {
-                    double v = wave_bcast(yv, i);
-                    if (i + 1 < n) v -= wave_bcast(u, i) * wave_bcast(yv, i + 1);
-                    if (i + 2 < n) v -= wave_bcast(u2, i) * wave_bcast(yv, i + 2);
-                    yv = wave_put(v / wave_bcast(g, i), i, yv);
+                    const double v = (t_b[i] - t_u[i] * c1 - t_u2[i] * c2) * t_g[i];
+                    c2 = c1;
+                    c1 = v;
+                    if (ln == i) yv = v;
                 }
                 yv *= 1.0 / sqrt(wave_sum(yv * yv));
+                wave_sync();
             }
+            // Rayleigh quotient y^T T y: (T y)_i = e_{i-1} y_{i-1} + d_i y_i + e_i y_{i+1}
+            const double y_dn = __shfl_up(yv, 1), y_up = __shfl_down(yv, 1), e_dn = __shfl_up(e_r, 1);
+            const double ty = row ? d_r * yv + (ln > 0 ? e_dn * y_dn : 0.0) + (ln + 1 < n ? e_r * y_up : 0.0) : 0.0;
+            const double lambda = wave_sum(yv * ty);
             for (int k = n - 3; k >= 0; --k) {  // back-transform z = H_0 H_1 ... H_{n-3} y
                 const double vk = ln > k && ln < n ? V[ln + n * k] : 0.0;
                 yv -= beta[k] * wave_sum(vk * yv) * vk;
@@ -2322,6 +2361,7 @@ hipError_t configure_kernels(int n, int ng, int n_str, int mmax) {
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_gep_lowest<true>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGepLdsLimit);
 }
+
 
 size_t gep_lds_bytes(int n) { return gep_core_doubles(n) * sizeof(double); }
 
