@@ -87,7 +87,9 @@ def test_budget_refusal_leaves_slot_unloaded():
 def test_served_mix_matches_alone():
     ctx = vb.Context(0)
     rng = np.random.default_rng(5)
-    server = serving.Server(ctx, capacity=16, max_batch=8)
+    # one launch in flight per pool, so that requests arriving during a launch queue behind it and
+    # are batched however fast the launches are
+    server = serving.Server(ctx, capacity=16, max_batch=8, max_inflight_per_pool=1)
     probs = {}
     for n_act in (4, 6, 8):
         topo = vb.Topology.build("COV", n_act, n_act, 1)

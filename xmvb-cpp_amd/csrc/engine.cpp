@@ -90,6 +90,13 @@ struct vbe_context {
         }
         return streams[next_stream++ % streams.size()];
     }
+    // the stream after `main` in the set: a lane's side stream for launches of few problems
+    hipStream_t side_stream(hipStream_t main) {
+        std::lock_guard<std::mutex> g(mu);
+        for (size_t k = 0; k < streams.size(); ++k)
+            if (streams[k] == main) return streams[(k + 1) % streams.size()];
+        return main;
+    }
     ~vbe_context() {
         for (hipStream_t st : streams) {
             (void)hipStreamSynchronize(st);
@@ -166,6 +173,7 @@ struct Lane {
     size_t stage_doubles = 0;
     hipEvent_t ev[xmvb_amd::EV_COUNT] = {};
     hipEvent_t done = nullptr;
+    xmvb_amd::SideStream side;  // concurrent kernels of launches of few problems
     bool busy = false;
     bool with_grad = false;
     bool events_valid = false;
@@ -208,6 +216,8 @@ struct vbe_pool {
             for (auto& e : l.ev)
                 if (e) (void)hipEventDestroy(e);
             if (l.done) (void)hipEventDestroy(l.done);
+            if (l.side.fork) (void)hipEventDestroy(l.side.fork);
+            if (l.side.join) (void)hipEventDestroy(l.side.join);
             if (l.h_desc) (void)hipHostFree(l.h_desc);
             if (l.h_energy) (void)hipHostFree(l.h_energy);
             if (l.h_flags) (void)hipHostFree(l.h_flags);
@@ -312,6 +322,9 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
         l.stream = ctx->take_stream();
         for (auto& e : l.ev) check(hipEventCreate(&e), "hipEventCreate");
         check(hipEventCreateWithFlags(&l.done, hipEventDisableTiming), "hipEventCreate");
+        l.side.st = ctx->side_stream(l.stream);
+        check(hipEventCreateWithFlags(&l.side.fork, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventCreateWithFlags(&l.side.join, hipEventDisableTiming), "hipEventCreate");
         check(hipHostMalloc(reinterpret_cast<void**>(&l.h_desc), sizeof(SlotDesc) * capacity, hipHostMallocDefault),
               "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void**>(&l.h_energy), sizeof(double) * 2 * capacity, hipHostMallocDefault),
@@ -665,6 +678,8 @@ void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool wit
     if (ws && count != 1) throw StatusError(VBE_ERR_ARGUMENT, "a replacement workspace serves one slot");
     hipStream_t st = lane.stream;
     const LaunchShape sh = shape_for(p, in, pinned);
+    // the side stream only when it is idle: queued behind another lane's launch it would delay this one
+    const xmvb_amd::SideStream* side = hipStreamQuery(lane.side.st) == hipSuccess ? &lane.side : nullptr;
     if (forward) {
         std::vector<char> seen(p->capacity, 0);
         for (int i = 0; i < count; ++i) {
@@ -685,11 +700,11 @@ void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool wit
         }
         upload(lane.d_desc, lane.h_desc, count, st);
         check(hipMemsetAsync(lane.d_flags, 0, sizeof(int) * FLAG_COUNT * count, st), "hipMemsetAsync");
-        xmvb_amd::launch_forward(st, lane.d_desc, count, p->topo, sh, lane.ev);
+        xmvb_amd::launch_forward(st, lane.d_desc, count, p->topo, sh, lane.ev, side);
         check(hipGetLastError(), "forward launch");
     }
     if (with_grad) {
-        xmvb_amd::launch_backward(st, lane.d_desc, count, p->topo, sh, lane.ev);
+        xmvb_amd::launch_backward(st, lane.d_desc, count, p->topo, sh, lane.ev, side);
         check(hipGetLastError(), "backward launch");
     }
     check(hipMemcpyAsync(lane.h_energy, lane.d_energy, sizeof(double) * 2 * count, hipMemcpyDeviceToHost, st),

@@ -23,59 +23,10 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "wave.hpp"
 
 namespace xmvb_amd {
 
-
-// wave-uniform broadcast of lane `src`
-__device__ __forceinline__ double wave_bcast(double v, int src) {
-    const long long bits = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), src);
-    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// returns `old` with lane `dst` replaced by the wave-uniform value v (one compare and select)
-__device__ __forceinline__ double wave_put(double v, int dst, double old) {
-    return (int)(threadIdx.x & 63) == dst ? v : old;
-}
-
-// 64-bit DPP move (both halves with the same control)
-template <int Ctrl>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), Ctrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), Ctrl, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// wave-uniform maximum: DPP within 16-lane rows (xor 1, xor 2, half-row mirror, row mirror),
-// then the four row maxima through readlane
-__device__ __forceinline__ double wave_max(double v) {
-    v = fmax(v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = fmax(v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = fmax(v, dpp_f64<0x141>(v));  // row_half_mirror
-    v = fmax(v, dpp_f64<0x140>(v));  // row_mirror
-    return fmax(fmax(wave_bcast(v, 0), wave_bcast(v, 16)), fmax(wave_bcast(v, 32), wave_bcast(v, 48)));
-}
-
-// wave-uniform sum: DPP within 16-lane rows, then the four row sums through readlane
-__device__ __forceinline__ double wave_sum(double v) {
-    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
-    v += dpp_f64<0x141>(v);  // row_half_mirror
-    v += dpp_f64<0x140>(v);  // row_mirror
-    return (wave_bcast(v, 0) + wave_bcast(v, 16)) + (wave_bcast(v, 32) + wave_bcast(v, 48));
-}
-
-// a / b to a few ulp: v_rcp_f64 refined by two Newton steps. Used where only a small relative
-// error matters (Sturm counts, the pivots of a tridiagonal LU), not for results.
-__device__ __forceinline__ double fast_div(double a, double b) {
-    double r = __builtin_amdgcn_rcp(b);
-    r = fma(r, fma(-b, r, 1.0), r);
-    r = fma(r, fma(-b, r, 1.0), r);
-    return a * r;
-}
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
     // wavefront sum, then across waves through LDS (red: >= 16 doubles); a single-wave block
@@ -449,11 +400,6 @@ __global__ void __launch_bounds__(256) k_ri_pair_gram_tiled(const SlotDesc* __re
 // ---------------------------------------------------------------------------------------------
 // determinant pairs: one wavefront per unordered determinant pair (d <= e)
 // ---------------------------------------------------------------------------------------------
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // k-th set bit of a mask (k < popcount)
 __device__ __forceinline__ int nth_bit(uint64_t m, int k) {
@@ -1615,126 +1561,14 @@ __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDes
     // For n <= 64 the tridiagonal lives in registers, lane i holding row i, and the serial
     // recurrences read and write single lanes (readlane/writelane) instead of going through LDS.
     if (n <= 64) {
-        // The first wavefront alone, with no block barrier: multisection on Sturm counts (LDL^T pivot
-        // recurrence, 64 points per step, read through readlane) to a relative width of 1e-10, then
-        // two steps of inverse iteration on T - sigma I whose serial recurrences run in every lane
-        // on wave-uniform values read from LDS (only their arithmetic is on the critical path), and
-        // the eigenvalue as the Rayleigh quotient of that vector.
-        __shared__ double t_d[64], t_e[64], t_f[64], t_g[64], t_u[64], t_u2[64], t_sw[64], t_y[64], t_b[64];
+        // the first wavefront alone (tridiag_lowest_wave), then the back-transform
+        __shared__ __align__(16) double t_scratch[288];
         if (tid < 64) {
             const int ln = tid;
-            const bool row = ln < n;
-            const double d_r = row ? dd[ln] : 0.0;
+            const double d_r = ln < n ? dd[ln] : 0.0;
             const double e_r = ln + 1 < n ? ee[ln] : 0.0;  // T[ln+1, ln]
-            const double e2_r = ln > 0 && row ? ee[ln - 1] * ee[ln - 1] : 0.0;
-            const double e_prev = ln > 0 && row ? fabs(ee[ln - 1]) : 0.0;
-            const double rad = e_prev + fabs(e_r);
-            double lo = -wave_max(row ? rad - d_r : -1e300);
-            double hi = wave_max(row ? d_r + rad : -1e300);
-            const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
-            lo -= pad;
-            hi += pad;
-            for (int it = 0; it < 64; ++it) {
-                if (!(hi - lo > 1e-10 * fmax(fabs(lo), fabs(hi)))) break;
-                const double x = lo + (hi - lo) * (double)(ln + 1) / 64.0;
-                int count = 0;
-                double q = wave_bcast(d_r, 0) - x;
-                if (q == 0.0) q = -1e-300;
-                count += q < 0.0;
-                for (int i = 1; i < n; ++i) {  // eigenvalues below x (LDL^T pivots)
-                    q = wave_bcast(d_r, i) - x - fast_div(wave_bcast(e2_r, i), q);
-                    if (q == 0.0) q = -1e-300;
-                    count += q < 0.0;
-                }
-                const uint64_t hit = __ballot(count >= 1);
-                const int f = hit ? __builtin_ctzll(hit) : 64;
-                const double nlo = f == 0 ? lo : lo + (hi - lo) * (double)f / 64.0;
-                const double nhi = f >= 64 ? hi : lo + (hi - lo) * (double)(f + 1) / 64.0;
-                lo = nlo;
-                hi = nhi;
-            }
-            const double sig = 0.5 * (lo + hi);
-            if (row) {
-                t_d[ln] = d_r - sig;
-                t_e[ln] = e_r;
-            }
-            wave_sync();
-            {  // LU of T - sig I with partial pivoting (LAPACK dgttrf), every lane on the same values
-                double gi = t_d[0], ui = t_e[0];
-                for (int i = 0; i + 1 < n; ++i) {
-                    const double li = t_e[i], gi1 = t_d[i + 1], ui1 = i + 2 < n ? t_e[i + 1] : 0.0;
-                    double fk, gk, uk, u2k, swk, gn, un;
-                    if (fabs(gi) >= fabs(li)) {
-                        if (gi == 0.0) gi = 1e-300;
-                        fk = fast_div(li, gi);
-                        gk = gi;
-                        uk = ui;
-                        u2k = 0.0;
-                        swk = 0.0;
-                        gn = gi1 - fk * ui;
-                        un = ui1;
-                    } else {
-                        fk = fast_div(gi, li);
-                        gk = li;
-                        uk = gi1;
-                        u2k = ui1;
-                        swk = 1.0;
-                        gn = ui - fk * gi1;
-                        un = -fk * ui1;
-                    }
-                    if (ln == i) {
-                        t_f[i] = fk;
-                        t_g[i] = gk;
-                        t_u[i] = uk;
-                        t_u2[i] = u2k;
-                        t_sw[i] = swk;
-                    }
-                    gi = gn;
-                    ui = un;
-                }
-                if (gi == 0.0) gi = 1e-300;
-                if (ln == n - 1) {
-                    t_g[n - 1] = gi;
-                    t_u[n - 1] = 0.0;
-                    t_u2[n - 1] = 0.0;
-                }
-            }
-            wave_sync();
-            if (row) t_g[ln] = 1.0 / t_g[ln];  // pivot reciprocals, all lanes at once
-            double yv = row ? 1.0 : 0.0;
-            for (int iter = 0; iter < 2; ++iter) {  // dgttrs from the all-ones vector, twice
-                if (row) t_y[ln] = yv;
-                wave_sync();
-                double cur = t_y[0], yo = 0.0;
-                for (int i = 0; i + 1 < n; ++i) {
-                    const double yn = t_y[i + 1], li = t_f[i];
-                    double out;
-                    if (t_sw[i] == 0.0) {
-                        out = cur;
-                        cur = yn - li * cur;
-                    } else {
-                        out = yn;
-                        cur = cur - li * yn;
-                    }
-                    if (ln == i) yo = out;
-                }
-                if (ln == n - 1) yo = cur;
-                if (row) t_b[ln] = yo;
-                wave_sync();
-                double c1 = 0.0, c2 = 0.0;  // y_{i+1}, y_{i+2}
-                for (int i = n - 1; i >= 0; --i) {
-                    const double v = (t_b[i] - t_u[i] * c1 - t_u2[i] * c2) * t_g[i];
-                    c2 = c1;
-                    c1 = v;
-                    if (ln == i) yv = v;
-                }
-                yv *= 1.0 / sqrt(wave_sum(yv * yv));
-                wave_sync();
-            }
-            // Rayleigh quotient y^T T y: (T y)_i = e_{i-1} y_{i-1} + d_i y_i + e_i y_{i+1}
-            const double y_dn = __shfl_up(yv, 1), y_up = __shfl_down(yv, 1), e_dn = __shfl_up(e_r, 1);
-            const double ty = row ? d_r * yv + (ln > 0 ? e_dn * y_dn : 0.0) + (ln + 1 < n ? e_r * y_up : 0.0) : 0.0;
-            const double lambda = wave_sum(yv * ty);
+            double yv;
+            const double lambda = tridiag_lowest_wave(ln, n, d_r, e_r, t_scratch, yv);
             for (int k = n - 3; k >= 0; --k) {  // back-transform z = H_0 H_1 ... H_{n-3} y
                 const double vk = ln > k && ln < n ? V[ln + n * k] : 0.0;
                 yv -= beta[k] * wave_sum(vk * yv) * vk;
@@ -2398,16 +2232,25 @@ void launch_gep_front(hipStream_t st, const SlotDesc* slots, unsigned z, const T
 }
 
 void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape,
-                    hipEvent_t* ev) {
+                    hipEvent_t* ev, const SideStream* side) {
     const int n = topo.n, np = topo.np, ng = topo.ng;
     const unsigned z = (unsigned)count;
     auto mark = [&](int k) {
         if (ev) (void)hipEventRecord(ev[k], st);
     };
+    // few problems: the active-space matrices (read first by the pair kernels) beside the RI transform
+    const bool fork = side && count <= kForkMaxProblems;
     mark(EV_START);
     hipLaunchKernelGGL(k_orbital_prep, dim3(n, 1, z), dim3(256), ((size_t)shape.max_n_bf + 16) * sizeof(double), st,
                        slots, n);
-    hipLaunchKernelGGL(k_active_matrices, dim3(1, 1, z), dim3(256), 0, st, slots, n);
+    if (fork) {
+        (void)hipEventRecord(side->fork, st);
+        (void)hipStreamWaitEvent(side->st, side->fork, 0);
+        hipLaunchKernelGGL(k_active_matrices, dim3(1, 1, z), dim3(256), 0, side->st, slots, n);
+        (void)hipEventRecord(side->join, side->st);
+    } else {
+        hipLaunchKernelGGL(k_active_matrices, dim3(1, 1, z), dim3(256), 0, st, slots, n);
+    }
     mark(EV_PREP);
     if (shape.any_tiled)
         launch_ri_one_index_tiled(st, slots, count, n, shape.max_n_bf_tiled, shape.max_n_aux, shape.b32c);
@@ -2446,6 +2289,7 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
         hipLaunchKernelGGL(k_ri_pair_gram, dim3((ng + 3) / 4, 1, z), dim3(256), 0, st, slots, np);
     else
         hipLaunchKernelGGL(k_ri_pair_gram_tiled, dim3(1, 1, z), dim3(256), 0, st, slots, np);
+    if (fork) (void)hipStreamWaitEvent(st, side->join, 0);
     mark(EV_TRANSFORM);
     if (shape.lane_fwd) {
         launch_pairs_lane(st, 0, shape.lane_blocks_fwd, slots, count, topo, shape.n_alpha, shape.n_beta,
@@ -2472,7 +2316,8 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
     const bool states = topo.n_states > 1 || topo.state_idx[0] != 0;
     if (shape.davidson)
         launch_davidson(st, slots, z, topo, shape);
-    else if (gep_in_lds(topo.n_str) && !states)
+    else if (!states && launch_gep_wave(st, slots, z, topo)) {
+    } else if (gep_in_lds(topo.n_str) && !states)
         hipLaunchKernelGGL(k_gep_lowest<true>, dim3(1, 1, z), dim3(threads), gep_lds_bytes(topo.n_str), st, slots,
                            topo);
     else {
@@ -2500,21 +2345,31 @@ void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const Topo
 }
 
 void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape,
-                     hipEvent_t* ev) {
+                     hipEvent_t* ev, const SideStream* side) {
     const int n = topo.n, np = topo.np, ng = topo.ng;
     const unsigned z = (unsigned)count;
     auto mark = [&](int k) {
         if (ev) (void)hipEventRecord(ev[k], st);
     };
     mark(EV_BWD_BEGIN);
+    // The pair-adjoint kernels write disjoint columns (field: ds/dh; adjoint: dg) or disjoint rows
+    // (deferred list, singular pairs) of the partial rows, so for few problems the adjoint kernel and
+    // the singular-pair kernel run on the side stream beside the field and deferred-list kernels.
+    const bool fork = side && count <= kForkMaxProblems && shape.lane_bwd;
+    hipStream_t st2 = st;
+    if (fork) {
+        (void)hipEventRecord(side->fork, st);
+        (void)hipStreamWaitEvent(side->st, side->fork, 0);
+        st2 = side->st;
+    }
     int part_rows = shape.pair_blocks_bwd, list_begin = 0, list_rows = 0, list_per_row = 1;
     if (shape.lane_bwd) {
         launch_pairs_lane(st, 1, shape.lane_blocks_bwd, slots, count, topo, shape.n_alpha, shape.n_beta,
                           shape.lane_cond_limit);
         mark(EV_BWD_FIELD);
-        launch_pairs_lane(st, 2, shape.lane_blocks_bwd, slots, count, topo, shape.n_alpha, shape.n_beta,
+        launch_pairs_lane(st2, 2, shape.lane_blocks_bwd, slots, count, topo, shape.n_alpha, shape.n_beta,
                           shape.lane_cond_limit);
-        mark(EV_BWD_ADJOINT);
+        if (!fork) mark(EV_BWD_ADJOINT);
         list_per_row = launch_pairs_group(st, true, true, shape.list_blocks, shape.lane_blocks_bwd, slots, count, topo,
                                           shape.mmax);
         list_begin = shape.lane_blocks_bwd;
@@ -2525,8 +2380,13 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
         mark(EV_BWD_FIELD);
         mark(EV_BWD_ADJOINT);
     }
-    launch_pairs_singular(st, true, part_rows, slots, count, topo);
+    launch_pairs_singular(st2, true, part_rows, slots, count, topo);
     part_rows += kSingularBlocks;
+    if (fork) {
+        (void)hipEventRecord(side->join, side->st);
+        (void)hipStreamWaitEvent(st, side->join, 0);
+        mark(EV_BWD_ADJOINT);  // field and deferred list done on the main stream, adjoint joined
+    }
     mark(EV_BWD_KERNEL);
     const int len = ng + 2 * n * n;
     hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, part_rows, len,

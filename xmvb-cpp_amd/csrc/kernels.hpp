@@ -183,13 +183,25 @@ void launch_pairs_lane(hipStream_t st, int which, int blocks, const SlotDesc* sl
 bool pairs_full_dg(int n, int ng);
 std::size_t gep_lds_bytes(int n);
 bool gep_in_lds(int n);
+// Lowest root of one selected state for n_str <= 64: one wavefront per problem, columns in
+// registers (gep_wave.hip). Returns false, launching nothing, for larger structure spaces.
+bool launch_gep_wave(hipStream_t st, const SlotDesc* slots, unsigned z, const TopoDesc& topo);
 hipError_t configure_kernels(int n, int ng, int n_str, int mmax);
+
+// A second stream for launches of few problems: kernels with disjoint outputs that each fill only
+// a corner of the device run on it concurrently with the main stream (fork: an event recorded on
+// the main stream that the side stream waits for; join: the reverse). Results do not depend on it.
+struct SideStream {
+    hipStream_t st = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int kForkMaxProblems = 8;  // launches of more problems fill the device kernel by kernel
 
 // forward: orbital prep, RI transform, determinant pairs, structure H/S, eigensolve (+ a_d)
 void launch_forward(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape,
-                    hipEvent_t* ev);
+                    hipEvent_t* ev, const SideStream* side = nullptr);
 // backward: pair adjoints, partial reduction, RI/orbital pullback to the raw-slot gradient
 void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const TopoDesc& topo, const LaunchShape& shape,
-                     hipEvent_t* ev);
+                     hipEvent_t* ev, const SideStream* side = nullptr);
 
 }  // namespace xmvb_amd
