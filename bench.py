@@ -403,8 +403,8 @@ def main():
                              "LDS gathers of the kernel"),
         "k_ri_one_index": (mean["ri_one_index_ms"], "hbm",
                            K * 8.0 * cfg["n_aux"] * (nbp + cfg["n_act"] * cfg["n_bf"]), "GB/s", "HBM"),
-        "k_gep_lowest": (mean["eigensolve_ms"], "mfma", K * 10.0 * topo.n_structures ** 3, "TFLOP/s",
-                         "latency: serial recurrences and block barriers"),
+        "k_gep_wave": (mean["eigensolve_ms"], "mfma", K * 10.0 * topo.n_structures ** 3, "TFLOP/s",
+                       "latency: per-step LDS round trips, wave reductions and serial recurrences"),
     }
     traffic_table = {}
     if os.path.exists(TRAFFIC_FILE):

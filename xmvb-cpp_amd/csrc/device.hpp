@@ -3,9 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include "gmem.hpp"
 #include "kernels.hpp"
 
 namespace xmvb_amd {
+
 
 // packed index of the unordered pair (a, b): hi(hi+1)/2 + lo
 __device__ __forceinline__ int pair_index(int a, int b) {

@@ -671,8 +671,10 @@ LaunchShape shape_for(const vbe_pool* p, const std::vector<SlotInput>& in, bool 
 
 // Enqueue forward and/or backward for a set of slots on a lane, then the D2H of energies and flags.
 // ws (optional, single-slot launches): a descriptor whose workspace replaces the slot's own.
+// timed: record the phase events (read by finish into the pool's phase times); the optimiser's
+// inner evaluations skip them, a dozen event records per launch being host time on its critical path.
 void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool with_grad, bool forward,
-             const SlotDesc* ws = nullptr, bool pinned = false) {
+             const SlotDesc* ws = nullptr, bool pinned = false, bool timed = true) {
     const int count = (int)in.size();
     if (count <= 0 || count > p->capacity) throw StatusError(VBE_ERR_ARGUMENT, "invalid number of slots");
     if (ws && count != 1) throw StatusError(VBE_ERR_ARGUMENT, "a replacement workspace serves one slot");
@@ -700,11 +702,11 @@ void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool wit
         }
         upload(lane.d_desc, lane.h_desc, count, st);
         check(hipMemsetAsync(lane.d_flags, 0, sizeof(int) * FLAG_COUNT * count, st), "hipMemsetAsync");
-        xmvb_amd::launch_forward(st, lane.d_desc, count, p->topo, sh, lane.ev, side);
+        xmvb_amd::launch_forward(st, lane.d_desc, count, p->topo, sh, timed ? lane.ev : nullptr, side);
         check(hipGetLastError(), "forward launch");
     }
     if (with_grad) {
-        xmvb_amd::launch_backward(st, lane.d_desc, count, p->topo, sh, lane.ev, side);
+        xmvb_amd::launch_backward(st, lane.d_desc, count, p->topo, sh, timed ? lane.ev : nullptr, side);
         check(hipGetLastError(), "backward launch");
     }
     check(hipMemcpyAsync(lane.h_energy, lane.d_energy, sizeof(double) * 2 * count, hipMemcpyDeviceToHost, st),
@@ -716,7 +718,7 @@ void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool wit
     lane.slots.clear();
     for (const auto& x : in) lane.slots.push_back(x.slot);
     lane.with_grad = with_grad;
-    lane.events_valid = forward;
+    lane.events_valid = forward && timed;
     lane.scratch = ws != nullptr;
 }
 
@@ -854,7 +856,7 @@ void optimize_slots(vbe_pool* p, vbe_problem* prob, const std::vector<int>& slot
         lane.busy = true;
         try {
             // pinned: each slot follows the iterates it would follow alone, whatever the batch
-            enqueue(p, lane, in, with_grad, true, nullptr, true);
+            enqueue(p, lane, in, with_grad, true, nullptr, true, false);
         } catch (...) {
             lane.busy = false;
             throw;

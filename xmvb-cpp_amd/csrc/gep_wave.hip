@@ -78,7 +78,7 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
 // lane j: a[i] = (M[i, j] + M[j, i]) / 2 for an n x n column-major M, read by rows (coalesced)
 // and transposed through LDS (W: N x P doubles)
 template <int N, int P>
-__device__ __forceinline__ void load_symmetric(double (&a)[N], const double* __restrict__ M, int n, int j, double* W) {
+__device__ __forceinline__ void load_symmetric(double (&a)[N], gptr<const double> M, int n, int j, double* W) {
 #pragma unroll
     for (int i = 0; i < N; ++i) a[i] = (j < n && i < n) ? M[j + n * i] : 0.0;  // M[j, i]
     if (j < N) {
@@ -103,8 +103,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
     __shared__ double rl[N];               // 1 / L[k, k]
     __shared__ double hb[N];               // Householder beta
     __shared__ __align__(16) double tri[288];  // tridiagonal solver scratch
-    const double* __restrict__ Ss = sd.Ss;
-    const double* __restrict__ Hs = sd.Hs;
+    const auto Ss = to_global(sd.Ss);
+    const auto Hs = to_global(sd.Hs);
     double a[N];
     load_symmetric<N, P>(a, Ss, n, j, Wb);
     // Cholesky S = L L^T, right-looking: lane j's a[k] is S[k, j] = S[j, k], so L[j, k] is local.
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
         }
     }
     if (!definite) {
-        if (j == 0) sd.flags[FLAG_METRIC] = 1;
+        if (j == 0) to_global(sd.flags)[FLAG_METRIC] = 1;
         return;
     }
     // X = L^-1 H, column j
@@ -261,21 +261,21 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
     }
     wave_sync();
     if (j < n) {
-        sd.coef[j] = c;
+        to_global(sd.coef)[j] = c;
         wv[j] = c;
     }
     if (j == 0) {
-        sd.energy[0] = lambda;
-        sd.energy[1] = lambda + sd.e_nuc;
+        to_global(sd.energy)[0] = lambda;
+        to_global(sd.energy)[1] = lambda + sd.e_nuc;
     }
     wave_sync();
-    const long long* __restrict__ off = topo.off;
-    const int* __restrict__ ts = topo.ts;
-    const double* __restrict__ tc = topo.tc;
+    const auto off = to_global(topo.off);
+    const auto ts = to_global(topo.ts);
+    const auto tc = to_global(topo.tc);
     for (int d = j; d < topo.n_det; d += 64) {
         double acc = 0.0;
         for (long long t = off[d]; t < off[d + 1]; ++t) acc += tc[t] * wv[ts[t]];
-        sd.dcoef[d] = acc;
+        to_global(sd.dcoef)[d] = acc;
     }
 }
 

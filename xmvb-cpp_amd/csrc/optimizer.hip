@@ -23,6 +23,7 @@
 // append (k_opt_append). The host reads scalars only: energies, the directional derivative, the
 // projected-gradient norm and the step size.
 
+#include "gmem.hpp"
 #include "optimizer.hpp"
 
 #include <algorithm>
@@ -42,7 +43,6 @@ namespace xmvb_amd {
 namespace {
 
 constexpr int kPointThreads = 256;
-constexpr int kBlockThreads = 1024;
 constexpr int kDirThreads = 256;  // two-loop block
 constexpr int kDirRegs = 8;       // elements per thread the two-loop keeps in registers (N <= 2048)
 constexpr int kOrbStats = 4;  // per orbital: dual norm, max |c|, norm before normalisation, c^T g
@@ -95,6 +95,26 @@ __device__ double block_total(double v, double* red) {
     return t;
 }
 
+// Barrier whose fences cover LDS only: outstanding global loads stay in flight across it (the
+// two-loop's prefetched pairs), where __syncthreads would wait for them.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// block_total with LDS-only barriers (the same sum, in the same order)
+__device__ double block_total_lds(double v, double* red) {
+    v = wave_total(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    lds_barrier();
+    if (lane == 0) red[wave] = v;
+    lds_barrier();
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += red[w];
+    return t;
+}
+
 __device__ double block_largest(double v, double* red) {
     v = wave_largest(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -127,6 +147,7 @@ struct OptDesc {
     double* T;                  // search direction
     double* q;                  // two-loop work vectors
     double* z;
+    double* apx;                // 6 per orbital: secant sums and max |s_i| (append kernels)
     double* stats;              // kOrbStats * n_act chart statistics
     double* out;                // kOutSlots direction / append scalars
     double* hs;                 // secant ring: steps, gradient changes, per-orbital statistics
@@ -236,12 +257,18 @@ __global__ void __launch_bounds__(256) k_opt_project(const OptDesc* __restrict__
 __global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __restrict__ descs, int n_act) {
     const OptDesc& o = descs[blockIdx.z];
     const int n_bf = o.n_bf, N = o.N, cap = o.cap, count = o.count, newest = o.newest;
-    const double* __restrict__ C = o.C;
-    const double* __restrict__ Sc = o.Sc;
-    const double* __restrict__ hs = o.hs;
-    const double* __restrict__ hy = o.hy;
-    double* __restrict__ q = o.q;
-    double* __restrict__ z = o.z;
+    const auto C = to_global(static_cast<const double*>(o.C));
+    const auto Sc = to_global(static_cast<const double*>(o.Sc));
+    const auto hs = to_global(static_cast<const double*>(o.hs));
+    const auto hy = to_global(static_cast<const double*>(o.hy));
+    const auto R = to_global(static_cast<const double*>(o.R));
+    const auto G = to_global(static_cast<const double*>(o.G));
+    const auto Sinv = to_global(o.Sinv);
+    const auto pat = to_global(o.pat);
+    const auto proj = to_global(static_cast<const double*>(o.proj));
+    const auto hstat = to_global(static_cast<const double*>(o.hstat));
+    const auto q = to_global(o.q);
+    const auto z = to_global(o.z);
     extern __shared__ double lds[];
     double* sig = lds;                          // (S c_i)^T s_a,i     count x n_act
     double* bet = sig + (size_t)count * n_act;  // c_i^T y_a,i         count x n_act
@@ -254,15 +281,15 @@ __global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __
     if (tid == 0) bad = 0;
     // projection coefficients of the stored pairs in this chart (k_opt_project)
     for (int t = tid; t < count * n_act; t += nt) {
-        sig[t] = o.proj[t];
-        bet[t] = o.proj[(size_t)cap * n_act + t];
+        sig[t] = proj[t];
+        bet[t] = proj[(size_t)cap * n_act + t];
     }
     __syncthreads();
     for (int a = tid; a < count; a += nt) {
         const int slot = (newest - a + cap) % cap;
         double ss = 0.0, yy = 0.0, sy = 0.0;
         for (int i = 0; i < n_act; ++i) {
-            const double* st = o.hstat + ((size_t)slot * n_act + i) * 3;
+            const auto st = hstat + ((size_t)slot * n_act + i) * 3;
             const double s_ = sig[a * n_act + i], b_ = bet[a * n_act + i];
             ss += st[0] - s_ * s_;
             yy += st[1] - b_ * b_;
@@ -278,65 +305,86 @@ __global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __
     int used = 0;
     double gamma = 1.0;
     if (N <= kDirRegs * nt) {
-        // register path: a thread's elements e = tid + m nt stay in registers (q, z, C, Sc), and the
-        // next pair's s and y are loaded while the current pair's block reduction runs; the
-        // arithmetic per element and its order are those of the memory path below
-        double qr[kDirRegs], cr[kDirRegs], scr[kDirRegs], sr[kDirRegs], yr[kDirRegs], sn[kDirRegs], yn[kDirRegs];
-        int orb[kDirRegs];
+        // register path: a thread's elements e = tid + m nt stay in registers (q, z, C, Sc); the
+        // pairs' s and y stream through a ring of three register slots, each refilled with the pair
+        // three ahead as soon as it is consumed, so the loads of a pair are in flight for three
+        // block reductions. The arithmetic per element and its order are those of the memory path.
+        // Loads are unconditional (indices past N clamped, those elements masked by selects) and a
+        // pair slot is always refilled: loads under branches make the compiler wait for every
+        // outstanding load (vmcnt(0)) at the next use, which would serialise the ring.
+        double qr[kDirRegs], cr[kDirRegs], scr[kDirRegs];
+        double s0[kDirRegs], y0[kDirRegs], s1[kDirRegs], y1[kDirRegs], s2[kDirRegs], y2[kDirRegs];
+        int orb[kDirRegs], ec[kDirRegs];
+        bool in[kDirRegs];
 #pragma unroll
         for (int m = 0; m < kDirRegs; ++m) {
             const int e = tid + m * nt;
-            const bool in = e < N;
-            qr[m] = in ? o.R[e] : 0.0;
-            cr[m] = in ? C[e] : 0.0;
-            scr[m] = in ? Sc[e] : 0.0;
-            orb[m] = in ? e / n_bf : 0;
+            in[m] = e < N;
+            ec[m] = in[m] ? e : N - 1;
+            qr[m] = in[m] ? R[ec[m]] : 0.0;
+            cr[m] = C[ec[m]];
+            scr[m] = Sc[ec[m]];
+            orb[m] = in[m] ? e / n_bf : 0;
         }
-        auto load_pair = [&](int a, double* sv, double* yv) {
+        auto load_pair = [&](int a, double (&sv)[kDirRegs], double (&yv)[kDirRegs]) {
             const int slot = (newest - a + cap) % cap;
-            const double* s = hs + (size_t)slot * N;
-            const double* y = hy + (size_t)slot * N;
+            const auto s = hs + (size_t)slot * N;
+            const auto y = hy + (size_t)slot * N;
 #pragma unroll
             for (int m = 0; m < kDirRegs; ++m) {
-                const int e = tid + m * nt;
-                sv[m] = e < N ? s[e] : 0.0;
-                yv[m] = e < N ? y[e] : 0.0;
+                sv[m] = s[ec[m]];
+                yv[m] = y[ec[m]];
             }
         };
-        // pairs in use, newest to oldest
-        int first = 0;
-        while (first < count && rho[first] == 0.0) ++first;
-        if (first < count) load_pair(first, sn, yn);
-        bool have_newest = false;
-        for (int a = first; a < count;) {
-#pragma unroll
-            for (int m = 0; m < kDirRegs; ++m) {
-                sr[m] = sn[m];
-                yr[m] = yn[m];
-            }
-            int next = a + 1;
-            while (next < count && rho[next] == 0.0) ++next;
-            if (next < count) load_pair(next, sn, yn);
-            if (!have_newest) {
-                gamma = gam[a];
-                have_newest = true;
-            }
+        // pairs in use (rho != 0) by age: next older / next newer after a (count / -1 at the end)
+        auto older = [&](int a) {
+            int b = a + 1;
+            while (b < count && rho[b] == 0.0) ++b;
+            return b < count ? b : count;
+        };
+        auto newer = [&](int a) {
+            int b = a - 1;
+            while (b >= 0 && rho[b] == 0.0) --b;
+            return b;
+        };
+        // first loop, newest to oldest (a slot past the last pair is refilled with the current one)
+        int cur = older(-1), ahead = cur;
+        if (cur < count) {
+            gamma = gam[cur];
+            load_pair(cur, s0, y0);
+            ahead = older(ahead);
+            load_pair(ahead < count ? ahead : cur, s1, y1);
+            ahead = ahead < count ? older(ahead) : count;
+            load_pair(ahead < count ? ahead : cur, s2, y2);
+        }
+        auto first_step = [&](double (&sr)[kDirRegs], double (&yr)[kDirRegs]) {
+            const int a = cur;
             ++used;
             double part = 0.0;
 #pragma unroll
-            for (int m = 0; m < kDirRegs; ++m)
-                if (tid + m * nt < N) part += (sr[m] - cr[m] * sig[a * n_act + orb[m]]) * qr[m];
-            const double alpha = rho[a] * block_total(part, red);
+            for (int m = 0; m < kDirRegs; ++m) {
+                const double t = (sr[m] - cr[m] * sig[a * n_act + orb[m]]) * qr[m];
+                part += in[m] ? t : 0.0;
+            }
+            const double alpha = rho[a] * block_total_lds(part, red);
             if (!isfinite(alpha) && tid == 0) bad = 1;
             if (tid == 0) alp[a] = alpha;
 #pragma unroll
-            for (int m = 0; m < kDirRegs; ++m)
-                if (tid + m * nt < N) qr[m] -= alpha * (yr[m] - scr[m] * bet[a * n_act + orb[m]]);
-            a = next;
+            for (int m = 0; m < kDirRegs; ++m) qr[m] -= alpha * (yr[m] - scr[m] * bet[a * n_act + orb[m]]);
+            ahead = ahead < count ? older(ahead) : count;
+            load_pair(ahead < count ? ahead : a, sr, yr);
+            cur = older(cur);
+        };
+        while (cur < count) {
+            first_step(s0, y0);
+            if (cur >= count) break;
+            first_step(s1, y1);
+            if (cur >= count) break;
+            first_step(s2, y2);
         }
 #pragma unroll
         for (int m = 0; m < kDirRegs; ++m)
-            if (tid + m * nt < N) q[tid + m * nt] = qr[m];
+            if (in[m]) q[tid + m * nt] = qr[m];
         __syncthreads();
         double zr[kDirRegs];
 #pragma unroll
@@ -345,44 +393,53 @@ __global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __
             zr[m] = 0.0;
             if (e < N) {
                 const int i = orb[m], k = e - i * n_bf;
-                const double* P = o.Sinv + (size_t)o.pat[i] * n_bf * n_bf;
-                const double* qi = q + (size_t)i * n_bf;
+                const auto P = Sinv + (size_t)pat[i] * n_bf * n_bf;
+                const auto qi = q + (size_t)i * n_bf;
                 double acc = 0.0;
                 for (int j = 0; j < n_bf; ++j) acc += P[k + (size_t)n_bf * j] * qi[j];
                 zr[m] = gamma * acc;
             }
         }
-        // pairs in use, oldest to newest
-        int last = count - 1;
-        while (last >= 0 && rho[last] == 0.0) --last;
-        if (last >= 0) load_pair(last, sn, yn);
-        for (int a = last; a >= 0;) {
-#pragma unroll
-            for (int m = 0; m < kDirRegs; ++m) {
-                sr[m] = sn[m];
-                yr[m] = yn[m];
-            }
-            int next = a - 1;
-            while (next >= 0 && rho[next] == 0.0) --next;
-            if (next >= 0) load_pair(next, sn, yn);
+        // second loop, oldest to newest
+        cur = newer(count);
+        ahead = cur;
+        if (cur >= 0) {
+            load_pair(cur, s0, y0);
+            ahead = newer(ahead);
+            load_pair(ahead >= 0 ? ahead : cur, s1, y1);
+            ahead = ahead >= 0 ? newer(ahead) : -1;
+            load_pair(ahead >= 0 ? ahead : cur, s2, y2);
+        }
+        auto second_step = [&](double (&sr)[kDirRegs], double (&yr)[kDirRegs]) {
+            const int a = cur;
             double part = 0.0;
 #pragma unroll
-            for (int m = 0; m < kDirRegs; ++m)
-                if (tid + m * nt < N) part += (yr[m] - scr[m] * bet[a * n_act + orb[m]]) * zr[m];
-            const double b = rho[a] * block_total(part, red);
-            if (!isfinite(b) && tid == 0) bad = 1;
-            const double coef = alp[a] - b;
+            for (int m = 0; m < kDirRegs; ++m) {
+                const double t = (yr[m] - scr[m] * bet[a * n_act + orb[m]]) * zr[m];
+                part += in[m] ? t : 0.0;
+            }
+            const double bb = rho[a] * block_total_lds(part, red);
+            if (!isfinite(bb) && tid == 0) bad = 1;
+            const double coef = alp[a] - bb;
 #pragma unroll
-            for (int m = 0; m < kDirRegs; ++m)
-                if (tid + m * nt < N) zr[m] += (sr[m] - cr[m] * sig[a * n_act + orb[m]]) * coef;
-            a = next;
+            for (int m = 0; m < kDirRegs; ++m) zr[m] += (sr[m] - cr[m] * sig[a * n_act + orb[m]]) * coef;
+            ahead = ahead >= 0 ? newer(ahead) : -1;
+            load_pair(ahead >= 0 ? ahead : a, sr, yr);
+            cur = newer(cur);
+        };
+        while (cur >= 0) {
+            second_step(s0, y0);
+            if (cur < 0) break;
+            second_step(s1, y1);
+            if (cur < 0) break;
+            second_step(s2, y2);
         }
 #pragma unroll
         for (int m = 0; m < kDirRegs; ++m)
-            if (tid + m * nt < N) z[tid + m * nt] = zr[m];
+            if (in[m]) z[tid + m * nt] = zr[m];
         __syncthreads();
     } else {
-    for (int e = tid; e < N; e += nt) q[e] = o.R[e];
+    for (int e = tid; e < N; e += nt) q[e] = R[e];
     __syncthreads();
     bool have_newest = false;
     for (int a = 0; a < count; ++a) {  // newest to oldest
@@ -393,8 +450,8 @@ __global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __
         }
         ++used;
         const int slot = (newest - a + cap) % cap;
-        const double* s = hs + (size_t)slot * N;
-        const double* y = hy + (size_t)slot * N;
+        const auto s = hs + (size_t)slot * N;
+        const auto y = hy + (size_t)slot * N;
         double part = 0.0;
         for (int e = tid; e < N; e += nt) part += (s[e] - C[e] * sig[a * n_act + e / n_bf]) * q[e];
         const double alpha = rho[a] * block_total(part, red);
@@ -405,8 +462,8 @@ __global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __
     __syncthreads();
     for (int e = tid; e < N; e += nt) {  // z = gamma S_i^-1 q_i
         const int i = e / n_bf, k = e - i * n_bf;
-        const double* P = o.Sinv + (size_t)o.pat[i] * n_bf * n_bf;
-        const double* qi = q + (size_t)i * n_bf;
+        const auto P = Sinv + (size_t)pat[i] * n_bf * n_bf;
+        const auto qi = q + (size_t)i * n_bf;
         double acc = 0.0;
         for (int j = 0; j < n_bf; ++j) acc += P[k + (size_t)n_bf * j] * qi[j];
         z[e] = gamma * acc;
@@ -415,8 +472,8 @@ __global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __
     for (int a = count - 1; a >= 0; --a) {  // oldest to newest
         if (rho[a] == 0.0) continue;
         const int slot = (newest - a + cap) % cap;
-        const double* s = hs + (size_t)slot * N;
-        const double* y = hy + (size_t)slot * N;
+        const auto s = hs + (size_t)slot * N;
+        const auto y = hy + (size_t)slot * N;
         double part = 0.0;
         for (int e = tid; e < N; e += nt) part += (y[e] - Sc[e] * bet[a * n_act + e / n_bf]) * z[e];
         const double b = rho[a] * block_total(part, red);
@@ -429,18 +486,19 @@ __global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __
     double dd = 0.0, tinf = 0.0;
     for (int e = tid; e < N; e += nt) {
         const double t = -z[e];
-        o.T[e] = t;
-        dd += o.G[e] * t;
+        to_global(o.T)[e] = t;
+        dd += G[e] * t;
         tinf = fmax(tinf, fabs(t));
     }
     dd = block_total(dd, red);
     tinf = block_largest(tinf, red);
     if (tid == 0) {
-        o.out[0] = dd;
-        o.out[1] = tinf;
-        o.out[2] = used;
-        o.out[3] = gamma;
-        o.out[4] = bad;
+        const auto out = to_global(o.out);
+        out[0] = dd;
+        out[1] = tinf;
+        out[2] = used;
+        out[3] = gamma;
+        out[4] = bad;
     }
 }
 
@@ -454,72 +512,82 @@ __global__ void k_opt_trial(const OptDesc* __restrict__ descs) {
 // Candidate secant pair (append_projected_secant_pair, transported_lbfgs.cpp:147-176) between the
 // previous and the new accepted point (both normalised): s = C - Cp, y = G - Gp are written to
 // the ring slot with their per-orbital chart-independent statistics s^T S s, y^T S^-1 y, s^T y;
-// the curvature test runs in the new chart. out[5] = 1 if accepted, out[6] = max |s|.
-__global__ void __launch_bounds__(kBlockThreads) k_opt_append(const OptDesc* __restrict__ descs, int n_act) {
+// the curvature test runs in the new chart. One wavefront per orbital (grid x = orbital, z =
+// problem); the per-orbital sums and max |s_i| go to apx (6 per orbital) for k_opt_append_finish,
+// which forms out[5] = 1 if accepted and out[6] = max |s|.
+__global__ void __launch_bounds__(64) k_opt_append(const OptDesc* __restrict__ descs) {
     const OptDesc& o = descs[blockIdx.z];
-    const int n_bf = o.n_bf, N = o.N;
-    double* __restrict__ hs = o.hs + (size_t)o.slot * N;
-    double* __restrict__ hy = o.hy + (size_t)o.slot * N;
-    double* __restrict__ hstat = o.hstat + (size_t)o.slot * n_act * 3;
-    __shared__ double red[16];
-    __shared__ double orb[16 * 5];
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    const int n_bf = o.n_bf, N = o.N, i = blockIdx.x, lane = threadIdx.x;
+    const size_t row = (size_t)i * n_bf;
+    double* __restrict__ s = o.hs + (size_t)o.slot * N + row;
+    double* __restrict__ y = o.hy + (size_t)o.slot * N + row;
+    double* __restrict__ hstat = o.hstat + (size_t)o.slot * (N / n_bf) * 3;
+    extern __shared__ double lds[];
+    double* sl = lds;         // s_i, n_bf
+    double* yl = lds + n_bf;  // y_i, n_bf
     double sinf = 0.0;
-    for (int e = tid; e < N; e += nt) {
-        const double s = o.C[e] - o.Cp[e];
-        hs[e] = s;
-        hy[e] = o.G[e] - o.Gp[e];
-        sinf = fmax(sinf, fabs(s));
+    for (int k = lane; k < n_bf; k += 64) {
+        const double sk = o.C[row + k] - o.Cp[row + k], yk = o.G[row + k] - o.Gp[row + k];
+        s[k] = sk;
+        y[k] = yk;
+        sl[k] = sk;
+        yl[k] = yk;
+        sinf = fmax(sinf, fabs(sk));
     }
-    sinf = block_largest(sinf, red);  // its barriers also publish hs / hy to the block
-    for (int i = wave; i < n_act; i += nw) {
-        const double* s = hs + (size_t)i * n_bf;
-        const double* y = hy + (size_t)i * n_bf;
-        const double* P = o.Sinv + (size_t)o.pat[i] * n_bf * n_bf;
-        double sss = 0.0, ysy = 0.0, sy = 0.0, sg = 0.0, bt = 0.0;
-        for (int k = lane; k < n_bf; k += 64) {
-            double a = 0.0, b = 0.0;
-            for (int j = 0; j < n_bf; ++j) {
-                a += o.S[k + (size_t)n_bf * j] * s[j];
-                b += P[k + (size_t)n_bf * j] * y[j];
-            }
-            sss += s[k] * a;
-            ysy += y[k] * b;
-            sy += s[k] * y[k];
-            sg += o.Sc[(size_t)i * n_bf + k] * s[k];
-            bt += o.C[(size_t)i * n_bf + k] * y[k];
+    sinf = wave_largest(sinf);
+    __syncthreads();  // publishes sl and yl to the wavefront
+    const double* __restrict__ P = o.Sinv + (size_t)o.pat[i] * n_bf * n_bf;
+    const double* __restrict__ S = o.S;
+    double sss = 0.0, ysy = 0.0, sy = 0.0, sg = 0.0, bt = 0.0;
+    for (int k = lane; k < n_bf; k += 64) {
+        double a = 0.0, b = 0.0;
+        // the operands from LDS: the S and S^-1 loads of several j are in flight at once
+#pragma unroll 8
+        for (int j = 0; j < n_bf; ++j) {
+            a += S[k + (size_t)n_bf * j] * sl[j];
+            b += P[k + (size_t)n_bf * j] * yl[j];
         }
-        sss = wave_total(sss);
-        ysy = wave_total(ysy);
-        sy = wave_total(sy);
-        sg = wave_total(sg);
-        bt = wave_total(bt);
-        if (lane == 0) {
-            hstat[i * 3 + 0] = sss;
-            hstat[i * 3 + 1] = ysy;
-            hstat[i * 3 + 2] = sy;
-            orb[i * 5 + 0] = sss;
-            orb[i * 5 + 1] = ysy;
-            orb[i * 5 + 2] = sy;
-            orb[i * 5 + 3] = sg;
-            orb[i * 5 + 4] = bt;
-        }
+        sss += sl[k] * a;
+        ysy += yl[k] * b;
+        sy += sl[k] * yl[k];
+        sg += o.Sc[row + k] * sl[k];
+        bt += o.C[row + k] * yl[k];
     }
-    __syncthreads();
-    if (tid == 0) {
-        double ss = 0.0, yy = 0.0, sy = 0.0;
-        for (int i = 0; i < n_act; ++i) {
-            const double* w = orb + i * 5;
-            ss += w[0] - w[3] * w[3];
-            yy += w[1] - w[4] * w[4];
-            sy += w[2] - w[3] * w[4];
-        }
-        const double sn = sqrt(fmax(ss, 0.0)), yn = sqrt(fmax(yy, 0.0));
-        const bool ok = sn > 0.0 && yn > 0.0 && isfinite(sn) && isfinite(yn) && isfinite(sy) &&
-                        sy > 1.4901161193847656e-08 * sn * yn;
-        o.out[5] = ok ? 1.0 : 0.0;
-        o.out[6] = sinf;
+    sss = wave_total(sss);
+    ysy = wave_total(ysy);
+    sy = wave_total(sy);
+    sg = wave_total(sg);
+    bt = wave_total(bt);
+    if (lane == 0) {
+        hstat[i * 3 + 0] = sss;
+        hstat[i * 3 + 1] = ysy;
+        hstat[i * 3 + 2] = sy;
+        double* w = o.apx + (size_t)i * 6;
+        w[0] = sss;
+        w[1] = ysy;
+        w[2] = sy;
+        w[3] = sg;
+        w[4] = bt;
+        w[5] = sinf;
     }
+}
+
+__global__ void __launch_bounds__(64) k_opt_append_finish(const OptDesc* __restrict__ descs, int n_act) {
+    const OptDesc& o = descs[blockIdx.z];
+    if (threadIdx.x != 0) return;
+    double ss = 0.0, yy = 0.0, sy = 0.0, sinf = 0.0;
+    for (int i = 0; i < n_act; ++i) {
+        const double* w = o.apx + (size_t)i * 6;
+        ss += w[0] - w[3] * w[3];
+        yy += w[1] - w[4] * w[4];
+        sy += w[2] - w[3] * w[4];
+        sinf = fmax(sinf, w[5]);
+    }
+    const double sn = sqrt(fmax(ss, 0.0)), yn = sqrt(fmax(yy, 0.0));
+    const bool ok = sn > 0.0 && yn > 0.0 && isfinite(sn) && isfinite(yn) && isfinite(sy) &&
+                    sy > 1.4901161193847656e-08 * sn * yn;
+    o.out[5] = ok ? 1.0 : 0.0;
+    o.out[6] = sinf;
 }
 
 // Support-restricted inverse overlap of one support pattern, zero-padded to n_bf x n_bf
@@ -586,6 +654,7 @@ struct Problem {
     std::vector<std::vector<int>> patterns;
     // device state
     double *S = nullptr, *Sinv = nullptr, *Sc = nullptr, *R = nullptr, *T = nullptr, *q = nullptr, *z = nullptr;
+    double* apx = nullptr;  // per-orbital secant sums between the two append kernels
     double *stats = nullptr, *out = nullptr, *hs = nullptr, *hy = nullptr, *hstat = nullptr, *proj = nullptr;
     int* d_pat = nullptr;
     uint8_t* d_support = nullptr;
@@ -675,7 +744,8 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
                  DeviceBlock::bytes_for(n_act, 4) + (s.p->support ? DeviceBlock::bytes_for(s.N, 1) : 0) +
                  11 * DeviceBlock::bytes_for(s.N, 8) + DeviceBlock::bytes_for((size_t)kOrbStats * n_act, 8) +
                  DeviceBlock::bytes_for(kOutSlots, 8) + 2 * DeviceBlock::bytes_for((size_t)cap * s.N, 8) +
-                 DeviceBlock::bytes_for((size_t)cap * n_act * 3, 8) + DeviceBlock::bytes_for((size_t)cap * n_act * 2, 8);
+                 DeviceBlock::bytes_for((size_t)cap * n_act * 3, 8) + DeviceBlock::bytes_for((size_t)cap * n_act * 2, 8) +
+                 DeviceBlock::bytes_for((size_t)6 * n_act, 8);
     }
     {  // the inverses, spread over host threads
         std::vector<std::pair<int, int>> jobs;
@@ -740,6 +810,7 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
         s.T = dev.take<double>(s.N);
         s.q = dev.take<double>(s.N);
         s.z = dev.take<double>(s.N);
+        s.apx = dev.take<double>((size_t)6 * n_act);
         s.stats = d_scal + scal_per * k;
         s.out = s.stats + (size_t)kOrbStats * n_act;
         s.h_stats = h_scal + scal_per * k;
@@ -786,6 +857,7 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
             d.R = s.R;
             d.T = s.T;
             d.q = s.q;
+            d.apx = s.apx;
             d.z = s.z;
             d.stats = s.stats;
             d.out = s.out;
@@ -814,7 +886,9 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
                            sizeof(double) * 2 * max_n_bf, st, d_desc);
         hip_check(hipGetLastError(), "k_opt_point");
         if (append) {
-            hipLaunchKernelGGL(k_opt_append, dim3(1, 1, (unsigned)list.size()), dim3(kBlockThreads), 0, st, d_desc,
+            hipLaunchKernelGGL(k_opt_append, dim3(n_act, 1, (unsigned)list.size()), dim3(64),
+                               sizeof(double) * 2 * max_n_bf, st, d_desc);
+            hipLaunchKernelGGL(k_opt_append_finish, dim3(1, 1, (unsigned)list.size()), dim3(64), 0, st, d_desc,
                                n_act);
             hip_check(hipGetLastError(), "k_opt_append");
         }

@@ -70,8 +70,9 @@ struct Stage {
                 float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
                 int lo = 0;
                 if (ok) {
-                    h = *reinterpret_cast<const float4*>(sd.B32 + at);
-                    lo = *reinterpret_cast<const int*>(sd.Blo + at);
+                    const f32x4 hv = *reinterpret_cast<gptr<const f32x4>>(to_global(sd.B32 + at));
+                    h = make_float4(hv[0], hv[1], hv[2], hv[3]);
+                    lo = *reinterpret_cast<gptr<const int>>(to_global(sd.Blo + at));
                 }
                 v[g][0] = decode_fp32c(h.x, (int)(signed char)(lo & 0xff));
                 v[g][1] = decode_fp32c(h.y, (int)(signed char)((lo >> 8) & 0xff));
@@ -80,9 +81,10 @@ struct Stage {
             } else {
                 double2 p0 = make_double2(0.0, 0.0), p1 = make_double2(0.0, 0.0);
                 if (ok) {
-                    const double2* src = reinterpret_cast<const double2*>(sd.B + at);
-                    p0 = src[0];
-                    p1 = src[1];
+                    const gptr<const f64x2> src = reinterpret_cast<gptr<const f64x2>>(to_global(sd.B + at));
+                    const f64x2 a0 = src[0], a1 = src[1];
+                    p0 = make_double2(a0[0], a0[1]);
+                    p1 = make_double2(a1[0], a1[1]);
                 }
                 v[g][0] = p0.x;
                 v[g][1] = p0.y;
@@ -193,7 +195,7 @@ __global__ void __launch_bounds__(kThreads) k_ri_one_index_tiled(const SlotDesc*
         if (mu >= n_bf) break;
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
-            if (q < n) sd.T[(size_t)A + (size_t)n_aux * (q + (size_t)n * mu)] = acc[k][q];
+            if (q < n) to_global(sd.T)[(size_t)A + (size_t)n_aux * (q + (size_t)n * mu)] = acc[k][q];
     }
 }
 
