@@ -29,6 +29,7 @@
 // the current Ritz vector and the previous one (the reference keeps the 16 lowest Ritz vectors);
 // this changes iteration counts, not the stopping test. The result c = L^-T y satisfies c^T S c = 1.
 
+#include "gmem.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -377,7 +378,7 @@ k_davidson_factored(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonP
     const int n_initial = sd.seeded ? 1 : (ms < 2 ? ms : 2);
     for (int c = 0; c < n_initial; ++c) {
         if (sd.seeded) {
-            for (int i = tid; i < n; i += nt) prv[i] = sd.coef[i];
+            for (int i = tid; i < n; i += nt) prv[i] = to_global(sd.coef)[i];
         } else if (c == 0) {
             // argmin_j H_jj / S_jj (ties: lowest index), by the first wavefront
             if (tid < 64) {
@@ -414,7 +415,7 @@ k_davidson_factored(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonP
         append();
     }
     if (state[0] < 1) {
-        if (tid == 0) sd.flags[FLAG_EIGEN] = 1;  // initial vectors dependent
+        if (tid == 0) to_global(sd.flags)[FLAG_EIGEN] = 1;  // initial vectors dependent
         return;
     }
     update_projected(0);
@@ -456,22 +457,22 @@ k_davidson_factored(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonP
             for (int i = tid; i < n; i += nt) {
                 double a = 0.0;
                 for (int k = i; k < n; ++k) a += Li[k + (size_t)n * i] * root[k];
-                sd.coef[i] = a * inv;
+                to_global(sd.coef)[i] = a * inv;
             }
             if (tid == 0) {
-                sd.energy[0] = lambda;
-                sd.energy[1] = lambda + sd.e_nuc;
+                to_global(sd.energy)[0] = lambda;
+                to_global(sd.energy)[1] = lambda + sd.e_nuc;
             }
             __syncthreads();
             for (int d = tid; d < topo.n_det; d += nt) {
                 double acc = 0.0;
-                for (long long q = topo.off[d]; q < topo.off[d + 1]; ++q) acc += topo.tc[q] * sd.coef[topo.ts[q]];
-                sd.dcoef[d] = acc;
+                for (long long q = topo.off[d]; q < topo.off[d + 1]; ++q) acc += topo.tc[q] * to_global(sd.coef)[topo.ts[q]];
+                to_global(sd.dcoef)[d] = acc;
             }
             return;
         }
         if (active == n) {
-            if (tid == 0) sd.flags[FLAG_EIGEN] = 2;  // complete basis cannot meet the residual
+            if (tid == 0) to_global(sd.flags)[FLAG_EIGEN] = 2;  // complete basis cannot meet the residual
             return;
         }
         lam_prev = lambda;
@@ -540,7 +541,7 @@ k_davidson_factored(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonP
         action();
         const int before = state[0];
         if (!append()) {
-            if (tid == 0) sd.flags[FLAG_EIGEN] = 3;  // correction space became linearly dependent
+            if (tid == 0) to_global(sd.flags)[FLAG_EIGEN] = 3;  // correction space became linearly dependent
             return;
         }
         update_projected(before);
@@ -550,7 +551,7 @@ k_davidson_factored(const SlotDesc* __restrict__ slots, TopoDesc topo, DavidsonP
             __syncthreads();
         }
     }
-    if (tid == 0) sd.flags[FLAG_EIGEN] = 4;  // iteration limit
+    if (tid == 0) to_global(sd.flags)[FLAG_EIGEN] = 4;  // iteration limit
 }
 
 }  // namespace

@@ -30,17 +30,18 @@ __device__ __forceinline__ void unpack_pair(long long t, int& d, int& e) {
 // wH H_de + wS S_de. One state: wH = a_d a_e mult, wS = -lambda wH.
 __device__ __forceinline__ void pair_weights(const SlotDesc& sd, const TopoDesc& topo, int d, int e, int fd, int fe,
                                              bool flip, double mult, double& wH, double& wS) {
+    const auto dc = to_global(sd.dcoef);
     if (topo.n_states <= 1) {
-        wH = (sd.dcoef[d] * sd.dcoef[e] + (flip ? sd.dcoef[fd] * sd.dcoef[fe] : 0.0)) * mult;
-        wS = -sd.energy[0] * wH;
+        wH = (dc[d] * dc[e] + (flip ? dc[fd] * dc[fe] : 0.0)) * mult;
+        wS = -to_global(sd.energy)[0] * wH;
         return;
     }
     double h = 0.0, s = 0.0;
     for (int k = 0; k < topo.n_states; ++k) {
-        const double* a = sd.dcoef + (size_t)k * topo.n_det;
+        const auto a = dc + (size_t)k * topo.n_det;
         const double c = topo.state_w[k] * ((a[d] * a[e] + (flip ? a[fd] * a[fe] : 0.0)) * mult);
         h += c;
-        s -= sd.state_energy[k] * c;
+        s -= to_global(sd.state_energy)[k] * c;
     }
     wH = h;
     wS = s;

@@ -48,16 +48,16 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // ---------------------------------------------------------------------------------------------
 __global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ C = sd.C;
-    const uint8_t* __restrict__ support = sd.support;
-    const double* __restrict__ S = sd.S;
-    const double* __restrict__ h = sd.h;
+    const auto C = to_global(sd.C);
+    const auto support = to_global(sd.support);
+    const auto S = to_global(sd.S);
+    const auto h = to_global(sd.h);
     const int n_bf = sd.n_bf;
-    double* __restrict__ Cn = sd.Cn;
-    double* __restrict__ SC = sd.SC;
-    double* __restrict__ HC = sd.HC;
-    double* __restrict__ nu2 = sd.nu2;
-    int* __restrict__ flags = sd.flags;
+    const auto Cn = to_global(sd.Cn);
+    const auto SC = to_global(sd.SC);
+    const auto HC = to_global(sd.HC);
+    const auto nu2 = to_global(sd.nu2);
+    const auto flags = to_global(sd.flags);
     extern __shared__ double lds[];
     double* c = lds;             // n_bf
     double* red = lds + n_bf;    // 16
@@ -91,8 +91,8 @@ __global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
     q = block_sum(q, red);
     const bool ok = isfinite(q) && q > 2.220446049250313e-16;
     const double f = ok ? 1.0 / sqrt(q) : 0.0;
-    if (!ok && threadIdx.x == 0) atomicMax(&flags[FLAG_NORMALIZATION], k + 1);
-    double* __restrict__ CnT = sd.CnT;
+    if (!ok && threadIdx.x == 0) atomicMax((int*)&flags[FLAG_NORMALIZATION], k + 1);
+    const auto CnT = to_global(sd.CnT);
     for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
         Cn[(size_t)k * n_bf + mu] = c[mu] * f;
         CnT[(size_t)mu * kCnStride + k] = c[mu] * f;
@@ -112,13 +112,13 @@ __global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
 // S_act and raw h_act (n x n), symmetrised; thread 0 checks that S_act is positive definite.
 __global__ void k_active_matrices(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ Cn = sd.Cn;
-    const double* __restrict__ SC = sd.SC;
-    const double* __restrict__ HC = sd.HC;
+    const auto Cn = to_global(sd.Cn);
+    const auto SC = to_global(sd.SC);
+    const auto HC = to_global(sd.HC);
     const int n_bf = sd.n_bf;
-    double* __restrict__ sact = sd.sact;
-    double* __restrict__ hact = sd.hact;
-    int* __restrict__ flags = sd.flags;
+    const auto sact = to_global(sd.sact);
+    const auto hact = to_global(sd.hact);
+    const auto flags = to_global(sd.flags);
     __shared__ double s_l[MAX_ACT * MAX_ACT], h_l[MAX_ACT * MAX_ACT];
     {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -191,15 +191,15 @@ struct RiFp32c {};
 template <typename BT>
 __device__ __forceinline__ double flat_b(const SlotDesc& sd, size_t at) {
     if constexpr (std::is_same_v<BT, double>) {
-        return sd.B[at];
+        return to_global(sd.B)[at];
     } else if constexpr (std::is_same_v<BT, float>) {
-        return (double)sd.B32[at];
+        return (double)to_global(sd.B32)[at];
     } else {
-        const float hi = sd.B32[at];
+        const float hi = to_global(sd.B32)[at];
         unsigned e = (__float_as_uint(hi) >> 23) & 0xffu;
         e = e ? e : 1u;
         const double scale = __hiloint2double((int)((e + 1023u - 158u) << 20), 0);
-        return fma((double)sd.Blo[at], scale, (double)hi);
+        return fma((double)to_global(sd.Blo)[at], scale, (double)hi);
     }
 }
 
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
     if (sd.b_tiled) return;  // read by k_ri_one_index_tiled
     if ((int)blockIdx.y * 4 * kOneIndexRows >= n_bf || (int)blockIdx.x * 64 >= n_aux) return;  // block-uniform
-    const double* __restrict__ Cn = sd.Cn;
+    const auto Cn = to_global(sd.Cn);
     __shared__ double cT[kOneIndexChunk][16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int A = blockIdx.x * 64 + lane;
@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(256) k_ri_one_index(const SlotDesc* __restrict
         if (mu0 + u >= n_bf) break;
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
-            if (q < n) sd.T[(size_t)A + (size_t)n_aux * (q + (size_t)n * (mu0 + u))] = acc[u][q];
+            if (q < n) to_global(sd.T)[(size_t)A + (size_t)n_aux * (q + (size_t)n * (mu0 + u))] = acc[u][q];
     }
 }
 
@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(256) k_ri_pair_factors(const SlotDesc* __restr
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
     if ((int)blockIdx.x * 64 >= n_aux) return;  // block-uniform
-    const double* __restrict__ Cn = sd.Cn;
+    const auto Cn = to_global(sd.Cn);
     __shared__ double cT[kOneIndexChunk][16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int A = blockIdx.x * 64 + lane;
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(256) k_ri_pair_factors(const SlotDesc* __restr
     const int q = blockIdx.y * 4 + wave;
     const int qc = q < n ? q : n - 1;
     const size_t step = (size_t)n_aux * n;
-    const double* __restrict__ t = sd.T + (size_t)Ac + (size_t)n_aux * qc;
+    const auto t = to_global(sd.T) + (size_t)Ac + (size_t)n_aux * qc;
     double acc[NQ];
 #pragma unroll
     for (int p = 0; p < NQ; ++p) acc[p] = 0.0;
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(256) k_ri_pair_factors(const SlotDesc* __restr
         }
     }
     if (q >= n || A >= n_aux) return;
-    double* __restrict__ L = sd.L;
+    const auto L = to_global(sd.L);
 #pragma unroll
     for (int p = 0; p < NQ; ++p)
         if (p >= q && p < n) L[(size_t)A + (size_t)n_aux * (p * (p + 1) / 2 + q)] = acc[p];
@@ -324,8 +324,8 @@ __global__ void __launch_bounds__(256) k_ri_pair_factors(const SlotDesc* __restr
 __global__ void k_ri_pair_gram(const SlotDesc* __restrict__ slots, int np) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_aux = sd.n_aux;
-    const double* __restrict__ L = sd.L;
-    double* __restrict__ G = sd.G;
+    const auto L = to_global(sd.L);
+    const auto G = to_global(sd.G);
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int ng = np * (np + 1) / 2;
@@ -351,7 +351,7 @@ constexpr unsigned kGramFewProblems = 16;                 // up to this many pro
 __global__ void __launch_bounds__(256) k_ri_pair_gram_tiled(const SlotDesc* __restrict__ slots, int np) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_aux = sd.n_aux;
-    const double* __restrict__ L = sd.L;
+    const auto L = to_global(sd.L);
     __shared__ double Ls[kGramChunk * kGramLd];
     const int nt = (np + 1) / 2, ntp = nt * (nt + 1) / 2;
     int p0[kGramTilesPerThread], q0[kGramTilesPerThread];
@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(256) k_ri_pair_gram_tiled(const SlotDesc* __re
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int P = p0[k] + (j >> 1), Q = q0[k] + (j & 1);
-            if (P < np && Q <= P) sd.G[pair_index(P, Q)] = acc[k][j];
+            if (P < np && Q <= P) to_global(sd.G)[pair_index(P, Q)] = acc[k][j];
         }
     }
 }
@@ -543,14 +543,14 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
     constexpr int MB = GroupShape<GROUP>::MB;
     constexpr int GPW = 64 / GROUP;  // groups per wavefront
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ G_glob = sd.G;
+    const auto G_glob = to_global(sd.G);
     const int n = topo.n, ng = topo.ng, n_det = topo.n_det;
     const int np = n * (n + 1) / 2;
     const uint64_t* __restrict__ amask = topo.amask;
     const uint64_t* __restrict__ bmask = topo.bmask;
     const int* __restrict__ det_a = topo.det_a;
     const int* __restrict__ det_b = topo.det_b;
-    int* __restrict__ flags = sd.flags;
+    const auto flags = to_global(sd.flags);
     const long long n_pairs = Listed ? (long long)flags[FLAG_DEFERRED] : (long long)n_det * (n_det + 1) / 2;
     // a list block with no pair stays idle and writes no partial row (k_reduce_partials skips it)
     if (Listed && (long long)blockIdx.x * (blockDim.x / GROUP) >= n_pairs) return;
@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
     extern __shared__ double lds[];
     double* G2 = lds;
     double* sact = G2 + np * np;
-    double* hact = sact + n * n;
+    const auto hact = sact + n * n;
     double* tail = hact + n * n;
     double *dg = nullptr, *ds = nullptr, *dh = nullptr;
     if (Backward) {
@@ -573,8 +573,8 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
         for (int Q = wave; Q < np; Q += nwaves)
             for (int P = ln; P < np; P += 64) G2[P + np * Q] = G_glob[pair_index(P, Q)];
         for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-            sact[t] = sd.sact[t];
-            hact[t] = sd.hact[t];
+            sact[t] = to_global(sd.sact)[t];
+            hact[t] = to_global(sd.hact)[t];
         }
     }
     if (Backward) {
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
         bool live = ti < n_pairs;
         int d = 0, ee = 0;
         // unordered pair t = e(e+1)/2 + d, d <= e; d is the left (bra) determinant
-        const long long tp = live ? (Listed ? (long long)sd.defer[ti] : ti) : 0;
+        const long long tp = live ? (Listed ? (long long)to_global(sd.defer)[ti] : ti) : 0;
         if (live) unpack_pair(tp, d, ee);
         double wH = 0.0, wS = 0.0;
         if (Backward) {
@@ -646,7 +646,7 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
         if (!Backward && ti < n_pairs && lig == 0) {
             // rank-deficient under the complete-pivoting rule: evaluated by k_pairs_singular, listed in
             // entry order by k_compact_pairs
-            sd.singular_flag[ti] = live && singular ? 1 : 0;
+            to_global(sd.singular_flag)[ti] = live && singular ? 1 : 0;
         }
         live = live && !singular;
         wave_sync();
@@ -704,10 +704,10 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
         if (!Backward) {
             if (live && lig == 0) {
                 const double H = Dtot * phi;
-                sd.Hdet[(size_t)d + (size_t)n_det * ee] = H;
-                sd.Hdet[(size_t)ee + (size_t)n_det * d] = H;
-                sd.Sdet[(size_t)d + (size_t)n_det * ee] = Dtot;
-                sd.Sdet[(size_t)ee + (size_t)n_det * d] = Dtot;
+                to_global(sd.Hdet)[(size_t)d + (size_t)n_det * ee] = H;
+                to_global(sd.Hdet)[(size_t)ee + (size_t)n_det * d] = H;
+                to_global(sd.Sdet)[(size_t)d + (size_t)n_det * ee] = Dtot;
+                to_global(sd.Sdet)[(size_t)ee + (size_t)n_det * d] = Dtot;
             }
             wave_sync();
             continue;
@@ -810,12 +810,12 @@ __global__ void __launch_bounds__(512) k_pairs(const SlotDesc* __restrict__ slot
 __global__ void k_reduce_partials(const SlotDesc* __restrict__ slots, int n_blocks, int len, int list_begin,
                                   int list_rows, int list_per_row) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ partials = sd.partials;
-    double* __restrict__ out = sd.dsum;
+    const auto partials = to_global(sd.partials);
+    const auto out = to_global(sd.dsum);
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (i >= len) return;
-    const long long deferred = list_rows > 0 ? (long long)sd.flags[FLAG_DEFERRED] : 0;
+    const long long deferred = list_rows > 0 ? (long long)to_global(sd.flags)[FLAG_DEFERRED] : 0;
     double acc = 0.0;
     for (int b = lane; b < n_blocks; b += 64) {
         const int j = b - list_begin;
@@ -842,7 +842,7 @@ struct CompactView {
         flag = Deferred ? sd.defer_flag : sd.singular_flag;
         out = Deferred ? sd.defer : sd.singular;
         entries = Deferred ? topo.n_lane_pairs
-                  : listed ? (long long)sd.flags[FLAG_DEFERRED]
+                  : listed ? (long long)to_global(sd.flags)[FLAG_DEFERRED]
                            : (long long)topo.n_det * (topo.n_det + 1) / 2;
     }
     // pairs listed for entry i (0, 1 or 2)
@@ -863,7 +863,7 @@ struct CompactView {
             second = (int)tf;
             return tf != t ? 2 : 1;
         }
-        first = listed ? sd.defer[i] : (int)i;
+        first = listed ? to_global(sd.defer)[i] : (int)i;
         return 1;
     }
 };
@@ -898,7 +898,7 @@ __global__ void __launch_bounds__(kCompactChunk) k_compact_count(const SlotDesc*
         const int count = v.emit(sd, topo, listed, c * kCompactChunk + threadIdx.x, first, second);
         int bw, total;
         (void)compact_prefix(count, wave_total, bw, total);
-        if (threadIdx.x == 0) sd.chunk_count[c] = total;
+        if (threadIdx.x == 0) to_global(sd.chunk_count)[c] = total;
     }
 }
 
@@ -912,7 +912,7 @@ __global__ void __launch_bounds__(1024) k_compact_scan(const SlotDesc* __restric
     long long carry = 0;
     for (long long base = 0; base < chunks; base += blockDim.x) {
         const long long c = base + threadIdx.x;
-        const int x = c < chunks ? sd.chunk_count[c] : 0;
+        const int x = c < chunks ? to_global(sd.chunk_count)[c] : 0;
         part[threadIdx.x] = x;
         __syncthreads();
         // Hillis-Steele inclusive scan over the block
@@ -922,11 +922,11 @@ __global__ void __launch_bounds__(1024) k_compact_scan(const SlotDesc* __restric
             part[threadIdx.x] += y;
             __syncthreads();
         }
-        if (c < chunks) sd.chunk_count[c] = (int)(carry + part[threadIdx.x] - x);
+        if (c < chunks) to_global(sd.chunk_count)[c] = (int)(carry + part[threadIdx.x] - x);
         carry += part[blockDim.x - 1];
         __syncthreads();
     }
-    if (threadIdx.x == 0) sd.flags[Deferred ? FLAG_DEFERRED : FLAG_SINGULAR_PAIRS] = (int)carry;
+    if (threadIdx.x == 0) to_global(sd.flags)[Deferred ? FLAG_DEFERRED : FLAG_SINGULAR_PAIRS] = (int)carry;
 }
 
 template <bool Deferred>
@@ -941,7 +941,7 @@ __global__ void __launch_bounds__(kCompactChunk) k_compact_scatter(const SlotDes
         const int count = v.emit(sd, topo, listed, c * kCompactChunk + threadIdx.x, first, second);
         int bw, total;
         const int before = compact_prefix(count, wave_total, bw, total);
-        const long long at = (long long)sd.chunk_count[c] + bw + before;
+        const long long at = (long long)to_global(sd.chunk_count)[c] + bw + before;
         if (count >= 1) v.out[at] = first;
         if (count == 2) v.out[at + 1] = second;
     }
@@ -966,14 +966,14 @@ void compact_pairs(hipStream_t st, const SlotDesc* slots, unsigned z, const Topo
 // ---------------------------------------------------------------------------------------------
 __global__ void k_structure_right(const SlotDesc* __restrict__ slots, TopoDesc topo) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ Hdet = sd.Hdet;
-    const double* __restrict__ Sdet = sd.Sdet;
+    const auto Hdet = to_global(sd.Hdet);
+    const auto Sdet = to_global(sd.Sdet);
     const int n_det = topo.n_det, n_str = topo.n_str;
     const long long* __restrict__ soff = topo.soff;
     const int* __restrict__ sdet = topo.sdet;
     const double* __restrict__ scoef = topo.scoef;
-    double* __restrict__ Yh = sd.Yh;
-    double* __restrict__ Ys = sd.Ys;
+    const auto Yh = to_global(sd.Yh);
+    const auto Ys = to_global(sd.Ys);
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)n_det * n_str) return;
     const int d = (int)(id % n_det), J = (int)(id / n_det);
@@ -989,14 +989,14 @@ __global__ void k_structure_right(const SlotDesc* __restrict__ slots, TopoDesc t
 
 __global__ void k_structure_left(const SlotDesc* __restrict__ slots, TopoDesc topo) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ Yh = sd.Yh;
-    const double* __restrict__ Ys = sd.Ys;
+    const auto Yh = to_global(sd.Yh);
+    const auto Ys = to_global(sd.Ys);
     const int n_det = topo.n_det, n_str = topo.n_str;
     const long long* __restrict__ soff = topo.soff;
     const int* __restrict__ sdet = topo.sdet;
     const double* __restrict__ scoef = topo.scoef;
-    double* __restrict__ Hs = sd.Hs;
-    double* __restrict__ Ss = sd.Ss;
+    const auto Hs = to_global(sd.Hs);
+    const auto Ss = to_global(sd.Ss);
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)n_str * n_str) return;
     const int I = (int)(id % n_str), J = (int)(id / n_str);
@@ -1031,20 +1031,20 @@ __device__ __forceinline__ size_t gep_core_dev(int n) { return 3 * (size_t)n * n
 __global__ void __launch_bounds__(1024) k_gep_chol(const SlotDesc* __restrict__ slots, TopoDesc topo) {
     const SlotDesc& sd = slots[blockIdx.z];
     const int n = topo.n_str;
-    double* __restrict__ Lm = sd.gep;
-    double* __restrict__ M = Lm + (size_t)n * n;
+    const auto Lm = to_global(sd.gep);
+    const auto M = Lm + (size_t)n * n;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int TX = 16, TY = nt / TX, tx = tid % TX, ty = tid / TX;
     for (int idx = tid; idx < n * n; idx += nt) {
         const int i = idx % n, j = idx / n;
-        Lm[idx] = 0.5 * (sd.Ss[i + n * j] + sd.Ss[j + n * i]);
-        M[idx] = 0.5 * (sd.Hs[i + n * j] + sd.Hs[j + n * i]);
+        Lm[idx] = 0.5 * (sd.Ss[i + n * j] + to_global(sd.Ss)[j + n * i]);
+        M[idx] = 0.5 * (sd.Hs[i + n * j] + to_global(sd.Hs)[j + n * i]);
     }
     __syncthreads();
     for (int k = 0; k < n; ++k) {
         const double a = Lm[k + n * k];
         if (!(a > 0.0)) {
-            if (tid == 0) sd.flags[FLAG_METRIC] = 1;
+            if (tid == 0) to_global(sd.flags)[FLAG_METRIC] = 1;
             return;  // uniform; the later kernels run on meaningless values the host discards
         }
         const double dk = sqrt(a), rdk = 1.0 / dk;
@@ -1067,9 +1067,9 @@ constexpr int kLinvWaveMaxN = 2048;
 
 __global__ void __launch_bounds__(256) k_gep_linv(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ L = sd.gep;
-    double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
-    double* __restrict__ X1 = sd.gep + gep_core_dev(n);
+    const auto L = to_global(sd.gep);
+    const auto V = to_global(sd.gep) + 2 * (size_t)n * n;
+    const auto X1 = to_global(sd.gep) + gep_core_dev(n);
     extern __shared__ double xs[];  // four columns of n
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = blockIdx.x * 4 + wave;
@@ -1096,9 +1096,9 @@ __global__ void __launch_bounds__(256) k_gep_linv(const SlotDesc* __restrict__ s
 // The same with one lane per column (no LDS; for n above kLinvWaveMaxN).
 __global__ void __launch_bounds__(64) k_gep_linv_lane(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ L = sd.gep;
-    double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
-    double* __restrict__ X1 = sd.gep + gep_core_dev(n);
+    const auto L = to_global(sd.gep);
+    const auto V = to_global(sd.gep) + 2 * (size_t)n * n;
+    const auto X1 = to_global(sd.gep) + gep_core_dev(n);
     const int j = blockIdx.x * 64 + threadIdx.x;
     if (j >= n) return;
     for (int i = 0; i < j; ++i) {
@@ -1122,9 +1122,9 @@ __global__ void __launch_bounds__(64) k_gep_linv_lane(const SlotDesc* __restrict
 // U = H L^-T: U[i][j] = sum_{k<=j} H[i][k] X[j][k] (thread = row i, grid y = column j)
 __global__ void __launch_bounds__(256) k_gep_hlt(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ H = sd.gep + (size_t)n * n;  // M holds the symmetrised H
-    const double* __restrict__ X1 = sd.gep + gep_core_dev(n);
-    double* __restrict__ X2 = sd.gep + gep_core_dev(n) + (size_t)n * n;
+    const auto H = to_global(sd.gep) + (size_t)n * n;  // M holds the symmetrised H
+    const auto X1 = to_global(sd.gep) + gep_core_dev(n);
+    const auto X2 = to_global(sd.gep) + gep_core_dev(n) + (size_t)n * n;
     const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
     if (i >= n) return;
     double s0 = 0.0, s1 = 0.0;
@@ -1140,9 +1140,9 @@ __global__ void __launch_bounds__(256) k_gep_hlt(const SlotDesc* __restrict__ sl
 // A = L^-1 U: A[i][j] = sum_{k<=i} X[i][k] U[k][j] into M (thread = row i, grid y = column j)
 __global__ void __launch_bounds__(256) k_gep_lul(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
-    double* __restrict__ A = sd.gep + (size_t)n * n;
-    const double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
-    const double* __restrict__ X2 = sd.gep + gep_core_dev(n) + (size_t)n * n;
+    const auto A = to_global(sd.gep) + (size_t)n * n;
+    const auto V = to_global(sd.gep) + 2 * (size_t)n * n;
+    const auto X2 = to_global(sd.gep) + gep_core_dev(n) + (size_t)n * n;
     const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
     if (i >= n) return;
     double s0 = 0.0, s1 = 0.0;
@@ -1163,20 +1163,20 @@ __global__ void __launch_bounds__(256) k_gep_lul(const SlotDesc* __restrict__ sl
 // trailing lower triangle (16 x 16 tiles). About 3 n / kCholPanel launches instead of 2 n.
 __global__ void __launch_bounds__(256) k_gep_load(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
-    double* __restrict__ Lm = sd.gep;
-    double* __restrict__ M = Lm + (size_t)n * n;
+    const auto Lm = to_global(sd.gep);
+    const auto M = Lm + (size_t)n * n;
     const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
     if (idx >= (long long)n * n) return;
     const int i = (int)(idx % n), j = (int)(idx / n);
-    Lm[idx] = 0.5 * (sd.Ss[i + (size_t)n * j] + sd.Ss[j + (size_t)n * i]);
-    M[idx] = 0.5 * (sd.Hs[i + (size_t)n * j] + sd.Hs[j + (size_t)n * i]);
+    Lm[idx] = 0.5 * (sd.Ss[i + (size_t)n * j] + to_global(sd.Ss)[j + (size_t)n * i]);
+    M[idx] = 0.5 * (sd.Hs[i + (size_t)n * j] + to_global(sd.Hs)[j + (size_t)n * i]);
 }
 
 constexpr int kCholPanel = 32;
 
 __global__ void __launch_bounds__(256) k_chol_diag(const SlotDesc* __restrict__ slots, int n, int j0, int jb) {
     const SlotDesc& sd = slots[blockIdx.z];
-    double* __restrict__ L = sd.gep;
+    const auto L = to_global(sd.gep);
     __shared__ double t[kCholPanel][kCholPanel + 1];  // t[i][j], lower triangle
     for (int idx = threadIdx.x; idx < jb * jb; idx += blockDim.x) {
         const int i = idx % jb, j = idx / jb;
@@ -1185,7 +1185,7 @@ __global__ void __launch_bounds__(256) k_chol_diag(const SlotDesc* __restrict__ 
     __syncthreads();
     for (int k = 0; k < jb; ++k) {
         const double a = t[k][k];
-        if (!(a > 0.0) && threadIdx.x == 0) sd.flags[FLAG_METRIC] = 1;
+        if (!(a > 0.0) && threadIdx.x == 0) to_global(sd.flags)[FLAG_METRIC] = 1;
         const double dk = sqrt(a), rdk = 1.0 / dk;
         __syncthreads();  // every thread has read the pivot before it is replaced
         for (int i = k + threadIdx.x; i < jb; i += blockDim.x) t[i][k] = (i == k) ? dk : t[i][k] * rdk;
@@ -1206,7 +1206,7 @@ __global__ void __launch_bounds__(256) k_chol_diag(const SlotDesc* __restrict__ 
 // rows i >= j0 + jb: L[i][j0..j0+jb) = A[i][j0..j0+jb) L_tile^-T (forward substitution per row)
 __global__ void __launch_bounds__(256) k_chol_trsm(const SlotDesc* __restrict__ slots, int n, int j0, int jb) {
     const SlotDesc& sd = slots[blockIdx.z];
-    double* __restrict__ L = sd.gep;
+    const auto L = to_global(sd.gep);
     __shared__ double t[kCholPanel][kCholPanel + 1];
     for (int idx = threadIdx.x; idx < jb * jb; idx += blockDim.x) {
         const int i = idx % jb, j = idx / jb;
@@ -1235,7 +1235,7 @@ __global__ void __launch_bounds__(256) k_chol_syrk(const SlotDesc* __restrict__ 
     const int ti = blockIdx.x, tj = blockIdx.y;
     if (tj > ti) return;  // block-uniform: tiles above the diagonal
     const SlotDesc& sd = slots[blockIdx.z];
-    double* __restrict__ L = sd.gep;
+    const auto L = to_global(sd.gep);
     __shared__ double li[16][kCholPanel + 1], lj[16][kCholPanel + 1];
     const int r0 = j0 + jb + 16 * ti, c0 = j0 + jb + 16 * tj;
     for (int idx = threadIdx.x; idx < 16 * jb; idx += blockDim.x) {
@@ -1263,9 +1263,8 @@ __global__ void __launch_bounds__(256) k_chol_syrk(const SlotDesc* __restrict__ 
 // written by its own wavefront only. k_hh_finish applies the last step to the trailing 2 x 2
 // block and writes the last reduced column. Workspace: M, V as in k_gep_lowest; p in y / tdl by
 // parity of k, alpha_k in tdu.
-__device__ __forceinline__ void hh_pending(const double* __restrict__ V, const double* __restrict__ pbuf,
-                                           const double* __restrict__ beta, int n, int k, double* vprev,
-                                           double* wprev, double* red) {
+__device__ __forceinline__ void hh_pending(gptr<const double> V, gptr<const double> pbuf, gptr<const double> beta,
+                                           int n, int k, double* vprev, double* wprev, double* red) {
     // v_{k-1} and w_{k-1} into LDS (zeros above row k and for k == 0)
     const int tid = threadIdx.x, nt = blockDim.x;
     double part = 0.0;
@@ -1281,13 +1280,13 @@ __device__ __forceinline__ void hh_pending(const double* __restrict__ V, const d
 
 __global__ void __launch_bounds__(256) k_hh_step(const SlotDesc* __restrict__ slots, int n, int k) {
     const SlotDesc& sd = slots[blockIdx.z];
-    double* __restrict__ M = sd.gep + (size_t)n * n;
-    double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
-    double* __restrict__ vec = sd.gep + 3 * (size_t)n * n;  // dd, ee, y, beta, tdl, tdu, ...
-    double* __restrict__ beta = vec + 3 * (size_t)n;
-    double* __restrict__ alphas = vec + 5 * (size_t)n;
-    const double* __restrict__ pprev = vec + ((k - 1) & 1 ? 4 : 2) * (size_t)n;
-    double* __restrict__ pnow = vec + (k & 1 ? 4 : 2) * (size_t)n;
+    const auto M = to_global(sd.gep) + (size_t)n * n;
+    const auto V = to_global(sd.gep) + 2 * (size_t)n * n;
+    const auto vec = to_global(sd.gep) + 3 * (size_t)n * n;  // dd, ee, y, beta, tdl, tdu, ...
+    const auto beta = vec + 3 * (size_t)n;
+    const auto alphas = vec + 5 * (size_t)n;
+    const auto pprev = vec + ((k - 1) & 1 ? 4 : 2) * (size_t)n;
+    const auto pnow = vec + (k & 1 ? 4 : 2) * (size_t)n;
     extern __shared__ double loc[];  // v_k, v_{k-1}, w_{k-1}: 3 n
     double* vloc = loc;
     double* vprev = loc + n;
@@ -1341,13 +1340,13 @@ __global__ void __launch_bounds__(256) k_hh_step(const SlotDesc* __restrict__ sl
 
 __global__ void __launch_bounds__(256) k_hh_finish(const SlotDesc* __restrict__ slots, int n) {
     const SlotDesc& sd = slots[blockIdx.z];
-    double* __restrict__ M = sd.gep + (size_t)n * n;
-    const double* __restrict__ V = sd.gep + 2 * (size_t)n * n;
-    double* __restrict__ vec = sd.gep + 3 * (size_t)n * n;
-    const double* __restrict__ beta = vec + 3 * (size_t)n;
-    const double* __restrict__ alphas = vec + 5 * (size_t)n;
+    const auto M = to_global(sd.gep) + (size_t)n * n;
+    const auto V = to_global(sd.gep) + 2 * (size_t)n * n;
+    const auto vec = to_global(sd.gep) + 3 * (size_t)n * n;
+    const auto beta = vec + 3 * (size_t)n;
+    const auto alphas = vec + 5 * (size_t)n;
     const int k = n - 2;  // the step whose update is pending is k - 1 = n - 3
-    const double* __restrict__ pprev = vec + ((k - 1) & 1 ? 4 : 2) * (size_t)n;
+    const auto pprev = vec + ((k - 1) & 1 ? 4 : 2) * (size_t)n;
     extern __shared__ double loc[];  // v_{k-1}, w_{k-1}: 2 n
     double* vprev = loc;
     double* wprev = loc + n;
@@ -1378,33 +1377,40 @@ __global__ void __launch_bounds__(256) k_hh_finish(const SlotDesc* __restrict__ 
 template <bool InLds, int Part = 0>
 __global__ void __launch_bounds__(InLds ? 256 : 1024) k_gep_lowest(const SlotDesc* __restrict__ slots, TopoDesc topo) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ Hs = sd.Hs;
-    const double* __restrict__ Ss = sd.Ss;
+    const auto Hs = to_global(sd.Hs);
+    const auto Ss = to_global(sd.Ss);
     const int n = topo.n_str;
     const double e_nuc = sd.e_nuc;
     double* __restrict__ gwork = sd.gep;
-    double* __restrict__ coef = sd.coef;
-    double* __restrict__ energy = sd.energy;
-    int* __restrict__ flags = sd.flags;
+    const auto coef = to_global(sd.coef);
+    const auto energy = to_global(sd.energy);
+    const auto flags = to_global(sd.flags);
     const long long* __restrict__ off = topo.off;
     const int* __restrict__ ts = topo.ts;
     const double* __restrict__ tc = topo.tc;
     const int n_det = topo.n_det;
-    double* __restrict__ dcoef = sd.dcoef;
+    const auto dcoef = to_global(sd.dcoef);
     extern __shared__ double lds[];
-    double* work = InLds ? lds : gwork;
-    double* Lm = work;              // n x n, lower Cholesky factor
-    double* M = work + n * n;       // n x n
-    double* V = work + 2 * n * n;   // Householder vectors (column k holds v_k)
-    double* dd = work + 3 * n * n;  // diagonal
-    double* ee = dd + n;            // off-diagonal
-    double* y = ee + n;             // tridiagonal eigenvector / scratch
-    double* beta = y + n;           // Householder scalars
-    double* tdl = beta + n;         // tridiagonal LU factors (inverse iteration)
-    double* tdu = tdl + n;
-    double* tdu2 = tdu + n;
-    double* tdg = tdu2 + n;
-    int* ipiv = reinterpret_cast<int*>(tdg + n);  // n ints in the last n doubles
+    // LDS or device memory, typed as such so that neither instantiation uses flat accesses
+    const auto work = [&] {
+        if constexpr (InLds) return static_cast<double*>(lds);
+        else return to_global(gwork);
+    }();
+    const auto Lm = work;              // n x n, lower Cholesky factor
+    const auto M = work + n * n;       // n x n
+    const auto V = work + 2 * n * n;   // Householder vectors (column k holds v_k)
+    const auto dd = work + 3 * n * n;  // diagonal
+    const auto ee = dd + n;            // off-diagonal
+    const auto y = ee + n;             // tridiagonal eigenvector / scratch
+    const auto beta = y + n;           // Householder scalars
+    const auto tdl = beta + n;         // tridiagonal LU factors (inverse iteration)
+    const auto tdu = tdl + n;
+    const auto tdu2 = tdu + n;
+    const auto tdg = tdu2 + n;
+    const auto ipiv = [&] {  // n ints in the last n doubles
+        if constexpr (InLds) return reinterpret_cast<int*>(tdg + n);
+        else return reinterpret_cast<gptr<int>>(tdg + n);
+    }();
     __shared__ double red[16];
     __shared__ double sh_scalar[4];
     __shared__ double lo_hi[2];
@@ -1730,18 +1736,18 @@ __global__ void __launch_bounds__(256) k_gep_states(const SlotDesc* __restrict__
     const SlotDesc& sd = slots[blockIdx.z];
     const int n = topo.n_str, S = topo.n_states, n_det = topo.n_det;
     if (sd.flags[FLAG_METRIC]) return;
-    double* __restrict__ work = sd.gep;
-    const double* Lm = work;            // L^-1, lower triangle
-    const double* V = work + 2 * (size_t)n * n;
-    const double* dd = work + 3 * (size_t)n * n;
-    const double* ee = dd + n;
-    const double* beta = ee + 2 * (size_t)n;
-    double* tdl = const_cast<double*>(beta) + n;
-    double* tdu = tdl + n;
-    double* tdu2 = tdu + n;
-    double* tdg = tdu2 + n;
-    int* ipiv = reinterpret_cast<int*>(tdg + n);
-    double* Z = work + gep_core_dev(n);  // n_states tridiagonal vectors of n
+    const auto work = to_global(sd.gep);
+    const auto Lm = work;            // L^-1, lower triangle
+    const auto V = work + 2 * (size_t)n * n;
+    const auto dd = work + 3 * (size_t)n * n;
+    const auto ee = dd + n;
+    const auto beta = ee + 2 * (size_t)n;
+    const auto tdl = beta + n;
+    const auto tdu = tdl + n;
+    const auto tdu2 = tdu + n;
+    const auto tdg = tdu2 + n;
+    const auto ipiv = reinterpret_cast<gptr<int>>(tdg + n);
+    const auto Z = work + gep_core_dev(n);  // n_states tridiagonal vectors of n
     __shared__ double red[16];
     __shared__ double lo_hi[2];
     __shared__ double lam_sh[kMaxStates];
@@ -1818,7 +1824,7 @@ __global__ void __launch_bounds__(256) k_gep_states(const SlotDesc* __restrict__
             }
             ipiv[n - 1] = n - 1;
             if (tdg[n - 1] == 0.0) tdg[n - 1] = 1e-300;
-            double* y = Z + (size_t)k * n;
+            const auto y = Z + (size_t)k * n;
             unsigned long long st = 0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL * (unsigned long long)k;
             for (int i = 0; i < n; ++i) {
                 st ^= st << 13;
@@ -1843,7 +1849,7 @@ __global__ void __launch_bounds__(256) k_gep_states(const SlotDesc* __restrict__
                     y[i] = v / tdg[i];
                 }
                 for (int j = 0; j < k; ++j) {  // against the states already found
-                    const double* zj = Z + (size_t)j * n;
+                    const auto zj = Z + (size_t)j * n;
                     double dot = 0.0;
                     for (int i = 0; i < n; ++i) dot += zj[i] * y[i];
                     for (int i = 0; i < n; ++i) y[i] -= dot * zj[i];
@@ -1858,7 +1864,7 @@ __global__ void __launch_bounds__(256) k_gep_states(const SlotDesc* __restrict__
     }
     double eavg = 0.0;
     for (int k = 0; k < S; ++k) {
-        double* y = Z + (size_t)k * n;
+        const auto y = Z + (size_t)k * n;
         for (int j = n - 3; j >= 0; --j) {  // z = H_0 ... H_{n-3} y
             const int m0 = j + 1;
             double part = 0.0;
@@ -1887,10 +1893,10 @@ __global__ void __launch_bounds__(256) k_gep_states(const SlotDesc* __restrict__
         }
         eavg += topo.state_w[k] * lam_sh[k];
     }
-    if (tid < S) sd.state_energy[tid] = lam_sh[tid];
+    if (tid < S) to_global(sd.state_energy)[tid] = lam_sh[tid];
     if (tid == 0) {
-        sd.energy[0] = eavg;
-        sd.energy[1] = eavg + sd.e_nuc;
+        to_global(sd.energy)[0] = eavg;
+        to_global(sd.energy)[1] = eavg + sd.e_nuc;
     }
 }
 
@@ -1906,8 +1912,8 @@ __global__ void __launch_bounds__(256) k_ri_pair_adjoint(const SlotDesc* __restr
     const SlotDesc& sd = slots[blockIdx.z];
     const int n_aux = sd.n_aux;
     if ((int)blockIdx.x * 256 >= n_aux) return;  // block-uniform
-    const double* __restrict__ L = sd.L;
-    const double* __restrict__ dg = sd.dsum;
+    const auto L = to_global(sd.L);
+    const auto dg = to_global(sd.dsum);
     __shared__ double m[8][kGramLd];
     const int P0 = blockIdx.y * 8;
     for (int idx = threadIdx.x; idx < 8 * np; idx += blockDim.x) {
@@ -1928,7 +1934,7 @@ __global__ void __launch_bounds__(256) k_ri_pair_adjoint(const SlotDesc* __restr
     if (A >= n_aux) return;
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-        if (P0 + j < np) sd.dL[(size_t)A + (size_t)n_aux * (P0 + j)] = acc[j];
+        if (P0 + j < np) to_global(sd.dL)[(size_t)A + (size_t)n_aux * (P0 + j)] = acc[j];
 }
 
 // gcn[mu, p] = sum_i SC[mu,i] (ds+ds^T)[i,p] + HC[mu,i] (dh+dh^T)[i,p]
@@ -1951,13 +1957,13 @@ __global__ void __launch_bounds__(256) k_orbital_gradient(const SlotDesc* __rest
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
     const int mu_blk = blockIdx.x * 4 * R;
     if (mu_blk >= n_bf) return;  // block-uniform
-    const double* __restrict__ T = sd.T;
-    const double* __restrict__ dL = sd.dL;
+    const auto T = to_global(sd.T);
+    const auto dL = to_global(sd.dL);
     __shared__ double wq[QC * NQ * 64];  // [q - q0][p][a]
     __shared__ double part[4 * R][NQ];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int mu0 = mu_blk + wave * R;
-    const double* tmu[R];
+    gptr<const double> tmu[R];
 #pragma unroll
     for (int u = 0; u < R; ++u) tmu[u] = T + (size_t)n_aux * n * min(mu0 + u, n_bf - 1);
     double acc[R][NQ];
@@ -2007,10 +2013,10 @@ __global__ void __launch_bounds__(256) k_orbital_gradient(const SlotDesc* __rest
         if (mu >= n_bf) continue;
         double g = part[u][p];
         for (int i = 0; i < n; ++i) {
-            g += sd.SC[mu + (size_t)n_bf * i] * (ds[i + n * p] + ds[p + n * i]);
-            g += sd.HC[mu + (size_t)n_bf * i] * (dh[i + n * p] + dh[p + n * i]);
+            g += to_global(sd.SC)[mu + (size_t)n_bf * i] * (ds[i + n * p] + ds[p + n * i]);
+            g += to_global(sd.HC)[mu + (size_t)n_bf * i] * (dh[i + n * p] + dh[p + n * i]);
         }
-        sd.gcn[mu + (size_t)n_bf * p] = g;
+        to_global(sd.gcn)[mu + (size_t)n_bf * p] = g;
     }
 }
 
@@ -2025,11 +2031,11 @@ __global__ void __launch_bounds__(256) k_orbital_gradient_few(const SlotDesc* __
     const int n_bf = sd.n_bf, n_aux = sd.n_aux;
     const int mu0 = blockIdx.x * kGradRows;
     if (mu0 >= n_bf) return;  // block-uniform
-    const double* __restrict__ T = sd.T;
-    const double* __restrict__ dL = sd.dL;
+    const auto T = to_global(sd.T);
+    const auto dL = to_global(sd.dL);
     __shared__ double part[4][kGradRows][NQ];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const double* tmu[kGradRows];
+    gptr<const double> tmu[kGradRows];
 #pragma unroll
     for (int u = 0; u < kGradRows; ++u) tmu[u] = T + (size_t)n_aux * n * min(mu0 + u, n_bf - 1);
     double acc[kGradRows][NQ];
@@ -2065,23 +2071,23 @@ __global__ void __launch_bounds__(256) k_orbital_gradient_few(const SlotDesc* __
         if (mu >= n_bf) continue;
         double g = (part[0][u][p] + part[1][u][p]) + (part[2][u][p] + part[3][u][p]);
         for (int i = 0; i < n; ++i) {
-            g += sd.SC[mu + (size_t)n_bf * i] * (ds[i + n * p] + ds[p + n * i]);
-            g += sd.HC[mu + (size_t)n_bf * i] * (dh[i + n * p] + dh[p + n * i]);
+            g += to_global(sd.SC)[mu + (size_t)n_bf * i] * (ds[i + n * p] + ds[p + n * i]);
+            g += to_global(sd.HC)[mu + (size_t)n_bf * i] * (dh[i + n * p] + dh[p + n * i]);
         }
-        sd.gcn[mu + (size_t)n_bf * p] = g;
+        to_global(sd.gcn)[mu + (size_t)n_bf * p] = g;
     }
 }
 
 // raw-slot gradient: grad[k, mu] = (g - (g.cn_k) S cn_k)[mu] / sqrt(nu2_k) on present slots.
 __global__ void k_normalization_pullback(const SlotDesc* __restrict__ slots) {
     const SlotDesc& sd = slots[blockIdx.z];
-    const double* __restrict__ gcn = sd.gcn;
-    const double* __restrict__ Cn = sd.Cn;
-    const double* __restrict__ SC = sd.SC;
-    const double* __restrict__ nu2 = sd.nu2;
-    const uint8_t* __restrict__ support = sd.support;
+    const auto gcn = to_global(sd.gcn);
+    const auto Cn = to_global(sd.Cn);
+    const auto SC = to_global(sd.SC);
+    const auto nu2 = to_global(sd.nu2);
+    const auto support = to_global(sd.support);
     const int n_bf = sd.n_bf;
-    double* __restrict__ grad = sd.grad;
+    const auto grad = to_global(sd.grad);
     __shared__ double red[16];
     const int k = blockIdx.x;
     double part = 0.0;

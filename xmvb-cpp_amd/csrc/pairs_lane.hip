@@ -320,7 +320,7 @@ __device__ __forceinline__ double w_terms(const double* W, int copies, int nc, i
 // dg += the expansion of W, every adjoint element computed by one thread (no atomics);
 // block-cooperative. FullDg: dg[X * np + Y] ordered; packed: dg[pair_index(X, Y)] takes both orders.
 template <bool FullDg>
-__device__ void w_gather(const double* W, int copies, int nc, double* dg, int np) {
+__device__ __forceinline__ void w_gather(const double* W, int copies, int nc, double* dg, int np) {
     const int targets = FullDg ? np * np : np * (np + 1) / 2;
     for (int idx = threadIdx.x; idx < targets; idx += blockDim.x) {
         double acc;
@@ -462,13 +462,13 @@ struct LaneBwdLayout {
 
 __device__ __forceinline__ void load_pair_tables(const SlotDesc& sd, int n, int np, double* G2, double* sact,
                                                  double* hact) {
-    const double* __restrict__ G_glob = sd.G;
+    const auto G_glob = to_global(static_cast<const double*>(sd.G));
     const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, ln = threadIdx.x & 63;
     for (int Q = wave; Q < np; Q += nwaves)
         for (int P = ln; P < np; P += 64) G2[P + np * Q] = G_glob[pair_index(P, Q)];
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-        sact[t] = sd.sact[t];
-        hact[t] = sd.hact[t];
+        sact[t] = to_global(sd.sact)[t];
+        hact[t] = to_global(sd.hact)[t];
     }
 }
 
@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
         const bool ok1 = lane_invert<M1>(sact, n, cond_limit, b1);
         // deferred work items (the partner pair has the same blocks and goes with it) are flagged
         // and listed in work-item order by k_compact_pairs, so the list does not depend on timing
-        sd.defer_flag[i] = ok0 && ok1 ? 0 : 1;
+        to_global(sd.defer_flag)[i] = ok0 && ok1 ? 0 : 1;
         if (!(ok0 && ok1)) continue;
         // spin-1 packed indices are kept (the opposite-spin loop reuses each one M0^2 times);
         // spin-0 indices are recomputed where needed, which costs VALU only
@@ -539,15 +539,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
                 phi += b0.a[l][r] * inner;
             }
         const double H = c * phi;
-        sd.Hdet[(size_t)d + (size_t)n_det * e] = H;
-        sd.Hdet[(size_t)e + (size_t)n_det * d] = H;
-        sd.Sdet[(size_t)d + (size_t)n_det * e] = c;
-        sd.Sdet[(size_t)e + (size_t)n_det * d] = c;
+        const auto Hd = to_global(sd.Hdet), Sd = to_global(sd.Sdet);
+        Hd[(size_t)d + (size_t)n_det * e] = H;
+        Hd[(size_t)e + (size_t)n_det * d] = H;
+        Sd[(size_t)d + (size_t)n_det * e] = c;
+        Sd[(size_t)e + (size_t)n_det * d] = c;
         if (tf != t) {
-            sd.Hdet[(size_t)fd + (size_t)n_det * fe] = H;
-            sd.Hdet[(size_t)fe + (size_t)n_det * fd] = H;
-            sd.Sdet[(size_t)fd + (size_t)n_det * fe] = c;
-            sd.Sdet[(size_t)fe + (size_t)n_det * fd] = c;
+            Hd[(size_t)fd + (size_t)n_det * fe] = H;
+            Hd[(size_t)fe + (size_t)n_det * fd] = H;
+            Sd[(size_t)fd + (size_t)n_det * fe] = c;
+            Sd[(size_t)fe + (size_t)n_det * fd] = c;
         }
     }
 }
@@ -660,7 +661,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lane_f
         overlap_adjoint<M1>(ds, dh, n, b1, F1, c, sc, wH);
     }
     __syncthreads();
-    double* out = sd.partials + (size_t)blockIdx.x * (ng + 2 * n * n) + ng;
+    const auto out = to_global(sd.partials) + (size_t)blockIdx.x * (ng + 2 * n * n) + ng;
     const int nw = blockDim.x >> 6;
     for (int t = threadIdx.x; t < 2 * n * n; t += blockDim.x) {
         double v = 0.0;
@@ -696,7 +697,7 @@ __global__ void __launch_bounds__(256)
     double* W = W_all + (private_w ? wave * lay.nc * lay.nc : 0);
     double* bufA = bufs + (size_t)wave * 32 * lay.np16;
     double* bufB = bufA + 16 * lay.np16;
-    for (int t = threadIdx.x; t < n * n; t += blockDim.x) sact[t] = sd.sact[t];
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) sact[t] = to_global(sd.sact)[t];
     {
         const int nzero = lay.dcopies * dglen + lay.wcopies * lay.nc * lay.nc + (blockDim.x >> 6) * 32 * lay.np16;
         for (int t = threadIdx.x; t < nzero; t += blockDim.x) dg_all[t] = 0.0;
@@ -749,7 +750,9 @@ __global__ void __launch_bounds__(256)
         auto pidx1 = [&](int r, int l) { return P1[r][l]; };
         const double cw = live ? b0.det * b1.det * w : 0.0;
         if constexpr (Fast) {
-            auto same_spin = [&]() {
+            // inlined at both call sites: an outlined lambda sees W only as a generic pointer, and
+            // its LDS accesses become flat instructions
+            auto same_spin = [&]() __attribute__((always_inline)) {
                 if (live) {
                     same_spin_w<M0>(W, lay.nc, b0, cw);
                     same_spin_w<M1>(W, lay.nc, b1, cw);
@@ -805,7 +808,7 @@ __global__ void __launch_bounds__(256)
         w_gather<FullDg>(W_all, lay.wcopies, lay.nc, dg, np);
         __syncthreads();
     }
-    double* out = sd.partials + (size_t)blockIdx.x * (ng + 2 * n * n);
+    const auto out = to_global(sd.partials) + (size_t)blockIdx.x * (ng + 2 * n * n);
     if (FullDg) {
         for (int t = threadIdx.x; t < ng; t += blockDim.x) {
             int Q, P;

@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(256) k_pairs_singular(const SlotDesc* __restri
     const SlotDesc& sd = slots[blockIdx.z];
     const int n = topo.n, ng = topo.ng, n_det = topo.n_det;
     const int np = n * (n + 1) / 2;
-    const int count = sd.flags[FLAG_SINGULAR_PAIRS];
+    const int count = to_global(sd.flags)[FLAG_SINGULAR_PAIRS];
     extern __shared__ double lds[];
     double* G2 = lds;
     double* sact = G2 + np * np;
@@ -391,11 +391,11 @@ __global__ void __launch_bounds__(256) k_pairs_singular(const SlotDesc* __restri
     if (count > 0) {
         for (int t = threadIdx.x; t < np * np; t += blockDim.x) {
             const int P = t % np, Q = t / np;
-            G2[t] = sd.G[pair_index(P, Q)];
+            G2[t] = to_global(sd.G)[pair_index(P, Q)];
         }
         for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-            sact[t] = sd.sact[t];
-            hact[t] = sd.hact[t];
+            sact[t] = to_global(sd.sact)[t];
+            hact[t] = to_global(sd.hact)[t];
         }
     }
     if (Backward) {
@@ -443,10 +443,10 @@ __global__ void __launch_bounds__(256) k_pairs_singular(const SlotDesc* __restri
             }
         }
         if (!Backward) {
-            sd.Hdet[(size_t)d + (size_t)n_det * e] = H;
-            sd.Hdet[(size_t)e + (size_t)n_det * d] = H;
-            sd.Sdet[(size_t)d + (size_t)n_det * e] = S;
-            sd.Sdet[(size_t)e + (size_t)n_det * d] = S;
+            to_global(sd.Hdet)[(size_t)d + (size_t)n_det * e] = H;
+            to_global(sd.Hdet)[(size_t)e + (size_t)n_det * d] = H;
+            to_global(sd.Sdet)[(size_t)d + (size_t)n_det * e] = S;
+            to_global(sd.Sdet)[(size_t)e + (size_t)n_det * d] = S;
         }
     }
     if (Backward) {
