@@ -17,10 +17,12 @@
 // orbitals) and projected into the current chart when used, as transport_secant_pairs_to_chart
 // does (preconditioners/transported_lbfgs.cpp:107-131).
 //
-// Device work per iteration: one block per orbital for the chart (k_opt_point), one block for the
-// two-loop recursion with the chart projections folded into its dot products (k_opt_direction),
-// a streaming kernel per Armijo trial (k_opt_trial) and one block for the secant test and history
-// append (k_opt_append). The host reads scalars only: energies, the directional derivative, the
+// Device work per iteration: one block per orbital for the chart (k_opt_point), the two-loop
+// recursion in compact form (k_dir_*: dot products of the stored pairs with the gradient and the
+// preconditioned vector over many workgroups, the two scalar loops on one wavefront, two streaming
+// combinations), a streaming kernel per Armijo trial (k_opt_trial) and, per accepted point, the
+// secant test and history append (k_opt_append, k_opt_gram: one new row and column of the ring's
+// Gram entries). The host reads scalars only: energies, the directional derivative, the
 // projected-gradient norm and the step size.
 
 #include "gmem.hpp"
@@ -43,8 +45,6 @@ namespace xmvb_amd {
 namespace {
 
 constexpr int kPointThreads = 256;
-constexpr int kDirThreads = 256;  // two-loop block
-constexpr int kDirRegs = 8;       // elements per thread the two-loop keeps in registers (N <= 2048)
 constexpr int kOrbStats = 4;  // per orbital: dual norm, max |c|, norm before normalisation, c^T g
 
 void hip_check(hipError_t e, const char* what) {
@@ -95,26 +95,6 @@ __device__ double block_total(double v, double* red) {
     return t;
 }
 
-// Barrier whose fences cover LDS only: outstanding global loads stay in flight across it (the
-// two-loop's prefetched pairs), where __syncthreads would wait for them.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// block_total with LDS-only barriers (the same sum, in the same order)
-__device__ double block_total_lds(double v, double* red) {
-    v = wave_total(v);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    lds_barrier();
-    if (lane == 0) red[wave] = v;
-    lds_barrier();
-    double t = 0.0;
-    for (int w = 0; w < nw; ++w) t += red[w];
-    return t;
-}
-
 __device__ double block_largest(double v, double* red) {
     v = wave_largest(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -148,6 +128,9 @@ struct OptDesc {
     double* q;                  // two-loop work vectors
     double* z;
     double* apx;                // 6 per orbital: secant sums and max |s_i| (append kernels)
+    double* gsy;                // Gram entries s_a,i . y_b,i of the ring: cap x cap x n_act (slots)
+    double* sy;                 // SY of the pairs in use: cap x cap (ages)
+    double* dws;                // direction workspace: 5 cap + 4 + 2 x combine workgroups
     double* stats;              // kOrbStats * n_act chart statistics
     double* out;                // kOutSlots direction / append scalars
     double* hs;                 // secant ring: steps, gradient changes, per-orbital statistics
@@ -232,11 +215,6 @@ __global__ void __launch_bounds__(kPointThreads) k_opt_point(const OptDesc* __re
     }
 }
 
-// Search direction t = -H r by the two-loop recursion (TransportedReducedLbfgsPreconditioner::
-// apply, transported_lbfgs.cpp:41-86) over the secant pairs projected into the current chart,
-// with the scaled-identity initial inverse gamma S^-1 (gamma = s'y'/y'y' of the newest kept pair)
-// and the try_add_pair curvature filter (transported_lbfgs.cpp:14-32). One block per problem;
-// pairs are addressed by age (0 = newest) in the ring.
 // Chart projections of the stored pairs for the two-loop: sig[a,i] = (S c_i)^T s_a,i and
 // bet[a,i] = c_i^T y_a,i (a = age, newest first), one wavefront per dot product over the orbital's
 // n_bf coefficients, spread over many blocks (grid x over tasks, y = which, z = problem).
@@ -254,43 +232,53 @@ __global__ void __launch_bounds__(256) k_opt_project(const OptDesc* __restrict__
     if (lane == 0) o.proj[(size_t)which * o.cap * n_act + task] = part;
 }
 
-__global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __restrict__ descs, int n_act) {
+// Search direction t = -H r of the two-loop recursion (TransportedReducedLbfgsPreconditioner::
+// apply, transported_lbfgs.cpp:41-86) over the secant pairs projected into the current chart,
+// with the scaled-identity initial inverse gamma S^-1 (gamma = s'y'/y'y' of the newest kept pair)
+// and the try_add_pair curvature filter (transported_lbfgs.cpp:14-32); pairs are addressed by
+// age (0 = newest) in the ring. The recursion is evaluated in compact form, which gives the same
+// coefficients without a reduction over the full vectors per pair: with s^_a, y^_a the projected
+// pairs (s^ = s - c (S c)^T s, y^ = y - (S c) c^T y per orbital) and SY[a][b] = s^_a . y^_b,
+//   first loop   alpha_a = rho_a (s^_a . r - sum_{b newer} alpha_b SY[a][b]),  q = r - sum_b alpha_b y^_b
+//   z0           = gamma S_i^-1 q_i
+//   second loop  coef_a = alpha_a - rho_a (y^_a . z0 + sum_{c older} coef_c SY[c][a])
+//   t            = -(z0 + sum_c coef_c s^_c)
+// so the vector work is four streaming passes over the stored pairs (s^ . r, q, y^ . z0, t) and
+// the two loops run on scalars. SY comes from chart-independent per-orbital Gram entries
+// Gamma[a][b][i] = s_a,i . y_b,i kept with the ring (k_opt_gram, one new row and column per
+// appended pair): s^_a,i . y^_b,i = Gamma[a][b][i] - sig_a,i bet_b,i, because (S c_i)^T c_i = 1.
+// One block per problem.
+// The direction runs as ten small launches over many workgroups (a single-problem direction
+// otherwise occupies one CU and waits on memory latency for every pair).
+constexpr int kPairsMax = 256;   // secant pairs the recursions hold (four per lane)
+constexpr int kCombineElems = 64;  // elements per combine workgroup (x 4 pair chunks = 256 threads)
+
+// per-problem direction workspace (o.dws): rho, gam, alp, cof, dts by age (cap each), then the
+// scalars [gamma, used, bad] and the final kernel's per-workgroup partials (dd, tinf)
+__device__ __forceinline__ size_t dws_scalars(int cap) { return (size_t)5 * cap; }
+__device__ __forceinline__ size_t dws_partials(int cap) { return (size_t)5 * cap + 4; }
+
+__device__ __forceinline__ double sig_of(const OptDesc& o, int n_act, int a, int i) {
+    return to_global(static_cast<const double*>(o.proj))[(size_t)a * n_act + i];
+}
+__device__ __forceinline__ double bet_of(const OptDesc& o, int n_act, int a, int i) {
+    return to_global(static_cast<const double*>(o.proj))[(size_t)o.cap * n_act + (size_t)a * n_act + i];
+}
+
+// rho_a = 1 / (s^_a . y^_a) and gam_a = s^'y^ / y^'y^ of the pairs that pass the curvature test
+// (0 otherwise), gamma of the newest one kept, the count of pairs kept. One workgroup per problem.
+__global__ void __launch_bounds__(256) k_dir_prep(const OptDesc* __restrict__ descs, int n_act) {
     const OptDesc& o = descs[blockIdx.z];
-    const int n_bf = o.n_bf, N = o.N, cap = o.cap, count = o.count, newest = o.newest;
-    const auto C = to_global(static_cast<const double*>(o.C));
-    const auto Sc = to_global(static_cast<const double*>(o.Sc));
-    const auto hs = to_global(static_cast<const double*>(o.hs));
-    const auto hy = to_global(static_cast<const double*>(o.hy));
-    const auto R = to_global(static_cast<const double*>(o.R));
-    const auto G = to_global(static_cast<const double*>(o.G));
-    const auto Sinv = to_global(o.Sinv);
-    const auto pat = to_global(o.pat);
-    const auto proj = to_global(static_cast<const double*>(o.proj));
+    const int cap = o.cap, count = o.count, newest = o.newest;
     const auto hstat = to_global(static_cast<const double*>(o.hstat));
-    const auto q = to_global(o.q);
-    const auto z = to_global(o.z);
-    extern __shared__ double lds[];
-    double* sig = lds;                          // (S c_i)^T s_a,i     count x n_act
-    double* bet = sig + (size_t)count * n_act;  // c_i^T y_a,i         count x n_act
-    double* alp = bet + (size_t)count * n_act;  // first-loop coefficients
-    double* rho = alp + count;                  // 1 / s'^T y' (0: pair rejected)
-    double* gam = rho + count;                  // s'^T y' / y'^T y'
-    __shared__ double red[16];
-    __shared__ int bad;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    if (tid == 0) bad = 0;
-    // projection coefficients of the stored pairs in this chart (k_opt_project)
-    for (int t = tid; t < count * n_act; t += nt) {
-        sig[t] = proj[t];
-        bet[t] = proj[(size_t)cap * n_act + t];
-    }
-    __syncthreads();
-    for (int a = tid; a < count; a += nt) {
+    const auto w = to_global(o.dws);
+    __shared__ double rho_s[kPairsMax], gam_s[kPairsMax];
+    for (int a = threadIdx.x; a < count; a += blockDim.x) {
         const int slot = (newest - a + cap) % cap;
         double ss = 0.0, yy = 0.0, sy = 0.0;
         for (int i = 0; i < n_act; ++i) {
             const auto st = hstat + ((size_t)slot * n_act + i) * 3;
-            const double s_ = sig[a * n_act + i], b_ = bet[a * n_act + i];
+            const double s_ = sig_of(o, n_act, a, i), b_ = bet_of(o, n_act, a, i);
             ss += st[0] - s_ * s_;
             yy += st[1] - b_ * b_;
             sy += st[2] - s_ * b_;
@@ -298,207 +286,239 @@ __global__ void __launch_bounds__(kDirThreads) k_opt_direction(const OptDesc* __
         const double sn = sqrt(fmax(ss, 0.0)), yn = sqrt(fmax(yy, 0.0));
         const bool ok = sn > 0.0 && yn > 0.0 && isfinite(sn) && isfinite(yn) && isfinite(sy) &&
                         sy > 1.4901161193847656e-08 * sn * yn;  // sqrt(DBL_EPSILON)
-        rho[a] = ok ? 1.0 / sy : 0.0;
-        gam[a] = ok ? sy / yy : 0.0;
+        rho_s[a] = ok ? 1.0 / sy : 0.0;
+        gam_s[a] = ok ? sy / yy : 0.0;
+        w[a] = rho_s[a];
+        w[cap + a] = gam_s[a];
     }
     __syncthreads();
-    int used = 0;
-    double gamma = 1.0;
-    if (N <= kDirRegs * nt) {
-        // register path: a thread's elements e = tid + m nt stay in registers (q, z, C, Sc); the
-        // pairs' s and y stream through a ring of three register slots, each refilled with the pair
-        // three ahead as soon as it is consumed, so the loads of a pair are in flight for three
-        // block reductions. The arithmetic per element and its order are those of the memory path.
-        // Loads are unconditional (indices past N clamped, those elements masked by selects) and a
-        // pair slot is always refilled: loads under branches make the compiler wait for every
-        // outstanding load (vmcnt(0)) at the next use, which would serialise the ring.
-        double qr[kDirRegs], cr[kDirRegs], scr[kDirRegs];
-        double s0[kDirRegs], y0[kDirRegs], s1[kDirRegs], y1[kDirRegs], s2[kDirRegs], y2[kDirRegs];
-        int orb[kDirRegs], ec[kDirRegs];
-        bool in[kDirRegs];
-#pragma unroll
-        for (int m = 0; m < kDirRegs; ++m) {
-            const int e = tid + m * nt;
-            in[m] = e < N;
-            ec[m] = in[m] ? e : N - 1;
-            qr[m] = in[m] ? R[ec[m]] : 0.0;
-            cr[m] = C[ec[m]];
-            scr[m] = Sc[ec[m]];
-            orb[m] = in[m] ? e / n_bf : 0;
-        }
-        auto load_pair = [&](int a, double (&sv)[kDirRegs], double (&yv)[kDirRegs]) {
-            const int slot = (newest - a + cap) % cap;
-            const auto s = hs + (size_t)slot * N;
-            const auto y = hy + (size_t)slot * N;
-#pragma unroll
-            for (int m = 0; m < kDirRegs; ++m) {
-                sv[m] = s[ec[m]];
-                yv[m] = y[ec[m]];
+    if (threadIdx.x == 0) {
+        int used = 0;
+        double gamma = 1.0;
+        for (int a = 0; a < count; ++a)
+            if (rho_s[a] != 0.0) {
+                if (used == 0) gamma = gam_s[a];
+                ++used;
             }
-        };
-        // pairs in use (rho != 0) by age: next older / next newer after a (count / -1 at the end)
-        auto older = [&](int a) {
-            int b = a + 1;
-            while (b < count && rho[b] == 0.0) ++b;
-            return b < count ? b : count;
-        };
-        auto newer = [&](int a) {
-            int b = a - 1;
-            while (b >= 0 && rho[b] == 0.0) --b;
-            return b;
-        };
-        // first loop, newest to oldest (a slot past the last pair is refilled with the current one)
-        int cur = older(-1), ahead = cur;
-        if (cur < count) {
-            gamma = gam[cur];
-            load_pair(cur, s0, y0);
-            ahead = older(ahead);
-            load_pair(ahead < count ? ahead : cur, s1, y1);
-            ahead = ahead < count ? older(ahead) : count;
-            load_pair(ahead < count ? ahead : cur, s2, y2);
+        w[dws_scalars(cap) + 0] = gamma;
+        w[dws_scalars(cap) + 1] = used;
+        w[dws_scalars(cap) + 2] = 0.0;  // bad
+    }
+}
+
+// dts[a] = s^_a . r (Mode 0) or y^_a . z0 (Mode 1); one workgroup per pair (grid x = pair)
+template <int Mode>
+__global__ void __launch_bounds__(256) k_dir_dots(const OptDesc* __restrict__ descs, int n_act) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int a = blockIdx.x, cap = o.cap, N = o.N, n_bf = o.n_bf;
+    if (a >= o.count) return;  // block-uniform
+    const auto w = to_global(o.dws);
+    __shared__ double red[16];
+    double part = 0.0;
+    if (w[a] != 0.0) {  // rho: block-uniform
+        const int slot = (o.newest - a + cap) % cap;
+        if (Mode == 0) {
+            const auto s = to_global(static_cast<const double*>(o.hs)) + (size_t)slot * N;
+            const auto C = to_global(static_cast<const double*>(o.C));
+            const auto R = to_global(static_cast<const double*>(o.R));
+            for (int e = threadIdx.x; e < N; e += blockDim.x)
+                part += (s[e] - C[e] * sig_of(o, n_act, a, e / n_bf)) * R[e];
+        } else {
+            const auto y = to_global(static_cast<const double*>(o.hy)) + (size_t)slot * N;
+            const auto Sc = to_global(static_cast<const double*>(o.Sc));
+            const auto z = to_global(static_cast<const double*>(o.z));
+            for (int e = threadIdx.x; e < N; e += blockDim.x)
+                part += (y[e] - Sc[e] * bet_of(o, n_act, a, e / n_bf)) * z[e];
         }
-        auto first_step = [&](double (&sr)[kDirRegs], double (&yr)[kDirRegs]) {
-            const int a = cur;
-            ++used;
-            double part = 0.0;
-#pragma unroll
-            for (int m = 0; m < kDirRegs; ++m) {
-                const double t = (sr[m] - cr[m] * sig[a * n_act + orb[m]]) * qr[m];
-                part += in[m] ? t : 0.0;
-            }
-            const double alpha = rho[a] * block_total_lds(part, red);
-            if (!isfinite(alpha) && tid == 0) bad = 1;
-            if (tid == 0) alp[a] = alpha;
-#pragma unroll
-            for (int m = 0; m < kDirRegs; ++m) qr[m] -= alpha * (yr[m] - scr[m] * bet[a * n_act + orb[m]]);
-            ahead = ahead < count ? older(ahead) : count;
-            load_pair(ahead < count ? ahead : a, sr, yr);
-            cur = older(cur);
-        };
-        while (cur < count) {
-            first_step(s0, y0);
-            if (cur >= count) break;
-            first_step(s1, y1);
-            if (cur >= count) break;
-            first_step(s2, y2);
+    }
+    part = block_total(part, red);
+    if (threadIdx.x == 0) w[4 * (size_t)cap + a] = part;
+}
+
+// SY[a][b] = s^_a . y^_b = sum_i Gamma[a][b][i] - sig_a,i bet_b,i for the pairs kept (0 otherwise)
+__global__ void __launch_bounds__(256) k_dir_sy(const OptDesc* __restrict__ descs, int n_act) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int cap = o.cap, count = o.count, newest = o.newest;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * count) return;
+    const int a = t / count, b = t - a * count;
+    const auto w = to_global(static_cast<const double*>(o.dws));
+    double v = 0.0;
+    if (w[a] != 0.0 && w[b] != 0.0) {
+        const auto g = to_global(static_cast<const double*>(o.gsy)) +
+                       ((size_t)((newest - a + cap) % cap) * cap + (newest - b + cap) % cap) * n_act;
+        for (int i = 0; i < n_act; ++i) v += g[i] - sig_of(o, n_act, a, i) * bet_of(o, n_act, b, i);
+    }
+    to_global(o.sy)[(size_t)a * cap + b] = v;
+}
+
+// The two loops on scalars, one wavefront per problem: lane l owns pairs a = l + 64 r (registers),
+// and a pair's accumulator is broadcast by readlane when its turn comes (column by column).
+//   Mode 0 (newest to oldest): alpha_b = rho_b acc_b, acc_a -= SY[a][b] alpha_b for a older
+//   Mode 1 (oldest to newest): coef_d = alpha_d - rho_d acc_d, acc_a += SY[d][a] coef_d for a newer
+// Staged: SY copied to LDS first (count <= kSyLdsMax), row a at a * count.
+constexpr int kSyLdsMax = 128;
+
+template <int Mode, bool Staged>
+__global__ void __launch_bounds__(64) k_dir_rec(const OptDesc* __restrict__ descs) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int cap = o.cap, count = o.count, lane = threadIdx.x;
+    const auto w = to_global(o.dws);
+    const auto syg = to_global(static_cast<const double*>(o.sy));
+    extern __shared__ double syl[];
+    if (Staged) {
+        for (int t = lane; t < count * count; t += 64) {
+            const int a = t / count, b = t - a * count;
+            syl[t] = syg[(size_t)a * cap + b];
         }
-#pragma unroll
-        for (int m = 0; m < kDirRegs; ++m)
-            if (in[m]) q[tid + m * nt] = qr[m];
         __syncthreads();
-        double zr[kDirRegs];
+    }
+    auto sy_at = [&](int a, int b) { return Staged ? syl[a * count + b] : syg[(size_t)a * cap + b]; };
+    const auto rho = w, alp = w + 2 * (size_t)cap, cof = w + 3 * (size_t)cap, dts = w + 4 * (size_t)cap;
+    double acc[4];
 #pragma unroll
-        for (int m = 0; m < kDirRegs; ++m) {  // z = gamma S_i^-1 q_i
-            const int e = tid + m * nt;
-            zr[m] = 0.0;
-            if (e < N) {
-                const int i = orb[m], k = e - i * n_bf;
-                const auto P = Sinv + (size_t)pat[i] * n_bf * n_bf;
-                const auto qi = q + (size_t)i * n_bf;
-                double acc = 0.0;
-                for (int j = 0; j < n_bf; ++j) acc += P[k + (size_t)n_bf * j] * qi[j];
-                zr[m] = gamma * acc;
+    for (int r = 0; r < 4; ++r) acc[r] = lane + 64 * r < count ? dts[lane + 64 * r] : 0.0;
+    bool bad = false;
+    auto take = [&](int b) {  // acc of pair b, wave-uniform
+        const int r = b >> 6, l = b & 63;
+        const double v = r == 0 ? acc[0] : r == 1 ? acc[1] : r == 2 ? acc[2] : acc[3];
+        return lane_read(v, l);
+    };
+    if (Mode == 0) {
+        for (int b = 0; b < count; ++b) {
+            const double ab = rho[b] * take(b);
+            bad |= !isfinite(ab);
+            if (lane == 0) alp[b] = ab;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = lane + 64 * r;
+                if (a > b && a < count) acc[r] -= sy_at(a, b) * ab;
             }
         }
-        // second loop, oldest to newest
-        cur = newer(count);
-        ahead = cur;
-        if (cur >= 0) {
-            load_pair(cur, s0, y0);
-            ahead = newer(ahead);
-            load_pair(ahead >= 0 ? ahead : cur, s1, y1);
-            ahead = ahead >= 0 ? newer(ahead) : -1;
-            load_pair(ahead >= 0 ? ahead : cur, s2, y2);
-        }
-        auto second_step = [&](double (&sr)[kDirRegs], double (&yr)[kDirRegs]) {
-            const int a = cur;
-            double part = 0.0;
-#pragma unroll
-            for (int m = 0; m < kDirRegs; ++m) {
-                const double t = (yr[m] - scr[m] * bet[a * n_act + orb[m]]) * zr[m];
-                part += in[m] ? t : 0.0;
-            }
-            const double bb = rho[a] * block_total_lds(part, red);
-            if (!isfinite(bb) && tid == 0) bad = 1;
-            const double coef = alp[a] - bb;
-#pragma unroll
-            for (int m = 0; m < kDirRegs; ++m) zr[m] += (sr[m] - cr[m] * sig[a * n_act + orb[m]]) * coef;
-            ahead = ahead >= 0 ? newer(ahead) : -1;
-            load_pair(ahead >= 0 ? ahead : a, sr, yr);
-            cur = newer(cur);
-        };
-        while (cur >= 0) {
-            second_step(s0, y0);
-            if (cur < 0) break;
-            second_step(s1, y1);
-            if (cur < 0) break;
-            second_step(s2, y2);
-        }
-#pragma unroll
-        for (int m = 0; m < kDirRegs; ++m)
-            if (in[m]) z[tid + m * nt] = zr[m];
-        __syncthreads();
     } else {
-    for (int e = tid; e < N; e += nt) q[e] = R[e];
-    __syncthreads();
-    bool have_newest = false;
-    for (int a = 0; a < count; ++a) {  // newest to oldest
-        if (rho[a] == 0.0) continue;
-        if (!have_newest) {
-            gamma = gam[a];
-            have_newest = true;
+        for (int d = count - 1; d >= 0; --d) {
+            const double bd = rho[d] * take(d);
+            const double c = alp[d] - bd;
+            bad |= !isfinite(bd);
+            if (lane == 0) cof[d] = c;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = lane + 64 * r;
+                if (a < d) acc[r] += sy_at(d, a) * c;
+            }
         }
-        ++used;
-        const int slot = (newest - a + cap) % cap;
-        const auto s = hs + (size_t)slot * N;
-        const auto y = hy + (size_t)slot * N;
-        double part = 0.0;
-        for (int e = tid; e < N; e += nt) part += (s[e] - C[e] * sig[a * n_act + e / n_bf]) * q[e];
-        const double alpha = rho[a] * block_total(part, red);
-        if (!isfinite(alpha) && tid == 0) bad = 1;
-        if (tid == 0) alp[a] = alpha;
-        for (int e = tid; e < N; e += nt) q[e] -= alpha * (y[e] - Sc[e] * bet[a * n_act + e / n_bf]);
     }
+    if (__ballot(bad) != 0 && lane == 0) w[dws_scalars(cap) + 2] = 1.0;
+}
+
+// Mode 0: q = r - sum_b alpha_b y^_b; Mode 1: t = -(z0 + sum_c coef_c s^_c) with per-workgroup
+// partials of g.t and max |t|. A workgroup owns kCombineElems elements; its four wavefronts take
+// interleaved quarters of the pairs, whose sums are added in wavefront order.
+template <int Mode>
+__global__ void __launch_bounds__(256) k_dir_combine(const OptDesc* __restrict__ descs, int n_act) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int cap = o.cap, count = o.count, newest = o.newest, N = o.N, n_bf = o.n_bf;
+    if (blockIdx.x * kCombineElems >= N) return;  // block-uniform: past this problem's elements
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int e = blockIdx.x * kCombineElems + lane;
+    const int e_c = e < N ? e : N - 1;
+    const int i = e_c / n_bf;
+    const auto w = to_global(o.dws);
+    const auto coefs = Mode == 0 ? w + 2 * (size_t)cap : w + 3 * (size_t)cap;
+    const auto vec = to_global(static_cast<const double*>(Mode == 0 ? o.hy : o.hs));
+    const double base = Mode == 0 ? to_global(static_cast<const double*>(o.Sc))[e_c]
+                                  : to_global(static_cast<const double*>(o.C))[e_c];
+    __shared__ double part[4][kCombineElems];
+    double v = 0.0;
+    for (int b = wave; b < count; b += 4) {
+        const double cb = coefs[b];
+        const double pr = Mode == 0 ? bet_of(o, n_act, b, i) : sig_of(o, n_act, b, i);
+        v += cb * (vec[(size_t)((newest - b + cap) % cap) * N + e_c] - base * pr);
+    }
+    part[wave][lane] = v;
     __syncthreads();
-    for (int e = tid; e < N; e += nt) {  // z = gamma S_i^-1 q_i
-        const int i = e / n_bf, k = e - i * n_bf;
-        const auto P = Sinv + (size_t)pat[i] * n_bf * n_bf;
-        const auto qi = q + (size_t)i * n_bf;
-        double acc = 0.0;
-        for (int j = 0; j < n_bf; ++j) acc += P[k + (size_t)n_bf * j] * qi[j];
-        z[e] = gamma * acc;
-    }
-    __syncthreads();
-    for (int a = count - 1; a >= 0; --a) {  // oldest to newest
-        if (rho[a] == 0.0) continue;
-        const int slot = (newest - a + cap) % cap;
-        const auto s = hs + (size_t)slot * N;
-        const auto y = hy + (size_t)slot * N;
-        double part = 0.0;
-        for (int e = tid; e < N; e += nt) part += (y[e] - Sc[e] * bet[a * n_act + e / n_bf]) * z[e];
-        const double b = rho[a] * block_total(part, red);
-        if (!isfinite(b) && tid == 0) bad = 1;
-        const double coef = alp[a] - b;
-        for (int e = tid; e < N; e += nt) z[e] += (s[e] - C[e] * sig[a * n_act + e / n_bf]) * coef;
-    }
-    __syncthreads();
-    }
+    if (wave != 0) return;
+    const double sum = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
     double dd = 0.0, tinf = 0.0;
-    for (int e = tid; e < N; e += nt) {
-        const double t = -z[e];
+    if (Mode == 0) {
+        if (e < N) to_global(o.q)[e] = to_global(static_cast<const double*>(o.R))[e] - sum;
+    } else if (e < N) {
+        const double t = -(to_global(static_cast<const double*>(o.z))[e] + sum);
         to_global(o.T)[e] = t;
-        dd += G[e] * t;
-        tinf = fmax(tinf, fabs(t));
+        dd = to_global(static_cast<const double*>(o.G))[e] * t;
+        tinf = fabs(t);
     }
-    dd = block_total(dd, red);
-    tinf = block_largest(tinf, red);
-    if (tid == 0) {
-        const auto out = to_global(o.out);
-        out[0] = dd;
-        out[1] = tinf;
-        out[2] = used;
-        out[3] = gamma;
-        out[4] = bad;
+    if (Mode == 1) {
+        dd = wave_total(dd);
+        tinf = wave_largest(tinf);
+        if (lane == 0) {
+            w[dws_partials(cap) + 2 * (size_t)blockIdx.x] = dd;
+            w[dws_partials(cap) + 2 * (size_t)blockIdx.x + 1] = tinf;
+        }
+    }
+}
+
+// z0 = gamma S_i^-1 q_i, one thread per element
+__global__ void __launch_bounds__(256) k_dir_precond(const OptDesc* __restrict__ descs) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= o.N) return;
+    const int n_bf = o.n_bf, i = e / n_bf, k = e - i * n_bf;
+    const auto P = to_global(o.Sinv) + (size_t)to_global(o.pat)[i] * n_bf * n_bf;
+    const auto qi = to_global(static_cast<const double*>(o.q)) + (size_t)i * n_bf;
+    double acc = 0.0;
+    for (int j = 0; j < n_bf; ++j) acc += P[k + (size_t)n_bf * j] * qi[j];
+    to_global(o.z)[e] = to_global(static_cast<const double*>(o.dws))[dws_scalars(o.cap)] * acc;
+}
+
+// out[0..4] = g.t, max |t|, pairs used, gamma, non-finite flag
+__global__ void __launch_bounds__(64) k_dir_out(const OptDesc* __restrict__ descs) {
+    const OptDesc& o = descs[blockIdx.z];
+    if (threadIdx.x != 0) return;
+    const auto w = to_global(static_cast<const double*>(o.dws));
+    const int blocks = (o.N + kCombineElems - 1) / kCombineElems;
+    double dd = 0.0, tinf = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+        dd += w[dws_partials(o.cap) + 2 * (size_t)b];
+        tinf = fmax(tinf, w[dws_partials(o.cap) + 2 * (size_t)b + 1]);
+    }
+    const auto out = to_global(o.out);
+    out[0] = dd;
+    out[1] = tinf;
+    out[2] = w[dws_scalars(o.cap) + 1];
+    out[3] = w[dws_scalars(o.cap)];
+    out[4] = w[dws_scalars(o.cap) + 2];
+}
+
+size_t dws_doubles(int cap, int N) { return (size_t)5 * cap + 4 + 2 * (size_t)((N + kCombineElems - 1) / kCombineElems); }
+
+// Gram entries of a new pair (append time): Gamma[n][b][i] = s_n,i . y_b,i and
+// Gamma[b][n][i] = s_b,i . y_n,i for the pairs b still kept after the append, and Gamma[n][n][i].
+// Grid x = kept pairs + 1 (the last block: the pair with itself), one wavefront per orbital.
+__global__ void __launch_bounds__(256) k_opt_gram(const OptDesc* __restrict__ descs, int n_act) {
+    const OptDesc& o = descs[blockIdx.z];
+    const int cap = o.cap, N = o.N, n_bf = o.n_bf;
+    const int kept = min(o.count, cap - 1), x = blockIdx.x;
+    if (x > kept) return;  // block-uniform
+    const int sn = o.slot, sb = x == kept ? sn : (o.newest - x + cap) % cap;
+    const auto hs = to_global(static_cast<const double*>(o.hs));
+    const auto hy = to_global(static_cast<const double*>(o.hy));
+    const auto gsy = to_global(o.gsy);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int i = wave; i < n_act; i += nw) {
+        const auto s_n = hs + (size_t)sn * N + (size_t)i * n_bf, y_n = hy + (size_t)sn * N + (size_t)i * n_bf;
+        const auto s_b = hs + (size_t)sb * N + (size_t)i * n_bf, y_b = hy + (size_t)sb * N + (size_t)i * n_bf;
+        double d1 = 0.0, d2 = 0.0;
+        for (int k = lane; k < n_bf; k += 64) {
+            d1 += s_n[k] * y_b[k];
+            d2 += s_b[k] * y_n[k];
+        }
+        d1 = wave_total(d1);
+        d2 = wave_total(d2);
+        if (lane == 0) {
+            gsy[((size_t)sn * cap + sb) * n_act + i] = d1;
+            gsy[((size_t)sb * cap + sn) * n_act + i] = d2;
+        }
     }
 }
 
@@ -655,6 +675,7 @@ struct Problem {
     // device state
     double *S = nullptr, *Sinv = nullptr, *Sc = nullptr, *R = nullptr, *T = nullptr, *q = nullptr, *z = nullptr;
     double* apx = nullptr;  // per-orbital secant sums between the two append kernels
+    double *gsy = nullptr, *sy = nullptr, *dws = nullptr;  // ring Gram entries, SY, direction workspace
     double *stats = nullptr, *out = nullptr, *hs = nullptr, *hy = nullptr, *hstat = nullptr, *proj = nullptr;
     int* d_pat = nullptr;
     uint8_t* d_support = nullptr;
@@ -698,6 +719,8 @@ void optimizer_defaults(vbe_optimizer_options* o) {
 
 void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, const vbe_optimizer_options& opt) {
     if (opt.max_iterations < 0 || opt.history_size < 0) throw std::invalid_argument("negative iteration or history count");
+    if (opt.history_size > kPairsMax)
+        throw std::invalid_argument("at most " + std::to_string(kPairsMax) + " secant pairs (history_size)");
     if (!(opt.gradient_tolerance >= 0.0) || !(opt.energy_tolerance >= 0.0) || !(opt.minimum_step_size > 0.0) ||
         !(opt.initial_step_size > 0.0) || !(opt.armijo_constant > 0.0 && opt.armijo_constant < 1.0))
         throw std::invalid_argument("optimizer tolerances and step sizes must be positive (Armijo constant in (0,1))");
@@ -710,6 +733,10 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
     constexpr double kEps = std::numeric_limits<double>::epsilon();
     const double zero_step = 128.0 * kEps;
     const int cap = std::max(opt.history_size, 1);
+    for (const void* f : {reinterpret_cast<const void*>(&k_dir_rec<0, true>), reinterpret_cast<const void*>(&k_dir_rec<1, true>)})
+        hip_check(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(sizeof(double) * kSyLdsMax * kSyLdsMax)),
+                  "hipFuncSetAttribute");
 
     // support patterns, restricted inverse overlaps (host, once per distinct S and pattern)
     std::vector<Problem> P(np);
@@ -745,7 +772,9 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
                  11 * DeviceBlock::bytes_for(s.N, 8) + DeviceBlock::bytes_for((size_t)kOrbStats * n_act, 8) +
                  DeviceBlock::bytes_for(kOutSlots, 8) + 2 * DeviceBlock::bytes_for((size_t)cap * s.N, 8) +
                  DeviceBlock::bytes_for((size_t)cap * n_act * 3, 8) + DeviceBlock::bytes_for((size_t)cap * n_act * 2, 8) +
-                 DeviceBlock::bytes_for((size_t)6 * n_act, 8);
+                 DeviceBlock::bytes_for((size_t)6 * n_act, 8) +
+                 DeviceBlock::bytes_for((size_t)cap * cap * n_act, 8) + DeviceBlock::bytes_for((size_t)cap * cap, 8) +
+                 DeviceBlock::bytes_for(dws_doubles(cap, s.N), 8);
     }
     {  // the inverses, spread over host threads
         std::vector<std::pair<int, int>> jobs;
@@ -811,6 +840,9 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
         s.q = dev.take<double>(s.N);
         s.z = dev.take<double>(s.N);
         s.apx = dev.take<double>((size_t)6 * n_act);
+        s.gsy = dev.take<double>((size_t)cap * cap * n_act);
+        s.sy = dev.take<double>((size_t)cap * cap);
+        s.dws = dev.take<double>(dws_doubles(cap, s.N));
         s.stats = d_scal + scal_per * k;
         s.out = s.stats + (size_t)kOrbStats * n_act;
         s.h_stats = h_scal + scal_per * k;
@@ -858,6 +890,9 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
             d.T = s.T;
             d.q = s.q;
             d.apx = s.apx;
+            d.gsy = s.gsy;
+            d.sy = s.sy;
+            d.dws = s.dws;
             d.z = s.z;
             d.stats = s.stats;
             d.out = s.out;
@@ -890,6 +925,10 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
                                sizeof(double) * 2 * max_n_bf, st, d_desc);
             hipLaunchKernelGGL(k_opt_append_finish, dim3(1, 1, (unsigned)list.size()), dim3(64), 0, st, d_desc,
                                n_act);
+            int kept = 0;
+            for (int k : list) kept = std::max(kept, std::min(P[k].use, cap - 1));
+            hipLaunchKernelGGL(k_opt_gram, dim3((unsigned)kept + 1, 1, (unsigned)list.size()), dim3(256), 0, st,
+                               d_desc, n_act);
             hip_check(hipGetLastError(), "k_opt_append");
         }
         read_back();
@@ -910,15 +949,32 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
         describe(list);
         int max_use = 0;
         for (int k : list) max_use = std::max(max_use, P[k].use);
-        const size_t lds = sizeof(double) * ((size_t)2 * max_use * n_act + 3 * (size_t)max_use);
+        const unsigned nz = (unsigned)list.size();
+        const bool staged = max_use <= kSyLdsMax;
+        const size_t sy_lds = staged ? sizeof(double) * (size_t)max_use * max_use : 0;
         if (max_use > 0) {
-            hipLaunchKernelGGL(k_opt_project, dim3((unsigned)((max_use * n_act + 3) / 4), 2, (unsigned)list.size()),
-                               dim3(256), 0, st, d_desc, n_act);
-            hip_check(hipGetLastError(), "k_opt_project");
+            hipLaunchKernelGGL(k_opt_project, dim3((unsigned)((max_use * n_act + 3) / 4), 2, nz), dim3(256), 0, st,
+                               d_desc, n_act);
         }
-        hipLaunchKernelGGL(k_opt_direction, dim3(1, 1, (unsigned)list.size()), dim3(kDirThreads), lds, st, d_desc,
-                           n_act);
-        hip_check(hipGetLastError(), "k_opt_direction");
+        hipLaunchKernelGGL(k_dir_prep, dim3(1, 1, nz), dim3(256), 0, st, d_desc, n_act);
+        if (max_use > 0) {
+            hipLaunchKernelGGL(k_dir_dots<0>, dim3((unsigned)max_use, 1, nz), dim3(256), 0, st, d_desc, n_act);
+            hipLaunchKernelGGL(k_dir_sy, dim3((unsigned)((max_use * max_use + 255) / 256), 1, nz), dim3(256), 0, st,
+                               d_desc, n_act);
+            if (staged) hipLaunchKernelGGL((k_dir_rec<0, true>), dim3(1, 1, nz), dim3(64), sy_lds, st, d_desc);
+            else hipLaunchKernelGGL((k_dir_rec<0, false>), dim3(1, 1, nz), dim3(64), 0, st, d_desc);
+        }
+        const unsigned cblocks = (unsigned)((max_N + kCombineElems - 1) / kCombineElems);
+        hipLaunchKernelGGL(k_dir_combine<0>, dim3(cblocks, 1, nz), dim3(256), 0, st, d_desc, n_act);
+        hipLaunchKernelGGL(k_dir_precond, dim3((unsigned)((max_N + 255) / 256), 1, nz), dim3(256), 0, st, d_desc);
+        if (max_use > 0) {
+            hipLaunchKernelGGL(k_dir_dots<1>, dim3((unsigned)max_use, 1, nz), dim3(256), 0, st, d_desc, n_act);
+            if (staged) hipLaunchKernelGGL((k_dir_rec<1, true>), dim3(1, 1, nz), dim3(64), sy_lds, st, d_desc);
+            else hipLaunchKernelGGL((k_dir_rec<1, false>), dim3(1, 1, nz), dim3(64), 0, st, d_desc);
+        }
+        hipLaunchKernelGGL(k_dir_combine<1>, dim3(cblocks, 1, nz), dim3(256), 0, st, d_desc, n_act);
+        hipLaunchKernelGGL(k_dir_out, dim3(1, 1, nz), dim3(64), 0, st, d_desc);
+        hip_check(hipGetLastError(), "direction kernels");
         read_back();
         for (int k : list) {
             Problem& s = P[k];
