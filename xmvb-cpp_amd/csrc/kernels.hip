@@ -68,25 +68,19 @@ __global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
     }
     __syncthreads();
     double q = 0.0;
-    {
-        // (S c)[mu] = sum_nu S[nu, mu] c[nu] (S and h are symmetric): one wavefront per row mu,
-        // lanes over nu read contiguous columns
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        for (int mu = wave; mu < n_bf; mu += nw) {
-            double sv = 0.0, hv = 0.0;
-            for (int nu = lane; nu < n_bf; nu += 64) {
-                const double cv = c[nu];
-                sv += S[nu + (size_t)n_bf * mu] * cv;
-                hv += h[nu + (size_t)n_bf * mu] * cv;
-            }
-            sv = wave_sum(sv);
-            hv = wave_sum(hv);
-            if (lane == 0) {
-                SC[mu + (size_t)n_bf * k] = sv;
-                HC[mu + (size_t)n_bf * k] = hv;
-                q += c[mu] * sv;
-            }
+    // (S c)[mu] = sum_nu S[mu, nu] c[nu]: one thread per row mu, consecutive threads on consecutive
+    // rows of each column (coalesced), c[nu] an LDS broadcast; no cross-lane reduction, and the
+    // loads of successive nu are independent of the accumulation, so they stay in flight
+    for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
+        double sv = 0.0, hv = 0.0;
+        for (int nu = 0; nu < n_bf; ++nu) {
+            const double cv = c[nu];
+            sv += S[mu + (size_t)n_bf * nu] * cv;
+            hv += h[mu + (size_t)n_bf * nu] * cv;
         }
+        SC[mu + (size_t)n_bf * k] = sv;
+        HC[mu + (size_t)n_bf * k] = hv;
+        q += c[mu] * sv;
     }
     q = block_sum(q, red);
     const bool ok = isfinite(q) && q > 2.220446049250313e-16;
@@ -1991,8 +1985,7 @@ __global__ void __launch_bounds__(256) k_orbital_gradient(const SlotDesc* __rest
             const double* w = wq + qq * NQ * 64 + lane;
 #pragma unroll
             for (int p = 0; p < NQ; ++p) {
-                if (p >= n) continue;
-                const double wp = w[p * 64];
+                const double wp = p < n ? w[p * 64] : 0.0;  // select, not branch: the LDS reads stay batched
 #pragma unroll
                 for (int u = 0; u < R; ++u) acc[u][p] += wp * t[u];
             }
@@ -2048,10 +2041,14 @@ __global__ void __launch_bounds__(256) k_orbital_gradient_few(const SlotDesc* __
             double t[kGradRows];
 #pragma unroll
             for (int u = 0; u < kGradRows; ++u) t[u] = tmu[u][(size_t)n_aux * q + A];
+            // every weight is loaded unconditionally (p past n clamped, its weight zeroed by a select):
+            // loads under a branch would each wait for all loads before them
+            double wv[NQ];
+#pragma unroll
+            for (int p = 0; p < NQ; ++p) wv[p] = dL[(size_t)A + (size_t)n_aux * pair_index(p < n ? p : n - 1, q)];
 #pragma unroll
             for (int p = 0; p < NQ; ++p) {
-                if (p >= n) continue;
-                const double w = (p == q ? 2.0 : 1.0) * dL[(size_t)A + (size_t)n_aux * pair_index(p, q)];
+                const double w = p < n ? (p == q ? 2.0 : 1.0) * wv[p] : 0.0;
 #pragma unroll
                 for (int u = 0; u < kGradRows; ++u) acc[u][p] += w * t[u];
             }
