@@ -142,12 +142,20 @@ def serve(vb, ctx, scans, rate: float, seconds: float, max_batch: int, replicas:
             for k in range(replicas):
                 server.register((si, p, k), topo, loader(sp))
     # direct reference energies: every point of a scan in one batched launch of a pool of its own
-    e_ref = {}
+    e_ref, alone_ms = {}, {}
     for si, (_, topo, sps) in enumerate(scans):
         ref_pool = vb.Pool(ctx, topo, capacity=len(sps))
         for p, sp in enumerate(sps):
             ref_pool.load_synthetic(p, sp)
         e, _ = ref_pool.evaluate(list(range(len(sps))), [sp.C for sp in sps], gradient=False)
+        # one problem of this size evaluated alone (E+grad, host orbitals in and gradient out, as the
+        # served requests): the yardstick of the served latencies
+        ts = []
+        for rep in range(6):
+            t = time.perf_counter()
+            ref_pool.evaluate([rep % len(sps)], [sps[rep % len(sps)].C], gradient=True)
+            ts.append((time.perf_counter() - t) * 1e3)
+        alone_ms[si] = float(np.median(ts[1:]))
         ref_pool.close()
         for p in range(len(sps)):
             e_ref[(si, p)] = e[p]
@@ -165,6 +173,7 @@ def serve(vb, ctx, scans, rate: float, seconds: float, max_batch: int, replicas:
         for si, (_, topo, sps) in enumerate(scans):
             for p, sp in enumerate(sps):
                 worker.register((si, p), topo, loader(sp))
+        worker.preload()  # resident before the window, like the served problems
         crng = np.random.default_rng(seed + 1)
         n_conv = max(1, int(converge_rate * seconds))
         c_arrivals = np.cumsum(crng.exponential(1.0 / converge_rate, n_conv))
@@ -230,8 +239,10 @@ def serve(vb, ctx, scans, rate: float, seconds: float, max_batch: int, replicas:
     for si, (name, topo, sps) in enumerate(scans):
         ls = [r.latency * 1e3 for r in reqs if r.key[0] == si]
         if ls:
-            per_scan[name] = dict(requests=len(ls), n_act=topo.n_act, n_bf=sps[0].n_bf,
-                                  p50=percentile(ls, 0.5), p99=percentile(ls, 0.99))
+            p50, p99 = percentile(ls, 0.5), percentile(ls, 0.99)
+            per_scan[name] = dict(requests=len(ls), n_act=topo.n_act, n_bf=sps[0].n_bf, p50=p50, p99=p99,
+                                  alone_ms=alone_ms[si], p50_over_alone=p50 / alone_ms[si],
+                                  p99_over_alone=p99 / alone_ms[si])
     used, budget = ctx.memory()
     launches = server.launches - launches0
     out = dict(requests=n_req, offered_rate=rate, throughput=n_req / makespan, seconds=makespan,

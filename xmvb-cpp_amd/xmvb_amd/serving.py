@@ -256,6 +256,13 @@ class ConvergenceWorker:
         self._wake.set()
         return r
 
+    def preload(self, keys=None):
+        """Admit problems ahead of their requests (all registered ones by default), so that pool
+        creation and factor uploads do not fall inside a serving window."""
+        with self._lock:
+            for key in (list(self.res.problems) if keys is None else keys):
+                self.res.admit(self.res.problems[key])
+
     def start(self):
         self._thread = threading.Thread(target=self._run, daemon=True)
         self._thread.start()
