@@ -160,7 +160,6 @@ __global__ void __launch_bounds__(kPointThreads) k_opt_point(const OptDesc* __re
     __shared__ double red[16];
     const int i = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const size_t row = (size_t)i * n_bf;
-    const double* __restrict__ S = o.S;
     const uint8_t* support = o.support;
     double* __restrict__ C = o.C;
     double* __restrict__ G = o.G;
@@ -168,10 +167,14 @@ __global__ void __launch_bounds__(kPointThreads) k_opt_point(const OptDesc* __re
     for (int k = tid; k < n_bf; k += nt) cs[k] = on_support(support, row + k) ? C[row + k] : 0.0;
     __syncthreads();
     double part = 0.0;
+    const auto Sg = to_global(o.S);
     for (int k = tid; k < n_bf; k += nt) {
         double a = 0.0;
-        if (on_support(support, row + k))
-            for (int j = 0; j < n_bf; ++j) a += S[k + (size_t)n_bf * j] * cs[j];
+        // unconditional, unrolled: the S column loads of several j in flight (off the support the
+        // result is discarded by the select below)
+#pragma unroll 8
+        for (int j = 0; j < n_bf; ++j) a += Sg[k + (size_t)n_bf * j] * cs[j];
+        a = on_support(support, row + k) ? a : 0.0;
         Sc[row + k] = a;
         part += cs[k] * a;
     }
@@ -199,10 +202,11 @@ __global__ void __launch_bounds__(kPointThreads) k_opt_point(const OptDesc* __re
         rs[k] = r;
     }
     __syncthreads();
-    const double* P = o.Sinv + (size_t)o.pat[i] * n_bf * n_bf;
+    const auto P = to_global(o.Sinv) + (size_t)to_global(o.pat)[i] * n_bf * n_bf;
     double npart = 0.0;
     for (int k = tid; k < n_bf; k += nt) {
         double a = 0.0;
+#pragma unroll 8
         for (int j = 0; j < n_bf; ++j) a += P[k + (size_t)n_bf * j] * rs[j];
         npart += rs[k] * a;
     }
@@ -366,15 +370,28 @@ __global__ void __launch_bounds__(64) k_dir_rec(const OptDesc* __restrict__ desc
     const auto w = to_global(o.dws);
     const auto syg = to_global(static_cast<const double*>(o.sy));
     extern __shared__ double syl[];
-    if (Staged) {
-        for (int t = lane; t < count * count; t += 64) {
-            const int a = t / count, b = t - a * count;
-            syl[t] = syg[(size_t)a * cap + b];
-        }
-        __syncthreads();
+    __shared__ double rho_l[kPairsMax], alp_l[kPairsMax];  // per-step scalars from LDS, not global loads
+    for (int t = lane; t < count; t += 64) {
+        rho_l[t] = w[t];
+        if (Mode == 1) alp_l[t] = w[2 * (size_t)cap + t];
     }
+    if (Staged) {  // eight loads per lane in flight (indices past the end clamped), then the stores
+        const int total = count * count;
+        for (int t0 = 0; t0 < total; t0 += 64 * 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = min(t0 + u * 64 + lane, total - 1), a = t / count, b = t - a * count;
+                v[u] = syg[(size_t)a * cap + b];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (t0 + u * 64 + lane < total) syl[t0 + u * 64 + lane] = v[u];
+        }
+    }
+    __syncthreads();
     auto sy_at = [&](int a, int b) { return Staged ? syl[a * count + b] : syg[(size_t)a * cap + b]; };
-    const auto rho = w, alp = w + 2 * (size_t)cap, cof = w + 3 * (size_t)cap, dts = w + 4 * (size_t)cap;
+    const auto alp = w + 2 * (size_t)cap, cof = w + 3 * (size_t)cap, dts = w + 4 * (size_t)cap;
     double acc[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] = lane + 64 * r < count ? dts[lane + 64 * r] : 0.0;
@@ -386,7 +403,7 @@ __global__ void __launch_bounds__(64) k_dir_rec(const OptDesc* __restrict__ desc
     };
     if (Mode == 0) {
         for (int b = 0; b < count; ++b) {
-            const double ab = rho[b] * take(b);
+            const double ab = rho_l[b] * take(b);
             bad |= !isfinite(ab);
             if (lane == 0) alp[b] = ab;
 #pragma unroll
@@ -397,8 +414,8 @@ __global__ void __launch_bounds__(64) k_dir_rec(const OptDesc* __restrict__ desc
         }
     } else {
         for (int d = count - 1; d >= 0; --d) {
-            const double bd = rho[d] * take(d);
-            const double c = alp[d] - bd;
+            const double bd = rho_l[d] * take(d);
+            const double c = alp_l[d] - bd;
             bad |= !isfinite(bd);
             if (lane == 0) cof[d] = c;
 #pragma unroll
@@ -458,17 +475,24 @@ __global__ void __launch_bounds__(256) k_dir_combine(const OptDesc* __restrict__
     }
 }
 
-// z0 = gamma S_i^-1 q_i, one thread per element
+// z0 = gamma S_i^-1 q_i: one workgroup per orbital (grid x), q_i staged in LDS, one thread per
+// coefficient with the S_i^-1 column loads unrolled so that several are in flight
 __global__ void __launch_bounds__(256) k_dir_precond(const OptDesc* __restrict__ descs) {
     const OptDesc& o = descs[blockIdx.z];
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= o.N) return;
-    const int n_bf = o.n_bf, i = e / n_bf, k = e - i * n_bf;
-    const auto P = to_global(o.Sinv) + (size_t)to_global(o.pat)[i] * n_bf * n_bf;
+    const int n_bf = o.n_bf, i = blockIdx.x;
+    if (i * n_bf >= o.N) return;  // block-uniform: past this problem's orbitals
+    extern __shared__ double ql[];
     const auto qi = to_global(static_cast<const double*>(o.q)) + (size_t)i * n_bf;
-    double acc = 0.0;
-    for (int j = 0; j < n_bf; ++j) acc += P[k + (size_t)n_bf * j] * qi[j];
-    to_global(o.z)[e] = to_global(static_cast<const double*>(o.dws))[dws_scalars(o.cap)] * acc;
+    for (int j = threadIdx.x; j < n_bf; j += blockDim.x) ql[j] = qi[j];
+    __syncthreads();
+    const auto P = to_global(o.Sinv) + (size_t)to_global(o.pat)[i] * n_bf * n_bf;
+    const double gamma = to_global(static_cast<const double*>(o.dws))[dws_scalars(o.cap)];
+    for (int k = threadIdx.x; k < n_bf; k += blockDim.x) {
+        double acc = 0.0;
+#pragma unroll 8
+        for (int j = 0; j < n_bf; ++j) acc += P[k + (size_t)n_bf * j] * ql[j];
+        to_global(o.z)[(size_t)i * n_bf + k] = gamma * acc;
+    }
 }
 
 // out[0..4] = g.t, max |t|, pairs used, gamma, non-finite flag
@@ -966,7 +990,8 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
         }
         const unsigned cblocks = (unsigned)((max_N + kCombineElems - 1) / kCombineElems);
         hipLaunchKernelGGL(k_dir_combine<0>, dim3(cblocks, 1, nz), dim3(256), 0, st, d_desc, n_act);
-        hipLaunchKernelGGL(k_dir_precond, dim3((unsigned)((max_N + 255) / 256), 1, nz), dim3(256), 0, st, d_desc);
+        hipLaunchKernelGGL(k_dir_precond, dim3((unsigned)n_act, 1, nz), dim3(128), sizeof(double) * max_n_bf, st,
+                           d_desc);
         if (max_use > 0) {
             hipLaunchKernelGGL(k_dir_dots<1>, dim3((unsigned)max_use, 1, nz), dim3(256), 0, st, d_desc, n_act);
             if (staged) hipLaunchKernelGGL((k_dir_rec<1, true>), dim3(1, 1, nz), dim3(64), sy_lds, st, d_desc);
