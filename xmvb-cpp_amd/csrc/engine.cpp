@@ -193,6 +193,13 @@ struct vbe_pool {
     TopoDesc topo{};
     DeviceArena topo_arena;
     DeviceArena ri_stage_arena;  // staging of flat RI factors for re-layout at load
+    // the optimizer's device state, kept between calls and grown when a call needs more: a
+    // convergence request then neither allocates nor frees (hipFree waits for the whole device)
+    DeviceArena opt_arena;
+    void* opt_block = nullptr;
+    size_t opt_block_bytes = 0;
+    void* opt_pinned = nullptr;  // and its pinned read-back buffer
+    size_t opt_pinned_bytes = 0;
     double* ri_stage = nullptr;
     size_t ri_stage_doubles = 0;
     std::vector<Slot> slots;
@@ -224,6 +231,7 @@ struct vbe_pool {
             if (l.h_C) (void)hipHostFree(l.h_C);
             if (l.h_grad) (void)hipHostFree(l.h_grad);
         }
+        if (opt_pinned) (void)hipHostFree(opt_pinned);
     }
 };
 
@@ -318,6 +326,7 @@ std::unique_ptr<vbe_pool> make_pool(vbe_context* ctx, const vbe_topology_view& t
     p->n_beta = n_beta;
     p->capacity = capacity;
     p->topo_arena.ctx = ctx;
+    p->opt_arena.ctx = ctx;
     for (auto& l : p->lanes) {
         l.stream = ctx->take_stream();
         for (auto& e : l.ev) check(hipEventCreate(&e), "hipEventCreate");
@@ -681,7 +690,8 @@ void enqueue(vbe_pool* p, Lane& lane, const std::vector<SlotInput>& in, bool wit
     hipStream_t st = lane.stream;
     const LaunchShape sh = shape_for(p, in, pinned);
     // the side stream only when it is idle: queued behind another lane's launch it would delay this one
-    const xmvb_amd::SideStream* side = hipStreamQuery(lane.side.st) == hipSuccess ? &lane.side : nullptr;
+    const xmvb_amd::SideStream* side =
+        lane.side.st != st && hipStreamQuery(lane.side.st) == hipSuccess ? &lane.side : nullptr;
     if (forward) {
         std::vector<char> seen(p->capacity, 0);
         for (int i = 0; i < count; ++i) {
@@ -868,6 +878,26 @@ void optimize_slots(vbe_pool* p, vbe_problem* prob, const std::vector<int>& slot
             E[j] = status[j] == VBE_OK ? lane.h_energy[2 * j + 1] : std::numeric_limits<double>::quiet_NaN();
     };
     hooks.energy_only_trials = prob != nullptr;
+    hooks.device_block = [p](size_t bytes) -> void* {
+        if (p->opt_block_bytes < bytes) {
+            p->opt_arena.release();
+            p->opt_block = nullptr;
+            p->opt_block_bytes = 0;
+            p->opt_block = p->opt_arena.alloc<unsigned char>(bytes);
+            p->opt_block_bytes = bytes;
+        }
+        return p->opt_block;
+    };
+    hooks.host_block = [p](size_t bytes) -> void* {
+        if (p->opt_pinned_bytes < bytes) {
+            if (p->opt_pinned) (void)hipHostFree(p->opt_pinned);
+            p->opt_pinned = nullptr;
+            p->opt_pinned_bytes = 0;
+            check(hipHostMalloc(&p->opt_pinned, bytes, hipHostMallocDefault), "hipHostMalloc");
+            p->opt_pinned_bytes = bytes;
+        }
+        return p->opt_pinned;
+    };
     hooks.complete = [&](int, const double* C, double* G) {
         if (!prob) throw StatusError(VBE_ERR_ARGUMENT, "energy-only trials need a single problem");
         eval_problem(prob, C, lane.h_desc[0].support, true, G, false);
@@ -961,7 +991,7 @@ bool pool_slot_busy(vbe_pool* p, int slot) {
 }
 
 int pool_launch(vbe_pool* p, int count, const int* slots, const double* const* C, bool gradient, void* stream,
-                int* lane_out, std::string* err) {
+                void* side, int* lane_out, std::string* err) {
     std::lock_guard<std::recursive_mutex> lk(p->mu);
     *lane_out = -1;
     Lane* lane = nullptr;
@@ -971,8 +1001,12 @@ int pool_launch(vbe_pool* p, int count, const int* slots, const double* const* C
     try {
         check(hipSetDevice(p->ctx->device), "hipSetDevice");
         check_slots_free(p, slots, count);
-        // an idle lane may change streams: its events are recorded afresh on the new one
-        if (stream) lane->stream = static_cast<hipStream_t>(stream);
+        // an idle lane may change streams: its events are recorded afresh on the new one; with no
+        // side stream the side work is ordered on the launch's own stream
+        if (stream) {
+            lane->stream = static_cast<hipStream_t>(stream);
+            lane->side.st = side ? static_cast<hipStream_t>(side) : lane->stream;
+        }
         size_t total = 0;
         for (int i = 0; i < count; ++i) {
             if (slots[i] < 0 || slots[i] >= p->capacity || !p->slots[slots[i]].loaded)

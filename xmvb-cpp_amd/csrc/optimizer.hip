@@ -668,13 +668,25 @@ std::vector<double> restricted_inverse(const double* S, int n_bf, const std::vec
     return out;
 }
 
-// One allocation for the device state of every problem of a batch, freed on scope exit.
+// One block for the device state of every problem of a batch: the caller's (hooks.device_block)
+// when it provides one, else allocated here and freed on scope exit.
 struct DeviceBlock {
     void* base = nullptr;
     size_t used = 0, size = 0;
-    explicit DeviceBlock(size_t bytes) : size(bytes) { hip_check(hipMalloc(&base, std::max<size_t>(bytes, 1)), "hipMalloc"); }
+    bool owned = false;
+    DeviceBlock(size_t bytes, const std::function<void*(size_t)>& provider) : size(std::max<size_t>(bytes, 1)) {
+        if (provider) {
+            base = provider(size);
+            if (!base) throw std::runtime_error("optimizer device block unavailable");
+        } else {
+            hip_check(hipMalloc(&base, size), "hipMalloc");
+            owned = true;
+        }
+    }
+    DeviceBlock(const DeviceBlock&) = delete;
+    DeviceBlock& operator=(const DeviceBlock&) = delete;
     ~DeviceBlock() {
-        if (base) (void)hipFree(base);
+        if (owned && base) (void)hipFree(base);
     }
     template <typename T>
     T* take(size_t count) {
@@ -827,25 +839,31 @@ void run_plbfgs(const OptimizerHooks& h, std::vector<OptProblem>& problems, cons
         if (failure) std::rethrow_exception(failure);
     }
 
-    DeviceBlock dev(bytes);
     const size_t scal_per = (size_t)kOrbStats * n_act + kOutSlots;
-    double* h_scal = nullptr;  // pinned read-back of every problem's stats and out
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_scal), sizeof(double) * scal_per * np, hipHostMallocDefault),
-              "hipHostMalloc");
-    OptDesc* h_desc = nullptr;
-    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_desc), sizeof(OptDesc) * np, hipHostMallocDefault),
-              "hipHostMalloc");
+    const size_t scal_bytes = DeviceBlock::bytes_for(scal_per * np, 8), desc_bytes = DeviceBlock::bytes_for((size_t)np, sizeof(OptDesc));
+    DeviceBlock dev(bytes + scal_bytes + desc_bytes, h.device_block);
+    // pinned read-back of every problem's stats and out, and the descriptors' host copy: the
+    // caller's block (hooks.host_block) when it provides one, else allocated here
+    void* pinned = nullptr;
+    bool pinned_owned = false;
+    if (h.host_block) {
+        pinned = h.host_block(scal_bytes + desc_bytes);
+        if (!pinned) throw std::runtime_error("optimizer host block unavailable");
+    } else {
+        hip_check(hipHostMalloc(&pinned, scal_bytes + desc_bytes, hipHostMallocDefault), "hipHostMalloc");
+        pinned_owned = true;
+    }
     struct PinnedFree {
         void* a;
-        void* b;
+        bool owned;
         ~PinnedFree() {
-            (void)hipHostFree(a);
-            (void)hipHostFree(b);
+            if (owned) (void)hipHostFree(a);
         }
-    } pinned_free{h_scal, h_desc};
-    DeviceBlock scal_dev(DeviceBlock::bytes_for(scal_per * np, 8) + DeviceBlock::bytes_for((size_t)np, sizeof(OptDesc)));
-    double* d_scal = scal_dev.take<double>(scal_per * np);
-    OptDesc* d_desc = scal_dev.take<OptDesc>(np);
+    } pinned_free{pinned, pinned_owned};
+    double* h_scal = static_cast<double*>(pinned);
+    OptDesc* h_desc = reinterpret_cast<OptDesc*>(static_cast<char*>(pinned) + scal_bytes);
+    double* d_scal = dev.take<double>(scal_per * np);
+    OptDesc* d_desc = dev.take<OptDesc>(np);
     std::vector<double> c0;
     for (int k = 0; k < np; ++k) {
         Problem& s = P[k];

@@ -38,6 +38,11 @@ struct OptimizerHooks {
         evaluate;
     std::function<void(int which, const double* C, double* G)> complete;
     bool energy_only_trials = false;  // `complete` is available (a single problem, not a pool)
+    // device memory of at least `bytes` for the optimizer's state, owned by the caller and valid
+    // for the whole call; unset: the call allocates and frees its own
+    std::function<void*(size_t bytes)> device_block;
+    // pinned host memory of at least `bytes`, likewise (hipHostFree also waits for the device)
+    std::function<void*(size_t bytes)> host_block;
 };
 
 void optimizer_defaults(vbe_optimizer_options* o);

@@ -74,11 +74,28 @@ struct vbe_server {
     // pools when another stream is idle
     std::vector<void*> streams;
     std::vector<int> stream_load;
+    // per pool, the running mean of its launches' device time (ms): a pool whose launches take
+    // kLongLaunch times longer than the shortest pool's is kept off stream 0 and forks nothing,
+    // so that short launches always find a stream with only short work ahead of them
+    static constexpr double kLongLaunch = 4.0;
+    std::vector<std::pair<vbe_pool*, double>> launch_ms;
     std::string error;  // first dispatcher failure (the request carries its status)
 
     void run();
     void complete(Flight& f, int rc, const std::string& err);
     bool dispatch();
+    double mean_launch_ms(const vbe_pool* p) const {
+        for (const auto& e : launch_ms)
+            if (e.first == p) return e.second;
+        return 0.0;
+    }
+    bool long_launches(const vbe_pool* p) const {
+        double least = 0.0;
+        for (const auto& e : launch_ms)
+            if (least == 0.0 || e.second < least) least = e.second;
+        const double mine = mean_launch_ms(p);
+        return streams.size() > 1 && least > 0.0 && mine > kLongLaunch * least;
+    }
 };
 
 void vbe_server::complete(Flight& f, int rc, const std::string& err) {
@@ -94,6 +111,11 @@ void vbe_server::complete(Flight& f, int rc, const std::string& err) {
         if (rc != VBE_OK) std::fill(st.begin(), st.end(), rc);
     }
     if (rc != VBE_OK && error.empty()) error = cerr;
+    if (rc == VBE_OK && device_ms > 0.f) {
+        auto it = std::find_if(launch_ms.begin(), launch_ms.end(), [&](const auto& e) { return e.first == f.pool; });
+        if (it == launch_ms.end()) launch_ms.emplace_back(f.pool, (double)device_ms);
+        else it->second = 0.8 * it->second + 0.2 * (double)device_ms;
+    }
     const double t = now_s();
     for (size_t i = 0; i < n; ++i) {
         Request* r = f.reqs[i];
@@ -138,10 +160,13 @@ bool vbe_server::dispatch() {
         }
         int lane = -1;
         std::string err;
-        const int si = (int)(std::min_element(stream_load.begin(), stream_load.end()) - stream_load.begin());
+        const bool long_pool = long_launches(p);
+        const int first = long_pool ? 1 : 0;
+        const int si = (int)(std::min_element(stream_load.begin() + first, stream_load.end()) - stream_load.begin());
+        void* side = long_pool ? nullptr : streams[(size_t)(si + 1) % streams.size()];
         const double t = now_s();  // before the enqueue: the device may start while it runs
         const int rc = xmvb_amd::pool_launch(p, (int)batch.size(), slots.data(), C.data(), gradient, streams[si],
-                                             &lane, &err);
+                                             side, &lane, &err);
         if (rc == VBE_OK && lane < 0) continue;  // the caller holds the pool's lanes; try again later
         for (Request* r : batch) {
             r->info.t_dispatch = t;

@@ -25,9 +25,11 @@ bool pool_slot_busy(vbe_pool* p, int slot);
 // device and enqueues their forward (and backward) on a free lane; *lane receives the lane index,
 // or -1 when every lane is busy (nothing is enqueued then). Errors come back as a status with
 // *err set.
-// stream: the stream to run it on (one of the context's; null: the lane's own).
+// stream: the stream to run it on (one of the context's; null: the lane's own). side: the stream
+// that small launches may fork their adjoint kernels onto (one of the context's; null: none, the
+// whole launch stays on `stream`); used only together with `stream`.
 int pool_launch(vbe_pool* p, int count, const int* slots, const double* const* C, bool gradient, void* stream,
-                int* lane, std::string* err);
+                void* side, int* lane, std::string* err);
 // 1 when the lane's launch has completed, 0 while it runs.
 int pool_launch_query(vbe_pool* p, int lane, std::string* err);
 // Completes the lane's launch: energies[i] (E_total), grads[i] (n_act x n_bf_i, skipped when null),
