@@ -1245,7 +1245,7 @@ int vbe_context_memory(const vbe_context* ctx, uint64_t* in_use, uint64_t* budge
 
 int vbe_pool_device_bytes(const vbe_pool* pool, uint64_t* bytes) {
     if (!pool || !bytes) return VBE_ERR_ARGUMENT;
-    uint64_t b = pool->topo_arena.bytes + pool->ri_stage_arena.bytes;
+    uint64_t b = pool->topo_arena.bytes + pool->ri_stage_arena.bytes + pool->opt_arena.bytes;
     for (const auto& s : pool->slots)
         if (s.arena) b += s.arena->bytes;
     *bytes = b;

@@ -78,6 +78,7 @@ struct vbe_server {
     // kLongLaunch times longer than the shortest pool's is kept off stream 0 and forks nothing,
     // so that short launches always find a stream with only short work ahead of them
     static constexpr double kLongLaunch = 4.0;
+    static constexpr size_t kLaunchHistory = 256;  // pools remembered
     std::vector<std::pair<vbe_pool*, double>> launch_ms;
     std::string error;  // first dispatcher failure (the request carries its status)
 
@@ -113,8 +114,12 @@ void vbe_server::complete(Flight& f, int rc, const std::string& err) {
     if (rc != VBE_OK && error.empty()) error = cerr;
     if (rc == VBE_OK && device_ms > 0.f) {
         auto it = std::find_if(launch_ms.begin(), launch_ms.end(), [&](const auto& e) { return e.first == f.pool; });
-        if (it == launch_ms.end()) launch_ms.emplace_back(f.pool, (double)device_ms);
-        else it->second = 0.8 * it->second + 0.2 * (double)device_ms;
+        if (it != launch_ms.end()) {
+            it->second = 0.8 * it->second + 0.2 * (double)device_ms;
+        } else {
+            if (launch_ms.size() >= kLaunchHistory) launch_ms.erase(launch_ms.begin());  // oldest pool first
+            launch_ms.emplace_back(f.pool, (double)device_ms);
+        }
     }
     const double t = now_s();
     for (size_t i = 0; i < n; ++i) {
