@@ -15,6 +15,24 @@ __device__ __forceinline__ int pair_index(int a, int b) {
     return hi * (hi + 1) / 2 + lo;
 }
 
+// (x, z) of this workgroup in a grid (nx, 1, nz) of nx workgroups per problem z, renumbered so
+// that all workgroups of one problem run on one XCD: workgroups go to the 8 XCDs in turn by linear
+// id, so linear id L runs on XCD L % 8 and takes the (L / 8)-th place of that XCD's sequence,
+// which is mapped to problem L % 8 + 8 (L / 8 / nx), workgroup (L / 8) % nx. A problem's
+// workgroups then share one L2, and the data they all read (S, h, the determinant matrices) comes
+// from HBM once. A bijection of the grid when nz is a multiple of 8; otherwise the identity.
+__device__ __forceinline__ void problem_block(int& x, int& z) {
+    const unsigned nx = gridDim.x, nz = gridDim.z;
+    if (nz % 8u != 0u || gridDim.y != 1u) {
+        x = (int)blockIdx.x;
+        z = (int)blockIdx.z;
+        return;
+    }
+    const unsigned L = blockIdx.x + nx * blockIdx.z, k = L / 8u;
+    z = (int)(L % 8u + 8u * (k / nx));
+    x = (int)(k % nx);
+}
+
 // unordered pair t = e(e+1)/2 + d with d <= e
 __device__ __forceinline__ void unpack_pair(long long t, int& d, int& e) {
     long long x = (long long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);

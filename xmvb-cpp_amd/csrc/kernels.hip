@@ -47,7 +47,9 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // orbital preparation: one block per orbital
 // ---------------------------------------------------------------------------------------------
 __global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
-    const SlotDesc& sd = slots[blockIdx.z];
+    int bx, bz;
+    problem_block(bx, bz);  // the n_act workgroups of a problem on one XCD: S and h read once
+    const SlotDesc& sd = slots[bz];
     const auto C = to_global(sd.C);
     const auto support = to_global(sd.support);
     const auto S = to_global(sd.S);
@@ -61,7 +63,7 @@ __global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
     extern __shared__ double lds[];
     double* c = lds;             // n_bf
     double* red = lds + n_bf;    // 16
-    const int k = blockIdx.x;
+    const int k = bx;
     for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
         const size_t at = (size_t)k * n_bf + mu;
         c[mu] = (support && !support[at]) ? 0.0 : C[at];
@@ -959,7 +961,9 @@ void compact_pairs(hipStream_t st, const SlotDesc* slots, unsigned z, const Topo
 // structure matrices: Y = Hdet T (thread per (det, structure)), then T^T Y (thread per (I, J))
 // ---------------------------------------------------------------------------------------------
 __global__ void k_structure_right(const SlotDesc* __restrict__ slots, TopoDesc topo) {
-    const SlotDesc& sd = slots[blockIdx.z];
+    int bx, bz;
+    problem_block(bx, bz);  // a problem's workgroups on one XCD: each Hdet/Sdet column comes from HBM once
+    const SlotDesc& sd = slots[bz];
     const auto Hdet = to_global(sd.Hdet);
     const auto Sdet = to_global(sd.Sdet);
     const int n_det = topo.n_det, n_str = topo.n_str;
@@ -968,7 +972,7 @@ __global__ void k_structure_right(const SlotDesc* __restrict__ slots, TopoDesc t
     const double* __restrict__ scoef = topo.scoef;
     const auto Yh = to_global(sd.Yh);
     const auto Ys = to_global(sd.Ys);
-    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long id = (long long)bx * blockDim.x + threadIdx.x;
     if (id >= (long long)n_det * n_str) return;
     const int d = (int)(id % n_det), J = (int)(id / n_det);
     double h = 0.0, s = 0.0;
