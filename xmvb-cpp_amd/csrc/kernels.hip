@@ -976,6 +976,8 @@ __global__ void k_structure_right(const SlotDesc* __restrict__ slots, TopoDesc t
     if (id >= (long long)n_det * n_str) return;
     const int d = (int)(id % n_det), J = (int)(id / n_det);
     double h = 0.0, s = 0.0;
+    // unrolled: the column loads of several determinants in flight at once
+#pragma unroll 8
     for (long long t = soff[J]; t < soff[J + 1]; ++t) {
         const size_t at = (size_t)d + (size_t)n_det * sdet[t];
         h += scoef[t] * Hdet[at];
@@ -999,6 +1001,7 @@ __global__ void k_structure_left(const SlotDesc* __restrict__ slots, TopoDesc to
     if (id >= (long long)n_str * n_str) return;
     const int I = (int)(id % n_str), J = (int)(id / n_str);
     double h = 0.0, s = 0.0;
+#pragma unroll 8
     for (long long t = soff[I]; t < soff[I + 1]; ++t) {
         const size_t at = (size_t)sdet[t] + (size_t)n_det * J;
         h += scoef[t] * Yh[at];
