@@ -822,6 +822,28 @@ __global__ void k_reduce_partials(const SlotDesc* __restrict__ slots, int n_bloc
     if (lane == 0) out[i] = acc;
 }
 
+// The same sum for launches with few partial rows (at most kReduceRowsFew): one thread per element,
+// the rows added in order, consecutive threads on consecutive elements of a row (coalesced), the
+// loads of several rows in flight.
+constexpr int kReduceRowsFew = 64;
+__global__ void __launch_bounds__(256) k_reduce_partials_few(const SlotDesc* __restrict__ slots, int n_blocks, int len,
+                                                             int list_begin, int list_rows, int list_per_row) {
+    const SlotDesc& sd = slots[blockIdx.z];
+    const auto partials = to_global(sd.partials);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    const long long deferred = list_rows > 0 ? (long long)to_global(sd.flags)[FLAG_DEFERRED] : 0;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int b = 0; b < n_blocks; ++b) {
+        const int j = b - list_begin;
+        const bool idle = j >= 0 && j < list_rows && (long long)j * list_per_row >= deferred;
+        const double v = partials[(size_t)b * len + i];  // every row is allocated: an idle row is read, not added
+        acc += idle ? 0.0 : v;
+    }
+    to_global(sd.dsum)[i] = acc;
+}
+
 // Ordered compaction of a pair list: the flagged entries of a per-entry byte array, listed in entry
 // order. Deferred == true: entries are the lane kernels' work items and each flagged item lists its
 // pair and, when it has one, the spin-flip partner pair (the lane kernels defer the two together);
@@ -2399,8 +2421,12 @@ void launch_backward(hipStream_t st, const SlotDesc* slots, int count, const Top
     }
     mark(EV_BWD_KERNEL);
     const int len = ng + 2 * n * n;
-    hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, part_rows, len,
-                       list_begin, list_rows, list_per_row);
+    if (part_rows <= kReduceRowsFew)
+        hipLaunchKernelGGL(k_reduce_partials_few, dim3((len + 255) / 256, 1, z), dim3(256), 0, st, slots, part_rows,
+                           len, list_begin, list_rows, list_per_row);
+    else
+        hipLaunchKernelGGL(k_reduce_partials, dim3((len + 3) / 4, 1, z), dim3(256), 0, st, slots, part_rows, len,
+                           list_begin, list_rows, list_per_row);
     mark(EV_PAIR_BWD);
     hipLaunchKernelGGL(k_ri_pair_adjoint, dim3((shape.max_n_aux + 255) / 256, (np + 7) / 8, z), dim3(256), 0, st, slots,
                        np);
