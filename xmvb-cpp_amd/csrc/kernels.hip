@@ -75,6 +75,7 @@ __global__ void k_orbital_prep(const SlotDesc* __restrict__ slots, int n_act) {
     // loads of successive nu are independent of the accumulation, so they stay in flight
     for (int mu = threadIdx.x; mu < n_bf; mu += blockDim.x) {
         double sv = 0.0, hv = 0.0;
+#pragma unroll 8
         for (int nu = 0; nu < n_bf; ++nu) {
             const double cv = c[nu];
             sv += S[mu + (size_t)n_bf * nu] * cv;
