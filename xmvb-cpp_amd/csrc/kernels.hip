@@ -1950,6 +1950,7 @@ __global__ void __launch_bounds__(256) k_ri_pair_adjoint(const SlotDesc* __restr
     double acc[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+#pragma unroll 8
     for (int Q = 0; Q < np; ++Q) {
         const double l = L[(size_t)Ac + (size_t)n_aux * Q];
 #pragma unroll
